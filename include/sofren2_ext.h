@@ -1,0 +1,82 @@
+/*
+ * sofren2_ext.h — additive entry points of libsofren2_gpu.so.
+ *
+ * None of these exist in the reference API (sofren2.c:794-864); they sit
+ * beside the drop-in boundary.  They give tests and tools read access to the
+ * intermediate results of the last sft_present (visibility buffer, screen-space
+ * meshlet cache, clipped-triangle records) and to per-frame counters and pass
+ * timings.  The same capture functions are implemented by the CPU reference
+ * build used as a checker, so a test can compare two backends field by field.
+ */
+#ifndef SFT_EXT_H
+#define SFT_EXT_H
+
+#include "sofren2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Size in bytes of one screen-space meshlet cache entry (sofren2.c:1464-1477):
+   sx, sy, inv_w, wx, wy, wz, wnx, wny, wnz, each f32[64]. */
+#define SFTX_SS_MESHLET_BYTES 2304
+/* Size in bytes of one clipped-triangle record (sofren2.c:1271-1280). */
+#define SFTX_CLIP_RECORD_BYTES 164
+
+/* Capture flags for sftx_set_capture. */
+#define SFTX_CAPTURE_VISIBILITY 1u   /* per-sample 32-bit IDs and 1/w depth */
+#define SFTX_CAPTURE_SS_CACHE   2u   /* screen-space meshlet cache + cull result */
+#define SFTX_CAPTURE_CLIPS      4u   /* clipped-triangle records */
+
+/* Enables capture of intermediates on the following sft_present calls. */
+void sftx_set_capture(u32 flags);
+
+/* Tile grid of the last present: columns, rows (64 px tiles) and samples per
+   pixel.  The captured visibility covers the whole grid (cols*64 x rows*64),
+   including pixels past the screen edge inside border tiles. */
+void sftx_cap_grid(i32* cols, i32* rows, i32* samples);
+
+/* Per-sample reference 32-bit visibility IDs (inst<<25 | meshlet<<7 | tri, or
+   clip_slot | 1<<31, 0xFFFFFFFF = empty) and depth (interpolated 1/w, 0 =
+   empty), laid out [sample][cols*64 * rows*64] row-major.  Either pointer may
+   be NULL. */
+void sftx_cap_visibility(u32* vid, f32* depth);
+
+/* Number of screen-space meshlet slots of the last present (sum of
+   total_meshlets over drawn instances), their contents (count *
+   SFTX_SS_MESHLET_BYTES) and the per-slot cull result (1 = visible). */
+u32 sftx_cap_ss_count(void);
+void sftx_cap_ss_cache(void* out);
+void sftx_cap_ss_visible(u8* out);
+
+/* Clipped-triangle records of the last present, in slot order. */
+u32 sftx_cap_clip_count(void);
+void sftx_cap_clip_records(void* out);
+
+/* Per-frame counters and pass timings of the last present. */
+typedef struct sftx_frame_stats {
+    u32 meshlets_tested;      /* N_t */
+    u32 meshlets_visible;     /* N_v */
+    u32 meshlets_skinned;     /* N_vs */
+    u32 batch_payloads;       /* batch payloads emitted */
+    u32 clip_triangles;       /* K: clipped-triangle slots used */
+    u32 tile_payloads;        /* P: (payload, tile) pairs */
+    u32 tiles;                /* 64x64 tiles */
+    u32 samples;              /* S: width*height*samples_per_pixel */
+    u32 pixels;               /* X */
+    u32 shaded_pixels;        /* pixels covered by geometry */
+    f32 ms_upload;            /* host->device copies of the frame's commands */
+    f32 ms_geometry;          /* cull + skin + project + classify */
+    f32 ms_binning;           /* clip + bin */
+    f32 ms_raster_shade;      /* tile raster + resolve */
+    f32 ms_download;          /* device->host colour copy */
+    f32 ms_total;             /* whole sft_present on the GPU stream */
+} sftx_frame_stats;
+
+void sftx_get_stats(sftx_frame_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,54 @@
+/*
+ * scenes.h — seeded synthetic workloads driven through the public sofren2 API.
+ *
+ * The same source is compiled against any implementation of the API (the HIP
+ * backend in this package, the CPU reference build used as a checker), so
+ * every backend renders byte-identical scene descriptions.  All randomness
+ * comes from a PCG32 stream seeded per scene.
+ *
+ * Workloads:
+ *   SFTX_SCENE_PLANE_SPHERES (0): 640x360, ground plane of 20x20 unit quads and
+ *       16 dmesh spheres (16 rings x 32 slices), radius U[0.5,1.5], centres
+ *       U[-8,8]^2 at y = radius, flat per-object colour U[0,1]^3, one point
+ *       light at (0,10,0), ambient 0.35, no textures, no shadows.
+ *   SFTX_SCENE_BOX_ROOM (1): 1920x1080 textured interior, ~260k triangles in
+ *       4096 axis-aligned boxes (log-uniform sizes in [0.1,4]) on a 16^3
+ *       lattice inside a room, 16 value-noise 256x256 textures with full mip
+ *       chains, a baked lightmap page (per-vertex light U[0.2,1]), 8 point
+ *       lights without shadows, a 120-frame camera path.  The scene is written
+ *       to a QSF2 file and loaded through sft_model_load_qsf.
+ */
+#ifndef SFTX_SCENES_H
+#define SFTX_SCENES_H
+
+#include "sofren2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sftx_scene_kind {
+    SFTX_SCENE_PLANE_SPHERES = 0,
+    SFTX_SCENE_BOX_ROOM = 1,
+    SFTX_SCENE_COUNT
+};
+
+/* Creates the scene's resources; call after sft_init.  Returns 0 on success. */
+i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height);
+
+/* Records one frame of the scene (camera, lights, draws) and presents it into
+   the caller's ARGB8888 buffer: frame_begin, camera, lights, draws,
+   sft_present, frame_end. */
+void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes);
+
+/* Frame statistics of the scene description (not of the renderer). */
+u32 sftx_scene_triangle_count(void);
+u32 sftx_scene_meshlet_count(void);
+
+void sftx_scene_destroy(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
