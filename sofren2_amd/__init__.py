@@ -1,0 +1,136 @@
+"""Python handle on the MI355X-native sofren2 renderer.
+
+The renderer is the C library ``lib/libsofren2_gpu.so`` (public API in
+``include/sofren2.h``, additive capture/stats API in ``include/sofren2_ext.h``);
+this module only loads it through ctypes for tests and benchmarks.  Loading
+fails loudly when the library has not been built: there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG, "lib")
+GPU_LIB = os.path.join(LIB_DIR, "libsofren2_gpu.so")
+SCENES_LIB = os.path.join(LIB_DIR, "libsofren2_scenes.so")
+
+SS_MESHLET_BYTES = 2304
+CLIP_RECORD_BYTES = 164
+CAPTURE_VISIBILITY = 1
+CAPTURE_SS_CACHE = 2
+CAPTURE_CLIPS = 4
+
+SCENE_PLANE_SPHERES = 0
+SCENE_BOX_ROOM = 1
+
+# default resolution of each workload
+SCENE_SIZE = {SCENE_PLANE_SPHERES: (640, 360), SCENE_BOX_ROOM: (1920, 1080)}
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "meshlets_tested", "meshlets_visible", "meshlets_skinned", "batch_payloads", "clip_triangles",
+        "tile_payloads", "tiles", "samples", "pixels", "shaded_pixels")] + [(n, ctypes.c_float) for n in (
+        "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_total")]
+
+
+def _bind(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+class Library:
+    """ctypes bindings of libsofren2_gpu.so and the scene generator."""
+
+    def __init__(self):
+        for path in (GPU_LIB, SCENES_LIB):
+            if not os.path.exists(path):
+                raise RuntimeError(
+                    f"{path} is missing: build it with __graft_entry__.build() or python sofren2_amd/build.py")
+        self.gpu = ctypes.CDLL(GPU_LIB)
+        self.scenes = ctypes.CDLL(SCENES_LIB)
+        g, s = self.gpu, self.scenes
+        i32, u32, u8, vp = ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint8, ctypes.c_void_p
+        self.sft_init = _bind(g, "sft_init", None, [i32, i32, i32])
+        self.sft_release = _bind(g, "sft_release", None, [])
+        self.sft_resize = _bind(g, "sft_resize", None, [i32, i32])
+        self.set_capture = _bind(g, "sftx_set_capture", None, [u32])
+        self.cap_grid = _bind(g, "sftx_cap_grid", None, [ctypes.POINTER(i32)] * 3)
+        self.cap_visibility = _bind(g, "sftx_cap_visibility", None, [vp, vp])
+        self.cap_ss_count = _bind(g, "sftx_cap_ss_count", u32, [])
+        self.cap_ss_cache = _bind(g, "sftx_cap_ss_cache", None, [vp])
+        self.cap_ss_visible = _bind(g, "sftx_cap_ss_visible", None, [vp])
+        self.cap_clip_count = _bind(g, "sftx_cap_clip_count", u32, [])
+        self.cap_clip_records = _bind(g, "sftx_cap_clip_records", None, [vp])
+        self.get_stats = _bind(g, "sftx_get_stats", None, [ctypes.POINTER(FrameStats)])
+        self.scene_create = _bind(s, "sftx_scene_create", i32, [i32, u32, i32, i32])
+        self.scene_render = _bind(s, "sftx_scene_render", None, [i32, vp, i32])
+        self.scene_destroy = _bind(s, "sftx_scene_destroy", None, [])
+        self.scene_tris = _bind(s, "sftx_scene_triangle_count", u32, [])
+        self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
+
+
+_LIB = None
+
+
+def library():
+    global _LIB
+    if _LIB is None:
+        _LIB = Library()
+    return _LIB
+
+
+class SceneRenderer:
+    """Renders a seeded workload through the public API on the GPU."""
+
+    def __init__(self, scene, seed=0, width=None, height=None, capture=0):
+        self.lib = library()
+        w, h = SCENE_SIZE[scene]
+        self.width = width or w
+        self.height = height or h
+        self.lib.sft_init(self.width, self.height, 0)
+        self.lib.set_capture(capture)
+        if self.lib.scene_create(scene, seed, self.width, self.height) != 0:
+            raise ValueError(f"unknown scene {scene}")
+        self.color = np.zeros((self.height, self.width), dtype=np.uint32)
+
+    def render(self, frame=0):
+        self.lib.scene_render(frame, self.color.ctypes.data, self.width * 4)
+        return self.color
+
+    def stats(self):
+        st = FrameStats()
+        self.lib.get_stats(ctypes.byref(st))
+        return st
+
+    def visibility(self):
+        cols, rows, samples = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self.lib.cap_grid(ctypes.byref(cols), ctypes.byref(rows), ctypes.byref(samples))
+        n = cols.value * 64 * rows.value * 64 * samples.value
+        vid = np.zeros(n, dtype=np.uint32)
+        depth = np.zeros(n, dtype=np.float32)
+        self.lib.cap_visibility(vid.ctypes.data, depth.ctypes.data)
+        return vid, depth
+
+    def ss_cache(self):
+        n = self.lib.cap_ss_count()
+        ss = np.zeros(n * SS_MESHLET_BYTES, dtype=np.uint8)
+        vis = np.zeros(n, dtype=np.uint8)
+        if n:
+            self.lib.cap_ss_cache(ss.ctypes.data)
+            self.lib.cap_ss_visible(vis.ctypes.data)
+        return ss.view(np.uint32).reshape(n, 9, 64), vis
+
+    def clips(self):
+        n = self.lib.cap_clip_count()
+        rec = np.zeros(n * CLIP_RECORD_BYTES, dtype=np.uint8)
+        if n:
+            self.lib.cap_clip_records(rec.ctypes.data)
+        return rec.view(np.uint32).reshape(n, 41)
+
+    def close(self):
+        self.lib.scene_destroy()
+        self.lib.sft_release()

@@ -1,0 +1,583 @@
+/*
+ * backend.cpp — HBM residency and per-frame orchestration of the HIP render
+ * path.  sft_present (sofren2.c:8139-8289) becomes: make new models and
+ * textures resident, build the frame's instance / work / light / material
+ * tables, upload them, run the kernels of kernels.hip on one stream, and copy
+ * the ARGB frame into the caller's buffer.  The reference's semaphore hand-off
+ * to its worker threads is the host->device boundary here.
+ *
+ * Capacities of the variable-size intermediates (clip records, tile lists) are
+ * not known before the geometry pass; the kernels flag an overflow instead of
+ * writing past them and the frame is re-run with larger buffers, so a frame
+ * needs no host round trip between passes.
+ */
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <algorithm>
+#include <chrono>
+#include "kernels.h"
+
+#define HIP_CHECK(x) do { \
+    const hipError_t e_ = (x); \
+    if (hipSuccess != e_) { \
+        fprintf(stderr, "fatal sofren2 HIP error %s at %s:%d: %s\n", hipGetErrorName(e_), __FILE__, __LINE__, #x); \
+        exit(1); \
+    } \
+} while (0)
+
+namespace sft {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+/* grows a device buffer to at least `bytes`; `keep` preserves its contents */
+static void dgrow(DBuf& b, size_t bytes, bool keep, hipStream_t s) {
+    if (bytes <= b.cap) return;
+    size_t nc = std::max<size_t>(bytes, b.cap + b.cap / 2);
+    nc = std::max<size_t>(nc, 4096);
+    void* np = nullptr;
+    HIP_CHECK(hipMalloc(&np, nc));
+    if (keep && b.p && b.cap) {
+        HIP_CHECK(hipMemcpyAsync(np, b.p, b.cap, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    if (b.p) HIP_CHECK(hipFree(b.p));
+    b.p = np;
+    b.cap = nc;
+}
+
+template <class T>
+static void upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
+    const size_t bytes = std::max<size_t>(v.size() * sizeof(T), sizeof(T));
+    dgrow(b, bytes, false, s);
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+struct ModelRes {
+    u32 base = 0;                   /* first slot */
+    u32 total = 0;                  /* total_meshlets */
+    u32 mesh_rec_base = 0;
+    u32 max_bone = 0;               /* bone matrices a pose must provide */
+    std::vector<u32> skin_of;       /* skin record per model meshlet, kInvalid if none */
+};
+
+struct Backend {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+
+    /* resident pools */
+    DBuf pos, attr, inds, cull, slot_mesh, skin, mesh_recs_d;
+    u32 resident_slots = 0, resident_skins = 0, resident_mesh_recs = 0;
+    std::vector<MeshRec> mesh_recs;
+    std::unordered_map<const sft_model*, ModelRes> models;
+    DBuf texels, tex_meta_d;
+    u64 texels_used = 0;
+    std::vector<TexMeta> tex_meta;
+    bool tex_meta_dirty = true;
+    DBuf est_rcp, est_rsq;
+    bool est_uploaded = false;
+
+    /* per frame */
+    DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
+    DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
+    DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, cap_vid, cap_depth, counters;
+    u32 clip_cap = 1u << 16;
+    u32 tile_list_cap = 1u << 20;
+
+    /* host copies of the last frame's intermediates (capture) */
+    i32 cap_cols = 0, cap_rows = 0;
+    std::vector<u32> h_vid;
+    std::vector<f32> h_depth;
+    std::vector<u8> h_ss, h_ss_vis, h_clips;
+    u32 h_ss_count = 0, h_clip_count = 0;
+};
+
+Backend* backend_create() {
+    Backend* b = new Backend();
+    int n = 0;
+    if (hipSuccess != hipGetDeviceCount(&n) || n < 1) {
+        fprintf(stderr, "fatal sofren2 error: no HIP device; the render path runs only on the GPU\n");
+        exit(1);
+    }
+    HIP_CHECK(hipSetDevice(0));
+    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventCreate(&e));
+    return b;
+}
+
+void backend_destroy(Backend* b) {
+    if (!b) return;
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    DBuf* all[] = { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->mesh_recs_d,
+                    &b->texels, &b->tex_meta_d, &b->est_rcp, &b->est_rsq, &b->fc_d, &b->insts, &b->work,
+                    &b->bones, &b->plights, &b->dlights, &b->mat_tex, &b->mat_color, &b->mat_shin,
+                    &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
+                    &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
+                    &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->cap_vid,
+                    &b->cap_depth, &b->counters };
+    for (DBuf* d : all) {
+        if (d->p) HIP_CHECK(hipFree(d->p));
+    }
+    for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventDestroy(e));
+    HIP_CHECK(hipStreamDestroy(b->stream));
+    delete b;
+}
+
+void backend_release_texture(Backend* b, u32 index) {
+    (void)index;
+    if (b) b->tex_meta_dirty = true;
+}
+
+/* writes one model's meshlets into slots [base, base + total) of the pools */
+static void upload_model(Backend* b, const sft_model* m, ModelRes& r, u32 base, u32 mesh_rec_base, std::vector<MeshRec>& recs, bool resident) {
+    const u32 T = m->total_meshlets;
+    r.base = base;
+    r.total = T;
+    r.mesh_rec_base = mesh_rec_base;
+    r.skin_of.assign(T, kInvalid);
+    r.max_bone = (u32)std::max(m->skeleton.bone_count, 0);
+    const size_t slots = (size_t)base + T;
+    dgrow(b->pos, slots * 768, true, b->stream);
+    dgrow(b->attr, slots * 1792, true, b->stream);
+    dgrow(b->inds, slots * 512, true, b->stream);
+    dgrow(b->cull, slots * 32, true, b->stream);
+    dgrow(b->slot_mesh, slots * 4, true, b->stream);
+    if (0 == T) return;
+
+    std::vector<f32> pos((size_t)T * 192), attr((size_t)T * 448), cull((size_t)T * 8, 0.f);
+    std::vector<u8> inds((size_t)T * 512, 0), skin;
+    std::vector<u32> slot_mesh(T, mesh_rec_base);
+    u32 n_skin = 0;
+    for (u32 i = 0; i < m->mesh_count; i += 1) {
+        if (m->meshes[i].meshlet_skins) n_skin += m->meshes[i].meshlet_count;
+    }
+    const u32 skin_base = b->resident_skins;
+    if (n_skin) skin.assign((size_t)n_skin * 768, 0);
+    u32 skin_next = 0;
+    for (u32 i = 0; i < m->mesh_count; i += 1) {
+        const sft_mesh* mesh = &m->meshes[i];
+        MeshRec rec;
+        rec.material = mesh->material;
+        rec.lightmapped = (0xFFFFFFFFu != mesh->lightmap_page) ? 1u : 0u;
+        rec.lightmap_tex = rec.lightmapped && m->lightmaps ? m->lightmaps[mesh->lightmap_page] : SFT_INVALID_HANDLE;
+        rec.pad = 0;
+        recs.push_back(rec);
+        for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
+            const u32 gm = mesh->model_meshlet_offset + k;
+            if (gm >= T) {
+                fprintf(stderr, "fatal sofren2 error: mesh %u meshlet %u lies outside the model's %u meshlets\n", i, k, T);
+                exit(1);
+            }
+            memcpy(&pos[(size_t)gm * 192], &mesh->meshlets[k], 768);
+            memcpy(&attr[(size_t)gm * 448], &mesh->meshlet_attribs[k], 1792);
+            memcpy(&inds[(size_t)gm * 512], &mesh->inds[k], 512);
+            f32* c = &cull[(size_t)gm * 8];
+            c[0] = mesh->bounds_center_x[k];
+            c[1] = mesh->bounds_center_y[k];
+            c[2] = mesh->bounds_center_z[k];
+            c[3] = mesh->bounds_radius[k];
+            c[4] = mesh->cone_axis_x[k];
+            c[5] = mesh->cone_axis_y[k];
+            c[6] = mesh->cone_axis_z[k];
+            c[7] = mesh->cone_cutoff[k];
+            slot_mesh[gm] = mesh_rec_base + i;
+            if (mesh->meshlet_skins) {
+                const sft_meshlet_skin* sk = &mesh->meshlet_skins[k];
+                memcpy(&skin[(size_t)skin_next * 768], sk, 768);
+                r.skin_of[gm] = skin_base + skin_next;
+                skin_next += 1;
+                for (int v = 0; v < 64; v += 1) {
+                    const u32 jm = std::max(std::max(sk->j0[v], sk->j1[v]), std::max(sk->j2[v], sk->j3[v]));
+                    r.max_bone = std::max(r.max_bone, jm + 1u);
+                }
+            }
+            if (mesh->culling_bone_inds && mesh->culling_bone_inds[k] >= 0) {
+                r.max_bone = std::max(r.max_bone, (u32)mesh->culling_bone_inds[k] + 1u);
+            }
+        }
+    }
+    hipStream_t s = b->stream;
+    HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + (size_t)base * 768, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + (size_t)base * 1792, attr.data(), attr.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + (size_t)base * 512, inds.data(), inds.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + (size_t)base * 32, cull.data(), cull.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + (size_t)base * 4, slot_mesh.data(), slot_mesh.size() * 4, hipMemcpyHostToDevice, s));
+    if (n_skin) {
+        dgrow(b->skin, ((size_t)skin_base + n_skin) * 768, true, s);
+        HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)skin_base * 768, skin.data(), skin.size(), hipMemcpyHostToDevice, s));
+        if (resident) b->resident_skins += n_skin;
+    }
+    /* the staging vectors die at scope exit: finish the copies first */
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static void sync_textures(Backend* b, Context& c) {
+    if (b->tex_meta.size() != c.tex.size()) {
+        b->tex_meta.resize(c.tex.size());
+        b->tex_meta_dirty = true;
+    }
+    for (size_t i = 0; i < c.tex.size(); i += 1) {
+        HostTexture& t = c.tex[i];
+        if (!t.live || t.uploaded) continue;
+        const u64 n = t.texels.size();
+        dgrow(b->texels, (size_t)(b->texels_used + n) * 4, true, b->stream);
+        HIP_CHECK(hipMemcpyAsync((u32*)b->texels.p + b->texels_used, t.texels.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+        t.dev_offset = b->texels_used;
+        t.uploaded = true;
+        b->texels_used += n;
+        TexMeta& m = b->tex_meta[i];
+        memset(&m, 0, sizeof(m));
+        m.w = t.w;
+        m.h = t.h;
+        m.mip_levels = t.mip_levels;
+        m.is_pot = t.is_pot;
+        for (int l = 0; l < 14; l += 1) {
+            m.mip_offset[l] = (u32)(t.dev_offset + t.mip_offset[l]);
+            m.blocks_w[l] = t.blocks_w[l];
+            m.stride_shift[l] = t.stride_shift[l];
+        }
+        b->tex_meta_dirty = true;
+    }
+    if (b->tex_meta_dirty) {
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+        upload(b->tex_meta_d, b->tex_meta, b->stream);
+        b->tex_meta_dirty = false;
+    }
+    dgrow(b->texels, 4, false, b->stream);
+}
+
+static f32 ev_ms(hipEvent_t a, hipEvent_t z) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, z));
+    return ms;
+}
+
+void backend_present(Backend* b, Context* cp) {
+    Context& c = *cp;
+    hipStream_t s = b->stream;
+    HIP_CHECK(hipEventRecord(b->ev[0], s));
+
+    if (!b->est_uploaded) {
+        upload(b->est_rcp, c.est.rcp, s);
+        upload(b->est_rsq, c.est.rsq, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        b->est_uploaded = true;
+    }
+
+    /* ---- residency: new resident models first, then this frame's transient ones ---- */
+    std::vector<const sft_model*> transient = c.transient_models;
+    auto is_transient = [&](const sft_model* m) {
+        return std::find(transient.begin(), transient.end(), m) != transient.end();
+    };
+    for (const sft_instance& in : c.instances) {
+        const sft_model* m = in.model;
+        if (is_transient(m)) continue;
+        auto it = b->models.find(m);
+        if (it != b->models.end() && it->second.total == m->total_meshlets) continue;
+        ModelRes r;
+        b->mesh_recs.resize(b->resident_mesh_recs);
+        upload_model(b, m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, true);
+        b->resident_slots += m->total_meshlets;
+        b->resident_mesh_recs = (u32)b->mesh_recs.size();
+        b->models[m] = std::move(r);
+    }
+    b->mesh_recs.resize(b->resident_mesh_recs);
+    std::unordered_map<const sft_model*, ModelRes> tmodels;
+    u32 tslot = b->resident_slots;
+    const u32 saved_skins = b->resident_skins;
+    for (const sft_model* m : transient) {
+        ModelRes r;
+        upload_model(b, m, r, tslot, (u32)b->mesh_recs.size(), b->mesh_recs, false);
+        tslot += m->total_meshlets;
+        tmodels[m] = std::move(r);
+    }
+    b->resident_skins = saved_skins;
+    upload(b->mesh_recs_d, b->mesh_recs, s);
+    sync_textures(b, c);
+
+    /* ---- frame tables ---- */
+    std::vector<InstGpu> insts;
+    std::vector<WorkItem> work;
+    std::vector<f32> bones;
+    insts.reserve(c.instances.size());
+    for (u32 ii = 0; ii < c.instances.size(); ii += 1) {
+        const sft_instance& in = c.instances[ii];
+        const sft_model* m = in.model;
+        auto ti = tmodels.find(m);
+        const ModelRes& r = (ti != tmodels.end()) ? ti->second : b->models[m];
+        InstGpu g;
+        memcpy(g.t, &in.transform, 64);
+        g.model_base = r.base;
+        g.ss_offset = in.ss_meshlet_offset;
+        g.bone_base = kInvalid;
+        g.pad = 0;
+        if (in.posed_bone_matrices) {
+            g.bone_base = (u32)(bones.size() / 16);
+            const u32 n = std::max(r.max_bone, 1u);
+            bones.insert(bones.end(), (const f32*)in.posed_bone_matrices, (const f32*)in.posed_bone_matrices + (size_t)n * 16);
+        }
+        insts.push_back(g);
+        for (u32 mi = 0; mi < m->mesh_count; mi += 1) {
+            const sft_mesh* mesh = &m->meshes[mi];
+            for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
+                WorkItem w;
+                w.gm = mesh->model_meshlet_offset + k;
+                w.slot = r.base + w.gm;
+                w.ss = in.ss_meshlet_offset + w.gm;
+                w.inst = ii;
+                w.cull_bone = (mesh->culling_bone_inds && in.posed_bone_matrices) ? mesh->culling_bone_inds[k & ~7u] : -1;
+                w.skin = (mesh->meshlet_skins && in.posed_bone_matrices) ? r.skin_of[w.gm] : kInvalid;
+                w.pad0 = w.pad1 = 0;
+                work.push_back(w);
+            }
+        }
+    }
+    if (work.size() >= (1u << 23)) {
+        fprintf(stderr, "fatal sofren2 error: %zu meshlets in one frame exceed the 2^23 emission keys\n", work.size());
+        exit(1);
+    }
+    std::vector<PointLightGpu> pl(c.point_lights.size());
+    for (size_t i = 0; i < pl.size(); i += 1) {
+        const sft_point_light& L = c.point_lights[i];
+        pl[i] = PointLightGpu{{L.pos.x, L.pos.y, L.pos.z}, {L.color.x, L.color.y, L.color.z}, L.radius, L.inv_radius_sqr};
+    }
+    std::vector<DirLightGpu> dl(c.dir_lights.size());
+    for (size_t i = 0; i < dl.size(); i += 1) {
+        const sft_directional_light& L = c.dir_lights[i];
+        dl[i] = DirLightGpu{{L.dir.x, L.dir.y, L.dir.z}, {L.color.x, L.color.y, L.color.z}, {0.f, 0.f}};
+    }
+
+    FrameConst fc;
+    memset(&fc, 0, sizeof(fc));
+    fc.width = c.width;
+    fc.height = c.height;
+    fc.screen_w = (i32)(c.half_width * 2.f);
+    fc.screen_h = (i32)(c.half_height * 2.f);
+    fc.tile_cols = c.tile_cols;
+    fc.tile_rows = c.tile_rows;
+    fc.n_tiles = c.tile_cols * c.tile_rows;
+    fc.samples = 1;
+    fc.hw = c.half_width;
+    fc.hh = c.half_height;
+    fc.vp_x = c.vp_x;
+    fc.vp_y = c.vp_y;
+    fc.bias_x = c.half_width + c.vp_x;
+    fc.bias_y = c.half_height + c.vp_y;
+    fc.neg_hh = -c.half_height;
+    memcpy(fc.vp, &c.view_proj, 64);
+    memcpy(fc.planes, c.planes, sizeof(fc.planes));
+    fc.cam[0] = c.camera_pos.x; fc.cam[1] = c.camera_pos.y; fc.cam[2] = c.camera_pos.z;
+    fc.ambient[0] = c.ambient.x; fc.ambient[1] = c.ambient.y; fc.ambient[2] = c.ambient.z;
+    fc.clear_color = c.clear_color;
+    fc.should_clear = c.should_clear;
+    fc.debug_mode = (u32)c.debug_mode;
+    fc.shadows_enabled = c.shadows_enabled;
+    fc.lm_brightness = c.lightmap_brightness;
+    fc.n_work = (u32)work.size();
+    fc.n_inst = (u32)insts.size();
+    fc.n_point = (u32)pl.size();
+    fc.n_dir = (u32)dl.size();
+    fc.n_proxies = (u32)c.proxies.size();
+    fc.base_tex = (c.base_mat != kInvalid) ? c.mat_tex[SFT_MAT_INDEX(c.base_mat)] : SFT_INVALID_HANDLE;
+    fc.rcp_bits = (u32)c.est.rcp_bits;
+    fc.rsq_bits = (u32)c.est.rsq_bits;
+    fc.capture = c.capture_flags;
+    fc.n_tex = (u32)c.tex.size();
+    fc.n_mat = (u32)c.mat_tex.size();
+    fc.n_tmat = (u32)c.tmat_tex.size();
+
+    /* ---- uploads ---- */
+    upload(b->insts, insts, s);
+    upload(b->work, work, s);
+    upload(b->bones, bones, s);
+    upload(b->plights, pl, s);
+    upload(b->dlights, dl, s);
+    upload(b->mat_tex, c.mat_tex, s);
+    upload(b->mat_color, c.mat_color, s);
+    upload(b->mat_shin, c.mat_shininess, s);
+    upload(b->tmat_tex, c.tmat_tex, s);
+    upload(b->tmat_color, c.tmat_color, s);
+    upload(b->tmat_shin, c.tmat_shininess, s);
+
+    const u32 nw = fc.n_work;
+    const u32 nt = (u32)fc.n_tiles;
+    const size_t px = (size_t)c.width * c.height;
+    dgrow(b->ss, std::max<size_t>((size_t)c.total_ss * 2304, 4), false, s);
+    dgrow(b->ss_vis, std::max<size_t>(c.total_ss, 4), false, s);
+    dgrow(b->clip_count, std::max<size_t>((size_t)nw * 4, 4), false, s);
+    dgrow(b->clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
+    dgrow(b->clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
+    dgrow(b->batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
+    dgrow(b->tile_count, (size_t)nt * 4, false, s);
+    dgrow(b->tile_off, (size_t)nt * 4, false, s);
+    dgrow(b->tile_fill, (size_t)nt * 4, false, s);
+    dgrow(b->color, std::max<size_t>(px * 4, 4), false, s);
+    dgrow(b->counters, CNT_COUNT * 4, false, s);
+    const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
+    const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64;
+    if (cap_vis) {
+        dgrow(b->cap_vid, grid * 4, false, s);
+        dgrow(b->cap_depth, grid * 4, false, s);
+    }
+    if (!c.should_clear && c.color_buffer && px) {
+        HIP_CHECK(hipMemcpy2DAsync(b->color.p, (size_t)c.width * 4, c.color_buffer, (size_t)c.color_pitch * 4,
+                                   (size_t)c.width * 4, c.height, hipMemcpyHostToDevice, s));
+    }
+
+    u32 counters[CNT_COUNT];
+    for (int attempt = 0;; attempt += 1) {
+        dgrow(b->tile_list, (size_t)b->tile_list_cap * 8, false, s);
+        dgrow(b->clip_rec, (size_t)b->clip_cap * 164, false, s);
+        dgrow(b->clip_key, (size_t)b->clip_cap * 4, false, s);
+        dgrow(b->clip_rect, (size_t)b->clip_cap * 4, false, s);
+        fc.clip_cap = b->clip_cap;
+        fc.tile_list_cap = b->tile_list_cap;
+        dgrow(b->fc_d, sizeof(FrameConst), false, s);
+        HIP_CHECK(hipMemcpyAsync(b->fc_d.p, &fc, sizeof(fc), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemsetAsync(b->counters.p, 0, CNT_COUNT * 4, s));
+        HIP_CHECK(hipMemsetAsync(b->tile_count.p, 0, (size_t)nt * 4, s));
+        HIP_CHECK(hipMemsetAsync(b->tile_fill.p, 0, (size_t)nt * 4, s));
+
+        FramePtrs p;
+        p.pos = (const f32*)b->pos.p;
+        p.attr = (const f32*)b->attr.p;
+        p.inds = (const u8*)b->inds.p;
+        p.cull = (const f32*)b->cull.p;
+        p.slot_mesh = (const u32*)b->slot_mesh.p;
+        p.skin = (const u8*)b->skin.p;
+        p.mesh_recs = (const MeshRec*)b->mesh_recs_d.p;
+        p.texels = (const u32*)b->texels.p;
+        p.tex_meta = (const TexMeta*)b->tex_meta_d.p;
+        p.est_rcp = (const f32*)b->est_rcp.p;
+        p.est_rsq = (const f32*)b->est_rsq.p;
+        p.fc = (const FrameConst*)b->fc_d.p;
+        p.insts = (const InstGpu*)b->insts.p;
+        p.work = (const WorkItem*)b->work.p;
+        p.bones = (const f32*)b->bones.p;
+        p.point_lights = (const PointLightGpu*)b->plights.p;
+        p.dir_lights = (const DirLightGpu*)b->dlights.p;
+        p.mat_tex = (const u32*)b->mat_tex.p;
+        p.mat_color = (const u32*)b->mat_color.p;
+        p.mat_shin = (const f32*)b->mat_shin.p;
+        p.tmat_tex = (const u32*)b->tmat_tex.p;
+        p.tmat_color = (const u32*)b->tmat_color.p;
+        p.tmat_shin = (const f32*)b->tmat_shin.p;
+        p.ss = (f32*)b->ss.p;
+        p.ss_vis = (u8*)b->ss_vis.p;
+        p.clip_count = (u32*)b->clip_count.p;
+        p.clip_base = (u32*)b->clip_base.p;
+        p.clip_mask = (u64*)b->clip_mask.p;
+        p.batch_pl = (uint2*)b->batch_pl.p;
+        p.tile_count = (u32*)b->tile_count.p;
+        p.tile_off = (u32*)b->tile_off.p;
+        p.tile_fill = (u32*)b->tile_fill.p;
+        p.tile_list = (u64*)b->tile_list.p;
+        p.clip_rec = (f32*)b->clip_rec.p;
+        p.clip_key = (u32*)b->clip_key.p;
+        p.clip_rect = (u32*)b->clip_rect.p;
+        p.color = (u32*)b->color.p;
+        p.cap_vid = cap_vis ? (u32*)b->cap_vid.p : nullptr;
+        p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
+        p.counters = (u32*)b->counters.p;
+
+        launch_frame(p, fc, s, b->ev);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(counters, b->counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (0u == counters[CNT_OVERFLOW]) break;
+        if (attempt > 8) {
+            fprintf(stderr, "fatal sofren2 error: frame intermediates keep overflowing\n");
+            exit(1);
+        }
+        if (counters[CNT_OVERFLOW] & 1u) b->clip_cap = std::max(b->clip_cap * 2u, counters[CNT_CLIP_TRIS] + 1024u);
+        if (counters[CNT_OVERFLOW] & 2u) b->tile_list_cap = std::max(b->tile_list_cap * 2u, counters[CNT_TILE_PAIRS] + 4096u);
+    }
+
+    if (c.color_buffer && px) {
+        HIP_CHECK(hipMemcpy2DAsync(c.color_buffer, (size_t)c.color_pitch * 4, b->color.p, (size_t)c.width * 4,
+                                   (size_t)c.width * 4, c.height, hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK(hipEventRecord(b->ev[5], s));
+    HIP_CHECK(hipStreamSynchronize(s));
+
+    sftx_frame_stats& st = c.stats;
+    memset(&st, 0, sizeof(st));
+    st.meshlets_tested = nw;
+    st.meshlets_visible = counters[CNT_VISIBLE];
+    st.meshlets_skinned = counters[CNT_SKINNED];
+    st.batch_payloads = counters[CNT_BATCH_PAYLOADS];
+    st.clip_triangles = counters[CNT_CLIP_TRIS];
+    st.tile_payloads = counters[CNT_TILE_PAIRS];
+    st.tiles = nt;
+    st.samples = (u32)px;
+    st.pixels = (u32)px;
+    st.shaded_pixels = counters[CNT_SHADED];
+    st.ms_upload = ev_ms(b->ev[0], b->ev[1]);
+    st.ms_geometry = ev_ms(b->ev[1], b->ev[2]);
+    st.ms_binning = ev_ms(b->ev[2], b->ev[3]);
+    st.ms_raster_shade = ev_ms(b->ev[3], b->ev[4]);
+    st.ms_download = ev_ms(b->ev[4], b->ev[5]);
+    st.ms_total = ev_ms(b->ev[0], b->ev[5]);
+
+    /* ---- capture ---- */
+    b->cap_cols = c.tile_cols;
+    b->cap_rows = c.tile_rows;
+    if (cap_vis) {
+        b->h_vid.resize(grid);
+        b->h_depth.resize(grid);
+        HIP_CHECK(hipMemcpy(b->h_vid.data(), b->cap_vid.p, grid * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(b->h_depth.data(), b->cap_depth.p, grid * 4, hipMemcpyDeviceToHost));
+    }
+    if (c.capture_flags & SFTX_CAPTURE_SS_CACHE) {
+        b->h_ss_count = c.total_ss;
+        b->h_ss.resize((size_t)c.total_ss * 2304);
+        b->h_ss_vis.assign(c.total_ss, 0);
+        if (c.total_ss) {
+            HIP_CHECK(hipMemcpy(b->h_ss.data(), b->ss.p, b->h_ss.size(), hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), b->ss_vis.p, c.total_ss, hipMemcpyDeviceToHost));
+        }
+    }
+    if (c.capture_flags & SFTX_CAPTURE_CLIPS) {
+        b->h_clip_count = std::min(counters[CNT_CLIP_TRIS], b->clip_cap);
+        b->h_clips.resize((size_t)b->h_clip_count * 164);
+        if (b->h_clip_count) HIP_CHECK(hipMemcpy(b->h_clips.data(), b->clip_rec.p, b->h_clips.size(), hipMemcpyDeviceToHost));
+    }
+}
+
+}  // namespace sft
+
+using namespace sft;
+
+extern "C" void sftx_cap_grid(i32* cols, i32* rows, i32* samples) {
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (cols) *cols = b ? b->cap_cols : 0;
+    if (rows) *rows = b ? b->cap_rows : 0;
+    if (samples) *samples = 1;
+}
+
+extern "C" void sftx_cap_visibility(u32* vid, f32* depth) {
+    Backend* b = g_ctx->backend;
+    if (vid && !b->h_vid.empty()) memcpy(vid, b->h_vid.data(), b->h_vid.size() * 4);
+    if (depth && !b->h_depth.empty()) memcpy(depth, b->h_depth.data(), b->h_depth.size() * 4);
+}
+
+extern "C" u32 sftx_cap_ss_count(void) { return g_ctx->backend->h_ss_count; }
+
+extern "C" void sftx_cap_ss_cache(void* out) {
+    Backend* b = g_ctx->backend;
+    if (!b->h_ss.empty()) memcpy(out, b->h_ss.data(), b->h_ss.size());
+}
+
+extern "C" void sftx_cap_ss_visible(u8* out) {
+    Backend* b = g_ctx->backend;
+    if (!b->h_ss_vis.empty()) memcpy(out, b->h_ss_vis.data(), b->h_ss_vis.size());
+}
+
+extern "C" u32 sftx_cap_clip_count(void) { return g_ctx->backend->h_clip_count; }
+
+extern "C" void sftx_cap_clip_records(void* out) {
+    Backend* b = g_ctx->backend;
+    if (!b->h_clips.empty()) memcpy(out, b->h_clips.data(), b->h_clips.size());
+}

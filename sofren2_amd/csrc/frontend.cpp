@@ -1,0 +1,965 @@
+/*
+ * frontend.cpp — host side of the public sofren2 API for the HIP backend.
+ *
+ * Everything here records the frame (camera, lights, instances, transient
+ * dmesh geometry) or manages resources (arenas, textures, materials, models);
+ * sft_present hands the recorded frame to the HIP backend (backend.hip), which
+ * does all per-pixel and per-vertex work on the GPU.  Behaviour follows the
+ * reference API function by function (citations per function); host float math
+ * that feeds the GPU (frustum planes, dmesh vertices, mips) keeps the
+ * reference's operation order and is compiled with -ffp-contract=off and
+ * -fno-builtin so both builds produce the same bits.
+ */
+#include <sys/mman.h>
+#include <math.h>
+#include <string.h>
+#include "sft_internal.h"
+
+#define SFT_ERR_EXIT(...) do { \
+    fprintf(stderr, "fatal sofren2 error in '%s':\n\t", __func__); \
+    fprintf(stderr, __VA_ARGS__); \
+    exit(1); \
+} while (0)
+#define SFT_ERR_RET(_r, ...) do { \
+    fprintf(stderr, "sofren2 error in '%s':\n\t", __func__); \
+    fprintf(stderr, __VA_ARGS__); \
+    return _r; \
+} while (0)
+
+namespace sft {
+Context* g_ctx = nullptr;
+}
+using namespace sft;
+
+static Context& C() {
+    if (!g_ctx) {
+        SFT_ERR_EXIT("sofren2 used before sft_init\n");
+    }
+    return *g_ctx;
+}
+
+/* ================================================================ arenas
+   mmap-reserved bump allocators with 64-byte alignment (sofren2.c:9458-9554) */
+extern "C" void sft_arena_init(sft_arena* arena, u64 reserve_size) {
+    arena->max_size = reserve_size;
+    arena->committed_size = 0;
+    arena->offset = 0;
+    arena->commit_lock = 0;
+    arena->base = (u8*)mmap(NULL, reserve_size, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    if (MAP_FAILED == (void*)arena->base) {
+        SFT_ERR_EXIT("mmap reserve failed for size %zu\n", (size_t)reserve_size);
+    }
+    arena->committed_size = reserve_size;
+}
+
+extern "C" void* sft_arena_alloc(sft_arena* arena, u64 size) {
+    if (!arena) {
+        SFT_ERR_RET(NULL, "arena pointer is NULL\n");
+    }
+    const u64 aligned = (arena->offset + 63ULL) & ~63ULL;
+    const u64 next = aligned + size;
+    if (next > arena->max_size) {
+        SFT_ERR_EXIT("arena out of memory, requested %zu KB, max %zu, offset %zu\n",
+            (size_t)(size / 1024), (size_t)(arena->max_size / 1024), (size_t)aligned);
+    }
+    arena->offset = next;
+    return arena->base + aligned;
+}
+
+extern "C" void sft_arena_commit_if_needed(sft_arena* arena, u64 required_bytes) {
+    (void)arena;
+    (void)required_bytes;   /* pages commit on first touch on Linux */
+}
+
+extern "C" void sft_arena_frame_begin(sft_arena* arena) {
+    arena->offset = 0;
+}
+
+/* ================================================================ textures
+   4x4-block swizzled mip chains with an integer tent filter
+   (sofren2.c:9181-9310) */
+static void tex_pool_grow(Context& c) {
+    const i32 old_cap = (i32)c.tex.size();
+    const i32 new_cap = (0 == old_cap) ? 1024 : old_cap * 2;
+    c.tex.resize(new_cap);
+    for (i32 i = old_cap; i < new_cap; i += 1) {
+        c.tex[i].generation = 1;
+        c.tex[i].next_free = i + 1;
+    }
+    c.tex[new_cap - 1].next_free = c.tex_free_head;
+    c.tex_free_head = old_cap;
+}
+
+static i32 ilog2(i32 v) { return 31 - __builtin_clz((u32)v); }
+
+/* copies a linear level into the 4x4-block layout: texel (x,y) goes to
+   ((y/4)*bw + x/4)*16 + (y%4)*4 + x%4 */
+static void swizzle_level(u32* dst, const u32* src, i32 w, i32 h) {
+    const i32 bw = ((w + 3) & ~3) / 4;
+    for (i32 y = 0; y < h; y += 1) {
+        for (i32 x = 0; x < w; x += 1) {
+            dst[((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3)] = src[y * w + x];
+        }
+    }
+}
+
+extern "C" sft_tex_handle sft_texture_create(i32 width, i32 height, const u32* pixels) {
+    Context& c = C();
+    if (width <= 0 || height <= 0 || !pixels) {
+        return SFT_INVALID_HANDLE;
+    }
+    const u8 is_pot = (0 == (width & (width - 1))) && (0 == (height & (height - 1)));
+
+    /* level sizes, each padded to a multiple of 4 in both axes */
+    i32 lw[14], lh[14];
+    i32 levels = 0;
+    u64 total = 0;
+    {
+        i32 cw = width, ch = height;
+        while (levels < 14) {
+            lw[levels] = cw;
+            lh[levels] = ch;
+            total += (u64)(((cw + 3) & ~3) * ((ch + 3) & ~3));
+            levels += 1;
+            if (1 == cw && 1 == ch) break;
+            cw = (cw > 1) ? cw / 2 : 1;
+            ch = (ch > 1) ? ch / 2 : 1;
+        }
+    }
+
+    if (-1 == c.tex_free_head) {
+        tex_pool_grow(c);
+    }
+    const i32 t = c.tex_free_head;
+    HostTexture& ht = c.tex[t];
+    c.tex_free_head = ht.next_free;
+    ht.live = true;
+    ht.w = width;
+    ht.h = height;
+    ht.mip_levels = levels;
+    ht.is_pot = is_pot;
+    ht.texels.assign(total, 0u);
+    ht.uploaded = false;
+
+    u64 off = 0;
+    std::vector<u32> cur(pixels, pixels + (u64)width * height), next;
+    for (i32 l = 0; l < levels; l += 1) {
+        const i32 pw = (lw[l] + 3) & ~3, ph = (lh[l] + 3) & ~3;
+        ht.mip_offset[l] = (u32)off;
+        ht.blocks_w[l] = pw / 4;
+        ht.stride_shift[l] = is_pot ? ilog2(pw / 4) : 0;
+        swizzle_level(ht.texels.data() + off, cur.data(), lw[l], lh[l]);
+        off += (u64)pw * ph;
+        if (l + 1 == levels) break;
+
+        /* [1 2 2 1] x [1 2 2 1] tent with edge clamp, truncating division */
+        const i32 cw = lw[l], ch = lh[l], nw = lw[l + 1], nh = lh[l + 1];
+        next.assign((u64)nw * nh, 0u);
+        for (i32 y = 0; y < nh; y += 1) {
+            for (i32 x = 0; x < nw; x += 1) {
+                u32 r = 0, g = 0, b = 0, a = 0, wsum = 0;
+                for (i32 ky = -1; ky <= 2; ky += 1) {
+                    const i32 sy = SFT_CLAMP(y * 2 + ky, 0, ch - 1);
+                    const u32 wy = (-1 == ky || 2 == ky) ? 1u : 2u;
+                    for (i32 kx = -1; kx <= 2; kx += 1) {
+                        const i32 sx = SFT_CLAMP(x * 2 + kx, 0, cw - 1);
+                        const u32 wx = (-1 == kx || 2 == kx) ? 1u : 2u;
+                        const u32 w = wx * wy;
+                        const u32 px = cur[(u64)sy * cw + sx];
+                        r += ((px >> 16) & 255u) * w;
+                        g += ((px >> 8) & 255u) * w;
+                        b += (px & 255u) * w;
+                        a += ((px >> 24) & 255u) * w;
+                        wsum += w;
+                    }
+                }
+                next[(u64)y * nw + x] = ((a / wsum) << 24) | ((r / wsum) << 16) | ((g / wsum) << 8) | (b / wsum);
+            }
+        }
+        cur.swap(next);
+    }
+    return (ht.generation << SFT_HANDLE_GEN_SHIFT) | (u32)t;
+}
+
+extern "C" sft_tex_handle sft_texture_create_solid(u32 color) {
+    return sft_texture_create(1, 1, &color);
+}
+
+static HostTexture* tex_lookup(sft_tex_handle h) {
+    Context& c = C();
+    if (SFT_INVALID_HANDLE == h) return nullptr;
+    const u32 i = SFT_HANDLE_INDEX(h);
+    if (i >= c.tex.size()) return nullptr;
+    return &c.tex[i];
+}
+
+extern "C" i32 sft_texture_get_width(sft_tex_handle tex) {
+    HostTexture* t = tex_lookup(tex);
+    return t ? t->w : 0;
+}
+
+extern "C" i32 sft_texture_get_height(sft_tex_handle tex) {
+    HostTexture* t = tex_lookup(tex);
+    return t ? t->h : 0;
+}
+
+extern "C" u32* sft_texture_get_linear_pixels(sft_tex_handle tex) {
+    HostTexture* t = tex_lookup(tex);
+    if (!t) return NULL;
+    u32* out = (u32*)malloc((u64)t->w * t->h * sizeof(u32));
+    const i32 bw = t->blocks_w[0];
+    for (i32 y = 0; y < t->h; y += 1) {
+        for (i32 x = 0; x < t->w; x += 1) {
+            out[(u64)y * t->w + x] = t->texels[t->mip_offset[0] + ((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3)];
+        }
+    }
+    return out;
+}
+
+extern "C" void sft_texture_release(sft_tex_handle tex) {
+    Context& c = C();
+    if (SFT_INVALID_HANDLE == tex) return;
+    const u32 i = SFT_HANDLE_INDEX(tex);
+    const u32 gen = tex >> SFT_HANDLE_GEN_SHIFT;
+    if (i >= c.tex.size() || c.tex[i].generation != gen) return;
+    HostTexture& t = c.tex[i];
+    backend_release_texture(c.backend, i);
+    t.texels.clear();
+    t.texels.shrink_to_fit();
+    t.live = false;
+    t.uploaded = false;
+    t.generation += 1;
+    t.next_free = c.tex_free_head;
+    c.tex_free_head = (i32)i;
+}
+
+/* ================================================================ materials
+   static pool with generation-checked handles, transient pool per frame
+   (sofren2.c:9393-9454) */
+static void mat_pool_grow(Context& c) {
+    const i32 old_cap = (i32)c.mat_gen.size();
+    const i32 new_cap = (0 == old_cap) ? 32 : old_cap * 2;
+    c.mat_gen.resize(new_cap);
+    c.mat_next_free.resize(new_cap);
+    c.mat_tex.resize(new_cap, SFT_INVALID_HANDLE);
+    c.mat_color.resize(new_cap, 0xFFFFFFFFu);
+    c.mat_shininess.resize(new_cap, 0.f);
+    for (i32 i = old_cap; i < new_cap; i += 1) {
+        c.mat_gen[i] = 1;
+        c.mat_next_free[i] = i + 1;
+    }
+    c.mat_next_free[new_cap - 1] = c.mat_free_head;
+    c.mat_free_head = old_cap;
+}
+
+extern "C" sft_mat_handle sft_material_create(sft_tex_handle albedo_tex, u32 base_color, f32 shininess) {
+    Context& c = C();
+    if (-1 == c.mat_free_head) mat_pool_grow(c);
+    const i32 i = c.mat_free_head;
+    c.mat_free_head = c.mat_next_free[i];
+    c.mat_tex[i] = albedo_tex;
+    c.mat_color[i] = base_color;
+    c.mat_shininess[i] = shininess;
+    return (c.mat_gen[i] << SFT_MAT_GEN_SHIFT) | (u32)i;
+}
+
+extern "C" void sft_material_release(sft_mat_handle mat) {
+    Context& c = C();
+    if (SFT_INVALID_HANDLE == mat || SFT_MAT_IS_TRANSIENT(mat)) return;
+    const u32 i = SFT_MAT_INDEX(mat);
+    const u32 gen = (mat & SFT_MAT_GEN_MASK) >> SFT_MAT_GEN_SHIFT;
+    if (i >= c.mat_gen.size() || c.mat_gen[i] != gen) return;
+    c.mat_gen[i] += 1;
+    c.mat_next_free[i] = c.mat_free_head;
+    c.mat_free_head = (i32)i;
+}
+
+extern "C" sft_mat_handle sft_material_clone(sft_mat_handle base_mat) {
+    Context& c = C();
+    const u32 n = (u32)c.tmat_tex.size();
+    if (SFT_INVALID_HANDLE != base_mat) {
+        const u32 b = SFT_MAT_INDEX(base_mat);
+        if (SFT_MAT_IS_TRANSIENT(base_mat)) {
+            c.tmat_tex.push_back(c.tmat_tex[b]);
+            c.tmat_color.push_back(c.tmat_color[b]);
+            c.tmat_shininess.push_back(c.tmat_shininess[b]);
+        }
+        else {
+            c.tmat_tex.push_back(c.mat_tex[b]);
+            c.tmat_color.push_back(c.mat_color[b]);
+            c.tmat_shininess.push_back(c.mat_shininess[b]);
+        }
+    }
+    else {
+        c.tmat_tex.push_back(SFT_INVALID_HANDLE);
+        c.tmat_color.push_back(0xFFFFFFFFu);
+        c.tmat_shininess.push_back(0.f);
+    }
+    return SFT_MAT_TRANSIENT_BIT | n;
+}
+
+/* ================================================================ lifecycle */
+static void set_display(Context& c, i32 width, i32 height) {
+    c.width = width;
+    c.height = height;
+    c.half_width = (f32)width * 0.5f;
+    c.half_height = (f32)height * 0.5f;
+    c.vp_x = 0.f;
+    c.vp_y = 0.f;
+    c.tile_cols = (width + SFT_TILE_SIZE - 1) / SFT_TILE_SIZE;
+    c.tile_rows = (height + SFT_TILE_SIZE - 1) / SFT_TILE_SIZE;
+}
+
+/* sofren2.c:7974-8123 */
+extern "C" void sft_init(i32 width, i32 height, i32 requested_threads) {
+    (void)requested_threads;   /* all per-frame work runs on the GPU */
+    if (g_ctx) {
+        SFT_ERR_EXIT("sft_init called twice without sft_release\n");
+    }
+    g_ctx = new Context();
+    Context& c = *g_ctx;
+    sft_arena_init(&c.scratch_arena, 2ULL * 1024 * 1024 * 1024);
+    set_display(c, width, height);
+
+    probe_estimate_tables(&c.est);
+    c.backend = backend_create();
+
+    /* 256x256 checkerboard of 32x32 squares and the base material */
+    std::vector<u32> checker(256 * 256);
+    for (i32 y = 0, i = 0; y < 256; y += 1) {
+        for (i32 x = 0; x < 256; x += 1, i += 1) {
+            checker[i] = (((x / 32) % 2) ^ ((y / 32) % 2)) ? 0xFFAFFFAFu : 0xFF555555u;
+        }
+    }
+    c.base_tex = sft_texture_create(256, 256, checker.data());
+    c.base_mat = sft_material_create(c.base_tex, 0xFFFFFFFFu, 2.f);
+    c.lightmap_brightness = 1.5f;
+    c.clear_color = 0xFF222222u;
+
+    /* dmesh index templates: 16 quads (tris (0,1,2), (0,2,3)) or 21 tris */
+    memset(&c.quad_inds, 0, sizeof(c.quad_inds));
+    memset(&c.tri_inds, 0, sizeof(c.tri_inds));
+    for (u32 tri = 0; tri < 128; tri += 1) {
+        const u32 off = (tri / 8) * 32 + (tri % 8);
+        if (tri < 32) {
+            const u8 bv = (u8)((tri / 2) * 4);
+            c.quad_inds.inds[off + 0] = bv;
+            c.quad_inds.inds[off + 8] = (u8)(bv + ((tri % 2) ? 2 : 1));
+            c.quad_inds.inds[off + 16] = (u8)(bv + ((tri % 2) ? 3 : 2));
+        }
+        if (tri < 21) {
+            const u8 bv = (u8)(tri * 3);
+            c.tri_inds.inds[off + 0] = bv;
+            c.tri_inds.inds[off + 8] = (u8)(bv + 1);
+            c.tri_inds.inds[off + 16] = (u8)(bv + 2);
+        }
+    }
+}
+
+extern "C" void sft_release(void) {
+    if (!g_ctx) return;
+    backend_destroy(g_ctx->backend);
+    munmap(g_ctx->scratch_arena.base, g_ctx->scratch_arena.max_size);
+    delete g_ctx;
+    g_ctx = nullptr;
+}
+
+extern "C" void sft_resize(i32 width, i32 height) {
+    if (!g_ctx) return;
+    set_display(*g_ctx, width, height);
+}
+
+extern "C" void sft_frame_begin(void) {
+    Context& c = C();
+    sft_arena_frame_begin(&c.scratch_arena);
+    c.transient_models.clear();
+}
+
+/* sofren2.c:8301-8312 */
+extern "C" void sft_frame_end(void) {
+    Context& c = C();
+    c.instance_count = 0;
+    c.instances.clear();
+    c.total_ss = 0;
+    c.tmat_tex.clear();
+    c.tmat_color.clear();
+    c.tmat_shininess.clear();
+    c.dir_lights.clear();
+    c.point_lights.clear();
+    c.proxies.clear();
+}
+
+extern "C" void sft_present(void) {
+    Context& c = C();
+    backend_present(c.backend, &c);
+}
+
+/* ================================================================ misc setters */
+/* view_proj = view * proj and six normalised planes (sofren2.c:8345-8376) */
+extern "C" void sft_set_camera(sftvec cam_pos, sftmat view, sftmat proj) {
+    Context& c = C();
+    c.camera_pos = cam_pos;
+    c.view = view;
+    c.proj = proj;
+    c.view_proj = sft_m_mul(view, proj);
+    const sftmat& vp = c.view_proj;
+    f32 planes[6][4];
+    for (i32 i = 0; i < 4; i += 1) {
+        planes[0][i] = vp.m[i][3] + vp.m[i][0];
+        planes[1][i] = vp.m[i][3] - vp.m[i][0];
+        planes[2][i] = vp.m[i][3] + vp.m[i][1];
+        planes[3][i] = vp.m[i][3] - vp.m[i][1];
+        planes[4][i] = vp.m[i][3] + vp.m[i][2];
+        planes[5][i] = vp.m[i][3] - vp.m[i][2];
+    }
+    for (i32 p = 0; p < 6; p += 1) {
+        const f32 len = sqrtf(planes[p][0] * planes[p][0] + planes[p][1] * planes[p][1] + planes[p][2] * planes[p][2]);
+        const f32 inv_len = (len < SFT_EPSILON) ? SFT_EPSILON : 1.f / len;
+        c.planes[p][0] = planes[p][0] * inv_len;
+        c.planes[p][1] = planes[p][1] * inv_len;
+        c.planes[p][2] = planes[p][2] * inv_len;
+        c.planes[p][3] = planes[p][3] * inv_len;
+    }
+}
+
+extern "C" void sft_set_debug_mode(enum sft_debug_mode mode) {
+    if (mode < 0 || mode >= SFT_DEBUG_COUNT) {
+        SFT_ERR_RET(, "invalid debug mode '%d', expected [0, %d)\n", (int)mode, (int)SFT_DEBUG_COUNT);
+    }
+    C().debug_mode = mode;
+}
+
+extern "C" const char* sft_get_debug_name(enum sft_debug_mode mode) {
+    switch (mode) {
+        case SFT_DEBUG_NONE: return "None";
+        case SFT_DEBUG_WIREFRAME: return "Wireframe";
+        case SFT_DEBUG_TILES: return "Tiles";
+        case SFT_DEBUG_DEPTH: return "Depth";
+        case SFT_DEBUG_LIGHT_COUNT: return "Light Count";
+        case SFT_DEBUG_CHECKERBOARD: return "Checkerboard";
+        default: return "INVALID MODE";
+    }
+}
+
+extern "C" void sft_set_clear(u8 enabled) { C().should_clear = enabled; }
+extern "C" void sft_set_clear_color(u32 color) { C().clear_color = color; }
+extern "C" void sft_set_color_buffer(u32* buf, i32 pitch_in_bytes) {
+    Context& c = C();
+    c.color_buffer = buf;
+    c.color_pitch = pitch_in_bytes / (i32)sizeof(u32);
+}
+
+/* ================================================================ lights
+   (sofren2.c:9109-9149) */
+extern "C" void sft_set_ambient_light(f32 r, f32 g, f32 b) {
+    Context& c = C();
+    c.ambient.x = r;
+    c.ambient.y = g;
+    c.ambient.z = b;
+}
+
+extern "C" void sft_draw_directional_light(sftvec dir, f32 r, f32 g, f32 b, f32 intensity) {
+    sft_directional_light l;
+    l.dir = sft_v_norm(dir);
+    l.color.x = r * intensity;
+    l.color.y = g * intensity;
+    l.color.z = b * intensity;
+    C().dir_lights.push_back(l);
+}
+
+extern "C" void sft_draw_point_light(sftvec pos, f32 r, f32 g, f32 b, f32 intensity, f32 radius, u32 light_id) {
+    (void)light_id;
+    sft_point_light l;
+    l.pos = pos;
+    l.color.x = r * intensity;
+    l.color.y = g * intensity;
+    l.color.z = b * intensity;
+    l.radius = radius;
+    l.inv_radius_sqr = 1.f / (radius * radius);
+    C().point_lights.push_back(l);
+}
+
+extern "C" void sft_set_shadows_enabled(u8 enabled) { C().shadows_enabled = enabled; }
+extern "C" void sft_set_lightmap_brightness(f32 b) { C().lightmap_brightness = b; }
+
+/* ================================================================ instances
+   (sofren2.c:8410-8478): slot claim with the 64-instance cap, screen-space
+   cache offset, world-space shadow proxies.  Geometry jobs are implicit: the
+   backend walks instances, meshes and meshlets in this same order. */
+extern "C" void sft_draw_instance(const sft_instance* inst) {
+    Context& c = C();
+    const i32 slot = c.instance_count++;
+    if (slot >= SFT_MAX_INSTANCES_PER_FRAME) {
+        return;
+    }
+    sft_instance copy = *inst;
+    copy.ss_meshlet_offset = c.total_ss;
+    c.total_ss += inst->model->total_meshlets;
+    c.instances.push_back(copy);
+
+    if (c.shadows_enabled && inst->model && inst->model->shadow_proxy_count > 0) {
+        for (u32 i = 0; i < inst->model->shadow_proxy_count; i += 1) {
+            const sft_shadow_proxy* local = &inst->model->shadow_proxies[i];
+            sftmat m = inst->transform;
+            if (local->bone_ind != 0xFFFFFFFFu && inst->posed_bone_matrices) {
+                m = sft_m_mul(inst->posed_bone_matrices[local->bone_ind], inst->transform);
+            }
+            WorldProxy w;
+            memset(&w, 0, sizeof(w));
+            w.posa.x = local->posa.x * m.m[0][0] + local->posa.y * m.m[1][0] + local->posa.z * m.m[2][0] + m.m[3][0];
+            w.posa.y = local->posa.x * m.m[0][1] + local->posa.y * m.m[1][1] + local->posa.z * m.m[2][1] + m.m[3][1];
+            w.posa.z = local->posa.x * m.m[0][2] + local->posa.y * m.m[1][2] + local->posa.z * m.m[2][2] + m.m[3][2];
+            w.posb.x = local->posb.x * m.m[0][0] + local->posb.y * m.m[1][0] + local->posb.z * m.m[2][0] + m.m[3][0];
+            w.posb.y = local->posb.x * m.m[0][1] + local->posb.y * m.m[1][1] + local->posb.z * m.m[2][1] + m.m[3][1];
+            w.posb.z = local->posb.x * m.m[0][2] + local->posb.y * m.m[1][2] + local->posb.z * m.m[2][2] + m.m[3][2];
+            const f32 sx = sqrtf(m.m[0][0] * m.m[0][0] + m.m[0][1] * m.m[0][1] + m.m[0][2] * m.m[0][2]);
+            w.radius = local->radius * sx;
+            c.proxies.push_back(w);
+        }
+    }
+}
+
+/* ================================================================ dmesh
+   transient meshlets built on the host each frame (sofren2.c:2816-2883,
+   8480-8938) */
+static void dmesh_finalize_chunk(sft_dmesh* dm) {
+    Context& c = C();
+    if (0 == dm->curr_vert) return;
+    const u32 mi = dm->curr_meshlet;
+    if (mi >= dm->max_meshlets) return;
+    sft_meshlet* m = &dm->meshlets[mi];
+    sft_meshlet_attribs* a = &dm->attribs[mi];
+
+    /* unused trailing vertices collapse onto vertex 0 so template triangles
+       that reference them are degenerate */
+    for (u32 i = dm->curr_vert; i < SFT_VERTS_PER_MESHLET; i += 1) {
+        m->x[i] = m->x[0]; m->y[i] = m->y[0]; m->z[i] = m->z[0];
+        a->nx[i] = a->nx[0]; a->ny[i] = a->ny[0]; a->nz[i] = a->nz[0];
+        a->u[i] = a->u[0]; a->v[i] = a->v[0];
+        a->lu[i] = a->lu[0]; a->lv[i] = a->lv[0];
+    }
+
+    f32 mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f, mnz = 1e30f, mxz = -1e30f;
+    for (u32 i = 0; i < dm->curr_vert; i += 1) {
+        const f32 x = m->x[i], y = m->y[i], z = m->z[i];
+        if (x < mnx) mnx = x;
+        if (x > mxx) mxx = x;
+        if (y < mny) mny = y;
+        if (y > mxy) mxy = y;
+        if (z < mnz) mnz = z;
+        if (z > mxz) mxz = z;
+    }
+    const f32 cx = (mnx + mxx) * 0.5f, cy = (mny + mxy) * 0.5f, cz = (mnz + mxz) * 0.5f;
+    const f32 dx = mxx - cx, dy = mxy - cy, dz = mxz - cz;
+    dm->bounds_min_x[mi] = mnx; dm->bounds_max_x[mi] = mxx;
+    dm->bounds_min_y[mi] = mny; dm->bounds_max_y[mi] = mxy;
+    dm->bounds_min_z[mi] = mnz; dm->bounds_max_z[mi] = mxz;
+    dm->bounds_center_x[mi] = cx;
+    dm->bounds_center_y[mi] = cy;
+    dm->bounds_center_z[mi] = cz;
+    dm->bounds_radius[mi] = sqrtf(dx * dx + dy * dy + dz * dz);
+    /* cutoff 2 disables the normal-cone test */
+    dm->cone_axis_x[mi] = 0.f;
+    dm->cone_axis_y[mi] = 0.f;
+    dm->cone_axis_z[mi] = 1.f;
+    dm->cone_cutoff[mi] = 2.f;
+    memcpy(&dm->inds[mi], (SFT_DMESH_QUADS == dm->type) ? &c.quad_inds : &c.tri_inds, sizeof(sft_meshlet_inds));
+    dm->curr_meshlet += 1;
+    dm->curr_vert = 0;
+}
+
+extern "C" sft_dmesh* sft_dmesh_begin(enum sft_dmesh_type type, sft_mat_handle material, u32 max_prims) {
+    Context& c = C();
+    if (0 == max_prims) return NULL;
+    const u32 per_chunk = (SFT_DMESH_QUADS == type) ? 64u : 63u;
+    const u32 per_prim = (SFT_DMESH_QUADS == type) ? 4u : 3u;
+    const u32 total = max_prims * per_prim;
+    const u32 max_meshlets = (((total + per_chunk - 1) / per_chunk) + 7u) & ~7u;
+    sft_arena* ar = &c.scratch_arena;
+    sft_dmesh* dm = (sft_dmesh*)sft_arena_alloc(ar, sizeof(sft_dmesh));
+    dm->type = type;
+    dm->material = material;
+    dm->max_meshlets = max_meshlets;
+    dm->curr_meshlet = 0;
+    dm->curr_vert = 0;
+    dm->max_verts_per_chunk = per_chunk;
+    dm->meshlets = (sft_meshlet*)sft_arena_alloc(ar, max_meshlets * sizeof(sft_meshlet));
+    dm->attribs = (sft_meshlet_attribs*)sft_arena_alloc(ar, max_meshlets * sizeof(sft_meshlet_attribs));
+    dm->inds = (sft_meshlet_inds*)sft_arena_alloc(ar, max_meshlets * sizeof(sft_meshlet_inds));
+    f32** arrays[] = {
+        &dm->bounds_center_x, &dm->bounds_center_y, &dm->bounds_center_z, &dm->bounds_radius,
+        &dm->bounds_min_x, &dm->bounds_min_y, &dm->bounds_min_z,
+        &dm->bounds_max_x, &dm->bounds_max_y, &dm->bounds_max_z,
+        &dm->cone_axis_x, &dm->cone_axis_y, &dm->cone_axis_z, &dm->cone_cutoff };
+    for (f32** a : arrays) {
+        *a = (f32*)sft_arena_alloc(ar, max_meshlets * sizeof(f32));
+    }
+    return dm;
+}
+
+extern "C" void sft_dmesh_push_quad_4n(sft_dmesh* dm, sftvec p0, sftvec p1, sftvec p2, sftvec p3,
+                                       f32 u0, f32 v0, f32 u1, f32 v1,
+                                       sftvec n0, sftvec n1, sftvec n2, sftvec n3) {
+    if (dm->curr_vert >= dm->max_verts_per_chunk) dmesh_finalize_chunk(dm);
+    if (dm->curr_meshlet >= dm->max_meshlets) return;
+    const u32 v = dm->curr_vert;
+    sft_meshlet* m = &dm->meshlets[dm->curr_meshlet];
+    sft_meshlet_attribs* a = &dm->attribs[dm->curr_meshlet];
+    const sftvec p[4] = { p0, p1, p2, p3 };
+    const sftvec n[4] = { n0, n1, n2, n3 };
+    const f32 us[4] = { u0, u1, u1, u0 };
+    const f32 vs[4] = { v0, v0, v1, v1 };
+    for (u32 i = 0; i < 4; i += 1) {
+        m->x[v + i] = p[i].x; m->y[v + i] = p[i].y; m->z[v + i] = p[i].z;
+        a->u[v + i] = us[i]; a->v[v + i] = vs[i];
+        a->nx[v + i] = n[i].x; a->ny[v + i] = n[i].y; a->nz[v + i] = n[i].z;
+        a->lu[v + i] = 0.f; a->lv[v + i] = 0.f;
+    }
+    dm->curr_vert += 4;
+}
+
+extern "C" void sft_dmesh_push_quad(sft_dmesh* dm, sftvec p0, sftvec p1, sftvec p2, sftvec p3,
+                                    f32 u0, f32 v0, f32 u1, f32 v1, sftvec normal) {
+    sft_dmesh_push_quad_4n(dm, p0, p1, p2, p3, u0, v0, u1, v1, normal, normal, normal, normal);
+}
+
+extern "C" void sft_dmesh_push_tri(sft_dmesh* dm, sftvec p0, sftvec p1, sftvec p2,
+                                   f32 u0, f32 v0, f32 u1, f32 v1, f32 u2, f32 v2, sftvec normal) {
+    if (dm->curr_vert >= dm->max_verts_per_chunk) dmesh_finalize_chunk(dm);
+    if (dm->curr_meshlet >= dm->max_meshlets) return;
+    const u32 v = dm->curr_vert;
+    sft_meshlet* m = &dm->meshlets[dm->curr_meshlet];
+    sft_meshlet_attribs* a = &dm->attribs[dm->curr_meshlet];
+    const sftvec p[3] = { p0, p1, p2 };
+    const f32 us[3] = { u0, u1, u2 };
+    const f32 vs[3] = { v0, v1, v2 };
+    for (u32 i = 0; i < 3; i += 1) {
+        m->x[v + i] = p[i].x; m->y[v + i] = p[i].y; m->z[v + i] = p[i].z;
+        a->u[v + i] = us[i]; a->v[v + i] = vs[i];
+        a->nx[v + i] = normal.x; a->ny[v + i] = normal.y; a->nz[v + i] = normal.z;
+        a->lu[v + i] = 0.f; a->lv[v + i] = 0.f;
+    }
+    dm->curr_vert += 3;
+}
+
+extern "C" u8 sft_dmesh_alloc_quads(sft_dmesh* dm, u32 count, sft_dmesh_qb* out) {
+    u32 space = (dm->max_verts_per_chunk - dm->curr_vert) / 4;
+    if (0 == space) {
+        dmesh_finalize_chunk(dm);
+        space = dm->max_verts_per_chunk / 4;
+    }
+    if (dm->curr_meshlet >= dm->max_meshlets) return 0;
+    const u32 n = (count < space) ? count : space;
+    sft_meshlet* m = &dm->meshlets[dm->curr_meshlet];
+    sft_meshlet_attribs* a = &dm->attribs[dm->curr_meshlet];
+    out->x = m->x; out->y = m->y; out->z = m->z;
+    out->nx = a->nx; out->ny = a->ny; out->nz = a->nz;
+    out->u = a->u; out->v = a->v;
+    out->vert_offset = dm->curr_vert;
+    out->allocated_count = n;
+    dm->curr_vert += n * 4;
+    return 1;
+}
+
+/* wraps the dmesh in a one-mesh transient model and draws it (8881-8938) */
+extern "C" void sft_dmesh_end(sft_dmesh* dm, const sftmat* opt_transform) {
+    Context& c = C();
+    if (!dm) return;
+    if (dm->curr_vert > 0) dmesh_finalize_chunk(dm);
+    if (0 == dm->curr_meshlet) return;
+    sft_arena* ar = &c.scratch_arena;
+    sft_mesh* mesh = (sft_mesh*)sft_arena_alloc(ar, sizeof(sft_mesh));
+    memset(mesh, 0, sizeof(sft_mesh));
+    mesh->meshlets = dm->meshlets;
+    mesh->meshlet_attribs = dm->attribs;
+    mesh->inds = dm->inds;
+    mesh->meshlet_count = dm->curr_meshlet;
+    mesh->material = dm->material;
+    mesh->lightmap_page = 0xFFFFFFFFu;
+    mesh->bounds_center_x = dm->bounds_center_x;
+    mesh->bounds_center_y = dm->bounds_center_y;
+    mesh->bounds_center_z = dm->bounds_center_z;
+    mesh->bounds_radius = dm->bounds_radius;
+    mesh->bounds_min_x = dm->bounds_min_x;
+    mesh->bounds_min_y = dm->bounds_min_y;
+    mesh->bounds_min_z = dm->bounds_min_z;
+    mesh->bounds_max_x = dm->bounds_max_x;
+    mesh->bounds_max_y = dm->bounds_max_y;
+    mesh->bounds_max_z = dm->bounds_max_z;
+    mesh->cone_axis_x = dm->cone_axis_x;
+    mesh->cone_axis_y = dm->cone_axis_y;
+    mesh->cone_axis_z = dm->cone_axis_z;
+    mesh->cone_cutoff = dm->cone_cutoff;
+    sft_model* model = (sft_model*)sft_arena_alloc(ar, sizeof(sft_model));
+    memset(model, 0, sizeof(sft_model));
+    model->meshes = mesh;
+    model->mesh_count = 1;
+    model->total_meshlets = dm->curr_meshlet;
+    model->meshlet_mesh_inds = (u16*)sft_arena_alloc(ar, dm->curr_meshlet * sizeof(u16));
+    memset(model->meshlet_mesh_inds, 0, dm->curr_meshlet * sizeof(u16));
+    c.transient_models.push_back(model);
+
+    sft_instance inst;
+    memset(&inst, 0, sizeof(inst));
+    inst.model = model;
+    inst.transform = opt_transform ? *opt_transform : sft_m_identity();
+    sft_draw_instance(&inst);
+}
+
+/* ---- primitive generators (same vertex math as sofren2.c:8480-8879, which
+   follows raylib's DrawCube / DrawSphereEx / DrawCapsule / DrawCylinderEx) */
+extern "C" void sft_draw_cube(sft_dmesh* dm) {
+    const f32 h = 0.5f;
+    sft_dmesh_push_quad(dm, SFT_V(-h,-h,-h), SFT_V(-h, h,-h), SFT_V( h, h,-h), SFT_V( h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 0.f, -1.f));
+    sft_dmesh_push_quad(dm, SFT_V( h,-h,-h), SFT_V( h, h,-h), SFT_V( h, h, h), SFT_V( h,-h, h), 0.f,0.f, 1.f,1.f, SFT_V(1.f, 0.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V( h,-h, h), SFT_V( h, h, h), SFT_V(-h, h, h), SFT_V(-h,-h, h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 0.f, 1.f));
+    sft_dmesh_push_quad(dm, SFT_V(-h,-h, h), SFT_V(-h, h, h), SFT_V(-h, h,-h), SFT_V(-h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(-1.f, 0.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V(-h, h,-h), SFT_V(-h, h, h), SFT_V( h, h, h), SFT_V( h, h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 1.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V( h,-h, h), SFT_V(-h,-h, h), SFT_V(-h,-h,-h), SFT_V( h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, -1.f, 0.f));
+}
+
+extern "C" void sft_draw_cube_inv(sft_dmesh* dm) {
+    const f32 h = 0.5f;
+    sft_dmesh_push_quad(dm, SFT_V( h, h,-h), SFT_V(-h, h,-h), SFT_V(-h,-h,-h), SFT_V( h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 0.f, 1.f));
+    sft_dmesh_push_quad(dm, SFT_V( h, h, h), SFT_V( h, h,-h), SFT_V( h,-h,-h), SFT_V( h,-h, h), 0.f,0.f, 1.f,1.f, SFT_V(-1.f, 0.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V(-h, h, h), SFT_V( h, h, h), SFT_V( h,-h, h), SFT_V(-h,-h, h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 0.f, -1.f));
+    sft_dmesh_push_quad(dm, SFT_V(-h, h,-h), SFT_V(-h, h, h), SFT_V(-h,-h, h), SFT_V(-h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(1.f, 0.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V( h, h, h), SFT_V(-h, h, h), SFT_V(-h, h,-h), SFT_V( h, h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, -1.f, 0.f));
+    sft_dmesh_push_quad(dm, SFT_V(-h,-h,-h), SFT_V(-h,-h, h), SFT_V( h,-h, h), SFT_V( h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 1.f, 0.f));
+}
+
+/* UV sphere: rings+1 latitude bands of `slices` quads, starting at the pole
+   and rotating the generating edge about y then z */
+extern "C" void sft_draw_sphere(sft_dmesh* dm, i32 rings, i32 slices) {
+    const f32 ring_angle = (f32)(SFT_PI / (rings + 1));
+    const f32 slice_angle = (f32)(2.f * SFT_PI / slices);
+    const f32 cr = cosf(ring_angle), sr = sinf(ring_angle);
+    const f32 cs = cosf(slice_angle), ss = sinf(slice_angle);
+    sftvec v[4] = { {0, 0, 0}, {0, 0, 0}, {0.f, 1.f, 0.f}, {sr, cr, 0.f} };
+    for (i32 i = 0; i <= rings; i += 1) {
+        for (i32 j = 0; j < slices; j += 1) {
+            v[0] = v[2];
+            v[1] = v[3];
+            const sftvec a = v[2], b = v[3];
+            v[2] = SFT_V(cs * a.x - ss * a.z, a.y, ss * a.x + cs * a.z);
+            v[3] = SFT_V(cs * b.x - ss * b.z, b.y, ss * b.x + cs * b.z);
+            sft_dmesh_push_quad_4n(dm, v[0], v[2], v[3], v[1], 0.f, 0.f, 1.f, 1.f,
+                sft_v_norm(v[0]), sft_v_norm(v[2]), sft_v_norm(v[3]), sft_v_norm(v[1]));
+        }
+        v[2] = v[3];
+        const sftvec b = v[3];
+        v[3] = SFT_V(cr * b.x + sr * b.y, -sr * b.x + cr * b.y, b.z);
+    }
+}
+
+static sftvec capsule_ring_point(sftvec c, sftvec b0, sftvec b1, sftvec b2, f32 sl, f32 rg, f32 radius) {
+    const f32 rs = sinf(sl) * cosf(rg);
+    const f32 rc = cosf(sl) * cosf(rg);
+    const f32 sg = sinf(rg);
+    return SFT_V(c.x + (sg * b0.x + rs * b1.x + rc * b2.x) * radius,
+                 c.y + (sg * b0.y + rs * b1.y + rc * b2.y) * radius,
+                 c.z + (sg * b0.z + rs * b1.z + rc * b2.z) * radius);
+}
+
+static sftvec ring_point(sftvec c, sftvec b1, sftvec b2, f32 s, f32 k) {
+    return SFT_V(c.x + s * b1.x + k * b2.x, c.y + s * b1.y + k * b2.y, c.z + s * b1.z + k * b2.z);
+}
+
+extern "C" void sft_draw_capsule(sft_dmesh* dm, sftvec start_pos, sftvec end_pos, f32 radius, i32 slices, i32 rings) {
+    if (slices < 3) slices = 3;
+    sftvec dir = sft_v_sub(end_pos, start_pos);
+    const u8 sphere_case = (0.f == dir.x && 0.f == dir.y && 0.f == dir.z);
+    if (sphere_case) dir = SFT_V(0.f, 1.f, 0.f);
+    sftvec b0 = sft_v_norm(dir);
+    const sftvec b1 = sft_v_norm(sft_v_perpendicular(dir));
+    const sftvec b2 = sft_v_norm(sft_v_cross(b1, dir));
+    sftvec cap = end_pos;
+    const f32 sa = (f32)(2.f * SFT_PI / slices);
+    const f32 ra = (f32)(0.5f * SFT_PI / rings);
+    for (i32 c = 0; c < 2; c += 1) {
+        for (i32 i = 0; i < rings; i += 1) {
+            for (i32 j = 0; j < slices; j += 1) {
+                const sftvec w1 = capsule_ring_point(cap, b0, b1, b2, sa * (j + 0), ra * (i + 0), radius);
+                const sftvec w2 = capsule_ring_point(cap, b0, b1, b2, sa * (j + 1), ra * (i + 0), radius);
+                const sftvec w3 = capsule_ring_point(cap, b0, b1, b2, sa * (j + 0), ra * (i + 1), radius);
+                const sftvec w4 = capsule_ring_point(cap, b0, b1, b2, sa * (j + 1), ra * (i + 1), radius);
+                const sftvec centre = sft_v_mul(sft_v_add(sft_v_add(w1, w2), sft_v_add(w3, w4)), 0.25f);
+                const sftvec n = sft_v_norm(sft_v_sub(centre, cap));
+                if (0 == c) sft_dmesh_push_quad(dm, w3, w1, w2, w4, 0.f, 0.f, 1.f, 1.f, n);
+                else        sft_dmesh_push_quad(dm, w3, w4, w2, w1, 0.f, 0.f, 1.f, 1.f, n);
+            }
+        }
+        cap = start_pos;
+        b0 = sft_v_mul(b0, -1.f);
+    }
+    if (!sphere_case) {
+        for (i32 j = 0; j < slices; j += 1) {
+            const f32 s1 = sinf(sa * (j + 0)) * radius, c1 = cosf(sa * (j + 0)) * radius;
+            const f32 s2 = sinf(sa * (j + 1)) * radius, c2 = cosf(sa * (j + 1)) * radius;
+            const sftvec w1 = ring_point(start_pos, b1, b2, s1, c1);
+            const sftvec w2 = ring_point(start_pos, b1, b2, s2, c2);
+            const sftvec w3 = ring_point(end_pos, b1, b2, s1, c1);
+            const sftvec w4 = ring_point(end_pos, b1, b2, s2, c2);
+            sft_dmesh_push_quad(dm, w3, w1, w2, w4, 0.f, 0.f, 1.f, 1.f, sft_v_face_normal(w3, w1, w2));
+        }
+    }
+}
+
+extern "C" void sft_draw_cylinder(sft_dmesh* dm, sftvec start_pos, sftvec end_pos, f32 start_radius, f32 end_radius, i32 sides) {
+    if (sides < 3) sides = 3;
+    const sftvec dir = sft_v_sub(end_pos, start_pos);
+    if (0 == dir.x && 0 == dir.y && 0 == dir.z) return;
+    const sftvec b1 = sft_v_norm(sft_v_perpendicular(dir));
+    const sftvec b2 = sft_v_norm(sft_v_cross(b1, dir));
+    const f32 a = (f32)((2.f * SFT_PI) / sides);
+    for (i32 i = 0; i < sides; i += 1) {
+        const f32 s1 = sinf(a * (i + 0)) * start_radius, c1 = cosf(a * (i + 0)) * start_radius;
+        const f32 s2 = sinf(a * (i + 1)) * start_radius, c2 = cosf(a * (i + 1)) * start_radius;
+        const f32 s3 = sinf(a * (i + 0)) * end_radius, c3 = cosf(a * (i + 0)) * end_radius;
+        const f32 s4 = sinf(a * (i + 1)) * end_radius, c4 = cosf(a * (i + 1)) * end_radius;
+        const sftvec w1 = ring_point(start_pos, b1, b2, s1, c1);
+        const sftvec w2 = ring_point(start_pos, b1, b2, s2, c2);
+        const sftvec w3 = ring_point(end_pos, b1, b2, s3, c3);
+        const sftvec w4 = ring_point(end_pos, b1, b2, s4, c4);
+        if (start_radius > 0) {
+            sft_dmesh_push_quad(dm, start_pos, w2, w1, w1, 0.f, 0.f, 1.f, 1.f, sft_v_face_normal(start_pos, w2, w1));
+        }
+        sft_dmesh_push_quad(dm, w3, w1, w2, w4, 0.f, 0.f, 1.f, 1.f, sft_v_face_normal(w3, w1, w2));
+        if (end_radius > 0) {
+            sft_dmesh_push_quad(dm, end_pos, w3, w4, w4, 0.f, 0.f, 1.f, 1.f, sft_v_face_normal(end_pos, w3, w4));
+        }
+    }
+}
+
+/* ================================================================ poses
+   Keyframe animation import is outside the accelerated path; a pose holds
+   per-bone matrices the caller may fill, initialised to identity. */
+extern "C" sft_pose* sft_pose_create(const sft_model* model, sft_arena* arena) {
+    if (!model || !arena) return NULL;
+    sft_pose* p = (sft_pose*)sft_arena_alloc(arena, sizeof(sft_pose));
+    p->model = model;
+    const i32 n = model->skeleton.bone_count > 0 ? model->skeleton.bone_count : 1;
+    p->global_matrices = (sftmat*)sft_arena_alloc(arena, n * sizeof(sftmat));
+    p->posed_bone_matrices = (sftmat*)sft_arena_alloc(arena, n * sizeof(sftmat));
+    for (i32 i = 0; i < n; i += 1) {
+        p->global_matrices[i] = sft_m_identity();
+        p->posed_bone_matrices[i] = sft_m_identity();
+    }
+    return p;
+}
+
+extern "C" void sft_pose_update(sft_pose* pose, i32 anim_id, f32 time) {
+    (void)pose;
+    (void)anim_id;
+    (void)time;
+}
+
+/* ================================================================ QSF2 loader
+   Binary scene format (sofren2.c:7816-7967): header, textures (each becomes a
+   material), lightmap pages, meshes with SoA meshlets, flat triangle indices
+   re-laid into 8-wide batches, bounds, cones, meshlet->mesh map, BLAS nodes. */
+static bool rd(FILE* f, void* dst, u64 bytes) { return 0 == bytes || 1 == fread(dst, bytes, 1, f); }
+
+extern "C" sft_model* sft_model_load_qsf(const char* path, sft_arena* arena) {
+    Context& c = C();
+    FILE* f = fopen(path, "rb");
+    if (!f) SFT_ERR_RET(NULL, "failed to open '%s'\n", path);
+    char magic[4];
+    if (4 != fread(magic, 1, 4, f) || 0 != memcmp(magic, "QSF2", 4)) {
+        fclose(f);
+        SFT_ERR_RET(NULL, "invalid QSF2 magic in '%s'\n", path);
+    }
+    u32 version = 0;
+    rd(f, &version, 4);
+    if (2 != version) {
+        fclose(f);
+        SFT_ERR_RET(NULL, "unsupported QSF2 version\n");
+    }
+    u64 game_data_offset = 0;
+    rd(f, &game_data_offset, 8);
+
+    u32 tex_count = 0;
+    rd(f, &tex_count, 4);
+    sft_mat_handle* mats = (sft_mat_handle*)sft_arena_alloc(arena, (u64)tex_count * sizeof(sft_mat_handle) + 1);
+    std::vector<u32> px;
+    for (u32 i = 0; i < tex_count; i += 1) {
+        u32 w = 0, h = 0;
+        rd(f, &w, 4);
+        rd(f, &h, 4);
+        px.resize((u64)w * h);
+        rd(f, px.data(), (u64)w * h * 4);
+        mats[i] = sft_material_create(sft_texture_create((i32)w, (i32)h, px.data()), 0xFFFFFFFFu, 2.f);
+    }
+
+    sft_model* model = (sft_model*)sft_arena_alloc(arena, sizeof(sft_model));
+    memset(model, 0, sizeof(sft_model));
+    rd(f, &model->lightmap_count, 4);
+    if (model->lightmap_count > 0) {
+        model->lightmaps = (sft_tex_handle*)sft_arena_alloc(arena, model->lightmap_count * sizeof(sft_tex_handle));
+        for (u32 i = 0; i < model->lightmap_count; i += 1) {
+            u32 w = 0, h = 0;
+            rd(f, &w, 4);
+            rd(f, &h, 4);
+            px.resize((u64)w * h);
+            rd(f, px.data(), (u64)w * h * 4);
+            model->lightmaps[i] = sft_texture_create((i32)w, (i32)h, px.data());
+        }
+    }
+    rd(f, &model->mesh_count, 4);
+    rd(f, &model->total_meshlets, 4);
+    rd(f, &model->blas_node_count, 4);
+    rd(f, &model->blas_root_ind, 4);
+
+    model->meshes = (sft_mesh*)sft_arena_alloc(arena, model->mesh_count * sizeof(sft_mesh));
+    memset(model->meshes, 0, model->mesh_count * sizeof(sft_mesh));
+    for (u32 i = 0; i < model->mesh_count; i += 1) {
+        sft_mesh* mesh = &model->meshes[i];
+        rd(f, &mesh->meshlet_count, 4);
+        rd(f, &mesh->model_meshlet_offset, 4);
+        u32 tex_ind = 0;
+        rd(f, &tex_ind, 4);
+        mesh->material = (tex_ind < tex_count) ? mats[tex_ind] : c.base_mat;
+        rd(f, &mesh->lightmap_page, 4);
+        const u32 mc = mesh->meshlet_count;
+        mesh->meshlets = (sft_meshlet*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet));
+        rd(f, mesh->meshlets, (u64)mc * sizeof(sft_meshlet));
+        mesh->meshlet_attribs = (sft_meshlet_attribs*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet_attribs));
+        rd(f, mesh->meshlet_attribs, (u64)mc * sizeof(sft_meshlet_attribs));
+        mesh->inds = (sft_meshlet_inds*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet_inds));
+        u8 flat[384];
+        for (u32 m = 0; m < mc; m += 1) {
+            rd(f, flat, 384);
+            sft_meshlet_inds* mi = &mesh->inds[m];
+            memset(mi, 0, sizeof(*mi));
+            for (u32 t = 0; t < 128; t += 1) {
+                const u32 off = (t / 8) * 32 + (t % 8);
+                mi->inds[off + 0] = flat[t * 3 + 0];
+                mi->inds[off + 8] = flat[t * 3 + 1];
+                mi->inds[off + 16] = flat[t * 3 + 2];
+            }
+        }
+        f32** groups[] = {
+            &mesh->bounds_center_x, &mesh->bounds_center_y, &mesh->bounds_center_z, &mesh->bounds_radius,
+            &mesh->bounds_min_x, &mesh->bounds_min_y, &mesh->bounds_min_z,
+            &mesh->bounds_max_x, &mesh->bounds_max_y, &mesh->bounds_max_z,
+            &mesh->cone_axis_x, &mesh->cone_axis_y, &mesh->cone_axis_z, &mesh->cone_cutoff };
+        for (f32** a : groups) {
+            *a = (f32*)sft_arena_alloc(arena, (u64)mc * sizeof(f32));
+            rd(f, *a, (u64)mc * sizeof(f32));
+        }
+    }
+    if (model->total_meshlets > 0) {
+        model->meshlet_mesh_inds = (u16*)sft_arena_alloc(arena, model->total_meshlets * sizeof(u16));
+        rd(f, model->meshlet_mesh_inds, model->total_meshlets * sizeof(u16));
+    }
+    if (model->blas_node_count > 0) {
+        model->blas_nodes = (sft_bvh_node*)sft_arena_alloc(arena, model->blas_node_count * sizeof(sft_bvh_node));
+        rd(f, model->blas_nodes, model->blas_node_count * sizeof(sft_bvh_node));
+    }
+    fclose(f);
+    return model;
+}
+
+/* ================================================================ capture + stats */
+extern "C" void sftx_set_capture(u32 flags) { C().capture_flags = flags; }
+extern "C" void sftx_get_stats(sftx_frame_stats* out) { *out = C().stats; }

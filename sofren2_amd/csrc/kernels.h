@@ -1,0 +1,73 @@
+/*
+ * kernels.h — device pointer bundle of one frame and the launch wrappers of the
+ * render-path kernels (kernels.hip).  Host orchestration is in backend.cpp.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sft_internal.h"
+
+namespace sft {
+
+/* counters written by the kernels (u32 each) */
+enum Counter : int {
+    CNT_VISIBLE = 0,        /* meshlets surviving the cull */
+    CNT_SKINNED,            /* visible skinned meshlets */
+    CNT_BATCH_PAYLOADS,     /* batch payloads with at least one triangle */
+    CNT_BATCH_TILE_PAIRS,   /* (batch payload, tile) pairs */
+    CNT_CLIP_TRIS,          /* clipped-triangle slots used */
+    CNT_CLIP_TILE_PAIRS,    /* (clip payload, tile) pairs */
+    CNT_TILE_PAIRS,         /* total entries of the tile lists */
+    CNT_OVERFLOW,           /* bit 0: clip pool, bit 1: tile lists */
+    CNT_SHADED,             /* pixels with geometry */
+    CNT_COUNT
+};
+
+struct FramePtrs {
+    /* resident */
+    const f32* pos;         /* [slot][3][64] */
+    const f32* attr;        /* [slot][7][64] */
+    const u8* inds;         /* [slot][512] */
+    const f32* cull;        /* [slot][8] */
+    const u32* slot_mesh;   /* [slot] */
+    const u8* skin;         /* [skin][768] */
+    const MeshRec* mesh_recs;
+    const u32* texels;
+    const TexMeta* tex_meta;
+    const f32* est_rcp;
+    const f32* est_rsq;
+    /* per frame inputs */
+    const FrameConst* fc;
+    const InstGpu* insts;
+    const WorkItem* work;
+    const f32* bones;       /* [n][16] */
+    const PointLightGpu* point_lights;
+    const DirLightGpu* dir_lights;
+    const u32* mat_tex;     /* static material pool */
+    const u32* mat_color;
+    const f32* mat_shin;
+    const u32* tmat_tex;    /* transient material pool */
+    const u32* tmat_color;
+    const f32* tmat_shin;
+    /* per frame outputs / scratch */
+    f32* ss;                /* [ss][9][64] */
+    u8* ss_vis;             /* [ss] */
+    u32* clip_count;        /* [n_work] */
+    u32* clip_base;         /* [n_work] */
+    u64* clip_mask;         /* [n_work][2] */
+    uint2* batch_pl;        /* [n_work][16]: x = lane mask, y = tile rect */
+    u32* tile_count;        /* [n_tiles] */
+    u32* tile_off;          /* [n_tiles] */
+    u32* tile_fill;         /* [n_tiles] */
+    u64* tile_list;         /* [tile_list_cap] */
+    f32* clip_rec;          /* [clip_cap][41] */
+    u32* clip_key;          /* [clip_cap] */
+    u32* clip_rect;         /* [clip_cap] */
+    u32* color;             /* [height][width] */
+    u32* cap_vid;           /* [S][grid] when capturing */
+    f32* cap_depth;
+    u32* counters;          /* [CNT_COUNT] */
+};
+
+void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev);
+
+}  // namespace sft

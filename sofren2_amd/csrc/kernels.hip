@@ -1,0 +1,1139 @@
+/*
+ * kernels.hip — the per-frame render path on gfx950.
+ *
+ * Pipeline of one sft_present (one HIP stream):
+ *   k_geom     one wave64 per drawn meshlet, lane = vertex: frustum + normal-cone
+ *              cull, 4-joint skinning, world transform, projection into the
+ *              screen-space cache, then lane = triangle (two passes of 64):
+ *              near-plane classification, area cull, 64-px tile bounds and the
+ *              8-triangle batch payloads (sofren2.c:4901-5145, 4775-4884).
+ *   k_scan     exclusive scan of clipped-triangle counts in emission order, so
+ *              clip slots are numbered exactly as the reference's single worker
+ *              numbers them (sofren2.c:4351-4361).
+ *   k_clip     one wave per meshlet with straddling triangles: near-plane
+ *              clipping into slot-numbered records (sofren2.c:4245-4381,
+ *              4691-4772).
+ *   k_scan     exclusive scan of per-tile payload counts.
+ *   k_scatter  writes payload descriptors into per-tile lists.
+ *   k_tile     one workgroup per 64x64 tile: rasterises the tile's payloads into
+ *              an LDS visibility buffer, then resolves and shades it
+ *              (sofren2.c:6756-6813, 5148-5499, 5809-6753).
+ *
+ * Bit-exactness: every float operation of the reference is replayed in the
+ * same order with explicit fmaf where the reference uses an FMA intrinsic, the
+ * file is compiled with -ffp-contract=off, and the host's rcpps/rsqrtps
+ * estimates come from tables probed at init.  SSE min/max (return the second
+ * operand unless the first compares smaller/larger) and cvttps (out of range or
+ * NaN gives INT_MIN) are emulated.  The raster replays the reference's
+ * incremental edge/depth accumulation (row steps of B, 8-pixel chunk steps of
+ * 8A from an 8-aligned start inside the 64x64 tile).  Depth ties are resolved
+ * without ordering the payload lists: each fragment does an LDS 64-bit atomic
+ * max of (depth bits << 32 | ~emission_key), and the emission key (meshlet in
+ * job order, batch, clipped-before-batch, lane, clip output) orders fragments
+ * exactly as the reference's single-threaded payload order does, so the
+ * earliest-drawn fragment wins at equal depth as with its strict depth test.
+ */
+#include "kernels.h"
+
+namespace sft {
+
+#define DEV __device__ __forceinline__
+
+DEV u32 f2u(f32 f) { return __float_as_uint(f); }
+DEV f32 u2f(u32 u) { return __uint_as_float(u); }
+/* _mm256_min_ps(a, b) / _mm256_max_ps(a, b) per lane */
+DEV f32 smin(f32 a, f32 b) { return a < b ? a : b; }
+DEV f32 smax(f32 a, f32 b) { return a > b ? a : b; }
+/* _mm256_cvttps_epi32 per lane */
+DEV i32 cvtt(f32 x) { return (x >= -2147483648.0f && x < 2147483648.0f) ? (i32)x : (i32)0x80000000; }
+
+struct V3 { f32 x, y, z; };
+DEV f32 dot3(V3 a, V3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }   /* sft_vv_dot */
+DEV V3 sub3(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+DEV V3 add3(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+/* sft_vv_transform with a row-major 4x4 (sofren2.c:1993-2012) */
+DEV V3 xform(V3 v, const f32* m) {
+    V3 r;
+    r.x = fmaf(v.x, m[0], fmaf(v.y, m[4], fmaf(v.z, m[8], m[12])));
+    r.y = fmaf(v.x, m[1], fmaf(v.y, m[5], fmaf(v.z, m[9], m[13])));
+    r.z = fmaf(v.x, m[2], fmaf(v.y, m[6], fmaf(v.z, m[10], m[14])));
+    return r;
+}
+/* sft_vv_transform_dir (sofren2.c:2013-2030) */
+DEV V3 xform_dir(V3 v, const f32* m) {
+    V3 r;
+    r.x = fmaf(v.x, m[0], fmaf(v.y, m[4], v.z * m[8]));
+    r.y = fmaf(v.x, m[1], fmaf(v.y, m[5], v.z * m[9]));
+    r.z = fmaf(v.x, m[2], fmaf(v.y, m[6], v.z * m[10]));
+    return r;
+}
+
+/* host rcpps estimate: table value for the mantissa bucket, exponent rescaled */
+DEV f32 rcp_est(const f32* tab, u32 bits, f32 a) {
+    const u32 b = f2u(a);
+    const u32 sign = b & 0x80000000u;
+    const u32 e = (b >> 23) & 0xFFu;
+    const u32 m = b & 0x7FFFFFu;
+    if (0u == e) return u2f(sign | 0x7F800000u);
+    if (255u == e) return m ? u2f(b | 0x00400000u) : u2f(sign);
+    const u32 tb = f2u(tab[m >> (23u - bits)]);
+    const i32 oe = (i32)((tb >> 23) & 0xFFu) + 127 - (i32)e;
+    if (oe <= 0) return u2f(sign);
+    return u2f(sign | ((u32)oe << 23) | (tb & 0x7FFFFFu));
+}
+
+/* host rsqrtps estimate: parity of the exponent selects [1,2) or [2,4) */
+DEV f32 rsq_est(const f32* tab, u32 bits, f32 a) {
+    const u32 b = f2u(a);
+    const u32 e = (b >> 23) & 0xFFu;
+    const u32 m = b & 0x7FFFFFu;
+    if (0u == e) return u2f((b & 0x80000000u) | 0x7F800000u);
+    if (b & 0x80000000u) return u2f(0xFFC00000u);
+    if (255u == e) return m ? u2f(b | 0x00400000u) : 0.f;
+    const i32 u = (i32)e - 127;
+    const i32 p = u & 1;
+    const u32 tb = f2u(tab[((u32)p << bits) | (m >> (23u - bits))]);
+    const i32 oe = (i32)((tb >> 23) & 0xFFu) - (u - p) / 2;
+    return u2f(((u32)oe << 23) | (tb & 0x7FFFFFu));
+}
+
+struct Est {
+    const f32* rcp;
+    const f32* rsq;
+    u32 rcp_bits, rsq_bits;
+    /* sft_vf_rcp_refined: estimate plus one Newton step (sofren2.c:632-639) */
+    DEV f32 rcp_refined(f32 a) const {
+        const f32 x = rcp_est(rcp, rcp_bits, a);
+        return x * fmaf(-a, x, 2.0f);
+    }
+    /* sft_vv_norm_fast (sofren2.c:1973-1992) */
+    DEV V3 norm_fast(V3 v) const {
+        const f32 sq = dot3(v, v);
+        const f32 ap = rsq_est(rsq, rsq_bits, sq);
+        const f32 ap2 = ap * ap;
+        const f32 r = (0.5f * ap) * fmaf(-sq, ap2, 3.0f);
+        return V3{v.x * r, v.y * r, v.z * r};
+    }
+};
+
+DEV Est make_est(const FramePtrs& P, const FrameConst& fc) {
+    Est e;
+    e.rcp = P.est_rcp;
+    e.rsq = P.est_rsq;
+    e.rcp_bits = fc.rcp_bits;
+    e.rsq_bits = fc.rsq_bits;
+    return e;
+}
+
+DEV i32 wave_sum(i32 v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+/* ================================================================== k_geom */
+__global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const u32 wi = blockIdx.x * 4u + (u32)wave;
+    if (wi >= fc.n_work) return;
+    const Est est = make_est(P, fc);
+
+    const WorkItem w = P.work[wi];
+    const InstGpu* inst = &P.insts[w.inst];
+    f32 T[16];
+    for (int i = 0; i < 16; i += 1) T[i] = inst->t[i];
+
+    /* ---- cull (sofren2.c:4942-4994) ---- */
+    const f32* cd = P.cull + (u64)w.slot * 8u;
+    V3 c{cd[0], cd[1], cd[2]};
+    V3 ax{cd[4], cd[5], cd[6]};
+    if (w.cull_bone >= 0) {
+        const f32* B = P.bones + (u64)(inst->bone_base + (u32)w.cull_bone) * 16u;
+        c = xform(c, B);
+        ax = xform_dir(ax, B);
+    }
+    c = xform(c, T);
+    ax = xform_dir(ax, T);
+    const f32 scale = sqrtf(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+    const f32 radius = cd[3] * scale;
+    f32 alen = sqrtf(dot3(ax, ax));
+    alen = smax(alen, 1e-6f);
+    ax = V3{ax.x / alen, ax.y / alen, ax.z / alen};
+    bool inside = true;
+    for (int p = 0; p < 6; p += 1) {
+        const V3 n{fc.planes[p][0], fc.planes[p][1], fc.planes[p][2]};
+        const f32 dist = dot3(c, n) + fc.planes[p][3];
+        inside = inside && (dist + radius >= 0.f);
+    }
+    const V3 raw = sub3(c, V3{fc.cam[0], fc.cam[1], fc.cam[2]});
+    const f32 d = sqrtf(dot3(raw, raw));
+    const f32 dv = dot3(raw, ax);
+    inside = inside && (dv <= fmaf(d, cd[7], radius));
+
+    if (0 == lane) P.ss_vis[w.ss] = inside ? 1 : 0;
+    if (!inside) {
+        if (lane < 16) P.batch_pl[wi * 16u + lane] = make_uint2(0u, 0u);
+        if (lane < 2) P.clip_mask[wi * 2u + lane] = 0ull;
+        if (0 == lane) P.clip_count[wi] = 0u;
+        return;
+    }
+
+    /* ---- vertex: skin, world, project (sofren2.c:5016-5123) ---- */
+    const f32* pp = P.pos + (u64)w.slot * 192u;
+    const f32* ap = P.attr + (u64)w.slot * 448u;
+    const V3 v{pp[lane], pp[64 + lane], pp[128 + lane]};
+    const V3 n{ap[lane], ap[64 + lane], ap[128 + lane]};
+    V3 vw, nw;
+    if (kInvalid != w.skin) {
+        const u8* sk = P.skin + (u64)w.skin * 768u;
+        const u16* j = (const u16*)sk;
+        const f32 inv255 = 1.f / 255.f;
+        const f32 w0 = (f32)sk[512 + lane] * inv255, w1 = (f32)sk[576 + lane] * inv255;
+        const f32 w2 = (f32)sk[640 + lane] * inv255, w3 = (f32)sk[704 + lane] * inv255;
+        const f32* bb = P.bones + (u64)inst->bone_base * 16u;
+        const f32* M0 = bb + (u64)j[lane] * 16u;
+        const f32* M1 = bb + (u64)j[64 + lane] * 16u;
+        const f32* M2 = bb + (u64)j[128 + lane] * 16u;
+        const f32* M3 = bb + (u64)j[192 + lane] * 16u;
+        const V3 p0 = xform(v, M0), q0 = xform_dir(n, M0);
+        const V3 p1 = xform(v, M1), q1 = xform_dir(n, M1);
+        const V3 p2 = xform(v, M2), q2 = xform_dir(n, M2);
+        const V3 p3 = xform(v, M3), q3 = xform_dir(n, M3);
+        vw.x = fmaf(p0.x, w0, fmaf(p1.x, w1, fmaf(p2.x, w2, p3.x * w3)));
+        vw.y = fmaf(p0.y, w0, fmaf(p1.y, w1, fmaf(p2.y, w2, p3.y * w3)));
+        vw.z = fmaf(p0.z, w0, fmaf(p1.z, w1, fmaf(p2.z, w2, p3.z * w3)));
+        nw.x = fmaf(q0.x, w0, fmaf(q1.x, w1, fmaf(q2.x, w2, q3.x * w3)));
+        nw.y = fmaf(q0.y, w0, fmaf(q1.y, w1, fmaf(q2.y, w2, q3.y * w3)));
+        nw.z = fmaf(q0.z, w0, fmaf(q1.z, w1, fmaf(q2.z, w2, q3.z * w3)));
+        vw = xform(vw, T);
+        nw = xform_dir(nw, T);
+    }
+    else {
+        vw = xform(v, T);
+        nw = xform_dir(n, T);
+    }
+    const f32* VP = fc.vp;
+    const f32 cw = fmaf(vw.x, VP[3], fmaf(vw.y, VP[7], fmaf(vw.z, VP[11], VP[15])));
+    const f32 tx = fmaf(vw.x, VP[0], fmaf(vw.y, VP[4], fmaf(vw.z, VP[8], VP[12])));
+    const f32 ty = fmaf(vw.x, VP[1], fmaf(vw.y, VP[5], fmaf(vw.z, VP[9], VP[13])));
+    const f32 tz = fmaf(vw.x, VP[2], fmaf(vw.y, VP[6], fmaf(vw.z, VP[10], VP[14])));
+    const f32 inv_w = est.rcp_refined(cw);
+    const f32 sx = fmaf(tx, inv_w * fc.hw, fc.bias_x);
+    const f32 sy = fmaf(ty, inv_w * fc.neg_hh, fc.bias_y);
+
+    f32* ss = P.ss + (u64)w.ss * 576u;
+    ss[lane] = sx;
+    ss[64 + lane] = sy;
+    ss[128 + lane] = inv_w;
+    ss[192 + lane] = vw.x;
+    ss[256 + lane] = vw.y;
+    ss[320 + lane] = vw.z;
+    ss[384 + lane] = nw.x;
+    ss[448 + lane] = nw.y;
+    ss[512 + lane] = nw.z;
+
+    /* ---- triangles: classify, area cull, tile bounds (sofren2.c:4775-4884) ---- */
+    const u8* ib = P.inds + (u64)w.slot * 512u;
+    const i32 max_tx = fc.tile_cols - 1, max_ty = fc.tile_rows - 1;
+    i32 n_clip = 0, n_payloads = 0, n_pairs = 0;
+    for (int half = 0; half < 2; half += 1) {
+        const int t = half * 64 + lane;
+        const int b = t >> 3, l = t & 7;
+        const i32 i0 = ib[b * 32 + l], i1 = ib[b * 32 + 8 + l], i2 = ib[b * 32 + 16 + l];
+        const f32 tz0 = __shfl(tz, i0), tz1 = __shfl(tz, i1), tz2 = __shfl(tz, i2);
+        const f32 sx0 = __shfl(sx, i0), sx1 = __shfl(sx, i1), sx2 = __shfl(sx, i2);
+        const f32 sy0 = __shfl(sy, i0), sy1 = __shfl(sy, i1), sy2 = __shfl(sy, i2);
+
+        const bool valid = (i0 | i1 | i2) > 0;
+        const bool in0 = tz0 >= 0.f, in1 = tz1 >= 0.f, in2 = tz2 >= 0.f;
+        const bool out0 = tz0 < 0.f, out1 = tz1 < 0.f, out2 = tz2 < 0.f;
+        const bool all_in = in0 && in1 && in2;
+        const bool all_out = out0 && out1 && out2;
+        const bool inter = valid && !(all_in || all_out);
+        const u32 cm = (u32)in0 | ((u32)in1 << 1) | ((u32)in2 << 2);
+        if (inter && 0u != cm) n_clip += (1 == __popc(cm)) ? 1 : 2;
+        const u64 inter_bits = __ballot(inter);
+        if (0 == lane) P.clip_mask[wi * 2u + half] = inter_bits;
+
+        const f32 dx1 = sx1 - sx0, dy1 = sy1 - sy0, dx2 = sx2 - sx0, dy2 = sy2 - sy0;
+        const bool area_ok = fmaf(dx1, dy2, -(dy1 * dx2)) > 0.1f;
+        const i32 tx0 = cvtt(smin(sx0, smin(sx1, sx2))) >> 6;
+        const i32 tx1 = cvtt(smax(sx0, smax(sx1, sx2))) >> 6;
+        const i32 ty0 = cvtt(smin(sy0, smin(sy1, sy2))) >> 6;
+        const i32 ty1 = cvtt(smax(sy0, smax(sy1, sy2))) >> 6;
+        const bool in_grid = tx1 >= 0 && tx0 <= max_tx && ty1 >= 0 && ty0 <= max_ty;
+        const bool ok = all_in && area_ok && in_grid;
+
+        const u64 okb = __ballot(ok);
+        const u32 mask8 = (u32)(okb >> (lane & ~7)) & 0xFFu;
+        i32 bx0 = ok ? tx0 : 9999, bx1 = ok ? tx1 : -9999;
+        i32 by0 = ok ? ty0 : 9999, by1 = ok ? ty1 : -9999;
+        for (int o = 1; o < 8; o <<= 1) {
+            bx0 = min(bx0, __shfl_xor(bx0, o));
+            bx1 = max(bx1, __shfl_xor(bx1, o));
+            by0 = min(by0, __shfl_xor(by0, o));
+            by1 = max(by1, __shfl_xor(by1, o));
+        }
+        uint2 pl = make_uint2(0u, 0u);
+        if (mask8) {
+            bx0 = max(bx0, 0); bx1 = min(bx1, max_tx);
+            by0 = max(by0, 0); by1 = min(by1, max_ty);
+            pl = make_uint2(mask8, (u32)bx0 | ((u32)bx1 << 8) | ((u32)by0 << 16) | ((u32)by1 << 24));
+            const i32 cols = bx1 - bx0 + 1;
+            const i32 nt = cols * (by1 - by0 + 1);
+            for (i32 k = l; k < nt; k += 8) {
+                atomicAdd(&P.tile_count[(by0 + k / cols) * fc.tile_cols + bx0 + k % cols], 1u);
+            }
+            if (0 == l) {
+                n_payloads += 1;
+                n_pairs += nt;
+            }
+        }
+        if (0 == l) P.batch_pl[wi * 16u + (u32)b] = pl;
+    }
+    n_clip = wave_sum(n_clip);
+    n_payloads = wave_sum(n_payloads);
+    n_pairs = wave_sum(n_pairs);
+    if (0 == lane) {
+        P.clip_count[wi] = (u32)n_clip;
+        atomicAdd(&P.counters[CNT_VISIBLE], 1u);
+        if (kInvalid != w.skin) atomicAdd(&P.counters[CNT_SKINNED], 1u);
+        if (n_payloads) atomicAdd(&P.counters[CNT_BATCH_PAYLOADS], (u32)n_payloads);
+        if (n_pairs) atomicAdd(&P.counters[CNT_BATCH_TILE_PAIRS], (u32)n_pairs);
+    }
+}
+
+/* ================================================================== k_scan
+   exclusive scan of n u32 by one workgroup of 1024 threads */
+__global__ __launch_bounds__(1024) void k_scan(const u32* in, u32* out, u32 n, u32* total, u32 cap, u32* overflow, u32 overflow_bit) {
+    __shared__ u32 s_wave[16];
+    __shared__ u32 s_carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (0 == tid) s_carry = 0u;
+    __syncthreads();
+    for (u32 base = 0; base < n; base += 4096u) {
+        u32 v[4];
+        u32 sum = 0;
+        for (int k = 0; k < 4; k += 1) {
+            const u32 i = base + (u32)tid * 4u + (u32)k;
+            v[k] = (i < n) ? in[i] : 0u;
+            sum += v[k];
+        }
+        u32 incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const u32 t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (63 == lane) s_wave[wave] = incl;
+        __syncthreads();
+        if (0 == wave) {
+            u32 x = (lane < 16) ? s_wave[lane] : 0u;
+            for (int o = 1; o < 16; o <<= 1) {
+                const u32 t = __shfl_up(x, o);
+                if (lane >= o) x += t;
+            }
+            if (lane < 16) s_wave[lane] = x;
+        }
+        __syncthreads();
+        u32 run = s_carry + (wave ? s_wave[wave - 1] : 0u) + (incl - sum);
+        for (int k = 0; k < 4; k += 1) {
+            const u32 i = base + (u32)tid * 4u + (u32)k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (0 == tid) s_carry += s_wave[15];
+        __syncthreads();
+    }
+    if (0 == tid) {
+        *total = s_carry;
+        if (s_carry > cap) atomicOr(overflow, overflow_bit);
+    }
+}
+
+/* ================================================================== k_clip */
+struct ClipVert { f32 tx, ty, tz, w, u, v, lu, lv, px, py, pz, nx, ny, nz; };
+
+DEV ClipVert lerp_cv(const ClipVert& a, const ClipVert& b, f32 t) {
+    ClipVert r;
+    r.tx = a.tx + t * (b.tx - a.tx);
+    r.ty = a.ty + t * (b.ty - a.ty);
+    r.tz = a.tz + t * (b.tz - a.tz);
+    r.w = a.w + t * (b.w - a.w);
+    r.u = a.u + t * (b.u - a.u);
+    r.v = a.v + t * (b.v - a.v);
+    r.lu = a.lu + t * (b.lu - a.lu);
+    r.lv = a.lv + t * (b.lv - a.lv);
+    r.px = a.px + t * (b.px - a.px);
+    r.py = a.py + t * (b.py - a.py);
+    r.pz = a.pz + t * (b.pz - a.pz);
+    r.nx = a.nx + t * (b.nx - a.nx);
+    r.ny = a.ny + t * (b.ny - a.ny);
+    r.nz = a.nz + t * (b.nz - a.nz);
+    return r;
+}
+
+/* clip-space vertex rebuilt from the cached world position; same FMA order as
+   k_geom, so the bits equal the ones the classification used */
+DEV ClipVert load_cv(const FrameConst& fc, const f32* ss, const f32* ap, int i) {
+    ClipVert c;
+    const f32 x = ss[192 + i], y = ss[256 + i], z = ss[320 + i];
+    const f32* VP = fc.vp;
+    c.w = fmaf(x, VP[3], fmaf(y, VP[7], fmaf(z, VP[11], VP[15])));
+    c.tx = fmaf(x, VP[0], fmaf(y, VP[4], fmaf(z, VP[8], VP[12])));
+    c.ty = fmaf(x, VP[1], fmaf(y, VP[5], fmaf(z, VP[9], VP[13])));
+    c.tz = fmaf(x, VP[2], fmaf(y, VP[6], fmaf(z, VP[10], VP[14])));
+    c.u = ap[192 + i];
+    c.v = ap[256 + i];
+    c.lu = ap[320 + i];
+    c.lv = ap[384 + i];
+    c.px = x;
+    c.py = y;
+    c.pz = z;
+    c.nx = ss[384 + i];
+    c.ny = ss[448 + i];
+    c.nz = ss[512 + i];
+    return c;
+}
+
+__global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const u32 wi = blockIdx.x * 4u + (u32)wave;
+    if (wi >= fc.n_work) return;
+    if (0u == P.clip_count[wi]) return;
+    const WorkItem w = P.work[wi];
+    const f32* ss = P.ss + (u64)w.ss * 576u;
+    const f32* ap = P.attr + (u64)w.slot * 448u;
+    const u8* ib = P.inds + (u64)w.slot * 512u;
+    const i32 max_tx = fc.tile_cols - 1, max_ty = fc.tile_rows - 1;
+    u32 base = P.clip_base[wi];
+    i32 pairs = 0;
+    for (int half = 0; half < 2; half += 1) {
+        const int t = half * 64 + lane;
+        const int b = t >> 3, l = t & 7;
+        const bool inter = (P.clip_mask[wi * 2u + half] >> lane) & 1ull;
+        ClipVert in[3];
+        u32 cm = 0u;
+        int nout = 0;
+        if (inter) {
+            const int idx[3] = { ib[b * 32 + l] & 63, ib[b * 32 + 8 + l] & 63, ib[b * 32 + 16 + l] & 63 };
+            for (int k = 0; k < 3; k += 1) {
+                in[k] = load_cv(fc, ss, ap, idx[k]);
+                cm |= (u32)(in[k].tz >= 0.f) << k;
+            }
+            nout = (0u == cm) ? 0 : ((1 == __popc(cm)) ? 1 : 2);
+        }
+        /* exclusive prefix of outputs in triangle order */
+        i32 incl = nout;
+        for (int o = 1; o < 64; o <<= 1) {
+            const i32 tv = __shfl_up(incl, o);
+            if (lane >= o) incl += tv;
+        }
+        const u32 first = base + (u32)(incl - nout);
+        base += (u32)__shfl(incl, 63);
+        if (0 == nout) continue;
+
+        ClipVert out[2][3];
+        if (1 == nout) {
+            const int a = (int)(cm >> 1);
+            const int bi = (2 == a) ? 0 : a + 1;
+            const int ci = (0 == a) ? 2 : a - 1;
+            const f32 da = in[a].tz;
+            const f32 t_ab = da / (da - in[bi].tz);
+            const f32 t_ac = da / (da - in[ci].tz);
+            out[0][0] = in[a];
+            out[0][1] = lerp_cv(in[a], in[bi], t_ab);
+            out[0][2] = lerp_cv(in[a], in[ci], t_ac);
+        }
+        else {
+            const int o = (int)((~cm & 7u) >> 1);
+            const int a = (0 == o) ? 2 : o - 1;
+            const int ci = (2 == o) ? 0 : o + 1;
+            const f32 da = in[a].tz, db = in[o].tz, dc = in[ci].tz;
+            const f32 t_ab = da / (da - db);
+            const f32 t_cb = dc / (dc - db);
+            const ClipVert nab = lerp_cv(in[a], in[o], t_ab);
+            const ClipVert ncb = lerp_cv(in[ci], in[o], t_cb);
+            out[0][0] = in[a]; out[0][1] = nab; out[0][2] = in[ci];
+            out[1][0] = nab;   out[1][1] = ncb; out[1][2] = in[ci];
+        }
+        for (int c = 0; c < nout; c += 1) {
+            const u32 slot = first + (u32)c;
+            if (slot >= fc.clip_cap) {
+                atomicOr(&P.counters[CNT_OVERFLOW], 1u);
+                continue;
+            }
+            f32* rec = P.clip_rec + (u64)slot * 41u;
+            f32 sxv[3], syv[3];
+            for (int k = 0; k < 3; k += 1) {
+                const ClipVert& q = out[c][k];
+                const f32 iw = 1.f / q.w;
+                sxv[k] = (q.tx * iw + 1.f) * fc.hw + fc.vp_x;
+                syv[k] = (-q.ty * iw + 1.f) * fc.hh + fc.vp_y;
+                rec[0 + k] = sxv[k];
+                rec[3 + k] = syv[k];
+                rec[6 + k] = iw;
+                rec[9 + k] = q.u;
+                rec[12 + k] = q.v;
+                rec[15 + k] = q.lu;
+                rec[18 + k] = q.lv;
+                rec[21 + k] = q.px;
+                rec[24 + k] = q.py;
+                rec[27 + k] = q.pz;
+                rec[30 + k] = q.nx;
+                rec[33 + k] = q.ny;
+                rec[36 + k] = q.nz;
+            }
+            ((u32*)rec)[39] = w.inst;
+            ((u32*)rec)[40] = w.gm;
+            P.clip_key[slot] = (wi << 9) | ((u32)b << 5) | (u32)(l * 2 + c);
+
+            u32 rect = kInvalid;
+            const f32 area = (sxv[1] - sxv[0]) * (syv[2] - syv[0]) - (syv[1] - syv[0]) * (sxv[2] - sxv[0]);
+            if (!(area <= 0.1f)) {
+                const i32 x0 = cvtt(floorf(smin(sxv[0], smin(sxv[1], sxv[2])))) >> 6;
+                const i32 x1 = cvtt(ceilf(smax(sxv[0], smax(sxv[1], sxv[2])))) >> 6;
+                const i32 y0 = cvtt(floorf(smin(syv[0], smin(syv[1], syv[2])))) >> 6;
+                const i32 y1 = cvtt(ceilf(smax(syv[0], smax(syv[1], syv[2])))) >> 6;
+                if (!(x1 < 0 || x0 > max_tx || y1 < 0 || y0 > max_ty)) {
+                    const i32 cx0 = max(x0, 0), cx1 = min(x1, max_tx);
+                    const i32 cy0 = max(y0, 0), cy1 = min(y1, max_ty);
+                    rect = (u32)cx0 | ((u32)cx1 << 8) | ((u32)cy0 << 16) | ((u32)cy1 << 24);
+                    for (i32 ty = cy0; ty <= cy1; ty += 1) {
+                        for (i32 tx = cx0; tx <= cx1; tx += 1) {
+                            atomicAdd(&P.tile_count[ty * fc.tile_cols + tx], 1u);
+                        }
+                    }
+                    pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+                }
+            }
+            P.clip_rect[slot] = rect;
+        }
+    }
+    pairs = wave_sum(pairs);
+    if (0 == lane && pairs) atomicAdd(&P.counters[CNT_CLIP_TILE_PAIRS], (u32)pairs);
+}
+
+/* ================================================================== k_scatter
+   descriptor: bits 0-31 index (work*16+batch, or clip slot), bits 32-39 lane
+   mask (0 marks a clipped triangle, as in the reference payload) */
+__global__ __launch_bounds__(256) void k_scatter(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    const u32 nb = fc.n_work * 16u;
+    const u32 nclip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
+    const u32 total = nb + nclip;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        u32 rect;
+        u64 desc;
+        if (i < nb) {
+            const uint2 pl = P.batch_pl[i];
+            if (0u == pl.x) continue;
+            rect = pl.y;
+            desc = ((u64)pl.x << 32) | (u64)i;
+        }
+        else {
+            const u32 slot = i - nb;
+            rect = P.clip_rect[slot];
+            if (kInvalid == rect) continue;
+            desc = (u64)slot;
+        }
+        const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
+        for (i32 ty = y0; ty <= y1; ty += 1) {
+            for (i32 tx = x0; tx <= x1; tx += 1) {
+                const u32 t = (u32)(ty * fc.tile_cols + tx);
+                const u32 pos = P.tile_off[t] + atomicAdd(&P.tile_fill[t], 1u);
+                if (pos < fc.tile_list_cap) P.tile_list[pos] = desc;
+            }
+        }
+    }
+}
+
+/* ================================================================== k_tile */
+struct Tri {
+    f32 sx[3], sy[3], iw[3];
+    f32 u[3], v[3], lu[3], lv[3];
+    f32 px[3], py[3], pz[3];
+    f32 nx[3], ny[3], nz[3];
+    u32 inst, slot;
+};
+
+/* rasterises one triangle of the tile, lane j = x offset inside an 8-pixel chunk
+   (sofren2.c:5213-5497, non-MSAA path) */
+DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, u32 key,
+                    f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2) {
+    const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
+    const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
+    const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
+    const f32 m01 = sx0 * sy1, m10 = sy0 * sx1;
+    const f32 C0 = (fmaf(sx0, sy1, -m10) - fmaf(sy0, sx1, -m01)) * 0.5f;
+    const f32 m12 = sx1 * sy2, m21 = sy1 * sx2;
+    const f32 C1 = (fmaf(sx1, sy2, -m21) - fmaf(sy1, sx2, -m12)) * 0.5f;
+    const f32 m20 = sx2 * sy0, m02 = sy2 * sx0;
+    const f32 C2 = (fmaf(sx2, sy0, -m02) - fmaf(sy2, sx0, -m20)) * 0.5f;
+    const f32 dx1 = sx1 - sx0, dx2 = sx2 - sx0;
+    const f32 dy1 = sy1 - sy0, dy2 = sy2 - sy0;
+    const f32 det = dx1 * dy2 - dx2 * dy1;
+    if (!(det > 1e-6f)) return;
+    const f32 inv_det = est.rcp_refined(det);
+    const f32 dzdx = ((iw1 - iw0) * dy2 - (iw2 - iw0) * dy1) * inv_det;
+    const f32 dzdy = ((iw2 - iw0) * dx1 - (iw1 - iw0) * dx2) * inv_det;
+    const f32 c0 = fmaf(B0, 0.5f, fmaf(A0, 0.5f, C0));
+    const f32 c1 = fmaf(B1, 0.5f, fmaf(A1, 0.5f, C1));
+    const f32 c2 = fmaf(B2, 0.5f, fmaf(A2, 0.5f, C2));
+    const f32 zb = fmaf(0.5f - sy0, dzdy, fmaf(0.5f - sx0, dzdx, iw0));
+
+    const i32 min_x = cvtt(smax(smin(sx0, smin(sx1, sx2)), (f32)tmx));
+    const i32 max_x = cvtt(smin(smax(sx0, smax(sx1, sx2)), (f32)(tmx + 63)));
+    const i32 min_y = cvtt(smax(smin(sy0, smin(sy1, sy2)), (f32)tmy));
+    const i32 max_y = cvtt(smin(smax(sy0, smax(sy1, sy2)), (f32)(tmy + 63)));
+    if (min_x > max_x || min_y > max_y) return;
+    const i32 amin = min_x & ~7;
+    const i32 amax = (max_x & ~7) + 8;
+    const f32 famin = (f32)amin, fminy = (f32)min_y;
+    const f32 se0 = c0 + A0 * famin + B0 * fminy;
+    const f32 se1 = c1 + A1 * famin + B1 * fminy;
+    const f32 se2 = c2 + A2 * famin + B2 * fminy;
+    const f32 sz = zb + dzdx * famin + dzdy * fminy;
+    const f32 box_w = (f32)(amax - 1 - amin);
+    const f32 box_h = (f32)(max_y - min_y);
+    const f32 me0 = se0 + (A0 < 0.f ? A0 * box_w : 0.f) + (B0 < 0.f ? B0 * box_h : 0.f);
+    const f32 me1 = se1 + (A1 < 0.f ? A1 * box_w : 0.f) + (B1 < 0.f ? B1 * box_h : 0.f);
+    const f32 me2 = se2 + (A2 < 0.f ? A2 * box_w : 0.f) + (B2 < 0.f ? B2 * box_h : 0.f);
+    const bool trivial = me0 >= 0.f && me1 >= 0.f && me2 >= 0.f;
+
+    const u32 nkey = ~key;
+    const f32 fj = (f32)j;
+    const f32 dz_chunk = dzdx * 8.f;
+    f32 z_row = sz + dzdx * fj;
+    const int row0 = (min_y - tmy) * 64 + (amin - tmx) + j;
+    if (trivial) {
+        for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
+            f32 z = z_row;
+            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
+                if (z > 0.f) atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
+                z = z + dz_chunk;
+            }
+            z_row = z_row + dzdy;
+        }
+    }
+    else {
+        const f32 A0c = A0 * 8.f, A1c = A1 * 8.f, A2c = A2 * 8.f;
+        f32 e0r = se0 + A0 * fj, e1r = se1 + A1 * fj, e2r = se2 + A2 * fj;
+        for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
+            f32 e0 = e0r, e1 = e1r, e2 = e2r, z = z_row;
+            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
+                if (e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f) {
+                    atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
+                }
+                e0 = e0 + A0c;
+                e1 = e1 + A1c;
+                e2 = e2 + A2c;
+                z = z + dz_chunk;
+            }
+            e0r = e0r + B0;
+            e1r = e1r + B1;
+            e2r = e2r + B2;
+            z_row = z_row + dzdy;
+        }
+    }
+}
+
+/* resolves a winning emission key to the reference's 32-bit visibility ID */
+DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip) {
+    const u32 sub = key & 31u;
+    if (sub >= 16u) {
+        const WorkItem w = P.work[key >> 9];
+        const u32 tri = ((key >> 5) & 15u) * 8u + (sub - 16u);
+        return (w.inst << (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) | (w.gm << SFT_TRIANGLE_BITS) | tri;
+    }
+    /* clip keys are increasing in slot order */
+    u32 lo = 0u, hi = n_clip;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (P.clip_key[mid] < key) lo = mid + 1u;
+        else hi = mid;
+    }
+    return lo | 0x80000000u;
+}
+
+DEV void load_tri(const FramePtrs& P, u32 vid, Tri& t) {
+    if (vid & 0x80000000u) {
+        const f32* r = P.clip_rec + (u64)(vid & 0x7FFFFFFFu) * 41u;
+        for (int k = 0; k < 3; k += 1) {
+            t.sx[k] = r[0 + k]; t.sy[k] = r[3 + k]; t.iw[k] = r[6 + k];
+            t.u[k] = r[9 + k]; t.v[k] = r[12 + k]; t.lu[k] = r[15 + k]; t.lv[k] = r[18 + k];
+            t.px[k] = r[21 + k]; t.py[k] = r[24 + k]; t.pz[k] = r[27 + k];
+            t.nx[k] = r[30 + k]; t.ny[k] = r[33 + k]; t.nz[k] = r[36 + k];
+        }
+        t.inst = ((const u32*)r)[39];
+        t.slot = P.insts[t.inst].model_base + ((const u32*)r)[40];
+        return;
+    }
+    t.inst = vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS);
+    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1u);
+    const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
+    const InstGpu* inst = &P.insts[t.inst];
+    t.slot = inst->model_base + gm;
+    const u8* ib = P.inds + (u64)t.slot * 512u + (tri >> 3) * 32u + (tri & 7u);
+    const f32* ss = P.ss + (u64)(inst->ss_offset + gm) * 576u;
+    const f32* ap = P.attr + (u64)t.slot * 448u;
+    for (int k = 0; k < 3; k += 1) {
+        const int i = ib[k * 8] & 63;
+        t.sx[k] = ss[i]; t.sy[k] = ss[64 + i]; t.iw[k] = ss[128 + i];
+        t.px[k] = ss[192 + i]; t.py[k] = ss[256 + i]; t.pz[k] = ss[320 + i];
+        t.nx[k] = ss[384 + i]; t.ny[k] = ss[448 + i]; t.nz[k] = ss[512 + i];
+        t.u[k] = ap[192 + i]; t.v[k] = ap[256 + i]; t.lu[k] = ap[320 + i]; t.lv[k] = ap[384 + i];
+    }
+}
+
+/* barycentric setup at a pixel centre (sofren2.c:6273-6312, non-MSAA) */
+struct Bary { f32 pw0, pw1, pw2; };
+DEV Bary bary_at(const Est& est, const Tri& t, f32 px, f32 py) {
+    const f32 dx1 = t.sx[1] - t.sx[0], dx2 = t.sx[2] - t.sx[0];
+    const f32 dy1 = t.sy[1] - t.sy[0], dy2 = t.sy[2] - t.sy[0];
+    const f32 inv_det = est.rcp_refined(dx1 * dy2 - dx2 * dy1);
+    const f32 ex = px - t.sx[0], ey = py - t.sy[0];
+    const f32 bu = (ex * dy2 - ey * dx2) * inv_det;
+    const f32 bv = (ey * dx1 - ex * dy1) * inv_det;
+    const f32 bw = (1.f - bu) - bv;
+    const f32 inv_z = fmaf(bw, t.iw[0], fmaf(bu, t.iw[1], bv * t.iw[2]));
+    const f32 z = est.rcp_refined(inv_z);
+    Bary b;
+    b.pw0 = (bw * t.iw[0]) * z;
+    b.pw1 = (bu * t.iw[1]) * z;
+    b.pw2 = (bv * t.iw[2]) * z;
+    return b;
+}
+
+DEV f32 interp(const Bary& b, const f32* a) {
+    return fmaf(b.pw0, a[0], fmaf(b.pw1, a[1], b.pw2 * a[2]));
+}
+
+DEV f32 unpack_ch(u32 c, int shift) { return (f32)((c >> shift) & 0xFFu) * (1.f / 255.f); }
+
+/* texel address inside the 4x4-block layout (sofren2.c:5606-5630) */
+DEV u32 texel_addr(const TexMeta& m, int lod, i32 x, i32 y) {
+    const u32 ux = (u32)x, uy = (u32)y;
+    if (m.is_pot) {
+        return (((uy >> 2) << m.stride_shift[lod]) + (ux >> 2)) * 16u + ((uy & 3u) << 2) + (ux & 3u);
+    }
+    return (((uy >> 2) * (u32)m.blocks_w[lod]) + (ux >> 2)) * 16u + ((uy & 3u) << 2) + (ux & 3u);
+}
+
+/* bilinear sample of the 4 nearest texels (sofren2.c:5543-5660) */
+DEV void sample_bilinear(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32& r, f32& g, f32& b) {
+    if (SFT_INVALID_HANDLE == h) { r = 1.f; g = 1.f; b = 1.f; return; }
+    const TexMeta& m = P.tex_meta[h & SFT_HANDLE_INDEX_MASK];
+    const u32* px = P.texels + m.mip_offset[lod];
+    const i32 cw = max(m.w >> lod, 1), ch = max(m.h >> lod, 1);
+    const f32 upx = cu * (f32)cw - 0.5f;
+    const f32 vpx = cv * (f32)ch - 0.5f;
+    const f32 uf = floorf(upx), vf = floorf(vpx);
+    const f32 ufr = upx - uf, vfr = vpx - vf;
+    const i32 u0 = cvtt(uf), v0 = cvtt(vf);
+    i32 u0w, v0w, u1, v1;
+    if (m.is_pot) {
+        u1 = (u0 + 1) & (cw - 1); v1 = (v0 + 1) & (ch - 1);
+        u0w = u0 & (cw - 1); v0w = v0 & (ch - 1);
+    }
+    else {
+        /* the reference clamps only the upper edge; a negative coordinate
+           would index before the level, so it is held at 0 here */
+        u1 = max(min(u0 + 1, cw - 1), 0); v1 = max(min(v0 + 1, ch - 1), 0);
+        u0w = max(min(u0, cw - 1), 0); v0w = max(min(v0, ch - 1), 0);
+    }
+    const u32 c00 = px[texel_addr(m, lod, u0w, v0w)];
+    const u32 c10 = px[texel_addr(m, lod, u1, v0w)];
+    const u32 c01 = px[texel_addr(m, lod, u0w, v1)];
+    const u32 c11 = px[texel_addr(m, lod, u1, v1)];
+    f32 o[3];
+    const int sh[3] = {16, 8, 0};
+    for (int k = 0; k < 3; k += 1) {
+        const f32 f00 = unpack_ch(c00, sh[k]), f10 = unpack_ch(c10, sh[k]);
+        const f32 f01 = unpack_ch(c01, sh[k]), f11 = unpack_ch(c11, sh[k]);
+        const f32 top = f00 + (f10 - f00) * ufr;
+        const f32 bot = f01 + (f11 - f01) * ufr;
+        o[k] = top + (bot - top) * vfr;
+    }
+    r = o[0]; g = o[1]; b = o[2];
+}
+
+/* nearest sample (sofren2.c:5662-5729) */
+DEV void sample_nearest(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32& r, f32& g, f32& b) {
+    if (SFT_INVALID_HANDLE == h) { r = 1.f; g = 1.f; b = 1.f; return; }
+    const TexMeta& m = P.tex_meta[h & SFT_HANDLE_INDEX_MASK];
+    const u32* px = P.texels + m.mip_offset[lod];
+    const i32 cw = max(m.w >> lod, 1), ch = max(m.h >> lod, 1);
+    const i32 u0 = cvtt(floorf(cu * (f32)cw));
+    const i32 v0 = cvtt(floorf(cv * (f32)ch));
+    i32 uw, vw;
+    if (m.is_pot) { uw = u0 & (cw - 1); vw = v0 & (ch - 1); }
+    else { uw = max(min(u0, cw - 1), 0); vw = max(min(v0, ch - 1), 0); }
+    const u32 c = px[texel_addr(m, lod, uw, vw)];
+    r = unpack_ch(c, 16); g = unpack_ch(c, 8); b = unpack_ch(c, 0);
+}
+
+/* mip level from the first covered lane's UV derivatives (sofren2.c:5501-5541);
+   the dither term cannot change the rounded level, so it is omitted */
+DEV int calc_lod(const FramePtrs& P, u32 h, f32 dux, f32 duy, f32 dvx, f32 dvy) {
+    if (SFT_INVALID_HANDLE == h) return 0;
+    const TexMeta& m = P.tex_meta[h & SFT_HANDLE_INDEX_MASK];
+    const f32 ux = dux * (f32)m.w, vx = dvx * (f32)m.h;
+    const f32 uy = duy * (f32)m.w, vy = dvy * (f32)m.h;
+    const f32 a = fabsf(ux), bb = fabsf(vx), c = fabsf(uy), d = fabsf(vy);
+    const f32 m1 = a > bb ? a : bb, m2 = c > d ? c : d;
+    const f32 rho = m1 > m2 ? m1 : m2;
+    i32 lod = 0;
+    if (rho > 1.f) {
+        const i32 ir = cvtt(rho);
+        lod = 31 - __clz((u32)ir);
+    }
+    return min(max(lod, 0), m.mip_levels - 1);
+}
+
+struct TileLights {
+    int count;
+    u16 idx[kMaxTileLights];
+};
+
+/* shades one pixel of triangle t (sofren2.c:6190-6620, non-MSAA); dux..dvy are
+   the block's first-lane UV derivatives */
+DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
+                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl) {
+    const f32 cu = interp(b, t.u), cv = interp(b, t.v);
+    const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
+    const V3 pos{interp(b, t.px), interp(b, t.py), interp(b, t.pz)};
+    const V3 nrm{interp(b, t.nx), interp(b, t.ny), interp(b, t.nz)};
+    const V3 N = est.norm_fast(nrm);
+    const V3 V = est.norm_fast(sub3(V3{fc.cam[0], fc.cam[1], fc.cam[2]}, pos));
+
+    /* material (sofren2.c:6349-6382) */
+    const MeshRec mr = P.mesh_recs[P.slot_mesh[t.slot]];
+    u32 base_color = 0xFFFFFFFFu;
+    u32 tex = SFT_INVALID_HANDLE;
+    f32 shin = 4.f;
+    if (SFT_INVALID_HANDLE != mr.material) {
+        const u32 mi = mr.material & SFT_MAT_INDEX_MASK;
+        if (mr.material & SFT_MAT_TRANSIENT_BIT) {
+            if (mi < fc.n_tmat) { base_color = P.tmat_color[mi]; tex = P.tmat_tex[mi]; shin = P.tmat_shin[mi]; }
+        }
+        else if (mi < fc.n_mat) {
+            base_color = P.mat_color[mi]; tex = P.mat_tex[mi]; shin = P.mat_shin[mi];
+        }
+    }
+    if (SFT_INVALID_HANDLE != tex && (tex & SFT_HANDLE_INDEX_MASK) >= fc.n_tex) tex = SFT_INVALID_HANDLE;
+    if (SFT_DEBUG_CHECKERBOARD == fc.debug_mode) {
+        tex = fc.base_tex;
+        base_color = 0xFFFFFFFFu;
+    }
+    const f32 base_r = (f32)((base_color >> 16) & 0xFFu) * (1.f / 255.f);
+    const f32 base_g = (f32)((base_color >> 8) & 0xFFu) * (1.f / 255.f);
+    const f32 base_b = (f32)(base_color & 0xFFu) * (1.f / 255.f);
+
+    const int lod = calc_lod(P, tex, dux, duy, dvx, dvy);
+    f32 tr, tg, tb;
+    if (lod > 1) sample_nearest(P, tex, lod, cu, cv, tr, tg, tb);
+    else sample_bilinear(P, tex, lod, cu, cv, tr, tg, tb);
+    const f32 alb_r = tr * base_r, alb_g = tg * base_g, alb_b = tb * base_b;
+
+    f32 fr, fg, fb;
+    if (shin < 0.f) {
+        fr = alb_r * -shin;
+        fg = alb_g * -shin;
+        fb = alb_b * -shin;
+    }
+    else {
+        f32 dr, dg, db;
+        f32 sr = 0.f, sg = 0.f, sb = 0.f;
+        f32 lm_shadow = 1.f;
+        const f32 spec_scale = ((shin + 8.f) / 8.f) * 0.04f;
+        const bool lightmapped = mr.lightmapped != 0u;
+        if (lightmapped) {
+            sample_bilinear(P, mr.lightmap_tex, 0, lu, lv, dr, dg, db);
+            dr = dr * fc.lm_brightness;
+            dg = dg * fc.lm_brightness;
+            db = db * fc.lm_brightness;
+        }
+        else {
+            dr = fc.ambient[0]; dg = fc.ambient[1]; db = fc.ambient[2];
+            for (u32 l = 0; l < fc.n_dir; l += 1) {
+                const DirLightGpu& L = P.dir_lights[l];
+                const V3 Ld{L.dir[0], L.dir[1], L.dir[2]};
+                const f32 ndl = smax(dot3(N, Ld), 0.f);
+                const bool lit = ndl > 0.f;
+                dr = fmaf(ndl, L.color[0], dr);
+                dg = fmaf(ndl, L.color[1], dg);
+                db = fmaf(ndl, L.color[2], db);
+                if (lit) {
+                    const V3 H = est.norm_fast(add3(Ld, V));
+                    const f32 ndh = smax(dot3(N, H), 0.f);
+                    const f32 den = (shin - shin * ndh) + ndh;
+                    const f32 spec = ndh * est.rcp_refined(den);
+                    const f32 term = (spec * spec_scale) * ndl;
+                    sr = fmaf(term, L.color[0], sr);
+                    sg = fmaf(term, L.color[1], sg);
+                    sb = fmaf(term, L.color[2], sb);
+                }
+            }
+        }
+        for (int i = 0; i < tl.count; i += 1) {
+            const PointLightGpu& L = P.point_lights[tl.idx[i]];
+            const V3 lraw = sub3(V3{L.pos[0], L.pos[1], L.pos[2]}, pos);
+            const f32 d2 = dot3(lraw, lraw);
+            f32 att = smax(0.f, fmaf(-d2, L.inv_r2, 1.f));
+            att = att * att;
+            const bool active = att > 0.f;
+            if (!active) continue;
+            const V3 Ld = est.norm_fast(lraw);
+            const f32 ndl = smax(dot3(N, Ld), 0.f);
+            const bool lit = (ndl > 0.f) && active;
+            const f32 max_shadow = 0.f;
+            const f32 si = smax(0.f, 1.f - max_shadow);
+            if (lightmapped) {
+                const f32 drop = (1.f - si) * att;
+                lm_shadow = lm_shadow * (1.f - drop);
+            }
+            else {
+                const f32 di = ndl * att;
+                const f32 mdi = lit ? di * si : 0.f;
+                dr = fmaf(mdi, L.color[0], dr);
+                dg = fmaf(mdi, L.color[1], dg);
+                db = fmaf(mdi, L.color[2], db);
+                if (lit) {
+                    const V3 H = est.norm_fast(add3(Ld, V));
+                    const f32 ndh = smax(dot3(N, H), 0.f);
+                    const f32 den = (shin - shin * ndh) + ndh;
+                    const f32 spec = ndh * est.rcp_refined(den);
+                    const f32 term = ((spec * spec_scale) * di) * si;
+                    sr = fmaf(term, L.color[0], sr);
+                    sg = fmaf(term, L.color[1], sg);
+                    sb = fmaf(term, L.color[2], sb);
+                }
+            }
+        }
+        if (lightmapped) {
+            const f32 floor_ = 0.4f;
+            const f32 mod = fmaf(lm_shadow, 1.f - floor_, floor_);
+            dr = dr * mod; dg = dg * mod; db = db * mod;
+            fr = alb_r * dr; fg = alb_g * dg; fb = alb_b * db;
+        }
+        else {
+            fr = fmaf(alb_r, dr, sr);
+            fg = fmaf(alb_g, dg, sg);
+            fb = fmaf(alb_b, db, sb);
+        }
+    }
+    fr = smax(0.f, smin(fr, 1.f)) * 255.f;
+    fg = smax(0.f, smin(fg, 1.f)) * 255.f;
+    fb = smax(0.f, smin(fb, 1.f)) * 255.f;
+    return 0xFF000000u | ((u32)cvtt(fr) << 16) | ((u32)cvtt(fg) << 8) | (u32)cvtt(fb);
+}
+
+/* tile light culling for one light against the tile's 1/w range and screen
+   rectangle (sofren2.c:5862-5933); scalar float math in C order */
+DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f32 tmax, i32 tile_x, i32 tile_y) {
+    const f32* vp = fc.vp;
+    const f32 px = L.pos[0], py = L.pos[1], pz = L.pos[2];
+    const f32 radius = L.radius;
+    const f32 w = px * vp[3] + py * vp[7] + pz * vp[11] + vp[15];
+    const f32 lmin = w - radius;
+    const f32 lmax = w + radius;
+    if (lmax <= 1e-6f) return false;
+    const f32 safe_min = (lmin > 1e-6f) ? lmin : 1e-6f;
+    const f32 inv_min = 1.f / lmax;
+    const f32 inv_max = 1.f / safe_min;
+    if (inv_max < tmin || inv_min > tmax) return false;
+    if (w > radius) {
+        const f32 tx = px * vp[0] + py * vp[4] + pz * vp[8] + vp[12];
+        const f32 ty = px * vp[1] + py * vp[5] + pz * vp[9] + vp[13];
+        const f32 sx = (tx / w + 1.f) * fc.hw + fc.vp_x;
+        const f32 sy = (-ty / w + 1.f) * fc.hh + fc.vp_y;
+        const f32 psx = sqrtf(vp[0] * vp[0] + vp[4] * vp[4] + vp[8] * vp[8]);
+        const f32 psy = sqrtf(vp[1] * vp[1] + vp[5] * vp[5] + vp[9] * vp[9]);
+        const f32 safe_w = ((w - radius) > 1e-6f) ? (w - radius) : 1e-6f;
+        const f32 rx = ((radius * psx) + fabsf(tx) * (radius / w)) / safe_w * fc.hw;
+        const f32 ry = ((radius * psy) + fabsf(ty) * (radius / w)) / safe_w * fc.hh;
+        const f32 l_min_x = sx - rx, l_max_x = sx + rx;
+        const f32 l_min_y = sy - ry, l_max_y = sy + ry;
+        const f32 t_min_x = (f32)tile_x, t_max_x = (f32)(tile_x + 64);
+        const f32 t_min_y = (f32)tile_y, t_max_y = (f32)(tile_y + 64);
+        if (l_max_x < t_min_x || l_min_x > t_max_x || l_max_y < t_min_y || l_min_y > t_max_y) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    __shared__ u64 s_vis[SFT_TILE_PIXELS];
+    __shared__ f32 s_red[2][4];
+    __shared__ u32 s_pass[256];
+    __shared__ TileLights s_tl;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 tile = blockIdx.x;
+    const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
+    const Est est = make_est(P, fc);
+
+    for (int i = tid; i < SFT_TILE_PIXELS; i += 256) s_vis[i] = 0ull;
+    __syncthreads();
+
+    /* ---- raster: a wave per payload, 8 lanes per triangle ---- */
+    {
+        const u32 off = P.tile_off[tile];
+        u32 cnt = P.tile_count[tile];
+        if (off >= fc.tile_list_cap) cnt = 0;
+        else if (off + cnt > fc.tile_list_cap) cnt = fc.tile_list_cap - off;
+        const int g = lane >> 3, j = lane & 7;
+        for (u32 e = (u32)wave; e < cnt; e += 4u) {
+            const u64 desc = P.tile_list[off + e];
+            const u32 mask = (u32)(desc >> 32) & 0xFFu;
+            const u32 idx = (u32)desc;
+            f32 sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2;
+            u32 key;
+            if (0u == mask) {
+                if (0 != g) continue;
+                const f32* r = P.clip_rec + (u64)idx * 41u;
+                sx0 = r[0]; sx1 = r[1]; sx2 = r[2];
+                sy0 = r[3]; sy1 = r[4]; sy2 = r[5];
+                iw0 = r[6]; iw1 = r[7]; iw2 = r[8];
+                key = P.clip_key[idx];
+            }
+            else {
+                if (0u == ((mask >> g) & 1u)) continue;
+                const u32 wi = idx >> 4, b = idx & 15u;
+                const WorkItem w = P.work[wi];
+                const u8* ib = P.inds + (u64)w.slot * 512u + b * 32u + (u32)g;
+                const f32* ss = P.ss + (u64)w.ss * 576u;
+                const int i0 = ib[0] & 63, i1 = ib[8] & 63, i2 = ib[16] & 63;
+                sx0 = ss[i0]; sy0 = ss[64 + i0]; iw0 = ss[128 + i0];
+                sx1 = ss[i1]; sy1 = ss[64 + i1]; iw1 = ss[128 + i1];
+                sx2 = ss[i2]; sy2 = ss[64 + i2]; iw2 = ss[128 + i2];
+                key = (wi << 9) | (b << 5) | (16u + (u32)g);
+            }
+            raster_tri(s_vis, est, tmx, tmy, j, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2);
+        }
+    }
+    __syncthreads();
+
+    /* ---- tile 1/w range over every sample, screen edge included (5820-5853) ---- */
+    f32 zmin = 1e9f, zmax = -1e9f;
+    for (int i = tid; i < SFT_TILE_PIXELS; i += 256) {
+        const f32 z = u2f((u32)(s_vis[i] >> 32));
+        if (z > 0.0001f) {
+            zmin = fminf(zmin, z);
+            zmax = fmaxf(zmax, z);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        zmin = fminf(zmin, __shfl_xor(zmin, o));
+        zmax = fmaxf(zmax, __shfl_xor(zmax, o));
+    }
+    if (0 == lane) { s_red[0][wave] = zmin; s_red[1][wave] = zmax; }
+    if (0 == tid) s_tl.count = 0;
+    __syncthreads();
+    zmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
+    zmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
+
+    /* ---- first 16 point lights touching the tile, in index order ---- */
+    if (zmin <= zmax && fc.n_point > 0u) {
+        for (u32 base = 0; base < fc.n_point; base += 256u) {
+            const u32 l = base + (u32)tid;
+            s_pass[tid] = (l < fc.n_point) ? (u32)light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy) : 0u;
+            __syncthreads();
+            if (0 == tid) {
+                for (u32 k = 0; k < 256u && base + k < fc.n_point && s_tl.count < kMaxTileLights; k += 1) {
+                    if (s_pass[k]) s_tl.idx[s_tl.count++] = (u16)(base + k);
+                }
+            }
+            __syncthreads();
+            if (s_tl.count >= kMaxTileLights) break;
+        }
+    }
+    __syncthreads();
+    const TileLights tl = s_tl;
+    const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
+
+    /* ---- shading: a wave covers a 32x2 strip = eight 4x2 blocks ---- */
+    i32 shaded = 0;
+    const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
+    const i32 gw = fc.tile_cols * 64;
+    for (int it = 0; it < 16; it += 1) {
+        const int strip = it * 4 + wave;
+        const i32 bx = tmx + (strip & 1) * 32 + blk * 4;
+        const i32 by = tmy + (strip >> 1) * 2;
+        const i32 gx = bx + ix, gy = by + iy;
+        const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
+        const u64 pk = s_vis[(gy - tmy) * 64 + (gx - tmx)];
+        const bool empty = 0ull == pk;
+        u32 vid = kInvalid;
+        if (!empty) vid = key_to_vid(P, ~(u32)pk, n_clip);
+        if (P.cap_vid) {
+            const u64 gi = (u64)gy * gw + gx;
+            P.cap_vid[gi] = vid;
+            P.cap_depth[gi] = u2f((u32)(pk >> 32));
+        }
+        /* first lane of the block holding the same triangle (the LOD lane) */
+        const u32 mine = (in_bounds && !empty) ? vid : kInvalid;
+        int first = l;
+        for (int k = 7; k >= 0; k -= 1) {
+            const u32 other = __shfl(mine, (lane & ~7) | k);
+            if (other == mine) first = k;
+        }
+        if (!in_bounds) continue;
+        if (empty) {
+            if (fc.should_clear) P.color[(u64)gy * fc.width + gx] = fc.clear_color;
+            continue;
+        }
+        shaded += 1;
+        Tri t;
+        load_tri(P, vid, t);
+        const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
+        const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy);
+        f32 dux = 0.f, duy = 0.f, dvx = 0.f, dvy = 0.f;
+        {
+            /* ddx pairs lanes l^1, ddy pairs rows l^4 (sofren2.c:272-273) */
+            const int f = first;
+            const Bary b0 = bary_at(est, t, fbx + (f32)(f & 3), fby + (f32)(f >> 2));
+            const Bary bx1 = bary_at(est, t, fbx + (f32)((f ^ 1) & 3), fby + (f32)((f ^ 1) >> 2));
+            const Bary by1 = bary_at(est, t, fbx + (f32)((f ^ 4) & 3), fby + (f32)((f ^ 4) >> 2));
+            const f32 u0 = interp(b0, t.u), v0 = interp(b0, t.v);
+            dux = u0 - interp(bx1, t.u);
+            dvx = v0 - interp(bx1, t.v);
+            duy = u0 - interp(by1, t.u);
+            dvy = v0 - interp(by1, t.v);
+        }
+        P.color[(u64)gy * fc.width + gx] = shade_pixel(P, fc, est, t, b, dux, duy, dvx, dvy, tl);
+    }
+    shaded = wave_sum(shaded);
+    if (0 == lane && shaded) atomicAdd(&P.counters[CNT_SHADED], (u32)shaded);
+}
+
+/* ================================================================== host */
+void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev) {
+    const u32 nw = fc.n_work;
+    const u32 geom_blocks = (nw + 3u) / 4u;
+    hipEventRecord(ev[1], s);
+    if (nw > 0) {
+        hipLaunchKernelGGL(k_geom, dim3(geom_blocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.clip_count, p.clip_base, nw,
+                           p.counters + CNT_CLIP_TRIS, fc.clip_cap, p.counters + CNT_OVERFLOW, 1u);
+    }
+    hipEventRecord(ev[2], s);
+    if (nw > 0) {
+        hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.tile_count, p.tile_off, (u32)fc.n_tiles,
+                       p.counters + CNT_TILE_PAIRS, fc.tile_list_cap, p.counters + CNT_OVERFLOW, 2u);
+    if (nw > 0) {
+        const u32 items = nw * 16u;
+        u32 blocks = (items + 255u) / 256u + 64u;
+        if (blocks > 4096u) blocks = 4096u;
+        hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, p);
+    }
+    hipEventRecord(ev[3], s);
+    hipLaunchKernelGGL(k_tile, dim3((u32)fc.n_tiles), dim3(256), 0, s, p);
+    hipEventRecord(ev[4], s);
+}
+
+}  // namespace sft

@@ -1,0 +1,186 @@
+/*
+ * sft_internal.h — state shared by the host front-end (frontend.cpp) and the
+ * HIP backend (backend.hip).  Not part of the public API.
+ *
+ * HBM layout (all resident pools are indexed by a "meshlet slot"; a model's
+ * meshlets occupy slots [model_base, model_base + total_meshlets) in the
+ * model-relative order used by visibility IDs, sofren2.c:8457-8477):
+ *   pos   f32[slot][3][64]   positions, SoA (sofren2.c:914-918)
+ *   attr  f32[slot][7][64]   nx ny nz u v lu lv (sofren2.c:920-933)
+ *   inds  u8 [slot][512]     16 batches x (v0[8] v1[8] v2[8] pad[8]) (949-953)
+ *   cull  f32[slot][8]       centre xyz, radius, cone axis xyz, cutoff
+ *   slot_mesh u32[slot]      mesh record (material + lightmap) of the slot
+ *   slot_skin u32[slot]      skin record or 0xFFFFFFFF
+ *   skin  u8 [skin][768]     j0..j3 u16[64], w0..w3 u8[64] (935-947)
+ * Per frame:
+ *   work  WorkItem[n]        drawn meshlets in emission order (instance draw
+ *                            order, mesh, meshlet: the reference's job order)
+ *   ss    f32[ss][9][64]     screen-space cache, reference layout (1464-1477)
+ *   tile lists               u64 payload descriptors per 64x64 tile
+ *   clip records             reference layout (1271-1280) in slot order
+ */
+#pragma once
+#include <stdint.h>
+#include <vector>
+#include <unordered_map>
+#include "../../include/sofren2.h"
+#include "../../include/sofren2_ext.h"
+
+namespace sft {
+
+constexpr u32 kInvalid = 0xFFFFFFFFu;
+constexpr int kMaxTileLights = 16;
+
+/* GPU-side per-frame constants. */
+struct FrameConst {
+    i32 width, height, screen_w, screen_h;
+    i32 tile_cols, tile_rows, n_tiles, samples;
+    f32 hw, hh, vp_x, vp_y;
+    f32 bias_x, bias_y, neg_hh, pad0;
+    f32 vp[16];
+    f32 planes[6][4];
+    f32 cam[4];
+    f32 ambient[4];
+    u32 clear_color, should_clear, debug_mode, shadows_enabled;
+    f32 lm_brightness, pad1, pad2, pad3;
+    u32 n_work, n_inst, n_point, n_dir;
+    u32 n_proxies, base_tex, rcp_bits, rsq_bits;
+    u32 clip_cap, tile_list_cap, capture, n_tex;
+    u32 n_mat, n_tmat, pad5, pad6;
+};
+
+struct InstGpu {
+    f32 t[16];
+    u32 model_base;     /* first meshlet slot of the model */
+    u32 ss_offset;      /* ss_meshlet_offset */
+    u32 bone_base;      /* first matrix in the frame's bone buffer, kInvalid if none */
+    u32 pad;
+};
+
+struct WorkItem {
+    u32 slot;           /* resident meshlet slot */
+    u32 ss;             /* screen-space cache index */
+    u32 inst;           /* instance slot */
+    u32 gm;             /* model-relative meshlet index (visibility ID field) */
+    i32 cull_bone;      /* bone used for the 8-meshlet group's cull volume, -1 none */
+    u32 skin;           /* skin record, kInvalid if unskinned or no pose */
+    u32 pad0, pad1;
+};
+
+struct MeshRec {
+    u32 material;       /* sft_mat_handle */
+    u32 lightmap_tex;   /* sft_tex_handle of the lightmap page */
+    u32 lightmapped;    /* mesh has a lightmap page (sofren2.c:6414) */
+    u32 pad;
+};
+
+/* texture metadata mirrored to the GPU (sofren2.c:1395-1412) */
+struct TexMeta {
+    i32 w, h, mip_levels, is_pot;
+    u32 mip_offset[14];     /* in texels, from the start of the device texel heap */
+    i32 blocks_w[14];
+    i32 stride_shift[14];
+};
+
+struct PointLightGpu { f32 pos[3], color[3], radius, inv_r2; };
+struct DirLightGpu { f32 dir[3], color[3], pad[2]; };
+
+/* world capsule proxy (sofren2.c:1432-1437) */
+struct WorldProxy {
+    sftvec posa, posb;
+    f32 radius;
+    u8 light_vis[64];
+};
+
+/* host texture pool entry */
+struct HostTexture {
+    u32 generation = 0;
+    i32 next_free = -1;
+    bool live = false;
+    i32 w = 0, h = 0, mip_levels = 0;
+    u8 is_pot = 0;
+    std::vector<u32> texels;    /* all swizzled mip levels, contiguous */
+    u32 mip_offset[14] = {0};   /* in texels from texels[0] */
+    i32 blocks_w[14] = {0};
+    i32 stride_shift[14] = {0};
+    u64 dev_offset = 0;         /* texel offset in the device heap */
+    bool uploaded = false;
+};
+
+struct Context;
+extern Context* g_ctx;
+
+struct Backend;
+Backend* backend_create();
+void backend_destroy(Backend* b);
+/* renders the recorded frame into ctx->color_buffer */
+void backend_present(Backend* b, Context* ctx);
+void backend_release_texture(Backend* b, u32 index);
+
+/* host estimate tables for _mm_rcp_ps / _mm_rsqrt_ps (rcp_probe.cpp) */
+struct EstimateTables {
+    int rcp_bits = 0;
+    int rsq_bits = 0;
+    std::vector<f32> rcp;       /* 2^rcp_bits entries: estimate for mantissa bucket in [1,2) */
+    std::vector<f32> rsq;       /* 2^(rsq_bits+1) entries: [1,2) then [2,4) */
+    u64 rcp_mismatches = 0;     /* emulation vs instruction on the probe's checks */
+    u64 rsq_mismatches = 0;
+};
+void probe_estimate_tables(EstimateTables* out);
+f32 emulate_rcp(const EstimateTables& t, f32 a);
+f32 emulate_rsqrt(const EstimateTables& t, f32 a);
+
+struct Context {
+    i32 width = 0, height = 0;
+    f32 half_width = 0, half_height = 0, vp_x = 0, vp_y = 0;
+    i32 tile_cols = 0, tile_rows = 0;
+    u32* color_buffer = nullptr;
+    i32 color_pitch = 0;            /* in u32 */
+    u8 should_clear = 0;
+    u32 clear_color = 0xFF222222u;
+    enum sft_debug_mode debug_mode = SFT_DEBUG_NONE;
+
+    sftvec camera_pos{0, 0, 0};
+    sftmat view{}, proj{}, view_proj{};
+    f32 planes[6][4] = {};
+
+    sftvec ambient{0, 0, 0};
+    std::vector<sft_directional_light> dir_lights;
+    std::vector<sft_point_light> point_lights;
+    f32 lightmap_brightness = 1.5f;
+
+    u8 shadows_enabled = 0;
+    std::vector<WorldProxy> proxies;
+
+    i32 instance_count = 0;         /* counts dropped instances too, like the reference */
+    std::vector<sft_instance> instances;
+    u32 total_ss = 0;
+
+    sft_arena scratch_arena{};      /* dmesh storage, reset each frame */
+    std::vector<const sft_model*> transient_models;
+
+    /* textures */
+    std::vector<HostTexture> tex;
+    i32 tex_free_head = -1;
+    /* static materials */
+    std::vector<u32> mat_gen;
+    std::vector<i32> mat_next_free;
+    std::vector<u32> mat_tex, mat_color;
+    std::vector<f32> mat_shininess;
+    i32 mat_free_head = -1;
+    /* transient materials, reset in frame_end */
+    std::vector<u32> tmat_tex, tmat_color;
+    std::vector<f32> tmat_shininess;
+
+    sft_meshlet_inds quad_inds{}, tri_inds{};
+    sft_mat_handle base_mat = kInvalid;
+    sft_tex_handle base_tex = kInvalid;
+
+    u32 capture_flags = 0;
+    sftx_frame_stats stats{};
+
+    EstimateTables est;
+    Backend* backend = nullptr;
+};
+
+}  // namespace sft
