@@ -75,6 +75,62 @@ typedef struct sftx_frame_stats {
 
 void sftx_get_stats(sftx_frame_stats* out);
 
+/* ---- frame description -------------------------------------------------
+   Everything the per-frame render path reads, as recorded by the front-end
+   since sft_frame_begin: the reference's frame state (sofren2.c:1335-1495)
+   minus its scratch.  Pointers stay valid until the next sft_frame_end or
+   resource call.  The CPU restatement of the render path consumes this. */
+typedef struct sftx_texture_desc {
+    const u32* texels;          /* all mip levels, 4x4-block swizzled, contiguous */
+    i32 w, h, mip_levels, is_pot;
+    u32 mip_offset[14];         /* in texels from texels[0] */
+    i32 blocks_w[14];
+    i32 stride_shift[14];
+} sftx_texture_desc;
+
+typedef struct sftx_proxy_desc {
+    sftvec posa, posb;
+    f32 radius;
+} sftx_proxy_desc;
+
+typedef struct sftx_frame_desc {
+    i32 width, height, tile_cols, tile_rows;
+    f32 half_width, half_height, vp_x, vp_y;
+    sftvec camera_pos;
+    sftmat view_proj;
+    f32 planes[6][4];               /* normalised nx, ny, nz, d */
+    sftvec ambient;
+    const sft_directional_light* dir_lights;
+    i32 n_dir;
+    const sft_point_light* point_lights;
+    i32 n_point;
+    f32 lightmap_brightness;
+    u8 shadows_enabled, should_clear;
+    u32 clear_color;
+    i32 debug_mode;
+    const sft_instance* instances;  /* ss_meshlet_offset filled in */
+    i32 n_instances;
+    u32 total_ss;
+    const sftx_proxy_desc* proxies;
+    i32 n_proxies;
+    const sftx_texture_desc* textures;
+    i32 n_textures;
+    const u32* mat_tex;
+    const u32* mat_color;
+    const f32* mat_shininess;
+    i32 n_mat;
+    const u32* tmat_tex;
+    const u32* tmat_color;
+    const f32* tmat_shininess;
+    i32 n_tmat;
+    u32 base_tex;                   /* albedo of the base material */
+    u32* color_buffer;
+    i32 color_pitch;                /* in u32 */
+} sftx_frame_desc;
+
+/* Fills `out` with the frame recorded so far.  Needs no GPU. */
+void sftx_export_frame(sftx_frame_desc* out);
+
 #ifdef __cplusplus
 }
 #endif

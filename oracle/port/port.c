@@ -1,0 +1,823 @@
+/*
+ * port.c — plain C restatement of sofren2's per-frame render path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the oracle the HIP kernels are checked
+ * against.  It is scalar C that replays the reference's 8-lane AVX2 code lane
+ * by lane, in the reference's structure (8-meshlet cull groups, 8-triangle
+ * batches, 8-pixel raster chunks, 4x2 shading blocks), single-threaded.
+ * Each function cites the reference lines it follows (sofren2.c).
+ *
+ * Arithmetic pins: the reciprocal and reciprocal-square-root estimates are
+ * the host's own VRCPPS / VRSQRTPS (the reference's _mm256_rcp_ps /
+ * _mm256_rsqrt_ps), conversions are CVTTSS2SI (INT_MIN out of range, like
+ * _mm256_cvttps_epi32), FMAs are explicit, min/max follow SSE operand rules,
+ * and the file is compiled with -ffp-contract=off.
+ */
+#include <immintrin.h>
+#include <stdlib.h>
+#include <string.h>
+#include "port.h"
+
+#define TS SFT_TILE_SIZE
+#define TP SFT_TILE_PIXELS
+
+/* ------------------------------------------------------------ primitives
+   (sofren2.c:263-385, 632-642, 1921-2030) */
+static inline f32 pfma(f32 a, f32 b, f32 c) { return __builtin_fmaf(a, b, c); }
+static inline f32 smin(f32 a, f32 b) { return a < b ? a : b; }     /* _mm256_min_ps(a, b) */
+static inline f32 smax(f32 a, f32 b) { return a > b ? a : b; }     /* _mm256_max_ps(a, b) */
+static inline i32 cvtt(f32 x) { return _mm_cvtt_ss2si(_mm_set_ss(x)); }
+static inline f32 rcp_est(f32 a) { return _mm256_cvtss_f32(_mm256_rcp_ps(_mm256_set1_ps(a))); }
+static inline f32 rsq_est(f32 a) { return _mm256_cvtss_f32(_mm256_rsqrt_ps(_mm256_set1_ps(a))); }
+/* estimate * (2 - a * estimate), sofren2.c:633-638 */
+static inline f32 rcp_refined(f32 a) { const f32 x = rcp_est(a); return x * pfma(-a, x, 2.f); }
+
+typedef struct { f32 x, y, z; } v3;
+static inline f32 vdot(v3 a, v3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+static inline v3 vsub(v3 a, v3 b) { v3 r = { a.x - b.x, a.y - b.y, a.z - b.z }; return r; }
+static inline v3 vadd(v3 a, v3 b) { v3 r = { a.x + b.x, a.y + b.y, a.z + b.z }; return r; }
+/* sft_vv_norm_fast, sofren2.c:1973-1992 */
+static inline v3 norm_fast(v3 v) {
+    const f32 sq = vdot(v, v);
+    const f32 ap = rsq_est(sq);
+    const f32 r = (0.5f * ap) * pfma(-sq, ap * ap, 3.f);
+    v3 o = { v.x * r, v.y * r, v.z * r };
+    return o;
+}
+/* sft_vv_transform, sofren2.c:1993-2012 */
+static inline v3 xf(v3 v, const sftmat* m) {
+    v3 r;
+    r.x = pfma(v.x, m->m[0][0], pfma(v.y, m->m[1][0], pfma(v.z, m->m[2][0], m->m[3][0])));
+    r.y = pfma(v.x, m->m[0][1], pfma(v.y, m->m[1][1], pfma(v.z, m->m[2][1], m->m[3][1])));
+    r.z = pfma(v.x, m->m[0][2], pfma(v.y, m->m[1][2], pfma(v.z, m->m[2][2], m->m[3][2])));
+    return r;
+}
+/* sft_vv_transform_dir, sofren2.c:2013-2030 */
+static inline v3 xf_dir(v3 v, const sftmat* m) {
+    v3 r;
+    r.x = pfma(v.x, m->m[0][0], pfma(v.y, m->m[1][0], v.z * m->m[2][0]));
+    r.y = pfma(v.x, m->m[0][1], pfma(v.y, m->m[1][1], v.z * m->m[2][1]));
+    r.z = pfma(v.x, m->m[0][2], pfma(v.y, m->m[1][2], v.z * m->m[2][2]));
+    return r;
+}
+
+/* ------------------------------------------------------------ frame state */
+typedef struct { u64* p; u32 n, cap; } bin_t;
+
+typedef struct {
+    f32 sx[3], sy[3], inv_w[3], u[3], v[3], lu[3], lv[3], px[3], py[3], pz[3], nx[3], ny[3], nz[3];
+    u32 inst_ind, m_ind;
+} clip_rec;
+
+typedef struct {
+    const sftx_frame_desc* f;
+    sftx_port_out* out;
+    f32* ss;            /* [total_ss][9][64] */
+    bin_t* bins;
+    u32 clip_next;
+    int clip_overflow;
+} port_state;
+
+static void bin_push(bin_t* b, u64 v) {
+    if (b->n == b->cap) {
+        b->cap = b->cap ? b->cap * 2 : 64;
+        b->p = (u64*)realloc(b->p, b->cap * sizeof(u64));
+    }
+    b->p[b->n++] = v;
+}
+
+/* sft__geom_bin_tiles, sofren2.c:4650-4688 */
+static void bin_tiles(port_state* ps, u64 payload, i32 x0, i32 x1, i32 y0, i32 y1) {
+    const i32 mx = ps->f->tile_cols - 1, my = ps->f->tile_rows - 1;
+    if (x1 < 0 || x0 > mx || y1 < 0 || y0 > my) return;
+    x0 = SFT_MAX(x0, 0); x1 = SFT_MIN(x1, mx);
+    y0 = SFT_MAX(y0, 0); y1 = SFT_MIN(y1, my);
+    for (i32 ty = y0; ty <= y1; ty += 1) {
+        for (i32 tx = x0; tx <= x1; tx += 1) bin_push(&ps->bins[ty * ps->f->tile_cols + tx], payload);
+    }
+}
+
+static inline f32* ss_at(port_state* ps, u32 ss_ind) { return ps->ss + (u64)ss_ind * 576u; }
+
+/* ------------------------------------------------------------ near-plane clip
+   sofren2.c:4245-4349 */
+typedef struct { f32 tx, ty, tz, w, u, v, lu, lv, px, py, pz, nx, ny, nz; } cvert;
+
+static cvert lerp_cv(cvert a, cvert b, f32 t) {
+    cvert r;
+    r.tx = a.tx + t * (b.tx - a.tx); r.ty = a.ty + t * (b.ty - a.ty);
+    r.tz = a.tz + t * (b.tz - a.tz); r.w = a.w + t * (b.w - a.w);
+    r.u = a.u + t * (b.u - a.u); r.v = a.v + t * (b.v - a.v);
+    r.lu = a.lu + t * (b.lu - a.lu); r.lv = a.lv + t * (b.lv - a.lv);
+    r.px = a.px + t * (b.px - a.px); r.py = a.py + t * (b.py - a.py); r.pz = a.pz + t * (b.pz - a.pz);
+    r.nx = a.nx + t * (b.nx - a.nx); r.ny = a.ny + t * (b.ny - a.ny); r.nz = a.nz + t * (b.nz - a.nz);
+    return r;
+}
+
+static int clip_near(cvert out[2][3], const cvert in[3]) {
+    int mask = 0;
+    for (int i = 0; i < 3; i += 1) mask |= (in[i].tz >= 0.f) << i;
+    switch (mask) {
+        case 0: return 0;
+        case 7: out[0][0] = in[0]; out[0][1] = in[1]; out[0][2] = in[2]; return 1;
+        case 1: case 2: case 4: {
+            const int a = mask >> 1, b = (2 == a) ? 0 : a + 1, c = (0 == a) ? 2 : a - 1;
+            const f32 da = in[a].tz;
+            const f32 tab = da / (da - in[b].tz), tac = da / (da - in[c].tz);
+            out[0][0] = in[a];
+            out[0][1] = lerp_cv(in[a], in[b], tab);
+            out[0][2] = lerp_cv(in[a], in[c], tac);
+            return 1;
+        }
+        default: {
+            const int o = (~mask & 7) >> 1, a = (0 == o) ? 2 : o - 1, c = (2 == o) ? 0 : o + 1;
+            const f32 da = in[a].tz, db = in[o].tz, dc = in[c].tz;
+            const f32 tab = da / (da - db), tcb = dc / (dc - db);
+            const cvert nab = lerp_cv(in[a], in[o], tab), ncb = lerp_cv(in[c], in[o], tcb);
+            out[0][0] = in[a]; out[0][1] = nab; out[0][2] = in[c];
+            out[1][0] = nab; out[1][1] = ncb; out[1][2] = in[c];
+            return 2;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ geometry
+   sofren2.c:4691-4884 (clip + setup + bin), 4901-5145 (cull, skin, project) */
+typedef struct {
+    f32 tx[64], ty[64], tz[64], w[64], sx[64], sy[64];
+} clip_scratch;
+
+static void clip_and_bin(port_state* ps, const u8 i[3][8], int lanes, const clip_scratch* cs, u32 inst_ind,
+                         u32 m_ind, const struct sft_meshlet_attribs* at, const f32* ssm) {
+    const sftx_frame_desc* f = ps->f;
+    for (int lane = 0; lane < 8; lane += 1) {
+        if (!(lanes & (1 << lane))) continue;
+        cvert in[3];
+        for (int k = 0; k < 3; k += 1) {
+            const int v = i[k][lane];
+            cvert c = { cs->tx[v], cs->ty[v], cs->tz[v], cs->w[v], at->u[v], at->v[v], at->lu[v], at->lv[v],
+                        ssm[192 + v], ssm[256 + v], ssm[320 + v], ssm[384 + v], ssm[448 + v], ssm[512 + v] };
+            in[k] = c;
+        }
+        cvert out[2][3];
+        const int n = clip_near(out, in);
+        for (int c = 0; c < n; c += 1) {
+            const u32 slot = ps->clip_next++;
+            if (slot >= ps->out->clip_cap) {
+                ps->clip_overflow = 1;
+                continue;
+            }
+            clip_rec* ct = (clip_rec*)(ps->out->clips + (u64)slot * sizeof(clip_rec));
+            for (int k = 0; k < 3; k += 1) {
+                const f32 iw = 1.f / out[c][k].w;
+                ct->sx[k] = (out[c][k].tx * iw + 1.f) * f->half_width + f->vp_x;
+                ct->sy[k] = (-out[c][k].ty * iw + 1.f) * f->half_height + f->vp_y;
+                ct->inv_w[k] = iw;
+            }
+            ct->inst_ind = inst_ind;
+            ct->m_ind = m_ind;
+            for (int k = 0; k < 3; k += 1) {
+                ct->u[k] = out[c][k].u; ct->v[k] = out[c][k].v;
+                ct->lu[k] = out[c][k].lu; ct->lv[k] = out[c][k].lv;
+                ct->px[k] = out[c][k].px; ct->py[k] = out[c][k].py; ct->pz[k] = out[c][k].pz;
+                ct->nx[k] = out[c][k].nx; ct->ny[k] = out[c][k].ny; ct->nz[k] = out[c][k].nz;
+            }
+            const f32 area = (ct->sx[1] - ct->sx[0]) * (ct->sy[2] - ct->sy[0]) - (ct->sy[1] - ct->sy[0]) * (ct->sx[2] - ct->sx[0]);
+            if (area <= 0.1f) continue;
+            const i32 x0 = cvtt(floorf(SFT_MIN(ct->sx[0], SFT_MIN(ct->sx[1], ct->sx[2]))));
+            const i32 x1 = cvtt(ceilf(SFT_MAX(ct->sx[0], SFT_MAX(ct->sx[1], ct->sx[2]))));
+            const i32 y0 = cvtt(floorf(SFT_MIN(ct->sy[0], SFT_MIN(ct->sy[1], ct->sy[2]))));
+            const i32 y1 = cvtt(ceilf(SFT_MAX(ct->sy[0], SFT_MAX(ct->sy[1], ct->sy[2]))));
+            bin_tiles(ps, (u64)slot, x0 >> 6, x1 >> 6, y0 >> 6, y1 >> 6);
+        }
+    }
+}
+
+static void setup_and_bin_batch(port_state* ps, u32 batch, u32 inst_ind, u32 m_ind, const struct sft_meshlet_inds* mi,
+                                const clip_scratch* cs, const struct sft_meshlet_attribs* at, const f32* ssm) {
+    u8 i[3][8];
+    int valid = 0;
+    for (int l = 0; l < 8; l += 1) {
+        for (int k = 0; k < 3; k += 1) i[k][l] = mi->inds[batch * 32 + k * 8 + l];
+        if ((i[0][l] | i[1][l] | i[2][l]) > 0) valid |= 1 << l;
+    }
+    if (!valid) return;
+    int inside = 0, inter = 0;
+    for (int l = 0; l < 8; l += 1) {
+        const f32 z0 = cs->tz[i[0][l]], z1 = cs->tz[i[1][l]], z2 = cs->tz[i[2][l]];
+        const int all_in = z0 >= 0.f && z1 >= 0.f && z2 >= 0.f;
+        const int all_out = z0 < 0.f && z1 < 0.f && z2 < 0.f;
+        if (all_in) inside |= 1 << l;
+        if ((valid & (1 << l)) && !(all_in || all_out)) inter |= 1 << l;
+    }
+    if (inter) clip_and_bin(ps, (const u8(*)[8])i, inter, cs, inst_ind, m_ind, at, ssm);
+    if (!inside) return;
+    const i32 mx = ps->f->tile_cols - 1, my = ps->f->tile_rows - 1;
+    int tris = 0;
+    i32 tx0[8], tx1[8], ty0[8], ty1[8];
+    for (int l = 0; l < 8; l += 1) {
+        const f32 sx0 = cs->sx[i[0][l]], sy0 = cs->sy[i[0][l]];
+        const f32 sx1 = cs->sx[i[1][l]], sy1 = cs->sy[i[1][l]];
+        const f32 sx2 = cs->sx[i[2][l]], sy2 = cs->sy[i[2][l]];
+        const f32 dx1 = sx1 - sx0, dy1 = sy1 - sy0, dx2 = sx2 - sx0, dy2 = sy2 - sy0;
+        const int area_ok = pfma(dx1, dy2, -(dy1 * dx2)) > 0.1f;
+        tx0[l] = cvtt(smin(sx0, smin(sx1, sx2))) >> 6;
+        tx1[l] = cvtt(smax(sx0, smax(sx1, sx2))) >> 6;
+        ty0[l] = cvtt(smin(sy0, smin(sy1, sy2))) >> 6;
+        ty1[l] = cvtt(smax(sy0, smax(sy1, sy2))) >> 6;
+        const int in_grid = tx1[l] >= 0 && tx0[l] <= mx && ty1[l] >= 0 && ty0[l] <= my;
+        if ((inside & (1 << l)) && area_ok && in_grid) tris |= 1 << l;
+    }
+    if (!tris) return;
+    i32 bx0 = 9999, bx1 = -9999, by0 = 9999, by1 = -9999;
+    for (int l = 0; l < 8; l += 1) {
+        if (!(tris & (1 << l))) continue;
+        if (tx0[l] < bx0) bx0 = tx0[l];
+        if (tx1[l] > bx1) bx1 = tx1[l];
+        if (ty0[l] < by0) by0 = ty0[l];
+        if (ty1[l] > by1) by1 = ty1[l];
+    }
+    const u64 payload = (u64)inst_ind | ((u64)m_ind << 24) | ((u64)batch << 48) | ((u64)tris << 56);
+    bin_tiles(ps, payload, bx0, bx1, by0, by1);
+}
+
+/* frustum + normal-cone test of one meshlet, culling volume moved by `bone`
+   (the 8-meshlet group's first culling bone) then the instance (4939-4994) */
+static int cull_meshlet(const sftx_frame_desc* f, const sft_mesh* mesh, u32 m, const sftmat* T, const sftmat* bone) {
+    v3 c = { mesh->bounds_center_x[m], mesh->bounds_center_y[m], mesh->bounds_center_z[m] };
+    v3 a = { mesh->cone_axis_x[m], mesh->cone_axis_y[m], mesh->cone_axis_z[m] };
+    if (bone) {
+        c = xf(c, bone);
+        a = xf_dir(a, bone);
+    }
+    c = xf(c, T);
+    a = xf_dir(a, T);
+    const f32 sx = sqrtf(T->m[0][0] * T->m[0][0] + T->m[0][1] * T->m[0][1] + T->m[0][2] * T->m[0][2]);
+    const f32 radius = mesh->bounds_radius[m] * sx;
+    f32 len = sqrtf(vdot(a, a));
+    len = smax(len, SFT_EPSILON);
+    a.x = a.x / len; a.y = a.y / len; a.z = a.z / len;
+    int inside = 1;
+    for (int p = 0; p < 6; p += 1) {
+        const v3 n = { f->planes[p][0], f->planes[p][1], f->planes[p][2] };
+        const f32 dist = vdot(c, n) + f->planes[p][3];
+        inside = inside && (dist + radius >= 0.f);
+    }
+    const v3 cam = { f->camera_pos.x, f->camera_pos.y, f->camera_pos.z };
+    const v3 raw = vsub(c, cam);
+    const f32 d = sqrtf(vdot(raw, raw));
+    inside = inside && (vdot(raw, a) <= pfma(d, mesh->cone_cutoff[m], radius));
+    return inside;
+}
+
+static void geometry_pass(port_state* ps) {
+    const sftx_frame_desc* f = ps->f;
+    const sftmat* vp = &f->view_proj;
+    const f32 sxs = f->half_width, sxb = f->half_width + f->vp_x;
+    const f32 sys = -f->half_height, syb = f->half_height + f->vp_y;
+    for (i32 ii = 0; ii < f->n_instances; ii += 1) {
+        const sft_instance* inst = &f->instances[ii];
+        const sft_model* model = inst->model;
+        for (u32 mi = 0; mi < model->mesh_count; mi += 1) {
+            const sft_mesh* mesh = &model->meshes[mi];
+            for (u32 start = 0; start < mesh->meshlet_count; start += SFT_MESHLETS_PER_JOB) {
+                const sftmat* bone = NULL;
+                if (mesh->culling_bone_inds && inst->posed_bone_matrices) {
+                    bone = &inst->posed_bone_matrices[mesh->culling_bone_inds[start]];
+                }
+                for (u32 lane = 0; lane < 8 && start + lane < mesh->meshlet_count; lane += 1) {
+                    const u32 m = start + lane;
+                    const u32 gm = mesh->model_meshlet_offset + m;
+                    const u32 ss_ind = inst->ss_meshlet_offset + gm;
+                    const int vis = cull_meshlet(f, mesh, m, &inst->transform, bone);
+                    ps->out->ss_vis[ss_ind] = (u8)vis;
+                    if (!vis) continue;
+                    const struct sft_meshlet* mv = &mesh->meshlets[m];
+                    const struct sft_meshlet_attribs* at = &mesh->meshlet_attribs[m];
+                    f32* ssm = ss_at(ps, ss_ind);
+                    clip_scratch cs;
+                    for (int v = 0; v < 64; v += 1) {
+                        const v3 p = { mv->x[v], mv->y[v], mv->z[v] };
+                        const v3 n = { at->nx[v], at->ny[v], at->nz[v] };
+                        v3 pw, nw;
+                        if (mesh->meshlet_skins && inst->posed_bone_matrices) {
+                            /* 4-joint skinning, weights u8/255 (5026-5080) */
+                            const struct sft_meshlet_skin* sk = &mesh->meshlet_skins[m];
+                            const f32 inv255 = 1.f / 255.f;
+                            const u16 j[4] = { sk->j0[v], sk->j1[v], sk->j2[v], sk->j3[v] };
+                            const f32 w[4] = { sk->w0[v] * inv255, sk->w1[v] * inv255, sk->w2[v] * inv255, sk->w3[v] * inv255 };
+                            v3 pp[4], nn[4];
+                            for (int k = 0; k < 4; k += 1) {
+                                pp[k] = xf(p, &inst->posed_bone_matrices[j[k]]);
+                                nn[k] = xf_dir(n, &inst->posed_bone_matrices[j[k]]);
+                            }
+                            pw.x = pfma(pp[0].x, w[0], pfma(pp[1].x, w[1], pfma(pp[2].x, w[2], pp[3].x * w[3])));
+                            pw.y = pfma(pp[0].y, w[0], pfma(pp[1].y, w[1], pfma(pp[2].y, w[2], pp[3].y * w[3])));
+                            pw.z = pfma(pp[0].z, w[0], pfma(pp[1].z, w[1], pfma(pp[2].z, w[2], pp[3].z * w[3])));
+                            nw.x = pfma(nn[0].x, w[0], pfma(nn[1].x, w[1], pfma(nn[2].x, w[2], nn[3].x * w[3])));
+                            nw.y = pfma(nn[0].y, w[0], pfma(nn[1].y, w[1], pfma(nn[2].y, w[2], nn[3].y * w[3])));
+                            nw.z = pfma(nn[0].z, w[0], pfma(nn[1].z, w[1], pfma(nn[2].z, w[2], nn[3].z * w[3])));
+                            pw = xf(pw, &inst->transform);
+                            nw = xf_dir(nw, &inst->transform);
+                        }
+                        else {
+                            pw = xf(p, &inst->transform);
+                            nw = xf_dir(n, &inst->transform);
+                        }
+                        ssm[192 + v] = pw.x; ssm[256 + v] = pw.y; ssm[320 + v] = pw.z;
+                        ssm[384 + v] = nw.x; ssm[448 + v] = nw.y; ssm[512 + v] = nw.z;
+                        /* projection with view_proj, FMA order z, y, x (5094-5123) */
+                        const f32 w = pfma(pw.x, vp->m[0][3], pfma(pw.y, vp->m[1][3], pfma(pw.z, vp->m[2][3], vp->m[3][3])));
+                        const f32 tx = pfma(pw.x, vp->m[0][0], pfma(pw.y, vp->m[1][0], pfma(pw.z, vp->m[2][0], vp->m[3][0])));
+                        const f32 ty = pfma(pw.x, vp->m[0][1], pfma(pw.y, vp->m[1][1], pfma(pw.z, vp->m[2][1], vp->m[3][1])));
+                        const f32 tz = pfma(pw.x, vp->m[0][2], pfma(pw.y, vp->m[1][2], pfma(pw.z, vp->m[2][2], vp->m[3][2])));
+                        const f32 iw = rcp_refined(w);
+                        const f32 sx = pfma(tx, iw * sxs, sxb);
+                        const f32 sy = pfma(ty, iw * sys, syb);
+                        cs.tx[v] = tx; cs.ty[v] = ty; cs.tz[v] = tz; cs.w[v] = w;
+                        cs.sx[v] = sx; cs.sy[v] = sy;
+                        ssm[v] = sx; ssm[64 + v] = sy; ssm[128 + v] = iw;
+                    }
+                    for (u32 b = 0; b < 16; b += 1) {
+                        setup_and_bin_batch(ps, b, (u32)ii, gm, &mesh->inds[m], &cs, at, ssm);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ raster
+   sft__rast_simd_batch, sofren2.c:5148-5499 (non-MSAA path) */
+static void raster_payload(port_state* ps, u64 payload, f32* depth, u32* vid, i32 tmx, i32 tmy) {
+    const sftx_frame_desc* f = ps->f;
+    u32 active = (u32)((payload >> 56) & 0xFF);
+    f32 sx[3][8], sy[3][8], iw[3][8];
+    u32 inst_ind = 0, m_ind = 0, batch = 0, slot = 0;
+    int clipped = 0;
+    if (0 == active) {
+        clipped = 1;
+        slot = (u32)payload;
+        const clip_rec* ct = (const clip_rec*)(ps->out->clips + (u64)slot * sizeof(clip_rec));
+        for (int k = 0; k < 3; k += 1) {
+            for (int l = 0; l < 8; l += 1) { sx[k][l] = ct->sx[k]; sy[k][l] = ct->sy[k]; iw[k][l] = ct->inv_w[k]; }
+        }
+        active = 1;
+    }
+    else {
+        inst_ind = (u32)(payload & 0xFFFFFF);
+        m_ind = (u32)((payload >> 24) & 0xFFFFFF);
+        batch = (u32)((payload >> 48) & 0xFF);
+        const sft_instance* inst = &f->instances[inst_ind];
+        const sft_mesh* mesh = &inst->model->meshes[inst->model->meshlet_mesh_inds[m_ind]];
+        const struct sft_meshlet_inds* mi = &mesh->inds[m_ind - mesh->model_meshlet_offset];
+        const f32* ssm = ss_at(ps, inst->ss_meshlet_offset + m_ind);
+        for (int k = 0; k < 3; k += 1) {
+            for (int l = 0; l < 8; l += 1) {
+                const int v = mi->inds[batch * 32 + k * 8 + l];
+                sx[k][l] = ssm[v]; sy[k][l] = ssm[64 + v]; iw[k][l] = ssm[128 + v];
+            }
+        }
+    }
+    for (int lane = 0; lane < 8; lane += 1) {
+        if (!(active & (1u << lane))) continue;
+        const f32 sx0 = sx[0][lane], sx1 = sx[1][lane], sx2 = sx[2][lane];
+        const f32 sy0 = sy[0][lane], sy1 = sy[1][lane], sy2 = sy[2][lane];
+        const f32 w0 = iw[0][lane], w1 = iw[1][lane], w2 = iw[2][lane];
+        const f32 A0 = sy0 - sy1, B0 = sx1 - sx0, A1 = sy1 - sy2, B1 = sx2 - sx1, A2 = sy2 - sy0, B2 = sx0 - sx2;
+        const f32 C0 = (pfma(sx0, sy1, -(sy0 * sx1)) - pfma(sy0, sx1, -(sx0 * sy1))) * 0.5f;
+        const f32 C1 = (pfma(sx1, sy2, -(sy1 * sx2)) - pfma(sy1, sx2, -(sx1 * sy2))) * 0.5f;
+        const f32 C2 = (pfma(sx2, sy0, -(sy2 * sx0)) - pfma(sy2, sx0, -(sx2 * sy0))) * 0.5f;
+        const f32 dx1 = sx1 - sx0, dx2 = sx2 - sx0, dy1 = sy1 - sy0, dy2 = sy2 - sy0;
+        const f32 det = dx1 * dy2 - dx2 * dy1;
+        if (!(det > SFT_EPSILON)) continue;
+        const f32 inv_det = rcp_refined(det);
+        const f32 dzdx = ((w1 - w0) * dy2 - (w2 - w0) * dy1) * inv_det;
+        const f32 dzdy = ((w2 - w0) * dx1 - (w1 - w0) * dx2) * inv_det;
+        const f32 c0 = pfma(B0, 0.5f, pfma(A0, 0.5f, C0));
+        const f32 c1 = pfma(B1, 0.5f, pfma(A1, 0.5f, C1));
+        const f32 c2 = pfma(B2, 0.5f, pfma(A2, 0.5f, C2));
+        const f32 zb = pfma(0.5f - sy0, dzdy, pfma(0.5f - sx0, dzdx, w0));
+        const i32 min_x = cvtt(smax(smin(sx0, smin(sx1, sx2)), (f32)tmx));
+        const i32 max_x = cvtt(smin(smax(sx0, smax(sx1, sx2)), (f32)(tmx + TS - 1)));
+        const i32 min_y = cvtt(smax(smin(sy0, smin(sy1, sy2)), (f32)tmy));
+        const i32 max_y = cvtt(smin(smax(sy0, smax(sy1, sy2)), (f32)(tmy + TS - 1)));
+        if (min_x > max_x || min_y > max_y) continue;
+        const u32 id = clipped ? (slot | (1u << 31))
+            : ((inst_ind << (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) | (m_ind << SFT_TRIANGLE_BITS) | (batch * 8 + (u32)lane));
+        const i32 amin = min_x & ~7, amax = (max_x & ~7) + 8;
+        const f32 se0 = c0 + A0 * (f32)amin + B0 * (f32)min_y;
+        const f32 se1 = c1 + A1 * (f32)amin + B1 * (f32)min_y;
+        const f32 se2 = c2 + A2 * (f32)amin + B2 * (f32)min_y;
+        const f32 sz = zb + dzdx * (f32)amin + dzdy * (f32)min_y;
+        const f32 bw = (f32)(amax - 1 - amin), bh = (f32)(max_y - min_y);
+        const f32 me0 = se0 + (A0 < 0.f ? A0 * bw : 0.f) + (B0 < 0.f ? B0 * bh : 0.f);
+        const f32 me1 = se1 + (A1 < 0.f ? A1 * bw : 0.f) + (B1 < 0.f ? B1 * bh : 0.f);
+        const f32 me2 = se2 + (A2 < 0.f ? A2 * bw : 0.f) + (B2 < 0.f ? B2 * bh : 0.f);
+        const int trivial = me0 >= 0.f && me1 >= 0.f && me2 >= 0.f;
+        f32 zr[8], e0r[8], e1r[8], e2r[8];
+        for (int j = 0; j < 8; j += 1) {
+            zr[j] = sz + dzdx * (f32)j;
+            e0r[j] = se0 + A0 * (f32)j;
+            e1r[j] = se1 + A1 * (f32)j;
+            e2r[j] = se2 + A2 * (f32)j;
+        }
+        const f32 dzc = dzdx * 8.f, a0c = A0 * 8.f, a1c = A1 * 8.f, a2c = A2 * 8.f;
+        for (i32 y = min_y; y <= max_y; y += 1) {
+            f32 z[8], e0[8], e1[8], e2[8];
+            memcpy(z, zr, sizeof(z)); memcpy(e0, e0r, sizeof(e0)); memcpy(e1, e1r, sizeof(e1)); memcpy(e2, e2r, sizeof(e2));
+            i32 p = (y - tmy) * TS + (amin - tmx);
+            for (i32 x = amin; x < amax; x += 8, p += 8) {
+                for (int j = 0; j < 8; j += 1) {
+                    const int cover = trivial || (e0[j] >= 0.f && e1[j] >= 0.f && e2[j] >= 0.f);
+                    if (cover && z[j] > depth[p + j]) {
+                        depth[p + j] = z[j];
+                        vid[p + j] = id;
+                    }
+                    z[j] = z[j] + dzc;
+                    e0[j] = e0[j] + a0c; e1[j] = e1[j] + a1c; e2[j] = e2[j] + a2c;
+                }
+            }
+            for (int j = 0; j < 8; j += 1) {
+                zr[j] = zr[j] + dzdy;
+                e0r[j] = e0r[j] + B0; e1r[j] = e1r[j] + B1; e2r[j] = e2r[j] + B2;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ texturing
+   sofren2.c:5501-5729 */
+static const sftx_texture_desc* tex_of(const sftx_frame_desc* f, u32 h) {
+    if (SFT_INVALID_HANDLE == h) return NULL;
+    const u32 i = h & SFT_HANDLE_INDEX_MASK;
+    if ((i32)i >= f->n_textures || !f->textures[i].texels) return NULL;
+    return &f->textures[i];
+}
+
+static u32 texel_addr(const sftx_texture_desc* t, int lod, i32 x, i32 y) {
+    const u32 ux = (u32)x, uy = (u32)y;
+    if (t->is_pot) return (((uy >> 2) << t->stride_shift[lod]) + (ux >> 2)) * 16u + ((uy & 3u) << 2) + (ux & 3u);
+    return (((uy >> 2) * (u32)t->blocks_w[lod]) + (ux >> 2)) * 16u + ((uy & 3u) << 2) + (ux & 3u);
+}
+
+static inline f32 ch(u32 c, int s) { return (f32)(i32)((c >> s) & 0xFFu) * (1.f / 255.f); }
+
+static void sample_bilinear(const sftx_frame_desc* f, u32 h, int lod, f32 cu, f32 cv, f32* r, f32* g, f32* b) {
+    const sftx_texture_desc* t = tex_of(f, h);
+    if (SFT_INVALID_HANDLE == h || !t) { *r = *g = *b = 1.f; return; }
+    const u32* px = t->texels + t->mip_offset[lod];
+    const i32 cw = SFT_MAX(t->w >> lod, 1), chh = SFT_MAX(t->h >> lod, 1);
+    const f32 upx = cu * (f32)cw - 0.5f, vpx = cv * (f32)chh - 0.5f;
+    const f32 uf = floorf(upx), vf = floorf(vpx);
+    const f32 ufr = upx - uf, vfr = vpx - vf;
+    const i32 u0 = cvtt(uf), v0 = cvtt(vf);
+    i32 u0w, v0w, u1, v1;
+    if (t->is_pot) {
+        u1 = (u0 + 1) & (cw - 1); v1 = (v0 + 1) & (chh - 1);
+        u0w = u0 & (cw - 1); v0w = v0 & (chh - 1);
+    }
+    else {
+        u1 = SFT_MIN(u0 + 1, cw - 1); v1 = SFT_MIN(v0 + 1, chh - 1);
+        u0w = SFT_MIN(u0, cw - 1); v0w = SFT_MIN(v0, chh - 1);
+    }
+    const u32 c00 = px[texel_addr(t, lod, u0w, v0w)], c10 = px[texel_addr(t, lod, u1, v0w)];
+    const u32 c01 = px[texel_addr(t, lod, u0w, v1)], c11 = px[texel_addr(t, lod, u1, v1)];
+    f32 o[3];
+    const int sh[3] = { 16, 8, 0 };
+    for (int k = 0; k < 3; k += 1) {
+        const f32 f00 = ch(c00, sh[k]), f10 = ch(c10, sh[k]), f01 = ch(c01, sh[k]), f11 = ch(c11, sh[k]);
+        const f32 top = f00 + (f10 - f00) * ufr, bot = f01 + (f11 - f01) * ufr;
+        o[k] = top + (bot - top) * vfr;
+    }
+    *r = o[0]; *g = o[1]; *b = o[2];
+}
+
+static void sample_nearest(const sftx_frame_desc* f, u32 h, int lod, f32 cu, f32 cv, f32* r, f32* g, f32* b) {
+    const sftx_texture_desc* t = tex_of(f, h);
+    if (SFT_INVALID_HANDLE == h || !t) { *r = *g = *b = 1.f; return; }
+    const u32* px = t->texels + t->mip_offset[lod];
+    const i32 cw = SFT_MAX(t->w >> lod, 1), chh = SFT_MAX(t->h >> lod, 1);
+    const i32 u0 = cvtt(floorf(cu * (f32)cw)), v0 = cvtt(floorf(cv * (f32)chh));
+    const i32 uw = t->is_pot ? (u0 & (cw - 1)) : SFT_MIN(u0, cw - 1);
+    const i32 vw = t->is_pot ? (v0 & (chh - 1)) : SFT_MIN(v0, chh - 1);
+    const u32 c = px[texel_addr(t, lod, uw, vw)];
+    *r = ch(c, 16); *g = ch(c, 8); *b = ch(c, 0);
+}
+
+/* scalar LOD from the first lane's UV gradients; the Bayer dither added before
+   truncation never changes an integer log2 (5501-5541) */
+static int calc_lod(const sftx_frame_desc* f, u32 h, f32 dux, f32 duy, f32 dvx, f32 dvy, i32 bx, i32 by) {
+    static const f32 bayer[4] = { -0.375f, 0.125f, 0.375f, -0.125f };
+    const sftx_texture_desc* t = tex_of(f, h);
+    if (SFT_INVALID_HANDLE == h || !t) return 0;
+    const f32 ux = dux * t->w, vx = dvx * t->h, uy = duy * t->w, vy = dvy * t->h;
+    const f32 rho = SFT_MAX(SFT_MAX(fabsf(ux), fabsf(vx)), SFT_MAX(fabsf(uy), fabsf(vy)));
+    i32 lod = 0;
+    if (rho > 1.f) {
+        const i32 ir = cvtt(rho);
+        const f32 lf = (f32)(31 - __builtin_clz((u32)ir));
+        lod = cvtt(lf + 0.5f + bayer[((by & 1) << 1) | (bx & 1)]);
+    }
+    return SFT_CLAMP(lod, 0, t->mip_levels - 1);
+}
+
+/* ------------------------------------------------------------ resolve
+   sft__resolve_tile, sofren2.c:5809-6753 (non-MSAA path) */
+typedef struct {
+    f32 sx[3], sy[3], iw[3], u[3], v[3], lu[3], lv[3];
+    v3 wp[3], wn[3];
+    u32 inst_ind, m_ind;
+} tri_t;
+
+static void fetch_tri(port_state* ps, u32 vid, tri_t* t) {
+    const sftx_frame_desc* f = ps->f;
+    if (vid & (1u << 31)) {
+        const clip_rec* ct = (const clip_rec*)(ps->out->clips + (u64)(vid & 0x7FFFFFFFu) * sizeof(clip_rec));
+        t->inst_ind = ct->inst_ind;
+        t->m_ind = ct->m_ind;
+        for (int k = 0; k < 3; k += 1) {
+            t->sx[k] = ct->sx[k]; t->sy[k] = ct->sy[k]; t->iw[k] = ct->inv_w[k];
+            t->u[k] = ct->u[k]; t->v[k] = ct->v[k]; t->lu[k] = ct->lu[k]; t->lv[k] = ct->lv[k];
+            t->wp[k].x = ct->px[k]; t->wp[k].y = ct->py[k]; t->wp[k].z = ct->pz[k];
+            t->wn[k].x = ct->nx[k]; t->wn[k].y = ct->ny[k]; t->wn[k].z = ct->nz[k];
+        }
+        return;
+    }
+    t->inst_ind = (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) & ((1u << SFT_INSTANCE_BITS) - 1);
+    t->m_ind = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1);
+    const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1);
+    const sft_instance* inst = &f->instances[t->inst_ind];
+    const sft_mesh* mesh = &inst->model->meshes[inst->model->meshlet_mesh_inds[t->m_ind]];
+    const u32 lm = t->m_ind - mesh->model_meshlet_offset;
+    const struct sft_meshlet_attribs* at = &mesh->meshlet_attribs[lm];
+    const f32* ssm = ss_at(ps, inst->ss_meshlet_offset + t->m_ind);
+    for (int k = 0; k < 3; k += 1) {
+        const int v = mesh->inds[lm].inds[(tri / 8) * 32 + k * 8 + (tri % 8)];
+        t->sx[k] = ssm[v]; t->sy[k] = ssm[64 + v]; t->iw[k] = ssm[128 + v];
+        t->u[k] = at->u[v]; t->v[k] = at->v[v]; t->lu[k] = at->lu[v]; t->lv[k] = at->lv[v];
+        t->wp[k].x = ssm[192 + v]; t->wp[k].y = ssm[256 + v]; t->wp[k].z = ssm[320 + v];
+        t->wn[k].x = ssm[384 + v]; t->wn[k].y = ssm[448 + v]; t->wn[k].z = ssm[512 + v];
+    }
+}
+
+/* perspective-correct weights at a pixel centre (6273-6318) */
+static void bary(const tri_t* t, f32 px, f32 py, f32* p0, f32* p1, f32* p2) {
+    const f32 dx1 = t->sx[1] - t->sx[0], dx2 = t->sx[2] - t->sx[0];
+    const f32 dy1 = t->sy[1] - t->sy[0], dy2 = t->sy[2] - t->sy[0];
+    const f32 inv_det = rcp_refined(dx1 * dy2 - dx2 * dy1);
+    const f32 ex = px - t->sx[0], ey = py - t->sy[0];
+    const f32 bu = (ex * dy2 - ey * dx2) * inv_det;
+    const f32 bv = (ey * dx1 - ex * dy1) * inv_det;
+    const f32 bw = (1.f - bu) - bv;
+    const f32 z = rcp_refined(pfma(bw, t->iw[0], pfma(bu, t->iw[1], bv * t->iw[2])));
+    *p0 = (bw * t->iw[0]) * z;
+    *p1 = (bu * t->iw[1]) * z;
+    *p2 = (bv * t->iw[2]) * z;
+}
+
+static inline f32 interp3(f32 p0, f32 p1, f32 p2, const f32* a) { return pfma(p0, a[0], pfma(p1, a[1], p2 * a[2])); }
+
+/* per-tile light list (5855-5934) */
+static int tile_lights(const sftx_frame_desc* f, f32 zmin, f32 zmax, i32 tmx, i32 tmy, u16 out[16]) {
+    int n = 0;
+    if (!(zmin <= zmax && f->n_point > 0)) return 0;
+    const sftmat* vp = &f->view_proj;
+    const f32 hw = f->half_width, hh = f->half_height;
+    for (i32 l = 0; l < f->n_point; l += 1) {
+        const sftvec pos = f->point_lights[l].pos;
+        const f32 radius = f->point_lights[l].radius;
+        const f32 w = pos.x * vp->m[0][3] + pos.y * vp->m[1][3] + pos.z * vp->m[2][3] + vp->m[3][3];
+        const f32 lmin = w - radius, lmax = w + radius;
+        if (lmax <= SFT_EPSILON) continue;
+        const f32 safe = (lmin > SFT_EPSILON) ? lmin : SFT_EPSILON;
+        const f32 iwmin = 1.f / lmax, iwmax = 1.f / safe;
+        if (iwmax < zmin || iwmin > zmax) continue;
+        if (w > radius) {
+            const f32 tx = pos.x * vp->m[0][0] + pos.y * vp->m[1][0] + pos.z * vp->m[2][0] + vp->m[3][0];
+            const f32 ty = pos.x * vp->m[0][1] + pos.y * vp->m[1][1] + pos.z * vp->m[2][1] + vp->m[3][1];
+            const f32 sx = (tx / w + 1.f) * hw + f->vp_x;
+            const f32 sy = (-ty / w + 1.f) * hh + f->vp_y;
+            const f32 psx = sqrtf(vp->m[0][0] * vp->m[0][0] + vp->m[1][0] * vp->m[1][0] + vp->m[2][0] * vp->m[2][0]);
+            const f32 psy = sqrtf(vp->m[0][1] * vp->m[0][1] + vp->m[1][1] * vp->m[1][1] + vp->m[2][1] * vp->m[2][1]);
+            const f32 sw = SFT_MAX(w - radius, SFT_EPSILON);
+            const f32 rx = ((radius * psx) + fabsf(tx) * (radius / w)) / sw * hw;
+            const f32 ry = ((radius * psy) + fabsf(ty) * (radius / w)) / sw * hh;
+            if (sx + rx < (f32)tmx || sx - rx > (f32)(tmx + TS) || sy + ry < (f32)tmy || sy - ry > (f32)(tmy + TS)) continue;
+        }
+        out[n++] = (u16)l;
+        if (16 == n) break;
+    }
+    return n;
+}
+
+/* shades one lane of a (block, triangle) group (6190-6620) */
+static u32 shade_lane(port_state* ps, const tri_t* t, f32 px, f32 py, i32 bx, i32 by, f32 dux, f32 duy, f32 dvx, f32 dvy,
+                      const u16* lights, int n_lights) {
+    const sftx_frame_desc* f = ps->f;
+    f32 p0, p1, p2;
+    bary(t, px, py, &p0, &p1, &p2);
+    const f32 cu = interp3(p0, p1, p2, t->u), cv = interp3(p0, p1, p2, t->v);
+    const f32 clu = interp3(p0, p1, p2, t->lu), clv = interp3(p0, p1, p2, t->lv);
+    const f32 ax[3] = { t->wp[0].x, t->wp[1].x, t->wp[2].x }, ay[3] = { t->wp[0].y, t->wp[1].y, t->wp[2].y }, az[3] = { t->wp[0].z, t->wp[1].z, t->wp[2].z };
+    const f32 nx[3] = { t->wn[0].x, t->wn[1].x, t->wn[2].x }, ny[3] = { t->wn[0].y, t->wn[1].y, t->wn[2].y }, nz[3] = { t->wn[0].z, t->wn[1].z, t->wn[2].z };
+    const v3 pos = { interp3(p0, p1, p2, ax), interp3(p0, p1, p2, ay), interp3(p0, p1, p2, az) };
+    const v3 nrm = { interp3(p0, p1, p2, nx), interp3(p0, p1, p2, ny), interp3(p0, p1, p2, nz) };
+    const v3 N = norm_fast(nrm);
+    const v3 cam = { f->camera_pos.x, f->camera_pos.y, f->camera_pos.z };
+    const v3 V = norm_fast(vsub(cam, pos));
+
+    const sft_instance* inst = &f->instances[t->inst_ind];
+    const sft_mesh* mesh = &inst->model->meshes[inst->model->meshlet_mesh_inds[t->m_ind]];
+    const sft_mat_handle mh = mesh->material;
+    const u32 lm_page = mesh->lightmap_page;
+    u32 base = 0xFFFFFFFFu, tex = SFT_INVALID_HANDLE;
+    f32 shin = 4.f;
+    if (SFT_INVALID_HANDLE != mh) {
+        const u32 mi = SFT_MAT_INDEX(mh);
+        if (SFT_MAT_IS_TRANSIENT(mh)) { base = f->tmat_color[mi]; tex = f->tmat_tex[mi]; shin = f->tmat_shininess[mi]; }
+        else { base = f->mat_color[mi]; tex = f->mat_tex[mi]; shin = f->mat_shininess[mi]; }
+    }
+    if (SFT_DEBUG_CHECKERBOARD == f->debug_mode) { tex = f->base_tex; base = 0xFFFFFFFFu; }
+    const f32 br = (f32)((base >> 16) & 0xFF) * (1.f / 255.f), bg = (f32)((base >> 8) & 0xFF) * (1.f / 255.f), bb = (f32)(base & 0xFF) * (1.f / 255.f);
+    const int lod = calc_lod(f, tex, dux, duy, dvx, dvy, bx, by);
+    f32 tr, tg, tb;
+    if (lod > 1) sample_nearest(f, tex, lod, cu, cv, &tr, &tg, &tb);
+    else sample_bilinear(f, tex, lod, cu, cv, &tr, &tg, &tb);
+    const f32 ar = tr * br, ag = tg * bg, ab = tb * bb;
+    f32 fr, fg, fb;
+    if (shin < 0.f) {
+        fr = ar * -shin; fg = ag * -shin; fb = ab * -shin;
+    }
+    else {
+        f32 dr, dg, db, sr = 0.f, sg = 0.f, sb = 0.f, lm_shadow = 1.f;
+        const f32 spec_scale = ((shin + 8.f) / 8.f) * 0.04f;
+        const int lightmapped = 0xFFFFFFFFu != lm_page;
+        if (lightmapped) {
+            sample_bilinear(f, inst->model->lightmaps[lm_page], 0, clu, clv, &dr, &dg, &db);
+            dr = dr * f->lightmap_brightness; dg = dg * f->lightmap_brightness; db = db * f->lightmap_brightness;
+        }
+        else {
+            dr = f->ambient.x; dg = f->ambient.y; db = f->ambient.z;
+            for (i32 l = 0; l < f->n_dir; l += 1) {
+                const v3 L = { f->dir_lights[l].dir.x, f->dir_lights[l].dir.y, f->dir_lights[l].dir.z };
+                const sftvec c = f->dir_lights[l].color;
+                const f32 ndl = smax(vdot(N, L), 0.f);
+                dr = pfma(ndl, c.x, dr); dg = pfma(ndl, c.y, dg); db = pfma(ndl, c.z, db);
+                if (ndl > 0.f) {
+                    const v3 H = norm_fast(vadd(L, V));
+                    const f32 ndh = smax(vdot(N, H), 0.f);
+                    const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
+                    const f32 term = (spec * spec_scale) * ndl;
+                    sr = pfma(term, c.x, sr); sg = pfma(term, c.y, sg); sb = pfma(term, c.z, sb);
+                }
+            }
+        }
+        for (int i = 0; i < n_lights; i += 1) {
+            const sft_point_light* L = &f->point_lights[lights[i]];
+            const v3 lp = { L->pos.x, L->pos.y, L->pos.z };
+            const v3 lraw = vsub(lp, pos);
+            const f32 d2 = vdot(lraw, lraw);
+            f32 att = smax(0.f, pfma(-d2, L->inv_radius_sqr, 1.f));
+            att = att * att;
+            if (!(att > 0.f)) continue;
+            const v3 Ld = norm_fast(lraw);
+            const f32 ndl = smax(vdot(N, Ld), 0.f);
+            const int lit = ndl > 0.f;
+            const f32 si = smax(0.f, 1.f - 0.f);
+            if (lightmapped) {
+                lm_shadow = lm_shadow * (1.f - (1.f - si) * att);
+            }
+            else {
+                const f32 di = ndl * att;
+                const f32 mdi = lit ? di * si : 0.f;
+                dr = pfma(mdi, L->color.x, dr); dg = pfma(mdi, L->color.y, dg); db = pfma(mdi, L->color.z, db);
+                if (lit) {
+                    const v3 H = norm_fast(vadd(Ld, V));
+                    const f32 ndh = smax(vdot(N, H), 0.f);
+                    const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
+                    const f32 term = ((spec * spec_scale) * di) * si;
+                    sr = pfma(term, L->color.x, sr); sg = pfma(term, L->color.y, sg); sb = pfma(term, L->color.z, sb);
+                }
+            }
+        }
+        if (lightmapped) {
+            const f32 mod = pfma(lm_shadow, 1.f - 0.4f, 0.4f);
+            dr = dr * mod; dg = dg * mod; db = db * mod;
+            fr = ar * dr; fg = ag * dg; fb = ab * db;
+        }
+        else {
+            fr = pfma(ar, dr, sr); fg = pfma(ag, dg, sg); fb = pfma(ab, db, sb);
+        }
+    }
+    fr = smax(0.f, smin(fr, 1.f)) * 255.f;
+    fg = smax(0.f, smin(fg, 1.f)) * 255.f;
+    fb = smax(0.f, smin(fb, 1.f)) * 255.f;
+    return 0xFF000000u | ((u32)cvtt(fr) << 16) | ((u32)cvtt(fg) << 8) | (u32)cvtt(fb);
+}
+
+static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 tmx, i32 tmy) {
+    const sftx_frame_desc* f = ps->f;
+    const i32 sw = (i32)(f->half_width * 2.f), sh = (i32)(f->half_height * 2.f);
+    const i32 lx = SFT_MIN(tmx + TS, sw), ly = SFT_MIN(tmy + TS, sh);
+    u32* color = ps->out->color;
+    const i32 pitch = f->width;
+    f32 zmin = 1e9f, zmax = -1e9f;
+    for (i32 i = 0; i < TP; i += 1) {
+        if (depth[i] > 0.0001f) {
+            if (depth[i] < zmin) zmin = depth[i];
+            if (depth[i] > zmax) zmax = depth[i];
+        }
+    }
+    u16 lights[16];
+    const int n_lights = tile_lights(f, zmin, zmax, tmx, tmy, lights);
+    static const i32 ox[8] = { 0, 1, 2, 3, 0, 1, 2, 3 }, oy[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+    for (i32 y = tmy; y < ly; y += 2) {
+        for (i32 x = tmx; x < lx; x += 4) {
+            u32 vids[8];
+            int inb = 0, active = 0;
+            for (int l = 0; l < 8; l += 1) {
+                const i32 gx = x + ox[l], gy = y + oy[l];
+                if (gx < sw && gy < sh) inb |= 1 << l;
+                vids[l] = vid[(gy - tmy) * TS + (gx - tmx)];
+                if ((inb & (1 << l)) && 0xFFFFFFFFu != vids[l]) active |= 1 << l;
+                if (f->should_clear && (inb & (1 << l)) && 0xFFFFFFFFu == vids[l]) color[gy * pitch + gx] = f->clear_color;
+            }
+            while (active) {
+                const int first = __builtin_ctz(active);
+                const u32 target = vids[first];
+                int lanes = 0;
+                for (int l = 0; l < 8; l += 1) {
+                    if ((inb & (1 << l)) && vids[l] == target) lanes |= 1 << l;
+                }
+                lanes &= active;
+                active &= ~lanes;
+                tri_t t;
+                fetch_tri(ps, target, &t);
+                /* LOD lane: first lane of the (in-bounds) match mask; ddx/ddy pair lanes l^1 and l^4 */
+                const int fl = __builtin_ctz(lanes);
+                f32 q0, q1, q2, cu0, cv0, cux, cvx, cuy, cvy;
+                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl], ((f32)y + 0.5f) + (f32)oy[fl], &q0, &q1, &q2);
+                cu0 = interp3(q0, q1, q2, t.u); cv0 = interp3(q0, q1, q2, t.v);
+                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl ^ 1], ((f32)y + 0.5f) + (f32)oy[fl ^ 1], &q0, &q1, &q2);
+                cux = interp3(q0, q1, q2, t.u); cvx = interp3(q0, q1, q2, t.v);
+                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl ^ 4], ((f32)y + 0.5f) + (f32)oy[fl ^ 4], &q0, &q1, &q2);
+                cuy = interp3(q0, q1, q2, t.u); cvy = interp3(q0, q1, q2, t.v);
+                for (int l = 0; l < 8; l += 1) {
+                    if (!(lanes & (1 << l))) continue;
+                    const f32 px = ((f32)x + 0.5f) + (f32)ox[l], py = ((f32)y + 0.5f) + (f32)oy[l];
+                    color[(y + oy[l]) * pitch + x + ox[l]] = shade_lane(ps, &t, px, py, x, y, cu0 - cux, cu0 - cuy, cv0 - cvx, cv0 - cvy,
+                                                                         lights, n_lights);
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ frame
+   sft_present + worker phases, sofren2.c:8139-8289, 6822-6867 */
+int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
+    port_state ps;
+    memset(&ps, 0, sizeof(ps));
+    ps.f = f;
+    ps.out = out;
+    ps.ss = (f32*)out->ss;
+    const i32 nt = f->tile_cols * f->tile_rows;
+    ps.bins = (bin_t*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(bin_t));
+    memset(out->ss_vis, 0, f->total_ss);
+    geometry_pass(&ps);
+    out->clip_count = ps.clip_next < out->clip_cap ? ps.clip_next : out->clip_cap;
+
+    const i32 gw = f->tile_cols * TS, gh = f->tile_rows * TS;
+    f32 depth[TP];
+    u32 vid[TP];
+    for (i32 t = 0; t < nt; t += 1) {
+        const i32 tmx = (t % f->tile_cols) * TS, tmy = (t / f->tile_cols) * TS;
+        memset(depth, 0, sizeof(depth));
+        memset(vid, 0xFF, sizeof(vid));
+        for (u32 k = 0; k < ps.bins[t].n; k += 1) raster_payload(&ps, ps.bins[t].p[k], depth, vid, tmx, tmy);
+        if (out->vid) {
+            for (i32 y = 0; y < TS; y += 1) {
+                for (i32 x = 0; x < TS; x += 1) {
+                    const u64 g = (u64)(tmy + y) * gw + tmx + x;
+                    out->vid[g] = vid[y * TS + x];
+                    out->depth[g] = depth[y * TS + x];
+                }
+            }
+        }
+        resolve_tile(&ps, depth, vid, tmx, tmy);
+        free(ps.bins[t].p);
+    }
+    (void)gh;
+    free(ps.bins);
+    return ps.clip_overflow ? -1 : 0;
+}
+
+u32 sftx_port_ss_count(const sftx_frame_desc* f) { return f->total_ss; }
+
+void sftx_port_grid(const sftx_frame_desc* f, i32* cols, i32* rows, i32* width, i32* height) {
+    *cols = f->tile_cols;
+    *rows = f->tile_rows;
+    *width = f->width;
+    *height = f->height;
+}

@@ -66,6 +66,10 @@ class Library:
         self.cap_clip_count = _bind(g, "sftx_cap_clip_count", u32, [])
         self.cap_clip_records = _bind(g, "sftx_cap_clip_records", None, [vp])
         self.get_stats = _bind(g, "sftx_get_stats", None, [ctypes.POINTER(FrameStats)])
+        self.export_frame = _bind(g, "sftx_export_frame", None, [vp])
+        self.sft_present = _bind(g, "sft_present", None, [])
+        self.sft_frame_end = _bind(g, "sft_frame_end", None, [])
+        self.scene_record = _bind(s, "sftx_scene_record", None, [i32, vp, i32])
         self.scene_create = _bind(s, "sftx_scene_create", i32, [i32, u32, i32, i32])
         self.scene_render = _bind(s, "sftx_scene_render", None, [i32, vp, i32])
         self.scene_destroy = _bind(s, "sftx_scene_destroy", None, [])

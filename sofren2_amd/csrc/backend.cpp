@@ -558,26 +558,30 @@ extern "C" void sftx_cap_grid(i32* cols, i32* rows, i32* samples) {
 }
 
 extern "C" void sftx_cap_visibility(u32* vid, f32* depth) {
-    Backend* b = g_ctx->backend;
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b) return;
     if (vid && !b->h_vid.empty()) memcpy(vid, b->h_vid.data(), b->h_vid.size() * 4);
     if (depth && !b->h_depth.empty()) memcpy(depth, b->h_depth.data(), b->h_depth.size() * 4);
 }
 
-extern "C" u32 sftx_cap_ss_count(void) { return g_ctx->backend->h_ss_count; }
+extern "C" u32 sftx_cap_ss_count(void) { return (g_ctx && g_ctx->backend) ? g_ctx->backend->h_ss_count : 0u; }
 
 extern "C" void sftx_cap_ss_cache(void* out) {
-    Backend* b = g_ctx->backend;
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b) return;
     if (!b->h_ss.empty()) memcpy(out, b->h_ss.data(), b->h_ss.size());
 }
 
 extern "C" void sftx_cap_ss_visible(u8* out) {
-    Backend* b = g_ctx->backend;
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b) return;
     if (!b->h_ss_vis.empty()) memcpy(out, b->h_ss_vis.data(), b->h_ss_vis.size());
 }
 
-extern "C" u32 sftx_cap_clip_count(void) { return g_ctx->backend->h_clip_count; }
+extern "C" u32 sftx_cap_clip_count(void) { return (g_ctx && g_ctx->backend) ? g_ctx->backend->h_clip_count : 0u; }
 
 extern "C" void sftx_cap_clip_records(void* out) {
-    Backend* b = g_ctx->backend;
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b) return;
     if (!b->h_clips.empty()) memcpy(out, b->h_clips.data(), b->h_clips.size());
 }

@@ -322,7 +322,6 @@ extern "C" void sft_init(i32 width, i32 height, i32 requested_threads) {
     set_display(c, width, height);
 
     probe_estimate_tables(&c.est);
-    c.backend = backend_create();
 
     /* 256x256 checkerboard of 32x32 squares and the base material */
     std::vector<u32> checker(256 * 256);
@@ -391,6 +390,7 @@ extern "C" void sft_frame_end(void) {
 
 extern "C" void sft_present(void) {
     Context& c = C();
+    if (!c.backend) c.backend = backend_create();   /* exits if there is no GPU */
     backend_present(c.backend, &c);
 }
 
@@ -963,3 +963,67 @@ extern "C" sft_model* sft_model_load_qsf(const char* path, sft_arena* arena) {
 /* ================================================================ capture + stats */
 extern "C" void sftx_set_capture(u32 flags) { C().capture_flags = flags; }
 extern "C" void sftx_get_stats(sftx_frame_stats* out) { *out = C().stats; }
+
+extern "C" void sftx_export_frame(sftx_frame_desc* f) {
+    Context& c = C();
+    memset(f, 0, sizeof(*f));
+    f->width = c.width;
+    f->height = c.height;
+    f->tile_cols = c.tile_cols;
+    f->tile_rows = c.tile_rows;
+    f->half_width = c.half_width;
+    f->half_height = c.half_height;
+    f->vp_x = c.vp_x;
+    f->vp_y = c.vp_y;
+    f->camera_pos = c.camera_pos;
+    f->view_proj = c.view_proj;
+    memcpy(f->planes, c.planes, sizeof(f->planes));
+    f->ambient = c.ambient;
+    f->dir_lights = c.dir_lights.data();
+    f->n_dir = (i32)c.dir_lights.size();
+    f->point_lights = c.point_lights.data();
+    f->n_point = (i32)c.point_lights.size();
+    f->lightmap_brightness = c.lightmap_brightness;
+    f->shadows_enabled = c.shadows_enabled;
+    f->should_clear = c.should_clear;
+    f->clear_color = c.clear_color;
+    f->debug_mode = (i32)c.debug_mode;
+    f->instances = c.instances.data();
+    f->n_instances = (i32)c.instances.size();
+    f->total_ss = c.total_ss;
+    c.export_proxies.resize(c.proxies.size());
+    for (size_t i = 0; i < c.proxies.size(); i += 1) {
+        c.export_proxies[i].posa = c.proxies[i].posa;
+        c.export_proxies[i].posb = c.proxies[i].posb;
+        c.export_proxies[i].radius = c.proxies[i].radius;
+    }
+    f->proxies = c.export_proxies.data();
+    f->n_proxies = (i32)c.export_proxies.size();
+    c.export_tex.resize(c.tex.size());
+    for (size_t i = 0; i < c.tex.size(); i += 1) {
+        const HostTexture& t = c.tex[i];
+        sftx_texture_desc& d = c.export_tex[i];
+        memset(&d, 0, sizeof(d));
+        d.texels = t.texels.empty() ? nullptr : t.texels.data();
+        d.w = t.w;
+        d.h = t.h;
+        d.mip_levels = t.mip_levels;
+        d.is_pot = t.is_pot;
+        memcpy(d.mip_offset, t.mip_offset, sizeof(d.mip_offset));
+        memcpy(d.blocks_w, t.blocks_w, sizeof(d.blocks_w));
+        memcpy(d.stride_shift, t.stride_shift, sizeof(d.stride_shift));
+    }
+    f->textures = c.export_tex.data();
+    f->n_textures = (i32)c.export_tex.size();
+    f->mat_tex = c.mat_tex.data();
+    f->mat_color = c.mat_color.data();
+    f->mat_shininess = c.mat_shininess.data();
+    f->n_mat = (i32)c.mat_tex.size();
+    f->tmat_tex = c.tmat_tex.data();
+    f->tmat_color = c.tmat_color.data();
+    f->tmat_shininess = c.tmat_shininess.data();
+    f->n_tmat = (i32)c.tmat_tex.size();
+    f->base_tex = (c.base_mat != kInvalid) ? c.mat_tex[SFT_MAT_INDEX(c.base_mat)] : SFT_INVALID_HANDLE;
+    f->color_buffer = c.color_buffer;
+    f->color_pitch = c.color_pitch;
+}

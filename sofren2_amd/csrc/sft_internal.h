@@ -180,6 +180,8 @@ struct Context {
     sftx_frame_stats stats{};
 
     EstimateTables est;
+    std::vector<sftx_texture_desc> export_tex;
+    std::vector<sftx_proxy_desc> export_proxies;
     Backend* backend = nullptr;
 };
 

@@ -123,13 +123,17 @@ i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
     }
 }
 
-void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes) {
+void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes) {
     sft_set_color_buffer(color, pitch_bytes);
     sft_frame_begin();
     switch (g_scene.kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_render(frame); break;
         default: break;
     }
+}
+
+void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes) {
+    sftx_scene_record(frame, color, pitch_bytes);
     sft_present();
     sft_frame_end();
 }

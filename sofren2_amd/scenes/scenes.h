@@ -36,6 +36,10 @@ enum sftx_scene_kind {
 /* Creates the scene's resources; call after sft_init.  Returns 0 on success. */
 i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height);
 
+/* Records one frame of the scene (buffer, frame_begin, camera, lights, draws)
+   without presenting it. */
+void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes);
+
 /* Records one frame of the scene (camera, lights, draws) and presents it into
    the caller's ARGB8888 buffer: frame_begin, camera, lights, draws,
    sft_present, frame_end. */

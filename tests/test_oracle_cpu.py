@@ -1,0 +1,37 @@
+"""The C restatement (oracle) against the reference renderer, on the CPU.
+
+The restatement renders the frame the product's front-end records; the
+reference renders the same seeded scene through its own front-end.  Every
+intermediate must agree bit for bit: this pins the oracle (and the
+front-end's host math) to the reference.
+"""
+import numpy as np
+import pytest
+
+import sofren2_amd as S
+from tests import parity
+from tests.oracle_port import port_render_scene
+from tests.refrun import ref_dump, have_ref
+
+pytestmark = pytest.mark.skipif(not have_ref(), reason="reference checker build missing")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_port_matches_reference_plane_spheres(seed):
+    w, h = S.SCENE_SIZE[S.SCENE_PLANE_SPHERES]
+    port = port_render_scene(S.SCENE_PLANE_SPHERES, seed, w, h)
+    ref = ref_dump(S.SCENE_PLANE_SPHERES, seed, w, h)
+    rep = parity.compare_frame(port, ref)
+    print(rep)
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
+
+
+def test_reference_dump_is_sane():
+    w, h = S.SCENE_SIZE[S.SCENE_PLANE_SPHERES]
+    ref = ref_dump(S.SCENE_PLANE_SPHERES, 0, w, h)
+    assert ref["meta"]["ss_count"] == 569
+    assert ref["ssvis"].sum() > 0
+    covered = (ref["vid"] != 0xFFFFFFFF).mean()
+    assert 0.2 < covered < 0.95
+    assert (ref["color"] >> 24 == 0xFF).all()
