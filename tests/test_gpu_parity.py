@@ -35,3 +35,14 @@ def test_plane_spheres_parity(seed):
     rep = parity.compare_frame(gpu, ref)
     print("plane_spheres seed", seed, rep, "visible", st.meshlets_visible, "ms", st.ms_total, flush=True)
     parity.assert_parity(rep)
+
+
+@pytest.mark.parametrize("frame", [0, 37])
+def test_box_room_parity(frame):
+    w, h = S.SCENE_SIZE[S.SCENE_BOX_ROOM]
+    gpu, st = render_gpu(S.SCENE_BOX_ROOM, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_BOX_ROOM, 0, w, h, frame)
+    rep = parity.compare_frame(gpu, ref)
+    print("box_room frame", frame, rep, "visible", st.meshlets_visible, "clips", st.clip_triangles,
+          "ms", st.ms_total, flush=True)
+    parity.assert_parity(rep)

@@ -35,3 +35,14 @@ def test_reference_dump_is_sane():
     covered = (ref["vid"] != 0xFFFFFFFF).mean()
     assert 0.2 < covered < 0.95
     assert (ref["color"] >> 24 == 0xFF).all()
+
+
+@pytest.mark.parametrize("frame", [0, 37])
+def test_port_matches_reference_box_room(frame):
+    w, h = S.SCENE_SIZE[S.SCENE_BOX_ROOM]
+    port = port_render_scene(S.SCENE_BOX_ROOM, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_BOX_ROOM, 0, w, h, frame)
+    rep = parity.compare_frame(port, ref)
+    print(rep, "clips", rep["clip_count"])
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
