@@ -1,0 +1,157 @@
+"""Frame-rate benchmark of the per-frame GPU render path on one MI355X.
+
+Workload: the 1920x1080 textured box-room interior (about 260k triangles in
+4096 boxes, 16 value-noise textures with mip chains, a lightmap page, 8 point
+lights), rendered along its fixed 120-frame camera path.  One step is one
+sft_present of one frame.  The scene is resident in HBM after warm-up and the
+frame stays in HBM (device output target), so the timed value is GPU-resident
+throughput; the PCIe-inclusive rate with the drop-in host output is reported
+beside it.
+
+Prints one JSON line: frames/s, Mpix/s, latency percentiles, the roofline of
+the dominant kernel (k_tile: tile raster + resolve) from the algorithmic-byte
+formulas, and the reference renderer timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "1080p frames/s & Mpix/s; p50/p99 request latency, served vs per-frame GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(st):
+    """Per-frame algorithmic HBM bytes of each pass (counts from the frame's
+    own counters): compulsory traffic plus the reference's materialised
+    intermediates, each counted once written and once read."""
+    geometry = (36 * st.meshlets_tested + st.meshlets_visible * (768 + 768 + 384) + 768 * st.meshlets_skinned
+                + 2304 * st.meshlets_visible)
+    binning = 8 * st.tile_payloads + 164 * st.clip_triangles
+    visibility = 8 * st.tile_payloads + 288 * st.tile_payloads + 4 * st.samples
+    shading = (4 * st.samples + 156 * st.shading_groups + 16 * st.tex_bilinear + 4 * st.tex_nearest
+               + 16 * st.lightmapped_pixels + 4 * st.pixels)
+    return {"geometry": geometry, "binning": binning, "visibility": visibility, "shading": shading}
+
+
+def cpu_baseline(width, height, frames):
+    exe = os.path.join(ROOT, "oracle", "_ref", "sofren2_ref")
+    if not os.path.exists(exe):
+        return None
+    cores = os.cpu_count() or 1
+    try:
+        proc = subprocess.run([exe, "bench", "1", "0", str(width), str(height), str(cores), "5", str(frames)],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600, check=True)
+        r = json.loads(proc.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError) as e:
+        print(f"cpu baseline failed: {e}", file=sys.stderr)
+        return None
+    return {"value": round(r["fps"], 3), "unit": "frames/s", "cores": r["threads"], "kind": "reference",
+            "sample": f"box-room 1920x1080, {frames} frames of the camera path after 5 warm-up frames, "
+                      f"reference AVX2 build with {r['threads']} worker threads (logical CPUs); "
+                      f"median {r['ms_median']:.2f} ms, p99 {r['ms_p99']:.2f} ms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=120)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
+              "(views and requests would shard across GPUs with no exchange).", file=sys.stderr)
+        sys.exit(2)
+
+    import sofren2_amd as S
+    lib = S.library()
+    w, h = S.SCENE_SIZE[S.SCENE_BOX_ROOM]
+    lib.sft_init(w, h, 0)
+    if lib.scene_create(S.SCENE_BOX_ROOM, 0, w, h) != 0:
+        raise SystemExit("scene creation failed")
+    tris = lib.scene_tris()
+    color = np.zeros((h, w), dtype=np.uint32)
+
+    # drop-in path with the frame copied to host memory (PCIe included)
+    lib.set_output(0)
+    for i in range(3):
+        lib.scene_render(i, color.ctypes.data, w * 4)
+    lib.synchronize()
+    t0 = time.perf_counter()
+    n_host = 30
+    for i in range(n_host):
+        lib.scene_render(i, color.ctypes.data, w * 4)
+    lib.synchronize()
+    host_fps = n_host / (time.perf_counter() - t0)
+
+    # GPU-resident output
+    lib.set_output(1)
+    for i in range(args.warmup):
+        lib.scene_render(i % 120, color.ctypes.data, w * 4)
+    lib.synchronize()
+    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms = [], [], [], [], []
+    bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
+    st = S.FrameStats()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        a = time.perf_counter()
+        lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
+        lat.append((time.perf_counter() - a) * 1e3)
+        lib.get_stats(st)
+        tile_ms.append(st.ms_raster_shade)
+        geom_ms.append(st.ms_geometry)
+        bin_ms.append(st.ms_binning)
+        total_gpu_ms.append(st.ms_total)
+        for k, v in algorithmic_bytes(st).items():
+            bytes_acc[k] += v
+    lib.synchronize()
+    elapsed = time.perf_counter() - t0
+    fps = args.steps / elapsed
+
+    tile_avg_ms = float(np.mean(tile_ms))
+    tile_bytes = (bytes_acc["visibility"] + bytes_acc["shading"]) / args.steps
+    achieved = tile_bytes / (tile_avg_ms * 1e-3) / 1e9
+    lat = np.array(lat)
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "box_room_1920x1080", "width": w, "height": h, "triangles": int(tris),
+                   "frames_in_path": 120, "msaa": 1, "output": "device"},
+        "mpix_s": round(fps * w * h / 1e6, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "pcie_inclusive_frames_s": round(host_fps, 3),
+        "pass_ms": {"geometry": round(float(np.mean(geom_ms)), 4), "binning": round(float(np.mean(bin_ms)), 4),
+                    "raster_shade": round(tile_avg_ms, 4), "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
+        "roofline": {"bound": "hbm", "kernel": "k_tile", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(tile_bytes)},
+        "cpu_baseline": None,
+    }
+    lib.scene_destroy()
+    lib.sft_release()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, h, args.cpu_frames)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,10 @@ typedef struct sftx_frame_stats {
     u32 samples;              /* S: width*height*samples_per_pixel */
     u32 pixels;               /* X */
     u32 shaded_pixels;        /* pixels covered by geometry */
+    u32 shading_groups;       /* Q: (4x2 block, triangle) groups shaded */
+    u32 tex_bilinear;         /* T_b: bilinear albedo samples */
+    u32 tex_nearest;          /* T_n: nearest albedo samples */
+    u32 lightmapped_pixels;   /* L_px: pixels with a lightmap sample */
     f32 ms_upload;            /* host->device copies of the frame's commands */
     f32 ms_geometry;          /* cull + skin + project + classify */
     f32 ms_binning;           /* clip + bin */
@@ -74,6 +78,22 @@ typedef struct sftx_frame_stats {
 } sftx_frame_stats;
 
 void sftx_get_stats(sftx_frame_stats* out);
+
+/* Output target of sft_present.  SFTX_OUTPUT_HOST (default) copies the frame
+   into the buffer given to sft_set_color_buffer, as the reference does.
+   SFTX_OUTPUT_DEVICE leaves the ARGB frame in HBM (sftx_device_color) for
+   consumers on the GPU; nothing crosses PCIe. */
+#define SFTX_OUTPUT_HOST   0
+#define SFTX_OUTPUT_DEVICE 1
+void sftx_set_output(i32 mode);
+const u32* sftx_device_color(void);     /* device pointer, width*height ARGB */
+
+/* Waits for all GPU work of the renderer. */
+void sftx_synchronize(void);
+
+/* Copies `bytes` between two device buffers with a plain streaming kernel
+   and returns its duration in ms; a calibration point for HBM counters. */
+f32 sftx_copy_calibration(u64 bytes);
 
 /* ---- frame description -------------------------------------------------
    Everything the per-frame render path reads, as recorded by the front-end

@@ -31,7 +31,8 @@ SCENE_SIZE = {SCENE_PLANE_SPHERES: (640, 360), SCENE_BOX_ROOM: (1920, 1080)}
 class FrameStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "meshlets_tested", "meshlets_visible", "meshlets_skinned", "batch_payloads", "clip_triangles",
-        "tile_payloads", "tiles", "samples", "pixels", "shaded_pixels")] + [(n, ctypes.c_float) for n in (
+        "tile_payloads", "tiles", "samples", "pixels", "shaded_pixels", "shading_groups", "tex_bilinear",
+        "tex_nearest", "lightmapped_pixels")] + [(n, ctypes.c_float) for n in (
         "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_total")]
 
 
@@ -70,6 +71,9 @@ class Library:
         self.sft_present = _bind(g, "sft_present", None, [])
         self.sft_frame_end = _bind(g, "sft_frame_end", None, [])
         self.scene_record = _bind(s, "sftx_scene_record", None, [i32, vp, i32])
+        self.set_output = _bind(g, "sftx_set_output", None, [i32])
+        self.synchronize = _bind(g, "sftx_synchronize", None, [])
+        self.copy_calibration = _bind(g, "sftx_copy_calibration", ctypes.c_float, [ctypes.c_uint64])
         self.scene_create = _bind(s, "sftx_scene_create", i32, [i32, u32, i32, i32])
         self.scene_render = _bind(s, "sftx_scene_render", None, [i32, vp, i32])
         self.scene_destroy = _bind(s, "sftx_scene_destroy", None, [])

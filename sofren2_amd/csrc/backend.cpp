@@ -83,6 +83,7 @@ struct Backend {
     DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
     DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
     DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, cap_vid, cap_depth, counters;
+    i32 output_mode = SFTX_OUTPUT_HOST;
     u32 clip_cap = 1u << 16;
     u32 tile_list_cap = 1u << 20;
 
@@ -421,7 +422,8 @@ void backend_present(Backend* b, Context* cp) {
         dgrow(b->cap_vid, grid * 4, false, s);
         dgrow(b->cap_depth, grid * 4, false, s);
     }
-    if (!c.should_clear && c.color_buffer && px) {
+    const bool to_host = SFTX_OUTPUT_HOST == b->output_mode && c.color_buffer;
+    if (!c.should_clear && to_host && px) {
         HIP_CHECK(hipMemcpy2DAsync(b->color.p, (size_t)c.width * 4, c.color_buffer, (size_t)c.color_pitch * 4,
                                    (size_t)c.width * 4, c.height, hipMemcpyHostToDevice, s));
     }
@@ -495,7 +497,7 @@ void backend_present(Backend* b, Context* cp) {
         if (counters[CNT_OVERFLOW] & 2u) b->tile_list_cap = std::max(b->tile_list_cap * 2u, counters[CNT_TILE_PAIRS] + 4096u);
     }
 
-    if (c.color_buffer && px) {
+    if (to_host && px) {
         HIP_CHECK(hipMemcpy2DAsync(c.color_buffer, (size_t)c.color_pitch * 4, b->color.p, (size_t)c.width * 4,
                                    (size_t)c.width * 4, c.height, hipMemcpyDeviceToHost, s));
     }
@@ -514,6 +516,10 @@ void backend_present(Backend* b, Context* cp) {
     st.samples = (u32)px;
     st.pixels = (u32)px;
     st.shaded_pixels = counters[CNT_SHADED];
+    st.shading_groups = counters[CNT_QGROUPS];
+    st.tex_bilinear = counters[CNT_TEX_BILINEAR];
+    st.tex_nearest = counters[CNT_TEX_NEAREST];
+    st.lightmapped_pixels = counters[CNT_LIGHTMAPPED];
     st.ms_upload = ev_ms(b->ev[0], b->ev[1]);
     st.ms_geometry = ev_ms(b->ev[1], b->ev[2]);
     st.ms_binning = ev_ms(b->ev[2], b->ev[3]);
@@ -584,4 +590,39 @@ extern "C" void sftx_cap_clip_records(void* out) {
     Backend* b = g_ctx ? g_ctx->backend : nullptr;
     if (!b) return;
     if (!b->h_clips.empty()) memcpy(out, b->h_clips.data(), b->h_clips.size());
+}
+
+static Backend* ensure_backend() {
+    if (!g_ctx) {
+        fprintf(stderr, "fatal sofren2 error: used before sft_init\n");
+        exit(1);
+    }
+    if (!g_ctx->backend) g_ctx->backend = backend_create();
+    return g_ctx->backend;
+}
+
+extern "C" void sftx_set_output(i32 mode) { ensure_backend()->output_mode = mode; }
+
+extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->color.p; }
+
+extern "C" void sftx_synchronize(void) {
+    Backend* b = ensure_backend();
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+}
+
+extern "C" f32 sftx_copy_calibration(u64 bytes) {
+    Backend* b = ensure_backend();
+    bytes = (bytes + 15) & ~15ull;
+    void *src = nullptr, *dst = nullptr;
+    HIP_CHECK(hipMalloc(&src, bytes));
+    HIP_CHECK(hipMalloc(&dst, bytes));
+    HIP_CHECK(hipMemsetAsync(src, 1, bytes, b->stream));
+    HIP_CHECK(hipEventRecord(b->ev[6], b->stream));
+    launch_copy((const uint4*)src, (uint4*)dst, bytes / 16, b->stream);
+    HIP_CHECK(hipEventRecord(b->ev[7], b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    const f32 ms = ev_ms(b->ev[6], b->ev[7]);
+    HIP_CHECK(hipFree(src));
+    HIP_CHECK(hipFree(dst));
+    return ms;
 }

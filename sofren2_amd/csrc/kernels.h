@@ -19,6 +19,10 @@ enum Counter : int {
     CNT_TILE_PAIRS,         /* total entries of the tile lists */
     CNT_OVERFLOW,           /* bit 0: clip pool, bit 1: tile lists */
     CNT_SHADED,             /* pixels with geometry */
+    CNT_QGROUPS,            /* (block, triangle) shading groups */
+    CNT_TEX_BILINEAR,
+    CNT_TEX_NEAREST,
+    CNT_LIGHTMAPPED,
     CNT_COUNT
 };
 
@@ -69,5 +73,6 @@ struct FramePtrs {
 };
 
 void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev);
+void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s);
 
 }  // namespace sft

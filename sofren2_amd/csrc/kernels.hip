@@ -801,7 +801,7 @@ struct TileLights {
 /* shades one pixel of triangle t (sofren2.c:6190-6620, non-MSAA); dux..dvy are
    the block's first-lane UV derivatives */
 DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
-                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl) {
+                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch) {
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
     const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
     const V3 pos{interp(b, t.px), interp(b, t.py), interp(b, t.pz)};
@@ -836,6 +836,7 @@ DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, co
     f32 tr, tg, tb;
     if (lod > 1) sample_nearest(P, tex, lod, cu, cv, tr, tg, tb);
     else sample_bilinear(P, tex, lod, cu, cv, tr, tg, tb);
+    if (SFT_INVALID_HANDLE != tex) fetch |= (lod > 1) ? 2u : 1u;
     const f32 alb_r = tr * base_r, alb_g = tg * base_g, alb_b = tb * base_b;
 
     f32 fr, fg, fb;
@@ -852,6 +853,7 @@ DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, co
         const bool lightmapped = mr.lightmapped != 0u;
         if (lightmapped) {
             sample_bilinear(P, mr.lightmap_tex, 0, lu, lv, dr, dg, db);
+            fetch |= 4u;
             dr = dr * fc.lm_brightness;
             dg = dg * fc.lm_brightness;
             db = db * fc.lm_brightness;
@@ -1055,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
     const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
 
     /* ---- shading: a wave covers a 32x2 strip = eight 4x2 blocks ---- */
-    i32 shaded = 0;
+    i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
     const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
     const i32 gw = fc.tile_cols * 64;
     for (int it = 0; it < 16; it += 1) {
@@ -1086,6 +1088,7 @@ __global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
             continue;
         }
         shaded += 1;
+        if (first == l) groups += 1;
         Tri t;
         load_tri(P, vid, t);
         const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
@@ -1103,13 +1106,36 @@ __global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
             duy = u0 - interp(by1, t.u);
             dvy = v0 - interp(by1, t.v);
         }
-        P.color[(u64)gy * fc.width + gx] = shade_pixel(P, fc, est, t, b, dux, duy, dvx, dvy, tl);
+        u32 fetch = 0;
+        P.color[(u64)gy * fc.width + gx] = shade_pixel(P, fc, est, t, b, dux, duy, dvx, dvy, tl, fetch);
+        n_bil += (fetch & 1u) ? 1 : 0;
+        n_near += (fetch & 2u) ? 1 : 0;
+        n_lm += (fetch & 4u) ? 1 : 0;
     }
     shaded = wave_sum(shaded);
-    if (0 == lane && shaded) atomicAdd(&P.counters[CNT_SHADED], (u32)shaded);
+    groups = wave_sum(groups);
+    n_bil = wave_sum(n_bil);
+    n_near = wave_sum(n_near);
+    n_lm = wave_sum(n_lm);
+    if (0 == lane && shaded) {
+        atomicAdd(&P.counters[CNT_SHADED], (u32)shaded);
+        atomicAdd(&P.counters[CNT_QGROUPS], (u32)groups);
+        atomicAdd(&P.counters[CNT_TEX_BILINEAR], (u32)n_bil);
+        atomicAdd(&P.counters[CNT_TEX_NEAREST], (u32)n_near);
+        atomicAdd(&P.counters[CNT_LIGHTMAPPED], (u32)n_lm);
+    }
+}
+
+/* streaming copy used to calibrate HBM byte counters */
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 n) {
+    for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (u64)gridDim.x * 256ull) dst[i] = src[i];
 }
 
 /* ================================================================== host */
+void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n16);
+}
+
 void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev) {
     const u32 nw = fc.n_work;
     const u32 geom_blocks = (nw + 3u) / 4u;
