@@ -74,6 +74,8 @@ typedef struct sftx_frame_stats {
     f32 ms_binning;           /* clip + bin */
     f32 ms_raster_shade;      /* tile raster + resolve */
     f32 ms_download;          /* device->host colour copy */
+    f32 ms_raster;            /* tile raster into the visibility buffer */
+    f32 ms_shade;             /* deferred shading */
     f32 ms_total;             /* whole sft_present on the GPU stream */
 } sftx_frame_stats;
 

@@ -33,7 +33,8 @@ class FrameStats(ctypes.Structure):
         "meshlets_tested", "meshlets_visible", "meshlets_skinned", "batch_payloads", "clip_triangles",
         "tile_payloads", "tiles", "samples", "pixels", "shaded_pixels", "shading_groups", "tex_bilinear",
         "tex_nearest", "lightmapped_pixels")] + [(n, ctypes.c_float) for n in (
-        "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_total")]
+        "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_raster", "ms_shade",
+        "ms_total")]
 
 
 def _bind(lib, name, res, args):

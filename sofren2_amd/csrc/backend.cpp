@@ -65,7 +65,7 @@ struct ModelRes {
 
 struct Backend {
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
 
     /* resident pools */
     DBuf pos, attr, inds, cull, slot_mesh, skin, mesh_recs_d;
@@ -82,7 +82,7 @@ struct Backend {
     /* per frame */
     DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
     DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
-    DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, cap_vid, cap_depth, counters;
+    DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth, counters;
     i32 output_mode = SFTX_OUTPUT_HOST;
     u32 clip_cap = 1u << 16;
     u32 tile_list_cap = 1u << 20;
@@ -116,7 +116,7 @@ void backend_destroy(Backend* b) {
                     &b->bones, &b->plights, &b->dlights, &b->mat_tex, &b->mat_color, &b->mat_shin,
                     &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
                     &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
-                    &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->cap_vid,
+                    &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
                     &b->cap_depth, &b->counters };
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
@@ -418,10 +418,9 @@ void backend_present(Backend* b, Context* cp) {
     dgrow(b->counters, CNT_COUNT * 4, false, s);
     const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
     const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64;
-    if (cap_vis) {
-        dgrow(b->cap_vid, grid * 4, false, s);
-        dgrow(b->cap_depth, grid * 4, false, s);
-    }
+    dgrow(b->vid, grid * 4, false, s);
+    dgrow(b->tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
+    if (cap_vis) dgrow(b->cap_depth, grid * 4, false, s);
     const bool to_host = SFTX_OUTPUT_HOST == b->output_mode && c.color_buffer;
     if (!c.should_clear && to_host && px) {
         HIP_CHECK(hipMemcpy2DAsync(b->color.p, (size_t)c.width * 4, c.color_buffer, (size_t)c.color_pitch * 4,
@@ -480,7 +479,8 @@ void backend_present(Backend* b, Context* cp) {
         p.clip_key = (u32*)b->clip_key.p;
         p.clip_rect = (u32*)b->clip_rect.p;
         p.color = (u32*)b->color.p;
-        p.cap_vid = cap_vis ? (u32*)b->cap_vid.p : nullptr;
+        p.vid = (u32*)b->vid.p;
+        p.tile_lights = (u32*)b->tile_lights.p;
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
         p.counters = (u32*)b->counters.p;
 
@@ -524,6 +524,8 @@ void backend_present(Backend* b, Context* cp) {
     st.ms_geometry = ev_ms(b->ev[1], b->ev[2]);
     st.ms_binning = ev_ms(b->ev[2], b->ev[3]);
     st.ms_raster_shade = ev_ms(b->ev[3], b->ev[4]);
+    st.ms_raster = ev_ms(b->ev[3], b->ev[6]);
+    st.ms_shade = ev_ms(b->ev[6], b->ev[4]);
     st.ms_download = ev_ms(b->ev[4], b->ev[5]);
     st.ms_total = ev_ms(b->ev[0], b->ev[5]);
 
@@ -533,7 +535,7 @@ void backend_present(Backend* b, Context* cp) {
     if (cap_vis) {
         b->h_vid.resize(grid);
         b->h_depth.resize(grid);
-        HIP_CHECK(hipMemcpy(b->h_vid.data(), b->cap_vid.p, grid * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(b->h_vid.data(), b->vid.p, grid * 4, hipMemcpyDeviceToHost));
         HIP_CHECK(hipMemcpy(b->h_depth.data(), b->cap_depth.p, grid * 4, hipMemcpyDeviceToHost));
     }
     if (c.capture_flags & SFTX_CAPTURE_SS_CACHE) {
@@ -617,11 +619,11 @@ extern "C" f32 sftx_copy_calibration(u64 bytes) {
     HIP_CHECK(hipMalloc(&src, bytes));
     HIP_CHECK(hipMalloc(&dst, bytes));
     HIP_CHECK(hipMemsetAsync(src, 1, bytes, b->stream));
-    HIP_CHECK(hipEventRecord(b->ev[6], b->stream));
+    HIP_CHECK(hipEventRecord(b->ev[8], b->stream));
     launch_copy((const uint4*)src, (uint4*)dst, bytes / 16, b->stream);
-    HIP_CHECK(hipEventRecord(b->ev[7], b->stream));
+    HIP_CHECK(hipEventRecord(b->ev[9], b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
-    const f32 ms = ev_ms(b->ev[6], b->ev[7]);
+    const f32 ms = ev_ms(b->ev[8], b->ev[9]);
     HIP_CHECK(hipFree(src));
     HIP_CHECK(hipFree(dst));
     return ms;

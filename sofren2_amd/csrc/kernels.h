@@ -67,8 +67,9 @@ struct FramePtrs {
     u32* clip_key;          /* [clip_cap] */
     u32* clip_rect;         /* [clip_cap] */
     u32* color;             /* [height][width] */
-    u32* cap_vid;           /* [S][grid] when capturing */
-    f32* cap_depth;
+    u32* vid;               /* [grid] reference visibility IDs of the frame */
+    u32* tile_lights;       /* [n_tiles][17]: count, then light indices */
+    f32* cap_depth;         /* [grid] 1/w when capturing, else null */
     u32* counters;          /* [CNT_COUNT] */
 };
 

@@ -966,12 +966,15 @@ DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
+/* one workgroup per 64x64 tile: rasterise the tile's payloads into an LDS
+   visibility buffer, then publish per-pixel reference IDs, the tile's 1/w
+   range and its light list (sofren2.c:6756-6813, 5820-5934) */
+__global__ __launch_bounds__(256) void k_raster(FramePtrs P) {
     const FrameConst& fc = *P.fc;
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     __shared__ f32 s_red[2][4];
     __shared__ u32 s_pass[256];
-    __shared__ TileLights s_tl;
+    __shared__ u32 s_count;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 tile = blockIdx.x;
     const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
@@ -980,7 +983,7 @@ __global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
     for (int i = tid; i < SFT_TILE_PIXELS; i += 256) s_vis[i] = 0ull;
     __syncthreads();
 
-    /* ---- raster: a wave per payload, 8 lanes per triangle ---- */
+    /* a wave per payload, 8 lanes per triangle, lane = x inside the 8-px chunk */
     {
         const u32 off = P.tile_off[tile];
         u32 cnt = P.tile_count[tile];
@@ -1018,106 +1021,135 @@ __global__ __launch_bounds__(256) void k_tile(FramePtrs P) {
     }
     __syncthreads();
 
-    /* ---- tile 1/w range over every sample, screen edge included (5820-5853) ---- */
+    /* publish IDs, and the tile 1/w range over every sample (screen edge included) */
+    const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
+    const i32 gw = fc.tile_cols * 64;
     f32 zmin = 1e9f, zmax = -1e9f;
     for (int i = tid; i < SFT_TILE_PIXELS; i += 256) {
-        const f32 z = u2f((u32)(s_vis[i] >> 32));
+        const u64 pk = s_vis[i];
+        const f32 z = u2f((u32)(pk >> 32));
         if (z > 0.0001f) {
             zmin = fminf(zmin, z);
             zmax = fmaxf(zmax, z);
         }
+        const u64 gi = (u64)(tmy + (i >> 6)) * gw + (tmx + (i & 63));
+        P.vid[gi] = (0ull == pk) ? kInvalid : key_to_vid(P, ~(u32)pk, n_clip);
+        if (P.cap_depth) P.cap_depth[gi] = z;
     }
     for (int o = 32; o > 0; o >>= 1) {
         zmin = fminf(zmin, __shfl_xor(zmin, o));
         zmax = fmaxf(zmax, __shfl_xor(zmax, o));
     }
     if (0 == lane) { s_red[0][wave] = zmin; s_red[1][wave] = zmax; }
-    if (0 == tid) s_tl.count = 0;
+    if (0 == tid) s_count = 0;
     __syncthreads();
     zmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
     zmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
 
-    /* ---- first 16 point lights touching the tile, in index order ---- */
+    /* first 16 point lights touching the tile, in index order */
+    u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
     if (zmin <= zmax && fc.n_point > 0u) {
         for (u32 base = 0; base < fc.n_point; base += 256u) {
             const u32 l = base + (u32)tid;
             s_pass[tid] = (l < fc.n_point) ? (u32)light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy) : 0u;
             __syncthreads();
             if (0 == tid) {
-                for (u32 k = 0; k < 256u && base + k < fc.n_point && s_tl.count < kMaxTileLights; k += 1) {
-                    if (s_pass[k]) s_tl.idx[s_tl.count++] = (u16)(base + k);
+                u32 n = s_count;
+                for (u32 k = 0; k < 256u && base + k < fc.n_point && n < (u32)kMaxTileLights; k += 1) {
+                    if (s_pass[k]) tl[1 + n++] = base + k;
                 }
+                s_count = n;
             }
             __syncthreads();
-            if (s_tl.count >= kMaxTileLights) break;
+            if (s_count >= (u32)kMaxTileLights) break;
         }
+    }
+    if (0 == tid) tl[0] = s_count;
+}
+
+/* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
+   32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
+   order (x offsets 0..3, rows 0..1) */
+__global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    __shared__ TileLights s_tl;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const i32 x0 = (i32)blockIdx.x * 32, y0 = (i32)blockIdx.y * 8;
+    const i32 tile = (y0 >> 6) * fc.tile_cols + (x0 >> 6);
+    const Est est = make_est(P, fc);
+    if (0 == tid) {
+        const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
+        s_tl.count = (int)tl[0];
+        for (int i = 0; i < s_tl.count; i += 1) s_tl.idx[i] = (u16)tl[1 + i];
     }
     __syncthreads();
     const TileLights tl = s_tl;
-    const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
 
-    /* ---- shading: a wave covers a 32x2 strip = eight 4x2 blocks ---- */
-    i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
     const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
+    const i32 bx = x0 + blk * 4, by = y0 + wave * 2;
+    const i32 gx = bx + ix, gy = by + iy;
     const i32 gw = fc.tile_cols * 64;
-    for (int it = 0; it < 16; it += 1) {
-        const int strip = it * 4 + wave;
-        const i32 bx = tmx + (strip & 1) * 32 + blk * 4;
-        const i32 by = tmy + (strip >> 1) * 2;
-        const i32 gx = bx + ix, gy = by + iy;
-        const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
-        const u64 pk = s_vis[(gy - tmy) * 64 + (gx - tmx)];
-        const bool empty = 0ull == pk;
-        u32 vid = kInvalid;
-        if (!empty) vid = key_to_vid(P, ~(u32)pk, n_clip);
-        if (P.cap_vid) {
-            const u64 gi = (u64)gy * gw + gx;
-            P.cap_vid[gi] = vid;
-            P.cap_depth[gi] = u2f((u32)(pk >> 32));
-        }
-        /* first lane of the block holding the same triangle (the LOD lane) */
-        const u32 mine = (in_bounds && !empty) ? vid : kInvalid;
-        int first = l;
-        for (int k = 7; k >= 0; k -= 1) {
-            const u32 other = __shfl(mine, (lane & ~7) | k);
-            if (other == mine) first = k;
-        }
-        if (!in_bounds) continue;
-        if (empty) {
-            if (fc.should_clear) P.color[(u64)gy * fc.width + gx] = fc.clear_color;
-            continue;
-        }
-        shaded += 1;
-        if (first == l) groups += 1;
-        Tri t;
-        load_tri(P, vid, t);
-        const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
-        const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy);
-        f32 dux = 0.f, duy = 0.f, dvx = 0.f, dvy = 0.f;
-        {
-            /* ddx pairs lanes l^1, ddy pairs rows l^4 (sofren2.c:272-273) */
-            const int f = first;
-            const Bary b0 = bary_at(est, t, fbx + (f32)(f & 3), fby + (f32)(f >> 2));
-            const Bary bx1 = bary_at(est, t, fbx + (f32)((f ^ 1) & 3), fby + (f32)((f ^ 1) >> 2));
-            const Bary by1 = bary_at(est, t, fbx + (f32)((f ^ 4) & 3), fby + (f32)((f ^ 4) >> 2));
-            const f32 u0 = interp(b0, t.u), v0 = interp(b0, t.v);
-            dux = u0 - interp(bx1, t.u);
-            dvx = v0 - interp(bx1, t.v);
-            duy = u0 - interp(by1, t.u);
-            dvy = v0 - interp(by1, t.v);
-        }
-        u32 fetch = 0;
-        P.color[(u64)gy * fc.width + gx] = shade_pixel(P, fc, est, t, b, dux, duy, dvx, dvy, tl, fetch);
-        n_bil += (fetch & 1u) ? 1 : 0;
-        n_near += (fetch & 2u) ? 1 : 0;
-        n_lm += (fetch & 4u) ? 1 : 0;
+    const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
+    const u32 vid = P.vid[(u64)gy * gw + gx];
+    const bool valid = in_bounds && kInvalid != vid;
+    const u32 mine = valid ? vid : kInvalid;
+
+    /* first lane of the block holding the same triangle: the LOD lane */
+    int first = l;
+    for (int k = 7; k >= 0; k -= 1) {
+        if (__shfl(mine, (lane & ~7) | k) == mine) first = k;
     }
-    shaded = wave_sum(shaded);
-    groups = wave_sum(groups);
-    n_bil = wave_sum(n_bil);
-    n_near = wave_sum(n_near);
-    n_lm = wave_sum(n_lm);
-    if (0 == lane && shaded) {
+    const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
+    Tri t;
+    Bary b{0.f, 0.f, 0.f};
+    f32 cu = 0.f, cv = 0.f;
+    if (valid) {
+        load_tri(P, vid, t);
+        b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy);
+        cu = interp(b, t.u);
+        cv = interp(b, t.v);
+    }
+    /* UV derivatives at the LOD lane: ddx pairs lanes l^1, ddy rows l^4
+       (sofren2.c:272-273).  A neighbour covered by the same triangle already
+       holds the value; otherwise it is evaluated here under this triangle. */
+    const int src0 = (lane & ~7) | first;
+    const int src1 = (lane & ~7) | (first ^ 1);
+    const int src4 = (lane & ~7) | (first ^ 4);
+    const f32 cu0 = __shfl(cu, src0), cv0 = __shfl(cv, src0);
+    f32 cu1 = __shfl(cu, src1), cv1 = __shfl(cv, src1);
+    f32 cu4 = __shfl(cu, src4), cv4 = __shfl(cv, src4);
+    const u32 vid1 = __shfl(mine, src1), vid4 = __shfl(mine, src4);
+    if (!in_bounds) return;
+    u32* out = &P.color[(u64)gy * fc.width + gx];
+    if (!valid) {
+        if (fc.should_clear) *out = fc.clear_color;
+        return;
+    }
+    if (vid1 != vid) {
+        const int f1 = first ^ 1;
+        const Bary q = bary_at(est, t, fbx + (f32)(f1 & 3), fby + (f32)(f1 >> 2));
+        cu1 = interp(q, t.u);
+        cv1 = interp(q, t.v);
+    }
+    if (vid4 != vid) {
+        const int f4 = first ^ 4;
+        const Bary q = bary_at(est, t, fbx + (f32)(f4 & 3), fby + (f32)(f4 >> 2));
+        cu4 = interp(q, t.u);
+        cv4 = interp(q, t.v);
+    }
+    u32 fetch = 0;
+    *out = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch);
+
+    i32 shaded = 1, groups = (first == l) ? 1 : 0;
+    i32 n_bil = (fetch & 1u) ? 1 : 0, n_near = (fetch & 2u) ? 1 : 0, n_lm = (fetch & 4u) ? 1 : 0;
+    /* counters: one atomic per wave (inactive lanes contribute nothing) */
+    const u64 act = __ballot(true);
+    shaded = __popcll(act);
+    groups = __popcll(__ballot(groups != 0));
+    n_bil = __popcll(__ballot(n_bil != 0));
+    n_near = __popcll(__ballot(n_near != 0));
+    n_lm = __popcll(__ballot(n_lm != 0));
+    if (lane == __ffsll((long long)act) - 1) {
         atomicAdd(&P.counters[CNT_SHADED], (u32)shaded);
         atomicAdd(&P.counters[CNT_QGROUPS], (u32)groups);
         atomicAdd(&P.counters[CNT_TEX_BILINEAR], (u32)n_bil);
@@ -1158,7 +1190,9 @@ void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEv
         hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, p);
     }
     hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_tile, dim3((u32)fc.n_tiles), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles), dim3(256), 0, s, p);
+    hipEventRecord(ev[6], s);
+    hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
     hipEventRecord(ev[4], s);
 }
 
