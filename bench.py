@@ -98,7 +98,7 @@ def main():
     for i in range(args.warmup):
         lib.scene_render(i % 120, color.ctypes.data, w * 4)
     lib.synchronize()
-    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms = [], [], [], [], []
+    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms, rast_ms, shade_ms = [], [], [], [], [], [], []
     bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
     st = S.FrameStats()
     t0 = time.perf_counter()
@@ -111,6 +111,8 @@ def main():
         geom_ms.append(st.ms_geometry)
         bin_ms.append(st.ms_binning)
         total_gpu_ms.append(st.ms_total)
+        rast_ms.append(st.ms_raster)
+        shade_ms.append(st.ms_shade)
         for k, v in algorithmic_bytes(st).items():
             bytes_acc[k] += v
     lib.synchronize()
@@ -140,7 +142,8 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "pcie_inclusive_frames_s": round(host_fps, 3),
         "pass_ms": {"geometry": round(float(np.mean(geom_ms)), 4), "binning": round(float(np.mean(bin_ms)), 4),
-                    "raster_shade": round(tile_avg_ms, 4), "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
+                    "raster_shade": round(tile_avg_ms, 4), "raster": round(float(np.mean(rast_ms)), 4),
+                    "shade": round(float(np.mean(shade_ms)), 4), "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
         "roofline": {"bound": "hbm", "kernel": "k_tile", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "algorithmic_bytes_per_launch": int(tile_bytes)},

@@ -415,7 +415,7 @@ void backend_present(Backend* b, Context* cp) {
     dgrow(b->tile_off, (size_t)nt * 4, false, s);
     dgrow(b->tile_fill, (size_t)nt * 4, false, s);
     dgrow(b->color, std::max<size_t>(px * 4, 4), false, s);
-    dgrow(b->counters, CNT_COUNT * 4, false, s);
+    dgrow(b->counters, COUNTER_WORDS * 4, false, s);
     const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
     const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64;
     dgrow(b->vid, grid * 4, false, s);
@@ -437,7 +437,7 @@ void backend_present(Backend* b, Context* cp) {
         fc.tile_list_cap = b->tile_list_cap;
         dgrow(b->fc_d, sizeof(FrameConst), false, s);
         HIP_CHECK(hipMemcpyAsync(b->fc_d.p, &fc, sizeof(fc), hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemsetAsync(b->counters.p, 0, CNT_COUNT * 4, s));
+        HIP_CHECK(hipMemsetAsync(b->counters.p, 0, COUNTER_WORDS * 4, s));
         HIP_CHECK(hipMemsetAsync(b->tile_count.p, 0, (size_t)nt * 4, s));
         HIP_CHECK(hipMemsetAsync(b->tile_fill.p, 0, (size_t)nt * 4, s));
 

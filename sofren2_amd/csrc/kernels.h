@@ -26,6 +26,13 @@ enum Counter : int {
     CNT_COUNT
 };
 
+/* Statistics are accumulated in CNT_STAT_SLOTS cache-line-separated copies per
+   counter (after the CNT_COUNT narrow words) so that thousands of waves do not
+   serialise on one address; k_stats folds them into the narrow words. */
+constexpr int CNT_STAT_SLOTS = 64;
+constexpr int CNT_STAT_STRIDE = 32;     /* u32 per slot: one 128-byte line */
+constexpr size_t COUNTER_WORDS = CNT_COUNT + (size_t)CNT_COUNT * CNT_STAT_SLOTS * CNT_STAT_STRIDE;
+
 struct FramePtrs {
     /* resident */
     const f32* pos;         /* [slot][3][64] */

@@ -126,6 +126,13 @@ DEV Est make_est(const FramePtrs& P, const FrameConst& fc) {
     return e;
 }
 
+/* adds to a statistics counter through one of its spread slots */
+DEV void stat_add(u32* counters, int cnt, u32 v, u32 slot) {
+    if (v) atomicAdd(&counters[CNT_COUNT + ((size_t)cnt * CNT_STAT_SLOTS + (slot & (CNT_STAT_SLOTS - 1))) * CNT_STAT_STRIDE], v);
+}
+
+DEV u32 stat_slot() { return blockIdx.x * 4u + blockIdx.y * 13u + (threadIdx.x >> 6); }
+
 DEV i32 wave_sum(i32 v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
@@ -280,11 +287,7 @@ __global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
             bx0 = max(bx0, 0); bx1 = min(bx1, max_tx);
             by0 = max(by0, 0); by1 = min(by1, max_ty);
             pl = make_uint2(mask8, (u32)bx0 | ((u32)bx1 << 8) | ((u32)by0 << 16) | ((u32)by1 << 24));
-            const i32 cols = bx1 - bx0 + 1;
-            const i32 nt = cols * (by1 - by0 + 1);
-            for (i32 k = l; k < nt; k += 8) {
-                atomicAdd(&P.tile_count[(by0 + k / cols) * fc.tile_cols + bx0 + k % cols], 1u);
-            }
+            const i32 nt = (bx1 - bx0 + 1) * (by1 - by0 + 1);
             if (0 == l) {
                 n_payloads += 1;
                 n_pairs += nt;
@@ -297,10 +300,11 @@ __global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
     n_pairs = wave_sum(n_pairs);
     if (0 == lane) {
         P.clip_count[wi] = (u32)n_clip;
-        atomicAdd(&P.counters[CNT_VISIBLE], 1u);
-        if (kInvalid != w.skin) atomicAdd(&P.counters[CNT_SKINNED], 1u);
-        if (n_payloads) atomicAdd(&P.counters[CNT_BATCH_PAYLOADS], (u32)n_payloads);
-        if (n_pairs) atomicAdd(&P.counters[CNT_BATCH_TILE_PAIRS], (u32)n_pairs);
+        const u32 sl = stat_slot();
+        stat_add(P.counters, CNT_VISIBLE, 1u, sl);
+        if (kInvalid != w.skin) stat_add(P.counters, CNT_SKINNED, 1u, sl);
+        stat_add(P.counters, CNT_BATCH_PAYLOADS, (u32)n_payloads, sl);
+        stat_add(P.counters, CNT_BATCH_TILE_PAIRS, (u32)n_pairs, sl);
     }
 }
 
@@ -501,11 +505,6 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
                     const i32 cx0 = max(x0, 0), cx1 = min(x1, max_tx);
                     const i32 cy0 = max(y0, 0), cy1 = min(y1, max_ty);
                     rect = (u32)cx0 | ((u32)cx1 << 8) | ((u32)cy0 << 16) | ((u32)cy1 << 24);
-                    for (i32 ty = cy0; ty <= cy1; ty += 1) {
-                        for (i32 tx = cx0; tx <= cx1; tx += 1) {
-                            atomicAdd(&P.tile_count[ty * fc.tile_cols + tx], 1u);
-                        }
-                    }
                     pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
                 }
             }
@@ -513,37 +512,88 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
         }
     }
     pairs = wave_sum(pairs);
-    if (0 == lane && pairs) atomicAdd(&P.counters[CNT_CLIP_TILE_PAIRS], (u32)pairs);
+    if (0 == lane) stat_add(P.counters, CNT_CLIP_TILE_PAIRS, (u32)pairs, stat_slot());
 }
 
-/* ================================================================== k_scatter
-   descriptor: bits 0-31 index (work*16+batch, or clip slot), bits 32-39 lane
-   mask (0 marks a clipped triangle, as in the reference payload) */
-__global__ __launch_bounds__(256) void k_scatter(FramePtrs P) {
+/* ================================================================== binning
+   Payload descriptors: bits 0-31 index (work*16+batch, or clip slot), bits
+   32-39 lane mask (0 marks a clipped triangle, as in the reference payload).
+   Order inside a tile list is free (depth ties are settled by emission keys),
+   so each block aggregates its payloads per tile in LDS and touches the global
+   tile counters once per tile it covers. */
+constexpr u32 kBinChunk = 1024;          /* payload items per block */
+constexpr u32 kBinLdsTiles = 8192;       /* tiles aggregated in LDS (8K frames) */
+
+DEV bool bin_item(const FramePtrs& P, u32 nb, u32 i, u32& rect, u64& desc) {
+    if (i < nb) {
+        const uint2 pl = P.batch_pl[i];
+        if (0u == pl.x) return false;
+        rect = pl.y;
+        desc = ((u64)pl.x << 32) | (u64)i;
+        return true;
+    }
+    const u32 slot = i - nb;
+    rect = P.clip_rect[slot];
+    if (kInvalid == rect) return false;
+    desc = (u64)slot;
+    return true;
+}
+
+/* scatter == false: count payloads per tile; scatter == true: write them */
+template <bool scatter>
+__global__ __launch_bounds__(256) void k_bin(FramePtrs P) {
     const FrameConst& fc = *P.fc;
+    __shared__ u32 s_t[kBinLdsTiles];
     const u32 nb = fc.n_work * 16u;
     const u32 nclip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
     const u32 total = nb + nclip;
-    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const u32 base = blockIdx.x * kBinChunk;
+    if (base >= total) return;
+    const u32 nt = (u32)fc.n_tiles;
+    const bool lds = nt <= kBinLdsTiles;
+    u32* gcount = scatter ? P.tile_fill : P.tile_count;
+    if (lds) {
+        for (u32 t = threadIdx.x; t < nt; t += 256u) s_t[t] = 0u;
+        __syncthreads();
+    }
+    const u32 end = min(base + kBinChunk, total);
+    for (u32 i = base + threadIdx.x; i < end; i += 256u) {
         u32 rect;
         u64 desc;
-        if (i < nb) {
-            const uint2 pl = P.batch_pl[i];
-            if (0u == pl.x) continue;
-            rect = pl.y;
-            desc = ((u64)pl.x << 32) | (u64)i;
-        }
-        else {
-            const u32 slot = i - nb;
-            rect = P.clip_rect[slot];
-            if (kInvalid == rect) continue;
-            desc = (u64)slot;
-        }
+        if (!bin_item(P, nb, i, rect, desc)) continue;
         const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
         for (i32 ty = y0; ty <= y1; ty += 1) {
             for (i32 tx = x0; tx <= x1; tx += 1) {
                 const u32 t = (u32)(ty * fc.tile_cols + tx);
-                const u32 pos = P.tile_off[t] + atomicAdd(&P.tile_fill[t], 1u);
+                if (lds) atomicAdd(&s_t[t], 1u);
+                else if (!scatter) atomicAdd(&gcount[t], 1u);
+                else {
+                    const u32 pos = P.tile_off[t] + atomicAdd(&gcount[t], 1u);
+                    if (pos < fc.tile_list_cap) P.tile_list[pos] = desc;
+                }
+            }
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    /* one global reservation per touched tile */
+    for (u32 t = threadIdx.x; t < nt; t += 256u) {
+        const u32 c = s_t[t];
+        if (c) {
+            const u32 at = atomicAdd(&gcount[t], c);
+            if (scatter) s_t[t] = P.tile_off[t] + at;
+        }
+    }
+    if (!scatter) return;
+    __syncthreads();
+    for (u32 i = base + threadIdx.x; i < end; i += 256u) {
+        u32 rect;
+        u64 desc;
+        if (!bin_item(P, nb, i, rect, desc)) continue;
+        const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
+        for (i32 ty = y0; ty <= y1; ty += 1) {
+            for (i32 tx = x0; tx <= x1; tx += 1) {
+                const u32 pos = atomicAdd(&s_t[(u32)(ty * fc.tile_cols + tx)], 1u);
                 if (pos < fc.tile_list_cap) P.tile_list[pos] = desc;
             }
         }
@@ -1150,11 +1200,22 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     n_near = __popcll(__ballot(n_near != 0));
     n_lm = __popcll(__ballot(n_lm != 0));
     if (lane == __ffsll((long long)act) - 1) {
-        atomicAdd(&P.counters[CNT_SHADED], (u32)shaded);
-        atomicAdd(&P.counters[CNT_QGROUPS], (u32)groups);
-        atomicAdd(&P.counters[CNT_TEX_BILINEAR], (u32)n_bil);
-        atomicAdd(&P.counters[CNT_TEX_NEAREST], (u32)n_near);
-        atomicAdd(&P.counters[CNT_LIGHTMAPPED], (u32)n_lm);
+        const u32 sl = stat_slot();
+        stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
+        stat_add(P.counters, CNT_QGROUPS, (u32)groups, sl);
+        stat_add(P.counters, CNT_TEX_BILINEAR, (u32)n_bil, sl);
+        stat_add(P.counters, CNT_TEX_NEAREST, (u32)n_near, sl);
+        stat_add(P.counters, CNT_LIGHTMAPPED, (u32)n_lm, sl);
+    }
+}
+
+/* folds the spread statistics slots into the narrow counter words */
+__global__ __launch_bounds__(64) void k_stats(u32* counters) {
+    const int lane = threadIdx.x;
+    for (int c = 0; c < CNT_COUNT; c += 1) {
+        u32 v = counters[CNT_COUNT + ((size_t)c * CNT_STAT_SLOTS + lane) * CNT_STAT_STRIDE];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (0 == lane && v) counters[c] += v;
     }
 }
 
@@ -1178,22 +1239,23 @@ void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEv
                            p.counters + CNT_CLIP_TRIS, fc.clip_cap, p.counters + CNT_OVERFLOW, 1u);
     }
     hipEventRecord(ev[2], s);
+    /* upper bound of payload items: batch slots plus every clip slot */
+    const u32 bin_blocks = (nw * 16u + fc.clip_cap + kBinChunk - 1u) / kBinChunk;
     if (nw > 0) {
         hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_bin<false>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.tile_count, p.tile_off, (u32)fc.n_tiles,
                        p.counters + CNT_TILE_PAIRS, fc.tile_list_cap, p.counters + CNT_OVERFLOW, 2u);
     if (nw > 0) {
-        const u32 items = nw * 16u;
-        u32 blocks = (items + 255u) / 256u + 64u;
-        if (blocks > 4096u) blocks = 4096u;
-        hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
     hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles), dim3(256), 0, s, p);
     hipEventRecord(ev[6], s);
     hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
     hipEventRecord(ev[4], s);
+    hipLaunchKernelGGL(k_stats, dim3(1), dim3(64), 0, s, p.counters);
 }
 
 }  // namespace sft
