@@ -55,6 +55,32 @@ static void upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
     if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+/* pinned host staging for the frame's small tables: one H2D copy per frame */
+struct Packer {
+    u8* host = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<std::pair<size_t, size_t>> pending;   /* (offset, bytes) */
+    size_t add(const void* src, size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + std::max<size_t>(bytes, 4);
+        if (used > cap) grow(used);
+        if (bytes) memcpy(host + off, src, bytes);
+        return off;
+    }
+    template <class T> size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
+    void grow(size_t need) {
+        size_t nc = std::max<size_t>(need * 2, 1 << 20);
+        u8* nh = nullptr;
+        HIP_CHECK(hipHostMalloc((void**)&nh, nc, hipHostMallocDefault));
+        if (host) {
+            memcpy(nh, host, cap);
+            HIP_CHECK(hipHostFree(host));
+        }
+        host = nh;
+        cap = nc;
+    }
+};
+
 struct ModelRes {
     u32 base = 0;                   /* first slot */
     u32 total = 0;                  /* total_meshlets */
@@ -84,6 +110,12 @@ struct Backend {
     DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
     DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth, counters;
     i32 output_mode = SFTX_OUTPUT_HOST;
+    Packer pack;                    /* per-frame tables, staged in pinned memory */
+    DBuf pack_d;
+    DBuf zero_d;                    /* counters + tile counts + tile fill cursors */
+    u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
+    u32* color_h = nullptr;         /* pinned staging of the host-output frame */
+    size_t color_h_cap = 0;
     u32 clip_cap = 1u << 16;
     u32 tile_list_cap = 1u << 20;
 
@@ -121,6 +153,12 @@ void backend_destroy(Backend* b) {
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
     }
+    for (DBuf* d : { &b->pack_d, &b->zero_d }) {
+        if (d->p) HIP_CHECK(hipFree(d->p));
+    }
+    if (b->pack.host) HIP_CHECK(hipHostFree(b->pack.host));
+    if (b->counters_h) HIP_CHECK(hipHostFree(b->counters_h));
+    if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
     for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventDestroy(e));
     HIP_CHECK(hipStreamDestroy(b->stream));
     delete b;
@@ -295,7 +333,6 @@ void backend_present(Backend* b, Context* cp) {
         tmodels[m] = std::move(r);
     }
     b->resident_skins = saved_skins;
-    upload(b->mesh_recs_d, b->mesh_recs, s);
     sync_textures(b, c);
 
     /* ---- frame tables ---- */
@@ -390,17 +427,21 @@ void backend_present(Backend* b, Context* cp) {
     fc.n_tmat = (u32)c.tmat_tex.size();
 
     /* ---- uploads ---- */
-    upload(b->insts, insts, s);
-    upload(b->work, work, s);
-    upload(b->bones, bones, s);
-    upload(b->plights, pl, s);
-    upload(b->dlights, dl, s);
-    upload(b->mat_tex, c.mat_tex, s);
-    upload(b->mat_color, c.mat_color, s);
-    upload(b->mat_shin, c.mat_shininess, s);
-    upload(b->tmat_tex, c.tmat_tex, s);
-    upload(b->tmat_color, c.tmat_color, s);
-    upload(b->tmat_shin, c.tmat_shininess, s);
+    Packer& pk = b->pack;
+    pk.used = 0;
+    const size_t o_fc = pk.add(&fc, sizeof(fc));
+    const size_t o_insts = pk.add(insts);
+    const size_t o_work = pk.add(work);
+    const size_t o_bones = pk.add(bones);
+    const size_t o_pl = pk.add(pl);
+    const size_t o_dl = pk.add(dl);
+    const size_t o_mt = pk.add(c.mat_tex);
+    const size_t o_mc = pk.add(c.mat_color);
+    const size_t o_ms = pk.add(c.mat_shininess);
+    const size_t o_tt = pk.add(c.tmat_tex);
+    const size_t o_tc = pk.add(c.tmat_color);
+    const size_t o_ts = pk.add(c.tmat_shininess);
+    const size_t o_mr = pk.add(b->mesh_recs);
 
     const u32 nw = fc.n_work;
     const u32 nt = (u32)fc.n_tiles;
@@ -411,11 +452,11 @@ void backend_present(Backend* b, Context* cp) {
     dgrow(b->clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
     dgrow(b->clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
     dgrow(b->batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
-    dgrow(b->tile_count, (size_t)nt * 4, false, s);
     dgrow(b->tile_off, (size_t)nt * 4, false, s);
-    dgrow(b->tile_fill, (size_t)nt * 4, false, s);
+    const size_t zero_words = COUNTER_WORDS + 2 * (size_t)nt;
+    dgrow(b->zero_d, zero_words * 4, false, s);
+    if (!b->counters_h) HIP_CHECK(hipHostMalloc((void**)&b->counters_h, CNT_COUNT * 4, hipHostMallocDefault));
     dgrow(b->color, std::max<size_t>(px * 4, 4), false, s);
-    dgrow(b->counters, COUNTER_WORDS * 4, false, s);
     const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
     const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64;
     dgrow(b->vid, grid * 4, false, s);
@@ -435,11 +476,12 @@ void backend_present(Backend* b, Context* cp) {
         dgrow(b->clip_rect, (size_t)b->clip_cap * 4, false, s);
         fc.clip_cap = b->clip_cap;
         fc.tile_list_cap = b->tile_list_cap;
-        dgrow(b->fc_d, sizeof(FrameConst), false, s);
-        HIP_CHECK(hipMemcpyAsync(b->fc_d.p, &fc, sizeof(fc), hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemsetAsync(b->counters.p, 0, COUNTER_WORDS * 4, s));
-        HIP_CHECK(hipMemsetAsync(b->tile_count.p, 0, (size_t)nt * 4, s));
-        HIP_CHECK(hipMemsetAsync(b->tile_fill.p, 0, (size_t)nt * 4, s));
+        memcpy(pk.host + o_fc, &fc, sizeof(fc));
+        dgrow(b->pack_d, pk.used, false, s);
+        HIP_CHECK(hipMemcpyAsync(b->pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemsetAsync(b->zero_d.p, 0, zero_words * 4, s));
+        u8* pd = (u8*)b->pack_d.p;
+        u32* zd = (u32*)b->zero_d.p;
 
         FramePtrs p;
         p.pos = (const f32*)b->pos.p;
@@ -448,32 +490,32 @@ void backend_present(Backend* b, Context* cp) {
         p.cull = (const f32*)b->cull.p;
         p.slot_mesh = (const u32*)b->slot_mesh.p;
         p.skin = (const u8*)b->skin.p;
-        p.mesh_recs = (const MeshRec*)b->mesh_recs_d.p;
         p.texels = (const u32*)b->texels.p;
         p.tex_meta = (const TexMeta*)b->tex_meta_d.p;
         p.est_rcp = (const f32*)b->est_rcp.p;
         p.est_rsq = (const f32*)b->est_rsq.p;
-        p.fc = (const FrameConst*)b->fc_d.p;
-        p.insts = (const InstGpu*)b->insts.p;
-        p.work = (const WorkItem*)b->work.p;
-        p.bones = (const f32*)b->bones.p;
-        p.point_lights = (const PointLightGpu*)b->plights.p;
-        p.dir_lights = (const DirLightGpu*)b->dlights.p;
-        p.mat_tex = (const u32*)b->mat_tex.p;
-        p.mat_color = (const u32*)b->mat_color.p;
-        p.mat_shin = (const f32*)b->mat_shin.p;
-        p.tmat_tex = (const u32*)b->tmat_tex.p;
-        p.tmat_color = (const u32*)b->tmat_color.p;
-        p.tmat_shin = (const f32*)b->tmat_shin.p;
+        p.fc = (const FrameConst*)(pd + o_fc);
+        p.insts = (const InstGpu*)(pd + o_insts);
+        p.work = (const WorkItem*)(pd + o_work);
+        p.bones = (const f32*)(pd + o_bones);
+        p.point_lights = (const PointLightGpu*)(pd + o_pl);
+        p.dir_lights = (const DirLightGpu*)(pd + o_dl);
+        p.mat_tex = (const u32*)(pd + o_mt);
+        p.mat_color = (const u32*)(pd + o_mc);
+        p.mat_shin = (const f32*)(pd + o_ms);
+        p.tmat_tex = (const u32*)(pd + o_tt);
+        p.tmat_color = (const u32*)(pd + o_tc);
+        p.tmat_shin = (const f32*)(pd + o_ts);
+        p.mesh_recs = (const MeshRec*)(pd + o_mr);
         p.ss = (f32*)b->ss.p;
         p.ss_vis = (u8*)b->ss_vis.p;
         p.clip_count = (u32*)b->clip_count.p;
         p.clip_base = (u32*)b->clip_base.p;
         p.clip_mask = (u64*)b->clip_mask.p;
         p.batch_pl = (uint2*)b->batch_pl.p;
-        p.tile_count = (u32*)b->tile_count.p;
+        p.tile_count = zd + COUNTER_WORDS;
         p.tile_off = (u32*)b->tile_off.p;
-        p.tile_fill = (u32*)b->tile_fill.p;
+        p.tile_fill = zd + COUNTER_WORDS + nt;
         p.tile_list = (u64*)b->tile_list.p;
         p.clip_rec = (f32*)b->clip_rec.p;
         p.clip_key = (u32*)b->clip_key.p;
@@ -482,12 +524,13 @@ void backend_present(Backend* b, Context* cp) {
         p.vid = (u32*)b->vid.p;
         p.tile_lights = (u32*)b->tile_lights.p;
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
-        p.counters = (u32*)b->counters.p;
+        p.counters = zd;
 
         launch_frame(p, fc, s, b->ev);
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(counters, b->counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(b->counters_h, zd, sizeof(counters), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
+        memcpy(counters, b->counters_h, sizeof(counters));
         if (0u == counters[CNT_OVERFLOW]) break;
         if (attempt > 8) {
             fprintf(stderr, "fatal sofren2 error: frame intermediates keep overflowing\n");
@@ -498,11 +541,20 @@ void backend_present(Backend* b, Context* cp) {
     }
 
     if (to_host && px) {
-        HIP_CHECK(hipMemcpy2DAsync(c.color_buffer, (size_t)c.color_pitch * 4, b->color.p, (size_t)c.width * 4,
-                                   (size_t)c.width * 4, c.height, hipMemcpyDeviceToHost, s));
+        if (b->color_h_cap < px * 4) {
+            if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
+            HIP_CHECK(hipHostMalloc((void**)&b->color_h, px * 4, hipHostMallocDefault));
+            b->color_h_cap = px * 4;
+        }
+        HIP_CHECK(hipMemcpyAsync(b->color_h, b->color.p, px * 4, hipMemcpyDeviceToHost, s));
     }
     HIP_CHECK(hipEventRecord(b->ev[5], s));
     HIP_CHECK(hipStreamSynchronize(s));
+    if (to_host && px) {
+        for (i32 y = 0; y < c.height; y += 1) {
+            memcpy(c.color_buffer + (size_t)y * c.color_pitch, b->color_h + (size_t)y * c.width, (size_t)c.width * 4);
+        }
+    }
 
     sftx_frame_stats& st = c.stats;
     memset(&st, 0, sizeof(st));

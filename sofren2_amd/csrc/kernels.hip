@@ -1019,18 +1019,21 @@ DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f
 /* one workgroup per 64x64 tile: rasterise the tile's payloads into an LDS
    visibility buffer, then publish per-pixel reference IDs, the tile's 1/w
    range and its light list (sofren2.c:6756-6813, 5820-5934) */
-__global__ __launch_bounds__(256) void k_raster(FramePtrs P) {
+constexpr int kRasterThreads = 1024;
+constexpr int kRasterWaves = kRasterThreads / 64;
+
+__global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     const FrameConst& fc = *P.fc;
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
-    __shared__ f32 s_red[2][4];
-    __shared__ u32 s_pass[256];
+    __shared__ f32 s_red[2][kRasterWaves];
+    __shared__ u32 s_pass[kRasterThreads];
     __shared__ u32 s_count;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 tile = blockIdx.x;
     const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
     const Est est = make_est(P, fc);
 
-    for (int i = tid; i < SFT_TILE_PIXELS; i += 256) s_vis[i] = 0ull;
+    for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
     __syncthreads();
 
     /* a wave per payload, 8 lanes per triangle, lane = x inside the 8-px chunk */
@@ -1040,7 +1043,7 @@ __global__ __launch_bounds__(256) void k_raster(FramePtrs P) {
         if (off >= fc.tile_list_cap) cnt = 0;
         else if (off + cnt > fc.tile_list_cap) cnt = fc.tile_list_cap - off;
         const int g = lane >> 3, j = lane & 7;
-        for (u32 e = (u32)wave; e < cnt; e += 4u) {
+        for (u32 e = (u32)wave; e < cnt; e += (u32)kRasterWaves) {
             const u64 desc = P.tile_list[off + e];
             const u32 mask = (u32)(desc >> 32) & 0xFFu;
             const u32 idx = (u32)desc;
@@ -1075,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_raster(FramePtrs P) {
     const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
     const i32 gw = fc.tile_cols * 64;
     f32 zmin = 1e9f, zmax = -1e9f;
-    for (int i = tid; i < SFT_TILE_PIXELS; i += 256) {
+    for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
         const u64 pk = s_vis[i];
         const f32 z = u2f((u32)(pk >> 32));
         if (z > 0.0001f) {
@@ -1093,19 +1096,23 @@ __global__ __launch_bounds__(256) void k_raster(FramePtrs P) {
     if (0 == lane) { s_red[0][wave] = zmin; s_red[1][wave] = zmax; }
     if (0 == tid) s_count = 0;
     __syncthreads();
-    zmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
-    zmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
+    zmin = s_red[0][0];
+    zmax = s_red[1][0];
+    for (int k = 1; k < kRasterWaves; k += 1) {
+        zmin = fminf(zmin, s_red[0][k]);
+        zmax = fmaxf(zmax, s_red[1][k]);
+    }
 
     /* first 16 point lights touching the tile, in index order */
     u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
     if (zmin <= zmax && fc.n_point > 0u) {
-        for (u32 base = 0; base < fc.n_point; base += 256u) {
+        for (u32 base = 0; base < fc.n_point; base += (u32)kRasterThreads) {
             const u32 l = base + (u32)tid;
             s_pass[tid] = (l < fc.n_point) ? (u32)light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy) : 0u;
             __syncthreads();
             if (0 == tid) {
                 u32 n = s_count;
-                for (u32 k = 0; k < 256u && base + k < fc.n_point && n < (u32)kMaxTileLights; k += 1) {
+                for (u32 k = 0; k < (u32)kRasterThreads && base + k < fc.n_point && n < (u32)kMaxTileLights; k += 1) {
                     if (s_pass[k]) tl[1 + n++] = base + k;
                 }
                 s_count = n;
@@ -1211,12 +1218,10 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
 
 /* folds the spread statistics slots into the narrow counter words */
 __global__ __launch_bounds__(64) void k_stats(u32* counters) {
-    const int lane = threadIdx.x;
-    for (int c = 0; c < CNT_COUNT; c += 1) {
-        u32 v = counters[CNT_COUNT + ((size_t)c * CNT_STAT_SLOTS + lane) * CNT_STAT_STRIDE];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (0 == lane && v) counters[c] += v;
-    }
+    const int lane = threadIdx.x, c = blockIdx.x;
+    u32 v = counters[CNT_COUNT + ((size_t)c * CNT_STAT_SLOTS + lane) * CNT_STAT_STRIDE];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (0 == lane && v) counters[c] += v;
 }
 
 /* streaming copy used to calibrate HBM byte counters */
@@ -1251,11 +1256,11 @@ void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEv
         hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
     hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles), dim3(kRasterThreads), 0, s, p);
     hipEventRecord(ev[6], s);
     hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
     hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL(k_stats, dim3(1), dim3(64), 0, s, p.counters);
+    hipLaunchKernelGGL(k_stats, dim3(CNT_COUNT), dim3(64), 0, s, p.counters);
 }
 
 }  // namespace sft
