@@ -9,8 +9,9 @@ throughput; the PCIe-inclusive rate with the drop-in host output is reported
 beside it.
 
 Prints one JSON line: frames/s, Mpix/s, latency percentiles, the roofline of
-the dominant kernel (k_tile: tile raster + resolve) from the algorithmic-byte
-formulas, and the reference renderer timed on this host's cores.
+the dominant kernel (the pass with the longest average time) from the
+algorithmic-byte formulas and the frame's own counters, and the reference
+renderer timed on this host's cores.
 """
 import argparse
 import json
@@ -26,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "1080p frames/s & Mpix/s; p50/p99 request latency, served vs per-frame GPU"
 HBM_PEAK_GBS = 8000.0
+KERNEL_OF_PASS = {"geometry": "k_geom", "binning": "k_clip+k_bin", "visibility": "k_raster", "shading": "k_shade"}
 
 
 def algorithmic_bytes(st):
@@ -101,6 +103,8 @@ def main():
     lat, tile_ms, geom_ms, bin_ms, total_gpu_ms, rast_ms, shade_ms = [], [], [], [], [], [], []
     bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
     st = S.FrameStats()
+    counter_names = [n for n, _ in S.FrameStats._fields_ if not n.startswith("ms_")]
+    counters_sum = {n: 0 for n in counter_names}
     t0 = time.perf_counter()
     for i in range(args.steps):
         a = time.perf_counter()
@@ -115,13 +119,23 @@ def main():
         shade_ms.append(st.ms_shade)
         for k, v in algorithmic_bytes(st).items():
             bytes_acc[k] += v
+        for n in counter_names:
+            counters_sum[n] += getattr(st, n)
     lib.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
 
     tile_avg_ms = float(np.mean(tile_ms))
-    tile_bytes = (bytes_acc["visibility"] + bytes_acc["shading"]) / args.steps
-    achieved = tile_bytes / (tile_avg_ms * 1e-3) / 1e9
+    per_pass = {
+        "geometry": (bytes_acc["geometry"] / args.steps, float(np.mean(geom_ms))),
+        "binning": (bytes_acc["binning"] / args.steps, float(np.mean(bin_ms))),
+        "visibility": (bytes_acc["visibility"] / args.steps, float(np.mean(rast_ms))),
+        "shading": (bytes_acc["shading"] / args.steps, float(np.mean(shade_ms))),
+    }
+    counters_avg = {n: round(v / args.steps, 1) for n, v in counters_sum.items()}
+    dominant = max(per_pass, key=lambda k: per_pass[k][1])
+    dom_bytes, dom_ms = per_pass[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     lat = np.array(lat)
     out = {
         "metric": METRIC,
@@ -144,9 +158,11 @@ def main():
         "pass_ms": {"geometry": round(float(np.mean(geom_ms)), 4), "binning": round(float(np.mean(bin_ms)), 4),
                     "raster_shade": round(tile_avg_ms, 4), "raster": round(float(np.mean(rast_ms)), 4),
                     "shade": round(float(np.mean(shade_ms)), 4), "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_tile", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(tile_bytes)},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PASS[dominant], "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(dom_bytes)},
+        "pass_roofline_gbs": {k: round(v[0] / (v[1] * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
+        "counters_per_frame": counters_avg,
         "cpu_baseline": None,
     }
     lib.scene_destroy()
