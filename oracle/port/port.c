@@ -70,12 +70,22 @@ typedef struct {
 } clip_rec;
 
 typedef struct {
+    sftvec posa, posb;
+    f32 radius;
+    u8 light_vis[64];
+} world_proxy;
+
+typedef struct {
     const sftx_frame_desc* f;
     sftx_port_out* out;
     f32* ss;            /* [total_ss][9][64] */
     bin_t* bins;
     u32 clip_next;
     int clip_overflow;
+    world_proxy* proxies;
+    struct sft_bvh_node* tlas;
+    u32 tlas_count, tlas_root;
+    int invalid_packets;    /* packet frusta the reference would leave uninitialised */
 } port_state;
 
 static void bin_push(bin_t* b, u64 v) {
@@ -611,22 +621,489 @@ static int tile_lights(const sftx_frame_desc* f, f32 zmin, f32 zmax, i32 tmx, i3
     return n;
 }
 
-/* shades one lane of a (block, triangle) group (6190-6620) */
-static u32 shade_lane(port_state* ps, const tri_t* t, f32 px, f32 py, i32 bx, i32 by, f32 dux, f32 duy, f32 dvx, f32 dvy,
-                      const u16* lights, int n_lights) {
+/* ------------------------------------------------------------ TLAS
+   sft__tlas_build and the 8-wide builder, sofren2.c:2885-3062, 3619-3703 */
+typedef struct { u32 ind; struct { sftvec min, max; } aabb; sftvec centroid; } build_item;
+
+static f32 axis_val(const build_item* it, int axis) {
+    return 0 == axis ? it->centroid.x : (1 == axis ? it->centroid.y : it->centroid.z);
+}
+
+/* Hoare-partition quicksort of sofren2.c:2895-2924 (same pivot and swaps) */
+static void sort_items(build_item* items, i32 count, int axis) {
+    if (count <= 1) return;
+    const f32 pivot = axis_val(&items[count / 2], axis);
+    i32 i = 0, j = count - 1;
+    while (i <= j) {
+        while (axis_val(&items[i], axis) < pivot) i += 1;
+        while (axis_val(&items[j], axis) > pivot) j -= 1;
+        if (i <= j) {
+            const build_item t = items[i]; items[i] = items[j]; items[j] = t;
+            i += 1;
+            j -= 1;
+        }
+    }
+    if (j > 0) sort_items(items, j + 1, axis);
+    if (i < count) sort_items(items + i, count - i, axis);
+}
+
+static void partition_to_8(build_item* items, u32 start, u32 count, u32 depth, u32* starts, u32* counts, u32* groups) {
+    if (0 == count) return;
+    if (3 == depth || 1 == count) {
+        starts[*groups] = start;
+        counts[*groups] = count;
+        *groups += 1;
+        return;
+    }
+    f32 mn[3] = { 1e30f, 1e30f, 1e30f }, mx[3] = { -1e30f, -1e30f, -1e30f };
+    for (u32 i = 0; i < count; i += 1) {
+        const f32 c[3] = { items[start + i].centroid.x, items[start + i].centroid.y, items[start + i].centroid.z };
+        for (int k = 0; k < 3; k += 1) {
+            if (c[k] < mn[k]) mn[k] = c[k];
+        }
+        for (int k = 0; k < 3; k += 1) {
+            if (c[k] > mx[k]) mx[k] = c[k];
+        }
+    }
+    const f32 dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    const int axis = (dy > dx && dy > dz) ? 1 : ((dz > dx && dz > dy) ? 2 : 0);
+    sort_items(items + start, (i32)count, axis);
+    const u32 mid = count / 2;
+    partition_to_8(items, start, mid, depth + 1, starts, counts, groups);
+    partition_to_8(items, start + mid, count - mid, depth + 1, starts, counts, groups);
+}
+
+static u32 bvh_build(build_item* items, u32 count, struct sft_bvh_node* nodes, u32* node_count, f32 out[6]) {
+    const u32 ni = (*node_count)++;
+    struct sft_bvh_node* n = &nodes[ni];
+    for (int i = 0; i < 8; i += 1) {
+        n->min_x[i] = 1e30f; n->min_y[i] = 1e30f; n->min_z[i] = 1e30f;
+        n->max_x[i] = -1e30f; n->max_y[i] = -1e30f; n->max_z[i] = -1e30f;
+        n->child_indices[i] = 0xFFFFFFFFu;
+    }
+    f32 b[6] = { 1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f };
+    if (count <= 8) {
+        n->is_leaf_group = 1;
+        n->child_count = (u8)count;
+        for (u32 i = 0; i < count; i += 1) {
+            const build_item* it = &items[i];
+            n->min_x[i] = it->aabb.min.x; n->min_y[i] = it->aabb.min.y; n->min_z[i] = it->aabb.min.z;
+            n->max_x[i] = it->aabb.max.x; n->max_y[i] = it->aabb.max.y; n->max_z[i] = it->aabb.max.z;
+            n->child_indices[i] = 0x80000000u | it->ind;
+            if (it->aabb.min.x < b[0]) b[0] = it->aabb.min.x;
+            if (it->aabb.min.y < b[1]) b[1] = it->aabb.min.y;
+            if (it->aabb.min.z < b[2]) b[2] = it->aabb.min.z;
+            if (it->aabb.max.x > b[3]) b[3] = it->aabb.max.x;
+            if (it->aabb.max.y > b[4]) b[4] = it->aabb.max.y;
+            if (it->aabb.max.z > b[5]) b[5] = it->aabb.max.z;
+        }
+    }
+    else {
+        n->is_leaf_group = 0;
+        u32 starts[8] = {0}, counts[8] = {0}, groups = 0;
+        partition_to_8(items, 0, count, 0, starts, counts, &groups);
+        n->child_count = (u8)groups;
+        for (u32 c = 0; c < groups; c += 1) {
+            f32 cb[6];
+            const u32 child = bvh_build(&items[starts[c]], counts[c], nodes, node_count, cb);
+            n = &nodes[ni];
+            n->child_indices[c] = child;
+            n->min_x[c] = cb[0]; n->min_y[c] = cb[1]; n->min_z[c] = cb[2];
+            n->max_x[c] = cb[3]; n->max_y[c] = cb[4]; n->max_z[c] = cb[5];
+            if (cb[0] < b[0]) b[0] = cb[0];
+            if (cb[1] < b[1]) b[1] = cb[1];
+            if (cb[2] < b[2]) b[2] = cb[2];
+            if (cb[3] > b[3]) b[3] = cb[3];
+            if (cb[4] > b[4]) b[4] = cb[4];
+            if (cb[5] > b[5]) b[5] = cb[5];
+        }
+    }
+    if (out) memcpy(out, b, sizeof(b));
+    return ni;
+}
+
+/* world AABB of each BLAS root under the instance transform; items are indexed
+   by instance while only BLAS-carrying instances are counted (sofren2.c:3630-3702:
+   a BLAS-less instance before a BLAS one leaves a hole the reference fills with
+   scratch memory; the workloads draw BLAS instances first) */
+static void tlas_build(port_state* ps) {
     const sftx_frame_desc* f = ps->f;
-    f32 p0, p1, p2;
-    bary(t, px, py, &p0, &p1, &p2);
-    const f32 cu = interp3(p0, p1, p2, t->u), cv = interp3(p0, p1, p2, t->v);
-    const f32 clu = interp3(p0, p1, p2, t->lu), clv = interp3(p0, p1, p2, t->lv);
+    const u32 n = (u32)f->n_instances;
+    ps->tlas_count = 0;
+    if (0 == n) return;
+    build_item* items = (build_item*)calloc(n, sizeof(build_item));
+    u32 valid = 0;
+    for (u32 i = 0; i < n; i += 1) {
+        const sft_instance* inst = &f->instances[i];
+        const sft_model* m = inst->model;
+        if (0 == m->blas_node_count) continue;
+        const struct sft_bvh_node* r = &m->blas_nodes[m->blas_root_ind];
+        f32 mnx = 1e30f, mny = 1e30f, mnz = 1e30f, mxx = -1e30f, mxy = -1e30f, mxz = -1e30f;
+        for (u32 c = 0; c < r->child_count; c += 1) {
+            if (r->min_x[c] < mnx) mnx = r->min_x[c];
+            if (r->min_y[c] < mny) mny = r->min_y[c];
+            if (r->min_z[c] < mnz) mnz = r->min_z[c];
+            if (r->max_x[c] > mxx) mxx = r->max_x[c];
+            if (r->max_y[c] > mxy) mxy = r->max_y[c];
+            if (r->max_z[c] > mxz) mxz = r->max_z[c];
+        }
+        const sftvec corners[8] = { {mnx, mny, mnz}, {mxx, mny, mnz}, {mnx, mxy, mnz}, {mxx, mxy, mnz},
+                                    {mnx, mny, mxz}, {mxx, mny, mxz}, {mnx, mxy, mxz}, {mxx, mxy, mxz} };
+        sftvec wmn = { 1e30f, 1e30f, 1e30f }, wmx = { -1e30f, -1e30f, -1e30f };
+        const sftmat* T = &inst->transform;
+        for (int k = 0; k < 8; k += 1) {
+            const f32 tx = corners[k].x * T->m[0][0] + corners[k].y * T->m[1][0] + corners[k].z * T->m[2][0] + T->m[3][0];
+            const f32 ty = corners[k].x * T->m[0][1] + corners[k].y * T->m[1][1] + corners[k].z * T->m[2][1] + T->m[3][1];
+            const f32 tz = corners[k].x * T->m[0][2] + corners[k].y * T->m[1][2] + corners[k].z * T->m[2][2] + T->m[3][2];
+            if (tx < wmn.x) wmn.x = tx;
+            if (ty < wmn.y) wmn.y = ty;
+            if (tz < wmn.z) wmn.z = tz;
+            if (tx > wmx.x) wmx.x = tx;
+            if (ty > wmx.y) wmx.y = ty;
+            if (tz > wmx.z) wmx.z = tz;
+        }
+        items[i].ind = i;
+        items[i].aabb.min = wmn;
+        items[i].aabb.max = wmx;
+        items[i].centroid = sft_v_mul(sft_v_add(wmn, wmx), 0.5f);
+        valid += 1;
+    }
+    ps->tlas = (struct sft_bvh_node*)calloc(valid * 2 + 1, sizeof(struct sft_bvh_node));
+    ps->tlas_root = bvh_build(items, valid, ps->tlas, &ps->tlas_count, NULL);
+    free(items);
+}
+
+/* ------------------------------------------------------------ occlusion rays
+   sofren2.c:3840-4243 (packet any-hit, frustum vs node, packet frustum,
+   TLAS/BLAS traversal) and the pass at 8163-8280 */
+typedef struct { f32 n[6][3], an[6][3], d[6]; int valid[6]; } pfrustum;
+
+/* 8 child boxes vs 6 planes: centre/extent form, FMA order of 3933-3938 */
+static int frustum_vs_node(const struct sft_bvh_node* nd, const pfrustum* fr) {
+    int mask = 0xFF;
+    f32 cx[8], cy[8], cz[8], ex[8], ey[8], ez[8];
+    for (int i = 0; i < 8; i += 1) {
+        cx[i] = (nd->max_x[i] + nd->min_x[i]) * 0.5f; ex[i] = (nd->max_x[i] - nd->min_x[i]) * 0.5f;
+        cy[i] = (nd->max_y[i] + nd->min_y[i]) * 0.5f; ey[i] = (nd->max_y[i] - nd->min_y[i]) * 0.5f;
+        cz[i] = (nd->max_z[i] + nd->min_z[i]) * 0.5f; ez[i] = (nd->max_z[i] - nd->min_z[i]) * 0.5f;
+    }
+    for (int p = 0; p < 6; p += 1) {
+        for (int i = 0; i < 8; i += 1) {
+            const f32 r = pfma(fr->an[p][0], ex[i], pfma(fr->an[p][1], ey[i], fr->an[p][2] * ez[i]));
+            const f32 dc = pfma(fr->n[p][0], cx[i], pfma(fr->n[p][1], cy[i], fr->n[p][2] * cz[i]));
+            if (!((dc + fr->d[p]) + r >= 0.f)) mask &= ~(1 << i);
+        }
+        if (0 == mask) return 0;
+    }
+    return mask;
+}
+
+static void build_packet_frustum(port_state* ps, const f32 ox[8], const f32 oy[8], const f32 oz[8], int active,
+                                 sftvec lp, f32 lrad, pfrustum* fr) {
+    f32 mnx = 1e9f, mxx = -1e9f, mny = 1e9f, mxy = -1e9f, mnz = 1e9f, mxz = -1e9f;
+    sftvec c = { 0.f, 0.f, 0.f };
+    int n = 0;
+    for (int i = 0; i < 8; i += 1) {
+        if (!(active & (1 << i))) continue;
+        if (ox[i] < mnx) mnx = ox[i];
+        if (ox[i] > mxx) mxx = ox[i];
+        if (oy[i] < mny) mny = oy[i];
+        if (oy[i] > mxy) mxy = oy[i];
+        if (oz[i] < mnz) mnz = oz[i];
+        if (oz[i] > mxz) mxz = oz[i];
+        c.x += ox[i]; c.y += oy[i]; c.z += oz[i];
+        n += 1;
+    }
+    if (0 == n) return;
+    c.x /= (f32)n; c.y /= (f32)n; c.z /= (f32)n;
+    const sftvec corners[8] = { {mnx, mny, mnz}, {mxx, mny, mnz}, {mnx, mxy, mnz}, {mxx, mxy, mnz},
+                                {mnx, mny, mxz}, {mxx, mny, mxz}, {mnx, mxy, mxz}, {mxx, mxy, mxz} };
+    const sftvec fwd = sft_v_norm(sft_v_sub(c, lp));
+    const sftvec up = (fabsf(fwd.y) > 0.99f) ? SFT_V(1, 0, 0) : SFT_V(0, 1, 0);
+    const sftmat lv = sft_m_view_look_at(lp, c, up);
+    f32 mnxz = 1e9f, mxxz = -1e9f, mnyz = 1e9f, mxyz = -1e9f;
+    int invalid = 0;
+    for (int i = 0; i < 8; i += 1) {
+        const sftquat q = { corners[i].x, corners[i].y, corners[i].z, 1.f };
+        const sftvec cl = sft_m_mul_quat(lv, q);
+        if (cl.z < SFT_EPSILON) { invalid = 1; break; }
+        if (cl.x / cl.z < mnxz) mnxz = cl.x / cl.z;
+        if (cl.x / cl.z > mxxz) mxxz = cl.x / cl.z;
+        if (cl.y / cl.z < mnyz) mnyz = cl.y / cl.z;
+        if (cl.y / cl.z > mxyz) mxyz = cl.y / cl.z;
+    }
+    const sftvec lr = SFT_V(lv.m[0][0], lv.m[1][0], lv.m[2][0]);
+    const sftvec lu = SFT_V(lv.m[0][1], lv.m[1][1], lv.m[2][1]);
+    const sftvec lf = SFT_V(lv.m[0][2], lv.m[1][2], lv.m[2][2]);
+    if (invalid) {
+        /* the reference sets only normal.x here and leaves the rest of the side
+           planes uninitialised; such packets are counted and the workload avoids them */
+        ps->invalid_packets += 1;
+        for (int p = 0; p < 4; p += 1) {
+            fr->n[p][0] = 0.f; fr->n[p][1] = 0.f; fr->n[p][2] = 0.f;
+            fr->an[p][0] = 0.f; fr->an[p][1] = 0.f; fr->an[p][2] = 0.f;
+            fr->d[p] = 0.f;
+        }
+    }
+    else {
+        const sftvec nl[4] = { sft_v_norm(SFT_V(1.f, 0.f, -mnxz)), sft_v_norm(SFT_V(-1.f, 0.f, mxxz)),
+                               sft_v_norm(SFT_V(0.f, 1.f, -mnyz)), sft_v_norm(SFT_V(0.f, -1.f, mxyz)) };
+        for (int p = 0; p < 4; p += 1) {
+            const sftvec nw = sft_v_add(sft_v_mul(lr, nl[p].x), sft_v_add(sft_v_mul(lu, nl[p].y), sft_v_mul(lf, nl[p].z)));
+            fr->n[p][0] = nw.x; fr->n[p][1] = nw.y; fr->n[p][2] = nw.z;
+            fr->d[p] = -sft_v_dot(nw, lp) + SFT_SHADOW_LIGHT_SIZE;
+            fr->an[p][0] = fabsf(nw.x); fr->an[p][1] = fabsf(nw.y); fr->an[p][2] = fabsf(nw.z);
+        }
+    }
+    fr->n[4][0] = lf.x; fr->n[4][1] = lf.y; fr->n[4][2] = lf.z;
+    fr->d[4] = -sft_v_dot(lf, lp) + SFT_SHADOW_LIGHT_SIZE;
+    fr->an[4][0] = fabsf(lf.x); fr->an[4][1] = fabsf(lf.y); fr->an[4][2] = fabsf(lf.z);
+    fr->n[5][0] = -lf.x; fr->n[5][1] = -lf.y; fr->n[5][2] = -lf.z;
+    fr->d[5] = -sft_v_dot(SFT_V(-lf.x, -lf.y, -lf.z), lp) + lrad;
+    fr->an[5][0] = fabsf(lf.x); fr->an[5][1] = fabsf(lf.y); fr->an[5][2] = fabsf(lf.z);
+}
+
+/* two-sided Moller-Trumbore any-hit of 8 rays vs one triangle (3840-3904) */
+static int packet_any_hit(sftvec v0, sftvec v1, sftvec v2, const v3* o, const v3* dir, const f32* tmax, int active) {
+    const sftvec e1s = sft_v_sub(v1, v0), e2s = sft_v_sub(v2, v0);
+    const v3 e1 = { e1s.x, e1s.y, e1s.z }, e2 = { e2s.x, e2s.y, e2s.z }, vv0 = { v0.x, v0.y, v0.z };
+    int hit = 0;
+    for (int l = 0; l < 8; l += 1) {
+        if (!(active & (1 << l))) continue;
+        const v3 d = dir[l];
+        const v3 h = { d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x };
+        const f32 a = vdot(e1, h);
+        if (!(fabsf(a) > SFT_EPSILON)) continue;
+        const f32 fi = rcp_refined(a);
+        const v3 sv = vsub(o[l], vv0);
+        const f32 u = fi * vdot(sv, h);
+        if (!(u >= 0.f && u <= 1.f)) continue;
+        const v3 q = { sv.y * e1.z - sv.z * e1.y, sv.z * e1.x - sv.x * e1.z, sv.x * e1.y - sv.y * e1.x };
+        const f32 v = fi * vdot(d, q);
+        if (!(v >= 0.f && u + v <= 1.f)) continue;
+        const f32 t = fi * vdot(e2, q);
+        if (t > SFT_EPSILON && t < tmax[l]) hit |= 1 << l;
+    }
+    return hit;
+}
+
+/* rays from the 8 origins to origin + L; returns the lanes that hit a static
+   BLAS triangle inside the packet frustum (4086-4243) */
+static int trace_packet(port_state* ps, const f32 ox[8], const f32 oy[8], const f32 oz[8],
+                        const f32 lx[8], const f32 ly[8], const f32 lz[8], int active, sftvec lp, f32 lrad) {
+    const sftx_frame_desc* f = ps->f;
+    if (!active || 0 == ps->tlas_count) return 0;
+    pfrustum fr;
+    build_packet_frustum(ps, ox, oy, oz, active, lp, lrad, &fr);
+    int sat = 0;
+    u32 tstack[64];
+    i32 ttop = 0;
+    tstack[ttop++] = ps->tlas_root;
+    while (ttop > 0) {
+        const struct sft_bvh_node* tn = &ps->tlas[tstack[--ttop]];
+        int tm = frustum_vs_node(tn, &fr);
+        while (tm) {
+            const int tl = __builtin_ctz(tm);
+            tm &= tm - 1;
+            if (0xFFFFFFFFu == tn->child_indices[tl]) continue;
+            if (!tn->is_leaf_group) { tstack[ttop++] = tn->child_indices[tl]; continue; }
+            const sft_instance* inst = &f->instances[tn->child_indices[tl] & ~0x80000000u];
+            const sft_model* model = inst->model;
+            if (model->skeleton.bone_count > 0 || 0 == model->blas_node_count) continue;
+            pfrustum mf;
+            const sftmat* M = &inst->transform;
+            for (int p = 0; p < 6; p += 1) {
+                const f32 nx = fr.n[p][0], ny = fr.n[p][1], nz = fr.n[p][2], d = fr.d[p];
+                const f32 nmx = nx * M->m[0][0] + ny * M->m[0][1] + nz * M->m[0][2];
+                const f32 nmy = nx * M->m[1][0] + ny * M->m[1][1] + nz * M->m[1][2];
+                const f32 nmz = nx * M->m[2][0] + ny * M->m[2][1] + nz * M->m[2][2];
+                const f32 dm = (nx * M->m[3][0] + ny * M->m[3][1] + nz * M->m[3][2]) + d;
+                const f32 il = 1.f / sqrtf(nmx * nmx + nmy * nmy + nmz * nmz);
+                mf.n[p][0] = nmx * il; mf.n[p][1] = nmy * il; mf.n[p][2] = nmz * il;
+                mf.d[p] = dm * il;
+                mf.an[p][0] = fabsf(nmx * il); mf.an[p][1] = fabsf(nmy * il); mf.an[p][2] = fabsf(nmz * il);
+            }
+            const sftmat im = sft_m_qinv(inst->transform);
+            v3 lo[8], ldir[8];
+            f32 ltmax[8];
+            for (int l = 0; l < 8; l += 1) {
+                const v3 o = { ox[l], oy[l], oz[l] };
+                const v3 e = { ox[l] + lx[l], oy[l] + ly[l], oz[l] + lz[l] };
+                lo[l] = xf(o, &im);
+                const v3 le = xf(e, &im);
+                const v3 L = vsub(le, lo[l]);
+                ltmax[l] = sqrtf(vdot(L, L));
+                ldir[l] = norm_fast(L);
+            }
+            u32 bstack[64];
+            i32 btop = 0;
+            bstack[btop++] = model->blas_root_ind;
+            while (btop > 0) {
+                const struct sft_bvh_node* bn = &model->blas_nodes[bstack[--btop]];
+                int bm = frustum_vs_node(bn, &mf);
+                while (bm) {
+                    const int bl = __builtin_ctz(bm);
+                    bm &= bm - 1;
+                    if (0xFFFFFFFFu == bn->child_indices[bl]) continue;
+                    if (!bn->is_leaf_group) { bstack[btop++] = bn->child_indices[bl]; continue; }
+                    const u32 pl = bn->child_indices[bl] & ~0x80000000u;
+                    const u32 m_ind = pl >> 7, tri = pl & 0x7F;
+                    const sft_mesh* mesh = &model->meshes[model->meshlet_mesh_inds[m_ind]];
+                    const u32 lm = m_ind - mesh->model_meshlet_offset;
+                    const struct sft_meshlet* mv = &mesh->meshlets[lm];
+                    const struct sft_meshlet_inds* mi = &mesh->inds[lm];
+                    const u32 off = (tri / 8) * 32 + (tri % 8);
+                    const u8 i0 = mi->inds[off], i1 = mi->inds[off + 8], i2 = mi->inds[off + 16];
+                    const sftvec a0 = SFT_V(mv->x[i0], mv->y[i0], mv->z[i0]);
+                    const sftvec a1 = SFT_V(mv->x[i1], mv->y[i1], mv->z[i1]);
+                    const sftvec a2 = SFT_V(mv->x[i2], mv->y[i2], mv->z[i2]);
+                    sat |= packet_any_hit(a0, a1, a2, lo, ldir, ltmax, active & ~sat);
+                    if (sat == active) return sat;
+                }
+            }
+        }
+    }
+    return sat;
+}
+
+static void occlusion_pass(port_state* ps) {
+    const sftx_frame_desc* f = ps->f;
+    const i32 pc = f->n_proxies;
+    ps->proxies = (world_proxy*)calloc((size_t)(pc > 0 ? pc : 1), sizeof(world_proxy));
+    for (i32 p = 0; p < pc; p += 1) {
+        ps->proxies[p].posa = f->proxies[p].posa;
+        ps->proxies[p].posb = f->proxies[p].posb;
+        ps->proxies[p].radius = f->proxies[p].radius;
+    }
+    if (!(f->shadows_enabled && f->n_point > 0 && pc > 0)) return;
+    const i32 lc = SFT_MIN(f->n_point, 64);
+    for (i32 p = 0; p < pc; p += 1) {
+        for (i32 l = 0; l < lc; l += 1) ps->proxies[p].light_vis[l] = 3;
+    }
+    for (i32 l = 0; l < lc; l += 1) {
+        const sft_point_light* L = &f->point_lights[l];
+        for (i32 base = 0; base < pc; base += 8) {
+            sftvec bc = { 0.f, 0.f, 0.f };
+            i32 act = 0;
+            for (i32 k = 0; k < 8; k += 1) {
+                if (base + k < pc) {
+                    const world_proxy* px = &ps->proxies[base + k];
+                    bc.x += (px->posa.x + px->posb.x) * 0.5f;
+                    bc.y += (px->posa.y + px->posb.y) * 0.5f;
+                    bc.z += (px->posa.z + px->posb.z) * 0.5f;
+                    act += 1;
+                }
+            }
+            if (act > 0) bc = sft_v_mul(bc, 1.f / (f32)act);
+            const sftvec ldir = sft_v_norm(sft_v_sub(L->pos, bc));
+            const sftvec lright = sft_v_norm(sft_v_perpendicular(ldir));
+            const sftvec lup = sft_v_cross(lright, ldir);
+            const f32 r = SFT_SHADOW_LIGHT_SIZE;
+            const sftvec targets[3] = {
+                sft_v_add(L->pos, sft_v_mul(lup, r)),
+                sft_v_add(L->pos, sft_v_add(sft_v_mul(lright, r * 0.866025f), sft_v_mul(lup, -r * 0.5f))),
+                sft_v_add(L->pos, sft_v_add(sft_v_mul(lright, -r * 0.866025f), sft_v_mul(lup, -r * 0.5f))) };
+            for (int ray = 0; ray < 3; ray += 1) {
+                f32 ox[8], oy[8], oz[8], lx[8], ly[8], lz[8];
+                int am = 0;
+                for (i32 k = 0; k < 8; k += 1) {
+                    if (base + k < pc) {
+                        const world_proxy* px = &ps->proxies[base + k];
+                        f32 x, y, z;
+                        if (0 == ray) { x = px->posa.x; y = px->posa.y; z = px->posa.z; }
+                        else if (1 == ray) { x = px->posb.x; y = px->posb.y; z = px->posb.z; }
+                        else { x = (px->posa.x + px->posb.x) * 0.5f; y = (px->posa.y + px->posb.y) * 0.5f; z = (px->posa.z + px->posb.z) * 0.5f; }
+                        ox[k] = x; oy[k] = y; oz[k] = z;
+                        lx[k] = targets[ray].x - x; ly[k] = targets[ray].y - y; lz[k] = targets[ray].z - z;
+                        am |= 1 << k;
+                    }
+                    else {
+                        ox[k] = oy[k] = oz[k] = 0.f;
+                        lx[k] = ly[k] = lz[k] = 0.f;
+                    }
+                }
+                const int occ = trace_packet(ps, ox, oy, oz, lx, ly, lz, am, L->pos, L->radius);
+                for (i32 k = 0; k < 8; k += 1) {
+                    if (occ & (1 << k)) ps->proxies[base + k].light_vis[l] -= 1;
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ shading
+   sofren2.c:6190-6620; the whole 4x2 block is evaluated lane by lane for one
+   triangle, as the reference's 8-wide code does, because the capsule-shadow
+   blend couples the lanes of the block */
+typedef struct {
+    v3 pa, ab, W;
+    f32 abdab, wdw, rb2, pr2, vis_mult;
+} proxy_cache;
+
+typedef struct {
+    int n_lights;
+    u16 lights[16];
+    u8 n_proxies[16];
+    proxy_cache cache[16][16];
+} tile_shading;
+
+/* first 16 proxies per tile light with visibility left, in index order, that
+   reach the light's sphere (5936-5990) */
+static void build_proxy_cache(port_state* ps, tile_shading* ts) {
+    const sftx_frame_desc* f = ps->f;
+    memset(ts->n_proxies, 0, sizeof(ts->n_proxies));
+    if (!(f->shadows_enabled && f->n_proxies > 0 && ts->n_lights > 0)) return;
+    for (int i = 0; i < ts->n_lights; i += 1) {
+        const sft_point_light* l = &f->point_lights[ts->lights[i]];
+        u8 pcnt = 0;
+        for (i32 p = 0; p < f->n_proxies && pcnt < 16; p += 1) {
+            const world_proxy* px = &ps->proxies[p];
+            const u8 vis = px->light_vis[ts->lights[i]];
+            if (0 == vis) continue;
+            const sftvec cc = SFT_V((px->posa.x + px->posb.x) * 0.5f, (px->posa.y + px->posb.y) * 0.5f, (px->posa.z + px->posb.z) * 0.5f);
+            const sftvec ab = sft_v_sub(px->posb, px->posa);
+            const f32 len = sft_v_length(ab);
+            const f32 br = px->radius + (len * 0.5f);
+            const f32 cr = br + l->radius;
+            const sftvec W = sft_v_sub(cc, l->pos);
+            const f32 d2 = sft_v_length2(W);
+            if (d2 <= cr * cr) {
+                proxy_cache* c = &ts->cache[i][pcnt];
+                c->pa.x = px->posa.x; c->pa.y = px->posa.y; c->pa.z = px->posa.z;
+                c->ab.x = ab.x; c->ab.y = ab.y; c->ab.z = ab.z;
+                c->W.x = W.x; c->W.y = W.y; c->W.z = W.z;
+                c->abdab = sft_v_dot(ab, ab);
+                c->wdw = d2;
+                c->rb2 = br * br;
+                c->pr2 = px->radius * px->radius;
+                c->vis_mult = (f32)vis * 0.3333333f;
+                pcnt += 1;
+            }
+        }
+        ts->n_proxies[i] = pcnt;
+    }
+}
+
+static const i32 k_ox[8] = { 0, 1, 2, 3, 0, 1, 2, 3 }, k_oy[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+
+/* shades one (block, triangle) group and stores the lanes in `lanes` */
+static void shade_group(port_state* ps, const tri_t* t, int lanes, i32 x, i32 y, const tile_shading* ts) {
+    const sftx_frame_desc* f = ps->f;
+    f32 cu[8], cv[8], clu[8], clv[8];
+    v3 pos[8], N[8], V[8];
     const f32 ax[3] = { t->wp[0].x, t->wp[1].x, t->wp[2].x }, ay[3] = { t->wp[0].y, t->wp[1].y, t->wp[2].y }, az[3] = { t->wp[0].z, t->wp[1].z, t->wp[2].z };
     const f32 nx[3] = { t->wn[0].x, t->wn[1].x, t->wn[2].x }, ny[3] = { t->wn[0].y, t->wn[1].y, t->wn[2].y }, nz[3] = { t->wn[0].z, t->wn[1].z, t->wn[2].z };
-    const v3 pos = { interp3(p0, p1, p2, ax), interp3(p0, p1, p2, ay), interp3(p0, p1, p2, az) };
-    const v3 nrm = { interp3(p0, p1, p2, nx), interp3(p0, p1, p2, ny), interp3(p0, p1, p2, nz) };
-    const v3 N = norm_fast(nrm);
     const v3 cam = { f->camera_pos.x, f->camera_pos.y, f->camera_pos.z };
-    const v3 V = norm_fast(vsub(cam, pos));
-
+    for (int l = 0; l < 8; l += 1) {
+        f32 p0, p1, p2;
+        bary(t, ((f32)x + 0.5f) + (f32)k_ox[l], ((f32)y + 0.5f) + (f32)k_oy[l], &p0, &p1, &p2);
+        cu[l] = interp3(p0, p1, p2, t->u); cv[l] = interp3(p0, p1, p2, t->v);
+        clu[l] = interp3(p0, p1, p2, t->lu); clv[l] = interp3(p0, p1, p2, t->lv);
+        pos[l].x = interp3(p0, p1, p2, ax); pos[l].y = interp3(p0, p1, p2, ay); pos[l].z = interp3(p0, p1, p2, az);
+        const v3 nr = { interp3(p0, p1, p2, nx), interp3(p0, p1, p2, ny), interp3(p0, p1, p2, nz) };
+        N[l] = norm_fast(nr);
+        V[l] = norm_fast(vsub(cam, pos[l]));
+    }
     const sft_instance* inst = &f->instances[t->inst_ind];
     const sft_mesh* mesh = &inst->model->meshes[inst->model->meshlet_mesh_inds[t->m_ind]];
     const sft_mat_handle mh = mesh->material;
@@ -640,88 +1117,145 @@ static u32 shade_lane(port_state* ps, const tri_t* t, f32 px, f32 py, i32 bx, i3
     }
     if (SFT_DEBUG_CHECKERBOARD == f->debug_mode) { tex = f->base_tex; base = 0xFFFFFFFFu; }
     const f32 br = (f32)((base >> 16) & 0xFF) * (1.f / 255.f), bg = (f32)((base >> 8) & 0xFF) * (1.f / 255.f), bb = (f32)(base & 0xFF) * (1.f / 255.f);
-    const int lod = calc_lod(f, tex, dux, duy, dvx, dvy, bx, by);
-    f32 tr, tg, tb;
-    if (lod > 1) sample_nearest(f, tex, lod, cu, cv, &tr, &tg, &tb);
-    else sample_bilinear(f, tex, lod, cu, cv, &tr, &tg, &tb);
-    const f32 ar = tr * br, ag = tg * bg, ab = tb * bb;
-    f32 fr, fg, fb;
+    const int fl = __builtin_ctz(lanes);
+    const int lod = calc_lod(f, tex, cu[fl] - cu[fl ^ 1], cu[fl] - cu[fl ^ 4], cv[fl] - cv[fl ^ 1], cv[fl] - cv[fl ^ 4], x, y);
+    f32 ar[8], ag[8], ab[8], fr[8], fg[8], fb[8];
+    for (int l = 0; l < 8; l += 1) {
+        f32 tr, tg, tb;
+        if (lod > 1) sample_nearest(f, tex, lod, cu[l], cv[l], &tr, &tg, &tb);
+        else sample_bilinear(f, tex, lod, cu[l], cv[l], &tr, &tg, &tb);
+        ar[l] = tr * br; ag[l] = tg * bg; ab[l] = tb * bb;
+    }
     if (shin < 0.f) {
-        fr = ar * -shin; fg = ag * -shin; fb = ab * -shin;
+        for (int l = 0; l < 8; l += 1) { fr[l] = ar[l] * -shin; fg[l] = ag[l] * -shin; fb[l] = ab[l] * -shin; }
     }
     else {
-        f32 dr, dg, db, sr = 0.f, sg = 0.f, sb = 0.f, lm_shadow = 1.f;
+        f32 dr[8], dg[8], db[8], sr[8] = {0}, sg[8] = {0}, sb[8] = {0}, lms[8];
         const f32 spec_scale = ((shin + 8.f) / 8.f) * 0.04f;
         const int lightmapped = 0xFFFFFFFFu != lm_page;
-        if (lightmapped) {
-            sample_bilinear(f, inst->model->lightmaps[lm_page], 0, clu, clv, &dr, &dg, &db);
-            dr = dr * f->lightmap_brightness; dg = dg * f->lightmap_brightness; db = db * f->lightmap_brightness;
-        }
-        else {
-            dr = f->ambient.x; dg = f->ambient.y; db = f->ambient.z;
-            for (i32 l = 0; l < f->n_dir; l += 1) {
-                const v3 L = { f->dir_lights[l].dir.x, f->dir_lights[l].dir.y, f->dir_lights[l].dir.z };
-                const sftvec c = f->dir_lights[l].color;
-                const f32 ndl = smax(vdot(N, L), 0.f);
-                dr = pfma(ndl, c.x, dr); dg = pfma(ndl, c.y, dg); db = pfma(ndl, c.z, db);
-                if (ndl > 0.f) {
-                    const v3 H = norm_fast(vadd(L, V));
-                    const f32 ndh = smax(vdot(N, H), 0.f);
-                    const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
-                    const f32 term = (spec * spec_scale) * ndl;
-                    sr = pfma(term, c.x, sr); sg = pfma(term, c.y, sg); sb = pfma(term, c.z, sb);
-                }
-            }
-        }
-        for (int i = 0; i < n_lights; i += 1) {
-            const sft_point_light* L = &f->point_lights[lights[i]];
-            const v3 lp = { L->pos.x, L->pos.y, L->pos.z };
-            const v3 lraw = vsub(lp, pos);
-            const f32 d2 = vdot(lraw, lraw);
-            f32 att = smax(0.f, pfma(-d2, L->inv_radius_sqr, 1.f));
-            att = att * att;
-            if (!(att > 0.f)) continue;
-            const v3 Ld = norm_fast(lraw);
-            const f32 ndl = smax(vdot(N, Ld), 0.f);
-            const int lit = ndl > 0.f;
-            const f32 si = smax(0.f, 1.f - 0.f);
+        for (int l = 0; l < 8; l += 1) {
+            lms[l] = 1.f;
             if (lightmapped) {
-                lm_shadow = lm_shadow * (1.f - (1.f - si) * att);
+                sample_bilinear(f, inst->model->lightmaps[lm_page], 0, clu[l], clv[l], &dr[l], &dg[l], &db[l]);
+                dr[l] = dr[l] * f->lightmap_brightness; dg[l] = dg[l] * f->lightmap_brightness; db[l] = db[l] * f->lightmap_brightness;
             }
             else {
-                const f32 di = ndl * att;
-                const f32 mdi = lit ? di * si : 0.f;
-                dr = pfma(mdi, L->color.x, dr); dg = pfma(mdi, L->color.y, dg); db = pfma(mdi, L->color.z, db);
-                if (lit) {
-                    const v3 H = norm_fast(vadd(Ld, V));
-                    const f32 ndh = smax(vdot(N, H), 0.f);
-                    const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
-                    const f32 term = ((spec * spec_scale) * di) * si;
-                    sr = pfma(term, L->color.x, sr); sg = pfma(term, L->color.y, sg); sb = pfma(term, L->color.z, sb);
+                dr[l] = f->ambient.x; dg[l] = f->ambient.y; db[l] = f->ambient.z;
+                for (i32 d = 0; d < f->n_dir; d += 1) {
+                    const v3 L = { f->dir_lights[d].dir.x, f->dir_lights[d].dir.y, f->dir_lights[d].dir.z };
+                    const sftvec c = f->dir_lights[d].color;
+                    const f32 ndl = smax(vdot(N[l], L), 0.f);
+                    dr[l] = pfma(ndl, c.x, dr[l]); dg[l] = pfma(ndl, c.y, dg[l]); db[l] = pfma(ndl, c.z, db[l]);
+                    if (ndl > 0.f) {
+                        const v3 H = norm_fast(vadd(L, V[l]));
+                        const f32 ndh = smax(vdot(N[l], H), 0.f);
+                        const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
+                        const f32 term = (spec * spec_scale) * ndl;
+                        sr[l] = pfma(term, c.x, sr[l]); sg[l] = pfma(term, c.y, sg[l]); sb[l] = pfma(term, c.z, sb[l]);
+                    }
                 }
             }
         }
-        if (lightmapped) {
-            const f32 mod = pfma(lm_shadow, 1.f - 0.4f, 0.4f);
-            dr = dr * mod; dg = dg * mod; db = db * mod;
-            fr = ar * dr; fg = ag * dg; fb = ab * db;
+        for (int i = 0; i < ts->n_lights; i += 1) {
+            const sft_point_light* L = &f->point_lights[ts->lights[i]];
+            const v3 lp = { L->pos.x, L->pos.y, L->pos.z };
+            v3 lraw[8], Ld[8];
+            f32 d2[8], att[8], ndl[8];
+            int any_att = 0, lit = 0;
+            for (int l = 0; l < 8; l += 1) {
+                lraw[l] = vsub(lp, pos[l]);
+                d2[l] = vdot(lraw[l], lraw[l]);
+                f32 a = smax(0.f, pfma(-d2[l], L->inv_radius_sqr, 1.f));
+                att[l] = a * a;
+                if (att[l] > 0.f) any_att = 1;
+            }
+            if (!any_att) continue;
+            for (int l = 0; l < 8; l += 1) {
+                Ld[l] = norm_fast(lraw[l]);
+                ndl[l] = smax(vdot(N[l], Ld[l]), 0.f);
+                if (ndl[l] > 0.f && att[l] > 0.f && (lanes & (1 << l))) lit |= 1 << l;
+            }
+            f32 msh[8] = {0};
+            for (int pi = 0; pi < ts->n_proxies[i]; pi += 1) {
+                const proxy_cache* c = &ts->cache[i][pi];
+                int act = 0;
+                f32 wdv[8];
+                for (int l = 0; l < 8; l += 1) {
+                    wdv[l] = 0.f - vdot(c->W, lraw[l]);
+                    const int between = wdv[l] > 0.f && wdv[l] < d2[l];
+                    const f32 tf = wdv[l] * rcp_refined(d2[l]);
+                    const f32 cd2 = pfma(-wdv[l], tf, c->wdw);
+                    const int sel = lightmapped ? ((lanes >> l) & 1) : ((lit >> l) & 1);
+                    if (between && cd2 < c->rb2 && sel) act |= 1 << l;
+                }
+                if (!act) continue;
+                for (int l = 0; l < 8; l += 1) {
+                    const f32 lrl = sqrtf(d2[l]);
+                    const v3 ao = vsub(pos[l], c->pa);
+                    const f32 abl = vdot(c->ab, Ld[l]), abao = vdot(c->ab, ao), lao = vdot(Ld[l], ao);
+                    const f32 den = pfma(-abl, abl, c->abdab);
+                    const f32 sden = smax(den, SFT_EPSILON);
+                    f32 t2 = pfma(-abl, lao, abao) * rcp_refined(sden);
+                    t2 = smax(0.f, smin(t2, 1.f));
+                    const f32 t1 = pfma(t2, abl, -lao);
+                    const int caster = t1 > SFT_EPSILON && t1 < lrl;
+                    const v3 pr = { pos[l].x + Ld[l].x * t1, pos[l].y + Ld[l].y * t1, pos[l].z + Ld[l].z * t1 };
+                    const v3 pcap = { c->pa.x + c->ab.x * t2, c->pa.y + c->ab.y * t2, c->pa.z + c->ab.z * t2 };
+                    const v3 dd = vsub(pr, pcap);
+                    const f32 dist2 = vdot(dd, dd);
+                    f32 sv = smax(0.f, 1.f - dist2 / c->pr2);
+                    if (!((act >> l) & 1) || !caster) sv = 0.f;
+                    sv = sv * c->vis_mult;
+                    const f32 diff = msh[l] - sv;
+                    const f32 h = smax(0.f, 0.2f - fabsf(diff));
+                    const f32 blend = (h * h) * (0.25f / 0.2f);
+                    msh[l] = smax(msh[l], sv) + blend;
+                }
+            }
+            for (int l = 0; l < 8; l += 1) {
+                const f32 si = smax(0.f, 1.f - msh[l]);
+                if (lightmapped) {
+                    const f32 drop = (1.f - si) * att[l];
+                    lms[l] = lms[l] * (1.f - drop);
+                }
+                else {
+                    const f32 di = ndl[l] * att[l];
+                    const int li = (lit >> l) & 1;
+                    const f32 mdi = li ? di * si : 0.f;
+                    dr[l] = pfma(mdi, L->color.x, dr[l]); dg[l] = pfma(mdi, L->color.y, dg[l]); db[l] = pfma(mdi, L->color.z, db[l]);
+                    if (li) {
+                        const v3 H = norm_fast(vadd(Ld[l], V[l]));
+                        const f32 ndh = smax(vdot(N[l], H), 0.f);
+                        const f32 spec = ndh * rcp_refined((shin - shin * ndh) + ndh);
+                        const f32 term = ((spec * spec_scale) * di) * si;
+                        sr[l] = pfma(term, L->color.x, sr[l]); sg[l] = pfma(term, L->color.y, sg[l]); sb[l] = pfma(term, L->color.z, sb[l]);
+                    }
+                }
+            }
         }
-        else {
-            fr = pfma(ar, dr, sr); fg = pfma(ag, dg, sg); fb = pfma(ab, db, sb);
+        for (int l = 0; l < 8; l += 1) {
+            if (lightmapped) {
+                const f32 mod = pfma(lms[l], 1.f - 0.4f, 0.4f);
+                fr[l] = ar[l] * (dr[l] * mod); fg[l] = ag[l] * (dg[l] * mod); fb[l] = ab[l] * (db[l] * mod);
+            }
+            else {
+                fr[l] = pfma(ar[l], dr[l], sr[l]); fg[l] = pfma(ag[l], dg[l], sg[l]); fb[l] = pfma(ab[l], db[l], sb[l]);
+            }
         }
     }
-    fr = smax(0.f, smin(fr, 1.f)) * 255.f;
-    fg = smax(0.f, smin(fg, 1.f)) * 255.f;
-    fb = smax(0.f, smin(fb, 1.f)) * 255.f;
-    return 0xFF000000u | ((u32)cvtt(fr) << 16) | ((u32)cvtt(fg) << 8) | (u32)cvtt(fb);
+    for (int l = 0; l < 8; l += 1) {
+        if (!(lanes & (1 << l))) continue;
+        const f32 r = smax(0.f, smin(fr[l], 1.f)) * 255.f;
+        const f32 g = smax(0.f, smin(fg[l], 1.f)) * 255.f;
+        const f32 b = smax(0.f, smin(fb[l], 1.f)) * 255.f;
+        ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]] = 0xFF000000u | ((u32)cvtt(r) << 16) | ((u32)cvtt(g) << 8) | (u32)cvtt(b);
+    }
 }
 
 static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 tmx, i32 tmy) {
     const sftx_frame_desc* f = ps->f;
     const i32 sw = (i32)(f->half_width * 2.f), sh = (i32)(f->half_height * 2.f);
     const i32 lx = SFT_MIN(tmx + TS, sw), ly = SFT_MIN(tmy + TS, sh);
-    u32* color = ps->out->color;
-    const i32 pitch = f->width;
     f32 zmin = 1e9f, zmax = -1e9f;
     for (i32 i = 0; i < TP; i += 1) {
         if (depth[i] > 0.0001f) {
@@ -729,23 +1263,22 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
             if (depth[i] > zmax) zmax = depth[i];
         }
     }
-    u16 lights[16];
-    const int n_lights = tile_lights(f, zmin, zmax, tmx, tmy, lights);
-    static const i32 ox[8] = { 0, 1, 2, 3, 0, 1, 2, 3 }, oy[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+    static tile_shading ts;
+    ts.n_lights = tile_lights(f, zmin, zmax, tmx, tmy, ts.lights);
+    build_proxy_cache(ps, &ts);
     for (i32 y = tmy; y < ly; y += 2) {
         for (i32 x = tmx; x < lx; x += 4) {
             u32 vids[8];
             int inb = 0, active = 0;
             for (int l = 0; l < 8; l += 1) {
-                const i32 gx = x + ox[l], gy = y + oy[l];
+                const i32 gx = x + k_ox[l], gy = y + k_oy[l];
                 if (gx < sw && gy < sh) inb |= 1 << l;
                 vids[l] = vid[(gy - tmy) * TS + (gx - tmx)];
                 if ((inb & (1 << l)) && 0xFFFFFFFFu != vids[l]) active |= 1 << l;
-                if (f->should_clear && (inb & (1 << l)) && 0xFFFFFFFFu == vids[l]) color[gy * pitch + gx] = f->clear_color;
+                if (f->should_clear && (inb & (1 << l)) && 0xFFFFFFFFu == vids[l]) ps->out->color[gy * f->width + gx] = f->clear_color;
             }
             while (active) {
-                const int first = __builtin_ctz(active);
-                const u32 target = vids[first];
+                const u32 target = vids[__builtin_ctz(active)];
                 int lanes = 0;
                 for (int l = 0; l < 8; l += 1) {
                     if ((inb & (1 << l)) && vids[l] == target) lanes |= 1 << l;
@@ -754,21 +1287,7 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
                 active &= ~lanes;
                 tri_t t;
                 fetch_tri(ps, target, &t);
-                /* LOD lane: first lane of the (in-bounds) match mask; ddx/ddy pair lanes l^1 and l^4 */
-                const int fl = __builtin_ctz(lanes);
-                f32 q0, q1, q2, cu0, cv0, cux, cvx, cuy, cvy;
-                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl], ((f32)y + 0.5f) + (f32)oy[fl], &q0, &q1, &q2);
-                cu0 = interp3(q0, q1, q2, t.u); cv0 = interp3(q0, q1, q2, t.v);
-                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl ^ 1], ((f32)y + 0.5f) + (f32)oy[fl ^ 1], &q0, &q1, &q2);
-                cux = interp3(q0, q1, q2, t.u); cvx = interp3(q0, q1, q2, t.v);
-                bary(&t, ((f32)x + 0.5f) + (f32)ox[fl ^ 4], ((f32)y + 0.5f) + (f32)oy[fl ^ 4], &q0, &q1, &q2);
-                cuy = interp3(q0, q1, q2, t.u); cvy = interp3(q0, q1, q2, t.v);
-                for (int l = 0; l < 8; l += 1) {
-                    if (!(lanes & (1 << l))) continue;
-                    const f32 px = ((f32)x + 0.5f) + (f32)ox[l], py = ((f32)y + 0.5f) + (f32)oy[l];
-                    color[(y + oy[l]) * pitch + x + ox[l]] = shade_lane(ps, &t, px, py, x, y, cu0 - cux, cu0 - cuy, cv0 - cvx, cv0 - cvy,
-                                                                         lights, n_lights);
-                }
+                shade_group(ps, &t, lanes, x, y, &ts);
             }
         }
     }
@@ -785,10 +1304,16 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
     const i32 nt = f->tile_cols * f->tile_rows;
     ps.bins = (bin_t*)calloc((size_t)(nt > 0 ? nt : 1), sizeof(bin_t));
     memset(out->ss_vis, 0, f->total_ss);
+    tlas_build(&ps);
+    occlusion_pass(&ps);
+    if (out->proxy_vis) {
+        for (i32 p = 0; p < f->n_proxies; p += 1) memcpy(out->proxy_vis + (u64)p * 64, ps.proxies[p].light_vis, 64);
+    }
+    out->invalid_packets = ps.invalid_packets;
     geometry_pass(&ps);
     out->clip_count = ps.clip_next < out->clip_cap ? ps.clip_next : out->clip_cap;
 
-    const i32 gw = f->tile_cols * TS, gh = f->tile_rows * TS;
+    const i32 gw = f->tile_cols * TS;
     f32 depth[TP];
     u32 vid[TP];
     for (i32 t = 0; t < nt; t += 1) {
@@ -808,12 +1333,15 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
         resolve_tile(&ps, depth, vid, tmx, tmy);
         free(ps.bins[t].p);
     }
-    (void)gh;
     free(ps.bins);
+    free(ps.proxies);
+    free(ps.tlas);
     return ps.clip_overflow ? -1 : 0;
 }
 
 u32 sftx_port_ss_count(const sftx_frame_desc* f) { return f->total_ss; }
+
+i32 sftx_port_proxy_count(const sftx_frame_desc* f) { return f->n_proxies; }
 
 void sftx_port_grid(const sftx_frame_desc* f, i32* cols, i32* rows, i32* width, i32* height) {
     *cols = f->tile_cols;

@@ -26,6 +26,8 @@ typedef struct sftx_port_out {
     u32 clip_cap;
     u32 clip_count;     /* set by the port */
     i32 samples;        /* 1, 2 or 4 */
+    u8* proxy_vis;      /* n_proxies * 64 light visibility counts (0..3), or null */
+    i32 invalid_packets;    /* set by the port: occlusion packets with a degenerate frustum */
 } sftx_port_out;
 
 /* Renders the frame.  Returns 0, or -1 if the clip buffer was too small. */
@@ -33,6 +35,7 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out);
 
 /* sizes the caller needs for the output buffers */
 u32 sftx_port_ss_count(const sftx_frame_desc* f);
+i32 sftx_port_proxy_count(const sftx_frame_desc* f);
 void sftx_port_grid(const sftx_frame_desc* f, i32* cols, i32* rows, i32* width, i32* height);
 
 #endif

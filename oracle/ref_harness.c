@@ -65,6 +65,8 @@ static struct {
     u8* ss_vis;
     u32 clip_count;
     u8* clips;          /* clip_count * 164 */
+    u32 proxy_count;
+    u8* proxies;        /* proxy_count * (posa, posb, radius, light_vis[64]) */
 } g_cap;
 
 static void sfth_capture_tile(const float* depth, const unsigned* vid, int tile_min_x, int tile_min_y) {
@@ -127,6 +129,10 @@ static void sfth_present(void) {
     while (used < clip_cap && 0xFFFFFFFFu != sft_state->clip_cache.pool[used].inst_ind) {
         used += 1;
     }
+    g_cap.proxy_count = (u32)sft_state->shadows.proxy_count.count;
+    free(g_cap.proxies);
+    g_cap.proxies = (u8*)malloc((u64)g_cap.proxy_count * sizeof(struct sft_world_shadow_proxy) + 1);
+    memcpy(g_cap.proxies, sft_state->shadows.proxies, (u64)g_cap.proxy_count * sizeof(struct sft_world_shadow_proxy));
     g_cap.clip_count = used;
     free(g_cap.clips);
     g_cap.clips = (u8*)malloc((u64)used * sizeof(struct sft_clipped_triangle) + 1);
@@ -198,6 +204,7 @@ int main(int argc, char** argv) {
         sfth_write(dir, "ss.bin", g_cap.ss, (u64)g_cap.ss_count * sizeof(struct sft_ss_meshlet));
         sfth_write(dir, "ssvis.bin", g_cap.ss_vis, g_cap.ss_count);
         sfth_write(dir, "clip.bin", g_cap.clips, (u64)g_cap.clip_count * sizeof(struct sft_clipped_triangle));
+        sfth_write(dir, "proxies.bin", g_cap.proxies, (u64)g_cap.proxy_count * sizeof(struct sft_world_shadow_proxy));
         char meta[512];
         const int n = snprintf(meta, sizeof(meta),
             "width %d\nheight %d\ncols %d\nrows %d\nsamples %d\nss_count %u\nclip_count %u\nthreads %d\n",

@@ -250,7 +250,7 @@ static void sftx_fwrite(FILE* f, const void* p, u64 n) {
 
 /* writes meshes (one per texture) as QSF2 (layout read by sofren2.c:7816-7967) */
 static void sftx_write_qsf(const char* path, sftx_mlist* meshes, i32 mesh_count, u32** tex, i32 tex_count,
-                           const u32* lightmap) {
+                           const u32* lightmap, const struct sft_bvh_node* nodes, u32 node_count) {
     FILE* f = fopen(path, "wb");
     if (!f) {
         fprintf(stderr, "cannot write %s\n", path);
@@ -274,10 +274,10 @@ static void sftx_write_qsf(const char* path, sftx_mlist* meshes, i32 mesh_count,
     sftx_fwrite(f, lightmap, (u64)ls * ls * 4);
     u32 total = 0;
     for (i32 i = 0; i < mesh_count; i += 1) total += meshes[i].count;
-    const u32 mc = (u32)mesh_count, nodes = 0, root = 0;
+    const u32 mc = (u32)mesh_count, root = 0;
     sftx_fwrite(f, &mc, 4);
     sftx_fwrite(f, &total, 4);
-    sftx_fwrite(f, &nodes, 4);
+    sftx_fwrite(f, &node_count, 4);
     sftx_fwrite(f, &root, 4);
     u32 offset = 0;
     for (i32 i = 0; i < mesh_count; i += 1) {
@@ -310,7 +310,139 @@ static void sftx_write_qsf(const char* path, sftx_mlist* meshes, i32 mesh_count,
         const u16 mi = (u16)i;
         for (u32 m = 0; m < meshes[i].count; m += 1) sftx_fwrite(f, &mi, 2);
     }
+    if (node_count) sftx_fwrite(f, nodes, (u64)node_count * sizeof(struct sft_bvh_node));
     fclose(f);
+}
+
+/* ---- 8-wide BVH over triangles (offline builder of the workload; leaf
+   payload 0x80000000 | meshlet << 7 | triangle, the layout read at
+   sofren2.c:4199-4201) */
+typedef struct { f32 mn[3], mx[3], c[3]; u32 payload; } sftx_bitem;
+
+static int sftx_bitem_axis;
+static int sftx_bitem_cmp(const void* a, const void* b) {
+    const f32 x = ((const sftx_bitem*)a)->c[sftx_bitem_axis], y = ((const sftx_bitem*)b)->c[sftx_bitem_axis];
+    if (x < y) return -1;
+    if (x > y) return 1;
+    return (((const sftx_bitem*)a)->payload < ((const sftx_bitem*)b)->payload) ? -1 : 1;
+}
+
+typedef struct { struct sft_bvh_node* n; u32 count, cap; } sftx_bvh;
+
+static u32 sftx_bvh_alloc(sftx_bvh* b) {
+    if (b->count == b->cap) {
+        b->cap = b->cap ? b->cap * 2 : 1024;
+        b->n = (struct sft_bvh_node*)realloc(b->n, b->cap * sizeof(struct sft_bvh_node));
+    }
+    struct sft_bvh_node* n = &b->n[b->count];
+    memset(n, 0, sizeof(*n));
+    for (int i = 0; i < 8; i += 1) {
+        n->min_x[i] = 1e30f; n->min_y[i] = 1e30f; n->min_z[i] = 1e30f;
+        n->max_x[i] = -1e30f; n->max_y[i] = -1e30f; n->max_z[i] = -1e30f;
+        n->child_indices[i] = 0xFFFFFFFFu;
+    }
+    return b->count++;
+}
+
+/* splits items into up to 8 groups by three levels of median cuts on the
+   longest centroid axis */
+static void sftx_bvh_split(sftx_bitem* it, u32 n, int depth, u32* starts, u32* counts, u32* groups, u32 base) {
+    if (0 == n) return;
+    if (3 == depth || 1 == n) {
+        starts[*groups] = base;
+        counts[*groups] = n;
+        *groups += 1;
+        return;
+    }
+    f32 mn[3] = { 1e30f, 1e30f, 1e30f }, mx[3] = { -1e30f, -1e30f, -1e30f };
+    for (u32 i = 0; i < n; i += 1) {
+        for (int k = 0; k < 3; k += 1) {
+            if (it[i].c[k] < mn[k]) mn[k] = it[i].c[k];
+            if (it[i].c[k] > mx[k]) mx[k] = it[i].c[k];
+        }
+    }
+    int axis = 0;
+    if (mx[1] - mn[1] > mx[axis] - mn[axis]) axis = 1;
+    if (mx[2] - mn[2] > mx[axis] - mn[axis]) axis = 2;
+    sftx_bitem_axis = axis;
+    qsort(it, n, sizeof(sftx_bitem), sftx_bitem_cmp);
+    const u32 h = n / 2;
+    sftx_bvh_split(it, h, depth + 1, starts, counts, groups, base);
+    sftx_bvh_split(it + h, n - h, depth + 1, starts, counts, groups, base + h);
+}
+
+static u32 sftx_bvh_build(sftx_bvh* b, sftx_bitem* it, u32 n, f32 box[6]) {
+    const u32 ni = sftx_bvh_alloc(b);
+    f32 bmn[3] = { 1e30f, 1e30f, 1e30f }, bmx[3] = { -1e30f, -1e30f, -1e30f };
+    if (n <= 8) {
+        b->n[ni].is_leaf_group = 1;
+        b->n[ni].child_count = (u8)n;
+        for (u32 i = 0; i < n; i += 1) {
+            struct sft_bvh_node* nd = &b->n[ni];
+            nd->min_x[i] = it[i].mn[0]; nd->min_y[i] = it[i].mn[1]; nd->min_z[i] = it[i].mn[2];
+            nd->max_x[i] = it[i].mx[0]; nd->max_y[i] = it[i].mx[1]; nd->max_z[i] = it[i].mx[2];
+            nd->child_indices[i] = 0x80000000u | it[i].payload;
+            for (int k = 0; k < 3; k += 1) {
+                if (it[i].mn[k] < bmn[k]) bmn[k] = it[i].mn[k];
+                if (it[i].mx[k] > bmx[k]) bmx[k] = it[i].mx[k];
+            }
+        }
+    }
+    else {
+        u32 starts[8], counts[8], groups = 0;
+        sftx_bvh_split(it, n, 0, starts, counts, &groups, 0);
+        b->n[ni].is_leaf_group = 0;
+        b->n[ni].child_count = (u8)groups;
+        for (u32 g = 0; g < groups; g += 1) {
+            f32 cb[6];
+            const u32 child = sftx_bvh_build(b, it + starts[g], counts[g], cb);
+            struct sft_bvh_node* nd = &b->n[ni];
+            nd->child_indices[g] = child;
+            nd->min_x[g] = cb[0]; nd->min_y[g] = cb[1]; nd->min_z[g] = cb[2];
+            nd->max_x[g] = cb[3]; nd->max_y[g] = cb[4]; nd->max_z[g] = cb[5];
+            for (int k = 0; k < 3; k += 1) {
+                if (cb[k] < bmn[k]) bmn[k] = cb[k];
+                if (cb[3 + k] > bmx[k]) bmx[k] = cb[3 + k];
+            }
+        }
+    }
+    box[0] = bmn[0]; box[1] = bmn[1]; box[2] = bmn[2];
+    box[3] = bmx[0]; box[4] = bmx[1]; box[5] = bmx[2];
+    return ni;
+}
+
+/* BLAS over every triangle of the meshes (model meshlet order = mesh order) */
+static sftx_bvh sftx_build_blas(sftx_mlist* meshes, i32 mesh_count) {
+    u32 n = 0;
+    for (i32 i = 0; i < mesh_count; i += 1) {
+        for (u32 m = 0; m < meshes[i].count; m += 1) n += meshes[i].m[m].tri_count;
+    }
+    sftx_bitem* it = (sftx_bitem*)malloc((u64)n * sizeof(sftx_bitem) + 1);
+    u32 k = 0, gm = 0;
+    for (i32 i = 0; i < mesh_count; i += 1) {
+        for (u32 m = 0; m < meshes[i].count; m += 1, gm += 1) {
+            const sftx_mbuild* b = &meshes[i].m[m];
+            for (u32 t = 0; t < b->tri_count; t += 1, k += 1) {
+                sftx_bitem* e = &it[k];
+                for (int a = 0; a < 3; a += 1) { e->mn[a] = 1e30f; e->mx[a] = -1e30f; }
+                for (int v = 0; v < 3; v += 1) {
+                    const u32 vi = b->inds[t * 3 + v];
+                    const f32 p[3] = { b->x[vi], b->y[vi], b->z[vi] };
+                    for (int a = 0; a < 3; a += 1) {
+                        if (p[a] < e->mn[a]) e->mn[a] = p[a];
+                        if (p[a] > e->mx[a]) e->mx[a] = p[a];
+                    }
+                }
+                for (int a = 0; a < 3; a += 1) e->c[a] = (e->mn[a] + e->mx[a]) * 0.5f;
+                e->payload = (gm << 7) | t;
+            }
+        }
+    }
+    sftx_bvh bvh = { NULL, 0, 0 };
+    f32 box[6];
+    sftx_bvh_build(&bvh, it, n, box);
+    free(it);
+    return bvh;
 }
 
 static void sftx_pad_meshlet(sftx_mbuild* m) {
@@ -329,7 +461,7 @@ static struct {
     sftvec light_col[8];
 } g_room;
 
-static void sftx_box_room_create(void) {
+static void sftx_box_room_create(int with_blas) {
     sftx_rng rng;
     sftx_rng_seed(&rng, g_scene.seed, 0x2545F491ULL);
     u32* tex[SFTX_TEXTURES];
@@ -404,7 +536,10 @@ static void sftx_box_room_create(void) {
     char path[256];
     const char* tmp = getenv("TMPDIR");
     snprintf(path, sizeof(path), "%s/sftx_room_%u_%d.qsf", tmp ? tmp : "/tmp", g_scene.seed, (int)getpid());
-    sftx_write_qsf(path, meshes, SFTX_TEXTURES, tex, SFTX_TEXTURES, lm);
+    sftx_bvh blas = { NULL, 0, 0 };
+    if (with_blas) blas = sftx_build_blas(meshes, SFTX_TEXTURES);
+    sftx_write_qsf(path, meshes, SFTX_TEXTURES, tex, SFTX_TEXTURES, lm, blas.n, blas.count);
+    free(blas.n);
     u32 meshlets = 0;
     for (i32 i = 0; i < SFTX_TEXTURES; i += 1) {
         meshlets += meshes[i].count;
@@ -445,6 +580,247 @@ static void sftx_box_room_render(i32 frame) {
     sft_draw_instance(&g_room.inst);
 }
 
+/* ---------------- workload 2: box room + 32 skinned humanoids + soft shadows
+   Each humanoid is a 96-band, 32-segment tube (6144 triangles, 96 meshlets of
+   64 vertices) around a 10-joint vertical chain.  Per vertex, Dirichlet(0.5)
+   weights over the six joints nearest its height keep their top four,
+   renormalised and quantised as w*255+0.5 (the reference's import rule,
+   sofren2.c:7407-7410).  Joint rotations are sinusoids of the frame with
+   random phase.  Ten capsule proxies per character follow the bones.  The
+   room's QSF2 carries a triangle BLAS for the proxy-occlusion rays; 16
+   shadow-casting point lights hang 15-25 units above the crowd, outside every
+   proxy cluster. */
+#define SFTX_CHARS 32
+#define SFTX_BONES 10
+#define SFTX_BANDS 96
+#define SFTX_SEGS 32
+#define SFTX_BONE_LEN 0.18f
+#define SFTX_SHADOW_LIGHTS 16
+
+/* Gamma(a) for a < 1 via Gamma(a+1) * U^(1/a); Marsaglia-Tsang for a+1 */
+static f32 sftx_gauss(sftx_rng* r) {
+    const f32 u1 = (sftx_u01(r) + 1e-7f), u2 = sftx_u01(r);
+    return sqrtf(-2.f * logf(u1)) * cosf(2.f * (f32)SFT_PI * u2);
+}
+
+static f32 sftx_gamma(sftx_rng* r, f32 a) {
+    const f32 d = (a + 1.f) - 1.f / 3.f, c = 1.f / sqrtf(9.f * d);
+    f32 g;
+    for (;;) {
+        f32 x, v;
+        do {
+            x = sftx_gauss(r);
+            v = 1.f + c * x;
+        } while (v <= 0.f);
+        v = v * v * v;
+        const f32 u = sftx_u01(r) + 1e-7f;
+        if (logf(u) < 0.5f * x * x + d - d * v + d * logf(v)) {
+            g = d * v;
+            break;
+        }
+    }
+    return g * powf(sftx_u01(r) + 1e-7f, 1.f / a);
+}
+
+static struct {
+    sft_model* human;
+    sft_model* person[SFTX_CHARS];     /* shallow copies: own mesh record (material), shared meshlets */
+    sft_mat_handle mat[SFTX_CHARS];
+    sftmat xform[SFTX_CHARS];
+    f32 phase[SFTX_CHARS][SFTX_BONES][2];
+    sftmat posed[SFTX_CHARS][SFTX_BONES];
+    sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
+} g_crowd;
+
+static sft_model* sftx_build_humanoid(sft_arena* ar, sftx_rng* rng) {
+    sft_model* m = (sft_model*)sft_arena_alloc(ar, sizeof(sft_model));
+    memset(m, 0, sizeof(*m));
+    sft_mesh* mesh = (sft_mesh*)sft_arena_alloc(ar, sizeof(sft_mesh));
+    memset(mesh, 0, sizeof(*mesh));
+    const u32 mc = SFTX_BANDS;
+    mesh->meshlets = (struct sft_meshlet*)sft_arena_alloc(ar, mc * sizeof(struct sft_meshlet));
+    mesh->meshlet_attribs = (struct sft_meshlet_attribs*)sft_arena_alloc(ar, mc * sizeof(struct sft_meshlet_attribs));
+    mesh->meshlet_skins = (struct sft_meshlet_skin*)sft_arena_alloc(ar, mc * sizeof(struct sft_meshlet_skin));
+    mesh->inds = (struct sft_meshlet_inds*)sft_arena_alloc(ar, mc * sizeof(struct sft_meshlet_inds));
+    f32** arr[] = { &mesh->bounds_center_x, &mesh->bounds_center_y, &mesh->bounds_center_z, &mesh->bounds_radius,
+                    &mesh->bounds_min_x, &mesh->bounds_max_x, &mesh->bounds_min_y, &mesh->bounds_max_y,
+                    &mesh->bounds_min_z, &mesh->bounds_max_z,
+                    &mesh->cone_axis_x, &mesh->cone_axis_y, &mesh->cone_axis_z, &mesh->cone_cutoff };
+    for (u32 i = 0; i < sizeof(arr) / sizeof(arr[0]); i += 1) *arr[i] = (f32*)sft_arena_alloc(ar, ((mc + 7) & ~7u) * sizeof(f32));
+    mesh->culling_bone_inds = (i32*)sft_arena_alloc(ar, ((mc + 7) & ~7u) * sizeof(i32));
+    mesh->meshlet_count = mc;
+    mesh->model_meshlet_offset = 0;
+    mesh->lightmap_page = 0xFFFFFFFFu;
+    mesh->material = SFT_INVALID_HANDLE;
+    const f32 height = SFTX_BONE_LEN * SFTX_BONES;
+    for (u32 k = 0; k < mc; k += 1) {
+        struct sft_meshlet* mv = &mesh->meshlets[k];
+        struct sft_meshlet_attribs* at = &mesh->meshlet_attribs[k];
+        struct sft_meshlet_skin* sk = &mesh->meshlet_skins[k];
+        memset(at, 0, sizeof(*at));
+        memset(sk, 0, sizeof(*sk));
+        for (u32 ring = 0; ring < 2; ring += 1) {
+            const f32 y = height * (f32)(k + ring) / (f32)SFTX_BANDS;
+            const f32 rad = 0.14f + 0.04f * sinf(y * 5.f);
+            i32 near_j = (i32)(y / SFTX_BONE_LEN + 0.5f);
+            if (near_j > SFTX_BONES - 1) near_j = SFTX_BONES - 1;
+            for (u32 sgi = 0; sgi < SFTX_SEGS; sgi += 1) {
+                const u32 v = ring * SFTX_SEGS + sgi;
+                const f32 th = 2.f * (f32)SFT_PI * (f32)sgi / (f32)SFTX_SEGS;
+                mv->x[v] = rad * cosf(th); mv->y[v] = y; mv->z[v] = rad * sinf(th);
+                at->nx[v] = cosf(th); at->ny[v] = 0.f; at->nz[v] = sinf(th);
+                at->u[v] = (f32)sgi / (f32)SFTX_SEGS; at->v[v] = y / height;
+                /* Dirichlet(0.5) over the six nearest joints, top four kept */
+                i32 cand[6];
+                f32 w[6], sum = 0.f;
+                i32 lo = near_j - 2;
+                if (lo < 0) lo = 0;
+                if (lo > SFTX_BONES - 6) lo = SFTX_BONES - 6;
+                for (int c = 0; c < 6; c += 1) {
+                    cand[c] = lo + c;
+                    w[c] = sftx_gamma(rng, 0.5f);
+                    sum += w[c];
+                }
+                for (int c = 0; c < 6; c += 1) w[c] = w[c] / sum;
+                for (int a = 0; a < 4; a += 1) {
+                    for (int b2 = a + 1; b2 < 6; b2 += 1) {
+                        if (w[b2] > w[a]) {
+                            const f32 tw = w[a]; w[a] = w[b2]; w[b2] = tw;
+                            const i32 tc = cand[a]; cand[a] = cand[b2]; cand[b2] = tc;
+                        }
+                    }
+                }
+                const f32 top = w[0] + w[1] + w[2] + w[3];
+                sk->j0[v] = (u16)cand[0]; sk->w0[v] = (u8)(w[0] / top * 255.f + 0.5f);
+                sk->j1[v] = (u16)cand[1]; sk->w1[v] = (u8)(w[1] / top * 255.f + 0.5f);
+                sk->j2[v] = (u16)cand[2]; sk->w2[v] = (u8)(w[2] / top * 255.f + 0.5f);
+                sk->j3[v] = (u16)cand[3]; sk->w3[v] = (u8)(w[3] / top * 255.f + 0.5f);
+            }
+        }
+        /* band quads, front faces clockwise from outside */
+        struct sft_meshlet_inds* mi = &mesh->inds[k];
+        memset(mi, 0, sizeof(*mi));
+        for (u32 sgi = 0; sgi < SFTX_SEGS; sgi += 1) {
+            const u8 a = (u8)sgi, b2 = (u8)((sgi + 1) % SFTX_SEGS), c = (u8)(SFTX_SEGS + sgi), d = (u8)(SFTX_SEGS + (sgi + 1) % SFTX_SEGS);
+            const u8 tri[2][3] = { { a, c, d }, { a, d, b2 } };
+            for (int h = 0; h < 2; h += 1) {
+                const u32 t = sgi * 2 + h;
+                const u32 off = (t / 8) * 32 + (t % 8);
+                mi->inds[off] = tri[h][0];
+                mi->inds[off + 8] = tri[h][1];
+                mi->inds[off + 16] = tri[h][2];
+            }
+        }
+        const f32 y0 = height * (f32)k / (f32)SFTX_BANDS, y1 = height * (f32)(k + 1) / (f32)SFTX_BANDS;
+        mesh->bounds_min_x[k] = -0.2f; mesh->bounds_max_x[k] = 0.2f;
+        mesh->bounds_min_y[k] = y0; mesh->bounds_max_y[k] = y1;
+        mesh->bounds_min_z[k] = -0.2f; mesh->bounds_max_z[k] = 0.2f;
+        mesh->bounds_center_x[k] = 0.f;
+        mesh->bounds_center_y[k] = (y0 + y1) * 0.5f;
+        mesh->bounds_center_z[k] = 0.f;
+        mesh->bounds_radius[k] = 0.6f;   /* generous: the pose can bend the band away from its cull bone */
+        mesh->cone_axis_x[k] = 0.f; mesh->cone_axis_y[k] = 0.f; mesh->cone_axis_z[k] = 1.f;
+        mesh->cone_cutoff[k] = 2.f;
+        i32 cb = (i32)(((y0 + y1) * 0.5f) / SFTX_BONE_LEN);
+        mesh->culling_bone_inds[k] = cb > SFTX_BONES - 1 ? SFTX_BONES - 1 : cb;
+    }
+    m->meshes = mesh;
+    m->mesh_count = 1;
+    m->total_meshlets = mc;
+    m->meshlet_mesh_inds = (u16*)sft_arena_alloc(ar, mc * sizeof(u16));
+    memset(m->meshlet_mesh_inds, 0, mc * sizeof(u16));
+    m->skeleton.bone_count = SFTX_BONES;
+    m->skeleton.bones = (struct sft_bone*)sft_arena_alloc(ar, SFTX_BONES * sizeof(struct sft_bone));
+    memset(m->skeleton.bones, 0, SFTX_BONES * sizeof(struct sft_bone));
+    m->shadow_proxy_count = SFTX_BONES;
+    m->shadow_proxies = (struct sft_shadow_proxy*)sft_arena_alloc(ar, SFTX_BONES * sizeof(struct sft_shadow_proxy));
+    for (i32 j = 0; j < SFTX_BONES; j += 1) {
+        struct sft_bone* bn = &m->skeleton.bones[j];
+        bn->inverse_bind_matrix = sft_m_translate(0.f, -SFTX_BONE_LEN * (f32)j, 0.f);
+        bn->local_bind_matrix = sft_m_translate(0.f, j ? SFTX_BONE_LEN : 0.f, 0.f);
+        bn->bind_pos = SFT_V(0.f, SFTX_BONE_LEN * (f32)j, 0.f);
+        bn->parent_ind = j - 1;
+        struct sft_shadow_proxy* p = &m->shadow_proxies[j];
+        p->posa = SFT_V(0.f, SFTX_BONE_LEN * (f32)j, 0.f);
+        p->posb = SFT_V(0.f, SFTX_BONE_LEN * (f32)(j + 1), 0.f);
+        p->radius = 0.17f;
+        p->bone_ind = (u32)j;
+    }
+    return m;
+}
+
+static void sftx_crowd_create(void) {
+    sftx_box_room_create(1);
+    sftx_rng rng;
+    sftx_rng_seed(&rng, g_scene.seed, 0x9E3779B9ULL);
+    g_crowd.human = sftx_build_humanoid(&g_room.arena, &rng);
+    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+        /* 8 x 4 grid around the origin, jittered, rotated about y */
+        const f32 x = -6.f + 12.f * (f32)(c % 8) / 7.f + sftx_uniform(&rng, -0.3f, 0.3f);
+        const f32 z = -3.f + 6.f * (f32)(c / 8) / 3.f + sftx_uniform(&rng, -0.3f, 0.3f);
+        g_crowd.xform[c] = sft_m_mul(sft_m_rot_y(sftx_uniform(&rng, 0.f, 6.2831853f)), sft_m_translate(x, -1.f, z));
+        g_crowd.mat[c] = sft_material_create(SFT_INVALID_HANDLE,
+            sftx_pack_rgb(sftx_uniform(&rng, 0.3f, 1.f), sftx_uniform(&rng, 0.3f, 1.f), sftx_uniform(&rng, 0.3f, 1.f)), 24.f);
+        sft_model* pm = (sft_model*)sft_arena_alloc(&g_room.arena, sizeof(sft_model));
+        sft_mesh* pmesh = (sft_mesh*)sft_arena_alloc(&g_room.arena, sizeof(sft_mesh));
+        *pm = *g_crowd.human;
+        *pmesh = g_crowd.human->meshes[0];
+        pmesh->material = g_crowd.mat[c];
+        pm->meshes = pmesh;
+        g_crowd.person[c] = pm;
+        for (i32 j = 0; j < SFTX_BONES; j += 1) {
+            g_crowd.phase[c][j][0] = sftx_uniform(&rng, 0.f, 6.2831853f);
+            g_crowd.phase[c][j][1] = sftx_uniform(&rng, 0.f, 6.2831853f);
+        }
+    }
+    for (i32 l = 0; l < SFTX_SHADOW_LIGHTS; l += 1) {
+        g_crowd.light_pos[l] = SFT_V(sftx_uniform(&rng, -15.f, 15.f), sftx_uniform(&rng, 15.f, 25.f), sftx_uniform(&rng, -15.f, 15.f));
+        g_crowd.light_col[l] = SFT_V(sftx_uniform(&rng, 0.4f, 1.f), sftx_uniform(&rng, 0.4f, 1.f), sftx_uniform(&rng, 0.4f, 1.f));
+    }
+    sft_set_shadows_enabled(1);
+    g_scene.tri_count += SFTX_CHARS * SFTX_BANDS * SFTX_SEGS * 2;
+    g_scene.meshlet_count += SFTX_CHARS * SFTX_BANDS;
+}
+
+/* posed = inverse_bind * global, global_j = local_j * global_(j-1)
+   (row vectors, sofren2.c:9102-9104) */
+static void sftx_crowd_pose(i32 frame) {
+    const f32 t = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
+    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+        sftmat global = sft_m_identity();
+        for (i32 j = 0; j < SFTX_BONES; j += 1) {
+            const f32 a = 0.35f * sinf(t * 2.f + g_crowd.phase[c][j][0]);
+            const f32 b = 0.25f * sinf(t * 3.f + g_crowd.phase[c][j][1]);
+            const sftmat local = sft_m_mul(sft_m_mul(sft_m_rot_z(a), sft_m_rot_x(b)), sft_m_translate(0.f, j ? SFTX_BONE_LEN : 0.f, 0.f));
+            global = sft_m_mul(local, global);
+            g_crowd.posed[c][j] = sft_m_mul(g_crowd.human->skeleton.bones[j].inverse_bind_matrix, global);
+        }
+    }
+}
+
+static void sftx_crowd_render(i32 frame) {
+    const f32 th = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
+    const sftvec cam = SFT_V(25.f * cosf(th), 3.f + 4.f * sinf(2.f * th), 25.f * sinf(th));
+    const sftvec target = SFT_V(5.f * sinf(th), 0.f, -5.f * cosf(th));
+    sft_set_camera(cam, sft_m_view_look_at(cam, target, SFT_V(0.f, 1.f, 0.f)),
+                   sft_m_proj(70.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    sft_set_ambient_light(0.3f, 0.3f, 0.3f);
+    for (i32 l = 0; l < SFTX_SHADOW_LIGHTS; l += 1) {
+        sft_draw_point_light(g_crowd.light_pos[l], g_crowd.light_col[l].x, g_crowd.light_col[l].y, g_crowd.light_col[l].z,
+                             1.f, 40.f, (u32)(l + 1));
+    }
+    sft_draw_instance(&g_room.inst);
+    sftx_crowd_pose(frame);
+    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+        sft_instance inst;
+        memset(&inst, 0, sizeof(inst));
+        inst.model = g_crowd.person[c];
+        inst.transform = g_crowd.xform[c];
+        inst.posed_bone_matrices = g_crowd.posed[c];
+        sft_draw_instance(&inst);
+    }
+}
+
 /* ---------------- dispatch ---------------- */
 i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
     memset(&g_scene, 0, sizeof(g_scene));
@@ -455,7 +831,8 @@ i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
     sft_set_clear(1);
     switch (kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_create(); return 0;
-        case SFTX_SCENE_BOX_ROOM: sftx_box_room_create(); return 0;
+        case SFTX_SCENE_BOX_ROOM: sftx_box_room_create(0); return 0;
+        case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_create(); return 0;
         default: return -1;
     }
 }
@@ -466,6 +843,7 @@ void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes) {
     switch (g_scene.kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_render(frame); break;
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_render(frame); break;
+        case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_render(frame); break;
         default: break;
     }
 }

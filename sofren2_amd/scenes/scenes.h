@@ -17,6 +17,10 @@
  *       chains, a baked lightmap page (per-vertex light U[0.2,1]), 8 point
  *       lights without shadows, a 120-frame camera path.  The scene is written
  *       to a QSF2 file and loaded through sft_model_load_qsf.
+ *   SFTX_SCENE_CROWD_SHADOWS (2): the box room (with a triangle BLAS) plus 32
+ *       skinned humanoids (6144 triangles, 10-joint chain, 4 weights per
+ *       vertex), 10 capsule shadow proxies each, 16 shadow-casting point
+ *       lights, shadows enabled.
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H
@@ -30,6 +34,7 @@ extern "C" {
 enum sftx_scene_kind {
     SFTX_SCENE_PLANE_SPHERES = 0,
     SFTX_SCENE_BOX_ROOM = 1,
+    SFTX_SCENE_CROWD_SHADOWS = 2,
     SFTX_SCENE_COUNT
 };
 

@@ -15,7 +15,8 @@ DESC_BYTES = 4096  # upper bound of sizeof(sftx_frame_desc)
 class PortOut(ctypes.Structure):
     _fields_ = [("color", ctypes.c_void_p), ("vid", ctypes.c_void_p), ("depth", ctypes.c_void_p),
                 ("ss", ctypes.c_void_p), ("ss_vis", ctypes.c_void_p), ("clips", ctypes.c_void_p),
-                ("clip_cap", ctypes.c_uint32), ("clip_count", ctypes.c_uint32), ("samples", ctypes.c_int32)]
+                ("clip_cap", ctypes.c_uint32), ("clip_count", ctypes.c_uint32), ("samples", ctypes.c_int32),
+                ("proxy_vis", ctypes.c_void_p), ("invalid_packets", ctypes.c_int32)]
 
 
 _PORT = None
@@ -31,6 +32,8 @@ def port_lib():
         lib.sftx_port_render.argtypes = [ctypes.c_void_p, ctypes.POINTER(PortOut)]
         lib.sftx_port_ss_count.restype = ctypes.c_uint32
         lib.sftx_port_ss_count.argtypes = [ctypes.c_void_p]
+        lib.sftx_port_proxy_count.restype = ctypes.c_int32
+        lib.sftx_port_proxy_count.argtypes = [ctypes.c_void_p]
         lib.sftx_port_grid.restype = None
         lib.sftx_port_grid.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
         _PORT = lib
@@ -43,6 +46,7 @@ def port_render_desc(desc, clip_cap=1 << 16):
     cols, rows, w, h = (ctypes.c_int32() for _ in range(4))
     lib.sftx_port_grid(desc, ctypes.byref(cols), ctypes.byref(rows), ctypes.byref(w), ctypes.byref(h))
     n_ss = lib.sftx_port_ss_count(desc)
+    n_proxies = lib.sftx_port_proxy_count(desc)
     grid = cols.value * 64 * rows.value * 64
     color = np.zeros((h.value, w.value), dtype=np.uint32)
     vid = np.zeros(grid, dtype=np.uint32)
@@ -50,15 +54,17 @@ def port_render_desc(desc, clip_cap=1 << 16):
     ss = np.zeros(max(n_ss, 1) * S.SS_MESHLET_BYTES, dtype=np.uint8)
     ssvis = np.zeros(max(n_ss, 1), dtype=np.uint8)
     clips = np.zeros(clip_cap * S.CLIP_RECORD_BYTES, dtype=np.uint8)
+    proxy_vis = np.zeros((max(n_proxies, 1), 64), dtype=np.uint8)
     out = PortOut(color.ctypes.data, vid.ctypes.data, depth.ctypes.data, ss.ctypes.data, ssvis.ctypes.data,
-                  clips.ctypes.data, clip_cap, 0, 1)
+                  clips.ctypes.data, clip_cap, 0, 1, proxy_vis.ctypes.data, 0)
     rc = lib.sftx_port_render(desc, ctypes.byref(out))
     if rc != 0:
         raise RuntimeError("clip buffer too small")
     n_clip = out.clip_count
     return {"color": color, "vid": vid, "depth": depth,
             "ss": ss[:n_ss * S.SS_MESHLET_BYTES].view(np.uint32).reshape(n_ss, 9, 64), "ssvis": ssvis[:n_ss],
-            "clip": clips[:n_clip * S.CLIP_RECORD_BYTES].view(np.uint32).reshape(n_clip, 41)}
+            "clip": clips[:n_clip * S.CLIP_RECORD_BYTES].view(np.uint32).reshape(n_clip, 41),
+            "proxy_vis": proxy_vis[:n_proxies], "invalid_packets": out.invalid_packets}
 
 
 def port_render_scene(scene, seed, width, height, frame=0):

@@ -43,6 +43,10 @@ def ref_dump(scene, seed, width, height, frame=0, samples=1, threads=1):
         out["ss"] = np.fromfile(os.path.join(d, "ss.bin"), dtype=np.uint32).reshape(-1, 9, 64)
         out["ssvis"] = np.fromfile(os.path.join(d, "ssvis.bin"), dtype=np.uint8)
         out["clip"] = np.fromfile(os.path.join(d, "clip.bin"), dtype=np.uint32).reshape(-1, 41)
+        # world shadow proxies: posa, posb (3 f32 each), radius, light_vis[64] (92 bytes)
+        px = np.fromfile(os.path.join(d, "proxies.bin"), dtype=np.uint8).reshape(-1, 92)
+        out["proxy_vis"] = px[:, 28:92].copy()
+        out["proxy_geom"] = px[:, :28].copy().view(np.float32)
         return out
 
 

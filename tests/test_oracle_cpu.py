@@ -46,3 +46,21 @@ def test_port_matches_reference_box_room(frame):
     print(rep, "clips", rep["clip_count"])
     parity.assert_parity(rep)
     assert rep["color_exact_frac"] == 1.0, rep
+
+
+@pytest.mark.parametrize("frame", [0, 50])
+def test_port_matches_reference_crowd_shadows(frame):
+    """Skinned crowd with capsule shadows: skinned screen-space cache, per-proxy
+    light visibility from the occlusion packets, and the shadowed colour."""
+    w, h = S.SCENE_SIZE[S.SCENE_CROWD_SHADOWS]
+    port = port_render_scene(S.SCENE_CROWD_SHADOWS, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_CROWD_SHADOWS, 0, w, h, frame)
+    assert port["invalid_packets"] == 0
+    assert ref["proxy_vis"].shape == port["proxy_vis"].shape
+    np.testing.assert_array_equal(port["proxy_vis"], ref["proxy_vis"])
+    vis = ref["proxy_vis"][:, :8]
+    assert (vis == 3).any() and (vis < 3).any(), "occlusion pass must see both lit and occluded proxies"
+    rep = parity.compare_frame(port, ref)
+    print(rep, "clips", rep["clip_count"])
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
