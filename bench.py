@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "1080p frames/s & Mpix/s; p50/p99 request latency, served vs per-frame GPU"
 HBM_PEAK_GBS = 8000.0
+SCENES = {"plane_spheres": 0, "box_room": 1, "crowd_shadows": 2}
 KERNEL_OF_PASS = {"geometry": "k_geom", "binning": "k_clip+k_bin", "visibility": "k_raster", "shading": "k_shade"}
 
 
@@ -43,20 +44,20 @@ def algorithmic_bytes(st):
     return {"geometry": geometry, "binning": binning, "visibility": visibility, "shading": shading}
 
 
-def cpu_baseline(width, height, frames):
+def cpu_baseline(scene, name, width, height, frames):
     exe = os.path.join(ROOT, "oracle", "_ref", "sofren2_ref")
     if not os.path.exists(exe):
         return None
     cores = os.cpu_count() or 1
     try:
-        proc = subprocess.run([exe, "bench", "1", "0", str(width), str(height), str(cores), "5", str(frames)],
+        proc = subprocess.run([exe, "bench", str(scene), "0", str(width), str(height), str(cores), "5", str(frames)],
                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600, check=True)
         r = json.loads(proc.stdout.strip().splitlines()[-1])
     except (subprocess.SubprocessError, ValueError) as e:
         print(f"cpu baseline failed: {e}", file=sys.stderr)
         return None
     return {"value": round(r["fps"], 3), "unit": "frames/s", "cores": r["threads"], "kind": "reference",
-            "sample": f"box-room 1920x1080, {frames} frames of the camera path after 5 warm-up frames, "
+            "sample": f"{name} {width}x{height}, {frames} frames of the camera path after 5 warm-up frames, "
                       f"reference AVX2 build with {r['threads']} worker threads (logical CPUs); "
                       f"median {r['ms_median']:.2f} ms, p99 {r['ms_p99']:.2f} ms"}
 
@@ -68,6 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=120)
+    ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
+                    help="workload; box_room is the headline 1080p textured interior")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
@@ -76,9 +79,10 @@ def main():
 
     import sofren2_amd as S
     lib = S.library()
-    w, h = S.SCENE_SIZE[S.SCENE_BOX_ROOM]
+    scene = SCENES[args.scene]
+    w, h = S.SCENE_SIZE[scene]
     lib.sft_init(w, h, 0)
-    if lib.scene_create(S.SCENE_BOX_ROOM, 0, w, h) != 0:
+    if lib.scene_create(scene, 0, w, h) != 0:
         raise SystemExit("scene creation failed")
     tris = lib.scene_tris()
     color = np.zeros((h, w), dtype=np.uint32)
@@ -100,7 +104,7 @@ def main():
     for i in range(args.warmup):
         lib.scene_render(i % 120, color.ctypes.data, w * 4)
     lib.synchronize()
-    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms, rast_ms, shade_ms = [], [], [], [], [], [], []
+    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms, rast_ms, shade_ms, shadow_ms = [], [], [], [], [], [], [], []
     bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
     st = S.FrameStats()
     counter_names = [n for n, _ in S.FrameStats._fields_ if not n.startswith("ms_")]
@@ -117,6 +121,7 @@ def main():
         total_gpu_ms.append(st.ms_total)
         rast_ms.append(st.ms_raster)
         shade_ms.append(st.ms_shade)
+        shadow_ms.append(st.ms_shadows)
         for k, v in algorithmic_bytes(st).items():
             bytes_acc[k] += v
         for n in counter_names:
@@ -150,14 +155,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "box_room_1920x1080", "width": w, "height": h, "triangles": int(tris),
+        "config": {"workload": f"{args.scene}_{w}x{h}", "width": w, "height": h, "triangles": int(tris),
                    "frames_in_path": 120, "msaa": 1, "output": "device"},
         "mpix_s": round(fps * w * h / 1e6, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "pcie_inclusive_frames_s": round(host_fps, 3),
         "pass_ms": {"geometry": round(float(np.mean(geom_ms)), 4), "binning": round(float(np.mean(bin_ms)), 4),
                     "raster_shade": round(tile_avg_ms, 4), "raster": round(float(np.mean(rast_ms)), 4),
-                    "shade": round(float(np.mean(shade_ms)), 4), "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
+                    "shade": round(float(np.mean(shade_ms)), 4), "shadow_occlusion": round(float(np.mean(shadow_ms)), 4),
+                    "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
+        "occlusion_mrays_s": (round(counters_sum["occlusion_rays"] / args.steps / (float(np.mean(shadow_ms)) * 1e3), 2)
+                              if counters_sum["occlusion_rays"] else None),
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PASS[dominant], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "algorithmic_bytes_per_launch": int(dom_bytes)},
@@ -168,7 +176,7 @@ def main():
     lib.scene_destroy()
     lib.sft_release()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, h, args.cpu_frames)
+        out["cpu_baseline"] = cpu_baseline(scene, args.scene, w, h, args.cpu_frames)
     print(json.dumps(out), flush=True)
 
 

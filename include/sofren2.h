@@ -420,7 +420,12 @@ sft_model* sft_model_load_qsf(const char* path, sft_arena* arena);
    bit-identical when compiled with -ffp-contract=off.
    ================================================================ */
 #ifndef SFT_INLINE
+#if defined(__HIPCC__)
+/* the GPU kernels evaluate the same scalar math (shadow-ray packet set-up) */
+#define SFT_INLINE static inline __host__ __device__
+#else
 #define SFT_INLINE static inline
+#endif
 #endif
 
 SFT_INLINE sftvec sft_v_add(sftvec a, sftvec b) { sftvec r; r.x = a.x + b.x; r.y = a.y + b.y; r.z = a.z + b.z; return r; }

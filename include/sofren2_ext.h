@@ -27,6 +27,7 @@ extern "C" {
 #define SFTX_CAPTURE_VISIBILITY 1u   /* per-sample 32-bit IDs and 1/w depth */
 #define SFTX_CAPTURE_SS_CACHE   2u   /* screen-space meshlet cache + cull result */
 #define SFTX_CAPTURE_CLIPS      4u   /* clipped-triangle records */
+#define SFTX_CAPTURE_SHADOWS    8u   /* per-proxy light visibility counts */
 
 /* Enables capture of intermediates on the following sft_present calls. */
 void sftx_set_capture(u32 flags);
@@ -52,6 +53,11 @@ void sftx_cap_ss_visible(u8* out);
 /* Clipped-triangle records of the last present, in slot order. */
 u32 sftx_cap_clip_count(void);
 void sftx_cap_clip_records(void* out);
+/* World shadow proxies of the last frame and, per proxy, the 64 light
+   visibility counts (0..3 of the 3 occlusion rays reach light l;
+   struct sft_world_shadow_proxy::light_vis, sofren2.c:1432-1437). */
+u32 sftx_cap_proxy_count(void);
+void sftx_cap_proxy_visibility(u8* out);    /* proxy_count * 64 bytes */
 
 /* Per-frame counters and pass timings of the last present. */
 typedef struct sftx_frame_stats {
@@ -69,6 +75,12 @@ typedef struct sftx_frame_stats {
     u32 tex_bilinear;         /* T_b: bilinear albedo samples */
     u32 tex_nearest;          /* T_n: nearest albedo samples */
     u32 lightmapped_pixels;   /* L_px: pixels with a lightmap sample */
+    u32 occlusion_rays;       /* R: shadow-proxy occlusion rays traced */
+    u32 occlusion_nodes;      /* BVH child boxes tested against ray-packet frusta */
+    u32 occlusion_tri_tests;  /* ray-triangle tests */
+    u32 occlusion_invalid;    /* packets with a degenerate frustum (corner behind the light) */
+    u32 occlusion_overflow;   /* packets whose traversal stack ran out (result incomplete) */
+    u32 shadow_proxies;       /* world capsule proxies of the frame */
     f32 ms_upload;            /* host->device copies of the frame's commands */
     f32 ms_geometry;          /* cull + skin + project + classify */
     f32 ms_binning;           /* clip + bin */
@@ -77,6 +89,7 @@ typedef struct sftx_frame_stats {
     f32 ms_raster;            /* tile raster into the visibility buffer */
     f32 ms_shade;             /* deferred shading */
     f32 ms_total;             /* whole sft_present on the GPU stream */
+    f32 ms_shadows;           /* proxy occlusion + per-light capsule lists */
 } sftx_frame_stats;
 
 void sftx_get_stats(sftx_frame_stats* out);

@@ -20,6 +20,7 @@ CLIP_RECORD_BYTES = 164
 CAPTURE_VISIBILITY = 1
 CAPTURE_SS_CACHE = 2
 CAPTURE_CLIPS = 4
+CAPTURE_SHADOWS = 8
 
 SCENE_PLANE_SPHERES = 0
 SCENE_BOX_ROOM = 1
@@ -33,9 +34,10 @@ class FrameStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "meshlets_tested", "meshlets_visible", "meshlets_skinned", "batch_payloads", "clip_triangles",
         "tile_payloads", "tiles", "samples", "pixels", "shaded_pixels", "shading_groups", "tex_bilinear",
-        "tex_nearest", "lightmapped_pixels")] + [(n, ctypes.c_float) for n in (
+        "tex_nearest", "lightmapped_pixels", "occlusion_rays", "occlusion_nodes", "occlusion_tri_tests",
+        "occlusion_invalid", "occlusion_overflow", "shadow_proxies")] + [(n, ctypes.c_float) for n in (
         "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_raster", "ms_shade",
-        "ms_total")]
+        "ms_total", "ms_shadows")]
 
 
 def _bind(lib, name, res, args):
@@ -68,6 +70,8 @@ class Library:
         self.cap_ss_visible = _bind(g, "sftx_cap_ss_visible", None, [vp])
         self.cap_clip_count = _bind(g, "sftx_cap_clip_count", u32, [])
         self.cap_clip_records = _bind(g, "sftx_cap_clip_records", None, [vp])
+        self.cap_proxy_count = _bind(g, "sftx_cap_proxy_count", u32, [])
+        self.cap_proxy_visibility = _bind(g, "sftx_cap_proxy_visibility", None, [vp])
         self.get_stats = _bind(g, "sftx_get_stats", None, [ctypes.POINTER(FrameStats)])
         self.export_frame = _bind(g, "sftx_export_frame", None, [vp])
         self.sft_present = _bind(g, "sft_present", None, [])
@@ -133,6 +137,13 @@ class SceneRenderer:
             self.lib.cap_ss_cache(ss.ctypes.data)
             self.lib.cap_ss_visible(vis.ctypes.data)
         return ss.view(np.uint32).reshape(n, 9, 64), vis
+
+    def proxy_visibility(self):
+        n = self.lib.cap_proxy_count()
+        vis = np.zeros((n, 64), dtype=np.uint8)
+        if n:
+            self.lib.cap_proxy_visibility(vis.ctypes.data)
+        return vis
 
     def clips(self):
         n = self.lib.cap_clip_count()

@@ -84,6 +84,8 @@ struct Packer {
 struct ModelRes {
     u32 base = 0;                   /* first slot */
     u32 total = 0;                  /* total_meshlets */
+    u32 blas_base = kInvalid;       /* first node in the BLAS pool; kInvalid: none or skinned */
+    u32 blas_count = 0, blas_root = 0;
     u32 mesh_rec_base = 0;
     u32 max_bone = 0;               /* bone matrices a pose must provide */
     std::vector<u32> skin_of;       /* skin record per model meshlet, kInvalid if none */
@@ -104,6 +106,9 @@ struct Backend {
     bool tex_meta_dirty = true;
     DBuf est_rcp, est_rsq;
     bool est_uploaded = false;
+    DBuf blas;                      /* resident BLAS nodes of static models */
+    u32 blas_used = 0;
+    DBuf proxy_vis, light_proxy_count, light_proxies;
 
     /* per frame */
     DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
@@ -123,8 +128,8 @@ struct Backend {
     i32 cap_cols = 0, cap_rows = 0;
     std::vector<u32> h_vid;
     std::vector<f32> h_depth;
-    std::vector<u8> h_ss, h_ss_vis, h_clips;
-    u32 h_ss_count = 0, h_clip_count = 0;
+    std::vector<u8> h_ss, h_ss_vis, h_clips, h_proxy_vis;
+    u32 h_ss_count = 0, h_clip_count = 0, h_proxy_count = 0;
 };
 
 Backend* backend_create() {
@@ -149,7 +154,7 @@ void backend_destroy(Backend* b) {
                     &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
                     &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
                     &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
-                    &b->cap_depth, &b->counters };
+                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies };
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
     }
@@ -248,6 +253,21 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r, u32 base, 
         HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)skin_base * 768, skin.data(), skin.size(), hipMemcpyHostToDevice, s));
         if (resident) b->resident_skins += n_skin;
     }
+    /* BLAS of a static resident model: shadow rays test static geometry only
+       (sofren2.c:4130), so skinned models keep none */
+    r.blas_base = kInvalid;
+    r.blas_count = 0;
+    r.blas_root = 0;
+    if (resident && m->blas_nodes && m->blas_node_count > 0 && m->skeleton.bone_count <= 0) {
+        const u32 nn = m->blas_node_count;
+        dgrow(b->blas, ((size_t)b->blas_used + nn) * sizeof(sft_bvh_node), true, s);
+        HIP_CHECK(hipMemcpyAsync((u8*)b->blas.p + (size_t)b->blas_used * sizeof(sft_bvh_node), m->blas_nodes,
+                                 (size_t)nn * sizeof(sft_bvh_node), hipMemcpyHostToDevice, s));
+        r.blas_base = b->blas_used;
+        r.blas_count = nn;
+        r.blas_root = m->blas_root_ind;
+        b->blas_used += nn;
+    }
     /* the staging vectors die at scope exit: finish the copies first */
     HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -323,6 +343,20 @@ void backend_present(Backend* b, Context* cp) {
         b->models[m] = std::move(r);
     }
     b->mesh_recs.resize(b->resident_mesh_recs);
+    /* materials and lightmap pages are read at present time, as the reference
+       does: refresh the records of the resident models drawn this frame */
+    for (const sft_instance& in : c.instances) {
+        const sft_model* m = in.model;
+        if (is_transient(m)) continue;
+        const ModelRes& r = b->models[m];
+        for (u32 i = 0; i < m->mesh_count; i += 1) {
+            const sft_mesh* mesh = &m->meshes[i];
+            MeshRec& rec = b->mesh_recs[r.mesh_rec_base + i];
+            rec.material = mesh->material;
+            rec.lightmapped = (0xFFFFFFFFu != mesh->lightmap_page) ? 1u : 0u;
+            rec.lightmap_tex = rec.lightmapped && m->lightmaps ? m->lightmaps[mesh->lightmap_page] : SFT_INVALID_HANDLE;
+        }
+    }
     std::unordered_map<const sft_model*, ModelRes> tmodels;
     u32 tslot = b->resident_slots;
     const u32 saved_skins = b->resident_skins;
@@ -347,9 +381,15 @@ void backend_present(Backend* b, Context* cp) {
         const ModelRes& r = (ti != tmodels.end()) ? ti->second : b->models[m];
         InstGpu g;
         memcpy(g.t, &in.transform, 64);
+        const sftmat inv = sft_m_qinv(in.transform);
+        memcpy(g.inv, &inv, 64);
         g.model_base = r.base;
         g.ss_offset = in.ss_meshlet_offset;
         g.bone_base = kInvalid;
+        g.blas_base = r.blas_base;
+        g.blas_root = r.blas_root;
+        g.blas_count = r.blas_count;
+        g.n_meshlets = r.total;
         g.pad = 0;
         if (in.posed_bone_matrices) {
             g.bone_base = (u32)(bones.size() / 16);
@@ -386,6 +426,18 @@ void backend_present(Backend* b, Context* cp) {
         const sft_directional_light& L = c.dir_lights[i];
         dl[i] = DirLightGpu{{L.dir.x, L.dir.y, L.dir.z}, {L.color.x, L.color.y, L.color.z}, {0.f, 0.f}};
     }
+
+    /* shadow proxies, and the frame TLAS when the occlusion pass runs
+       (sofren2.c:8161-8164) */
+    std::vector<ProxyGpu> proxies(c.proxies.size());
+    for (size_t i = 0; i < proxies.size(); i += 1) {
+        const WorldProxy& w = c.proxies[i];
+        proxies[i] = ProxyGpu{{w.posa.x, w.posa.y, w.posa.z}, {w.posb.x, w.posb.y, w.posb.z}, w.radius, 0.f};
+    }
+    const bool occlusion = c.shadows_enabled && !pl.empty() && !proxies.empty();
+    std::vector<sft_bvh_node> tlas;
+    u32 tlas_root = 0;
+    if (occlusion) tlas_root = build_tlas(c.instances, tlas);
 
     FrameConst fc;
     memset(&fc, 0, sizeof(fc));
@@ -425,6 +477,10 @@ void backend_present(Backend* b, Context* cp) {
     fc.n_tex = (u32)c.tex.size();
     fc.n_mat = (u32)c.mat_tex.size();
     fc.n_tmat = (u32)c.tmat_tex.size();
+    fc.tlas_root = tlas_root;
+    fc.tlas_nodes = (u32)tlas.size();
+    fc.occ_lights = (occlusion && !tlas.empty()) ? (u32)std::min<size_t>(pl.size(), 64) : 0u;
+    fc.occ_batches = (u32)((proxies.size() + 7) / 8);
 
     /* ---- uploads ---- */
     Packer& pk = b->pack;
@@ -442,6 +498,8 @@ void backend_present(Backend* b, Context* cp) {
     const size_t o_tc = pk.add(c.tmat_color);
     const size_t o_ts = pk.add(c.tmat_shininess);
     const size_t o_mr = pk.add(b->mesh_recs);
+    const size_t o_px = pk.add(proxies);
+    const size_t o_tlas = pk.add(tlas);
 
     const u32 nw = fc.n_work;
     const u32 nt = (u32)fc.n_tiles;
@@ -462,6 +520,13 @@ void backend_present(Backend* b, Context* cp) {
     dgrow(b->vid, grid * 4, false, s);
     dgrow(b->tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
     if (cap_vis) dgrow(b->cap_depth, grid * 4, false, s);
+    dgrow(b->proxy_vis, std::max<size_t>(proxies.size() * 64, 64), false, s);
+    dgrow(b->light_proxy_count, std::max<size_t>(pl.size() * 4, 4), false, s);
+    dgrow(b->light_proxies, std::max<size_t>(pl.size() * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)), false, s);
+    if (occlusion && tlas.empty()) {
+        /* no instance to occlude: every ray reaches its light */
+        HIP_CHECK(hipMemsetAsync(b->proxy_vis.p, 3, proxies.size() * 64, s));
+    }
     const bool to_host = SFTX_OUTPUT_HOST == b->output_mode && c.color_buffer;
     if (!c.should_clear && to_host && px) {
         HIP_CHECK(hipMemcpy2DAsync(b->color.p, (size_t)c.width * 4, c.color_buffer, (size_t)c.color_pitch * 4,
@@ -524,6 +589,12 @@ void backend_present(Backend* b, Context* cp) {
         p.vid = (u32*)b->vid.p;
         p.tile_lights = (u32*)b->tile_lights.p;
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
+        p.blas = (const sft_bvh_node*)b->blas.p;
+        p.tlas = (const sft_bvh_node*)(pd + o_tlas);
+        p.proxies = (const ProxyGpu*)(pd + o_px);
+        p.proxy_vis = (u8*)b->proxy_vis.p;
+        p.light_proxy_count = (u32*)b->light_proxy_count.p;
+        p.light_proxies = (ProxyRec*)b->light_proxies.p;
         p.counters = zd;
 
         launch_frame(p, fc, s, b->ev);
@@ -572,8 +643,15 @@ void backend_present(Backend* b, Context* cp) {
     st.tex_bilinear = counters[CNT_TEX_BILINEAR];
     st.tex_nearest = counters[CNT_TEX_NEAREST];
     st.lightmapped_pixels = counters[CNT_LIGHTMAPPED];
+    st.occlusion_rays = counters[CNT_OCC_RAYS];
+    st.occlusion_nodes = counters[CNT_OCC_NODES];
+    st.occlusion_tri_tests = counters[CNT_OCC_TRIS];
+    st.occlusion_invalid = counters[CNT_OCC_INVALID];
+    st.occlusion_overflow = counters[CNT_OCC_OVERFLOW];
+    st.shadow_proxies = (u32)proxies.size();
     st.ms_upload = ev_ms(b->ev[0], b->ev[1]);
-    st.ms_geometry = ev_ms(b->ev[1], b->ev[2]);
+    st.ms_shadows = ev_ms(b->ev[1], b->ev[7]);
+    st.ms_geometry = ev_ms(b->ev[7], b->ev[2]);
     st.ms_binning = ev_ms(b->ev[2], b->ev[3]);
     st.ms_raster_shade = ev_ms(b->ev[3], b->ev[4]);
     st.ms_raster = ev_ms(b->ev[3], b->ev[6]);
@@ -597,6 +675,16 @@ void backend_present(Backend* b, Context* cp) {
         if (c.total_ss) {
             HIP_CHECK(hipMemcpy(b->h_ss.data(), b->ss.p, b->h_ss.size(), hipMemcpyDeviceToHost));
             HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), b->ss_vis.p, c.total_ss, hipMemcpyDeviceToHost));
+        }
+    }
+    if (c.capture_flags & SFTX_CAPTURE_SHADOWS) {
+        b->h_proxy_count = (u32)proxies.size();
+        b->h_proxy_vis.assign(proxies.size() * 64, 0);
+        if (occlusion) {
+            HIP_CHECK(hipMemcpy(b->h_proxy_vis.data(), b->proxy_vis.p, b->h_proxy_vis.size(), hipMemcpyDeviceToHost));
+            /* counts of lights beyond the traced ones are not kept */
+            const size_t lc = std::min<size_t>(pl.size(), 64);
+            for (size_t i = 0; i < proxies.size(); i += 1) memset(&b->h_proxy_vis[i * 64 + lc], 0, 64 - lc);
         }
     }
     if (c.capture_flags & SFTX_CAPTURE_CLIPS) {
@@ -644,6 +732,14 @@ extern "C" void sftx_cap_clip_records(void* out) {
     Backend* b = g_ctx ? g_ctx->backend : nullptr;
     if (!b) return;
     if (!b->h_clips.empty()) memcpy(out, b->h_clips.data(), b->h_clips.size());
+}
+
+extern "C" u32 sftx_cap_proxy_count(void) { return (g_ctx && g_ctx->backend) ? g_ctx->backend->h_proxy_count : 0u; }
+
+extern "C" void sftx_cap_proxy_visibility(u8* out) {
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b || !out) return;
+    if (!b->h_proxy_vis.empty()) memcpy(out, b->h_proxy_vis.data(), b->h_proxy_vis.size());
 }
 
 static Backend* ensure_backend() {

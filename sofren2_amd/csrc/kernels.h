@@ -23,6 +23,11 @@ enum Counter : int {
     CNT_TEX_BILINEAR,
     CNT_TEX_NEAREST,
     CNT_LIGHTMAPPED,
+    CNT_OCC_RAYS,           /* shadow-proxy occlusion rays traced */
+    CNT_OCC_NODES,          /* BVH child boxes tested against packet frusta */
+    CNT_OCC_TRIS,           /* ray-triangle tests */
+    CNT_OCC_INVALID,        /* packets whose frustum the reference leaves unset */
+    CNT_OCC_OVERFLOW,       /* packets whose traversal stack overflowed */
     CNT_COUNT
 };
 
@@ -76,6 +81,13 @@ struct FramePtrs {
     u32* color;             /* [height][width] */
     u32* vid;               /* [grid] reference visibility IDs of the frame */
     u32* tile_lights;       /* [n_tiles][17]: count, then light indices */
+    /* shadows */
+    const sft_bvh_node* blas;   /* resident BLAS pool */
+    const sft_bvh_node* tlas;   /* frame TLAS */
+    const ProxyGpu* proxies;    /* [n_proxies] */
+    u8* proxy_vis;              /* [n_proxies][64] rays of 3 reaching each light */
+    u32* light_proxy_count;     /* [n_point] */
+    ProxyRec* light_proxies;    /* [n_point][16] */
     f32* cap_depth;         /* [grid] 1/w when capturing, else null */
     u32* counters;          /* [CNT_COUNT] */
 };

@@ -849,9 +849,14 @@ struct TileLights {
 };
 
 /* shades one pixel of triangle t (sofren2.c:6190-6620, non-MSAA); dux..dvy are
-   the block's first-lane UV derivatives */
+   the block's first-lane UV derivatives.  `group` holds the lanes of this
+   lane's 4x2 block (bits 0-7, block starting at lane `block_lane`) covered by
+   the same triangle: the capsule-shadow loop couples them.  Every lane of a
+   wave that reaches the lighting loop runs the same light and proxy
+   iterations, so the group's lanes meet at each ballot. */
 DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
-                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch) {
+                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch,
+                    u32 group, int block_lane) {
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
     const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
     const V3 pos{interp(b, t.px), interp(b, t.py), interp(b, t.pz)};
@@ -930,18 +935,62 @@ DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, co
                 }
             }
         }
+        const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
         for (int i = 0; i < tl.count; i += 1) {
-            const PointLightGpu& L = P.point_lights[tl.idx[i]];
+            const u32 li = tl.idx[i];
+            const PointLightGpu& L = P.point_lights[li];
             const V3 lraw = sub3(V3{L.pos[0], L.pos[1], L.pos[2]}, pos);
             const f32 d2 = dot3(lraw, lraw);
             f32 att = smax(0.f, fmaf(-d2, L.inv_r2, 1.f));
             att = att * att;
             const bool active = att > 0.f;
-            if (!active) continue;
+            /* the reference skips a light no lane of the block reaches; a lane
+               with zero attenuation gains nothing from it either, so the skip
+               is taken only when the whole wave is out of reach */
+            if (0ull == __ballot(active)) continue;
             const V3 Ld = est.norm_fast(lraw);
             const f32 ndl = smax(dot3(N, Ld), 0.f);
             const bool lit = (ndl > 0.f) && active;
-            const f32 max_shadow = 0.f;
+
+            /* capsule soft shadows (6509-6568): closest approach of the pixel-
+               to-light segment and each cached capsule, smooth-max blended */
+            f32 max_shadow = 0.f;
+            const u32 np = shadows ? P.light_proxy_count[li] : 0u;
+            if (np > 0u) {
+                const f32 lrl = sqrtf(d2);
+                const ProxyRec* recs = P.light_proxies + (u64)li * kMaxLightProxies;
+                for (u32 k = 0; k < np; k += 1) {
+                    const ProxyRec& c = recs[k];
+                    const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
+                    const bool between = wdv > 0.f && wdv < d2;
+                    const f32 tf = wdv * est.rcp_refined(d2);
+                    const f32 cd2 = fmaf(-wdv, tf, c.wdw);
+                    const bool ray_active = between && cd2 < c.rb2 && (lightmapped || lit);
+                    const u64 am = __ballot(ray_active);
+                    /* a capsule no lane of the group reaches is skipped for the group */
+                    if (0u == ((u32)(am >> block_lane) & group)) continue;
+                    const V3 pa{c.pa[0], c.pa[1], c.pa[2]}, ab{c.ab[0], c.ab[1], c.ab[2]};
+                    const V3 ao = sub3(pos, pa);
+                    const f32 abl = dot3(ab, Ld), abao = dot3(ab, ao), lao = dot3(Ld, ao);
+                    const f32 den = fmaf(-abl, abl, c.abdab);
+                    const f32 sden = smax(den, SFT_EPSILON);
+                    f32 t2 = fmaf(-abl, lao, abao) * est.rcp_refined(sden);
+                    t2 = smax(0.f, smin(t2, 1.f));
+                    const f32 t1 = fmaf(t2, abl, -lao);
+                    const bool caster = t1 > SFT_EPSILON && t1 < lrl;
+                    const V3 pr{pos.x + Ld.x * t1, pos.y + Ld.y * t1, pos.z + Ld.z * t1};
+                    const V3 pc{pa.x + ab.x * t2, pa.y + ab.y * t2, pa.z + ab.z * t2};
+                    const V3 dd = sub3(pr, pc);
+                    const f32 dist2 = dot3(dd, dd);
+                    f32 sv = smax(0.f, 1.f - dist2 / c.pr2);
+                    if (!ray_active || !caster) sv = 0.f;
+                    sv = sv * c.vis_mult;
+                    const f32 diff = max_shadow - sv;
+                    const f32 h = smax(0.f, 0.2f - fabsf(diff));
+                    const f32 blend = (h * h) * (0.25f / 0.2f);
+                    max_shadow = smax(max_shadow, sv) + blend;
+                }
+            }
             const f32 si = smax(0.f, 1.f - max_shadow);
             if (lightmapped) {
                 const f32 drop = (1.f - si) * att;
@@ -1151,11 +1200,12 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const bool valid = in_bounds && kInvalid != vid;
     const u32 mine = valid ? vid : kInvalid;
 
-    /* first lane of the block holding the same triangle: the LOD lane */
-    int first = l;
-    for (int k = 7; k >= 0; k -= 1) {
-        if (__shfl(mine, (lane & ~7) | k) == mine) first = k;
+    /* lanes of the block holding the same triangle; the first is the LOD lane */
+    u32 group = 0;
+    for (int k = 0; k < 8; k += 1) {
+        if (__shfl(mine, (lane & ~7) | k) == mine) group |= 1u << k;
     }
+    const int first = __ffs(group) - 1;
     const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
     Tri t;
     Bary b{0.f, 0.f, 0.f};
@@ -1195,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         cv4 = interp(q, t.v);
     }
     u32 fetch = 0;
-    *out = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch);
+    *out = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
     i32 n_bil = (fetch & 1u) ? 1 : 0, n_near = (fetch & 2u) ? 1 : 0, n_lm = (fetch & 4u) ? 1 : 0;
@@ -1214,6 +1264,380 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         stat_add(P.counters, CNT_TEX_NEAREST, (u32)n_near, sl);
         stat_add(P.counters, CNT_LIGHTMAPPED, (u32)n_lm, sl);
     }
+}
+
+/* ================================================================ shadows
+   Proxy occlusion (sofren2.c:8163-8280): for each light l < 64 and packet of 8
+   proxies, 3 rays per proxy (from posa, posb and the midpoint) to 3 points on
+   a disk of radius SFT_SHADOW_LIGHT_SIZE facing the packet; each ray that hits
+   static geometry takes one from the proxy's visibility count for l.  A packet
+   is traced by one wave: the packet frustum (4 side planes around the origins
+   as seen from the light, a near cap at the light, a far cap at its radius,
+   sofren2.c:3970-4084) culls 8-wide BVH nodes, 8 nodes x 8 children are box-
+   tested per step across the 64 lanes, and leaf triangles are tested 8 at a
+   time against the 8 rays (lane = triangle x ray).  A ray's result is the OR
+   over every triangle the frustum reaches, so the visiting order (the
+   reference's depth-first stack) does not change it. */
+constexpr int kOccStack = 1024;
+constexpr int kOccWaves = 3;
+
+struct Frustum {
+    f32 n[6][3], an[6][3], d[6];
+};
+
+struct OccLds {
+    u32 stack[kOccStack];
+    u32 tris[72];
+    u32 insts[64];
+};
+
+DEV void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+DEV u32 lanes_below(u64 m, int lane) { return (u32)__popcll(m & ((1ull << lane) - 1ull)); }
+
+/* origin of ray `ray` of a proxy: posa, posb or the midpoint (8222-8245) */
+DEV sftvec ray_origin(const ProxyGpu& p, int ray) {
+    if (0 == ray) return SFT_V(p.a[0], p.a[1], p.a[2]);
+    if (1 == ray) return SFT_V(p.b[0], p.b[1], p.b[2]);
+    return SFT_V((p.a[0] + p.b[0]) * 0.5f, (p.a[1] + p.b[1]) * 0.5f, (p.a[2] + p.b[2]) * 0.5f);
+}
+
+/* sft__build_packet_frustum (3970-4084); returns false where the reference
+   leaves the side planes unset (a corner of the origin box behind the light) */
+DEV bool packet_frustum(const ProxyGpu* px, u32 n, int ray, sftvec lp, f32 lrad, Frustum& fr) {
+    f32 mnx = 1e9f, mxx = -1e9f, mny = 1e9f, mxy = -1e9f, mnz = 1e9f, mxz = -1e9f;
+    sftvec c = SFT_V(0.f, 0.f, 0.f);
+    for (u32 k = 0; k < n; k += 1) {
+        const sftvec o = ray_origin(px[k], ray);
+        if (o.x < mnx) mnx = o.x;
+        if (o.x > mxx) mxx = o.x;
+        if (o.y < mny) mny = o.y;
+        if (o.y > mxy) mxy = o.y;
+        if (o.z < mnz) mnz = o.z;
+        if (o.z > mxz) mxz = o.z;
+        c.x += o.x; c.y += o.y; c.z += o.z;
+    }
+    c.x /= (f32)n; c.y /= (f32)n; c.z /= (f32)n;
+    const sftvec fwd = sft_v_norm(sft_v_sub(c, lp));
+    const sftvec up = (fabsf(fwd.y) > 0.99f) ? SFT_V(1, 0, 0) : SFT_V(0, 1, 0);
+    const sftmat lv = sft_m_view_look_at(lp, c, up);
+    f32 mnxz = 1e9f, mxxz = -1e9f, mnyz = 1e9f, mxyz = -1e9f;
+    bool valid = true;
+    for (int i = 0; i < 8; i += 1) {
+        const sftquat q = { (i & 1) ? mxx : mnx, (i & 2) ? mxy : mny, (i & 4) ? mxz : mnz, 1.f };
+        const sftvec cl = sft_m_mul_quat(lv, q);
+        if (cl.z < SFT_EPSILON) { valid = false; break; }
+        const f32 xz = cl.x / cl.z, yz = cl.y / cl.z;
+        if (xz < mnxz) mnxz = xz;
+        if (xz > mxxz) mxxz = xz;
+        if (yz < mnyz) mnyz = yz;
+        if (yz > mxyz) mxyz = yz;
+    }
+    const sftvec lr = SFT_V(lv.m[0][0], lv.m[1][0], lv.m[2][0]);
+    const sftvec lu = SFT_V(lv.m[0][1], lv.m[1][1], lv.m[2][1]);
+    const sftvec lf = SFT_V(lv.m[0][2], lv.m[1][2], lv.m[2][2]);
+    for (int p = 0; p < 4; p += 1) {
+        fr.n[p][0] = fr.n[p][1] = fr.n[p][2] = 0.f;
+        fr.an[p][0] = fr.an[p][1] = fr.an[p][2] = 0.f;
+        fr.d[p] = 0.f;
+    }
+    if (valid) {
+        const sftvec nl[4] = { sft_v_norm(SFT_V(1.f, 0.f, -mnxz)), sft_v_norm(SFT_V(-1.f, 0.f, mxxz)),
+                               sft_v_norm(SFT_V(0.f, 1.f, -mnyz)), sft_v_norm(SFT_V(0.f, -1.f, mxyz)) };
+        for (int p = 0; p < 4; p += 1) {
+            const sftvec nw = sft_v_add(sft_v_mul(lr, nl[p].x), sft_v_add(sft_v_mul(lu, nl[p].y), sft_v_mul(lf, nl[p].z)));
+            fr.n[p][0] = nw.x; fr.n[p][1] = nw.y; fr.n[p][2] = nw.z;
+            fr.d[p] = -sft_v_dot(nw, lp) + SFT_SHADOW_LIGHT_SIZE;
+            fr.an[p][0] = fabsf(nw.x); fr.an[p][1] = fabsf(nw.y); fr.an[p][2] = fabsf(nw.z);
+        }
+    }
+    fr.n[4][0] = lf.x; fr.n[4][1] = lf.y; fr.n[4][2] = lf.z;
+    fr.d[4] = -sft_v_dot(lf, lp) + SFT_SHADOW_LIGHT_SIZE;
+    fr.an[4][0] = fabsf(lf.x); fr.an[4][1] = fabsf(lf.y); fr.an[4][2] = fabsf(lf.z);
+    fr.n[5][0] = -lf.x; fr.n[5][1] = -lf.y; fr.n[5][2] = -lf.z;
+    fr.d[5] = -sft_v_dot(SFT_V(-lf.x, -lf.y, -lf.z), lp) + lrad;
+    fr.an[5][0] = fabsf(lf.x); fr.an[5][1] = fabsf(lf.y); fr.an[5][2] = fabsf(lf.z);
+    return valid;
+}
+
+/* world frustum into a model's space through its transform M (4146-4169) */
+DEV void model_frustum(const Frustum& fr, const f32* M, Frustum& mf) {
+    for (int p = 0; p < 6; p += 1) {
+        const f32 nx = fr.n[p][0], ny = fr.n[p][1], nz = fr.n[p][2], d = fr.d[p];
+        const f32 nmx = nx * M[0] + ny * M[1] + nz * M[2];
+        const f32 nmy = nx * M[4] + ny * M[5] + nz * M[6];
+        const f32 nmz = nx * M[8] + ny * M[9] + nz * M[10];
+        const f32 dm = (nx * M[12] + ny * M[13] + nz * M[14]) + d;
+        const f32 il = 1.f / sqrtf(nmx * nmx + nmy * nmy + nmz * nmz);
+        mf.n[p][0] = nmx * il; mf.n[p][1] = nmy * il; mf.n[p][2] = nmz * il;
+        mf.d[p] = dm * il;
+        mf.an[p][0] = fabsf(nmx * il); mf.an[p][1] = fabsf(nmy * il); mf.an[p][2] = fabsf(nmz * il);
+    }
+}
+
+/* child box c of a node against the 6 planes, centre/extent form (3906-3968) */
+DEV bool box_in_frustum(const sft_bvh_node* nd, int c, const Frustum& f) {
+    const f32 mnx = nd->min_x[c], mny = nd->min_y[c], mnz = nd->min_z[c];
+    const f32 mxx = nd->max_x[c], mxy = nd->max_y[c], mxz = nd->max_z[c];
+    const f32 cx = (mxx + mnx) * 0.5f, cy = (mxy + mny) * 0.5f, cz = (mxz + mnz) * 0.5f;
+    const f32 ex = (mxx - mnx) * 0.5f, ey = (mxy - mny) * 0.5f, ez = (mxz - mnz) * 0.5f;
+    bool in = true;
+    for (int p = 0; p < 6; p += 1) {
+        const f32 r = fmaf(f.an[p][0], ex, fmaf(f.an[p][1], ey, f.an[p][2] * ez));
+        const f32 dc = fmaf(f.n[p][0], cx, fmaf(f.n[p][1], cy, f.n[p][2] * cz));
+        in = in && ((dc + f.d[p]) + r >= 0.f);
+    }
+    return in;
+}
+
+/* two-sided Moller-Trumbore, one ray (3840-3904) */
+DEV bool ray_hits_tri(const Est& est, V3 v0, V3 v1, V3 v2, V3 o, V3 d, f32 tmax) {
+    const V3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
+    const V3 h{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
+    const f32 a = dot3(e1, h);
+    if (!(fabsf(a) > SFT_EPSILON)) return false;
+    const f32 f = est.rcp_refined(a);
+    const V3 s = sub3(o, v0);
+    const f32 u = f * dot3(s, h);
+    if (!(u >= 0.f && u <= 1.f)) return false;
+    const V3 q{s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x};
+    const f32 v = f * dot3(d, q);
+    if (!(v >= 0.f && u + v <= 1.f)) return false;
+    const f32 t = f * dot3(e2, q);
+    return t > SFT_EPSILON && t < tmax;
+}
+
+/* Pops up to 8 nodes, box-tests their 8 children (lane = node x child) and
+   pushes the internal hits; leaf hits are returned in `leaf`/`child`.  The pop
+   count is limited so that pushes never pass the stack's end; a stack with no
+   room left sets `overflow`. */
+DEV void bvh_step(OccLds& W, u32& top, const sft_bvh_node* nodes, u32 n_nodes, const Frustum& f, int lane,
+                  bool& leaf, u32& child, u32& tested, bool& overflow) {
+    u32 k = min(8u, top);
+    k = min(k, ((u32)kOccStack - top) / 7u);
+    leaf = false;
+    child = kInvalid;
+    if (0u == k) {
+        overflow = true;
+        top = 0;
+        return;
+    }
+    const u32 slot = (u32)lane >> 3, c = (u32)lane & 7u;
+    bool push = false;
+    if (slot < k) {
+        const sft_bvh_node* nd = nodes + W.stack[top - 1u - slot];
+        child = nd->child_indices[c];
+        if (kInvalid != child && box_in_frustum(nd, (int)c, f)) {
+            if (nd->is_leaf_group) leaf = true;
+            else push = child < n_nodes;
+        }
+    }
+    tested += k * 8u;
+    wave_sync();
+    top -= k;
+    const u64 pm = __ballot(push);
+    if (push) W.stack[top + lanes_below(pm, lane)] = child;
+    top += (u32)__popcll(pm);
+    wave_sync();
+}
+
+/* traces one packet (8 rays, `active` lanes) against the frame's static
+   geometry; returns the rays that hit (4086-4243) */
+DEV u32 trace_packet(const FramePtrs& P, const FrameConst& fc, const Est& est, OccLds& W, const Frustum& fr,
+                     sftvec o_r, sftvec l_r, u32 active, int lane, u32& nodes_tested, u32& tris_tested, bool& overflow) {
+    u32 sat = 0;
+    /* TLAS: collect the instance leaves the frustum reaches */
+    u32 top = (fc.tlas_root < fc.tlas_nodes) ? 1u : 0u, ninst = 0;
+    if (0 == lane) W.stack[0] = fc.tlas_root;
+    wave_sync();
+    /* every step pops at least one node and a tree holds each node once, so a
+       traversal longer than the node count means a malformed (cyclic) BVH */
+    u32 steps = 0;
+    while (top > 0u) {
+        if (++steps > fc.tlas_nodes + 1u) {
+            overflow = true;
+            break;
+        }
+        bool leaf;
+        u32 child;
+        bvh_step(W, top, P.tlas, fc.tlas_nodes, fr, lane, leaf, child, nodes_tested, overflow);
+        const u64 lm = __ballot(leaf);
+        if (leaf) W.insts[min(ninst + lanes_below(lm, lane), 63u)] = child & 0x7FFFFFFFu;
+        ninst = min(ninst + (u32)__popcll(lm), 64u);
+        wave_sync();
+    }
+    const int r = lane & 7, tslot = lane >> 3;
+    for (u32 ii = 0; ii < ninst && sat != active; ii += 1) {
+        const u32 inst_id = W.insts[ii];
+        if (inst_id >= fc.n_inst) continue;
+        const InstGpu& in = P.insts[inst_id];
+        if (kInvalid == in.blas_base || in.blas_root >= in.blas_count) continue;
+        Frustum mf;
+        model_frustum(fr, in.t, mf);
+        /* this lane's ray (r) in model space (4171-4178) */
+        const V3 ow{o_r.x, o_r.y, o_r.z};
+        const V3 ew{o_r.x + l_r.x, o_r.y + l_r.y, o_r.z + l_r.z};
+        const V3 lo = xform(ow, in.inv);
+        const V3 le = xform(ew, in.inv);
+        const V3 L = sub3(le, lo);
+        const f32 tmax = sqrtf(dot3(L, L));
+        const V3 ld = est.norm_fast(L);
+        const sft_bvh_node* nodes = P.blas + in.blas_base;
+        const u32 slot_base = in.model_base;
+        top = 1;
+        u32 ntri = 0;
+        if (0 == lane) W.stack[0] = in.blas_root;
+        wave_sync();
+        steps = 0;
+        while ((top > 0u || ntri > 0u) && sat != active) {
+            if (top > 0u && ++steps > in.blas_count + 1u) {
+                overflow = true;
+                top = 0;
+            }
+            if (top > 0u) {
+                bool leaf;
+                u32 child;
+                bvh_step(W, top, nodes, in.blas_count, mf, lane, leaf, child, nodes_tested, overflow);
+                const u64 lm = __ballot(leaf);
+                if (leaf) W.tris[ntri + lanes_below(lm, lane)] = child & 0x7FFFFFFFu;
+                ntri += (u32)__popcll(lm);
+                wave_sync();
+            }
+            while (ntri >= 8u || (0u == top && ntri > 0u)) {
+                const u32 n = min(ntri, 8u);
+                bool hit = false;
+                const u32 pl = ((u32)tslot < n) ? W.tris[ntri - 1u - (u32)tslot] : 0u;
+                const u32 m = pl >> 7, tri = pl & 0x7Fu;
+                if ((u32)tslot < n && (((active & ~sat) >> r) & 1u) && m < in.n_meshlets) {
+                    const u32 slot = slot_base + m;
+                    const u8* ib = P.inds + (u64)slot * 512u + (tri >> 3) * 32u + (tri & 7u);
+                    const f32* pp = P.pos + (u64)slot * 192u;
+                    const u32 i0 = ib[0] & 63u, i1 = ib[8] & 63u, i2 = ib[16] & 63u;
+                    const V3 v0{pp[i0], pp[64 + i0], pp[128 + i0]};
+                    const V3 v1{pp[i1], pp[64 + i1], pp[128 + i1]};
+                    const V3 v2{pp[i2], pp[64 + i2], pp[128 + i2]};
+                    hit = ray_hits_tri(est, v0, v1, v2, lo, ld, tmax);
+                }
+                tris_tested += n * (u32)__popc(active & ~sat);
+                const u64 hm = __ballot(hit);
+                u32 h8 = 0;
+                for (int t = 0; t < 8; t += 1) h8 |= (u32)(hm >> (8 * t)) & 0xFFu;
+                sat |= h8;
+                ntri -= n;
+                wave_sync();
+                if (sat == active) break;
+            }
+        }
+    }
+    return sat;
+}
+
+/* one workgroup per (light, packet of 8 proxies); wave w traces ray w */
+__global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    __shared__ OccLds s_w[kOccWaves];
+    __shared__ u32 s_hit[kOccWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u32 l = blockIdx.x / fc.occ_batches, batch = blockIdx.x % fc.occ_batches;
+    const u32 pbase = batch * 8u;
+    const u32 n = min(8u, fc.n_proxies - pbase);
+    const ProxyGpu* px = P.proxies + pbase;
+    const Est est = make_est(P, fc);
+    const PointLightGpu& L = P.point_lights[l];
+    const sftvec lp = SFT_V(L.pos[0], L.pos[1], L.pos[2]);
+
+    /* light-disk targets facing the packet's centre (8178-8210) */
+    sftvec bc = SFT_V(0.f, 0.f, 0.f);
+    for (u32 k = 0; k < n; k += 1) {
+        bc.x += (px[k].a[0] + px[k].b[0]) * 0.5f;
+        bc.y += (px[k].a[1] + px[k].b[1]) * 0.5f;
+        bc.z += (px[k].a[2] + px[k].b[2]) * 0.5f;
+    }
+    bc = sft_v_mul(bc, 1.f / (f32)n);
+    const sftvec ldir = sft_v_norm(sft_v_sub(lp, bc));
+    const sftvec lright = sft_v_norm(sft_v_perpendicular(ldir));
+    const sftvec lup = sft_v_cross(lright, ldir);
+    const f32 rr = SFT_SHADOW_LIGHT_SIZE;
+    sftvec target;
+    if (0 == wave) target = sft_v_add(lp, sft_v_mul(lup, rr));
+    else if (1 == wave) target = sft_v_add(lp, sft_v_add(sft_v_mul(lright, rr * 0.866025f), sft_v_mul(lup, -rr * 0.5f)));
+    else target = sft_v_add(lp, sft_v_add(sft_v_mul(lright, -rr * 0.866025f), sft_v_mul(lup, -rr * 0.5f)));
+
+    Frustum fr;
+    const bool valid = packet_frustum(px, n, wave, lp, L.radius, fr);
+    const u32 r = (u32)lane & 7u;
+    sftvec o_r = SFT_V(0.f, 0.f, 0.f), l_r = SFT_V(0.f, 0.f, 0.f);
+    if (r < n) {
+        o_r = ray_origin(px[r], wave);
+        l_r = SFT_V(target.x - o_r.x, target.y - o_r.y, target.z - o_r.z);
+    }
+    const u32 active = (1u << n) - 1u;
+    u32 nodes_tested = 0, tris_tested = 0;
+    bool overflow = false;
+    const u32 sat = trace_packet(P, fc, est, s_w[wave], fr, o_r, l_r, active, lane, nodes_tested, tris_tested, overflow);
+    if (0 == lane) {
+        s_hit[wave] = sat;
+        const u32 sl = stat_slot();
+        stat_add(P.counters, CNT_OCC_RAYS, n, sl);
+        stat_add(P.counters, CNT_OCC_NODES, nodes_tested, sl);
+        stat_add(P.counters, CNT_OCC_TRIS, tris_tested, sl);
+        stat_add(P.counters, CNT_OCC_INVALID, valid ? 0u : 1u, sl);
+        stat_add(P.counters, CNT_OCC_OVERFLOW, overflow ? 1u : 0u, sl);
+    }
+    __syncthreads();
+    if (0 == wave && r < n && lane < 8) {
+        const u32 hits = ((s_hit[0] >> r) & 1u) + ((s_hit[1] >> r) & 1u) + ((s_hit[2] >> r) & 1u);
+        P.proxy_vis[(u64)(pbase + r) * 64u + l] = (u8)(3u - hits);
+    }
+}
+
+/* per point light: the first 16 proxies, in index order, with visibility left
+   for the light and a bounding sphere reaching the light's sphere, and their
+   shading constants (sofren2.c:5936-5990).  The selection depends on the
+   light only, so it is done once per light instead of once per tile. */
+__global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    const int lane = threadIdx.x;
+    const u32 l = blockIdx.x;
+    const PointLightGpu& L = P.point_lights[l];
+    const sftvec lpos = SFT_V(L.pos[0], L.pos[1], L.pos[2]);
+    u32 count = 0;
+    for (u32 base = 0; base < fc.n_proxies && count < (u32)kMaxLightProxies; base += 64u) {
+        const u32 p = base + (u32)lane;
+        bool ok = false;
+        ProxyRec rec;
+        if (p < fc.n_proxies) {
+            const ProxyGpu& px = P.proxies[p];
+            /* the visibility count exists for the first 64 lights only */
+            const u32 vis = (l < 64u) ? (u32)P.proxy_vis[(u64)p * 64u + l] : 3u;
+            if (0u != vis) {
+                const sftvec a = SFT_V(px.a[0], px.a[1], px.a[2]), b = SFT_V(px.b[0], px.b[1], px.b[2]);
+                const sftvec cc = SFT_V((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
+                const sftvec ab = sft_v_sub(b, a);
+                const f32 len = sft_v_length(ab);
+                const f32 br = px.radius + (len * 0.5f);
+                const f32 cr = br + L.radius;
+                const sftvec w = sft_v_sub(cc, lpos);
+                const f32 d2 = sft_v_length2(w);
+                if (d2 <= cr * cr) {
+                    ok = true;
+                    rec.pa[0] = a.x; rec.pa[1] = a.y; rec.pa[2] = a.z;
+                    rec.ab[0] = ab.x; rec.ab[1] = ab.y; rec.ab[2] = ab.z;
+                    rec.w[0] = w.x; rec.w[1] = w.y; rec.w[2] = w.z;
+                    rec.abdab = sft_v_dot(ab, ab);
+                    rec.wdw = d2;
+                    rec.rb2 = br * br;
+                    rec.pr2 = px.radius * px.radius;
+                    rec.vis_mult = (f32)vis * 0.3333333f;
+                    rec.pad[0] = rec.pad[1] = 0.f;
+                }
+            }
+        }
+        const u64 m = __ballot(ok);
+        const u32 rank = count + lanes_below(m, lane);
+        if (ok && rank < (u32)kMaxLightProxies) P.light_proxies[(u64)l * kMaxLightProxies + rank] = rec;
+        count = min(count + (u32)__popcll(m), (u32)kMaxLightProxies);
+    }
+    if (0 == lane) P.light_proxy_count[l] = count;
 }
 
 /* folds the spread statistics slots into the narrow counter words */
@@ -1238,6 +1662,14 @@ void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEv
     const u32 nw = fc.n_work;
     const u32 geom_blocks = (nw + 3u) / 4u;
     hipEventRecord(ev[1], s);
+    /* shadow-proxy visibility and the per-light capsule lists */
+    if (fc.occ_lights > 0u && fc.occ_batches > 0u) {
+        hipLaunchKernelGGL(k_occlude, dim3(fc.occ_lights * fc.occ_batches), dim3(64 * kOccWaves), 0, s, p);
+    }
+    if (fc.shadows_enabled && fc.n_proxies > 0u && fc.n_point > 0u) {
+        hipLaunchKernelGGL(k_proxy_cache, dim3(fc.n_point), dim3(64), 0, s, p);
+    }
+    hipEventRecord(ev[7], s);
     if (nw > 0) {
         hipLaunchKernelGGL(k_geom, dim3(geom_blocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.clip_count, p.clip_base, nw,

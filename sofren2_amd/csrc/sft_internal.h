@@ -46,16 +46,38 @@ struct FrameConst {
     u32 n_work, n_inst, n_point, n_dir;
     u32 n_proxies, base_tex, rcp_bits, rsq_bits;
     u32 clip_cap, tile_list_cap, capture, n_tex;
-    u32 n_mat, n_tmat, pad5, pad6;
+    u32 n_mat, n_tmat, tlas_root, occ_lights;   /* occ_lights = min(point lights, 64) when shadows run */
+    u32 occ_batches, tlas_nodes, pad8, pad9;    /* proxy packets of 8 per light */
 };
 
 struct InstGpu {
     f32 t[16];
+    f32 inv[16];        /* rigid inverse of t (sft_m_qinv) for shadow rays */
     u32 model_base;     /* first meshlet slot of the model */
     u32 ss_offset;      /* ss_meshlet_offset */
     u32 bone_base;      /* first matrix in the frame's bone buffer, kInvalid if none */
+    u32 blas_base;      /* first node in the resident BLAS pool; kInvalid when the
+                           model has no BLAS or has a skeleton (shadow rays test
+                           static geometry only, sofren2.c:4130) */
+    u32 blas_root;      /* root node relative to blas_base */
+    u32 blas_count;     /* nodes of the model's BLAS */
+    u32 n_meshlets;     /* total_meshlets of the model */
     u32 pad;
 };
+
+/* world-space capsule proxy as the GPU reads it */
+struct ProxyGpu {
+    f32 a[3], b[3], radius, pad;
+};
+
+/* per (light, proxy) shading constants (sofren2.c:5960-5985, 991-998) */
+struct ProxyRec {
+    f32 pa[3], abdab;
+    f32 ab[3], wdw;
+    f32 w[3], rb2;
+    f32 pr2, vis_mult, pad[2];
+};
+constexpr int kMaxLightProxies = 16;
 
 struct WorkItem {
     u32 slot;           /* resident meshlet slot */
@@ -109,6 +131,9 @@ struct HostTexture {
 
 struct Context;
 extern Context* g_ctx;
+
+/* frame TLAS over the drawn instances (tlas.cpp); returns the root node */
+u32 build_tlas(const std::vector<sft_instance>& instances, std::vector<sft_bvh_node>& nodes);
 
 struct Backend;
 Backend* backend_create();

@@ -15,16 +15,19 @@ pytestmark = pytest.mark.gpu
 
 
 def render_gpu(scene, seed, w, h, frame=0):
-    r = S.SceneRenderer(scene, seed, w, h, capture=S.CAPTURE_VISIBILITY | S.CAPTURE_SS_CACHE | S.CAPTURE_CLIPS)
+    r = S.SceneRenderer(scene, seed, w, h, capture=S.CAPTURE_VISIBILITY | S.CAPTURE_SS_CACHE | S.CAPTURE_CLIPS
+                        | S.CAPTURE_SHADOWS)
     try:
         color = r.render(frame).copy()
         vid, depth = r.visibility()
         ss, ssvis = r.ss_cache()
         clip = r.clips()
+        proxy_vis = r.proxy_visibility()
         st = r.stats()
     finally:
         r.close()
-    return {"color": color, "vid": vid, "depth": depth, "ss": ss, "ssvis": ssvis, "clip": clip}, st
+    return {"color": color, "vid": vid, "depth": depth, "ss": ss, "ssvis": ssvis, "clip": clip,
+            "proxy_vis": proxy_vis}, st
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -45,4 +48,23 @@ def test_box_room_parity(frame):
     rep = parity.compare_frame(gpu, ref)
     print("box_room frame", frame, rep, "visible", st.meshlets_visible, "clips", st.clip_triangles,
           "ms", st.ms_total, flush=True)
+    parity.assert_parity(rep)
+
+
+@pytest.mark.parametrize("frame", [0, 50])
+def test_crowd_shadows_parity(frame):
+    """Skinned crowd in the box room with 16 shadow lights: skinned screen-space
+    cache bit-exact, per-proxy light visibility from the occlusion packets
+    exact, and the capsule-shadowed colour."""
+    w, h = S.SCENE_SIZE[S.SCENE_CROWD_SHADOWS]
+    print("crowd_shadows frame", frame, flush=True)
+    gpu, st = render_gpu(S.SCENE_CROWD_SHADOWS, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_CROWD_SHADOWS, 0, w, h, frame)
+    assert st.occlusion_invalid == 0 and st.occlusion_overflow == 0
+    assert st.occlusion_rays == 16 * 320 * 3
+    np.testing.assert_array_equal(gpu["proxy_vis"], ref["proxy_vis"])
+    rep = parity.compare_frame(gpu, ref)
+    print("crowd_shadows frame", frame, rep, "visible", st.meshlets_visible, "skinned", st.meshlets_skinned,
+          "occ nodes", st.occlusion_nodes, "tri tests", st.occlusion_tri_tests, "ms", st.ms_total,
+          "shadows ms", st.ms_shadows, flush=True)
     parity.assert_parity(rep)
