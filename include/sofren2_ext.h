@@ -103,6 +103,12 @@ void sftx_get_stats(sftx_frame_stats* out);
 void sftx_set_output(i32 mode);
 const u32* sftx_device_color(void);     /* device pointer, width*height ARGB */
 
+/* Multisampling of the following frames: 1 (off), 2 or 4 samples per pixel
+   at the reference's rotated-grid offsets (sofren2.c:186-196); the reference
+   fixes this at compile time with SFT_USE_AA / SFT_AA_SAMPLES.  Other values
+   are rejected with a message and leave the setting unchanged. */
+void sftx_set_msaa(i32 samples);
+
 /* Waits for all GPU work of the renderer. */
 void sftx_synchronize(void);
 
@@ -161,6 +167,7 @@ typedef struct sftx_frame_desc {
     u32 base_tex;                   /* albedo of the base material */
     u32* color_buffer;
     i32 color_pitch;                /* in u32 */
+    i32 samples;                    /* MSAA samples per pixel: 1, 2 or 4 */
 } sftx_frame_desc;
 
 /* Fills `out` with the frame recorded so far.  Needs no GPU. */

@@ -358,9 +358,17 @@ static void geometry_pass(port_state* ps) {
 }
 
 /* ------------------------------------------------------------ raster
-   sft__rast_simd_batch, sofren2.c:5148-5499 (non-MSAA path) */
+   sft__rast_simd_batch, sofren2.c:5148-5499; rotated-grid sample offsets
+   from the pixel centre, sofren2.c:186-196 */
+static const f32 k_msaa_ox[3][4] = { { 0.f }, { -0.25f, 0.25f }, { -0.125f, 0.375f, -0.375f, 0.125f } };
+static const f32 k_msaa_oy[3][4] = { { 0.f }, { -0.25f, 0.25f }, { -0.375f, -0.125f, 0.125f, 0.375f } };
+static int msaa_row(int samples) { return 4 == samples ? 2 : (2 == samples ? 1 : 0); }
+
 static void raster_payload(port_state* ps, u64 payload, f32* depth, u32* vid, i32 tmx, i32 tmy) {
     const sftx_frame_desc* f = ps->f;
+    const int S = f->samples, aa = S > 1;
+    const f32* ox = k_msaa_ox[msaa_row(S)];
+    const f32* oy = k_msaa_oy[msaa_row(S)];
     u32 active = (u32)((payload >> 56) & 0xFF);
     f32 sx[3][8], sy[3][8], iw[3][8];
     u32 inst_ind = 0, m_ind = 0, batch = 0, slot = 0;
@@ -424,7 +432,17 @@ static void raster_payload(port_state* ps, u64 payload, f32* depth, u32* vid, i3
         const f32 me0 = se0 + (A0 < 0.f ? A0 * bw : 0.f) + (B0 < 0.f ? B0 * bh : 0.f);
         const f32 me1 = se1 + (A1 < 0.f ? A1 * bw : 0.f) + (B1 < 0.f ? B1 * bh : 0.f);
         const f32 me2 = se2 + (A2 < 0.f ? A2 * bw : 0.f) + (B2 < 0.f ? B2 * bh : 0.f);
-        const int trivial = me0 >= 0.f && me1 >= 0.f && me2 >= 0.f;
+        const f32 pad0 = aa ? 0.375f * (fabsf(A0) + fabsf(B0)) : 0.f;
+        const f32 pad1 = aa ? 0.375f * (fabsf(A1) + fabsf(B1)) : 0.f;
+        const f32 pad2 = aa ? 0.375f * (fabsf(A2) + fabsf(B2)) : 0.f;
+        const int trivial = me0 >= pad0 && me1 >= pad1 && me2 >= pad2;
+        f32 de0[4], de1[4], de2[4], dz[4];
+        for (int q = 0; q < S; q += 1) {
+            de0[q] = A0 * ox[q] + B0 * oy[q];
+            de1[q] = A1 * ox[q] + B1 * oy[q];
+            de2[q] = A2 * ox[q] + B2 * oy[q];
+            dz[q] = dzdx * ox[q] + dzdy * oy[q];
+        }
         f32 zr[8], e0r[8], e1r[8], e2r[8];
         for (int j = 0; j < 8; j += 1) {
             zr[j] = sz + dzdx * (f32)j;
@@ -438,12 +456,32 @@ static void raster_payload(port_state* ps, u64 payload, f32* depth, u32* vid, i3
             memcpy(z, zr, sizeof(z)); memcpy(e0, e0r, sizeof(e0)); memcpy(e1, e1r, sizeof(e1)); memcpy(e2, e2r, sizeof(e2));
             i32 p = (y - tmy) * TS + (amin - tmx);
             for (i32 x = amin; x < amax; x += 8, p += 8) {
-                for (int j = 0; j < 8; j += 1) {
-                    const int cover = trivial || (e0[j] >= 0.f && e1[j] >= 0.f && e2[j] >= 0.f);
-                    if (cover && z[j] > depth[p + j]) {
-                        depth[p + j] = z[j];
-                        vid[p + j] = id;
+                if (!aa) {
+                    for (int j = 0; j < 8; j += 1) {
+                        const int cover = trivial || (e0[j] >= 0.f && e1[j] >= 0.f && e2[j] >= 0.f);
+                        if (cover && z[j] > depth[p + j]) {
+                            depth[p + j] = z[j];
+                            vid[p + j] = id;
+                        }
                     }
+                }
+                else {
+                    /* edge path: a chunk no pixel centre comes within the sample
+                       pad of is skipped (5414-5420) */
+                    int might = trivial;
+                    for (int j = 0; j < 8 && !might; j += 1) might = e0[j] >= -pad0 && e1[j] >= -pad1 && e2[j] >= -pad2;
+                    for (int q = 0; q < S && might; q += 1) {
+                        for (int j = 0; j < 8; j += 1) {
+                            const int cover = trivial || (e0[j] + de0[q] >= 0.f && e1[j] + de1[q] >= 0.f && e2[j] + de2[q] >= 0.f);
+                            const f32 zs = z[j] + dz[q];
+                            if (cover && zs > depth[q * TP + p + j]) {
+                                depth[q * TP + p + j] = zs;
+                                vid[q * TP + p + j] = id;
+                            }
+                        }
+                    }
+                }
+                for (int j = 0; j < 8; j += 1) {
                     z[j] = z[j] + dzc;
                     e0[j] = e0[j] + a0c; e1[j] = e1[j] + a1c; e2[j] = e2[j] + a2c;
                 }
@@ -572,19 +610,32 @@ static void fetch_tri(port_state* ps, u32 vid, tri_t* t) {
 }
 
 /* perspective-correct weights at a pixel centre (6273-6318) */
-static void bary(const tri_t* t, f32 px, f32 py, f32* p0, f32* p1, f32* p2) {
+/* with MSAA the screen barycentrics are clamped to the triangle and
+   renormalised, so pixel centres outside it never extrapolate (6296-6305) */
+static void bary_ms(const tri_t* t, f32 px, f32 py, int aa, f32* p0, f32* p1, f32* p2) {
     const f32 dx1 = t->sx[1] - t->sx[0], dx2 = t->sx[2] - t->sx[0];
     const f32 dy1 = t->sy[1] - t->sy[0], dy2 = t->sy[2] - t->sy[0];
     const f32 inv_det = rcp_refined(dx1 * dy2 - dx2 * dy1);
     const f32 ex = px - t->sx[0], ey = py - t->sy[0];
-    const f32 bu = (ex * dy2 - ey * dx2) * inv_det;
-    const f32 bv = (ey * dx1 - ex * dy1) * inv_det;
-    const f32 bw = (1.f - bu) - bv;
+    f32 bu = (ex * dy2 - ey * dx2) * inv_det;
+    f32 bv = (ey * dx1 - ex * dy1) * inv_det;
+    f32 bw = (1.f - bu) - bv;
+    if (aa) {
+        bu = smax(0.f, bu);
+        bv = smax(0.f, bv);
+        bw = smax(0.f, bw);
+        const f32 inv_sum = rcp_refined(bu + (bv + bw));
+        bu = bu * inv_sum;
+        bv = bv * inv_sum;
+        bw = bw * inv_sum;
+    }
     const f32 z = rcp_refined(pfma(bw, t->iw[0], pfma(bu, t->iw[1], bv * t->iw[2])));
     *p0 = (bw * t->iw[0]) * z;
     *p1 = (bu * t->iw[1]) * z;
     *p2 = (bv * t->iw[2]) * z;
 }
+
+static void bary(const tri_t* t, f32 px, f32 py, f32* p0, f32* p1, f32* p2) { bary_ms(t, px, py, 0, p0, p1, p2); }
 
 static inline f32 interp3(f32 p0, f32 p1, f32 p2, const f32* a) { return pfma(p0, a[0], pfma(p1, a[1], p2 * a[2])); }
 
@@ -1086,8 +1137,9 @@ static void build_proxy_cache(port_state* ps, tile_shading* ts) {
 
 static const i32 k_ox[8] = { 0, 1, 2, 3, 0, 1, 2, 3 }, k_oy[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
 
-/* shades one (block, triangle) group and stores the lanes in `lanes` */
-static void shade_group(port_state* ps, const tri_t* t, int lanes, i32 x, i32 y, const tile_shading* ts) {
+/* shades one (block, triangle) group; out[c][lane] is the colour clamped to
+   [0, 1] for every lane (only the lanes in `lanes` are meaningful) */
+static void shade_group(port_state* ps, const tri_t* t, int lanes, i32 x, i32 y, const tile_shading* ts, f32 out[3][8]) {
     const sftx_frame_desc* f = ps->f;
     f32 cu[8], cv[8], clu[8], clv[8];
     v3 pos[8], N[8], V[8];
@@ -1096,7 +1148,7 @@ static void shade_group(port_state* ps, const tri_t* t, int lanes, i32 x, i32 y,
     const v3 cam = { f->camera_pos.x, f->camera_pos.y, f->camera_pos.z };
     for (int l = 0; l < 8; l += 1) {
         f32 p0, p1, p2;
-        bary(t, ((f32)x + 0.5f) + (f32)k_ox[l], ((f32)y + 0.5f) + (f32)k_oy[l], &p0, &p1, &p2);
+        bary_ms(t, ((f32)x + 0.5f) + (f32)k_ox[l], ((f32)y + 0.5f) + (f32)k_oy[l], f->samples > 1, &p0, &p1, &p2);
         cu[l] = interp3(p0, p1, p2, t->u); cv[l] = interp3(p0, p1, p2, t->v);
         clu[l] = interp3(p0, p1, p2, t->lu); clv[l] = interp3(p0, p1, p2, t->lv);
         pos[l].x = interp3(p0, p1, p2, ax); pos[l].y = interp3(p0, p1, p2, ay); pos[l].z = interp3(p0, p1, p2, az);
@@ -1244,11 +1296,100 @@ static void shade_group(port_state* ps, const tri_t* t, int lanes, i32 x, i32 y,
         }
     }
     for (int l = 0; l < 8; l += 1) {
-        if (!(lanes & (1 << l))) continue;
-        const f32 r = smax(0.f, smin(fr[l], 1.f)) * 255.f;
-        const f32 g = smax(0.f, smin(fg[l], 1.f)) * 255.f;
-        const f32 b = smax(0.f, smin(fb[l], 1.f)) * 255.f;
-        ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]] = 0xFF000000u | ((u32)cvtt(r) << 16) | ((u32)cvtt(g) << 8) | (u32)cvtt(b);
+        out[0][l] = smax(0.f, smin(fr[l], 1.f));
+        out[1][l] = smax(0.f, smin(fg[l], 1.f));
+        out[2][l] = smax(0.f, smin(fb[l], 1.f));
+    }
+}
+
+static u32 pack_argb(f32 r, f32 g, f32 b) {
+    return 0xFF000000u | ((u32)cvtt(r) << 16) | ((u32)cvtt(g) << 8) | (u32)cvtt(b);
+}
+
+/* MSAA block loop (6048-6135, 6677-6743): groups in the order of each
+   triangle's first (sample, lane) occurrence; each shades once per pixel and
+   adds its colour weighted by the fraction of the pixel's samples it covers,
+   over a start of the background colour times the empty-sample fraction */
+static void resolve_block_aa(port_state* ps, const u32* vid, i32 x, i32 y, i32 tmx, i32 tmy, const tile_shading* ts) {
+    const sftx_frame_desc* f = ps->f;
+    const int S = f->samples;
+    const i32 sw = (i32)(f->half_width * 2.f), sh = (i32)(f->half_height * 2.f);
+    const f32 inv_s = 1.f / (f32)S;
+    u32 v[4][8];
+    int act[4] = {0}, inb = 0, active = 0;
+    i32 empty[8] = {0};
+    for (int l = 0; l < 8; l += 1) {
+        const i32 gx = x + k_ox[l], gy = y + k_oy[l];
+        if (gx < sw && gy < sh) inb |= 1 << l;
+    }
+    for (int q = 0; q < S; q += 1) {
+        for (int l = 0; l < 8; l += 1) {
+            v[q][l] = vid[q * TP + (y + k_oy[l] - tmy) * TS + (x + k_ox[l] - tmx)];
+            const int is_empty = 0xFFFFFFFFu == v[q][l];
+            if (!is_empty && (inb >> l & 1)) act[q] |= 1 << l;
+            if (is_empty && (inb >> l & 1)) empty[l] += 1;
+        }
+        active |= act[q];
+    }
+    if (0 == active) {
+        if (f->should_clear) {
+            for (int l = 0; l < 8; l += 1) {
+                if (inb >> l & 1) ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]] = f->clear_color;
+            }
+        }
+        return;
+    }
+    f32 acc[3][8] = {{0}}, ew[8];
+    int any_ew = 0;
+    for (int l = 0; l < 8; l += 1) {
+        ew[l] = (f32)empty[l] * inv_s;
+        if (ew[l] > 0.f) any_ew = 1;
+    }
+    const u32 cc = f->clear_color;
+    if (f->should_clear) {
+        const f32 cr = (f32)((cc >> 16) & 0xFF) * (1.f / 255.f), cg = (f32)((cc >> 8) & 0xFF) * (1.f / 255.f), cb = (f32)(cc & 0xFF) * (1.f / 255.f);
+        for (int l = 0; l < 8; l += 1) { acc[0][l] = cr * ew[l]; acc[1][l] = cg * ew[l]; acc[2][l] = cb * ew[l]; }
+    }
+    else if (any_ew) {
+        for (int l = 0; l < 8; l += 1) {
+            f32 br = 0.f, bg = 0.f, bb = 0.f;
+            if (inb >> l & 1) {
+                const u32 c = ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]];
+                br = (f32)((c >> 16) & 0xFF) * (1.f / 255.f); bg = (f32)((c >> 8) & 0xFF) * (1.f / 255.f); bb = (f32)(c & 0xFF) * (1.f / 255.f);
+            }
+            acc[0][l] = br * ew[l]; acc[1][l] = bg * ew[l]; acc[2][l] = bb * ew[l];
+        }
+    }
+    const int write_mask = active;
+    while (active) {
+        int fs = 0;
+        while (0 == act[fs]) fs += 1;
+        const u32 target = v[fs][__builtin_ctz(act[fs])];
+        int tri_mask = 0, hits[8] = {0};
+        for (int q = 0; q < S; q += 1) {
+            int m = 0;
+            for (int l = 0; l < 8; l += 1) {
+                if ((inb >> l & 1) && v[q][l] == target) { m |= 1 << l; hits[l] += 1; }
+            }
+            act[q] &= ~m;
+            tri_mask |= m;
+        }
+        active = 0;
+        for (int q = 0; q < S; q += 1) active |= act[q];
+        tri_t t;
+        fetch_tri(ps, target, &t);
+        f32 col[3][8];
+        shade_group(ps, &t, tri_mask, x, y, ts, col);
+        for (int l = 0; l < 8; l += 1) {
+            const f32 w = (tri_mask >> l & 1) ? (f32)hits[l] * inv_s : 0.f;
+            for (int c = 0; c < 3; c += 1) acc[c][l] = pfma(col[c][l], w, acc[c][l]);
+        }
+    }
+    const int store = f->should_clear ? inb : write_mask;
+    for (int l = 0; l < 8; l += 1) {
+        if (!(store >> l & 1)) continue;
+        ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]] =
+            pack_argb(smin(acc[0][l], 1.f) * 255.f, smin(acc[1][l], 1.f) * 255.f, smin(acc[2][l], 1.f) * 255.f);
     }
 }
 
@@ -1257,7 +1398,7 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
     const i32 sw = (i32)(f->half_width * 2.f), sh = (i32)(f->half_height * 2.f);
     const i32 lx = SFT_MIN(tmx + TS, sw), ly = SFT_MIN(tmy + TS, sh);
     f32 zmin = 1e9f, zmax = -1e9f;
-    for (i32 i = 0; i < TP; i += 1) {
+    for (i32 i = 0; i < TP * f->samples; i += 1) {
         if (depth[i] > 0.0001f) {
             if (depth[i] < zmin) zmin = depth[i];
             if (depth[i] > zmax) zmax = depth[i];
@@ -1268,6 +1409,10 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
     build_proxy_cache(ps, &ts);
     for (i32 y = tmy; y < ly; y += 2) {
         for (i32 x = tmx; x < lx; x += 4) {
+            if (f->samples > 1) {
+                resolve_block_aa(ps, vid, x, y, tmx, tmy, &ts);
+                continue;
+            }
             u32 vids[8];
             int inb = 0, active = 0;
             for (int l = 0; l < 8; l += 1) {
@@ -1287,7 +1432,12 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
                 active &= ~lanes;
                 tri_t t;
                 fetch_tri(ps, target, &t);
-                shade_group(ps, &t, lanes, x, y, &ts);
+                f32 col[3][8];
+                shade_group(ps, &t, lanes, x, y, &ts, col);
+                for (int l = 0; l < 8; l += 1) {
+                    if (!(lanes & (1 << l))) continue;
+                    ps->out->color[(y + k_oy[l]) * f->width + x + k_ox[l]] = pack_argb(col[0][l] * 255.f, col[1][l] * 255.f, col[2][l] * 255.f);
+                }
             }
         }
     }
@@ -1313,20 +1463,23 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
     geometry_pass(&ps);
     out->clip_count = ps.clip_next < out->clip_cap ? ps.clip_next : out->clip_cap;
 
-    const i32 gw = f->tile_cols * TS;
-    f32 depth[TP];
-    u32 vid[TP];
+    const i32 gw = f->tile_cols * TS, gh = f->tile_rows * TS;
+    const int S = f->samples;
+    static f32 depth[TP * 4];
+    static u32 vid[TP * 4];
     for (i32 t = 0; t < nt; t += 1) {
         const i32 tmx = (t % f->tile_cols) * TS, tmy = (t / f->tile_cols) * TS;
         memset(depth, 0, sizeof(depth));
         memset(vid, 0xFF, sizeof(vid));
         for (u32 k = 0; k < ps.bins[t].n; k += 1) raster_payload(&ps, ps.bins[t].p[k], depth, vid, tmx, tmy);
         if (out->vid) {
-            for (i32 y = 0; y < TS; y += 1) {
-                for (i32 x = 0; x < TS; x += 1) {
-                    const u64 g = (u64)(tmy + y) * gw + tmx + x;
-                    out->vid[g] = vid[y * TS + x];
-                    out->depth[g] = depth[y * TS + x];
+            for (int q = 0; q < S; q += 1) {
+                for (i32 y = 0; y < TS; y += 1) {
+                    for (i32 x = 0; x < TS; x += 1) {
+                        const u64 g = (u64)q * gw * gh + (u64)(tmy + y) * gw + tmx + x;
+                        out->vid[g] = vid[q * TP + y * TS + x];
+                        out->depth[g] = depth[q * TP + y * TS + x];
+                    }
                 }
             }
         }
@@ -1342,6 +1495,8 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
 u32 sftx_port_ss_count(const sftx_frame_desc* f) { return f->total_ss; }
 
 i32 sftx_port_proxy_count(const sftx_frame_desc* f) { return f->n_proxies; }
+
+i32 sftx_port_samples(const sftx_frame_desc* f) { return f->samples; }
 
 void sftx_port_grid(const sftx_frame_desc* f, i32* cols, i32* rows, i32* width, i32* height) {
     *cols = f->tile_cols;

@@ -36,6 +36,7 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out);
 /* sizes the caller needs for the output buffers */
 u32 sftx_port_ss_count(const sftx_frame_desc* f);
 i32 sftx_port_proxy_count(const sftx_frame_desc* f);
+i32 sftx_port_samples(const sftx_frame_desc* f);
 void sftx_port_grid(const sftx_frame_desc* f, i32* cols, i32* rows, i32* width, i32* height);
 
 #endif

@@ -78,6 +78,7 @@ class Library:
         self.sft_frame_end = _bind(g, "sft_frame_end", None, [])
         self.scene_record = _bind(s, "sftx_scene_record", None, [i32, vp, i32])
         self.set_output = _bind(g, "sftx_set_output", None, [i32])
+        self.set_msaa = _bind(g, "sftx_set_msaa", None, [i32])
         self.synchronize = _bind(g, "sftx_synchronize", None, [])
         self.copy_calibration = _bind(g, "sftx_copy_calibration", ctypes.c_float, [ctypes.c_uint64])
         self.scene_create = _bind(s, "sftx_scene_create", i32, [i32, u32, i32, i32])
@@ -100,13 +101,15 @@ def library():
 class SceneRenderer:
     """Renders a seeded workload through the public API on the GPU."""
 
-    def __init__(self, scene, seed=0, width=None, height=None, capture=0):
+    def __init__(self, scene, seed=0, width=None, height=None, capture=0, samples=1):
         self.lib = library()
         w, h = SCENE_SIZE[scene]
         self.width = width or w
         self.height = height or h
+        self.samples = samples
         self.lib.sft_init(self.width, self.height, 0)
         self.lib.set_capture(capture)
+        self.lib.set_msaa(samples)
         if self.lib.scene_create(scene, seed, self.width, self.height) != 0:
             raise ValueError(f"unknown scene {scene}")
         self.color = np.zeros((self.height, self.width), dtype=np.uint32)

@@ -442,6 +442,14 @@ extern "C" const char* sft_get_debug_name(enum sft_debug_mode mode) {
 }
 
 extern "C" void sft_set_clear(u8 enabled) { C().should_clear = enabled; }
+
+extern "C" void sftx_set_msaa(i32 samples) {
+    if (1 != samples && 2 != samples && 4 != samples) {
+        fprintf(stderr, "sofren2: %d MSAA samples per pixel not supported (1, 2 or 4)\n", samples);
+        return;
+    }
+    C().samples = samples;
+}
 extern "C" void sft_set_clear_color(u32 color) { C().clear_color = color; }
 extern "C" void sft_set_color_buffer(u32* buf, i32 pitch_in_bytes) {
     Context& c = C();
@@ -986,6 +994,7 @@ extern "C" void sftx_export_frame(sftx_frame_desc* f) {
     f->lightmap_brightness = c.lightmap_brightness;
     f->shadows_enabled = c.shadows_enabled;
     f->should_clear = c.should_clear;
+    f->samples = c.samples;
     f->clear_color = c.clear_color;
     f->debug_mode = (i32)c.debug_mode;
     f->instances = c.instances.data();

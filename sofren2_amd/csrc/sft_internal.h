@@ -163,6 +163,7 @@ struct Context {
     i32 color_pitch = 0;            /* in u32 */
     u8 should_clear = 0;
     u32 clear_color = 0xFF222222u;
+    i32 samples = 1;                /* MSAA samples per pixel */
     enum sft_debug_mode debug_mode = SFT_DEBUG_NONE;
 
     sftvec camera_pos{0, 0, 0};

@@ -34,6 +34,8 @@ def port_lib():
         lib.sftx_port_ss_count.argtypes = [ctypes.c_void_p]
         lib.sftx_port_proxy_count.restype = ctypes.c_int32
         lib.sftx_port_proxy_count.argtypes = [ctypes.c_void_p]
+        lib.sftx_port_samples.restype = ctypes.c_int32
+        lib.sftx_port_samples.argtypes = [ctypes.c_void_p]
         lib.sftx_port_grid.restype = None
         lib.sftx_port_grid.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
         _PORT = lib
@@ -47,7 +49,8 @@ def port_render_desc(desc, clip_cap=1 << 16):
     lib.sftx_port_grid(desc, ctypes.byref(cols), ctypes.byref(rows), ctypes.byref(w), ctypes.byref(h))
     n_ss = lib.sftx_port_ss_count(desc)
     n_proxies = lib.sftx_port_proxy_count(desc)
-    grid = cols.value * 64 * rows.value * 64
+    samples = lib.sftx_port_samples(desc)
+    grid = cols.value * 64 * rows.value * 64 * samples
     color = np.zeros((h.value, w.value), dtype=np.uint32)
     vid = np.zeros(grid, dtype=np.uint32)
     depth = np.zeros(grid, dtype=np.float32)
@@ -56,7 +59,7 @@ def port_render_desc(desc, clip_cap=1 << 16):
     clips = np.zeros(clip_cap * S.CLIP_RECORD_BYTES, dtype=np.uint8)
     proxy_vis = np.zeros((max(n_proxies, 1), 64), dtype=np.uint8)
     out = PortOut(color.ctypes.data, vid.ctypes.data, depth.ctypes.data, ss.ctypes.data, ssvis.ctypes.data,
-                  clips.ctypes.data, clip_cap, 0, 1, proxy_vis.ctypes.data, 0)
+                  clips.ctypes.data, clip_cap, 0, samples, proxy_vis.ctypes.data, 0)
     rc = lib.sftx_port_render(desc, ctypes.byref(out))
     if rc != 0:
         raise RuntimeError("clip buffer too small")
@@ -67,10 +70,11 @@ def port_render_desc(desc, clip_cap=1 << 16):
             "proxy_vis": proxy_vis[:n_proxies], "invalid_packets": out.invalid_packets}
 
 
-def port_render_scene(scene, seed, width, height, frame=0):
+def port_render_scene(scene, seed, width, height, frame=0, samples=1):
     """Records a frame with the product front-end and renders it with the restatement (no GPU)."""
     lib = S.library()
     lib.sft_init(width, height, 0)
+    lib.set_msaa(samples)
     try:
         if lib.scene_create(scene, seed, width, height) != 0:
             raise ValueError(scene)

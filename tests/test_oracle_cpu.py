@@ -64,3 +64,18 @@ def test_port_matches_reference_crowd_shadows(frame):
     print(rep, "clips", rep["clip_count"])
     parity.assert_parity(rep)
     assert rep["color_exact_frac"] == 1.0, rep
+
+
+@pytest.mark.parametrize("samples", [2, 4])
+@pytest.mark.parametrize("scene,frame", [(S.SCENE_PLANE_SPHERES, 0), (S.SCENE_BOX_ROOM, 37)])
+def test_port_matches_reference_msaa(scene, frame, samples):
+    """Rotated-grid MSAA: per-sample IDs and depth, sample-weighted resolve."""
+    if not have_ref(samples):
+        pytest.skip("MSAA reference build missing")
+    w, h = S.SCENE_SIZE[scene]
+    port = port_render_scene(scene, 0, w, h, frame, samples=samples)
+    ref = ref_dump(scene, 0, w, h, frame, samples=samples)
+    rep = parity.compare_frame(port, ref)
+    print(rep)
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
