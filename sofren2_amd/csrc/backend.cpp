@@ -108,7 +108,7 @@ struct Backend {
     bool est_uploaded = false;
     DBuf blas;                      /* resident BLAS nodes of static models */
     u32 blas_used = 0;
-    DBuf proxy_vis, light_proxy_count, light_proxies;
+    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z;
 
     /* per frame */
     DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
@@ -125,7 +125,7 @@ struct Backend {
     u32 tile_list_cap = 1u << 20;
 
     /* host copies of the last frame's intermediates (capture) */
-    i32 cap_cols = 0, cap_rows = 0;
+    i32 cap_cols = 0, cap_rows = 0, cap_samples = 1;
     std::vector<u32> h_vid;
     std::vector<f32> h_depth;
     std::vector<u8> h_ss, h_ss_vis, h_clips, h_proxy_vis;
@@ -154,7 +154,7 @@ void backend_destroy(Backend* b) {
                     &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
                     &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
                     &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
-                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies };
+                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z };
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
     }
@@ -448,7 +448,7 @@ void backend_present(Backend* b, Context* cp) {
     fc.tile_cols = c.tile_cols;
     fc.tile_rows = c.tile_rows;
     fc.n_tiles = c.tile_cols * c.tile_rows;
-    fc.samples = 1;
+    fc.samples = c.samples;
     fc.hw = c.half_width;
     fc.hh = c.half_height;
     fc.vp_x = c.vp_x;
@@ -516,8 +516,9 @@ void backend_present(Backend* b, Context* cp) {
     if (!b->counters_h) HIP_CHECK(hipHostMalloc((void**)&b->counters_h, CNT_COUNT * 4, hipHostMallocDefault));
     dgrow(b->color, std::max<size_t>(px * 4, 4), false, s);
     const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
-    const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64;
+    const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64 * (size_t)c.samples;
     dgrow(b->vid, grid * 4, false, s);
+    dgrow(b->tile_z, (size_t)nt * c.samples * 8, false, s);
     dgrow(b->tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
     if (cap_vis) dgrow(b->cap_depth, grid * 4, false, s);
     dgrow(b->proxy_vis, std::max<size_t>(proxies.size() * 64, 64), false, s);
@@ -588,6 +589,7 @@ void backend_present(Backend* b, Context* cp) {
         p.color = (u32*)b->color.p;
         p.vid = (u32*)b->vid.p;
         p.tile_lights = (u32*)b->tile_lights.p;
+        p.tile_z = (f32*)b->tile_z.p;
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
         p.blas = (const sft_bvh_node*)b->blas.p;
         p.tlas = (const sft_bvh_node*)(pd + o_tlas);
@@ -636,7 +638,7 @@ void backend_present(Backend* b, Context* cp) {
     st.clip_triangles = counters[CNT_CLIP_TRIS];
     st.tile_payloads = counters[CNT_TILE_PAIRS];
     st.tiles = nt;
-    st.samples = (u32)px;
+    st.samples = (u32)(px * (size_t)c.samples);
     st.pixels = (u32)px;
     st.shaded_pixels = counters[CNT_SHADED];
     st.shading_groups = counters[CNT_QGROUPS];
@@ -662,6 +664,7 @@ void backend_present(Backend* b, Context* cp) {
     /* ---- capture ---- */
     b->cap_cols = c.tile_cols;
     b->cap_rows = c.tile_rows;
+    b->cap_samples = c.samples;
     if (cap_vis) {
         b->h_vid.resize(grid);
         b->h_depth.resize(grid);
@@ -702,7 +705,7 @@ extern "C" void sftx_cap_grid(i32* cols, i32* rows, i32* samples) {
     Backend* b = g_ctx ? g_ctx->backend : nullptr;
     if (cols) *cols = b ? b->cap_cols : 0;
     if (rows) *rows = b ? b->cap_rows : 0;
-    if (samples) *samples = 1;
+    if (samples) *samples = b ? b->cap_samples : 1;
 }
 
 extern "C" void sftx_cap_visibility(u32* vid, f32* depth) {

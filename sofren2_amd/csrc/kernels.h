@@ -79,8 +79,9 @@ struct FramePtrs {
     u32* clip_key;          /* [clip_cap] */
     u32* clip_rect;         /* [clip_cap] */
     u32* color;             /* [height][width] */
-    u32* vid;               /* [grid] reference visibility IDs of the frame */
+    u32* vid;               /* [samples][grid] reference visibility IDs of the frame */
     u32* tile_lights;       /* [n_tiles][17]: count, then light indices */
+    f32* tile_z;            /* [n_tiles][samples][2]: 1/w range per sample plane (MSAA) */
     /* shadows */
     const sft_bvh_node* blas;   /* resident BLAS pool */
     const sft_bvh_node* tlas;   /* frame TLAS */
@@ -88,7 +89,7 @@ struct FramePtrs {
     u8* proxy_vis;              /* [n_proxies][64] rays of 3 reaching each light */
     u32* light_proxy_count;     /* [n_point] */
     ProxyRec* light_proxies;    /* [n_point][16] */
-    f32* cap_depth;         /* [grid] 1/w when capturing, else null */
+    f32* cap_depth;         /* [samples][grid] 1/w when capturing, else null */
     u32* counters;          /* [CNT_COUNT] */
 };
 

@@ -133,6 +133,14 @@ DEV void stat_add(u32* counters, int cnt, u32 v, u32 slot) {
 
 DEV u32 stat_slot() { return blockIdx.x * 4u + blockIdx.y * 13u + (threadIdx.x >> 6); }
 
+DEV void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+DEV u32 lanes_below(u64 m, int lane) { return (u32)__popcll(m & ((1ull << lane) - 1ull)); }
+
+/* rotated-grid MSAA sample offsets from the pixel centre (sofren2.c:186-196) */
+__constant__ f32 c_msaa_ox[3][4] = { { 0.f }, { -0.25f, 0.25f }, { -0.125f, 0.375f, -0.375f, 0.125f } };
+__constant__ f32 c_msaa_oy[3][4] = { { 0.f }, { -0.25f, 0.25f }, { -0.375f, -0.125f, 0.125f, 0.375f } };
+DEV int msaa_row(int samples) { return 4 == samples ? 2 : (2 == samples ? 1 : 0); }
+
 DEV i32 wave_sum(i32 v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
@@ -609,10 +617,15 @@ struct Tri {
     u32 inst, slot;
 };
 
-/* rasterises one triangle of the tile, lane j = x offset inside an 8-pixel chunk
-   (sofren2.c:5213-5497, non-MSAA path) */
-DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, u32 key,
-                    f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2) {
+/* rasterises one triangle of the tile into one sample plane, lane j = x offset
+   inside an 8-pixel chunk, the triangle's 8 lanes at group `g` of the wave
+   (sofren2.c:5213-5497).  With MSAA (`aa`) the sample sits at (ox, oy) from
+   the pixel centre: edge and depth values gain A*ox + B*oy, trivial accept
+   needs a margin of 0.375 (|A| + |B|), and a chunk whose 8 pixel centres are
+   all beyond that margin of an edge is skipped. */
+DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u32 key,
+                    f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2,
+                    bool aa, f32 ox, f32 oy) {
     const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
     const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
     const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
@@ -651,7 +664,12 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, u32 key,
     const f32 me0 = se0 + (A0 < 0.f ? A0 * box_w : 0.f) + (B0 < 0.f ? B0 * box_h : 0.f);
     const f32 me1 = se1 + (A1 < 0.f ? A1 * box_w : 0.f) + (B1 < 0.f ? B1 * box_h : 0.f);
     const f32 me2 = se2 + (A2 < 0.f ? A2 * box_w : 0.f) + (B2 < 0.f ? B2 * box_h : 0.f);
-    const bool trivial = me0 >= 0.f && me1 >= 0.f && me2 >= 0.f;
+    const f32 pad0 = aa ? 0.375f * (fabsf(A0) + fabsf(B0)) : 0.f;
+    const f32 pad1 = aa ? 0.375f * (fabsf(A1) + fabsf(B1)) : 0.f;
+    const f32 pad2 = aa ? 0.375f * (fabsf(A2) + fabsf(B2)) : 0.f;
+    const bool trivial = me0 >= pad0 && me1 >= pad1 && me2 >= pad2;
+    const f32 de0 = A0 * ox + B0 * oy, de1 = A1 * ox + B1 * oy, de2 = A2 * ox + B2 * oy;
+    const f32 dzs = dzdx * ox + dzdy * oy;
 
     const u32 nkey = ~key;
     const f32 fj = (f32)j;
@@ -662,7 +680,8 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, u32 key,
         for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
             f32 z = z_row;
             for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                if (z > 0.f) atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
+                const f32 zs = aa ? z + dzs : z;
+                if (zs > 0.f) atomicMax((unsigned long long*)&vis[p], ((u64)f2u(zs) << 32) | nkey);
                 z = z + dz_chunk;
             }
             z_row = z_row + dzdy;
@@ -674,8 +693,19 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, u32 key,
         for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
             f32 e0 = e0r, e1 = e1r, e2 = e2r, z = z_row;
             for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                if (e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f) {
-                    atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
+                if (!aa) {
+                    if (e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f) {
+                        atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
+                    }
+                }
+                else {
+                    /* the triangle's 8 lanes run the same rows and chunks */
+                    const bool near = e0 >= -pad0 && e1 >= -pad1 && e2 >= -pad2;
+                    const u32 any_near = (u32)(__ballot(near) >> (g * 8)) & 0xFFu;
+                    const f32 zs = z + dzs;
+                    if (any_near && e0 + de0 >= 0.f && e1 + de1 >= 0.f && e2 + de2 >= 0.f && zs > 0.f) {
+                        atomicMax((unsigned long long*)&vis[p], ((u64)f2u(zs) << 32) | nkey);
+                    }
                 }
                 e0 = e0 + A0c;
                 e1 = e1 + A1c;
@@ -740,14 +770,25 @@ DEV void load_tri(const FramePtrs& P, u32 vid, Tri& t) {
 
 /* barycentric setup at a pixel centre (sofren2.c:6273-6312, non-MSAA) */
 struct Bary { f32 pw0, pw1, pw2; };
-DEV Bary bary_at(const Est& est, const Tri& t, f32 px, f32 py) {
+/* with MSAA (`aa`) the screen barycentrics are clamped to the triangle and
+   renormalised so pixel centres outside it never extrapolate (6296-6305) */
+DEV Bary bary_at(const Est& est, const Tri& t, f32 px, f32 py, bool aa = false) {
     const f32 dx1 = t.sx[1] - t.sx[0], dx2 = t.sx[2] - t.sx[0];
     const f32 dy1 = t.sy[1] - t.sy[0], dy2 = t.sy[2] - t.sy[0];
     const f32 inv_det = est.rcp_refined(dx1 * dy2 - dx2 * dy1);
     const f32 ex = px - t.sx[0], ey = py - t.sy[0];
-    const f32 bu = (ex * dy2 - ey * dx2) * inv_det;
-    const f32 bv = (ey * dx1 - ex * dy1) * inv_det;
-    const f32 bw = (1.f - bu) - bv;
+    f32 bu = (ex * dy2 - ey * dx2) * inv_det;
+    f32 bv = (ey * dx1 - ex * dy1) * inv_det;
+    f32 bw = (1.f - bu) - bv;
+    if (aa) {
+        bu = smax(0.f, bu);
+        bv = smax(0.f, bv);
+        bw = smax(0.f, bw);
+        const f32 inv_sum = est.rcp_refined(bu + (bv + bw));
+        bu = bu * inv_sum;
+        bv = bv * inv_sum;
+        bw = bw * inv_sum;
+    }
     const f32 inv_z = fmaf(bw, t.iw[0], fmaf(bu, t.iw[1], bv * t.iw[2]));
     const f32 z = est.rcp_refined(inv_z);
     Bary b;
@@ -854,9 +895,9 @@ struct TileLights {
    the same triangle: the capsule-shadow loop couples them.  Every lane of a
    wave that reaches the lighting loop runs the same light and proxy
    iterations, so the group's lanes meet at each ballot. */
-DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
-                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch,
-                    u32 group, int block_lane) {
+DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
+                   const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch,
+                   u32 group, int block_lane) {
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
     const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
     const V3 pos{interp(b, t.px), interp(b, t.py), interp(b, t.pz)};
@@ -1026,10 +1067,11 @@ DEV u32 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, co
             fb = fmaf(alb_b, db, sb);
         }
     }
-    fr = smax(0.f, smin(fr, 1.f)) * 255.f;
-    fg = smax(0.f, smin(fg, 1.f)) * 255.f;
-    fb = smax(0.f, smin(fb, 1.f)) * 255.f;
-    return 0xFF000000u | ((u32)cvtt(fr) << 16) | ((u32)cvtt(fg) << 8) | (u32)cvtt(fb);
+    return V3{smax(0.f, smin(fr, 1.f)), smax(0.f, smin(fg, 1.f)), smax(0.f, smin(fb, 1.f))};
+}
+
+DEV u32 pack_argb(f32 r, f32 g, f32 b) {
+    return 0xFF000000u | ((u32)cvtt(r) << 16) | ((u32)cvtt(g) << 8) | (u32)cvtt(b);
 }
 
 /* tile light culling for one light against the tile's 1/w range and screen
@@ -1065,9 +1107,29 @@ DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f
     return true;
 }
 
-/* one workgroup per 64x64 tile: rasterise the tile's payloads into an LDS
-   visibility buffer, then publish per-pixel reference IDs, the tile's 1/w
-   range and its light list (sofren2.c:6756-6813, 5820-5934) */
+/* first 16 point lights touching the tile, in index order, by one wave
+   (sofren2.c:5862-5934) */
+DEV void build_tile_lights(const FramePtrs& P, const FrameConst& fc, u32 tile, f32 zmin, f32 zmax, int lane) {
+    const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
+    u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
+    u32 n = 0;
+    if (zmin <= zmax) {
+        for (u32 base = 0; base < fc.n_point && n < (u32)kMaxTileLights; base += 64u) {
+            const u32 l = base + (u32)lane;
+            const bool pass = l < fc.n_point && light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy);
+            const u64 m = __ballot(pass);
+            const u32 rank = n + lanes_below(m, lane);
+            if (pass && rank < (u32)kMaxTileLights) tl[1 + rank] = l;
+            n = min(n + (u32)__popcll(m), (u32)kMaxTileLights);
+        }
+    }
+    if (0 == lane) tl[0] = n;
+}
+
+/* one workgroup per (64x64 tile, sample): rasterise the tile's payloads into
+   an LDS visibility plane, then publish per-sample reference IDs and the
+   plane's 1/w range; without MSAA the tile's light list is built here too
+   (sofren2.c:6756-6813, 5820-5934) */
 constexpr int kRasterThreads = 1024;
 constexpr int kRasterWaves = kRasterThreads / 64;
 
@@ -1075,10 +1137,11 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     const FrameConst& fc = *P.fc;
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     __shared__ f32 s_red[2][kRasterWaves];
-    __shared__ u32 s_pass[kRasterThreads];
-    __shared__ u32 s_count;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 tile = blockIdx.x;
+    const int sample = (int)blockIdx.y;
+    const bool aa = fc.samples > 1;
+    const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
     const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
     const Est est = make_est(P, fc);
 
@@ -1118,14 +1181,15 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
                 sx2 = ss[i2]; sy2 = ss[64 + i2]; iw2 = ss[128 + i2];
                 key = (wi << 9) | (b << 5) | (16u + (u32)g);
             }
-            raster_tri(s_vis, est, tmx, tmy, j, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2);
+            raster_tri(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, aa, ox, oy);
         }
     }
     __syncthreads();
 
-    /* publish IDs, and the tile 1/w range over every sample (screen edge included) */
+    /* publish IDs, and the plane's 1/w range over every sample (screen edge included) */
     const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
     const i32 gw = fc.tile_cols * 64;
+    const u64 plane = (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
     f32 zmin = 1e9f, zmax = -1e9f;
     for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
         const u64 pk = s_vis[i];
@@ -1134,7 +1198,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             zmin = fminf(zmin, z);
             zmax = fmaxf(zmax, z);
         }
-        const u64 gi = (u64)(tmy + (i >> 6)) * gw + (tmx + (i & 63));
+        const u64 gi = plane + (u64)(tmy + (i >> 6)) * gw + (tmx + (i & 63));
         P.vid[gi] = (0ull == pk) ? kInvalid : key_to_vid(P, ~(u32)pk, n_clip);
         if (P.cap_depth) P.cap_depth[gi] = z;
     }
@@ -1143,34 +1207,35 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         zmax = fmaxf(zmax, __shfl_xor(zmax, o));
     }
     if (0 == lane) { s_red[0][wave] = zmin; s_red[1][wave] = zmax; }
-    if (0 == tid) s_count = 0;
     __syncthreads();
+    if (0 != wave) return;
     zmin = s_red[0][0];
     zmax = s_red[1][0];
     for (int k = 1; k < kRasterWaves; k += 1) {
         zmin = fminf(zmin, s_red[0][k]);
         zmax = fmaxf(zmax, s_red[1][k]);
     }
-
-    /* first 16 point lights touching the tile, in index order */
-    u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
-    if (zmin <= zmax && fc.n_point > 0u) {
-        for (u32 base = 0; base < fc.n_point; base += (u32)kRasterThreads) {
-            const u32 l = base + (u32)tid;
-            s_pass[tid] = (l < fc.n_point) ? (u32)light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy) : 0u;
-            __syncthreads();
-            if (0 == tid) {
-                u32 n = s_count;
-                for (u32 k = 0; k < (u32)kRasterThreads && base + k < fc.n_point && n < (u32)kMaxTileLights; k += 1) {
-                    if (s_pass[k]) tl[1 + n++] = base + k;
-                }
-                s_count = n;
-            }
-            __syncthreads();
-            if (s_count >= (u32)kMaxTileLights) break;
+    if (aa) {
+        if (0 == lane) {
+            P.tile_z[((u64)tile * (u32)fc.samples + (u32)sample) * 2u] = zmin;
+            P.tile_z[((u64)tile * (u32)fc.samples + (u32)sample) * 2u + 1u] = zmax;
         }
+        return;
     }
-    if (0 == tid) tl[0] = s_count;
+    build_tile_lights(P, fc, tile, zmin, zmax, lane);
+}
+
+/* MSAA: the tile's 1/w range over all sample planes, then its light list */
+__global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    const u32 tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    f32 zmin = 1e9f, zmax = -1e9f;
+    for (int s = 0; s < fc.samples; s += 1) {
+        zmin = fminf(zmin, P.tile_z[((u64)tile * (u32)fc.samples + (u32)s) * 2u]);
+        zmax = fmaxf(zmax, P.tile_z[((u64)tile * (u32)fc.samples + (u32)s) * 2u + 1u]);
+    }
+    build_tile_lights(P, fc, tile, zmin, zmax, lane);
 }
 
 /* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
@@ -1245,7 +1310,8 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         cv4 = interp(q, t.v);
     }
     u32 fetch = 0;
-    *out = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
+    const V3 col = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
+    *out = pack_argb(col.x * 255.f, col.y * 255.f, col.z * 255.f);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
     i32 n_bil = (fetch & 1u) ? 1 : 0, n_near = (fetch & 2u) ? 1 : 0, n_lm = (fetch & 4u) ? 1 : 0;
@@ -1256,6 +1322,136 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     n_bil = __popcll(__ballot(n_bil != 0));
     n_near = __popcll(__ballot(n_near != 0));
     n_lm = __popcll(__ballot(n_lm != 0));
+    if (lane == __ffsll((long long)act) - 1) {
+        const u32 sl = stat_slot();
+        stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
+        stat_add(P.counters, CNT_QGROUPS, (u32)groups, sl);
+        stat_add(P.counters, CNT_TEX_BILINEAR, (u32)n_bil, sl);
+        stat_add(P.counters, CNT_TEX_NEAREST, (u32)n_near, sl);
+        stat_add(P.counters, CNT_LIGHTMAPPED, (u32)n_lm, sl);
+    }
+}
+
+/* MSAA resolve (sofren2.c:6048-6135, 6677-6743).  Lanes are laid out as in
+   k_shade.  A block's triangles are taken in the order of their first
+   (sample, lane) occurrence, as the reference iterates them; every lane of
+   the block evaluates the triangle at its pixel centre (clamped barycentrics)
+   so the LOD lane's neighbours are at hand, the lanes covering it shade once
+   and add the colour weighted by their share of samples.  Each pixel starts
+   from the background colour times its empty-sample share. */
+__global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
+    const FrameConst& fc = *P.fc;
+    __shared__ TileLights s_tl;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const i32 x0 = (i32)blockIdx.x * 32, y0 = (i32)blockIdx.y * 8;
+    const i32 tile = (y0 >> 6) * fc.tile_cols + (x0 >> 6);
+    const Est est = make_est(P, fc);
+    if (0 == tid) {
+        const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
+        s_tl.count = (int)tl[0];
+        for (int i = 0; i < s_tl.count; i += 1) s_tl.idx[i] = (u16)tl[1 + i];
+    }
+    __syncthreads();
+    const TileLights tl = s_tl;
+
+    const int S = fc.samples;
+    const f32 inv_s = 1.f / (f32)S;
+    const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
+    const int base = lane & ~7;
+    const i32 bx = x0 + blk * 4, by = y0 + wave * 2;
+    const i32 gx = bx + ix, gy = by + iy;
+    const i32 gw = fc.tile_cols * 64;
+    const u64 plane = (u64)gw * (u64)(fc.tile_rows * 64);
+    const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
+    u32 sv[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+    u32 mine = 0;
+    i32 empty = 0;
+    for (int s = 0; s < 4; s += 1) {
+        if (s < S) {
+            const u32 v = P.vid[(u64)s * plane + (u64)gy * gw + gx];
+            sv[s] = in_bounds ? v : kInvalid;
+            if (in_bounds && kInvalid != v) mine |= 1u << (s * 8 + l);
+            if (in_bounds && kInvalid == v) empty += 1;
+        }
+    }
+    /* block-wide (sample, lane) occupancy, bit s * 8 + lane */
+    u32 remaining = mine;
+    remaining |= __shfl_xor(remaining, 1);
+    remaining |= __shfl_xor(remaining, 2);
+    remaining |= __shfl_xor(remaining, 4);
+    u32* out = &P.color[(u64)gy * fc.width + gx];
+    if (0u == remaining) {
+        if (in_bounds && fc.should_clear) *out = fc.clear_color;
+        return;
+    }
+    const f32 ew = (f32)empty * inv_s;
+    f32 acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
+    if (fc.should_clear) {
+        const u32 cc = fc.clear_color;
+        acc_r = unpack_ch(cc, 16) * ew;
+        acc_g = unpack_ch(cc, 8) * ew;
+        acc_b = unpack_ch(cc, 0) * ew;
+    }
+    else {
+        const u32 any_ew = (u32)(__ballot(ew > 0.f) >> base) & 0xFFu;
+        if (any_ew) {
+            const u32 c = in_bounds ? *out : 0u;
+            acc_r = unpack_ch(c, 16) * ew;
+            acc_g = unpack_ch(c, 8) * ew;
+            acc_b = unpack_ch(c, 0) * ew;
+        }
+    }
+    const bool write = fc.should_clear ? in_bounds : (0u != mine);
+    const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
+    i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
+    while (0u != remaining) {
+        /* next triangle: the first remaining (sample, lane) of the block */
+        const int pos = __ffs(remaining) - 1;
+        const int s0 = pos >> 3, l0 = pos & 7;
+        const u32 own = (0 == s0) ? sv[0] : (1 == s0) ? sv[1] : (2 == s0) ? sv[2] : sv[3];
+        const u32 target = __shfl(own, base | l0);
+        u32 hits = 0, match = 0;
+        for (int s = 0; s < 4; s += 1) {
+            if (s < S && sv[s] == target) {
+                hits += 1;
+                match |= 1u << (s * 8 + l);
+            }
+        }
+        match |= __shfl_xor(match, 1);
+        match |= __shfl_xor(match, 2);
+        match |= __shfl_xor(match, 4);
+        remaining &= ~match;
+        const u32 group = (u32)(__ballot(hits > 0u) >> base) & 0xFFu;
+        Tri t;
+        load_tri(P, target, t);
+        const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
+        const f32 cu = interp(b, t.u), cv = interp(b, t.v);
+        const int first = __ffs(group) - 1;
+        const f32 cu0 = __shfl(cu, base | first), cv0 = __shfl(cv, base | first);
+        const f32 cu1 = __shfl(cu, base | (first ^ 1)), cv1 = __shfl(cv, base | (first ^ 1));
+        const f32 cu4 = __shfl(cu, base | (first ^ 4)), cv4 = __shfl(cv, base | (first ^ 4));
+        if (hits > 0u) {
+            u32 fetch = 0;
+            const V3 col = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, base);
+            const f32 w = (f32)hits * inv_s;
+            acc_r = fmaf(col.x, w, acc_r);
+            acc_g = fmaf(col.y, w, acc_g);
+            acc_b = fmaf(col.z, w, acc_b);
+            shaded += 1;
+            groups += (first == l) ? 1 : 0;
+            n_bil += (fetch & 1u) ? 1 : 0;
+            n_near += (fetch & 2u) ? 1 : 0;
+            n_lm += (fetch & 4u) ? 1 : 0;
+        }
+    }
+    if (write) *out = pack_argb(smin(acc_r, 1.f) * 255.f, smin(acc_g, 1.f) * 255.f, smin(acc_b, 1.f) * 255.f);
+
+    const u64 act = __ballot(true);
+    shaded = wave_sum(shaded);
+    groups = wave_sum(groups);
+    n_bil = wave_sum(n_bil);
+    n_near = wave_sum(n_near);
+    n_lm = wave_sum(n_lm);
     if (lane == __ffsll((long long)act) - 1) {
         const u32 sl = stat_slot();
         stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
@@ -1291,8 +1487,6 @@ struct OccLds {
     u32 insts[64];
 };
 
-DEV void wave_sync() { __builtin_amdgcn_wave_barrier(); }
-DEV u32 lanes_below(u64 m, int lane) { return (u32)__popcll(m & ((1ull << lane) - 1ull)); }
 
 /* origin of ray `ray` of a proxy: posa, posb or the midpoint (8222-8245) */
 DEV sftvec ray_origin(const ProxyGpu& p, int ray) {
@@ -1688,9 +1882,15 @@ void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEv
         hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
     hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles), dim3(kRasterThreads), 0, s, p);
+    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles, (u32)fc.samples), dim3(kRasterThreads), 0, s, p);
+    if (fc.samples > 1) hipLaunchKernelGGL(k_tile_lights, dim3((u32)fc.n_tiles), dim3(64), 0, s, p);
     hipEventRecord(ev[6], s);
-    hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
+    if (fc.samples > 1) {
+        hipLaunchKernelGGL(k_shade_ms, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
+    }
+    else {
+        hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
+    }
     hipEventRecord(ev[4], s);
     hipLaunchKernelGGL(k_stats, dim3(CNT_COUNT), dim3(64), 0, s, p.counters);
 }
