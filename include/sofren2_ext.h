@@ -109,6 +109,28 @@ const u32* sftx_device_color(void);     /* device pointer, width*height ARGB */
    are rejected with a message and leave the setting unchanged. */
 void sftx_set_msaa(i32 samples);
 
+/* ---- batched views ---------------------------------------------------------
+   Records several views with the ordinary calls and renders them with one
+   pass of the kernels over all of them (each view is exactly what sft_present
+   would have produced for it alone):
+
+     sftx_batch_begin();
+     for each view:
+         sftx_view_begin(width, height, samples);
+         sft_set_camera(...); sft_draw_*(...); lights ...
+         sftx_view_end(argb, pitch_bytes, vis64_or_null);
+     sftx_batch_submit();          blocking; outputs are written when it returns
+     sftx_batch_end();             releases the views' transient storage
+
+   vis64, when given, receives per sample (1/w bits << 32) | visibility ID
+   over the view's 64-px tile grid: [samples][tile_rows*64][tile_cols*64].
+   The display size set by sftx_view_begin stays in effect after the batch. */
+void sftx_batch_begin(void);
+void sftx_view_begin(i32 width, i32 height, i32 samples);
+i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64);
+void sftx_batch_submit(void);
+void sftx_batch_end(void);
+
 /* Waits for all GPU work of the renderer. */
 void sftx_synchronize(void);
 

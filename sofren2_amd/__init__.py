@@ -85,6 +85,15 @@ class Library:
         self.scene_render = _bind(s, "sftx_scene_render", None, [i32, vp, i32])
         self.scene_destroy = _bind(s, "sftx_scene_destroy", None, [])
         self.scene_tris = _bind(s, "sftx_scene_triangle_count", u32, [])
+        self.scene_new = _bind(s, "sftx_scene_new", i32, [i32, u32, i32, i32])
+        self.scene_use = _bind(s, "sftx_scene_use", i32, [i32])
+        self.scene_draw = _bind(s, "sftx_scene_draw", None, [i32, i32, i32, i32])
+        self.scene_free = _bind(s, "sftx_scene_free", None, [i32])
+        self.batch_begin = _bind(g, "sftx_batch_begin", None, [])
+        self.view_begin = _bind(g, "sftx_view_begin", None, [i32, i32, i32])
+        self.view_end = _bind(g, "sftx_view_end", i32, [vp, i32, vp])
+        self.batch_submit = _bind(g, "sftx_batch_submit", None, [])
+        self.batch_end = _bind(g, "sftx_batch_end", None, [])
         self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
 
 
@@ -96,6 +105,37 @@ def library():
     if _LIB is None:
         _LIB = Library()
     return _LIB
+
+
+class View:
+    """One view of a batch: a frame of a scene's camera path at a resolution
+    and MSAA setting, with its ARGB frame and 64-bit visibility planes."""
+
+    def __init__(self, scene_handle, frame, width, height, samples=1, vis64=True):
+        self.scene, self.frame, self.width, self.height, self.samples = scene_handle, frame, width, height, samples
+        self.color = np.zeros((height, width), dtype=np.uint32)
+        cols, rows = (width + 63) // 64, (height + 63) // 64
+        self.vis = np.zeros(cols * 64 * rows * 64 * samples, dtype=np.uint64) if vis64 else None
+
+    def vid(self):
+        return (self.vis & 0xFFFFFFFF).astype(np.uint32)
+
+    def depth(self):
+        return (self.vis >> 32).astype(np.uint32).view(np.float32)
+
+
+def render_batch(lib, views):
+    """Records every view and renders them with one submission."""
+    lib.batch_begin()
+    try:
+        for v in views:
+            lib.view_begin(v.width, v.height, v.samples)
+            lib.scene_draw(v.scene, v.frame, v.width, v.height)
+            lib.view_end(v.color.ctypes.data, v.width * 4, v.vis.ctypes.data if v.vis is not None else None)
+        lib.batch_submit()
+    finally:
+        lib.batch_end()
+    return views
 
 
 class SceneRenderer:

@@ -108,10 +108,10 @@ struct Backend {
     bool est_uploaded = false;
     DBuf blas;                      /* resident BLAS nodes of static models */
     u32 blas_used = 0;
-    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z;
+    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64;
 
     /* per frame */
-    DBuf fc_d, insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
+    DBuf insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
     DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
     DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth, counters;
     i32 output_mode = SFTX_OUTPUT_HOST;
@@ -149,12 +149,12 @@ void backend_destroy(Backend* b) {
     if (!b) return;
     HIP_CHECK(hipStreamSynchronize(b->stream));
     DBuf* all[] = { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->mesh_recs_d,
-                    &b->texels, &b->tex_meta_d, &b->est_rcp, &b->est_rsq, &b->fc_d, &b->insts, &b->work,
+                    &b->texels, &b->tex_meta_d, &b->est_rcp, &b->est_rsq, &b->insts, &b->work,
                     &b->bones, &b->plights, &b->dlights, &b->mat_tex, &b->mat_color, &b->mat_shin,
                     &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
                     &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
                     &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
-                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z };
+                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z, &b->vis64 };
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
     }
@@ -313,10 +313,20 @@ static f32 ev_ms(hipEvent_t a, hipEvent_t z) {
     return ms;
 }
 
-void backend_present(Backend* b, Context* cp) {
+/* frame-table rows of one view inside the batch */
+struct ViewPlan {
+    u32 inst_base = 0, work_base = 0, n_work = 0, pl_base = 0, dl_base = 0, tmat_base = 0;
+    u32 tile_base = 0, proxy_base = 0, tlas_base = 0, occ_base = 0, occ_count = 0;
+    u64 vid_base = 0, color_base = 0, grid = 0;
+    u32 n_tiles = 0;
+};
+
+void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     Context& c = *cp;
     hipStream_t s = b->stream;
     HIP_CHECK(hipEventRecord(b->ev[0], s));
+    const u32 nv = (u32)views.size();
+    if (0 == nv) return;
 
     if (!b->est_uploaded) {
         upload(b->est_rcp, c.est.rcp, s);
@@ -325,167 +335,258 @@ void backend_present(Backend* b, Context* cp) {
         b->est_uploaded = true;
     }
 
-    /* ---- residency: new resident models first, then this frame's transient ones ---- */
-    std::vector<const sft_model*> transient = c.transient_models;
-    auto is_transient = [&](const sft_model* m) {
-        return std::find(transient.begin(), transient.end(), m) != transient.end();
+    /* ---- residency: new resident models first, then each view's transient ones ---- */
+    auto is_transient = [](const ViewRecord& v, const sft_model* m) {
+        return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
     };
-    for (const sft_instance& in : c.instances) {
-        const sft_model* m = in.model;
-        if (is_transient(m)) continue;
-        auto it = b->models.find(m);
-        if (it != b->models.end() && it->second.total == m->total_meshlets) continue;
-        ModelRes r;
-        b->mesh_recs.resize(b->resident_mesh_recs);
-        upload_model(b, m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, true);
-        b->resident_slots += m->total_meshlets;
-        b->resident_mesh_recs = (u32)b->mesh_recs.size();
-        b->models[m] = std::move(r);
+    for (const ViewRecord& v : views) {
+        for (const sft_instance& in : v.instances) {
+            const sft_model* m = in.model;
+            if (is_transient(v, m)) continue;
+            auto it = b->models.find(m);
+            if (it != b->models.end() && it->second.total == m->total_meshlets) continue;
+            ModelRes r;
+            b->mesh_recs.resize(b->resident_mesh_recs);
+            upload_model(b, m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, true);
+            b->resident_slots += m->total_meshlets;
+            b->resident_mesh_recs = (u32)b->mesh_recs.size();
+            b->models[m] = std::move(r);
+        }
     }
     b->mesh_recs.resize(b->resident_mesh_recs);
     /* materials and lightmap pages are read at present time, as the reference
-       does: refresh the records of the resident models drawn this frame */
-    for (const sft_instance& in : c.instances) {
-        const sft_model* m = in.model;
-        if (is_transient(m)) continue;
-        const ModelRes& r = b->models[m];
-        for (u32 i = 0; i < m->mesh_count; i += 1) {
-            const sft_mesh* mesh = &m->meshes[i];
-            MeshRec& rec = b->mesh_recs[r.mesh_rec_base + i];
-            rec.material = mesh->material;
-            rec.lightmapped = (0xFFFFFFFFu != mesh->lightmap_page) ? 1u : 0u;
-            rec.lightmap_tex = rec.lightmapped && m->lightmaps ? m->lightmaps[mesh->lightmap_page] : SFT_INVALID_HANDLE;
+       does: refresh the records of the resident models drawn this batch */
+    for (const ViewRecord& v : views) {
+        for (const sft_instance& in : v.instances) {
+            const sft_model* m = in.model;
+            if (is_transient(v, m)) continue;
+            const ModelRes& r = b->models[m];
+            for (u32 i = 0; i < m->mesh_count; i += 1) {
+                const sft_mesh* mesh = &m->meshes[i];
+                MeshRec& rec = b->mesh_recs[r.mesh_rec_base + i];
+                rec.material = mesh->material;
+                rec.lightmapped = (0xFFFFFFFFu != mesh->lightmap_page) ? 1u : 0u;
+                rec.lightmap_tex = rec.lightmapped && m->lightmaps ? m->lightmaps[mesh->lightmap_page] : SFT_INVALID_HANDLE;
+            }
         }
     }
-    std::unordered_map<const sft_model*, ModelRes> tmodels;
+    std::vector<std::unordered_map<const sft_model*, ModelRes>> tmodels(nv);
     u32 tslot = b->resident_slots;
     const u32 saved_skins = b->resident_skins;
-    for (const sft_model* m : transient) {
-        ModelRes r;
-        upload_model(b, m, r, tslot, (u32)b->mesh_recs.size(), b->mesh_recs, false);
-        tslot += m->total_meshlets;
-        tmodels[m] = std::move(r);
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        for (const sft_model* m : views[vi].transient_models) {
+            ModelRes r;
+            upload_model(b, m, r, tslot, (u32)b->mesh_recs.size(), b->mesh_recs, false);
+            tslot += m->total_meshlets;
+            tmodels[vi][m] = std::move(r);
+        }
     }
     b->resident_skins = saved_skins;
     sync_textures(b, c);
 
-    /* ---- frame tables ---- */
+    /* ---- batch tables: views' rows back to back ---- */
     std::vector<InstGpu> insts;
     std::vector<WorkItem> work;
     std::vector<f32> bones;
-    insts.reserve(c.instances.size());
-    for (u32 ii = 0; ii < c.instances.size(); ii += 1) {
-        const sft_instance& in = c.instances[ii];
-        const sft_model* m = in.model;
-        auto ti = tmodels.find(m);
-        const ModelRes& r = (ti != tmodels.end()) ? ti->second : b->models[m];
-        InstGpu g;
-        memcpy(g.t, &in.transform, 64);
-        const sftmat inv = sft_m_qinv(in.transform);
-        memcpy(g.inv, &inv, 64);
-        g.model_base = r.base;
-        g.ss_offset = in.ss_meshlet_offset;
-        g.bone_base = kInvalid;
-        g.blas_base = r.blas_base;
-        g.blas_root = r.blas_root;
-        g.blas_count = r.blas_count;
-        g.n_meshlets = r.total;
-        g.pad = 0;
-        if (in.posed_bone_matrices) {
-            g.bone_base = (u32)(bones.size() / 16);
-            const u32 n = std::max(r.max_bone, 1u);
-            bones.insert(bones.end(), (const f32*)in.posed_bone_matrices, (const f32*)in.posed_bone_matrices + (size_t)n * 16);
+    std::vector<PointLightGpu> pl;
+    std::vector<DirLightGpu> dl;
+    std::vector<u32> tmat_tex, tmat_color;
+    std::vector<f32> tmat_shin;
+    std::vector<ProxyGpu> proxies;
+    std::vector<sft_bvh_node> tlas;
+    std::vector<ViewGpu> vgpu(nv);
+    std::vector<ViewPlan> plan(nv);
+    std::vector<u32> tile_view, shade_tiles, shade_tiles_ms;
+    u32 ss_total = 0, max_samples = 1, occ_total = 0;
+    u64 vid_total = 0, color_total = 0;
+    bool any_vis64 = false;
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        const ViewRecord& v = views[vi];
+        ViewPlan& vp = plan[vi];
+        vp.inst_base = (u32)insts.size();
+        vp.work_base = (u32)work.size();
+        vp.pl_base = (u32)pl.size();
+        vp.dl_base = (u32)dl.size();
+        vp.tmat_base = (u32)tmat_tex.size();
+        vp.proxy_base = (u32)proxies.size();
+        vp.tlas_base = (u32)tlas.size();
+        vp.tile_base = (u32)tile_view.size();
+        vp.n_tiles = (u32)(v.tile_cols * v.tile_rows);
+        vp.grid = (u64)v.tile_cols * 64 * v.tile_rows * 64 * (u64)v.samples;
+        vp.vid_base = vid_total;
+        vp.color_base = color_total;
+        vid_total += vp.grid;
+        color_total += (u64)v.width * v.height;
+        max_samples = std::max<u32>(max_samples, (u32)v.samples);
+        any_vis64 = any_vis64 || v.vis64;
+        for (u32 t = 0; t < vp.n_tiles; t += 1) {
+            tile_view.push_back(vi);
+            (v.samples > 1 ? shade_tiles_ms : shade_tiles).push_back(vp.tile_base + t);
         }
-        insts.push_back(g);
-        for (u32 mi = 0; mi < m->mesh_count; mi += 1) {
-            const sft_mesh* mesh = &m->meshes[mi];
-            for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
-                WorkItem w;
-                w.gm = mesh->model_meshlet_offset + k;
-                w.slot = r.base + w.gm;
-                w.ss = in.ss_meshlet_offset + w.gm;
-                w.inst = ii;
-                w.cull_bone = (mesh->culling_bone_inds && in.posed_bone_matrices) ? mesh->culling_bone_inds[k & ~7u] : -1;
-                w.skin = (mesh->meshlet_skins && in.posed_bone_matrices) ? r.skin_of[w.gm] : kInvalid;
-                w.pad0 = w.pad1 = 0;
-                work.push_back(w);
+        const u32 ss_base = ss_total;
+        ss_total += v.total_ss;
+
+        for (u32 ii = 0; ii < v.instances.size(); ii += 1) {
+            const sft_instance& in = v.instances[ii];
+            const sft_model* m = in.model;
+            auto ti = tmodels[vi].find(m);
+            const ModelRes& r = (ti != tmodels[vi].end()) ? ti->second : b->models[m];
+            InstGpu g;
+            memcpy(g.t, &in.transform, 64);
+            const sftmat inv = sft_m_qinv(in.transform);
+            memcpy(g.inv, &inv, 64);
+            g.model_base = r.base;
+            g.ss_offset = ss_base + in.ss_meshlet_offset;
+            g.bone_base = kInvalid;
+            g.blas_base = r.blas_base;
+            g.blas_root = r.blas_root;
+            g.blas_count = r.blas_count;
+            g.n_meshlets = r.total;
+            g.pad = 0;
+            if (in.posed_bone_matrices) {
+                g.bone_base = (u32)(bones.size() / 16);
+                const u32 n = std::max(r.max_bone, 1u);
+                /* a batched view kept the skeleton's matrices; joints beyond
+                   them (a model whose skin names more joints than its
+                   skeleton has) read identity */
+                const u32 kept = v.bones_kept.empty() ? n : std::min(n, v.bones_kept[ii]);
+                bones.insert(bones.end(), (const f32*)in.posed_bone_matrices, (const f32*)in.posed_bone_matrices + (size_t)kept * 16);
+                const sftmat id = sft_m_identity();
+                for (u32 k = kept; k < n; k += 1) bones.insert(bones.end(), (const f32*)&id, (const f32*)&id + 16);
+            }
+            const u32 ginst = (u32)insts.size();
+            insts.push_back(g);
+            for (u32 mi = 0; mi < m->mesh_count; mi += 1) {
+                const sft_mesh* mesh = &m->meshes[mi];
+                for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
+                    WorkItem w;
+                    w.gm = mesh->model_meshlet_offset + k;
+                    w.slot = r.base + w.gm;
+                    w.ss = ss_base + in.ss_meshlet_offset + w.gm;
+                    w.inst = ii;
+                    w.cull_bone = (mesh->culling_bone_inds && in.posed_bone_matrices) ? mesh->culling_bone_inds[k & ~7u] : -1;
+                    w.skin = (mesh->meshlet_skins && in.posed_bone_matrices) ? r.skin_of[w.gm] : kInvalid;
+                    w.view = vi;
+                    w.ginst = ginst;
+                    work.push_back(w);
+                }
             }
         }
+        vp.n_work = (u32)work.size() - vp.work_base;
+        for (const sft_point_light& L : v.point_lights) {
+            pl.push_back(PointLightGpu{{L.pos.x, L.pos.y, L.pos.z}, {L.color.x, L.color.y, L.color.z}, L.radius, L.inv_radius_sqr});
+        }
+        for (const sft_directional_light& L : v.dir_lights) {
+            dl.push_back(DirLightGpu{{L.dir.x, L.dir.y, L.dir.z}, {L.color.x, L.color.y, L.color.z}, {0.f, 0.f}});
+        }
+        tmat_tex.insert(tmat_tex.end(), v.tmat_tex.begin(), v.tmat_tex.end());
+        tmat_color.insert(tmat_color.end(), v.tmat_color.begin(), v.tmat_color.end());
+        tmat_shin.insert(tmat_shin.end(), v.tmat_shininess.begin(), v.tmat_shininess.end());
+        for (const WorldProxy& w : v.proxies) {
+            proxies.push_back(ProxyGpu{{w.posa.x, w.posa.y, w.posa.z}, {w.posb.x, w.posb.y, w.posb.z}, w.radius, 0.f});
+        }
+        /* frame TLAS when the occlusion pass runs (sofren2.c:8161-8164) */
+        const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
+        std::vector<sft_bvh_node> vt;
+        u32 tlas_root = 0;
+        if (occlusion) tlas_root = build_tlas(v.instances, vt);
+        tlas.insert(tlas.end(), vt.begin(), vt.end());
+
+        FrameConst& fc = vgpu[vi].fc;
+        memset(&fc, 0, sizeof(fc));
+        fc.width = v.width;
+        fc.height = v.height;
+        fc.screen_w = (i32)(v.half_width * 2.f);
+        fc.screen_h = (i32)(v.half_height * 2.f);
+        fc.tile_cols = v.tile_cols;
+        fc.tile_rows = v.tile_rows;
+        fc.n_tiles = v.tile_cols * v.tile_rows;
+        fc.samples = v.samples;
+        fc.hw = v.half_width;
+        fc.hh = v.half_height;
+        fc.vp_x = v.vp_x;
+        fc.vp_y = v.vp_y;
+        fc.bias_x = v.half_width + v.vp_x;
+        fc.bias_y = v.half_height + v.vp_y;
+        fc.neg_hh = -v.half_height;
+        memcpy(fc.vp, &v.view_proj, 64);
+        memcpy(fc.planes, v.planes, sizeof(fc.planes));
+        fc.cam[0] = v.camera_pos.x; fc.cam[1] = v.camera_pos.y; fc.cam[2] = v.camera_pos.z;
+        fc.ambient[0] = v.ambient.x; fc.ambient[1] = v.ambient.y; fc.ambient[2] = v.ambient.z;
+        fc.clear_color = v.clear_color;
+        fc.should_clear = v.should_clear;
+        fc.debug_mode = (u32)v.debug_mode;
+        fc.shadows_enabled = v.shadows_enabled;
+        fc.lm_brightness = v.lightmap_brightness;
+        fc.n_work = vp.n_work;
+        fc.n_inst = (u32)v.instances.size();
+        fc.n_point = (u32)v.point_lights.size();
+        fc.n_dir = (u32)v.dir_lights.size();
+        fc.n_proxies = (u32)v.proxies.size();
+        fc.base_tex = (c.base_mat != kInvalid) ? c.mat_tex[SFT_MAT_INDEX(c.base_mat)] : SFT_INVALID_HANDLE;
+        fc.rcp_bits = (u32)c.est.rcp_bits;
+        fc.rsq_bits = (u32)c.est.rsq_bits;
+        fc.capture = v.capture_flags;
+        fc.n_tex = (u32)c.tex.size();
+        fc.n_mat = (u32)c.mat_tex.size();
+        fc.n_tmat = (u32)v.tmat_tex.size();
+        fc.tlas_root = tlas_root;
+        fc.tlas_nodes = (u32)vt.size();
+        fc.occ_lights = (occlusion && !vt.empty()) ? (u32)std::min<size_t>(v.point_lights.size(), 64) : 0u;
+        fc.occ_batches = (u32)((v.proxies.size() + 7) / 8);
+        vp.occ_base = occ_total;
+        vp.occ_count = fc.occ_lights * fc.occ_batches;
+        occ_total += vp.occ_count;
+
+        ViewGpu& g = vgpu[vi];
+        g.inst_base = vp.inst_base;
+        g.work_base = vp.work_base;
+        g.n_work = vp.n_work;
+        g.pl_base = vp.pl_base;
+        g.dl_base = vp.dl_base;
+        g.tmat_base = vp.tmat_base;
+        g.tile_base = vp.tile_base;
+        g.proxy_base = vp.proxy_base;
+        g.tlas_base = vp.tlas_base;
+        g.occ_base = vp.occ_base;
+        g.occ_count = vp.occ_count;
+        g.pad0 = 0;
+        g.vid_base = vp.vid_base;
+        g.color_base = vp.color_base;
     }
     if (work.size() >= (1u << 23)) {
-        fprintf(stderr, "fatal sofren2 error: %zu meshlets in one frame exceed the 2^23 emission keys\n", work.size());
+        fprintf(stderr, "fatal sofren2 error: %zu meshlets in one batch exceed the 2^23 emission keys\n", work.size());
         exit(1);
     }
-    std::vector<PointLightGpu> pl(c.point_lights.size());
-    for (size_t i = 0; i < pl.size(); i += 1) {
-        const sft_point_light& L = c.point_lights[i];
-        pl[i] = PointLightGpu{{L.pos.x, L.pos.y, L.pos.z}, {L.color.x, L.color.y, L.color.z}, L.radius, L.inv_radius_sqr};
+    const u32 nt = (u32)tile_view.size();
+    const u32 n_shade1 = (u32)shade_tiles.size();
+    shade_tiles.insert(shade_tiles.end(), shade_tiles_ms.begin(), shade_tiles_ms.end());
+    /* proxy lists exist for the lights of views with shadows; the kernel skips the others */
+    u32 n_lights_cache = 0;
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        if (vgpu[vi].fc.shadows_enabled && vgpu[vi].fc.n_proxies > 0) n_lights_cache = (u32)pl.size();
     }
-    std::vector<DirLightGpu> dl(c.dir_lights.size());
-    for (size_t i = 0; i < dl.size(); i += 1) {
-        const sft_directional_light& L = c.dir_lights[i];
-        dl[i] = DirLightGpu{{L.dir.x, L.dir.y, L.dir.z}, {L.color.x, L.color.y, L.color.z}, {0.f, 0.f}};
-    }
-
-    /* shadow proxies, and the frame TLAS when the occlusion pass runs
-       (sofren2.c:8161-8164) */
-    std::vector<ProxyGpu> proxies(c.proxies.size());
-    for (size_t i = 0; i < proxies.size(); i += 1) {
-        const WorldProxy& w = c.proxies[i];
-        proxies[i] = ProxyGpu{{w.posa.x, w.posa.y, w.posa.z}, {w.posb.x, w.posb.y, w.posb.z}, w.radius, 0.f};
-    }
-    const bool occlusion = c.shadows_enabled && !pl.empty() && !proxies.empty();
-    std::vector<sft_bvh_node> tlas;
-    u32 tlas_root = 0;
-    if (occlusion) tlas_root = build_tlas(c.instances, tlas);
-
-    FrameConst fc;
-    memset(&fc, 0, sizeof(fc));
-    fc.width = c.width;
-    fc.height = c.height;
-    fc.screen_w = (i32)(c.half_width * 2.f);
-    fc.screen_h = (i32)(c.half_height * 2.f);
-    fc.tile_cols = c.tile_cols;
-    fc.tile_rows = c.tile_rows;
-    fc.n_tiles = c.tile_cols * c.tile_rows;
-    fc.samples = c.samples;
-    fc.hw = c.half_width;
-    fc.hh = c.half_height;
-    fc.vp_x = c.vp_x;
-    fc.vp_y = c.vp_y;
-    fc.bias_x = c.half_width + c.vp_x;
-    fc.bias_y = c.half_height + c.vp_y;
-    fc.neg_hh = -c.half_height;
-    memcpy(fc.vp, &c.view_proj, 64);
-    memcpy(fc.planes, c.planes, sizeof(fc.planes));
-    fc.cam[0] = c.camera_pos.x; fc.cam[1] = c.camera_pos.y; fc.cam[2] = c.camera_pos.z;
-    fc.ambient[0] = c.ambient.x; fc.ambient[1] = c.ambient.y; fc.ambient[2] = c.ambient.z;
-    fc.clear_color = c.clear_color;
-    fc.should_clear = c.should_clear;
-    fc.debug_mode = (u32)c.debug_mode;
-    fc.shadows_enabled = c.shadows_enabled;
-    fc.lm_brightness = c.lightmap_brightness;
-    fc.n_work = (u32)work.size();
-    fc.n_inst = (u32)insts.size();
-    fc.n_point = (u32)pl.size();
-    fc.n_dir = (u32)dl.size();
-    fc.n_proxies = (u32)c.proxies.size();
-    fc.base_tex = (c.base_mat != kInvalid) ? c.mat_tex[SFT_MAT_INDEX(c.base_mat)] : SFT_INVALID_HANDLE;
-    fc.rcp_bits = (u32)c.est.rcp_bits;
-    fc.rsq_bits = (u32)c.est.rsq_bits;
-    fc.capture = c.capture_flags;
-    fc.n_tex = (u32)c.tex.size();
-    fc.n_mat = (u32)c.mat_tex.size();
-    fc.n_tmat = (u32)c.tmat_tex.size();
-    fc.tlas_root = tlas_root;
-    fc.tlas_nodes = (u32)tlas.size();
-    fc.occ_lights = (occlusion && !tlas.empty()) ? (u32)std::min<size_t>(pl.size(), 64) : 0u;
-    fc.occ_batches = (u32)((proxies.size() + 7) / 8);
+    BatchGpu bg;
+    memset(&bg, 0, sizeof(bg));
+    bg.n_views = nv;
+    bg.n_work = (u32)work.size();
+    bg.n_tiles = nt;
+    bg.max_samples = max_samples;
+    bg.rcp_bits = (u32)c.est.rcp_bits;
+    bg.rsq_bits = (u32)c.est.rsq_bits;
+    bg.n_shade_tiles = n_shade1;
+    bg.n_shade_tiles_ms = (u32)shade_tiles_ms.size();
+    bg.occ_total = occ_total;
+    bg.n_lights = n_lights_cache;
+    bg.vis64 = any_vis64 ? 1u : 0u;
 
     /* ---- uploads ---- */
     Packer& pk = b->pack;
     pk.used = 0;
-    const size_t o_fc = pk.add(&fc, sizeof(fc));
+    const size_t o_batch = pk.add(&bg, sizeof(bg));
+    const size_t o_views = pk.add(vgpu);
+    const size_t o_tv = pk.add(tile_view);
+    const size_t o_st = pk.add(shade_tiles);
     const size_t o_insts = pk.add(insts);
     const size_t o_work = pk.add(work);
     const size_t o_bones = pk.add(bones);
@@ -494,18 +595,19 @@ void backend_present(Backend* b, Context* cp) {
     const size_t o_mt = pk.add(c.mat_tex);
     const size_t o_mc = pk.add(c.mat_color);
     const size_t o_ms = pk.add(c.mat_shininess);
-    const size_t o_tt = pk.add(c.tmat_tex);
-    const size_t o_tc = pk.add(c.tmat_color);
-    const size_t o_ts = pk.add(c.tmat_shininess);
+    const size_t o_tt = pk.add(tmat_tex);
+    const size_t o_tc = pk.add(tmat_color);
+    const size_t o_ts = pk.add(tmat_shin);
     const size_t o_mr = pk.add(b->mesh_recs);
     const size_t o_px = pk.add(proxies);
     const size_t o_tlas = pk.add(tlas);
 
-    const u32 nw = fc.n_work;
-    const u32 nt = (u32)fc.n_tiles;
-    const size_t px = (size_t)c.width * c.height;
-    dgrow(b->ss, std::max<size_t>((size_t)c.total_ss * 2304, 4), false, s);
-    dgrow(b->ss_vis, std::max<size_t>(c.total_ss, 4), false, s);
+    const u32 nw = bg.n_work;
+    const bool single = 1 == nv;
+    const u32 cap_flags = single ? views[0].capture_flags : 0u;
+    const bool cap_vis = 0 != (cap_flags & SFTX_CAPTURE_VISIBILITY);
+    dgrow(b->ss, std::max<size_t>((size_t)ss_total * 2304, 4), false, s);
+    dgrow(b->ss_vis, std::max<size_t>(ss_total, 4), false, s);
     dgrow(b->clip_count, std::max<size_t>((size_t)nw * 4, 4), false, s);
     dgrow(b->clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
     dgrow(b->clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
@@ -514,24 +616,27 @@ void backend_present(Backend* b, Context* cp) {
     const size_t zero_words = COUNTER_WORDS + 2 * (size_t)nt;
     dgrow(b->zero_d, zero_words * 4, false, s);
     if (!b->counters_h) HIP_CHECK(hipHostMalloc((void**)&b->counters_h, CNT_COUNT * 4, hipHostMallocDefault));
-    dgrow(b->color, std::max<size_t>(px * 4, 4), false, s);
-    const bool cap_vis = 0 != (c.capture_flags & SFTX_CAPTURE_VISIBILITY);
-    const size_t grid = (size_t)c.tile_cols * 64 * c.tile_rows * 64 * (size_t)c.samples;
-    dgrow(b->vid, grid * 4, false, s);
-    dgrow(b->tile_z, (size_t)nt * c.samples * 8, false, s);
+    dgrow(b->color, std::max<size_t>(color_total * 4, 4), false, s);
+    dgrow(b->vid, std::max<size_t>(vid_total * 4, 4), false, s);
+    dgrow(b->tile_z, (size_t)nt * max_samples * 8, false, s);
     dgrow(b->tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
-    if (cap_vis) dgrow(b->cap_depth, grid * 4, false, s);
+    if (cap_vis) dgrow(b->cap_depth, vid_total * 4, false, s);
+    if (any_vis64) dgrow(b->vis64, vid_total * 8, false, s);
     dgrow(b->proxy_vis, std::max<size_t>(proxies.size() * 64, 64), false, s);
     dgrow(b->light_proxy_count, std::max<size_t>(pl.size() * 4, 4), false, s);
     dgrow(b->light_proxies, std::max<size_t>(pl.size() * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)), false, s);
-    if (occlusion && tlas.empty()) {
-        /* no instance to occlude: every ray reaches its light */
-        HIP_CHECK(hipMemsetAsync(b->proxy_vis.p, 3, proxies.size() * 64, s));
-    }
-    const bool to_host = SFTX_OUTPUT_HOST == b->output_mode && c.color_buffer;
-    if (!c.should_clear && to_host && px) {
-        HIP_CHECK(hipMemcpy2DAsync(b->color.p, (size_t)c.width * 4, c.color_buffer, (size_t)c.color_pitch * 4,
-                                   (size_t)c.width * 4, c.height, hipMemcpyHostToDevice, s));
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        const ViewRecord& v = views[vi];
+        const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
+        if (occlusion && 0 == vgpu[vi].fc.tlas_nodes) {
+            /* no instance to occlude: every ray reaches its light */
+            HIP_CHECK(hipMemsetAsync((u8*)b->proxy_vis.p + (size_t)plan[vi].proxy_base * 64, 3, v.proxies.size() * 64, s));
+        }
+        /* a view without clearing starts from the caller's frame */
+        if (!v.should_clear && v.color_buffer && v.width > 0 && v.height > 0) {
+            HIP_CHECK(hipMemcpy2DAsync((u32*)b->color.p + plan[vi].color_base, (size_t)v.width * 4, v.color_buffer,
+                                       (size_t)v.color_pitch * 4, (size_t)v.width * 4, v.height, hipMemcpyHostToDevice, s));
+        }
     }
 
     u32 counters[CNT_COUNT];
@@ -540,9 +645,14 @@ void backend_present(Backend* b, Context* cp) {
         dgrow(b->clip_rec, (size_t)b->clip_cap * 164, false, s);
         dgrow(b->clip_key, (size_t)b->clip_cap * 4, false, s);
         dgrow(b->clip_rect, (size_t)b->clip_cap * 4, false, s);
-        fc.clip_cap = b->clip_cap;
-        fc.tile_list_cap = b->tile_list_cap;
-        memcpy(pk.host + o_fc, &fc, sizeof(fc));
+        bg.clip_cap = b->clip_cap;
+        bg.tile_list_cap = b->tile_list_cap;
+        memcpy(pk.host + o_batch, &bg, sizeof(bg));
+        for (u32 vi = 0; vi < nv; vi += 1) {
+            vgpu[vi].fc.clip_cap = b->clip_cap;
+            vgpu[vi].fc.tile_list_cap = b->tile_list_cap;
+        }
+        memcpy(pk.host + o_views, vgpu.data(), vgpu.size() * sizeof(ViewGpu));
         dgrow(b->pack_d, pk.used, false, s);
         HIP_CHECK(hipMemcpyAsync(b->pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipMemsetAsync(b->zero_d.p, 0, zero_words * 4, s));
@@ -560,7 +670,10 @@ void backend_present(Backend* b, Context* cp) {
         p.tex_meta = (const TexMeta*)b->tex_meta_d.p;
         p.est_rcp = (const f32*)b->est_rcp.p;
         p.est_rsq = (const f32*)b->est_rsq.p;
-        p.fc = (const FrameConst*)(pd + o_fc);
+        p.batch = (const BatchGpu*)(pd + o_batch);
+        p.views = (const ViewGpu*)(pd + o_views);
+        p.tile_view = (const u32*)(pd + o_tv);
+        p.shade_tiles = (const u32*)(pd + o_st);
         p.insts = (const InstGpu*)(pd + o_insts);
         p.work = (const WorkItem*)(pd + o_work);
         p.bones = (const f32*)(pd + o_bones);
@@ -591,15 +704,16 @@ void backend_present(Backend* b, Context* cp) {
         p.tile_lights = (u32*)b->tile_lights.p;
         p.tile_z = (f32*)b->tile_z.p;
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
+        p.vis64 = any_vis64 ? (u64*)b->vis64.p : nullptr;
+        p.counters = zd;
         p.blas = (const sft_bvh_node*)b->blas.p;
         p.tlas = (const sft_bvh_node*)(pd + o_tlas);
         p.proxies = (const ProxyGpu*)(pd + o_px);
         p.proxy_vis = (u8*)b->proxy_vis.p;
         p.light_proxy_count = (u32*)b->light_proxy_count.p;
         p.light_proxies = (ProxyRec*)b->light_proxies.p;
-        p.counters = zd;
 
-        launch_frame(p, fc, s, b->ev);
+        launch_frame(p, bg, s, b->ev);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(b->counters_h, zd, sizeof(counters), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
@@ -613,22 +727,39 @@ void backend_present(Backend* b, Context* cp) {
         if (counters[CNT_OVERFLOW] & 2u) b->tile_list_cap = std::max(b->tile_list_cap * 2u, counters[CNT_TILE_PAIRS] + 4096u);
     }
 
-    if (to_host && px) {
-        if (b->color_h_cap < px * 4) {
+    /* ---- outputs: one copy of the colour planes (and 64-bit visibility) into pinned staging ---- */
+    bool any_host = false;
+    for (const ViewRecord& v : views) any_host = any_host || (SFTX_OUTPUT_HOST == b->output_mode && v.color_buffer) || v.vis64;
+    size_t stage_bytes = 0;
+    const size_t color_bytes = (SFTX_OUTPUT_HOST == b->output_mode) ? (size_t)color_total * 4 : 0;
+    if (any_host) stage_bytes = color_bytes + (any_vis64 ? (size_t)vid_total * 8 : 0);
+    if (stage_bytes) {
+        if (b->color_h_cap < stage_bytes) {
             if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
-            HIP_CHECK(hipHostMalloc((void**)&b->color_h, px * 4, hipHostMallocDefault));
-            b->color_h_cap = px * 4;
+            HIP_CHECK(hipHostMalloc((void**)&b->color_h, stage_bytes, hipHostMallocDefault));
+            b->color_h_cap = stage_bytes;
         }
-        HIP_CHECK(hipMemcpyAsync(b->color_h, b->color.p, px * 4, hipMemcpyDeviceToHost, s));
+        if (color_bytes) HIP_CHECK(hipMemcpyAsync(b->color_h, b->color.p, color_bytes, hipMemcpyDeviceToHost, s));
+        if (any_vis64) HIP_CHECK(hipMemcpyAsync((u8*)b->color_h + color_bytes, b->vis64.p, (size_t)vid_total * 8, hipMemcpyDeviceToHost, s));
     }
     HIP_CHECK(hipEventRecord(b->ev[5], s));
     HIP_CHECK(hipStreamSynchronize(s));
-    if (to_host && px) {
-        for (i32 y = 0; y < c.height; y += 1) {
-            memcpy(c.color_buffer + (size_t)y * c.color_pitch, b->color_h + (size_t)y * c.width, (size_t)c.width * 4);
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        const ViewRecord& v = views[vi];
+        if (color_bytes && v.color_buffer) {
+            for (i32 y = 0; y < v.height; y += 1) {
+                memcpy(v.color_buffer + (size_t)y * v.color_pitch, b->color_h + plan[vi].color_base + (size_t)y * v.width,
+                       (size_t)v.width * 4);
+            }
         }
+        if (v.vis64) memcpy(v.vis64, (u8*)b->color_h + color_bytes + plan[vi].vid_base * 8, plan[vi].grid * 8);
     }
 
+    u64 px = 0, samples = 0;
+    for (const ViewRecord& v : views) {
+        px += (u64)v.width * v.height;
+        samples += (u64)v.width * v.height * (u64)v.samples;
+    }
     sftx_frame_stats& st = c.stats;
     memset(&st, 0, sizeof(st));
     st.meshlets_tested = nw;
@@ -638,7 +769,7 @@ void backend_present(Backend* b, Context* cp) {
     st.clip_triangles = counters[CNT_CLIP_TRIS];
     st.tile_payloads = counters[CNT_TILE_PAIRS];
     st.tiles = nt;
-    st.samples = (u32)(px * (size_t)c.samples);
+    st.samples = (u32)samples;
     st.pixels = (u32)px;
     st.shaded_pixels = counters[CNT_SHADED];
     st.shading_groups = counters[CNT_QGROUPS];
@@ -661,36 +792,40 @@ void backend_present(Backend* b, Context* cp) {
     st.ms_download = ev_ms(b->ev[4], b->ev[5]);
     st.ms_total = ev_ms(b->ev[0], b->ev[5]);
 
-    /* ---- capture ---- */
-    b->cap_cols = c.tile_cols;
-    b->cap_rows = c.tile_rows;
-    b->cap_samples = c.samples;
+    /* ---- capture (single-view presents) ---- */
+    if (!single) return;
+    const ViewRecord& v0 = views[0];
+    b->cap_cols = v0.tile_cols;
+    b->cap_rows = v0.tile_rows;
+    b->cap_samples = v0.samples;
+    const size_t grid = plan[0].grid;
     if (cap_vis) {
         b->h_vid.resize(grid);
         b->h_depth.resize(grid);
         HIP_CHECK(hipMemcpy(b->h_vid.data(), b->vid.p, grid * 4, hipMemcpyDeviceToHost));
         HIP_CHECK(hipMemcpy(b->h_depth.data(), b->cap_depth.p, grid * 4, hipMemcpyDeviceToHost));
     }
-    if (c.capture_flags & SFTX_CAPTURE_SS_CACHE) {
-        b->h_ss_count = c.total_ss;
-        b->h_ss.resize((size_t)c.total_ss * 2304);
-        b->h_ss_vis.assign(c.total_ss, 0);
-        if (c.total_ss) {
+    if (cap_flags & SFTX_CAPTURE_SS_CACHE) {
+        b->h_ss_count = v0.total_ss;
+        b->h_ss.resize((size_t)v0.total_ss * 2304);
+        b->h_ss_vis.assign(v0.total_ss, 0);
+        if (v0.total_ss) {
             HIP_CHECK(hipMemcpy(b->h_ss.data(), b->ss.p, b->h_ss.size(), hipMemcpyDeviceToHost));
-            HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), b->ss_vis.p, c.total_ss, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), b->ss_vis.p, v0.total_ss, hipMemcpyDeviceToHost));
         }
     }
-    if (c.capture_flags & SFTX_CAPTURE_SHADOWS) {
+    if (cap_flags & SFTX_CAPTURE_SHADOWS) {
+        const bool occlusion = v0.shadows_enabled && !v0.point_lights.empty() && !v0.proxies.empty();
         b->h_proxy_count = (u32)proxies.size();
         b->h_proxy_vis.assign(proxies.size() * 64, 0);
         if (occlusion) {
             HIP_CHECK(hipMemcpy(b->h_proxy_vis.data(), b->proxy_vis.p, b->h_proxy_vis.size(), hipMemcpyDeviceToHost));
             /* counts of lights beyond the traced ones are not kept */
-            const size_t lc = std::min<size_t>(pl.size(), 64);
+            const size_t lc = std::min<size_t>(v0.point_lights.size(), 64);
             for (size_t i = 0; i < proxies.size(); i += 1) memset(&b->h_proxy_vis[i * 64 + lc], 0, 64 - lc);
         }
     }
-    if (c.capture_flags & SFTX_CAPTURE_CLIPS) {
+    if (cap_flags & SFTX_CAPTURE_CLIPS) {
         b->h_clip_count = std::min(counters[CNT_CLIP_TRIS], b->clip_cap);
         b->h_clips.resize((size_t)b->h_clip_count * 164);
         if (b->h_clip_count) HIP_CHECK(hipMemcpy(b->h_clips.data(), b->clip_rec.p, b->h_clips.size(), hipMemcpyDeviceToHost));

@@ -388,10 +388,129 @@ extern "C" void sft_frame_end(void) {
     c.proxies.clear();
 }
 
+/* the frame recorded since sft_frame_begin (or sftx_view_begin) as one view */
+static void snapshot_view(const Context& c, ViewRecord& v) {
+    v.width = c.width;
+    v.height = c.height;
+    v.tile_cols = c.tile_cols;
+    v.tile_rows = c.tile_rows;
+    v.samples = c.samples;
+    v.half_width = c.half_width;
+    v.half_height = c.half_height;
+    v.vp_x = c.vp_x;
+    v.vp_y = c.vp_y;
+    v.color_buffer = c.color_buffer;
+    v.color_pitch = c.color_pitch;
+    v.vis64 = nullptr;
+    v.should_clear = c.should_clear;
+    v.clear_color = c.clear_color;
+    v.debug_mode = (i32)c.debug_mode;
+    v.camera_pos = c.camera_pos;
+    v.view_proj = c.view_proj;
+    memcpy(v.planes, c.planes, sizeof(v.planes));
+    v.ambient = c.ambient;
+    v.dir_lights = c.dir_lights;
+    v.point_lights = c.point_lights;
+    v.lightmap_brightness = c.lightmap_brightness;
+    v.shadows_enabled = c.shadows_enabled;
+    v.proxies = c.proxies;
+    v.instances = c.instances;
+    v.total_ss = c.total_ss;
+    v.transient_models = c.transient_models;
+    v.tmat_tex = c.tmat_tex;
+    v.tmat_color = c.tmat_color;
+    v.tmat_shininess = c.tmat_shininess;
+    v.capture_flags = c.capture_flags;
+}
+
 extern "C" void sft_present(void) {
     Context& c = C();
+    if (c.batch_open) {
+        fprintf(stderr, "sofren2: sft_present inside a batch; use sftx_view_end and sftx_batch_submit\n");
+        return;
+    }
     if (!c.backend) c.backend = backend_create();   /* exits if there is no GPU */
-    backend_present(c.backend, &c);
+    std::vector<ViewRecord> one(1);
+    snapshot_view(c, one[0]);
+    backend_render(c.backend, &c, one);
+}
+
+/* ================================================================ batches
+   Several views recorded with the ordinary calls and rendered by one pass of
+   the kernels over all of them.  Transient storage (dmesh geometry, transient
+   materials) of every view lives until sftx_batch_end. */
+extern "C" void sftx_batch_begin(void) {
+    Context& c = C();
+    sft_frame_begin();
+    c.batch_views.clear();
+    c.batch_open = true;
+}
+
+extern "C" void sftx_view_begin(i32 width, i32 height, i32 samples) {
+    Context& c = C();
+    if (!c.batch_open) {
+        fprintf(stderr, "sofren2: sftx_view_begin outside sftx_batch_begin/sftx_batch_end\n");
+        return;
+    }
+    set_display(c, width, height);
+    sftx_set_msaa(samples);
+    c.instance_count = 0;
+    c.instances.clear();
+    c.total_ss = 0;
+    c.dir_lights.clear();
+    c.point_lights.clear();
+    c.proxies.clear();
+    c.transient_models.clear();
+    c.tmat_tex.clear();
+    c.tmat_color.clear();
+    c.tmat_shininess.clear();
+}
+
+extern "C" i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64) {
+    Context& c = C();
+    if (!c.batch_open) {
+        fprintf(stderr, "sofren2: sftx_view_end outside sftx_batch_begin/sftx_batch_end\n");
+        return -1;
+    }
+    c.batch_views.emplace_back();
+    ViewRecord& v = c.batch_views.back();
+    snapshot_view(c, v);
+    v.color_buffer = color;
+    v.color_pitch = pitch_bytes / 4;
+    v.vis64 = vis64;
+    v.capture_flags = 0;
+    /* the view is rendered later: keep the poses as they are now */
+    size_t nb = 0;
+    for (const sft_instance& in : v.instances) {
+        if (in.posed_bone_matrices) nb += (size_t)std::max(in.model->skeleton.bone_count, 1);
+    }
+    v.bone_store.resize(nb);
+    v.bones_kept.assign(v.instances.size(), 0);
+    size_t at = 0;
+    for (size_t i = 0; i < v.instances.size(); i += 1) {
+        sft_instance& in = v.instances[i];
+        if (!in.posed_bone_matrices) continue;
+        const size_t n = (size_t)std::max(in.model->skeleton.bone_count, 1);
+        memcpy(&v.bone_store[at], in.posed_bone_matrices, n * sizeof(sftmat));
+        in.posed_bone_matrices = &v.bone_store[at];
+        v.bones_kept[i] = (u32)n;
+        at += n;
+    }
+    return (i32)c.batch_views.size() - 1;
+}
+
+extern "C" void sftx_batch_submit(void) {
+    Context& c = C();
+    if (!c.batch_open || c.batch_views.empty()) return;
+    if (!c.backend) c.backend = backend_create();
+    backend_render(c.backend, &c, c.batch_views);
+}
+
+extern "C" void sftx_batch_end(void) {
+    Context& c = C();
+    c.batch_views.clear();
+    c.batch_open = false;
+    sft_frame_end();
 }
 
 /* ================================================================ misc setters */

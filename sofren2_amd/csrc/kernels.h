@@ -51,8 +51,11 @@ struct FramePtrs {
     const TexMeta* tex_meta;
     const f32* est_rcp;
     const f32* est_rsq;
-    /* per frame inputs */
-    const FrameConst* fc;
+    /* per batch inputs */
+    const BatchGpu* batch;
+    const ViewGpu* views;
+    const u32* tile_view;   /* [n_tiles] view of each batch-wide tile */
+    const u32* shade_tiles; /* tiles shaded without MSAA, then tiles with MSAA */
     const InstGpu* insts;
     const WorkItem* work;
     const f32* bones;       /* [n][16] */
@@ -90,10 +93,13 @@ struct FramePtrs {
     u32* light_proxy_count;     /* [n_point] */
     ProxyRec* light_proxies;    /* [n_point][16] */
     f32* cap_depth;         /* [samples][grid] 1/w when capturing, else null */
+    u64* vis64;             /* [samples][grid] (1/w bits << 32) | ID when requested, else null */
     u32* counters;          /* [CNT_COUNT] */
 };
 
-void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev);
+/* enqueues the kernels of one batch of views; `ev` (10 events) brackets the
+   passes when non-null */
+void launch_frame(const FramePtrs& p, const BatchGpu& batch, hipStream_t s, hipEvent_t* ev);
 void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s);
 
 }  // namespace sft

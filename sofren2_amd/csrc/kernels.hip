@@ -117,13 +117,31 @@ struct Est {
     }
 };
 
-DEV Est make_est(const FramePtrs& P, const FrameConst& fc) {
+DEV Est make_est(const FramePtrs& P) {
     Est e;
     e.rcp = P.est_rcp;
     e.rsq = P.est_rsq;
-    e.rcp_bits = fc.rcp_bits;
-    e.rsq_bits = fc.rsq_bits;
+    e.rcp_bits = P.batch->rcp_bits;
+    e.rsq_bits = P.batch->rsq_bits;
     return e;
+}
+
+/* first clip slot of a view: clip slots are numbered batch-wide in work order */
+DEV u32 view_clip_base(const FramePtrs& P, const ViewGpu& V) {
+    return V.n_work ? P.clip_base[V.work_base] : 0u;
+}
+
+/* view owning batch-wide row `i` of a table whose per-view starts are given
+   by `start(view)`, increasing (binary search over the views) */
+template <class F>
+DEV u32 find_view(const FramePtrs& P, u32 i, F start) {
+    u32 lo = 0, hi = P.batch->n_views;
+    while (hi - lo > 1u) {
+        const u32 mid = (lo + hi) >> 1;
+        if (start(P.views[mid]) <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 /* adds to a statistics counter through one of its spread slots */
@@ -148,14 +166,14 @@ DEV i32 wave_sum(i32 v) {
 
 /* ================================================================== k_geom */
 __global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const u32 wi = blockIdx.x * 4u + (u32)wave;
-    if (wi >= fc.n_work) return;
-    const Est est = make_est(P, fc);
+    if (wi >= P.batch->n_work) return;
+    const Est est = make_est(P);
 
     const WorkItem w = P.work[wi];
-    const InstGpu* inst = &P.insts[w.inst];
+    const FrameConst& fc = P.views[w.view].fc;
+    const InstGpu* inst = &P.insts[w.ginst];
     f32 T[16];
     for (int i = 0; i < 16; i += 1) T[i] = inst->t[i];
 
@@ -410,12 +428,13 @@ DEV ClipVert load_cv(const FrameConst& fc, const f32* ss, const f32* ap, int i) 
 }
 
 __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const u32 wi = blockIdx.x * 4u + (u32)wave;
-    if (wi >= fc.n_work) return;
+    if (wi >= P.batch->n_work) return;
     if (0u == P.clip_count[wi]) return;
     const WorkItem w = P.work[wi];
+    const FrameConst& fc = P.views[w.view].fc;
+    const u32 clip_cap = P.batch->clip_cap;
     const f32* ss = P.ss + (u64)w.ss * 576u;
     const f32* ap = P.attr + (u64)w.slot * 448u;
     const u8* ib = P.inds + (u64)w.slot * 512u;
@@ -473,7 +492,7 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
         }
         for (int c = 0; c < nout; c += 1) {
             const u32 slot = first + (u32)c;
-            if (slot >= fc.clip_cap) {
+            if (slot >= clip_cap) {
                 atomicOr(&P.counters[CNT_OVERFLOW], 1u);
                 continue;
             }
@@ -530,54 +549,76 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
    so each block aggregates its payloads per tile in LDS and touches the global
    tile counters once per tile it covers. */
 constexpr u32 kBinChunk = 1024;          /* payload items per block */
-constexpr u32 kBinLdsTiles = 8192;       /* tiles aggregated in LDS (8K frames) */
+constexpr u32 kBinLdsTiles = 8192;       /* tiles aggregated in LDS */
 
-DEV bool bin_item(const FramePtrs& P, u32 nb, u32 i, u32& rect, u64& desc) {
+/* payload item i: a batch slot (work item * 16 + batch) or a clip slot after
+   them; returns its tile rectangle, descriptor and view */
+DEV bool bin_item(const FramePtrs& P, u32 nb, u32 i, u32& rect, u64& desc, u32& view) {
     if (i < nb) {
         const uint2 pl = P.batch_pl[i];
         if (0u == pl.x) return false;
         rect = pl.y;
         desc = ((u64)pl.x << 32) | (u64)i;
+        view = P.work[i >> 4].view;
         return true;
     }
     const u32 slot = i - nb;
     rect = P.clip_rect[slot];
     if (kInvalid == rect) return false;
     desc = (u64)slot;
+    view = P.work[P.clip_key[slot] >> 9].view;
     return true;
+}
+
+/* views of a block's items are contiguous: batch slots and clip slots both
+   follow work order */
+DEV u32 item_view(const FramePtrs& P, u32 nb, u32 i) {
+    return (i < nb) ? P.work[i >> 4].view : P.work[P.clip_key[i - nb] >> 9].view;
 }
 
 /* scatter == false: count payloads per tile; scatter == true: write them */
 template <bool scatter>
 __global__ __launch_bounds__(256) void k_bin(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
+    const BatchGpu& B = *P.batch;
     __shared__ u32 s_t[kBinLdsTiles];
-    const u32 nb = fc.n_work * 16u;
-    const u32 nclip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
+    __shared__ u32 s_range[2];
+    const u32 nb = B.n_work * 16u;
+    const u32 nclip = min(P.counters[CNT_CLIP_TRIS], B.clip_cap);
     const u32 total = nb + nclip;
     const u32 base = blockIdx.x * kBinChunk;
     if (base >= total) return;
-    const u32 nt = (u32)fc.n_tiles;
-    const bool lds = nt <= kBinLdsTiles;
+    const u32 end = min(base + kBinChunk, total);
+    /* batch-wide tiles this block can touch: those of the views of its first
+       and last items and every view between */
+    if (0 == threadIdx.x) {
+        const ViewGpu& v0 = P.views[item_view(P, nb, base)];
+        const ViewGpu& v1 = P.views[item_view(P, nb, end - 1u)];
+        s_range[0] = v0.tile_base;
+        s_range[1] = v1.tile_base + (u32)v1.fc.n_tiles;
+    }
+    __syncthreads();
+    const u32 t_lo = s_range[0], t_n = s_range[1] - s_range[0];
+    const bool lds = t_n <= kBinLdsTiles;
     u32* gcount = scatter ? P.tile_fill : P.tile_count;
+    const u32 cap = B.tile_list_cap;
     if (lds) {
-        for (u32 t = threadIdx.x; t < nt; t += 256u) s_t[t] = 0u;
+        for (u32 t = threadIdx.x; t < t_n; t += 256u) s_t[t] = 0u;
         __syncthreads();
     }
-    const u32 end = min(base + kBinChunk, total);
     for (u32 i = base + threadIdx.x; i < end; i += 256u) {
-        u32 rect;
+        u32 rect, view;
         u64 desc;
-        if (!bin_item(P, nb, i, rect, desc)) continue;
+        if (!bin_item(P, nb, i, rect, desc, view)) continue;
+        const ViewGpu& V = P.views[view];
         const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
         for (i32 ty = y0; ty <= y1; ty += 1) {
             for (i32 tx = x0; tx <= x1; tx += 1) {
-                const u32 t = (u32)(ty * fc.tile_cols + tx);
-                if (lds) atomicAdd(&s_t[t], 1u);
+                const u32 t = V.tile_base + (u32)(ty * V.fc.tile_cols + tx);
+                if (lds) atomicAdd(&s_t[t - t_lo], 1u);
                 else if (!scatter) atomicAdd(&gcount[t], 1u);
                 else {
                     const u32 pos = P.tile_off[t] + atomicAdd(&gcount[t], 1u);
-                    if (pos < fc.tile_list_cap) P.tile_list[pos] = desc;
+                    if (pos < cap) P.tile_list[pos] = desc;
                 }
             }
         }
@@ -585,24 +626,26 @@ __global__ __launch_bounds__(256) void k_bin(FramePtrs P) {
     if (!lds) return;
     __syncthreads();
     /* one global reservation per touched tile */
-    for (u32 t = threadIdx.x; t < nt; t += 256u) {
+    for (u32 t = threadIdx.x; t < t_n; t += 256u) {
         const u32 c = s_t[t];
         if (c) {
-            const u32 at = atomicAdd(&gcount[t], c);
-            if (scatter) s_t[t] = P.tile_off[t] + at;
+            const u32 at = atomicAdd(&gcount[t_lo + t], c);
+            if (scatter) s_t[t] = P.tile_off[t_lo + t] + at;
         }
     }
     if (!scatter) return;
     __syncthreads();
     for (u32 i = base + threadIdx.x; i < end; i += 256u) {
-        u32 rect;
+        u32 rect, view;
         u64 desc;
-        if (!bin_item(P, nb, i, rect, desc)) continue;
+        if (!bin_item(P, nb, i, rect, desc, view)) continue;
+        const ViewGpu& V = P.views[view];
         const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
         for (i32 ty = y0; ty <= y1; ty += 1) {
             for (i32 tx = x0; tx <= x1; tx += 1) {
-                const u32 pos = atomicAdd(&s_t[(u32)(ty * fc.tile_cols + tx)], 1u);
-                if (pos < fc.tile_list_cap) P.tile_list[pos] = desc;
+                const u32 t = V.tile_base + (u32)(ty * V.fc.tile_cols + tx);
+                const u32 pos = atomicAdd(&s_t[t - t_lo], 1u);
+                if (pos < cap) P.tile_list[pos] = desc;
             }
         }
     }
@@ -721,7 +764,9 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
 }
 
 /* resolves a winning emission key to the reference's 32-bit visibility ID */
-DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip) {
+/* emission key -> the view's reference visibility ID; `clip_base` is the
+   view's first batch-wide clip slot */
+DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip, u32 clip_base) {
     const u32 sub = key & 31u;
     if (sub >= 16u) {
         const WorkItem w = P.work[key >> 9];
@@ -735,23 +780,24 @@ DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip) {
         if (P.clip_key[mid] < key) lo = mid + 1u;
         else hi = mid;
     }
-    return lo | 0x80000000u;
+    return (lo - clip_base) | 0x80000000u;
 }
 
-DEV void load_tri(const FramePtrs& P, u32 vid, Tri& t) {
+/* fetches triangle `vid` (a reference visibility ID of view V) */
+DEV void load_tri(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
     if (vid & 0x80000000u) {
-        const f32* r = P.clip_rec + (u64)(vid & 0x7FFFFFFFu) * 41u;
+        const f32* r = P.clip_rec + (u64)((vid & 0x7FFFFFFFu) + view_clip_base(P, V)) * 41u;
         for (int k = 0; k < 3; k += 1) {
             t.sx[k] = r[0 + k]; t.sy[k] = r[3 + k]; t.iw[k] = r[6 + k];
             t.u[k] = r[9 + k]; t.v[k] = r[12 + k]; t.lu[k] = r[15 + k]; t.lv[k] = r[18 + k];
             t.px[k] = r[21 + k]; t.py[k] = r[24 + k]; t.pz[k] = r[27 + k];
             t.nx[k] = r[30 + k]; t.ny[k] = r[33 + k]; t.nz[k] = r[36 + k];
         }
-        t.inst = ((const u32*)r)[39];
+        t.inst = V.inst_base + ((const u32*)r)[39];
         t.slot = P.insts[t.inst].model_base + ((const u32*)r)[40];
         return;
     }
-    t.inst = vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS);
+    t.inst = V.inst_base + (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS));
     const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1u);
     const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
     const InstGpu* inst = &P.insts[t.inst];
@@ -895,9 +941,10 @@ struct TileLights {
    the same triangle: the capsule-shadow loop couples them.  Every lane of a
    wave that reaches the lighting loop runs the same light and proxy
    iterations, so the group's lanes meet at each ballot. */
-DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, const Tri& t,
+DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const Tri& t,
                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch,
                    u32 group, int block_lane) {
+    const FrameConst& fc = VW.fc;
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
     const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
     const V3 pos{interp(b, t.px), interp(b, t.py), interp(b, t.pz)};
@@ -913,7 +960,10 @@ DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, con
     if (SFT_INVALID_HANDLE != mr.material) {
         const u32 mi = mr.material & SFT_MAT_INDEX_MASK;
         if (mr.material & SFT_MAT_TRANSIENT_BIT) {
-            if (mi < fc.n_tmat) { base_color = P.tmat_color[mi]; tex = P.tmat_tex[mi]; shin = P.tmat_shin[mi]; }
+            if (mi < fc.n_tmat) {
+                const u32 k = VW.tmat_base + mi;
+                base_color = P.tmat_color[k]; tex = P.tmat_tex[k]; shin = P.tmat_shin[k];
+            }
         }
         else if (mi < fc.n_mat) {
             base_color = P.mat_color[mi]; tex = P.mat_tex[mi]; shin = P.mat_shin[mi];
@@ -957,7 +1007,7 @@ DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, con
         else {
             dr = fc.ambient[0]; dg = fc.ambient[1]; db = fc.ambient[2];
             for (u32 l = 0; l < fc.n_dir; l += 1) {
-                const DirLightGpu& L = P.dir_lights[l];
+                const DirLightGpu& L = P.dir_lights[VW.dl_base + l];
                 const V3 Ld{L.dir[0], L.dir[1], L.dir[2]};
                 const f32 ndl = smax(dot3(N, Ld), 0.f);
                 const bool lit = ndl > 0.f;
@@ -979,7 +1029,7 @@ DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, con
         const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
         for (int i = 0; i < tl.count; i += 1) {
             const u32 li = tl.idx[i];
-            const PointLightGpu& L = P.point_lights[li];
+            const PointLightGpu& L = P.point_lights[VW.pl_base + li];
             const V3 lraw = sub3(V3{L.pos[0], L.pos[1], L.pos[2]}, pos);
             const f32 d2 = dot3(lraw, lraw);
             f32 att = smax(0.f, fmaf(-d2, L.inv_r2, 1.f));
@@ -996,10 +1046,10 @@ DEV V3 shade_pixel(const FramePtrs& P, const FrameConst& fc, const Est& est, con
             /* capsule soft shadows (6509-6568): closest approach of the pixel-
                to-light segment and each cached capsule, smooth-max blended */
             f32 max_shadow = 0.f;
-            const u32 np = shadows ? P.light_proxy_count[li] : 0u;
+            const u32 np = shadows ? P.light_proxy_count[VW.pl_base + li] : 0u;
             if (np > 0u) {
                 const f32 lrl = sqrtf(d2);
-                const ProxyRec* recs = P.light_proxies + (u64)li * kMaxLightProxies;
+                const ProxyRec* recs = P.light_proxies + (u64)(VW.pl_base + li) * kMaxLightProxies;
                 for (u32 k = 0; k < np; k += 1) {
                     const ProxyRec& c = recs[k];
                     const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
@@ -1109,14 +1159,16 @@ DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f
 
 /* first 16 point lights touching the tile, in index order, by one wave
    (sofren2.c:5862-5934) */
-DEV void build_tile_lights(const FramePtrs& P, const FrameConst& fc, u32 tile, f32 zmin, f32 zmax, int lane) {
-    const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
+DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 zmin, f32 zmax, int lane) {
+    const FrameConst& fc = V.fc;
+    const u32 local = tile - V.tile_base;
+    const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
     u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
     u32 n = 0;
     if (zmin <= zmax) {
         for (u32 base = 0; base < fc.n_point && n < (u32)kMaxTileLights; base += 64u) {
             const u32 l = base + (u32)lane;
-            const bool pass = l < fc.n_point && light_in_tile(fc, P.point_lights[l], zmin, zmax, tmx, tmy);
+            const bool pass = l < fc.n_point && light_in_tile(fc, P.point_lights[V.pl_base + l], zmin, zmax, tmx, tmy);
             const u64 m = __ballot(pass);
             const u32 rank = n + lanes_below(m, lane);
             if (pass && rank < (u32)kMaxTileLights) tl[1 + rank] = l;
@@ -1134,16 +1186,20 @@ constexpr int kRasterThreads = 1024;
 constexpr int kRasterWaves = kRasterThreads / 64;
 
 __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     __shared__ f32 s_red[2][kRasterWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 tile = blockIdx.x;
     const int sample = (int)blockIdx.y;
+    const ViewGpu& V = P.views[P.tile_view[tile]];
+    const FrameConst& fc = V.fc;
+    if (sample >= fc.samples) return;
     const bool aa = fc.samples > 1;
     const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
-    const i32 tmx = (i32)(tile % (u32)fc.tile_cols) * 64, tmy = (i32)(tile / (u32)fc.tile_cols) * 64;
-    const Est est = make_est(P, fc);
+    const u32 local = tile - V.tile_base;
+    const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
+    const Est est = make_est(P);
+    const u32 list_cap = P.batch->tile_list_cap;
 
     for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
     __syncthreads();
@@ -1152,8 +1208,8 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     {
         const u32 off = P.tile_off[tile];
         u32 cnt = P.tile_count[tile];
-        if (off >= fc.tile_list_cap) cnt = 0;
-        else if (off + cnt > fc.tile_list_cap) cnt = fc.tile_list_cap - off;
+        if (off >= list_cap) cnt = 0;
+        else if (off + cnt > list_cap) cnt = list_cap - off;
         const int g = lane >> 3, j = lane & 7;
         for (u32 e = (u32)wave; e < cnt; e += (u32)kRasterWaves) {
             const u64 desc = P.tile_list[off + e];
@@ -1187,9 +1243,10 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     __syncthreads();
 
     /* publish IDs, and the plane's 1/w range over every sample (screen edge included) */
-    const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], fc.clip_cap);
+    const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], P.batch->clip_cap);
+    const u32 clip_base = view_clip_base(P, V);
     const i32 gw = fc.tile_cols * 64;
-    const u64 plane = (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
+    const u64 plane = V.vid_base + (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
     f32 zmin = 1e9f, zmax = -1e9f;
     for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
         const u64 pk = s_vis[i];
@@ -1199,8 +1256,10 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             zmax = fmaxf(zmax, z);
         }
         const u64 gi = plane + (u64)(tmy + (i >> 6)) * gw + (tmx + (i & 63));
-        P.vid[gi] = (0ull == pk) ? kInvalid : key_to_vid(P, ~(u32)pk, n_clip);
+        const u32 vid = (0ull == pk) ? kInvalid : key_to_vid(P, ~(u32)pk, n_clip, clip_base);
+        P.vid[gi] = vid;
         if (P.cap_depth) P.cap_depth[gi] = z;
+        if (P.vis64) P.vis64[gi] = ((u64)(u32)(pk >> 32) << 32) | vid;
     }
     for (int o = 32; o > 0; o >>= 1) {
         zmin = fminf(zmin, __shfl_xor(zmin, o));
@@ -1217,37 +1276,43 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     }
     if (aa) {
         if (0 == lane) {
-            P.tile_z[((u64)tile * (u32)fc.samples + (u32)sample) * 2u] = zmin;
-            P.tile_z[((u64)tile * (u32)fc.samples + (u32)sample) * 2u + 1u] = zmax;
+            P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u] = zmin;
+            P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u + 1u] = zmax;
         }
         return;
     }
-    build_tile_lights(P, fc, tile, zmin, zmax, lane);
+    build_tile_lights(P, V, tile, zmin, zmax, lane);
 }
 
 /* MSAA: the tile's 1/w range over all sample planes, then its light list */
 __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     const u32 tile = blockIdx.x;
+    const ViewGpu& V = P.views[P.tile_view[tile]];
+    const FrameConst& fc = V.fc;
+    if (fc.samples < 2) return;
     const int lane = threadIdx.x;
     f32 zmin = 1e9f, zmax = -1e9f;
     for (int s = 0; s < fc.samples; s += 1) {
-        zmin = fminf(zmin, P.tile_z[((u64)tile * (u32)fc.samples + (u32)s) * 2u]);
-        zmax = fmaxf(zmax, P.tile_z[((u64)tile * (u32)fc.samples + (u32)s) * 2u + 1u]);
+        zmin = fminf(zmin, P.tile_z[((u64)tile * P.batch->max_samples + (u32)s) * 2u]);
+        zmax = fmaxf(zmax, P.tile_z[((u64)tile * P.batch->max_samples + (u32)s) * 2u + 1u]);
     }
-    build_tile_lights(P, fc, tile, zmin, zmax, lane);
+    build_tile_lights(P, V, tile, zmin, zmax, lane);
 }
 
 /* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
    32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
    order (x offsets 0..3, rows 0..1) */
 __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     __shared__ TileLights s_tl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const i32 x0 = (i32)blockIdx.x * 32, y0 = (i32)blockIdx.y * 8;
-    const i32 tile = (y0 >> 6) * fc.tile_cols + (x0 >> 6);
-    const Est est = make_est(P, fc);
+    /* 16 blocks of 32x8 pixels per tile */
+    const u32 tile = P.shade_tiles[blockIdx.x >> 4];
+    const ViewGpu& VW = P.views[P.tile_view[tile]];
+    const FrameConst& fc = VW.fc;
+    const u32 local = tile - VW.tile_base, sub = blockIdx.x & 15u;
+    const i32 x0 = (i32)(local % (u32)fc.tile_cols) * 64 + (i32)(sub & 1u) * 32;
+    const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
+    const Est est = make_est(P);
     if (0 == tid) {
         const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
         s_tl.count = (int)tl[0];
@@ -1261,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const i32 gx = bx + ix, gy = by + iy;
     const i32 gw = fc.tile_cols * 64;
     const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
-    const u32 vid = P.vid[(u64)gy * gw + gx];
+    const u32 vid = P.vid[VW.vid_base + (u64)gy * gw + gx];
     const bool valid = in_bounds && kInvalid != vid;
     const u32 mine = valid ? vid : kInvalid;
 
@@ -1276,7 +1341,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     Bary b{0.f, 0.f, 0.f};
     f32 cu = 0.f, cv = 0.f;
     if (valid) {
-        load_tri(P, vid, t);
+        load_tri(P, VW, vid, t);
         b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy);
         cu = interp(b, t.u);
         cv = interp(b, t.v);
@@ -1292,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     f32 cu4 = __shfl(cu, src4), cv4 = __shfl(cv, src4);
     const u32 vid1 = __shfl(mine, src1), vid4 = __shfl(mine, src4);
     if (!in_bounds) return;
-    u32* out = &P.color[(u64)gy * fc.width + gx];
+    u32* out = &P.color[VW.color_base + (u64)gy * fc.width + gx];
     if (!valid) {
         if (fc.should_clear) *out = fc.clear_color;
         return;
@@ -1310,7 +1375,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         cv4 = interp(q, t.v);
     }
     u32 fetch = 0;
-    const V3 col = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
+    const V3 col = shade_pixel(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
     *out = pack_argb(col.x * 255.f, col.y * 255.f, col.z * 255.f);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
@@ -1340,12 +1405,16 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    and add the colour weighted by their share of samples.  Each pixel starts
    from the background colour times its empty-sample share. */
 __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     __shared__ TileLights s_tl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const i32 x0 = (i32)blockIdx.x * 32, y0 = (i32)blockIdx.y * 8;
-    const i32 tile = (y0 >> 6) * fc.tile_cols + (x0 >> 6);
-    const Est est = make_est(P, fc);
+    /* 16 blocks of 32x8 pixels per tile */
+    const u32 tile = P.shade_tiles[P.batch->n_shade_tiles + (blockIdx.x >> 4)];
+    const ViewGpu& VW = P.views[P.tile_view[tile]];
+    const FrameConst& fc = VW.fc;
+    const u32 local = tile - VW.tile_base, sub = blockIdx.x & 15u;
+    const i32 x0 = (i32)(local % (u32)fc.tile_cols) * 64 + (i32)(sub & 1u) * 32;
+    const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
+    const Est est = make_est(P);
     if (0 == tid) {
         const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
         s_tl.count = (int)tl[0];
@@ -1363,12 +1432,13 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     const i32 gw = fc.tile_cols * 64;
     const u64 plane = (u64)gw * (u64)(fc.tile_rows * 64);
     const bool in_bounds = gx < fc.screen_w && gy < fc.screen_h;
+    const u32* vid_planes = P.vid + VW.vid_base;
     u32 sv[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
     u32 mine = 0;
     i32 empty = 0;
     for (int s = 0; s < 4; s += 1) {
         if (s < S) {
-            const u32 v = P.vid[(u64)s * plane + (u64)gy * gw + gx];
+            const u32 v = vid_planes[(u64)s * plane + (u64)gy * gw + gx];
             sv[s] = in_bounds ? v : kInvalid;
             if (in_bounds && kInvalid != v) mine |= 1u << (s * 8 + l);
             if (in_bounds && kInvalid == v) empty += 1;
@@ -1379,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     remaining |= __shfl_xor(remaining, 1);
     remaining |= __shfl_xor(remaining, 2);
     remaining |= __shfl_xor(remaining, 4);
-    u32* out = &P.color[(u64)gy * fc.width + gx];
+    u32* out = &P.color[VW.color_base + (u64)gy * fc.width + gx];
     if (0u == remaining) {
         if (in_bounds && fc.should_clear) *out = fc.clear_color;
         return;
@@ -1423,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
         remaining &= ~match;
         const u32 group = (u32)(__ballot(hits > 0u) >> base) & 0xFFu;
         Tri t;
-        load_tri(P, target, t);
+        load_tri(P, VW, target, t);
         const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
         const f32 cu = interp(b, t.u), cv = interp(b, t.v);
         const int first = __ffs(group) - 1;
@@ -1432,7 +1502,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
         const f32 cu4 = __shfl(cu, base | (first ^ 4)), cv4 = __shfl(cv, base | (first ^ 4));
         if (hits > 0u) {
             u32 fetch = 0;
-            const V3 col = shade_pixel(P, fc, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, base);
+            const V3 col = shade_pixel(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, base);
             const f32 w = (f32)hits * inv_s;
             acc_r = fmaf(col.x, w, acc_r);
             acc_g = fmaf(col.y, w, acc_g);
@@ -1636,8 +1706,9 @@ DEV void bvh_step(OccLds& W, u32& top, const sft_bvh_node* nodes, u32 n_nodes, c
 
 /* traces one packet (8 rays, `active` lanes) against the frame's static
    geometry; returns the rays that hit (4086-4243) */
-DEV u32 trace_packet(const FramePtrs& P, const FrameConst& fc, const Est& est, OccLds& W, const Frustum& fr,
+DEV u32 trace_packet(const FramePtrs& P, const ViewGpu& VW, const Est& est, OccLds& W, const Frustum& fr,
                      sftvec o_r, sftvec l_r, u32 active, int lane, u32& nodes_tested, u32& tris_tested, bool& overflow) {
+    const FrameConst& fc = VW.fc;
     u32 sat = 0;
     /* TLAS: collect the instance leaves the frustum reaches */
     u32 top = (fc.tlas_root < fc.tlas_nodes) ? 1u : 0u, ninst = 0;
@@ -1653,7 +1724,7 @@ DEV u32 trace_packet(const FramePtrs& P, const FrameConst& fc, const Est& est, O
         }
         bool leaf;
         u32 child;
-        bvh_step(W, top, P.tlas, fc.tlas_nodes, fr, lane, leaf, child, nodes_tested, overflow);
+        bvh_step(W, top, P.tlas + VW.tlas_base, fc.tlas_nodes, fr, lane, leaf, child, nodes_tested, overflow);
         const u64 lm = __ballot(leaf);
         if (leaf) W.insts[min(ninst + lanes_below(lm, lane), 63u)] = child & 0x7FFFFFFFu;
         ninst = min(ninst + (u32)__popcll(lm), 64u);
@@ -1663,7 +1734,7 @@ DEV u32 trace_packet(const FramePtrs& P, const FrameConst& fc, const Est& est, O
     for (u32 ii = 0; ii < ninst && sat != active; ii += 1) {
         const u32 inst_id = W.insts[ii];
         if (inst_id >= fc.n_inst) continue;
-        const InstGpu& in = P.insts[inst_id];
+        const InstGpu& in = P.insts[VW.inst_base + inst_id];
         if (kInvalid == in.blas_base || in.blas_root >= in.blas_count) continue;
         Frustum mf;
         model_frustum(fr, in.t, mf);
@@ -1727,16 +1798,18 @@ DEV u32 trace_packet(const FramePtrs& P, const FrameConst& fc, const Est& est, O
 
 /* one workgroup per (light, packet of 8 proxies); wave w traces ray w */
 __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     __shared__ OccLds s_w[kOccWaves];
     __shared__ u32 s_hit[kOccWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const u32 l = blockIdx.x / fc.occ_batches, batch = blockIdx.x % fc.occ_batches;
+    const ViewGpu& VW = P.views[find_view(P, blockIdx.x, [](const ViewGpu& v) { return v.occ_base; })];
+    const FrameConst& fc = VW.fc;
+    const u32 g = blockIdx.x - VW.occ_base;
+    const u32 l = g / fc.occ_batches, batch = g % fc.occ_batches;
     const u32 pbase = batch * 8u;
     const u32 n = min(8u, fc.n_proxies - pbase);
-    const ProxyGpu* px = P.proxies + pbase;
-    const Est est = make_est(P, fc);
-    const PointLightGpu& L = P.point_lights[l];
+    const ProxyGpu* px = P.proxies + VW.proxy_base + pbase;
+    const Est est = make_est(P);
+    const PointLightGpu& L = P.point_lights[VW.pl_base + l];
     const sftvec lp = SFT_V(L.pos[0], L.pos[1], L.pos[2]);
 
     /* light-disk targets facing the packet's centre (8178-8210) */
@@ -1767,7 +1840,7 @@ __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
     const u32 active = (1u << n) - 1u;
     u32 nodes_tested = 0, tris_tested = 0;
     bool overflow = false;
-    const u32 sat = trace_packet(P, fc, est, s_w[wave], fr, o_r, l_r, active, lane, nodes_tested, tris_tested, overflow);
+    const u32 sat = trace_packet(P, VW, est, s_w[wave], fr, o_r, l_r, active, lane, nodes_tested, tris_tested, overflow);
     if (0 == lane) {
         s_hit[wave] = sat;
         const u32 sl = stat_slot();
@@ -1780,7 +1853,7 @@ __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
     __syncthreads();
     if (0 == wave && r < n && lane < 8) {
         const u32 hits = ((s_hit[0] >> r) & 1u) + ((s_hit[1] >> r) & 1u) + ((s_hit[2] >> r) & 1u);
-        P.proxy_vis[(u64)(pbase + r) * 64u + l] = (u8)(3u - hits);
+        P.proxy_vis[(u64)(VW.proxy_base + pbase + r) * 64u + l] = (u8)(3u - hits);
     }
 }
 
@@ -1789,10 +1862,12 @@ __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
    shading constants (sofren2.c:5936-5990).  The selection depends on the
    light only, so it is done once per light instead of once per tile. */
 __global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
-    const FrameConst& fc = *P.fc;
     const int lane = threadIdx.x;
-    const u32 l = blockIdx.x;
-    const PointLightGpu& L = P.point_lights[l];
+    const ViewGpu& VW = P.views[find_view(P, blockIdx.x, [](const ViewGpu& v) { return v.pl_base; })];
+    const FrameConst& fc = VW.fc;
+    if (!(fc.shadows_enabled && fc.n_proxies > 0u)) return;
+    const u32 l = blockIdx.x - VW.pl_base;
+    const PointLightGpu& L = P.point_lights[VW.pl_base + l];
     const sftvec lpos = SFT_V(L.pos[0], L.pos[1], L.pos[2]);
     u32 count = 0;
     for (u32 base = 0; base < fc.n_proxies && count < (u32)kMaxLightProxies; base += 64u) {
@@ -1800,9 +1875,9 @@ __global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
         bool ok = false;
         ProxyRec rec;
         if (p < fc.n_proxies) {
-            const ProxyGpu& px = P.proxies[p];
+            const ProxyGpu& px = P.proxies[VW.proxy_base + p];
             /* the visibility count exists for the first 64 lights only */
-            const u32 vis = (l < 64u) ? (u32)P.proxy_vis[(u64)p * 64u + l] : 3u;
+            const u32 vis = (l < 64u) ? (u32)P.proxy_vis[(u64)(VW.proxy_base + p) * 64u + l] : 3u;
             if (0u != vis) {
                 const sftvec a = SFT_V(px.a[0], px.a[1], px.a[2]), b = SFT_V(px.b[0], px.b[1], px.b[2]);
                 const sftvec cc = SFT_V((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
@@ -1828,10 +1903,10 @@ __global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
         }
         const u64 m = __ballot(ok);
         const u32 rank = count + lanes_below(m, lane);
-        if (ok && rank < (u32)kMaxLightProxies) P.light_proxies[(u64)l * kMaxLightProxies + rank] = rec;
+        if (ok && rank < (u32)kMaxLightProxies) P.light_proxies[(u64)(VW.pl_base + l) * kMaxLightProxies + rank] = rec;
         count = min(count + (u32)__popcll(m), (u32)kMaxLightProxies);
     }
-    if (0 == lane) P.light_proxy_count[l] = count;
+    if (0 == lane) P.light_proxy_count[VW.pl_base + l] = count;
 }
 
 /* folds the spread statistics slots into the narrow counter words */
@@ -1852,46 +1927,37 @@ void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n16);
 }
 
-void launch_frame(const FramePtrs& p, const FrameConst& fc, hipStream_t s, hipEvent_t* ev) {
-    const u32 nw = fc.n_work;
+void launch_frame(const FramePtrs& p, const BatchGpu& B, hipStream_t s, hipEvent_t* ev) {
+    auto mark = [&](int k) { if (ev) hipEventRecord(ev[k], s); };
+    const u32 nw = B.n_work;
     const u32 geom_blocks = (nw + 3u) / 4u;
-    hipEventRecord(ev[1], s);
+    mark(1);
     /* shadow-proxy visibility and the per-light capsule lists */
-    if (fc.occ_lights > 0u && fc.occ_batches > 0u) {
-        hipLaunchKernelGGL(k_occlude, dim3(fc.occ_lights * fc.occ_batches), dim3(64 * kOccWaves), 0, s, p);
-    }
-    if (fc.shadows_enabled && fc.n_proxies > 0u && fc.n_point > 0u) {
-        hipLaunchKernelGGL(k_proxy_cache, dim3(fc.n_point), dim3(64), 0, s, p);
-    }
-    hipEventRecord(ev[7], s);
+    if (B.occ_total > 0u) hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, s, p);
+    if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, s, p);
+    mark(7);
     if (nw > 0) {
         hipLaunchKernelGGL(k_geom, dim3(geom_blocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.clip_count, p.clip_base, nw,
-                           p.counters + CNT_CLIP_TRIS, fc.clip_cap, p.counters + CNT_OVERFLOW, 1u);
+                           p.counters + CNT_CLIP_TRIS, B.clip_cap, p.counters + CNT_OVERFLOW, 1u);
     }
-    hipEventRecord(ev[2], s);
+    mark(2);
     /* upper bound of payload items: batch slots plus every clip slot */
-    const u32 bin_blocks = (nw * 16u + fc.clip_cap + kBinChunk - 1u) / kBinChunk;
+    const u32 bin_blocks = (nw * 16u + B.clip_cap + kBinChunk - 1u) / kBinChunk;
     if (nw > 0) {
         hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(k_bin<false>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.tile_count, p.tile_off, (u32)fc.n_tiles,
-                       p.counters + CNT_TILE_PAIRS, fc.tile_list_cap, p.counters + CNT_OVERFLOW, 2u);
-    if (nw > 0) {
-        hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
-    }
-    hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_raster, dim3((u32)fc.n_tiles, (u32)fc.samples), dim3(kRasterThreads), 0, s, p);
-    if (fc.samples > 1) hipLaunchKernelGGL(k_tile_lights, dim3((u32)fc.n_tiles), dim3(64), 0, s, p);
-    hipEventRecord(ev[6], s);
-    if (fc.samples > 1) {
-        hipLaunchKernelGGL(k_shade_ms, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
-    }
-    else {
-        hipLaunchKernelGGL(k_shade, dim3((u32)fc.tile_cols * 2u, (u32)fc.tile_rows * 8u), dim3(256), 0, s, p);
-    }
-    hipEventRecord(ev[4], s);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.tile_count, p.tile_off, B.n_tiles,
+                       p.counters + CNT_TILE_PAIRS, B.tile_list_cap, p.counters + CNT_OVERFLOW, 2u);
+    if (nw > 0) hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
+    mark(3);
+    hipLaunchKernelGGL(k_raster, dim3(B.n_tiles, B.max_samples), dim3(kRasterThreads), 0, s, p);
+    if (B.max_samples > 1u) hipLaunchKernelGGL(k_tile_lights, dim3(B.n_tiles), dim3(64), 0, s, p);
+    mark(6);
+    if (B.n_shade_tiles) hipLaunchKernelGGL(k_shade, dim3(B.n_shade_tiles * 16u), dim3(256), 0, s, p);
+    if (B.n_shade_tiles_ms) hipLaunchKernelGGL(k_shade_ms, dim3(B.n_shade_tiles_ms * 16u), dim3(256), 0, s, p);
+    mark(4);
     hipLaunchKernelGGL(k_stats, dim3(CNT_COUNT), dim3(64), 0, s, p.counters);
 }
 

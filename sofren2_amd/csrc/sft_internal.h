@@ -50,11 +50,31 @@ struct FrameConst {
     u32 occ_batches, tlas_nodes, pad8, pad9;    /* proxy packets of 8 per light */
 };
 
+/* One view of a batch on the GPU: its frame constants and where its rows
+   start in the batch-wide tables (instances, work items, lights, transient
+   materials, proxies, TLAS nodes, 64x64 tiles) and output planes. */
+struct ViewGpu {
+    FrameConst fc;
+    u32 inst_base, work_base, n_work, pl_base;
+    u32 dl_base, tmat_base, tile_base, proxy_base;
+    u32 tlas_base, occ_base, occ_count, pad0;
+    u64 vid_base;       /* first element of the view's [samples][grid] planes */
+    u64 color_base;     /* first pixel of the view's width x height colour */
+};
+
+/* batch-wide counts and capacities */
+struct BatchGpu {
+    u32 n_views, n_work, n_tiles, max_samples;
+    u32 clip_cap, tile_list_cap, rcp_bits, rsq_bits;
+    u32 n_shade_tiles, n_shade_tiles_ms, occ_total, n_lights;
+    u32 vis64, pad0, pad1, pad2;
+};
+
 struct InstGpu {
     f32 t[16];
     f32 inv[16];        /* rigid inverse of t (sft_m_qinv) for shadow rays */
     u32 model_base;     /* first meshlet slot of the model */
-    u32 ss_offset;      /* ss_meshlet_offset */
+    u32 ss_offset;      /* ss_meshlet_offset, batch-wide */
     u32 bone_base;      /* first matrix in the frame's bone buffer, kInvalid if none */
     u32 blas_base;      /* first node in the resident BLAS pool; kInvalid when the
                            model has no BLAS or has a skeleton (shadow rays test
@@ -81,12 +101,13 @@ constexpr int kMaxLightProxies = 16;
 
 struct WorkItem {
     u32 slot;           /* resident meshlet slot */
-    u32 ss;             /* screen-space cache index */
-    u32 inst;           /* instance slot */
+    u32 ss;             /* screen-space cache index (batch-wide) */
+    u32 inst;           /* instance slot in its view (visibility ID field) */
     u32 gm;             /* model-relative meshlet index (visibility ID field) */
     i32 cull_bone;      /* bone used for the 8-meshlet group's cull volume, -1 none */
     u32 skin;           /* skin record, kInvalid if unskinned or no pose */
-    u32 pad0, pad1;
+    u32 view;           /* view of the batch */
+    u32 ginst;          /* instance row in the batch-wide table */
 };
 
 struct MeshRec {
@@ -129,6 +150,39 @@ struct HostTexture {
     bool uploaded = false;
 };
 
+/* Everything the render of one view reads, snapshot from the front-end at
+   sft_present (a batch of one) or at sftx_view_end (sofren2.c:1335-1495 minus
+   the reference's scratch).  Output targets are host memory or null. */
+struct ViewRecord {
+    i32 width = 0, height = 0, tile_cols = 0, tile_rows = 0, samples = 1;
+    f32 half_width = 0, half_height = 0, vp_x = 0, vp_y = 0;
+    u32* color_buffer = nullptr;
+    i32 color_pitch = 0;            /* in u32 */
+    u64* vis64 = nullptr;           /* [samples][tile_rows*64][tile_cols*64] */
+    u8 should_clear = 0;
+    u32 clear_color = 0;
+    i32 debug_mode = 0;
+    sftvec camera_pos{0, 0, 0};
+    sftmat view_proj{};
+    f32 planes[6][4] = {};
+    sftvec ambient{0, 0, 0};
+    std::vector<sft_directional_light> dir_lights;
+    std::vector<sft_point_light> point_lights;
+    f32 lightmap_brightness = 1.5f;
+    u8 shadows_enabled = 0;
+    std::vector<WorldProxy> proxies;
+    std::vector<sft_instance> instances;
+    u32 total_ss = 0;
+    std::vector<const sft_model*> transient_models;
+    std::vector<u32> tmat_tex, tmat_color;
+    std::vector<f32> tmat_shininess;
+    u32 capture_flags = 0;
+    /* batched views keep their own copy of the posed bone matrices, taken when
+       the view is ended (the reference reads them at present time) */
+    std::vector<sftmat> bone_store;
+    std::vector<u32> bones_kept;    /* per instance: matrices in bone_store (batched views) */
+};
+
 struct Context;
 extern Context* g_ctx;
 
@@ -138,8 +192,9 @@ u32 build_tlas(const std::vector<sft_instance>& instances, std::vector<sft_bvh_n
 struct Backend;
 Backend* backend_create();
 void backend_destroy(Backend* b);
-/* renders the recorded frame into ctx->color_buffer */
-void backend_present(Backend* b, Context* ctx);
+/* renders a batch of views (one pass of kernels over all of them) and writes
+   each view's outputs; blocks until they are in host memory */
+void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
 void backend_release_texture(Backend* b, u32 index);
 
 /* host estimate tables for _mm_rcp_ps / _mm_rsqrt_ps (rcp_probe.cpp) */
@@ -204,6 +259,10 @@ struct Context {
 
     u32 capture_flags = 0;
     sftx_frame_stats stats{};
+
+    /* batched views being recorded (sftx_batch_begin .. sftx_batch_submit) */
+    bool batch_open = false;
+    std::vector<ViewRecord> batch_views;
 
     EstimateTables est;
     std::vector<sftx_texture_desc> export_tex;

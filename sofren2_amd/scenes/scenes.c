@@ -44,7 +44,7 @@ static u32 sftx_pack_rgb(f32 r, f32 g, f32 b) {
 /* ---------------- scene state ---------------- */
 #define SFTX_MAX_SPHERES 16
 
-static struct {
+typedef struct {
     i32 kind;
     u32 seed;
     i32 width, height;
@@ -55,7 +55,48 @@ static struct {
     f32 sphere_r[SFTX_MAX_SPHERES], sphere_cx[SFTX_MAX_SPHERES], sphere_cz[SFTX_MAX_SPHERES];
 
     u32 tri_count, meshlet_count;
-} g_scene;
+} sftx_scene_common;
+
+#define SFTX_CHARS 32
+#define SFTX_BONES 10
+#define SFTX_BANDS 96
+#define SFTX_SEGS 32
+#define SFTX_BONE_LEN 0.18f
+#define SFTX_SHADOW_LIGHTS 16
+
+typedef struct {
+    sft_arena arena;
+    sft_model* model;
+    sft_instance inst;
+    sftvec light_pos[8];
+    sftvec light_col[8];
+} sftx_room_state;
+
+typedef struct {
+    sft_model* human;
+    sft_model* person[SFTX_CHARS];     /* shallow copies: own mesh record (material), shared meshlets */
+    sft_mat_handle mat[SFTX_CHARS];
+    sftmat xform[SFTX_CHARS];
+    f32 phase[SFTX_CHARS][SFTX_BONES][2];
+    sftmat posed[SFTX_CHARS][SFTX_BONES];
+    sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
+} sftx_crowd_state;
+
+struct sftx_scene_state {
+    sftx_scene_common scene;
+    sftx_room_state room;
+    sftx_crowd_state crowd;
+};
+#define SFTX_MAX_SCENES 64
+static struct sftx_scene_state* sftx_slots[SFTX_MAX_SCENES];
+
+/* All state of one scene lives in a slot; the functions below work on the
+   current slot through these names. */
+struct sftx_scene_state;
+static struct sftx_scene_state* sftx_cur;
+#define g_scene (sftx_cur->scene)
+#define g_room (sftx_cur->room)
+#define g_crowd (sftx_cur->crowd)
 
 /* ---------------- workload 0: ground plane + 16 dmesh spheres -------------- */
 static void sftx_plane_spheres_create(void) {
@@ -453,13 +494,7 @@ static void sftx_pad_meshlet(sftx_mbuild* m) {
     }
 }
 
-static struct {
-    sft_arena arena;
-    sft_model* model;
-    sft_instance inst;
-    sftvec light_pos[8];
-    sftvec light_col[8];
-} g_room;
+
 
 static void sftx_box_room_create(int with_blas) {
     sftx_rng rng;
@@ -590,12 +625,7 @@ static void sftx_box_room_render(i32 frame) {
    room's QSF2 carries a triangle BLAS for the proxy-occlusion rays; 16
    shadow-casting point lights hang 15-25 units above the crowd, outside every
    proxy cluster. */
-#define SFTX_CHARS 32
-#define SFTX_BONES 10
-#define SFTX_BANDS 96
-#define SFTX_SEGS 32
-#define SFTX_BONE_LEN 0.18f
-#define SFTX_SHADOW_LIGHTS 16
+
 
 /* Gamma(a) for a < 1 via Gamma(a+1) * U^(1/a); Marsaglia-Tsang for a+1 */
 static f32 sftx_gauss(sftx_rng* r) {
@@ -622,15 +652,7 @@ static f32 sftx_gamma(sftx_rng* r, f32 a) {
     return g * powf(sftx_u01(r) + 1e-7f, 1.f / a);
 }
 
-static struct {
-    sft_model* human;
-    sft_model* person[SFTX_CHARS];     /* shallow copies: own mesh record (material), shared meshlets */
-    sft_mat_handle mat[SFTX_CHARS];
-    sftmat xform[SFTX_CHARS];
-    f32 phase[SFTX_CHARS][SFTX_BONES][2];
-    sftmat posed[SFTX_CHARS][SFTX_BONES];
-    sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
-} g_crowd;
+
 
 static sft_model* sftx_build_humanoid(sft_arena* ar, sftx_rng* rng) {
     sft_model* m = (sft_model*)sft_arena_alloc(ar, sizeof(sft_model));
@@ -822,12 +844,7 @@ static void sftx_crowd_render(i32 frame) {
 }
 
 /* ---------------- dispatch ---------------- */
-i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
-    memset(&g_scene, 0, sizeof(g_scene));
-    g_scene.kind = kind;
-    g_scene.seed = seed;
-    g_scene.width = width;
-    g_scene.height = height;
+static i32 sftx_build(i32 kind) {
     sft_set_clear(1);
     switch (kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_create(); return 0;
@@ -837,9 +854,33 @@ i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
     }
 }
 
-void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes) {
-    sft_set_color_buffer(color, pitch_bytes);
-    sft_frame_begin();
+i32 sftx_scene_new(i32 kind, u32 seed, i32 width, i32 height) {
+    i32 h = 0;
+    while (h < SFTX_MAX_SCENES && sftx_slots[h]) h += 1;
+    if (h == SFTX_MAX_SCENES || kind < 0 || kind >= SFTX_SCENE_COUNT) return -1;
+    struct sftx_scene_state* st = (struct sftx_scene_state*)calloc(1, sizeof(struct sftx_scene_state));
+    sftx_slots[h] = st;
+    sftx_cur = st;
+    g_scene.kind = kind;
+    g_scene.seed = seed;
+    g_scene.width = width;
+    g_scene.height = height;
+    if (0 != sftx_build(kind)) {
+        free(st);
+        sftx_slots[h] = NULL;
+        sftx_cur = NULL;
+        return -1;
+    }
+    return h;
+}
+
+i32 sftx_scene_use(i32 handle) {
+    if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return -1;
+    sftx_cur = sftx_slots[handle];
+    return 0;
+}
+
+static void sftx_draw_current(i32 frame) {
     switch (g_scene.kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_render(frame); break;
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_render(frame); break;
@@ -848,15 +889,43 @@ void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes) {
     }
 }
 
+void sftx_scene_draw(i32 handle, i32 frame, i32 width, i32 height) {
+    if (0 != sftx_scene_use(handle)) return;
+    const i32 w = g_scene.width, h = g_scene.height;
+    g_scene.width = width;
+    g_scene.height = height;
+    sftx_draw_current(frame);
+    g_scene.width = w;
+    g_scene.height = h;
+}
+
+void sftx_scene_free(i32 handle) {
+    if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return;
+    if (sftx_cur == sftx_slots[handle]) sftx_cur = NULL;
+    free(sftx_slots[handle]);
+    sftx_slots[handle] = NULL;
+}
+
+i32 sftx_scene_create(i32 kind, u32 seed, i32 width, i32 height) {
+    for (i32 h = 0; h < SFTX_MAX_SCENES; h += 1) sftx_scene_free(h);
+    return sftx_scene_new(kind, seed, width, height) >= 0 ? 0 : -1;
+}
+
+void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes) {
+    sft_set_color_buffer(color, pitch_bytes);
+    sft_frame_begin();
+    if (sftx_cur) sftx_draw_current(frame);
+}
+
 void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes) {
     sftx_scene_record(frame, color, pitch_bytes);
     sft_present();
     sft_frame_end();
 }
 
-u32 sftx_scene_triangle_count(void) { return g_scene.tri_count; }
-u32 sftx_scene_meshlet_count(void) { return g_scene.meshlet_count; }
+u32 sftx_scene_triangle_count(void) { return sftx_cur ? g_scene.tri_count : 0u; }
+u32 sftx_scene_meshlet_count(void) { return sftx_cur ? g_scene.meshlet_count : 0u; }
 
 void sftx_scene_destroy(void) {
-    memset(&g_scene, 0, sizeof(g_scene));
+    for (i32 h = 0; h < SFTX_MAX_SCENES; h += 1) sftx_scene_free(h);
 }

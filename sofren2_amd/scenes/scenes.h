@@ -50,6 +50,17 @@ void sftx_scene_record(i32 frame, u32* color, i32 pitch_bytes);
    sft_present, frame_end. */
 void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes);
 
+/* Several scenes at once (handles 0..63): sftx_scene_new builds one and
+   makes it current; sftx_scene_draw records a frame of a scene's camera path
+   (camera for a width x height view, lights, draws) into the frame or view
+   being recorded, without sft_frame_begin; sftx_scene_use selects the scene
+   the single-scene calls work on.  sftx_scene_create frees every scene and
+   builds one. */
+i32 sftx_scene_new(i32 kind, u32 seed, i32 width, i32 height);
+i32 sftx_scene_use(i32 handle);
+void sftx_scene_draw(i32 handle, i32 frame, i32 width, i32 height);
+void sftx_scene_free(i32 handle);
+
 /* Frame statistics of the scene description (not of the renderer). */
 u32 sftx_scene_triangle_count(void);
 u32 sftx_scene_meshlet_count(void);

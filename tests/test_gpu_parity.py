@@ -98,3 +98,43 @@ def test_msaa_parity(scene, frame, samples):
     rep = parity.compare_frame(gpu, ref)
     print("msaa", scene, frame, samples, rep, "groups", st.shading_groups, "ms", st.ms_total, flush=True)
     parity.assert_parity(rep)
+
+
+BATCH_VIEWS = [  # (scene, seed, width, height, frame, samples)
+    (S.SCENE_BOX_ROOM, 1, 640, 400, 5, 1),
+    (S.SCENE_PLANE_SPHERES, 2, 320, 256, 0, 4),
+    (S.SCENE_CROWD_SHADOWS, 3, 800, 450, 10, 2),
+    (S.SCENE_BOX_ROOM, 1, 1024, 512, 77, 1),
+    (S.SCENE_PLANE_SPHERES, 2, 300, 300, 0, 1),
+    (S.SCENE_CROWD_SHADOWS, 3, 512, 288, 64, 4),
+]
+
+
+def test_batched_views_match_reference():
+    """Views of several scenes at mixed resolutions and MSAA rendered in one
+    submission: each is the reference's frame for that scene, size and frame
+    (visibility IDs and depth bit-exact, colour within the bar)."""
+    lib = S.library()
+    lib.sft_init(640, 360, 0)
+    try:
+        handles = {}
+        for scene, seed, w, h, frame, samples in BATCH_VIEWS:
+            if (scene, seed) not in handles:
+                handles[(scene, seed)] = lib.scene_new(scene, seed, w, h)
+                assert handles[(scene, seed)] >= 0
+        views = [S.View(handles[(sc, sd)], fr, w, h, ns) for sc, sd, w, h, fr, ns in BATCH_VIEWS]
+        print("batch of", len(views), flush=True)
+        S.render_batch(lib, views)
+        st = S.FrameStats()
+        lib.get_stats(st)
+        print("batch stats visible", st.meshlets_visible, "tiles", st.tiles, "ms", st.ms_total, flush=True)
+        assert st.occlusion_invalid == 0 and st.occlusion_overflow == 0
+        for (scene, seed, w, h, frame, samples), v in zip(BATCH_VIEWS, views):
+            ref = ref_dump(scene, seed, w, h, frame, samples=samples)
+            got = {"color": v.color, "vid": v.vid(), "depth": v.depth()}
+            rep = parity.compare_frame(got, ref)
+            print("view", scene, seed, w, h, frame, samples, rep, flush=True)
+            parity.assert_parity(rep)
+    finally:
+        lib.scene_destroy()
+        lib.sft_release()
