@@ -62,6 +62,42 @@ def cpu_baseline(scene, name, width, height, frames):
                       f"median {r['ms_median']:.2f} ms, p99 {r['ms_p99']:.2f} ms"}
 
 
+def mixed_batch_leg(lib, steps):
+    """64 views over 4 scene instances at mixed resolution and MSAA: one
+    batched submission per step against the same views presented one by one."""
+    from sofren2_amd.workloads import MixedBatch
+    mb = MixedBatch(lib)
+    try:
+        lib.set_output(1)
+        mb.render_batched()
+        mb.render_per_frame()
+        lib.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            mb.render_batched()
+            lat.append((time.perf_counter() - a) * 1e3)
+        batched_s = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        per_steps = max(1, steps // 2)
+        for _ in range(per_steps):
+            mb.render_per_frame()
+        lib.synchronize()
+        per_frame_s = (time.perf_counter() - t0) / per_steps
+        n = len(mb.views)
+        px = mb.pixels()
+        lat = np.array(lat)
+        return {"views": n, "megapixels_per_batch": round(px / 1e6, 3),
+                "batched_views_s": round(n / batched_s, 2), "per_frame_views_s": round(n / per_frame_s, 2),
+                "batched_mpix_s": round(px / batched_s / 1e6, 1), "per_frame_mpix_s": round(px / per_frame_s / 1e6, 1),
+                "speedup": round(per_frame_s / batched_s, 3),
+                "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                     "p99": round(float(np.percentile(lat, 99)), 3)}}
+    finally:
+        mb.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +107,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=120)
     ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
                     help="workload; box_room is the headline 1080p textured interior")
+    ap.add_argument("--batch-steps", type=int, default=10, help="steps of the 64-view mixed batch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
@@ -99,13 +136,13 @@ def main():
     lib.synchronize()
     host_fps = n_host / (time.perf_counter() - t0)
 
-    # GPU-resident output
+    # GPU-resident output; the timed frames replay the captured HIP graph
     lib.set_output(1)
+    lib.set_graphs(1)
     for i in range(args.warmup):
         lib.scene_render(i % 120, color.ctypes.data, w * 4)
     lib.synchronize()
-    lat, tile_ms, geom_ms, bin_ms, total_gpu_ms, rast_ms, shade_ms, shadow_ms = [], [], [], [], [], [], [], []
-    bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
+    lat, total_gpu_ms = [], []
     st = S.FrameStats()
     counter_names = [n for n, _ in S.FrameStats._fields_ if not n.startswith("ms_")]
     counters_sum = {n: 0 for n in counter_names}
@@ -115,27 +152,38 @@ def main():
         lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
         lat.append((time.perf_counter() - a) * 1e3)
         lib.get_stats(st)
-        tile_ms.append(st.ms_raster_shade)
-        geom_ms.append(st.ms_geometry)
-        bin_ms.append(st.ms_binning)
         total_gpu_ms.append(st.ms_total)
-        rast_ms.append(st.ms_raster)
-        shade_ms.append(st.ms_shade)
-        shadow_ms.append(st.ms_shadows)
-        for k, v in algorithmic_bytes(st).items():
-            bytes_acc[k] += v
         for n in counter_names:
             counters_sum[n] += getattr(st, n)
     lib.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
 
+    # per-pass kernel durations: the same frames launched eagerly so HIP
+    # events can bracket each pass on the render stream
+    lib.set_graphs(0)
+    n_inst = min(args.steps, 60)
+    tile_ms, geom_ms, bin_ms, rast_ms, shade_ms, shadow_ms = [], [], [], [], [], []
+    bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
+    for i in range(n_inst):
+        lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
+        lib.get_stats(st)
+        tile_ms.append(st.ms_raster_shade)
+        geom_ms.append(st.ms_geometry)
+        bin_ms.append(st.ms_binning)
+        rast_ms.append(st.ms_raster)
+        shade_ms.append(st.ms_shade)
+        shadow_ms.append(st.ms_shadows)
+        for k, v in algorithmic_bytes(st).items():
+            bytes_acc[k] += v
+    lib.set_graphs(1)
+    n_div = n_inst
     tile_avg_ms = float(np.mean(tile_ms))
     per_pass = {
-        "geometry": (bytes_acc["geometry"] / args.steps, float(np.mean(geom_ms))),
-        "binning": (bytes_acc["binning"] / args.steps, float(np.mean(bin_ms))),
-        "visibility": (bytes_acc["visibility"] / args.steps, float(np.mean(rast_ms))),
-        "shading": (bytes_acc["shading"] / args.steps, float(np.mean(shade_ms))),
+        "geometry": (bytes_acc["geometry"] / n_div, float(np.mean(geom_ms))),
+        "binning": (bytes_acc["binning"] / n_div, float(np.mean(bin_ms))),
+        "visibility": (bytes_acc["visibility"] / n_div, float(np.mean(rast_ms))),
+        "shading": (bytes_acc["shading"] / n_div, float(np.mean(shade_ms))),
     }
     counters_avg = {n: round(v / args.steps, 1) for n, v in counters_sum.items()}
     dominant = max(per_pass, key=lambda k: per_pass[k][1])
@@ -174,6 +222,8 @@ def main():
         "cpu_baseline": None,
     }
     lib.scene_destroy()
+    if args.batch_steps > 0:
+        out["mixed_batch"] = mixed_batch_leg(lib, args.batch_steps)
     lib.sft_release()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.scene, w, h, args.cpu_frames)

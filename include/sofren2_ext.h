@@ -131,6 +131,12 @@ i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64);
 void sftx_batch_submit(void);
 void sftx_batch_end(void);
 
+/* Frame launches replayed from a HIP graph (default on): the kernel sequence
+   of a frame is captured once and replayed while launch sizes and buffers are
+   unchanged.  sftx_graph_captures counts the captures made so far. */
+void sftx_set_graphs(i32 enabled);
+u64 sftx_graph_captures(void);
+
 /* Waits for all GPU work of the renderer. */
 void sftx_synchronize(void);
 

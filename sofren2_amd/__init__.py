@@ -75,10 +75,14 @@ class Library:
         self.get_stats = _bind(g, "sftx_get_stats", None, [ctypes.POINTER(FrameStats)])
         self.export_frame = _bind(g, "sftx_export_frame", None, [vp])
         self.sft_present = _bind(g, "sft_present", None, [])
+        self.sft_frame_begin = _bind(g, "sft_frame_begin", None, [])
+        self.set_color_buffer = _bind(g, "sft_set_color_buffer", None, [vp, i32])
         self.sft_frame_end = _bind(g, "sft_frame_end", None, [])
         self.scene_record = _bind(s, "sftx_scene_record", None, [i32, vp, i32])
         self.set_output = _bind(g, "sftx_set_output", None, [i32])
         self.set_msaa = _bind(g, "sftx_set_msaa", None, [i32])
+        self.set_graphs = _bind(g, "sftx_set_graphs", None, [i32])
+        self.graph_captures = _bind(g, "sftx_graph_captures", ctypes.c_uint64, [])
         self.synchronize = _bind(g, "sftx_synchronize", None, [])
         self.copy_calibration = _bind(g, "sftx_copy_calibration", ctypes.c_float, [ctypes.c_uint64])
         self.scene_create = _bind(s, "sftx_scene_create", i32, [i32, u32, i32, i32])

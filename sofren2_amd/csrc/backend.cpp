@@ -68,6 +68,15 @@ struct Packer {
         return off;
     }
     template <class T> size_t add(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
+    /* a zeroed section to be filled through host + offset once no later add
+       can move the staging memory */
+    size_t reserve(size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + std::max<size_t>(bytes, 4);
+        if (used > cap) grow(used);
+        memset(host + off, 0, std::max<size_t>(bytes, 4));
+        return off;
+    }
     void grow(size_t need) {
         size_t nc = std::max<size_t>(need * 2, 1 << 20);
         u8* nh = nullptr;
@@ -108,7 +117,13 @@ struct Backend {
     bool est_uploaded = false;
     DBuf blas;                      /* resident BLAS nodes of static models */
     u32 blas_used = 0;
-    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64;
+    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64, scan_sums, raster_items, tile_nch, merge;
+    u32 raster_chunk = 128;         /* payloads per visibility work item */
+    bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<u8> graph_key, last_key;
+    u64 graph_captures = 0;
+    u32 raster_groups = 512;        /* persistent visibility workgroups */
 
     /* per frame */
     DBuf insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
@@ -141,6 +156,7 @@ Backend* backend_create() {
     }
     HIP_CHECK(hipSetDevice(0));
     HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->raster_groups = raster_groups_per_device();
     for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventCreate(&e));
     return b;
 }
@@ -154,7 +170,7 @@ void backend_destroy(Backend* b) {
                     &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
                     &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
                     &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
-                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z, &b->vis64 };
+                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z, &b->vis64, &b->scan_sums, &b->raster_items, &b->tile_nch, &b->merge };
     for (DBuf* d : all) {
         if (d->p) HIP_CHECK(hipFree(d->p));
     }
@@ -164,6 +180,7 @@ void backend_destroy(Backend* b) {
     if (b->pack.host) HIP_CHECK(hipHostFree(b->pack.host));
     if (b->counters_h) HIP_CHECK(hipHostFree(b->counters_h));
     if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
+    if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
     for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventDestroy(e));
     HIP_CHECK(hipStreamDestroy(b->stream));
     delete b;
@@ -174,32 +191,21 @@ void backend_release_texture(Backend* b, u32 index) {
     if (b) b->tex_meta_dirty = true;
 }
 
-/* writes one model's meshlets into slots [base, base + total) of the pools */
-static void upload_model(Backend* b, const sft_model* m, ModelRes& r, u32 base, u32 mesh_rec_base, std::vector<MeshRec>& recs, bool resident) {
+/* Slots, mesh records and skin records of one model whose meshlets go into
+   slots [base, base + total_meshlets) (sofren2.c:8457-8477 order); the rows
+   themselves are written by fill_model_rows. */
+static void plan_model(const sft_model* m, ModelRes& r, u32 base, u32 mesh_rec_base, std::vector<MeshRec>& recs,
+                       u32 skin_base, u32* n_skin_out) {
     const u32 T = m->total_meshlets;
     r.base = base;
     r.total = T;
     r.mesh_rec_base = mesh_rec_base;
     r.skin_of.assign(T, kInvalid);
     r.max_bone = (u32)std::max(m->skeleton.bone_count, 0);
-    const size_t slots = (size_t)base + T;
-    dgrow(b->pos, slots * 768, true, b->stream);
-    dgrow(b->attr, slots * 1792, true, b->stream);
-    dgrow(b->inds, slots * 512, true, b->stream);
-    dgrow(b->cull, slots * 32, true, b->stream);
-    dgrow(b->slot_mesh, slots * 4, true, b->stream);
-    if (0 == T) return;
-
-    std::vector<f32> pos((size_t)T * 192), attr((size_t)T * 448), cull((size_t)T * 8, 0.f);
-    std::vector<u8> inds((size_t)T * 512, 0), skin;
-    std::vector<u32> slot_mesh(T, mesh_rec_base);
+    r.blas_base = kInvalid;
+    r.blas_count = 0;
+    r.blas_root = 0;
     u32 n_skin = 0;
-    for (u32 i = 0; i < m->mesh_count; i += 1) {
-        if (m->meshes[i].meshlet_skins) n_skin += m->meshes[i].meshlet_count;
-    }
-    const u32 skin_base = b->resident_skins;
-    if (n_skin) skin.assign((size_t)n_skin * 768, 0);
-    u32 skin_next = 0;
     for (u32 i = 0; i < m->mesh_count; i += 1) {
         const sft_mesh* mesh = &m->meshes[i];
         MeshRec rec;
@@ -214,24 +220,10 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r, u32 base, 
                 fprintf(stderr, "fatal sofren2 error: mesh %u meshlet %u lies outside the model's %u meshlets\n", i, k, T);
                 exit(1);
             }
-            memcpy(&pos[(size_t)gm * 192], &mesh->meshlets[k], 768);
-            memcpy(&attr[(size_t)gm * 448], &mesh->meshlet_attribs[k], 1792);
-            memcpy(&inds[(size_t)gm * 512], &mesh->inds[k], 512);
-            f32* c = &cull[(size_t)gm * 8];
-            c[0] = mesh->bounds_center_x[k];
-            c[1] = mesh->bounds_center_y[k];
-            c[2] = mesh->bounds_center_z[k];
-            c[3] = mesh->bounds_radius[k];
-            c[4] = mesh->cone_axis_x[k];
-            c[5] = mesh->cone_axis_y[k];
-            c[6] = mesh->cone_axis_z[k];
-            c[7] = mesh->cone_cutoff[k];
-            slot_mesh[gm] = mesh_rec_base + i;
             if (mesh->meshlet_skins) {
                 const sft_meshlet_skin* sk = &mesh->meshlet_skins[k];
-                memcpy(&skin[(size_t)skin_next * 768], sk, 768);
-                r.skin_of[gm] = skin_base + skin_next;
-                skin_next += 1;
+                r.skin_of[gm] = skin_base + n_skin;
+                n_skin += 1;
                 for (int v = 0; v < 64; v += 1) {
                     const u32 jm = std::max(std::max(sk->j0[v], sk->j1[v]), std::max(sk->j2[v], sk->j3[v]));
                     r.max_bone = std::max(r.max_bone, jm + 1u);
@@ -242,34 +234,88 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r, u32 base, 
             }
         }
     }
-    hipStream_t s = b->stream;
-    HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + (size_t)base * 768, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + (size_t)base * 1792, attr.data(), attr.size() * 4, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + (size_t)base * 512, inds.data(), inds.size(), hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + (size_t)base * 32, cull.data(), cull.size() * 4, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + (size_t)base * 4, slot_mesh.data(), slot_mesh.size() * 4, hipMemcpyHostToDevice, s));
-    if (n_skin) {
-        dgrow(b->skin, ((size_t)skin_base + n_skin) * 768, true, s);
-        HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)skin_base * 768, skin.data(), skin.size(), hipMemcpyHostToDevice, s));
-        if (resident) b->resident_skins += n_skin;
+    *n_skin_out = n_skin;
+}
+
+/* writes a planned model's rows into staging arrays whose row 0 is slot
+   `row0` (and skin row 0 is skin record `skin_row0`) */
+static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32 skin_row0, f32* pos, f32* attr,
+                            u8* inds, f32* cull, u32* slot_mesh, u8* skin) {
+    for (u32 i = 0; i < m->mesh_count; i += 1) {
+        const sft_mesh* mesh = &m->meshes[i];
+        for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
+            const u32 gm = mesh->model_meshlet_offset + k;
+            const size_t row = (size_t)(r.base - row0) + gm;
+            memcpy(&pos[row * 192], &mesh->meshlets[k], 768);
+            memcpy(&attr[row * 448], &mesh->meshlet_attribs[k], 1792);
+            memcpy(&inds[row * 512], &mesh->inds[k], 512);
+            f32* c = &cull[row * 8];
+            c[0] = mesh->bounds_center_x[k];
+            c[1] = mesh->bounds_center_y[k];
+            c[2] = mesh->bounds_center_z[k];
+            c[3] = mesh->bounds_radius[k];
+            c[4] = mesh->cone_axis_x[k];
+            c[5] = mesh->cone_axis_y[k];
+            c[6] = mesh->cone_axis_z[k];
+            c[7] = mesh->cone_cutoff[k];
+            slot_mesh[row] = r.mesh_rec_base + i;
+            if (mesh->meshlet_skins) memcpy(&skin[(size_t)(r.skin_of[gm] - skin_row0) * 768], &mesh->meshlet_skins[k], 768);
+        }
     }
+}
+
+static void grow_pools(Backend* b, size_t slots, size_t skins) {
+    dgrow(b->pos, slots * 768, true, b->stream);
+    dgrow(b->attr, slots * 1792, true, b->stream);
+    dgrow(b->inds, slots * 512, true, b->stream);
+    dgrow(b->cull, slots * 32, true, b->stream);
+    dgrow(b->slot_mesh, slots * 4, true, b->stream);
+    dgrow(b->skin, std::max<size_t>(skins, 1) * 768, true, b->stream);
+}
+
+/* makes a model resident: plans it after the resident slots and copies its
+   rows (and, for a static model, its BLAS) to HBM */
+static void upload_model(Backend* b, const sft_model* m, ModelRes& r) {
+    u32 n_skin = 0;
+    plan_model(m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, b->resident_skins, &n_skin);
+    const u32 T = m->total_meshlets;
+    grow_pools(b, (size_t)r.base + T, (size_t)b->resident_skins + n_skin);
+    hipStream_t s = b->stream;
+    if (T) {
+        std::vector<f32> pos((size_t)T * 192), attr((size_t)T * 448), cull((size_t)T * 8, 0.f);
+        std::vector<u8> inds((size_t)T * 512, 0), skin((size_t)std::max(n_skin, 1u) * 768, 0);
+        std::vector<u32> slot_mesh(T, r.mesh_rec_base);
+        fill_model_rows(m, r, r.base, b->resident_skins, pos.data(), attr.data(), inds.data(), cull.data(), slot_mesh.data(),
+                        skin.data());
+        HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + (size_t)r.base * 768, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + (size_t)r.base * 1792, attr.data(), attr.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + (size_t)r.base * 512, inds.data(), inds.size(), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + (size_t)r.base * 32, cull.data(), cull.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + (size_t)r.base * 4, slot_mesh.data(), slot_mesh.size() * 4,
+                                 hipMemcpyHostToDevice, s));
+        if (n_skin) {
+            HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)b->resident_skins * 768, skin.data(), (size_t)n_skin * 768,
+                                     hipMemcpyHostToDevice, s));
+        }
+        /* the staging vectors die at scope exit: finish the copies first */
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    b->resident_slots += T;
+    b->resident_skins += n_skin;
+    b->resident_mesh_recs = (u32)b->mesh_recs.size();
     /* BLAS of a static resident model: shadow rays test static geometry only
        (sofren2.c:4130), so skinned models keep none */
-    r.blas_base = kInvalid;
-    r.blas_count = 0;
-    r.blas_root = 0;
-    if (resident && m->blas_nodes && m->blas_node_count > 0 && m->skeleton.bone_count <= 0) {
+    if (m->blas_nodes && m->blas_node_count > 0 && m->skeleton.bone_count <= 0) {
         const u32 nn = m->blas_node_count;
         dgrow(b->blas, ((size_t)b->blas_used + nn) * sizeof(sft_bvh_node), true, s);
         HIP_CHECK(hipMemcpyAsync((u8*)b->blas.p + (size_t)b->blas_used * sizeof(sft_bvh_node), m->blas_nodes,
                                  (size_t)nn * sizeof(sft_bvh_node), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
         r.blas_base = b->blas_used;
         r.blas_count = nn;
         r.blas_root = m->blas_root_ind;
         b->blas_used += nn;
     }
-    /* the staging vectors die at scope exit: finish the copies first */
-    HIP_CHECK(hipStreamSynchronize(s));
 }
 
 static void sync_textures(Backend* b, Context& c) {
@@ -313,6 +359,48 @@ static f32 ev_ms(hipEvent_t a, hipEvent_t z) {
     return ms;
 }
 
+/* Runs the batch's kernels.  With graphs on, the launch sequence is captured
+   once into a HIP graph and replayed while the kernel arguments and launch
+   sizes stay the same (per-frame data changes only buffer contents); any
+   change re-captures it.  Per-pass timings need graphs off. */
+static void run_frame(Backend* b, const FramePtrs& p, const BatchGpu& bg, size_t zero_words) {
+    hipStream_t s = b->stream;
+    if (!b->use_graphs) {
+        launch_frame(p, bg, zero_words, s, b->ev);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    std::vector<u8> key(sizeof(p) + sizeof(bg) + sizeof(zero_words));
+    memcpy(key.data(), &p, sizeof(p));
+    memcpy(key.data() + sizeof(p), &bg, sizeof(bg));
+    memcpy(key.data() + sizeof(p) + sizeof(bg), &zero_words, sizeof(zero_words));
+    if (!b->graph_exec || key != b->graph_key) {
+        /* capture a launch sequence the second time in a row it is seen, so
+           frames of changing shape are not charged a capture each */
+        if (key != b->last_key) {
+            b->last_key = key;
+            launch_frame(p, bg, zero_words, s, b->ev);
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+        if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
+        b->graph_exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_frame(p, bg, zero_words, s, nullptr);
+        HIP_CHECK(hipStreamEndCapture(s, &g));
+        HIP_CHECK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(g));
+        b->graph_key = key;
+        b->graph_captures += 1;
+    }
+    /* pass boundaries are not visible inside the graph: the passes read as
+       one span between the first and the last mark */
+    HIP_CHECK(hipEventRecord(b->ev[1], s));
+    HIP_CHECK(hipGraphLaunch(b->graph_exec, s));
+    for (int k : { 7, 2, 3, 6, 4 }) HIP_CHECK(hipEventRecord(b->ev[k], s));
+}
+
 /* frame-table rows of one view inside the batch */
 struct ViewPlan {
     u32 inst_base = 0, work_base = 0, n_work = 0, pl_base = 0, dl_base = 0, tmat_base = 0;
@@ -347,9 +435,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
             if (it != b->models.end() && it->second.total == m->total_meshlets) continue;
             ModelRes r;
             b->mesh_recs.resize(b->resident_mesh_recs);
-            upload_model(b, m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, true);
-            b->resident_slots += m->total_meshlets;
-            b->resident_mesh_recs = (u32)b->mesh_recs.size();
+            upload_model(b, m, r);
             b->models[m] = std::move(r);
         }
     }
@@ -370,18 +456,22 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
             }
         }
     }
+    /* transient models (dmesh geometry) take the slots after the resident
+       ones; their rows are staged with the frame tables and copied in bulk */
     std::vector<std::unordered_map<const sft_model*, ModelRes>> tmodels(nv);
-    u32 tslot = b->resident_slots;
-    const u32 saved_skins = b->resident_skins;
+    u32 tslot = b->resident_slots, tskin = b->resident_skins;
     for (u32 vi = 0; vi < nv; vi += 1) {
         for (const sft_model* m : views[vi].transient_models) {
             ModelRes r;
-            upload_model(b, m, r, tslot, (u32)b->mesh_recs.size(), b->mesh_recs, false);
+            u32 n_skin = 0;
+            plan_model(m, r, tslot, (u32)b->mesh_recs.size(), b->mesh_recs, tskin, &n_skin);
             tslot += m->total_meshlets;
+            tskin += n_skin;
             tmodels[vi][m] = std::move(r);
         }
     }
-    b->resident_skins = saved_skins;
+    const u32 t_slots = tslot - b->resident_slots, t_skins = tskin - b->resident_skins;
+    if (t_slots) grow_pools(b, tslot, tskin);
     sync_textures(b, c);
 
     /* ---- batch tables: views' rows back to back ---- */
@@ -579,6 +669,10 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     bg.occ_total = occ_total;
     bg.n_lights = n_lights_cache;
     bg.vis64 = any_vis64 ? 1u : 0u;
+    bg.raster_chunk = b->raster_chunk;
+    bg.raster_groups = b->raster_groups;
+    dgrow(b->tile_nch, (size_t)nt * 4, false, s);
+    dgrow(b->merge, (size_t)nt * max_samples * 4096 * 8, false, s);
 
     /* ---- uploads ---- */
     Packer& pk = b->pack;
@@ -601,6 +695,20 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     const size_t o_mr = pk.add(b->mesh_recs);
     const size_t o_px = pk.add(proxies);
     const size_t o_tlas = pk.add(tlas);
+    /* transient rows, filled in place once every section is reserved */
+    const size_t o_tpos = pk.reserve((size_t)t_slots * 768);
+    const size_t o_tattr = pk.reserve((size_t)t_slots * 1792);
+    const size_t o_tinds = pk.reserve((size_t)t_slots * 512);
+    const size_t o_tcull = pk.reserve((size_t)t_slots * 32);
+    const size_t o_tsm = pk.reserve((size_t)t_slots * 4);
+    const size_t o_tskin = pk.reserve((size_t)t_skins * 768);
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        for (const auto& kv : tmodels[vi]) {
+            fill_model_rows(kv.first, kv.second, b->resident_slots, b->resident_skins, (f32*)(pk.host + o_tpos),
+                            (f32*)(pk.host + o_tattr), pk.host + o_tinds, (f32*)(pk.host + o_tcull),
+                            (u32*)(pk.host + o_tsm), pk.host + o_tskin);
+        }
+    }
 
     const u32 nw = bg.n_work;
     const bool single = 1 == nv;
@@ -613,7 +721,8 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     dgrow(b->clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
     dgrow(b->batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
     dgrow(b->tile_off, (size_t)nt * 4, false, s);
-    const size_t zero_words = COUNTER_WORDS + 2 * (size_t)nt;
+    dgrow(b->scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4, false, s);
+    const size_t zero_words = COUNTER_WORDS + 2 * (size_t)nt + (size_t)nt * max_samples;
     dgrow(b->zero_d, zero_words * 4, false, s);
     if (!b->counters_h) HIP_CHECK(hipHostMalloc((void**)&b->counters_h, CNT_COUNT * 4, hipHostMallocDefault));
     dgrow(b->color, std::max<size_t>(color_total * 4, 4), false, s);
@@ -647,6 +756,10 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         dgrow(b->clip_rect, (size_t)b->clip_cap * 4, false, s);
         bg.clip_cap = b->clip_cap;
         bg.tile_list_cap = b->tile_list_cap;
+        /* visibility work items: at most one per tile and sample plus one per
+           kRasterChunk payloads */
+        const size_t max_items = ((size_t)nt + b->tile_list_cap / bg.raster_chunk + 1) * max_samples;
+        dgrow(b->raster_items, max_items * 8, false, s);
         memcpy(pk.host + o_batch, &bg, sizeof(bg));
         for (u32 vi = 0; vi < nv; vi += 1) {
             vgpu[vi].fc.clip_cap = b->clip_cap;
@@ -655,8 +768,19 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         memcpy(pk.host + o_views, vgpu.data(), vgpu.size() * sizeof(ViewGpu));
         dgrow(b->pack_d, pk.used, false, s);
         HIP_CHECK(hipMemcpyAsync(b->pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemsetAsync(b->zero_d.p, 0, zero_words * 4, s));
         u8* pd = (u8*)b->pack_d.p;
+        if (t_slots) {
+            const size_t r0 = b->resident_slots;
+            HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + r0 * 768, pd + o_tpos, (size_t)t_slots * 768, hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + r0 * 1792, pd + o_tattr, (size_t)t_slots * 1792, hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + r0 * 512, pd + o_tinds, (size_t)t_slots * 512, hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + r0 * 32, pd + o_tcull, (size_t)t_slots * 32, hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + r0 * 4, pd + o_tsm, (size_t)t_slots * 4, hipMemcpyDeviceToDevice, s));
+            if (t_skins) {
+                HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)b->resident_skins * 768, pd + o_tskin, (size_t)t_skins * 768,
+                                         hipMemcpyDeviceToDevice, s));
+            }
+        }
         u32* zd = (u32*)b->zero_d.p;
 
         FramePtrs p;
@@ -706,6 +830,11 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
         p.vis64 = any_vis64 ? (u64*)b->vis64.p : nullptr;
         p.counters = zd;
+        p.scan_sums = (u32*)b->scan_sums.p;
+        p.raster_items = (uint2*)b->raster_items.p;
+        p.tile_nch = (u32*)b->tile_nch.p;
+        p.tile_done = zd + COUNTER_WORDS + 2 * (size_t)nt;
+        p.merge = (u64*)b->merge.p;
         p.blas = (const sft_bvh_node*)b->blas.p;
         p.tlas = (const sft_bvh_node*)(pd + o_tlas);
         p.proxies = (const ProxyGpu*)(pd + o_px);
@@ -713,8 +842,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         p.light_proxy_count = (u32*)b->light_proxy_count.p;
         p.light_proxies = (ProxyRec*)b->light_proxies.p;
 
-        launch_frame(p, bg, s, b->ev);
-        HIP_CHECK(hipGetLastError());
+        run_frame(b, p, bg, zero_words);
         HIP_CHECK(hipMemcpyAsync(b->counters_h, zd, sizeof(counters), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         memcpy(counters, b->counters_h, sizeof(counters));
@@ -890,6 +1018,10 @@ static Backend* ensure_backend() {
 }
 
 extern "C" void sftx_set_output(i32 mode) { ensure_backend()->output_mode = mode; }
+
+extern "C" void sftx_set_graphs(i32 enabled) { ensure_backend()->use_graphs = 0 != enabled; }
+
+extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captures; }
 
 extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->color.p; }
 

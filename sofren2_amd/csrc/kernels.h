@@ -28,6 +28,8 @@ enum Counter : int {
     CNT_OCC_TRIS,           /* ray-triangle tests */
     CNT_OCC_INVALID,        /* packets whose frustum the reference leaves unset */
     CNT_OCC_OVERFLOW,       /* packets whose traversal stack overflowed */
+    CNT_RASTER_ITEMS,       /* visibility work items (tile, sample, chunk) listed */
+    CNT_RASTER_NEXT,        /* queue head of the visibility pass */
     CNT_COUNT
 };
 
@@ -95,11 +97,18 @@ struct FramePtrs {
     f32* cap_depth;         /* [samples][grid] 1/w when capturing, else null */
     u64* vis64;             /* [samples][grid] (1/w bits << 32) | ID when requested, else null */
     u32* counters;          /* [CNT_COUNT] */
+    u32* scan_sums;         /* per-4096-item block sums of the scans */
+    uint2* raster_items;    /* visibility work items: tile, chunk << 2 | sample */
+    u32* tile_nch;          /* [n_tiles] payload chunks of each tile */
+    u32* tile_done;         /* [n_tiles][max_samples] chunks finished (zeroed per frame) */
+    u64* merge;             /* [n_tiles][max_samples][4096] merge planes of split tiles */
 };
 
-/* enqueues the kernels of one batch of views; `ev` (10 events) brackets the
+/* enqueues the kernels of one batch of views, starting with the clear of the
+   `zero_words` counter words at p.counters; `ev` (10 events) brackets the
    passes when non-null */
-void launch_frame(const FramePtrs& p, const BatchGpu& batch, hipStream_t s, hipEvent_t* ev);
+void launch_frame(const FramePtrs& p, const BatchGpu& batch, u64 zero_words, hipStream_t s, hipEvent_t* ev);
 void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s);
+u32 raster_groups_per_device();
 
 }  // namespace sft

@@ -334,51 +334,63 @@ __global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
     }
 }
 
-/* ================================================================== k_scan
-   exclusive scan of n u32 by one workgroup of 1024 threads */
-__global__ __launch_bounds__(1024) void k_scan(const u32* in, u32* out, u32 n, u32* total, u32 cap, u32* overflow, u32 overflow_bit) {
-    __shared__ u32 s_wave[16];
-    __shared__ u32 s_carry;
+/* ================================================================== scans
+   Exclusive scan of n u32 in two passes: k_scan_sums reduces blocks of 4096,
+   k_scan_apply adds the sums of the blocks before it and scans its own block.
+   The last block also writes the total and flags totals over `cap`. */
+constexpr u32 kScanItems = 4096;
+
+DEV u32 block_sum_1024(u32 v, u32* s_wave) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (0 == tid) s_carry = 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (0 == lane) s_wave[wave] = v;
     __syncthreads();
-    for (u32 base = 0; base < n; base += 4096u) {
-        u32 v[4];
-        u32 sum = 0;
-        for (int k = 0; k < 4; k += 1) {
-            const u32 i = base + (u32)tid * 4u + (u32)k;
-            v[k] = (i < n) ? in[i] : 0u;
-            sum += v[k];
-        }
-        u32 incl = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const u32 t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
-        if (63 == lane) s_wave[wave] = incl;
-        __syncthreads();
-        if (0 == wave) {
-            u32 x = (lane < 16) ? s_wave[lane] : 0u;
-            for (int o = 1; o < 16; o <<= 1) {
-                const u32 t = __shfl_up(x, o);
-                if (lane >= o) x += t;
-            }
-            if (lane < 16) s_wave[lane] = x;
-        }
-        __syncthreads();
-        u32 run = s_carry + (wave ? s_wave[wave - 1] : 0u) + (incl - sum);
-        for (int k = 0; k < 4; k += 1) {
-            const u32 i = base + (u32)tid * 4u + (u32)k;
-            if (i < n) out[i] = run;
-            run += v[k];
-        }
-        __syncthreads();
-        if (0 == tid) s_carry += s_wave[15];
-        __syncthreads();
+    u32 t = 0;
+    for (int k = 0; k < 16; k += 1) t += s_wave[k];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(const u32* in, u32 n, u32* sums) {
+    __shared__ u32 s_wave[16];
+    const u32 base = blockIdx.x * kScanItems + threadIdx.x * 4u;
+    u32 v = 0;
+    for (u32 k = 0; k < 4u; k += 1) v += (base + k < n) ? in[base + k] : 0u;
+    const u32 t = block_sum_1024(v, s_wave);
+    if (0 == threadIdx.x) sums[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const u32* in, u32* out, u32 n, const u32* sums, u32* total,
+                                                      u32 cap, u32* overflow, u32 overflow_bit) {
+    __shared__ u32 s_wave[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    /* carry: the sums of all earlier blocks */
+    u32 c = 0;
+    for (u32 b = (u32)tid; b < blockIdx.x; b += 1024u) c += sums[b];
+    const u32 carry = block_sum_1024(c, s_wave);
+    const u32 base = blockIdx.x * kScanItems + (u32)tid * 4u;
+    u32 v[4], sum = 0;
+    for (int k = 0; k < 4; k += 1) {
+        v[k] = (base + (u32)k < n) ? in[base + k] : 0u;
+        sum += v[k];
     }
-    if (0 == tid) {
-        *total = s_carry;
-        if (s_carry > cap) atomicOr(overflow, overflow_bit);
+    u32 incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (63 == lane) s_wave[wave] = incl;
+    __syncthreads();
+    u32 before = 0;
+    for (int k = 0; k < wave; k += 1) before += s_wave[k];
+    u32 run = carry + before + (incl - sum);
+    for (int k = 0; k < 4; k += 1) {
+        if (base + (u32)k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1u && 1023 == tid) {
+        *total = run;
+        if (run > cap) atomicOr(overflow, overflow_bit);
     }
 }
 
@@ -666,9 +678,10 @@ struct Tri {
    the pixel centre: edge and depth values gain A*ox + B*oy, trivial accept
    needs a margin of 0.375 (|A| + |B|), and a chunk whose 8 pixel centres are
    all beyond that margin of an edge is skipped. */
+template <bool aa>
 DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u32 key,
                     f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2,
-                    bool aa, f32 ox, f32 oy) {
+                    f32 ox, f32 oy) {
     const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
     const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
     const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
@@ -736,7 +749,7 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
         for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
             f32 e0 = e0r, e1 = e1r, e2 = e2r, z = z_row;
             for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                if (!aa) {
+                if constexpr (!aa) {
                     if (e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f) {
                         atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
                     }
@@ -1027,7 +1040,11 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
             }
         }
         const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
-        for (int i = 0; i < tl.count; i += 1) {
+        /* on a lightmapped surface point lights act only through capsule
+           shadows (sofren2.c:6574-6579): without proxies every light leaves
+           lm_shadow at exactly 1, so the loop is skipped */
+        const int n_eval = (lightmapped && !shadows) ? 0 : tl.count;
+        for (int i = 0; i < n_eval; i += 1) {
             const u32 li = tl.idx[i];
             const PointLightGpu& L = P.point_lights[VW.pl_base + li];
             const V3 lraw = sub3(V3{L.pos[0], L.pos[1], L.pos[2]}, pos);
@@ -1178,78 +1195,64 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
     if (0 == lane) tl[0] = n;
 }
 
-/* one workgroup per (64x64 tile, sample): rasterise the tile's payloads into
-   an LDS visibility plane, then publish per-sample reference IDs and the
-   plane's 1/w range; without MSAA the tile's light list is built here too
-   (sofren2.c:6756-6813, 5820-5934) */
-constexpr int kRasterThreads = 1024;
+/* Visibility pass.  A work item is (tile, sample plane, chunk of at most
+   kRasterChunk payloads of the tile's list); k_raster_plan lists the items,
+   and a persistent k_raster takes them from a queue, so a tile covered by
+   thousands of payloads is spread over many workgroups.  Each item rasterises
+   into an LDS plane; a tile with one chunk publishes from LDS, the chunks of a
+   split tile merge with 64-bit atomic max into a zeroed HBM plane and the
+   last one to finish publishes from there.  Publishing writes per-sample
+   reference IDs and the plane's 1/w range; without MSAA it also builds the
+   tile's light list (sofren2.c:6756-6813, 5820-5934). */
+constexpr int kRasterThreads = 512;
 constexpr int kRasterWaves = kRasterThreads / 64;
 
-__global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
-    __shared__ u64 s_vis[SFT_TILE_PIXELS];
-    __shared__ f32 s_red[2][kRasterWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+DEV u32 tile_payloads(const FramePtrs& P, u32 tile, u32& off) {
+    const u32 cap = P.batch->tile_list_cap;
+    off = P.tile_off[tile];
+    u32 cnt = P.tile_count[tile];
+    if (off >= cap) cnt = 0;
+    else if (off + cnt > cap) cnt = cap - off;
+    return cnt;
+}
+
+/* one workgroup per tile: chunk count, work items, zeroed merge planes */
+__global__ __launch_bounds__(256) void k_raster_plan(FramePtrs P) {
+    __shared__ u32 s_at;
     const u32 tile = blockIdx.x;
-    const int sample = (int)blockIdx.y;
     const ViewGpu& V = P.views[P.tile_view[tile]];
-    const FrameConst& fc = V.fc;
-    if (sample >= fc.samples) return;
-    const bool aa = fc.samples > 1;
-    const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
-    const u32 local = tile - V.tile_base;
-    const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
-    const Est est = make_est(P);
-    const u32 list_cap = P.batch->tile_list_cap;
-
-    for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
-    __syncthreads();
-
-    /* a wave per payload, 8 lanes per triangle, lane = x inside the 8-px chunk */
-    {
-        const u32 off = P.tile_off[tile];
-        u32 cnt = P.tile_count[tile];
-        if (off >= list_cap) cnt = 0;
-        else if (off + cnt > list_cap) cnt = list_cap - off;
-        const int g = lane >> 3, j = lane & 7;
-        for (u32 e = (u32)wave; e < cnt; e += (u32)kRasterWaves) {
-            const u64 desc = P.tile_list[off + e];
-            const u32 mask = (u32)(desc >> 32) & 0xFFu;
-            const u32 idx = (u32)desc;
-            f32 sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2;
-            u32 key;
-            if (0u == mask) {
-                if (0 != g) continue;
-                const f32* r = P.clip_rec + (u64)idx * 41u;
-                sx0 = r[0]; sx1 = r[1]; sx2 = r[2];
-                sy0 = r[3]; sy1 = r[4]; sy2 = r[5];
-                iw0 = r[6]; iw1 = r[7]; iw2 = r[8];
-                key = P.clip_key[idx];
-            }
-            else {
-                if (0u == ((mask >> g) & 1u)) continue;
-                const u32 wi = idx >> 4, b = idx & 15u;
-                const WorkItem w = P.work[wi];
-                const u8* ib = P.inds + (u64)w.slot * 512u + b * 32u + (u32)g;
-                const f32* ss = P.ss + (u64)w.ss * 576u;
-                const int i0 = ib[0] & 63, i1 = ib[8] & 63, i2 = ib[16] & 63;
-                sx0 = ss[i0]; sy0 = ss[64 + i0]; iw0 = ss[128 + i0];
-                sx1 = ss[i1]; sy1 = ss[64 + i1]; iw1 = ss[128 + i1];
-                sx2 = ss[i2]; sy2 = ss[64 + i2]; iw2 = ss[128 + i2];
-                key = (wi << 9) | (b << 5) | (16u + (u32)g);
-            }
-            raster_tri(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, aa, ox, oy);
-        }
+    const u32 S = (u32)V.fc.samples, K = P.batch->raster_chunk, maxS = P.batch->max_samples;
+    u32 off;
+    const u32 cnt = tile_payloads(P, tile, off);
+    const u32 nch = max(1u, (cnt + K - 1u) / K);
+    if (0 == threadIdx.x) {
+        s_at = atomicAdd(&P.counters[CNT_RASTER_ITEMS], nch * S);
+        P.tile_nch[tile] = nch;
     }
     __syncthreads();
+    for (u32 i = threadIdx.x; i < nch * S; i += 256u) {
+        P.raster_items[s_at + i] = make_uint2(tile, ((i / S) << 2) | (i % S));
+    }
+    if (nch > 1u) {
+        u64* m = P.merge + (u64)tile * maxS * SFT_TILE_PIXELS;
+        for (u32 i = threadIdx.x; i < S * (u32)SFT_TILE_PIXELS; i += 256u) m[i] = 0ull;
+    }
+}
 
-    /* publish IDs, and the plane's 1/w range over every sample (screen edge included) */
+/* per-pixel publish of one plane of a tile; `get(i)` is the packed
+   (1/w bits << 32 | ~key) of pixel i */
+template <class Get>
+DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sample, i32 tmx, i32 tmy, Get get,
+                       f32 (*s_red)[kRasterWaves]) {
+    const FrameConst& fc = V.fc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const u32 n_clip = min(P.counters[CNT_CLIP_TRIS], P.batch->clip_cap);
     const u32 clip_base = view_clip_base(P, V);
     const i32 gw = fc.tile_cols * 64;
     const u64 plane = V.vid_base + (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
     f32 zmin = 1e9f, zmax = -1e9f;
     for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
-        const u64 pk = s_vis[i];
+        const u64 pk = get(i);
         const f32 z = u2f((u32)(pk >> 32));
         if (z > 0.0001f) {
             zmin = fminf(zmin, z);
@@ -1267,21 +1270,110 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     }
     if (0 == lane) { s_red[0][wave] = zmin; s_red[1][wave] = zmax; }
     __syncthreads();
-    if (0 != wave) return;
-    zmin = s_red[0][0];
-    zmax = s_red[1][0];
-    for (int k = 1; k < kRasterWaves; k += 1) {
-        zmin = fminf(zmin, s_red[0][k]);
-        zmax = fmaxf(zmax, s_red[1][k]);
-    }
-    if (aa) {
-        if (0 == lane) {
-            P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u] = zmin;
-            P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u + 1u] = zmax;
+    if (0 == wave) {
+        zmin = s_red[0][0];
+        zmax = s_red[1][0];
+        for (int k = 1; k < kRasterWaves; k += 1) {
+            zmin = fminf(zmin, s_red[0][k]);
+            zmax = fmaxf(zmax, s_red[1][k]);
         }
-        return;
+        if (fc.samples > 1) {
+            if (0 == lane) {
+                P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u] = zmin;
+                P.tile_z[((u64)tile * P.batch->max_samples + (u32)sample) * 2u + 1u] = zmax;
+            }
+        }
+        else {
+            build_tile_lights(P, V, tile, zmin, zmax, lane);
+        }
     }
-    build_tile_lights(P, V, tile, zmin, zmax, lane);
+}
+
+__global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
+    __shared__ u64 s_vis[SFT_TILE_PIXELS];
+    __shared__ f32 s_red[2][kRasterWaves];
+    __shared__ u32 s_item, s_last;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Est est = make_est(P);
+    const u32 n_items = P.counters[CNT_RASTER_ITEMS];
+    const u32 K = P.batch->raster_chunk, maxS = P.batch->max_samples;
+    for (;;) {
+        if (0 == tid) s_item = atomicAdd(&P.counters[CNT_RASTER_NEXT], 1u);
+        for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
+        __syncthreads();
+        const u32 it = s_item;
+        if (it >= n_items) break;
+        const uint2 w = P.raster_items[it];
+        const u32 tile = w.x, chunk = w.y >> 2;
+        const int sample = (int)(w.y & 3u);
+        const ViewGpu& V = P.views[P.tile_view[tile]];
+        const FrameConst& fc = V.fc;
+        const bool aa = fc.samples > 1;
+        const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
+        const u32 local = tile - V.tile_base;
+        const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
+
+        /* a wave per payload, 8 lanes per triangle, lane = x inside the 8-px chunk */
+        u32 off;
+        const u32 cnt = tile_payloads(P, tile, off);
+        const u32 e0 = chunk * K, e1 = min(cnt, e0 + K);
+        const int g = lane >> 3, j = lane & 7;
+        for (u32 e = e0 + (u32)wave; e < e1; e += (u32)kRasterWaves) {
+            const u64 desc = P.tile_list[off + e];
+            const u32 mask = (u32)(desc >> 32) & 0xFFu;
+            const u32 idx = (u32)desc;
+            f32 sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2;
+            u32 key;
+            if (0u == mask) {
+                if (0 != g) continue;
+                const f32* r = P.clip_rec + (u64)idx * 41u;
+                sx0 = r[0]; sx1 = r[1]; sx2 = r[2];
+                sy0 = r[3]; sy1 = r[4]; sy2 = r[5];
+                iw0 = r[6]; iw1 = r[7]; iw2 = r[8];
+                key = P.clip_key[idx];
+            }
+            else {
+                if (0u == ((mask >> g) & 1u)) continue;
+                const u32 wi = idx >> 4, b = idx & 15u;
+                const WorkItem wk = P.work[wi];
+                const u8* ib = P.inds + (u64)wk.slot * 512u + b * 32u + (u32)g;
+                const f32* ss = P.ss + (u64)wk.ss * 576u;
+                const int i0 = ib[0] & 63, i1 = ib[8] & 63, i2 = ib[16] & 63;
+                sx0 = ss[i0]; sy0 = ss[64 + i0]; iw0 = ss[128 + i0];
+                sx1 = ss[i1]; sy1 = ss[64 + i1]; iw1 = ss[128 + i1];
+                sx2 = ss[i2]; sy2 = ss[64 + i2]; iw2 = ss[128 + i2];
+                key = (wi << 9) | (b << 5) | (16u + (u32)g);
+            }
+            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy);
+            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy);
+        }
+        __syncthreads();
+
+        if (1u == P.tile_nch[tile]) {
+            publish_plane(P, V, tile, sample, tmx, tmy, [&](int i) { return s_vis[i]; }, s_red);
+        }
+        else {
+            u64* m = P.merge + ((u64)tile * maxS + (u32)sample) * SFT_TILE_PIXELS;
+            for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
+                const u64 v = s_vis[i];
+                if (v) atomicMax((unsigned long long*)&m[i], (unsigned long long)v);
+            }
+            __threadfence();
+            __syncthreads();
+            if (0 == tid) {
+                const u32 done = atomicAdd(&P.tile_done[(u64)tile * maxS + (u32)sample], 1u);
+                s_last = (done + 1u == P.tile_nch[tile]) ? 1u : 0u;
+            }
+            __syncthreads();
+            if (s_last) {
+                __threadfence();
+                publish_plane(P, V, tile, sample, tmx, tmy,
+                              [&](int i) { return __hip_atomic_load(&m[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); },
+                              s_red);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 /* MSAA: the tile's 1/w range over all sample planes, then its light list */
@@ -1918,6 +2010,12 @@ __global__ __launch_bounds__(64) void k_stats(u32* counters) {
 }
 
 /* streaming copy used to calibrate HBM byte counters */
+/* clears the frame's counters, tile counters and chunk completion counts; a
+   kernel rather than a memset so that the whole frame can live in a HIP graph */
+__global__ __launch_bounds__(256) void k_zero(u32* __restrict__ p, u64 n) {
+    for (u64 i = (u64)blockIdx.x * 256u + threadIdx.x; i < n; i += (u64)gridDim.x * 256u) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, u64 n) {
     for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (u64)gridDim.x * 256ull) dst[i] = src[i];
 }
@@ -1927,8 +2025,25 @@ void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n16);
 }
 
-void launch_frame(const FramePtrs& p, const BatchGpu& B, hipStream_t s, hipEvent_t* ev) {
+/* resident workgroups of the persistent visibility kernel on the device */
+u32 raster_groups_per_device() {
+    int cus = 0, per_cu = 0;
+    if (hipSuccess != hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0)) cus = 1;
+    if (hipSuccess != hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_raster, kRasterThreads, 0)) per_cu = 1;
+    return (u32)std::max(cus, 1) * (u32)std::max(per_cu, 1);
+}
+
+void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStream_t s, hipEvent_t* ev) {
     auto mark = [&](int k) { if (ev) hipEventRecord(ev[k], s); };
+    hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((zero_words + 255u) / 256u, 1024u)), dim3(256), 0, s, p.counters,
+                       zero_words);
+    auto scan = [&](const u32* in, u32* out, u32 n, u32* total, u32 cap, u32 bit) {
+        const u32 blocks = (n + kScanItems - 1u) / kScanItems;
+        if (0u == blocks) return;
+        hipLaunchKernelGGL(k_scan_sums, dim3(blocks), dim3(1024), 0, s, in, n, p.scan_sums);
+        hipLaunchKernelGGL(k_scan_apply, dim3(blocks), dim3(1024), 0, s, in, out, n, (const u32*)p.scan_sums, total, cap,
+                           p.counters + CNT_OVERFLOW, bit);
+    };
     const u32 nw = B.n_work;
     const u32 geom_blocks = (nw + 3u) / 4u;
     mark(1);
@@ -1938,8 +2053,7 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, hipStream_t s, hipEvent
     mark(7);
     if (nw > 0) {
         hipLaunchKernelGGL(k_geom, dim3(geom_blocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.clip_count, p.clip_base, nw,
-                           p.counters + CNT_CLIP_TRIS, B.clip_cap, p.counters + CNT_OVERFLOW, 1u);
+        scan(p.clip_count, p.clip_base, nw, p.counters + CNT_CLIP_TRIS, B.clip_cap, 1u);
     }
     mark(2);
     /* upper bound of payload items: batch slots plus every clip slot */
@@ -1948,11 +2062,11 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, hipStream_t s, hipEvent
         hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(k_bin<false>, dim3(bin_blocks), dim3(256), 0, s, p);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const u32*)p.tile_count, p.tile_off, B.n_tiles,
-                       p.counters + CNT_TILE_PAIRS, B.tile_list_cap, p.counters + CNT_OVERFLOW, 2u);
+    scan(p.tile_count, p.tile_off, B.n_tiles, p.counters + CNT_TILE_PAIRS, B.tile_list_cap, 2u);
     if (nw > 0) hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
     mark(3);
-    hipLaunchKernelGGL(k_raster, dim3(B.n_tiles, B.max_samples), dim3(kRasterThreads), 0, s, p);
+    hipLaunchKernelGGL(k_raster_plan, dim3(B.n_tiles), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_raster, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
     if (B.max_samples > 1u) hipLaunchKernelGGL(k_tile_lights, dim3(B.n_tiles), dim3(64), 0, s, p);
     mark(6);
     if (B.n_shade_tiles) hipLaunchKernelGGL(k_shade, dim3(B.n_shade_tiles * 16u), dim3(256), 0, s, p);

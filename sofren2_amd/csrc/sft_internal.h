@@ -67,7 +67,7 @@ struct BatchGpu {
     u32 n_views, n_work, n_tiles, max_samples;
     u32 clip_cap, tile_list_cap, rcp_bits, rsq_bits;
     u32 n_shade_tiles, n_shade_tiles_ms, occ_total, n_lights;
-    u32 vis64, pad0, pad1, pad2;
+    u32 vis64, raster_chunk, raster_groups, pad2;
 };
 
 struct InstGpu {

@@ -138,3 +138,29 @@ def test_batched_views_match_reference():
     finally:
         lib.scene_destroy()
         lib.sft_release()
+
+
+@pytest.mark.parametrize("scene", [S.SCENE_BOX_ROOM, S.SCENE_CROWD_SHADOWS])
+def test_graph_replays_match_eager_launches(scene):
+    """Frames replayed from the captured HIP graph equal frames launched
+    eagerly, replay after replay, with eager frames in between."""
+    w, h = S.SCENE_SIZE[scene]
+    r = S.SceneRenderer(scene, 0, w, h)
+    try:
+        frames = [0, 7, 19, 44, 90]
+        r.lib.set_graphs(0)
+        eager = [r.render(f).copy() for f in frames]
+        r.lib.set_graphs(1)
+        c0 = r.lib.graph_captures()
+        for k, f in enumerate(frames):
+            got = r.render(f)
+            print("graph replay", scene, f, flush=True)
+            np.testing.assert_array_equal(got, eager[k])
+            r.lib.set_graphs(0)
+            np.testing.assert_array_equal(r.render(frames[-1 - k]), eager[-1 - k])
+            r.lib.set_graphs(1)
+        captures = r.lib.graph_captures() - c0
+        print("captures", captures, flush=True)
+        assert 1 <= captures <= 2  # the second frame of a shape captures it
+    finally:
+        r.close()
