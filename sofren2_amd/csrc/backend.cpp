@@ -100,12 +100,51 @@ struct ModelRes {
     std::vector<u32> skin_of;       /* skin record per model meshlet, kInvalid if none */
 };
 
-struct Backend {
+/* Everything one batch in flight owns: its stream and events, the per-batch
+   device buffers, the pinned staging, the captured graph, and what the host
+   prepared so the batch can be completed (or re-run after an overflow). */
+struct FrameSet {
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
+    bool in_flight = false;
 
+    DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
+    DBuf tile_off, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth;
+    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64, scan_sums, raster_items, tile_nch, merge;
+    Packer pack;                    /* per-batch tables, staged in pinned memory */
+    DBuf pack_d;
+    DBuf zero_d;                    /* counters, tile counts and fill cursors, chunk completions */
+    u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
+    u8* out_h = nullptr;            /* pinned staging of host outputs (colour, 64-bit visibility) */
+    size_t out_h_cap = 0;
+    u32 clip_cap = 1u << 16;
+    u32 tile_list_cap = 1u << 20;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<u8> graph_key, last_key;
+
+    /* prepared batch */
+    BatchGpu bg{};
+    std::vector<ViewGpu> vgpu;
+    struct ViewPlan {
+        u32 inst_base = 0, work_base = 0, n_work = 0, pl_base = 0, dl_base = 0, tmat_base = 0;
+        u32 tile_base = 0, proxy_base = 0, tlas_base = 0, occ_base = 0, occ_count = 0;
+        u64 vid_base = 0, color_base = 0, grid = 0;
+        u32 n_tiles = 0;
+    };
+    std::vector<ViewPlan> plan;
+    u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
+    u64 vid_total = 0, color_total = 0;
+    size_t zero_words = 0;
+    bool any_vis64 = false, cap_vis = false, host_color = false;
+    size_t o_batch = 0, o_views = 0, o_tv = 0, o_st = 0, o_insts = 0, o_work = 0, o_bones = 0, o_pl = 0, o_dl = 0;
+    size_t o_mt = 0, o_mc = 0, o_ms = 0, o_tt = 0, o_tc = 0, o_ts = 0, o_mr = 0, o_px = 0, o_tlas = 0;
+    size_t o_tpos = 0, o_tattr = 0, o_tinds = 0, o_tcull = 0, o_tsm = 0, o_tskin = 0;
+    u32 counters[CNT_COUNT] = {};
+};
+
+struct Backend {
     /* resident pools */
-    DBuf pos, attr, inds, cull, slot_mesh, skin, mesh_recs_d;
+    DBuf pos, attr, inds, cull, slot_mesh, skin;
     u32 resident_slots = 0, resident_skins = 0, resident_mesh_recs = 0;
     std::vector<MeshRec> mesh_recs;
     std::unordered_map<const sft_model*, ModelRes> models;
@@ -117,27 +156,16 @@ struct Backend {
     bool est_uploaded = false;
     DBuf blas;                      /* resident BLAS nodes of static models */
     u32 blas_used = 0;
-    DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64, scan_sums, raster_items, tile_nch, merge;
-    u32 raster_chunk = 128;         /* payloads per visibility work item */
-    bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<u8> graph_key, last_key;
-    u64 graph_captures = 0;
-    u32 raster_groups = 512;        /* persistent visibility workgroups */
 
-    /* per frame */
-    DBuf insts, work, bones, plights, dlights, mat_tex, mat_color, mat_shin, tmat_tex, tmat_color, tmat_shin;
-    DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
-    DBuf tile_count, tile_off, tile_fill, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth, counters;
+    /* settings */
     i32 output_mode = SFTX_OUTPUT_HOST;
-    Packer pack;                    /* per-frame tables, staged in pinned memory */
-    DBuf pack_d;
-    DBuf zero_d;                    /* counters + tile counts + tile fill cursors */
-    u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
-    u32* color_h = nullptr;         /* pinned staging of the host-output frame */
-    size_t color_h_cap = 0;
-    u32 clip_cap = 1u << 16;
-    u32 tile_list_cap = 1u << 20;
+    u32 raster_chunk = 128;         /* payloads per visibility work item */
+    u32 raster_groups = 512;        /* persistent visibility workgroups */
+    bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
+    u64 graph_captures = 0;
+
+    /* two batches can be in flight (the serving loop); blocking calls use set 0 */
+    FrameSet sets[2];
 
     /* host copies of the last frame's intermediates (capture) */
     i32 cap_cols = 0, cap_rows = 0, cap_samples = 1;
@@ -155,35 +183,48 @@ Backend* backend_create() {
         exit(1);
     }
     HIP_CHECK(hipSetDevice(0));
-    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (FrameSet& fs : b->sets) {
+        HIP_CHECK(hipStreamCreateWithFlags(&fs.stream, hipStreamNonBlocking));
+        for (hipEvent_t& e : fs.ev) HIP_CHECK(hipEventCreate(&e));
+        HIP_CHECK(hipHostMalloc((void**)&fs.counters_h, CNT_COUNT * 4, hipHostMallocDefault));
+    }
     b->raster_groups = raster_groups_per_device();
-    for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventCreate(&e));
     return b;
+}
+
+static void free_buf(DBuf& d) {
+    if (d.p) HIP_CHECK(hipFree(d.p));
+    d.p = nullptr;
+    d.cap = 0;
 }
 
 void backend_destroy(Backend* b) {
     if (!b) return;
-    HIP_CHECK(hipStreamSynchronize(b->stream));
-    DBuf* all[] = { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->mesh_recs_d,
-                    &b->texels, &b->tex_meta_d, &b->est_rcp, &b->est_rsq, &b->insts, &b->work,
-                    &b->bones, &b->plights, &b->dlights, &b->mat_tex, &b->mat_color, &b->mat_shin,
-                    &b->tmat_tex, &b->tmat_color, &b->tmat_shin, &b->ss, &b->ss_vis, &b->clip_count,
-                    &b->clip_base, &b->clip_mask, &b->batch_pl, &b->tile_count, &b->tile_off, &b->tile_fill,
-                    &b->tile_list, &b->clip_rec, &b->clip_key, &b->clip_rect, &b->color, &b->vid, &b->tile_lights,
-                    &b->cap_depth, &b->counters, &b->blas, &b->proxy_vis, &b->light_proxy_count, &b->light_proxies, &b->tile_z, &b->vis64, &b->scan_sums, &b->raster_items, &b->tile_nch, &b->merge };
-    for (DBuf* d : all) {
-        if (d->p) HIP_CHECK(hipFree(d->p));
+    for (FrameSet& fs : b->sets) HIP_CHECK(hipStreamSynchronize(fs.stream));
+    for (DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d,
+                     &b->est_rcp, &b->est_rsq, &b->blas }) {
+        free_buf(*d);
     }
-    for (DBuf* d : { &b->pack_d, &b->zero_d }) {
-        if (d->p) HIP_CHECK(hipFree(d->p));
+    for (FrameSet& fs : b->sets) {
+        for (DBuf* d : { &fs.ss, &fs.ss_vis, &fs.clip_count, &fs.clip_base, &fs.clip_mask, &fs.batch_pl, &fs.tile_off,
+                         &fs.tile_list, &fs.clip_rec, &fs.clip_key, &fs.clip_rect, &fs.color, &fs.vid, &fs.tile_lights,
+                         &fs.cap_depth, &fs.proxy_vis, &fs.light_proxy_count, &fs.light_proxies, &fs.tile_z, &fs.vis64,
+                         &fs.scan_sums, &fs.raster_items, &fs.tile_nch, &fs.merge, &fs.pack_d, &fs.zero_d }) {
+            free_buf(*d);
+        }
+        if (fs.pack.host) HIP_CHECK(hipHostFree(fs.pack.host));
+        if (fs.counters_h) HIP_CHECK(hipHostFree(fs.counters_h));
+        if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
+        if (fs.graph_exec) HIP_CHECK(hipGraphExecDestroy(fs.graph_exec));
+        for (hipEvent_t& e : fs.ev) HIP_CHECK(hipEventDestroy(e));
+        HIP_CHECK(hipStreamDestroy(fs.stream));
     }
-    if (b->pack.host) HIP_CHECK(hipHostFree(b->pack.host));
-    if (b->counters_h) HIP_CHECK(hipHostFree(b->counters_h));
-    if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
-    if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
-    for (hipEvent_t& e : b->ev) HIP_CHECK(hipEventDestroy(e));
-    HIP_CHECK(hipStreamDestroy(b->stream));
     delete b;
+}
+
+/* waits for every batch in flight: shared pools may move or be rewritten next */
+static void quiesce(Backend* b) {
+    for (FrameSet& fs : b->sets) HIP_CHECK(hipStreamSynchronize(fs.stream));
 }
 
 void backend_release_texture(Backend* b, u32 index) {
@@ -264,13 +305,15 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
     }
 }
 
+/* grows the shared pools; no batch may be in flight (quiesce first) */
 static void grow_pools(Backend* b, size_t slots, size_t skins) {
-    dgrow(b->pos, slots * 768, true, b->stream);
-    dgrow(b->attr, slots * 1792, true, b->stream);
-    dgrow(b->inds, slots * 512, true, b->stream);
-    dgrow(b->cull, slots * 32, true, b->stream);
-    dgrow(b->slot_mesh, slots * 4, true, b->stream);
-    dgrow(b->skin, std::max<size_t>(skins, 1) * 768, true, b->stream);
+    hipStream_t s = b->sets[0].stream;
+    dgrow(b->pos, slots * 768, true, s);
+    dgrow(b->attr, slots * 1792, true, s);
+    dgrow(b->inds, slots * 512, true, s);
+    dgrow(b->cull, slots * 32, true, s);
+    dgrow(b->slot_mesh, slots * 4, true, s);
+    dgrow(b->skin, std::max<size_t>(skins, 1) * 768, true, s);
 }
 
 /* makes a model resident: plans it after the resident slots and copies its
@@ -280,7 +323,7 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r) {
     plan_model(m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, b->resident_skins, &n_skin);
     const u32 T = m->total_meshlets;
     grow_pools(b, (size_t)r.base + T, (size_t)b->resident_skins + n_skin);
-    hipStream_t s = b->stream;
+    hipStream_t s = b->sets[0].stream;
     if (T) {
         std::vector<f32> pos((size_t)T * 192), attr((size_t)T * 448), cull((size_t)T * 8, 0.f);
         std::vector<u8> inds((size_t)T * 512, 0), skin((size_t)std::max(n_skin, 1u) * 768, 0);
@@ -318,7 +361,18 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r) {
     }
 }
 
+/* uploads textures created since the last batch; the caller has quiesced
+   when any is pending (texture_changes) */
+static bool texture_changes(const Backend* b, const Context& c) {
+    if (b->tex_meta_dirty || b->tex_meta.size() != c.tex.size()) return true;
+    for (const HostTexture& t : c.tex) {
+        if (t.live && !t.uploaded) return true;
+    }
+    return false;
+}
+
 static void sync_textures(Backend* b, Context& c) {
+    hipStream_t stream = b->sets[0].stream;
     if (b->tex_meta.size() != c.tex.size()) {
         b->tex_meta.resize(c.tex.size());
         b->tex_meta_dirty = true;
@@ -327,8 +381,8 @@ static void sync_textures(Backend* b, Context& c) {
         HostTexture& t = c.tex[i];
         if (!t.live || t.uploaded) continue;
         const u64 n = t.texels.size();
-        dgrow(b->texels, (size_t)(b->texels_used + n) * 4, true, b->stream);
-        HIP_CHECK(hipMemcpyAsync((u32*)b->texels.p + b->texels_used, t.texels.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+        dgrow(b->texels, (size_t)(b->texels_used + n) * 4, true, stream);
+        HIP_CHECK(hipMemcpyAsync((u32*)b->texels.p + b->texels_used, t.texels.data(), n * 4, hipMemcpyHostToDevice, stream));
         t.dev_offset = b->texels_used;
         t.uploaded = true;
         b->texels_used += n;
@@ -346,11 +400,11 @@ static void sync_textures(Backend* b, Context& c) {
         b->tex_meta_dirty = true;
     }
     if (b->tex_meta_dirty) {
-        HIP_CHECK(hipStreamSynchronize(b->stream));
-        upload(b->tex_meta_d, b->tex_meta, b->stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        upload(b->tex_meta_d, b->tex_meta, stream);
         b->tex_meta_dirty = false;
     }
-    dgrow(b->texels, 4, false, b->stream);
+    dgrow(b->texels, 4, false, stream);
 }
 
 static f32 ev_ms(hipEvent_t a, hipEvent_t z) {
@@ -360,73 +414,74 @@ static f32 ev_ms(hipEvent_t a, hipEvent_t z) {
 }
 
 /* Runs the batch's kernels.  With graphs on, the launch sequence is captured
-   once into a HIP graph and replayed while the kernel arguments and launch
-   sizes stay the same (per-frame data changes only buffer contents); any
-   change re-captures it.  Per-pass timings need graphs off. */
-static void run_frame(Backend* b, const FramePtrs& p, const BatchGpu& bg, size_t zero_words) {
-    hipStream_t s = b->stream;
+   into a HIP graph the second time in a row it is seen and replayed while
+   the kernel arguments and launch sizes stay the same (per-frame data
+   changes only buffer contents).  Per-pass timings need graphs off. */
+static void run_frame(Backend* b, FrameSet& fs, const FramePtrs& p) {
+    hipStream_t s = fs.stream;
     if (!b->use_graphs) {
-        launch_frame(p, bg, zero_words, s, b->ev);
+        launch_frame(p, fs.bg, fs.zero_words, s, fs.ev);
         HIP_CHECK(hipGetLastError());
         return;
     }
-    std::vector<u8> key(sizeof(p) + sizeof(bg) + sizeof(zero_words));
+    std::vector<u8> key(sizeof(p) + sizeof(fs.bg) + sizeof(fs.zero_words));
     memcpy(key.data(), &p, sizeof(p));
-    memcpy(key.data() + sizeof(p), &bg, sizeof(bg));
-    memcpy(key.data() + sizeof(p) + sizeof(bg), &zero_words, sizeof(zero_words));
-    if (!b->graph_exec || key != b->graph_key) {
-        /* capture a launch sequence the second time in a row it is seen, so
-           frames of changing shape are not charged a capture each */
-        if (key != b->last_key) {
-            b->last_key = key;
-            launch_frame(p, bg, zero_words, s, b->ev);
+    memcpy(key.data() + sizeof(p), &fs.bg, sizeof(fs.bg));
+    memcpy(key.data() + sizeof(p) + sizeof(fs.bg), &fs.zero_words, sizeof(fs.zero_words));
+    if (!fs.graph_exec || key != fs.graph_key) {
+        if (key != fs.last_key) {
+            fs.last_key = key;
+            launch_frame(p, fs.bg, fs.zero_words, s, fs.ev);
             HIP_CHECK(hipGetLastError());
             return;
         }
-        if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
-        b->graph_exec = nullptr;
+        if (fs.graph_exec) HIP_CHECK(hipGraphExecDestroy(fs.graph_exec));
+        fs.graph_exec = nullptr;
         hipGraph_t g = nullptr;
         HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch_frame(p, bg, zero_words, s, nullptr);
+        launch_frame(p, fs.bg, fs.zero_words, s, nullptr);
         HIP_CHECK(hipStreamEndCapture(s, &g));
-        HIP_CHECK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphInstantiate(&fs.graph_exec, g, nullptr, nullptr, 0));
         HIP_CHECK(hipGraphDestroy(g));
-        b->graph_key = key;
+        fs.graph_key = key;
         b->graph_captures += 1;
     }
     /* pass boundaries are not visible inside the graph: the passes read as
        one span between the first and the last mark */
-    HIP_CHECK(hipEventRecord(b->ev[1], s));
-    HIP_CHECK(hipGraphLaunch(b->graph_exec, s));
-    for (int k : { 7, 2, 3, 6, 4 }) HIP_CHECK(hipEventRecord(b->ev[k], s));
+    HIP_CHECK(hipEventRecord(fs.ev[1], s));
+    HIP_CHECK(hipGraphLaunch(fs.graph_exec, s));
+    for (int k : { 7, 2, 3, 6, 4 }) HIP_CHECK(hipEventRecord(fs.ev[k], s));
 }
 
-/* frame-table rows of one view inside the batch */
-struct ViewPlan {
-    u32 inst_base = 0, work_base = 0, n_work = 0, pl_base = 0, dl_base = 0, tmat_base = 0;
-    u32 tile_base = 0, proxy_base = 0, tlas_base = 0, occ_base = 0, occ_count = 0;
-    u64 vid_base = 0, color_base = 0, grid = 0;
-    u32 n_tiles = 0;
-};
+static bool is_transient(const ViewRecord& v, const sft_model* m) {
+    return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
+}
 
-void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
-    Context& c = *cp;
-    hipStream_t s = b->stream;
-    HIP_CHECK(hipEventRecord(b->ev[0], s));
+/* Host side of a batch: residency, then every view's rows of the batch
+   tables packed into the set's pinned staging. */
+static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
     const u32 nv = (u32)views.size();
-    if (0 == nv) return;
-
+    /* shared pools change only with no batch in flight */
+    bool shared_change = !b->est_uploaded || texture_changes(b, c);
+    for (const ViewRecord& v : views) {
+        if (!v.transient_models.empty()) shared_change = true;
+        for (const sft_instance& in : v.instances) {
+            auto it = b->models.find(in.model);
+            if (!is_transient(v, in.model) && (it == b->models.end() || it->second.total != in.model->total_meshlets)) {
+                shared_change = true;
+            }
+        }
+    }
+    if (shared_change) quiesce(b);
+    hipStream_t s0 = b->sets[0].stream;
     if (!b->est_uploaded) {
-        upload(b->est_rcp, c.est.rcp, s);
-        upload(b->est_rsq, c.est.rsq, s);
-        HIP_CHECK(hipStreamSynchronize(s));
+        upload(b->est_rcp, c.est.rcp, s0);
+        upload(b->est_rsq, c.est.rsq, s0);
+        HIP_CHECK(hipStreamSynchronize(s0));
         b->est_uploaded = true;
     }
 
     /* ---- residency: new resident models first, then each view's transient ones ---- */
-    auto is_transient = [](const ViewRecord& v, const sft_model* m) {
-        return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
-    };
     for (const ViewRecord& v : views) {
         for (const sft_instance& in : v.instances) {
             const sft_model* m = in.model;
@@ -457,7 +512,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         }
     }
     /* transient models (dmesh geometry) take the slots after the resident
-       ones; their rows are staged with the frame tables and copied in bulk */
+       ones; their rows are staged with the batch tables and copied in bulk */
     std::vector<std::unordered_map<const sft_model*, ModelRes>> tmodels(nv);
     u32 tslot = b->resident_slots, tskin = b->resident_skins;
     for (u32 vi = 0; vi < nv; vi += 1) {
@@ -470,9 +525,11 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
             tmodels[vi][m] = std::move(r);
         }
     }
-    const u32 t_slots = tslot - b->resident_slots, t_skins = tskin - b->resident_skins;
-    if (t_slots) grow_pools(b, tslot, tskin);
-    sync_textures(b, c);
+    fs.t_slots = tslot - b->resident_slots;
+    fs.t_skins = tskin - b->resident_skins;
+    if (fs.t_slots) grow_pools(b, tslot, tskin);
+    if (texture_changes(b, c)) sync_textures(b, c);
+    dgrow(b->texels, 4, false, s0);
 
     /* ---- batch tables: views' rows back to back ---- */
     std::vector<InstGpu> insts;
@@ -484,15 +541,15 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     std::vector<f32> tmat_shin;
     std::vector<ProxyGpu> proxies;
     std::vector<sft_bvh_node> tlas;
-    std::vector<ViewGpu> vgpu(nv);
-    std::vector<ViewPlan> plan(nv);
     std::vector<u32> tile_view, shade_tiles, shade_tiles_ms;
+    fs.vgpu.assign(nv, ViewGpu{});
+    fs.plan.assign(nv, FrameSet::ViewPlan{});
     u32 ss_total = 0, max_samples = 1, occ_total = 0;
     u64 vid_total = 0, color_total = 0;
-    bool any_vis64 = false;
+    bool any_vis64 = false, host_color = false;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
-        ViewPlan& vp = plan[vi];
+        FrameSet::ViewPlan& vp = fs.plan[vi];
         vp.inst_base = (u32)insts.size();
         vp.work_base = (u32)work.size();
         vp.pl_base = (u32)pl.size();
@@ -509,6 +566,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         color_total += (u64)v.width * v.height;
         max_samples = std::max<u32>(max_samples, (u32)v.samples);
         any_vis64 = any_vis64 || v.vis64;
+        host_color = host_color || (SFTX_OUTPUT_HOST == b->output_mode && v.color_buffer);
         for (u32 t = 0; t < vp.n_tiles; t += 1) {
             tile_view.push_back(vi);
             (v.samples > 1 ? shade_tiles_ms : shade_tiles).push_back(vp.tile_base + t);
@@ -582,7 +640,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         if (occlusion) tlas_root = build_tlas(v.instances, vt);
         tlas.insert(tlas.end(), vt.begin(), vt.end());
 
-        FrameConst& fc = vgpu[vi].fc;
+        FrameConst& fc = fs.vgpu[vi].fc;
         memset(&fc, 0, sizeof(fc));
         fc.width = v.width;
         fc.height = v.height;
@@ -628,7 +686,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
         vp.occ_count = fc.occ_lights * fc.occ_batches;
         occ_total += vp.occ_count;
 
-        ViewGpu& g = vgpu[vi];
+        ViewGpu& g = fs.vgpu[vi];
         g.inst_base = vp.inst_base;
         g.work_base = vp.work_base;
         g.n_work = vp.n_work;
@@ -654,9 +712,9 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     /* proxy lists exist for the lights of views with shadows; the kernel skips the others */
     u32 n_lights_cache = 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
-        if (vgpu[vi].fc.shadows_enabled && vgpu[vi].fc.n_proxies > 0) n_lights_cache = (u32)pl.size();
+        if (fs.vgpu[vi].fc.shadows_enabled && fs.vgpu[vi].fc.n_proxies > 0) n_lights_cache = (u32)pl.size();
     }
-    BatchGpu bg;
+    BatchGpu& bg = fs.bg;
     memset(&bg, 0, sizeof(bg));
     bg.n_views = nv;
     bg.n_work = (u32)work.size();
@@ -671,216 +729,237 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     bg.vis64 = any_vis64 ? 1u : 0u;
     bg.raster_chunk = b->raster_chunk;
     bg.raster_groups = b->raster_groups;
-    dgrow(b->tile_nch, (size_t)nt * 4, false, s);
-    dgrow(b->merge, (size_t)nt * max_samples * 4096 * 8, false, s);
 
-    /* ---- uploads ---- */
-    Packer& pk = b->pack;
+    /* ---- staging ---- */
+    Packer& pk = fs.pack;
     pk.used = 0;
-    const size_t o_batch = pk.add(&bg, sizeof(bg));
-    const size_t o_views = pk.add(vgpu);
-    const size_t o_tv = pk.add(tile_view);
-    const size_t o_st = pk.add(shade_tiles);
-    const size_t o_insts = pk.add(insts);
-    const size_t o_work = pk.add(work);
-    const size_t o_bones = pk.add(bones);
-    const size_t o_pl = pk.add(pl);
-    const size_t o_dl = pk.add(dl);
-    const size_t o_mt = pk.add(c.mat_tex);
-    const size_t o_mc = pk.add(c.mat_color);
-    const size_t o_ms = pk.add(c.mat_shininess);
-    const size_t o_tt = pk.add(tmat_tex);
-    const size_t o_tc = pk.add(tmat_color);
-    const size_t o_ts = pk.add(tmat_shin);
-    const size_t o_mr = pk.add(b->mesh_recs);
-    const size_t o_px = pk.add(proxies);
-    const size_t o_tlas = pk.add(tlas);
+    fs.o_batch = pk.add(&bg, sizeof(bg));
+    fs.o_views = pk.add(fs.vgpu);
+    fs.o_tv = pk.add(tile_view);
+    fs.o_st = pk.add(shade_tiles);
+    fs.o_insts = pk.add(insts);
+    fs.o_work = pk.add(work);
+    fs.o_bones = pk.add(bones);
+    fs.o_pl = pk.add(pl);
+    fs.o_dl = pk.add(dl);
+    fs.o_mt = pk.add(c.mat_tex);
+    fs.o_mc = pk.add(c.mat_color);
+    fs.o_ms = pk.add(c.mat_shininess);
+    fs.o_tt = pk.add(tmat_tex);
+    fs.o_tc = pk.add(tmat_color);
+    fs.o_ts = pk.add(tmat_shin);
+    fs.o_mr = pk.add(b->mesh_recs);
+    fs.o_px = pk.add(proxies);
+    fs.o_tlas = pk.add(tlas);
     /* transient rows, filled in place once every section is reserved */
-    const size_t o_tpos = pk.reserve((size_t)t_slots * 768);
-    const size_t o_tattr = pk.reserve((size_t)t_slots * 1792);
-    const size_t o_tinds = pk.reserve((size_t)t_slots * 512);
-    const size_t o_tcull = pk.reserve((size_t)t_slots * 32);
-    const size_t o_tsm = pk.reserve((size_t)t_slots * 4);
-    const size_t o_tskin = pk.reserve((size_t)t_skins * 768);
+    fs.o_tpos = pk.reserve((size_t)fs.t_slots * 768);
+    fs.o_tattr = pk.reserve((size_t)fs.t_slots * 1792);
+    fs.o_tinds = pk.reserve((size_t)fs.t_slots * 512);
+    fs.o_tcull = pk.reserve((size_t)fs.t_slots * 32);
+    fs.o_tsm = pk.reserve((size_t)fs.t_slots * 4);
+    fs.o_tskin = pk.reserve((size_t)fs.t_skins * 768);
     for (u32 vi = 0; vi < nv; vi += 1) {
         for (const auto& kv : tmodels[vi]) {
-            fill_model_rows(kv.first, kv.second, b->resident_slots, b->resident_skins, (f32*)(pk.host + o_tpos),
-                            (f32*)(pk.host + o_tattr), pk.host + o_tinds, (f32*)(pk.host + o_tcull),
-                            (u32*)(pk.host + o_tsm), pk.host + o_tskin);
+            fill_model_rows(kv.first, kv.second, b->resident_slots, b->resident_skins, (f32*)(pk.host + fs.o_tpos),
+                            (f32*)(pk.host + fs.o_tattr), pk.host + fs.o_tinds, (f32*)(pk.host + fs.o_tcull),
+                            (u32*)(pk.host + fs.o_tsm), pk.host + fs.o_tskin);
         }
     }
 
-    const u32 nw = bg.n_work;
-    const bool single = 1 == nv;
-    const u32 cap_flags = single ? views[0].capture_flags : 0u;
-    const bool cap_vis = 0 != (cap_flags & SFTX_CAPTURE_VISIBILITY);
-    dgrow(b->ss, std::max<size_t>((size_t)ss_total * 2304, 4), false, s);
-    dgrow(b->ss_vis, std::max<size_t>(ss_total, 4), false, s);
-    dgrow(b->clip_count, std::max<size_t>((size_t)nw * 4, 4), false, s);
-    dgrow(b->clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
-    dgrow(b->clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
-    dgrow(b->batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
-    dgrow(b->tile_off, (size_t)nt * 4, false, s);
-    dgrow(b->scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4, false, s);
-    const size_t zero_words = COUNTER_WORDS + 2 * (size_t)nt + (size_t)nt * max_samples;
-    dgrow(b->zero_d, zero_words * 4, false, s);
-    if (!b->counters_h) HIP_CHECK(hipHostMalloc((void**)&b->counters_h, CNT_COUNT * 4, hipHostMallocDefault));
-    dgrow(b->color, std::max<size_t>(color_total * 4, 4), false, s);
-    dgrow(b->vid, std::max<size_t>(vid_total * 4, 4), false, s);
-    dgrow(b->tile_z, (size_t)nt * max_samples * 8, false, s);
-    dgrow(b->tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
-    if (cap_vis) dgrow(b->cap_depth, vid_total * 4, false, s);
-    if (any_vis64) dgrow(b->vis64, vid_total * 8, false, s);
-    dgrow(b->proxy_vis, std::max<size_t>(proxies.size() * 64, 64), false, s);
-    dgrow(b->light_proxy_count, std::max<size_t>(pl.size() * 4, 4), false, s);
-    dgrow(b->light_proxies, std::max<size_t>(pl.size() * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)), false, s);
+    fs.nt = nt;
+    fs.nw = bg.n_work;
+    fs.max_samples = max_samples;
+    fs.ss_total = ss_total;
+    fs.n_proxies = (u32)proxies.size();
+    fs.n_lights = (u32)pl.size();
+    fs.vid_total = vid_total;
+    fs.color_total = color_total;
+    fs.any_vis64 = any_vis64;
+    fs.host_color = host_color;
+    fs.cap_vis = 1 == nv && 0 != (views[0].capture_flags & SFTX_CAPTURE_VISIBILITY);
+    fs.zero_words = COUNTER_WORDS + 2 * (size_t)nt + (size_t)nt * max_samples;
+}
+
+/* Device side of a prepared batch: buffers, one H2D copy of the tables,
+   the kernels, and asynchronous copies of the counters and host outputs. */
+static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs) {
+    hipStream_t s = fs.stream;
+    const u32 nv = (u32)views.size(), nt = fs.nt, nw = fs.nw;
+    HIP_CHECK(hipEventRecord(fs.ev[0], s));
+    dgrow(fs.ss, std::max<size_t>((size_t)fs.ss_total * 2304, 4), false, s);
+    dgrow(fs.ss_vis, std::max<size_t>(fs.ss_total, 4), false, s);
+    dgrow(fs.clip_count, std::max<size_t>((size_t)nw * 4, 4), false, s);
+    dgrow(fs.clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
+    dgrow(fs.clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
+    dgrow(fs.batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
+    dgrow(fs.tile_off, (size_t)nt * 4, false, s);
+    dgrow(fs.scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4, false, s);
+    dgrow(fs.zero_d, fs.zero_words * 4, false, s);
+    dgrow(fs.color, std::max<size_t>(fs.color_total * 4, 4), false, s);
+    dgrow(fs.vid, std::max<size_t>(fs.vid_total * 4, 4), false, s);
+    dgrow(fs.tile_z, (size_t)nt * fs.max_samples * 8, false, s);
+    dgrow(fs.tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
+    dgrow(fs.tile_nch, (size_t)nt * 4, false, s);
+    dgrow(fs.merge, (size_t)nt * fs.max_samples * 4096 * 8, false, s);
+    if (fs.cap_vis) dgrow(fs.cap_depth, fs.vid_total * 4, false, s);
+    if (fs.any_vis64) dgrow(fs.vis64, fs.vid_total * 8, false, s);
+    dgrow(fs.proxy_vis, std::max<size_t>((size_t)fs.n_proxies * 64, 64), false, s);
+    dgrow(fs.light_proxy_count, std::max<size_t>((size_t)fs.n_lights * 4, 4), false, s);
+    dgrow(fs.light_proxies, std::max<size_t>((size_t)fs.n_lights * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)), false, s);
+    dgrow(fs.tile_list, (size_t)fs.tile_list_cap * 8, false, s);
+    dgrow(fs.clip_rec, (size_t)fs.clip_cap * 164, false, s);
+    dgrow(fs.clip_key, (size_t)fs.clip_cap * 4, false, s);
+    dgrow(fs.clip_rect, (size_t)fs.clip_cap * 4, false, s);
+    fs.bg.clip_cap = fs.clip_cap;
+    fs.bg.tile_list_cap = fs.tile_list_cap;
+    /* visibility work items: at most one per tile and sample plus one per
+       raster_chunk payloads */
+    const size_t max_items = ((size_t)nt + fs.tile_list_cap / fs.bg.raster_chunk + 1) * fs.max_samples;
+    dgrow(fs.raster_items, max_items * 8, false, s);
+    for (u32 vi = 0; vi < nv; vi += 1) {
+        fs.vgpu[vi].fc.clip_cap = fs.clip_cap;
+        fs.vgpu[vi].fc.tile_list_cap = fs.tile_list_cap;
+    }
+    Packer& pk = fs.pack;
+    memcpy(pk.host + fs.o_batch, &fs.bg, sizeof(fs.bg));
+    memcpy(pk.host + fs.o_views, fs.vgpu.data(), fs.vgpu.size() * sizeof(ViewGpu));
+    dgrow(fs.pack_d, pk.used, false, s);
+    HIP_CHECK(hipMemcpyAsync(fs.pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
+    u8* pd = (u8*)fs.pack_d.p;
+    if (fs.t_slots) {
+        const size_t r0 = b->resident_slots;
+        HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + r0 * 768, pd + fs.o_tpos, (size_t)fs.t_slots * 768, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + r0 * 1792, pd + fs.o_tattr, (size_t)fs.t_slots * 1792, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + r0 * 512, pd + fs.o_tinds, (size_t)fs.t_slots * 512, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + r0 * 32, pd + fs.o_tcull, (size_t)fs.t_slots * 32, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + r0 * 4, pd + fs.o_tsm, (size_t)fs.t_slots * 4, hipMemcpyDeviceToDevice, s));
+        if (fs.t_skins) {
+            HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)b->resident_skins * 768, pd + fs.o_tskin, (size_t)fs.t_skins * 768,
+                                     hipMemcpyDeviceToDevice, s));
+        }
+    }
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
         const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
-        if (occlusion && 0 == vgpu[vi].fc.tlas_nodes) {
+        if (occlusion && 0 == fs.vgpu[vi].fc.tlas_nodes) {
             /* no instance to occlude: every ray reaches its light */
-            HIP_CHECK(hipMemsetAsync((u8*)b->proxy_vis.p + (size_t)plan[vi].proxy_base * 64, 3, v.proxies.size() * 64, s));
+            HIP_CHECK(hipMemsetAsync((u8*)fs.proxy_vis.p + (size_t)fs.plan[vi].proxy_base * 64, 3, v.proxies.size() * 64, s));
         }
         /* a view without clearing starts from the caller's frame */
         if (!v.should_clear && v.color_buffer && v.width > 0 && v.height > 0) {
-            HIP_CHECK(hipMemcpy2DAsync((u32*)b->color.p + plan[vi].color_base, (size_t)v.width * 4, v.color_buffer,
+            HIP_CHECK(hipMemcpy2DAsync((u32*)fs.color.p + fs.plan[vi].color_base, (size_t)v.width * 4, v.color_buffer,
                                        (size_t)v.color_pitch * 4, (size_t)v.width * 4, v.height, hipMemcpyHostToDevice, s));
         }
     }
+    u32* zd = (u32*)fs.zero_d.p;
 
-    u32 counters[CNT_COUNT];
+    FramePtrs p;
+    p.pos = (const f32*)b->pos.p;
+    p.attr = (const f32*)b->attr.p;
+    p.inds = (const u8*)b->inds.p;
+    p.cull = (const f32*)b->cull.p;
+    p.slot_mesh = (const u32*)b->slot_mesh.p;
+    p.skin = (const u8*)b->skin.p;
+    p.texels = (const u32*)b->texels.p;
+    p.tex_meta = (const TexMeta*)b->tex_meta_d.p;
+    p.est_rcp = (const f32*)b->est_rcp.p;
+    p.est_rsq = (const f32*)b->est_rsq.p;
+    p.batch = (const BatchGpu*)(pd + fs.o_batch);
+    p.views = (const ViewGpu*)(pd + fs.o_views);
+    p.tile_view = (const u32*)(pd + fs.o_tv);
+    p.shade_tiles = (const u32*)(pd + fs.o_st);
+    p.insts = (const InstGpu*)(pd + fs.o_insts);
+    p.work = (const WorkItem*)(pd + fs.o_work);
+    p.bones = (const f32*)(pd + fs.o_bones);
+    p.point_lights = (const PointLightGpu*)(pd + fs.o_pl);
+    p.dir_lights = (const DirLightGpu*)(pd + fs.o_dl);
+    p.mat_tex = (const u32*)(pd + fs.o_mt);
+    p.mat_color = (const u32*)(pd + fs.o_mc);
+    p.mat_shin = (const f32*)(pd + fs.o_ms);
+    p.tmat_tex = (const u32*)(pd + fs.o_tt);
+    p.tmat_color = (const u32*)(pd + fs.o_tc);
+    p.tmat_shin = (const f32*)(pd + fs.o_ts);
+    p.mesh_recs = (const MeshRec*)(pd + fs.o_mr);
+    p.ss = (f32*)fs.ss.p;
+    p.ss_vis = (u8*)fs.ss_vis.p;
+    p.clip_count = (u32*)fs.clip_count.p;
+    p.clip_base = (u32*)fs.clip_base.p;
+    p.clip_mask = (u64*)fs.clip_mask.p;
+    p.batch_pl = (uint2*)fs.batch_pl.p;
+    p.tile_count = zd + COUNTER_WORDS;
+    p.tile_off = (u32*)fs.tile_off.p;
+    p.tile_fill = zd + COUNTER_WORDS + nt;
+    p.tile_list = (u64*)fs.tile_list.p;
+    p.clip_rec = (f32*)fs.clip_rec.p;
+    p.clip_key = (u32*)fs.clip_key.p;
+    p.clip_rect = (u32*)fs.clip_rect.p;
+    p.color = (u32*)fs.color.p;
+    p.vid = (u32*)fs.vid.p;
+    p.tile_lights = (u32*)fs.tile_lights.p;
+    p.tile_z = (f32*)fs.tile_z.p;
+    p.cap_depth = fs.cap_vis ? (f32*)fs.cap_depth.p : nullptr;
+    p.vis64 = fs.any_vis64 ? (u64*)fs.vis64.p : nullptr;
+    p.counters = zd;
+    p.scan_sums = (u32*)fs.scan_sums.p;
+    p.raster_items = (uint2*)fs.raster_items.p;
+    p.tile_nch = (u32*)fs.tile_nch.p;
+    p.tile_done = zd + COUNTER_WORDS + 2 * (size_t)nt;
+    p.merge = (u64*)fs.merge.p;
+    p.blas = (const sft_bvh_node*)b->blas.p;
+    p.tlas = (const sft_bvh_node*)(pd + fs.o_tlas);
+    p.proxies = (const ProxyGpu*)(pd + fs.o_px);
+    p.proxy_vis = (u8*)fs.proxy_vis.p;
+    p.light_proxy_count = (u32*)fs.light_proxy_count.p;
+    p.light_proxies = (ProxyRec*)fs.light_proxies.p;
+
+    run_frame(b, fs, p);
+    HIP_CHECK(hipMemcpyAsync(fs.counters_h, zd, CNT_COUNT * 4, hipMemcpyDeviceToHost, s));
+
+    /* host outputs: the colour planes (and 64-bit visibility) into pinned staging */
+    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
+    const size_t stage = color_bytes + (fs.any_vis64 ? (size_t)fs.vid_total * 8 : 0);
+    if (stage) {
+        if (fs.out_h_cap < stage) {
+            if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
+            HIP_CHECK(hipHostMalloc((void**)&fs.out_h, stage, hipHostMallocDefault));
+            fs.out_h_cap = stage;
+        }
+        if (color_bytes) HIP_CHECK(hipMemcpyAsync(fs.out_h, fs.color.p, color_bytes, hipMemcpyDeviceToHost, s));
+        if (fs.any_vis64) HIP_CHECK(hipMemcpyAsync(fs.out_h + color_bytes, fs.vis64.p, (size_t)fs.vid_total * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK(hipEventRecord(fs.ev[5], s));
+    fs.in_flight = true;
+}
+
+/* Waits for a launched batch; re-runs it with larger intermediates if any
+   overflowed; writes the host outputs, the statistics and the captures. */
+static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
+    hipStream_t s = fs.stream;
     for (int attempt = 0;; attempt += 1) {
-        dgrow(b->tile_list, (size_t)b->tile_list_cap * 8, false, s);
-        dgrow(b->clip_rec, (size_t)b->clip_cap * 164, false, s);
-        dgrow(b->clip_key, (size_t)b->clip_cap * 4, false, s);
-        dgrow(b->clip_rect, (size_t)b->clip_cap * 4, false, s);
-        bg.clip_cap = b->clip_cap;
-        bg.tile_list_cap = b->tile_list_cap;
-        /* visibility work items: at most one per tile and sample plus one per
-           kRasterChunk payloads */
-        const size_t max_items = ((size_t)nt + b->tile_list_cap / bg.raster_chunk + 1) * max_samples;
-        dgrow(b->raster_items, max_items * 8, false, s);
-        memcpy(pk.host + o_batch, &bg, sizeof(bg));
-        for (u32 vi = 0; vi < nv; vi += 1) {
-            vgpu[vi].fc.clip_cap = b->clip_cap;
-            vgpu[vi].fc.tile_list_cap = b->tile_list_cap;
-        }
-        memcpy(pk.host + o_views, vgpu.data(), vgpu.size() * sizeof(ViewGpu));
-        dgrow(b->pack_d, pk.used, false, s);
-        HIP_CHECK(hipMemcpyAsync(b->pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
-        u8* pd = (u8*)b->pack_d.p;
-        if (t_slots) {
-            const size_t r0 = b->resident_slots;
-            HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + r0 * 768, pd + o_tpos, (size_t)t_slots * 768, hipMemcpyDeviceToDevice, s));
-            HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + r0 * 1792, pd + o_tattr, (size_t)t_slots * 1792, hipMemcpyDeviceToDevice, s));
-            HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + r0 * 512, pd + o_tinds, (size_t)t_slots * 512, hipMemcpyDeviceToDevice, s));
-            HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + r0 * 32, pd + o_tcull, (size_t)t_slots * 32, hipMemcpyDeviceToDevice, s));
-            HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + r0 * 4, pd + o_tsm, (size_t)t_slots * 4, hipMemcpyDeviceToDevice, s));
-            if (t_skins) {
-                HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)b->resident_skins * 768, pd + o_tskin, (size_t)t_skins * 768,
-                                         hipMemcpyDeviceToDevice, s));
-            }
-        }
-        u32* zd = (u32*)b->zero_d.p;
-
-        FramePtrs p;
-        p.pos = (const f32*)b->pos.p;
-        p.attr = (const f32*)b->attr.p;
-        p.inds = (const u8*)b->inds.p;
-        p.cull = (const f32*)b->cull.p;
-        p.slot_mesh = (const u32*)b->slot_mesh.p;
-        p.skin = (const u8*)b->skin.p;
-        p.texels = (const u32*)b->texels.p;
-        p.tex_meta = (const TexMeta*)b->tex_meta_d.p;
-        p.est_rcp = (const f32*)b->est_rcp.p;
-        p.est_rsq = (const f32*)b->est_rsq.p;
-        p.batch = (const BatchGpu*)(pd + o_batch);
-        p.views = (const ViewGpu*)(pd + o_views);
-        p.tile_view = (const u32*)(pd + o_tv);
-        p.shade_tiles = (const u32*)(pd + o_st);
-        p.insts = (const InstGpu*)(pd + o_insts);
-        p.work = (const WorkItem*)(pd + o_work);
-        p.bones = (const f32*)(pd + o_bones);
-        p.point_lights = (const PointLightGpu*)(pd + o_pl);
-        p.dir_lights = (const DirLightGpu*)(pd + o_dl);
-        p.mat_tex = (const u32*)(pd + o_mt);
-        p.mat_color = (const u32*)(pd + o_mc);
-        p.mat_shin = (const f32*)(pd + o_ms);
-        p.tmat_tex = (const u32*)(pd + o_tt);
-        p.tmat_color = (const u32*)(pd + o_tc);
-        p.tmat_shin = (const f32*)(pd + o_ts);
-        p.mesh_recs = (const MeshRec*)(pd + o_mr);
-        p.ss = (f32*)b->ss.p;
-        p.ss_vis = (u8*)b->ss_vis.p;
-        p.clip_count = (u32*)b->clip_count.p;
-        p.clip_base = (u32*)b->clip_base.p;
-        p.clip_mask = (u64*)b->clip_mask.p;
-        p.batch_pl = (uint2*)b->batch_pl.p;
-        p.tile_count = zd + COUNTER_WORDS;
-        p.tile_off = (u32*)b->tile_off.p;
-        p.tile_fill = zd + COUNTER_WORDS + nt;
-        p.tile_list = (u64*)b->tile_list.p;
-        p.clip_rec = (f32*)b->clip_rec.p;
-        p.clip_key = (u32*)b->clip_key.p;
-        p.clip_rect = (u32*)b->clip_rect.p;
-        p.color = (u32*)b->color.p;
-        p.vid = (u32*)b->vid.p;
-        p.tile_lights = (u32*)b->tile_lights.p;
-        p.tile_z = (f32*)b->tile_z.p;
-        p.cap_depth = cap_vis ? (f32*)b->cap_depth.p : nullptr;
-        p.vis64 = any_vis64 ? (u64*)b->vis64.p : nullptr;
-        p.counters = zd;
-        p.scan_sums = (u32*)b->scan_sums.p;
-        p.raster_items = (uint2*)b->raster_items.p;
-        p.tile_nch = (u32*)b->tile_nch.p;
-        p.tile_done = zd + COUNTER_WORDS + 2 * (size_t)nt;
-        p.merge = (u64*)b->merge.p;
-        p.blas = (const sft_bvh_node*)b->blas.p;
-        p.tlas = (const sft_bvh_node*)(pd + o_tlas);
-        p.proxies = (const ProxyGpu*)(pd + o_px);
-        p.proxy_vis = (u8*)b->proxy_vis.p;
-        p.light_proxy_count = (u32*)b->light_proxy_count.p;
-        p.light_proxies = (ProxyRec*)b->light_proxies.p;
-
-        run_frame(b, p, bg, zero_words);
-        HIP_CHECK(hipMemcpyAsync(b->counters_h, zd, sizeof(counters), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        memcpy(counters, b->counters_h, sizeof(counters));
-        if (0u == counters[CNT_OVERFLOW]) break;
+        memcpy(fs.counters, fs.counters_h, sizeof(fs.counters));
+        if (0u == fs.counters[CNT_OVERFLOW]) break;
         if (attempt > 8) {
             fprintf(stderr, "fatal sofren2 error: frame intermediates keep overflowing\n");
             exit(1);
         }
-        if (counters[CNT_OVERFLOW] & 1u) b->clip_cap = std::max(b->clip_cap * 2u, counters[CNT_CLIP_TRIS] + 1024u);
-        if (counters[CNT_OVERFLOW] & 2u) b->tile_list_cap = std::max(b->tile_list_cap * 2u, counters[CNT_TILE_PAIRS] + 4096u);
+        if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
+        if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
+        launch_batch(b, views, fs);
     }
-
-    /* ---- outputs: one copy of the colour planes (and 64-bit visibility) into pinned staging ---- */
-    bool any_host = false;
-    for (const ViewRecord& v : views) any_host = any_host || (SFTX_OUTPUT_HOST == b->output_mode && v.color_buffer) || v.vis64;
-    size_t stage_bytes = 0;
-    const size_t color_bytes = (SFTX_OUTPUT_HOST == b->output_mode) ? (size_t)color_total * 4 : 0;
-    if (any_host) stage_bytes = color_bytes + (any_vis64 ? (size_t)vid_total * 8 : 0);
-    if (stage_bytes) {
-        if (b->color_h_cap < stage_bytes) {
-            if (b->color_h) HIP_CHECK(hipHostFree(b->color_h));
-            HIP_CHECK(hipHostMalloc((void**)&b->color_h, stage_bytes, hipHostMallocDefault));
-            b->color_h_cap = stage_bytes;
-        }
-        if (color_bytes) HIP_CHECK(hipMemcpyAsync(b->color_h, b->color.p, color_bytes, hipMemcpyDeviceToHost, s));
-        if (any_vis64) HIP_CHECK(hipMemcpyAsync((u8*)b->color_h + color_bytes, b->vis64.p, (size_t)vid_total * 8, hipMemcpyDeviceToHost, s));
-    }
-    HIP_CHECK(hipEventRecord(b->ev[5], s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    fs.in_flight = false;
+    const u32* counters = fs.counters;
+    const u32 nv = (u32)views.size();
+    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
         if (color_bytes && v.color_buffer) {
+            const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
             for (i32 y = 0; y < v.height; y += 1) {
-                memcpy(v.color_buffer + (size_t)y * v.color_pitch, b->color_h + plan[vi].color_base + (size_t)y * v.width,
-                       (size_t)v.width * 4);
+                memcpy(v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4);
             }
         }
-        if (v.vis64) memcpy(v.vis64, (u8*)b->color_h + color_bytes + plan[vi].vid_base * 8, plan[vi].grid * 8);
+        if (v.vis64) memcpy(v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8);
     }
 
     u64 px = 0, samples = 0;
@@ -890,13 +969,13 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     }
     sftx_frame_stats& st = c.stats;
     memset(&st, 0, sizeof(st));
-    st.meshlets_tested = nw;
+    st.meshlets_tested = fs.nw;
     st.meshlets_visible = counters[CNT_VISIBLE];
     st.meshlets_skinned = counters[CNT_SKINNED];
     st.batch_payloads = counters[CNT_BATCH_PAYLOADS];
     st.clip_triangles = counters[CNT_CLIP_TRIS];
     st.tile_payloads = counters[CNT_TILE_PAIRS];
-    st.tiles = nt;
+    st.tiles = fs.nt;
     st.samples = (u32)samples;
     st.pixels = (u32)px;
     st.shaded_pixels = counters[CNT_SHADED];
@@ -909,55 +988,88 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     st.occlusion_tri_tests = counters[CNT_OCC_TRIS];
     st.occlusion_invalid = counters[CNT_OCC_INVALID];
     st.occlusion_overflow = counters[CNT_OCC_OVERFLOW];
-    st.shadow_proxies = (u32)proxies.size();
-    st.ms_upload = ev_ms(b->ev[0], b->ev[1]);
-    st.ms_shadows = ev_ms(b->ev[1], b->ev[7]);
-    st.ms_geometry = ev_ms(b->ev[7], b->ev[2]);
-    st.ms_binning = ev_ms(b->ev[2], b->ev[3]);
-    st.ms_raster_shade = ev_ms(b->ev[3], b->ev[4]);
-    st.ms_raster = ev_ms(b->ev[3], b->ev[6]);
-    st.ms_shade = ev_ms(b->ev[6], b->ev[4]);
-    st.ms_download = ev_ms(b->ev[4], b->ev[5]);
-    st.ms_total = ev_ms(b->ev[0], b->ev[5]);
+    st.shadow_proxies = fs.n_proxies;
+    st.ms_upload = ev_ms(fs.ev[0], fs.ev[1]);
+    st.ms_shadows = ev_ms(fs.ev[1], fs.ev[7]);
+    st.ms_geometry = ev_ms(fs.ev[7], fs.ev[2]);
+    st.ms_binning = ev_ms(fs.ev[2], fs.ev[3]);
+    st.ms_raster_shade = ev_ms(fs.ev[3], fs.ev[4]);
+    st.ms_raster = ev_ms(fs.ev[3], fs.ev[6]);
+    st.ms_shade = ev_ms(fs.ev[6], fs.ev[4]);
+    st.ms_download = ev_ms(fs.ev[4], fs.ev[5]);
+    st.ms_total = ev_ms(fs.ev[0], fs.ev[5]);
 
     /* ---- capture (single-view presents) ---- */
-    if (!single) return;
+    if (1 != nv) return;
     const ViewRecord& v0 = views[0];
+    const u32 cap_flags = v0.capture_flags;
     b->cap_cols = v0.tile_cols;
     b->cap_rows = v0.tile_rows;
     b->cap_samples = v0.samples;
-    const size_t grid = plan[0].grid;
-    if (cap_vis) {
+    const size_t grid = fs.plan[0].grid;
+    if (fs.cap_vis) {
         b->h_vid.resize(grid);
         b->h_depth.resize(grid);
-        HIP_CHECK(hipMemcpy(b->h_vid.data(), b->vid.p, grid * 4, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(b->h_depth.data(), b->cap_depth.p, grid * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(b->h_vid.data(), fs.vid.p, grid * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(b->h_depth.data(), fs.cap_depth.p, grid * 4, hipMemcpyDeviceToHost));
     }
     if (cap_flags & SFTX_CAPTURE_SS_CACHE) {
         b->h_ss_count = v0.total_ss;
         b->h_ss.resize((size_t)v0.total_ss * 2304);
         b->h_ss_vis.assign(v0.total_ss, 0);
         if (v0.total_ss) {
-            HIP_CHECK(hipMemcpy(b->h_ss.data(), b->ss.p, b->h_ss.size(), hipMemcpyDeviceToHost));
-            HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), b->ss_vis.p, v0.total_ss, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(b->h_ss.data(), fs.ss.p, b->h_ss.size(), hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(b->h_ss_vis.data(), fs.ss_vis.p, v0.total_ss, hipMemcpyDeviceToHost));
         }
     }
     if (cap_flags & SFTX_CAPTURE_SHADOWS) {
         const bool occlusion = v0.shadows_enabled && !v0.point_lights.empty() && !v0.proxies.empty();
-        b->h_proxy_count = (u32)proxies.size();
-        b->h_proxy_vis.assign(proxies.size() * 64, 0);
+        b->h_proxy_count = fs.n_proxies;
+        b->h_proxy_vis.assign((size_t)fs.n_proxies * 64, 0);
         if (occlusion) {
-            HIP_CHECK(hipMemcpy(b->h_proxy_vis.data(), b->proxy_vis.p, b->h_proxy_vis.size(), hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(b->h_proxy_vis.data(), fs.proxy_vis.p, b->h_proxy_vis.size(), hipMemcpyDeviceToHost));
             /* counts of lights beyond the traced ones are not kept */
             const size_t lc = std::min<size_t>(v0.point_lights.size(), 64);
-            for (size_t i = 0; i < proxies.size(); i += 1) memset(&b->h_proxy_vis[i * 64 + lc], 0, 64 - lc);
+            for (size_t i = 0; i < fs.n_proxies; i += 1) memset(&b->h_proxy_vis[i * 64 + lc], 0, 64 - lc);
         }
     }
     if (cap_flags & SFTX_CAPTURE_CLIPS) {
-        b->h_clip_count = std::min(counters[CNT_CLIP_TRIS], b->clip_cap);
+        b->h_clip_count = std::min(counters[CNT_CLIP_TRIS], fs.clip_cap);
         b->h_clips.resize((size_t)b->h_clip_count * 164);
-        if (b->h_clip_count) HIP_CHECK(hipMemcpy(b->h_clips.data(), b->clip_rec.p, b->h_clips.size(), hipMemcpyDeviceToHost));
+        if (b->h_clip_count) HIP_CHECK(hipMemcpy(b->h_clips.data(), fs.clip_rec.p, b->h_clips.size(), hipMemcpyDeviceToHost));
     }
+}
+
+void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
+    if (views.empty()) return;
+    FrameSet& fs = b->sets[0];
+    if (fs.in_flight) HIP_CHECK(hipStreamSynchronize(fs.stream));
+    prepare_batch(b, *cp, views, fs);
+    launch_batch(b, views, fs);
+    complete_batch(b, *cp, views, fs);
+}
+
+void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {
+    FrameSet& fs = b->sets[set];
+    prepare_batch(b, *cp, views, fs);
+    launch_batch(b, views, fs);
+}
+
+void backend_complete(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {
+    complete_batch(b, *cp, views, b->sets[set]);
+}
+
+bool backend_batch_done(Backend* b, int set) {
+    return hipSuccess == hipEventQuery(b->sets[set].ev[5]);
+}
+
+u64 backend_resident_bytes(const Backend* b) {
+    u64 n = 0;
+    for (const DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d,
+                           &b->blas }) {
+        n += d->cap;
+    }
+    return n;
 }
 
 }  // namespace sft
@@ -1023,12 +1135,9 @@ extern "C" void sftx_set_graphs(i32 enabled) { ensure_backend()->use_graphs = 0 
 
 extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captures; }
 
-extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->color.p; }
+extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->sets[0].color.p; }
 
-extern "C" void sftx_synchronize(void) {
-    Backend* b = ensure_backend();
-    HIP_CHECK(hipStreamSynchronize(b->stream));
-}
+extern "C" void sftx_synchronize(void) { quiesce(ensure_backend()); }
 
 extern "C" f32 sftx_copy_calibration(u64 bytes) {
     Backend* b = ensure_backend();
@@ -1036,12 +1145,13 @@ extern "C" f32 sftx_copy_calibration(u64 bytes) {
     void *src = nullptr, *dst = nullptr;
     HIP_CHECK(hipMalloc(&src, bytes));
     HIP_CHECK(hipMalloc(&dst, bytes));
-    HIP_CHECK(hipMemsetAsync(src, 1, bytes, b->stream));
-    HIP_CHECK(hipEventRecord(b->ev[8], b->stream));
-    launch_copy((const uint4*)src, (uint4*)dst, bytes / 16, b->stream);
-    HIP_CHECK(hipEventRecord(b->ev[9], b->stream));
-    HIP_CHECK(hipStreamSynchronize(b->stream));
-    const f32 ms = ev_ms(b->ev[8], b->ev[9]);
+    FrameSet& fs = b->sets[0];
+    HIP_CHECK(hipMemsetAsync(src, 1, bytes, fs.stream));
+    HIP_CHECK(hipEventRecord(fs.ev[8], fs.stream));
+    launch_copy((const uint4*)src, (uint4*)dst, bytes / 16, fs.stream);
+    HIP_CHECK(hipEventRecord(fs.ev[9], fs.stream));
+    HIP_CHECK(hipStreamSynchronize(fs.stream));
+    const f32 ms = ev_ms(fs.ev[8], fs.ev[9]);
     HIP_CHECK(hipFree(src));
     HIP_CHECK(hipFree(dst));
     return ms;

@@ -195,6 +195,12 @@ void backend_destroy(Backend* b);
 /* renders a batch of views (one pass of kernels over all of them) and writes
    each view's outputs; blocks until they are in host memory */
 void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
+/* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be
+   in flight; complete blocks until its batch is done */
+void backend_submit(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
+void backend_complete(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
+bool backend_batch_done(Backend* b, int set);
+u64 backend_resident_bytes(const Backend* b);
 void backend_release_texture(Backend* b, u32 index);
 
 /* host estimate tables for _mm_rcp_ps / _mm_rsqrt_ps (rcp_probe.cpp) */
