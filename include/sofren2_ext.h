@@ -131,6 +131,44 @@ i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64);
 void sftx_batch_submit(void);
 void sftx_batch_end(void);
 
+/* ---- request server ---------------------------------------------------------
+   Renders requests against resident scene templates with continuous
+   batching.  sftx_server_template keeps the frame recorded since
+   sft_frame_begin (instances, poses, lights, materials, proxies; dmesh
+   geometry is copied) as a template; a request renders that frame with its
+   own camera, size and MSAA exactly as sft_present would.  While the server
+   runs it owns the renderer: sft_present and sftx_batch_submit are refused,
+   and textures, materials and models must not be created or released.
+   Requests are admitted in arrival order into the next batch while the
+   resident scene state, the framebuffer state of the batches in flight and
+   the request's own fit in the HBM budget; up to two batches are in flight,
+   so a batch is prepared and launched while the previous one executes. */
+typedef struct sftx_request {
+    i32 scene;                  /* template handle */
+    i32 width, height, samples; /* samples: 1, 2 or 4 */
+    sftvec camera_pos;
+    sftmat view, proj;
+    u32* color;                 /* host ARGB output, pitch width * 4, or null */
+    u64* vis64;                 /* host 64-bit visibility (see sftx_view_end), or null */
+} sftx_request;
+
+typedef struct sftx_server_stats {
+    u64 submitted, completed;
+    u64 batches, views_in_batches, max_batch;
+    u64 admission_waits;        /* times waiting requests did not fit the budget */
+    u64 resident_bytes;         /* scene, BVH and texture state in HBM */
+    u64 pending, in_flight;
+} sftx_server_stats;
+
+i32 sftx_server_template(void);
+i32 sftx_server_start(u64 hbm_budget_bytes, i32 max_views_per_batch);
+u64 sftx_server_submit(const sftx_request* r);         /* request id, 0 if rejected */
+i32 sftx_server_poll(u64 id);                          /* 1 done, 0 pending, -1 unknown */
+i32 sftx_server_wait(u64 id, f64* latency_ms);         /* blocks; 0 on success; forgets the id */
+void sftx_server_get_stats(sftx_server_stats* out);
+void sftx_server_stop(void);                           /* finishes every admitted request */
+void sftx_server_release(void);                        /* stops and drops the templates */
+
 /* Frame launches replayed from a HIP graph (default on): the kernel sequence
    of a frame is captured once and replayed while launch sizes and buffers are
    unchanged.  sftx_graph_captures counts the captures made so far. */

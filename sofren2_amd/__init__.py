@@ -40,6 +40,19 @@ class FrameStats(ctypes.Structure):
         "ms_total", "ms_shadows")]
 
 
+class Request(ctypes.Structure):
+    """sftx_request: a render request against a scene template."""
+    _fields_ = [("scene", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("samples", ctypes.c_int32), ("camera_pos", ctypes.c_float * 3), ("view", ctypes.c_float * 16),
+                ("proj", ctypes.c_float * 16), ("color", ctypes.c_void_p), ("vis64", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "completed", "batches", "views_in_batches", "max_batch", "admission_waits", "resident_bytes",
+        "pending", "in_flight")]
+
+
 def _bind(lib, name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -93,6 +106,16 @@ class Library:
         self.scene_use = _bind(s, "sftx_scene_use", i32, [i32])
         self.scene_draw = _bind(s, "sftx_scene_draw", None, [i32, i32, i32, i32])
         self.scene_free = _bind(s, "sftx_scene_free", None, [i32])
+        fp = ctypes.POINTER(ctypes.c_float)
+        self.scene_camera = _bind(s, "sftx_scene_camera", None, [i32, i32, i32, i32, fp, fp, fp])
+        self.server_template = _bind(g, "sftx_server_template", i32, [])
+        self.server_start = _bind(g, "sftx_server_start", i32, [ctypes.c_uint64, i32])
+        self.server_submit = _bind(g, "sftx_server_submit", ctypes.c_uint64, [ctypes.POINTER(Request)])
+        self.server_poll = _bind(g, "sftx_server_poll", i32, [ctypes.c_uint64])
+        self.server_wait = _bind(g, "sftx_server_wait", i32, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)])
+        self.server_get_stats = _bind(g, "sftx_server_get_stats", None, [ctypes.POINTER(ServerStats)])
+        self.server_stop = _bind(g, "sftx_server_stop", None, [])
+        self.server_release = _bind(g, "sftx_server_release", None, [])
         self.batch_begin = _bind(g, "sftx_batch_begin", None, [])
         self.view_begin = _bind(g, "sftx_view_begin", None, [i32, i32, i32])
         self.view_end = _bind(g, "sftx_view_end", i32, [vp, i32, vp])

@@ -1063,6 +1063,8 @@ bool backend_batch_done(Backend* b, int set) {
     return hipSuccess == hipEventQuery(b->sets[set].ev[5]);
 }
 
+void backend_forget_model(Backend* b, const sft_model* m) { b->models.erase(m); }
+
 u64 backend_resident_bytes(const Backend* b) {
     u64 n = 0;
     for (const DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d,

@@ -357,6 +357,7 @@ extern "C" void sft_init(i32 width, i32 height, i32 requested_threads) {
 
 extern "C" void sft_release(void) {
     if (!g_ctx) return;
+    sftx_server_release();
     backend_destroy(g_ctx->backend);
     munmap(g_ctx->scratch_arena.base, g_ctx->scratch_arena.max_size);
     delete g_ctx;
@@ -389,7 +390,7 @@ extern "C" void sft_frame_end(void) {
 }
 
 /* the frame recorded since sft_frame_begin (or sftx_view_begin) as one view */
-static void snapshot_view(const Context& c, ViewRecord& v) {
+void sft::snapshot_view(const Context& c, ViewRecord& v) {
     v.width = c.width;
     v.height = c.height;
     v.tile_cols = c.tile_cols;
@@ -425,6 +426,10 @@ static void snapshot_view(const Context& c, ViewRecord& v) {
 
 extern "C" void sft_present(void) {
     Context& c = C();
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sft_present while the request server runs; submit requests instead\n");
+        return;
+    }
     if (c.batch_open) {
         fprintf(stderr, "sofren2: sft_present inside a batch; use sftx_view_end and sftx_batch_submit\n");
         return;
@@ -502,6 +507,10 @@ extern "C" i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64) {
 extern "C" void sftx_batch_submit(void) {
     Context& c = C();
     if (!c.batch_open || c.batch_views.empty()) return;
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sftx_batch_submit while the request server runs; submit requests instead\n");
+        return;
+    }
     if (!c.backend) c.backend = backend_create();
     backend_render(c.backend, &c, c.batch_views);
 }
@@ -515,13 +524,9 @@ extern "C" void sftx_batch_end(void) {
 
 /* ================================================================ misc setters */
 /* view_proj = view * proj and six normalised planes (sofren2.c:8345-8376) */
-extern "C" void sft_set_camera(sftvec cam_pos, sftmat view, sftmat proj) {
-    Context& c = C();
-    c.camera_pos = cam_pos;
-    c.view = view;
-    c.proj = proj;
-    c.view_proj = sft_m_mul(view, proj);
-    const sftmat& vp = c.view_proj;
+void sft::camera_setup(const sftmat& view, const sftmat& proj, sftmat* view_proj, f32 out[6][4]) {
+    *view_proj = sft_m_mul(view, proj);
+    const sftmat& vp = *view_proj;
     f32 planes[6][4];
     for (i32 i = 0; i < 4; i += 1) {
         planes[0][i] = vp.m[i][3] + vp.m[i][0];
@@ -534,11 +539,19 @@ extern "C" void sft_set_camera(sftvec cam_pos, sftmat view, sftmat proj) {
     for (i32 p = 0; p < 6; p += 1) {
         const f32 len = sqrtf(planes[p][0] * planes[p][0] + planes[p][1] * planes[p][1] + planes[p][2] * planes[p][2]);
         const f32 inv_len = (len < SFT_EPSILON) ? SFT_EPSILON : 1.f / len;
-        c.planes[p][0] = planes[p][0] * inv_len;
-        c.planes[p][1] = planes[p][1] * inv_len;
-        c.planes[p][2] = planes[p][2] * inv_len;
-        c.planes[p][3] = planes[p][3] * inv_len;
+        out[p][0] = planes[p][0] * inv_len;
+        out[p][1] = planes[p][1] * inv_len;
+        out[p][2] = planes[p][2] * inv_len;
+        out[p][3] = planes[p][3] * inv_len;
     }
+}
+
+extern "C" void sft_set_camera(sftvec cam_pos, sftmat view, sftmat proj) {
+    Context& c = C();
+    c.camera_pos = cam_pos;
+    c.view = view;
+    c.proj = proj;
+    camera_setup(view, proj, &c.view_proj, c.planes);
 }
 
 extern "C" void sft_set_debug_mode(enum sft_debug_mode mode) {

@@ -186,6 +186,12 @@ struct ViewRecord {
 struct Context;
 extern Context* g_ctx;
 
+/* view_proj = view * proj and the six normalised frustum planes
+   (sofren2.c:8345-8376) */
+void camera_setup(const sftmat& view, const sftmat& proj, sftmat* view_proj, f32 planes[6][4]);
+/* the frame recorded since sft_frame_begin as one view */
+void snapshot_view(const Context& c, ViewRecord& v);
+
 /* frame TLAS over the drawn instances (tlas.cpp); returns the root node */
 u32 build_tlas(const std::vector<sft_instance>& instances, std::vector<sft_bvh_node>& nodes);
 
@@ -201,6 +207,8 @@ void backend_submit(Backend* b, Context* ctx, std::vector<ViewRecord>& views, in
 void backend_complete(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
 bool backend_batch_done(Backend* b, int set);
 u64 backend_resident_bytes(const Backend* b);
+/* forgets a model's residency (its slots stay allocated) */
+void backend_forget_model(Backend* b, const sft_model* m);
 void backend_release_texture(Backend* b, u32 index);
 
 /* host estimate tables for _mm_rcp_ps / _mm_rsqrt_ps (rcp_probe.cpp) */
@@ -268,6 +276,7 @@ struct Context {
 
     /* batched views being recorded (sftx_batch_begin .. sftx_batch_submit) */
     bool batch_open = false;
+    bool server_running = false;    /* the request server owns the backend */
     std::vector<ViewRecord> batch_views;
 
     EstimateTables est;

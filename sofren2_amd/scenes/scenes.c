@@ -119,12 +119,45 @@ static void sftx_plane_spheres_create(void) {
     g_scene.meshlet_count = 25u + SFTX_MAX_SPHERES * 34u;
 }
 
+/* camera of frame `frame` of a scene kind's path for a width x height view */
+static void sftx_camera_for(i32 kind, i32 frame, i32 width, i32 height, sftvec* pos, sftmat* view, sftmat* proj) {
+    if (SFTX_SCENE_PLANE_SPHERES == kind) {
+        *pos = SFT_V(0.f, 6.f, -14.f);
+        *view = sft_m_view_look_at(*pos, SFT_V(0.f, 0.f, 0.f), SFT_V(0.f, 1.f, 0.f));
+        *proj = sft_m_proj(60.f, (f32)width / (f32)height, 0.1f, 1000.f);
+        return;
+    }
+    const f32 th = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
+    *pos = SFT_V(25.f * cosf(th), 3.f + 4.f * sinf(2.f * th), 25.f * sinf(th));
+    const sftvec target = SFT_V(5.f * sinf(th), 0.f, -5.f * cosf(th));
+    *view = sft_m_view_look_at(*pos, target, SFT_V(0.f, 1.f, 0.f));
+    *proj = sft_m_proj(70.f, (f32)width / (f32)height, 0.1f, 1000.f);
+}
+
+/* a frame number below 1000 sets both the pose and the camera; 1000 * (p + 1)
+   + c poses the scene at frame p and looks from camera frame c */
+static void sftx_split_frame(i32 frame, i32* pose, i32* cam) {
+    if (frame >= 1000) {
+        *pose = frame / 1000 - 1;
+        *cam = frame % 1000;
+    }
+    else {
+        *pose = frame;
+        *cam = frame;
+    }
+}
+
+static void sftx_set_scene_camera(i32 frame) {
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    sftvec pos;
+    sftmat view, proj;
+    sftx_camera_for(g_scene.kind, cam, g_scene.width, g_scene.height, &pos, &view, &proj);
+    sft_set_camera(pos, view, proj);
+}
+
 static void sftx_plane_spheres_render(i32 frame) {
-    (void)frame;
-    const sftvec cam = SFT_V(0.f, 6.f, -14.f);
-    sft_set_camera(cam,
-        sft_m_view_look_at(cam, SFT_V(0.f, 0.f, 0.f), SFT_V(0.f, 1.f, 0.f)),
-        sft_m_proj(60.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    sftx_set_scene_camera(frame);
 
     sft_set_ambient_light(0.35f, 0.35f, 0.35f);
     sft_draw_point_light(SFT_V(0.f, 10.f, 0.f), 1.f, 1.f, 1.f, 1.f, 30.f, 1);
@@ -602,11 +635,7 @@ static void sftx_box_room_create(int with_blas) {
 
 /* fixed 120-frame orbit through the lattice */
 static void sftx_box_room_render(i32 frame) {
-    const f32 th = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
-    const sftvec cam = SFT_V(25.f * cosf(th), 3.f + 4.f * sinf(2.f * th), 25.f * sinf(th));
-    const sftvec target = SFT_V(5.f * sinf(th), 0.f, -5.f * cosf(th));
-    sft_set_camera(cam, sft_m_view_look_at(cam, target, SFT_V(0.f, 1.f, 0.f)),
-                   sft_m_proj(70.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    sftx_set_scene_camera(frame);
     sft_set_ambient_light(0.3f, 0.3f, 0.3f);
     for (int i = 0; i < 8; i += 1) {
         sft_draw_point_light(g_room.light_pos[i], g_room.light_col[i].x, g_room.light_col[i].y, g_room.light_col[i].z,
@@ -821,18 +850,16 @@ static void sftx_crowd_pose(i32 frame) {
 }
 
 static void sftx_crowd_render(i32 frame) {
-    const f32 th = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
-    const sftvec cam = SFT_V(25.f * cosf(th), 3.f + 4.f * sinf(2.f * th), 25.f * sinf(th));
-    const sftvec target = SFT_V(5.f * sinf(th), 0.f, -5.f * cosf(th));
-    sft_set_camera(cam, sft_m_view_look_at(cam, target, SFT_V(0.f, 1.f, 0.f)),
-                   sft_m_proj(70.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    sftx_set_scene_camera(frame);
     sft_set_ambient_light(0.3f, 0.3f, 0.3f);
     for (i32 l = 0; l < SFTX_SHADOW_LIGHTS; l += 1) {
         sft_draw_point_light(g_crowd.light_pos[l], g_crowd.light_col[l].x, g_crowd.light_col[l].y, g_crowd.light_col[l].z,
                              1.f, 40.f, (u32)(l + 1));
     }
     sft_draw_instance(&g_room.inst);
-    sftx_crowd_pose(frame);
+    sftx_crowd_pose(pose);
     for (i32 c = 0; c < SFTX_CHARS; c += 1) {
         sft_instance inst;
         memset(&inst, 0, sizeof(inst));
@@ -897,6 +924,13 @@ void sftx_scene_draw(i32 handle, i32 frame, i32 width, i32 height) {
     sftx_draw_current(frame);
     g_scene.width = w;
     g_scene.height = h;
+}
+
+void sftx_scene_camera(i32 handle, i32 frame, i32 width, i32 height, sftvec* pos, sftmat* view, sftmat* proj) {
+    if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return;
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    sftx_camera_for(sftx_slots[handle]->scene.kind, cam, width, height, pos, view, proj);
 }
 
 void sftx_scene_free(i32 handle) {

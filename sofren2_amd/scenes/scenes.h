@@ -60,6 +60,9 @@ i32 sftx_scene_new(i32 kind, u32 seed, i32 width, i32 height);
 i32 sftx_scene_use(i32 handle);
 void sftx_scene_draw(i32 handle, i32 frame, i32 width, i32 height);
 void sftx_scene_free(i32 handle);
+/* the camera sftx_scene_draw would set for that frame and view size; frame
+   numbers 1000 * (p + 1) + c pose the scene at frame p with camera frame c */
+void sftx_scene_camera(i32 handle, i32 frame, i32 width, i32 height, sftvec* pos, sftmat* view, sftmat* proj);
 
 /* Frame statistics of the scene description (not of the renderer). */
 u32 sftx_scene_triangle_count(void);

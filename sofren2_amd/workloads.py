@@ -68,3 +68,130 @@ class MixedBatch:
         for h in self.handles:
             self.lib.scene_free(h)
         self.handles = []
+
+
+# ---------------------------------------------------------------- serving
+def served_scene_specs(n_scenes=32):
+    """Seeded variants of the three scene kinds for the served workload:
+    (kind, seed, pose_frame).  Kinds cycle interior, crowd, plane."""
+    kinds = [1, 2, 0]
+    return [(kinds[i % 3], 10 + i, (7 * i) % 120) for i in range(n_scenes)]
+
+
+def served_requests(n, rate_hz, n_scenes, seed=1, min_px=128 * 128, max_px=3840 * 2160):
+    """Poisson arrivals: [(t_arrival_s, template_index, camera_frame, width, height, samples)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        px = math.exp(rng.uniform(math.log(min_px), math.log(max_px)))
+        aspect = rng.uniform(1.0, 2.0)
+        h = max(16, int(round(math.sqrt(px / aspect))))
+        w = max(16, int(round(h * aspect)))
+        r = rng.uniform()
+        samples = 1 if r < 1 / 3 else (2 if r < 2 / 3 else 4)
+        out.append((t, int(rng.integers(0, n_scenes)), int(rng.integers(0, 120)), w, h, samples))
+    return out
+
+
+class Server:
+    """Resident scene templates behind the request server."""
+
+    def __init__(self, lib, specs, budget_bytes=200 << 30, max_views=64):
+        import ctypes
+        import sofren2_amd as S
+        self.S, self.lib, self.ct = S, lib, ctypes
+        self.specs = specs
+        self.handles, self.templates = [], []
+        for kind, seed, pose in specs:
+            h = lib.scene_new(kind, seed, 640, 360)
+            if h < 0:
+                raise RuntimeError("scene creation failed")
+            self.handles.append(h)
+            lib.sft_frame_begin()
+            lib.scene_draw(h, pose, 640, 360)
+            self.templates.append(lib.server_template())
+            lib.sft_frame_end()
+        if lib.server_start(budget_bytes, max_views) != 0:
+            raise RuntimeError("server start failed")
+
+    def request(self, template_index, camera_frame, width, height, samples, color=None, vis=None):
+        ct = self.ct
+        kind, seed, pose = self.specs[template_index]
+        q = self.S.Request()
+        q.scene = self.templates[template_index]
+        q.width, q.height, q.samples = width, height, samples
+        pos, view, proj = (ct.c_float * 3)(), (ct.c_float * 16)(), (ct.c_float * 16)()
+        self.lib.scene_camera(self.handles[template_index], camera_frame, width, height, pos, view, proj)
+        q.camera_pos, q.view, q.proj = pos, view, proj
+        q.color = color.ctypes.data if color is not None else None
+        q.vis64 = vis.ctypes.data if vis is not None else None
+        return q
+
+    def submit(self, q):
+        rid = self.lib.server_submit(self.ct.byref(q))
+        if rid == 0:
+            raise RuntimeError("request rejected")
+        return rid
+
+    def wait(self, rid):
+        lat = self.ct.c_double()
+        if self.lib.server_wait(rid, self.ct.byref(lat)) != 0:
+            raise RuntimeError("unknown request")
+        return lat.value
+
+    def stats(self):
+        st = self.S.ServerStats()
+        self.lib.server_get_stats(self.ct.byref(st))
+        return {n: int(getattr(st, n)) for n, _ in self.S.ServerStats._fields_}
+
+    def close(self):
+        self.lib.server_release()
+        for h in self.handles:
+            self.lib.scene_free(h)
+
+
+def reference_frame_code(pose_frame, camera_frame):
+    """Frame number that makes a scene take the template's pose and the
+    request's camera (what the reference checker is given)."""
+    return 1000 * (pose_frame + 1) + camera_frame
+
+
+def run_served(server, requests, keep=None):
+    """Replays Poisson arrivals against the server from a submitting thread;
+    returns per-request latencies (ms), wall time and, for the indices in
+    `keep`, the colour and visibility outputs."""
+    import threading
+    import time
+    kept = {}
+    ids = [None] * len(requests)
+    bufs = [None] * len(requests)
+    keep = set(keep or ())
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, (ta, si, cf, w, h, ns) in enumerate(requests):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            color = vis = None
+            if i in keep:
+                color = np.zeros((h, w), dtype=np.uint32)
+                vis = np.zeros(((w + 63) // 64) * 64 * ((h + 63) // 64) * 64 * ns, dtype=np.uint64)
+                kept[i] = (color, vis)
+            q = server.request(si, cf, w, h, ns, color, vis)
+            bufs[i] = q
+            ids[i] = server.submit(q)
+
+    t_start = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = []
+    for i in range(len(requests)):
+        while ids[i] is None:
+            time.sleep(0.0002)
+        lat.append(server.wait(ids[i]))
+    th.join()
+    wall = time.perf_counter() - t_start
+    return np.array(lat), wall, kept
