@@ -98,6 +98,63 @@ def mixed_batch_leg(lib, steps):
         mb.close()
 
 
+def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
+    """Poisson arrivals over resident scene templates at each rate; each
+    request a random template, camera frame, resolution (128x128 to 3840x2160
+    pixels, log-uniform) and MSAA.  Returns throughput and latency per rate,
+    and the accuracy of a sample against the reference checker when it is
+    present."""
+    from sofren2_amd import workloads
+    specs = workloads.served_scene_specs(n_scenes)
+    srv = workloads.Server(lib, specs, budget_bytes=200 << 30, max_views=64)
+    out = {"scenes": n_scenes, "budget_gb": 200, "rates": []}
+    try:
+        # warm-up: make every template resident
+        warm = [srv.submit(srv.request(i, 0, 256, 256, 1)) for i in range(n_scenes)]
+        for r in warm:
+            srv.wait(r)
+        sample_reqs, sample_outs = [], {}
+        for k, rate in enumerate(rates):
+            reqs = workloads.served_requests(n_requests, rate, n_scenes, seed=100 + k)
+            keep = range(0, n_requests, max(1, int(round(1 / sample_frac)))) if k == 0 else ()
+            st0 = srv.stats()
+            lat, wall, kept = workloads.run_served(srv, reqs, keep=keep)
+            st1 = srv.stats()
+            batches = st1["batches"] - st0["batches"]
+            out["rates"].append({
+                "offered_req_s": rate, "requests": n_requests,
+                "throughput_req_s": round(n_requests / wall, 2),
+                "mpix_s": round(sum(w * h for _, _, _, w, h, _ in reqs) / wall / 1e6, 1),
+                "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                               "p99": round(float(np.percentile(lat, 99)), 3)},
+                "mean_batch": round((st1["views_in_batches"] - st0["views_in_batches"]) / max(batches, 1), 2),
+                "batches": batches})
+            if k == 0:
+                sample_reqs = [(i, reqs[i]) for i in kept]
+                sample_outs = kept
+        out["resident_gb"] = round(srv.stats()["resident_bytes"] / 2**30, 3)
+    finally:
+        srv.close()
+    # accuracy of the sample against the reference checker, when it is here
+    ref = os.path.join(ROOT, "oracle", "_ref", "sofren2_ref")
+    if os.path.exists(ref):
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from tests import parity
+        from tests.refrun import ref_dump
+        checked, exact_vis, color_ok = 0, 0, 0
+        for i, (_, si, cf, w, h, ns) in sample_reqs:
+            kind, seed, pose = specs[si]
+            r = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns)
+            color, vis = sample_outs[i]
+            rep = parity.compare_frame({"color": color, "vid": (vis & 0xFFFFFFFF).astype(np.uint32),
+                                        "depth": (vis >> 32).astype(np.uint32).view(np.float32)}, r)
+            checked += 1
+            exact_vis += int(rep["vid_mismatch"] == 0 and rep["depth_mismatch"] == 0)
+            color_ok += int(rep["color_within1_frac"] >= 0.999)
+        out["accuracy_sample"] = {"checked": checked, "visibility_bit_exact": exact_vis, "color_within_bar": color_ok}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +165,9 @@ def main():
     ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
                     help="workload; box_room is the headline 1080p textured interior")
     ap.add_argument("--batch-steps", type=int, default=10, help="steps of the 64-view mixed batch (0: skip)")
+    ap.add_argument("--served-requests", type=int, default=200, help="requests per offered rate (0: skip serving)")
+    ap.add_argument("--served-rates", default="100,500,2000", help="offered request rates (req/s)")
+    ap.add_argument("--served-scenes", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
@@ -224,6 +284,9 @@ def main():
     lib.scene_destroy()
     if args.batch_steps > 0:
         out["mixed_batch"] = mixed_batch_leg(lib, args.batch_steps)
+    if args.served_requests > 0:
+        rates = [float(x) for x in args.served_rates.split(",") if x]
+        out["served"] = served_leg(lib, rates, args.served_requests, args.served_scenes)
     lib.sft_release()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.scene, w, h, args.cpu_frames)

@@ -566,7 +566,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         color_total += (u64)v.width * v.height;
         max_samples = std::max<u32>(max_samples, (u32)v.samples);
         any_vis64 = any_vis64 || v.vis64;
-        host_color = host_color || (SFTX_OUTPUT_HOST == b->output_mode && v.color_buffer);
+        host_color = host_color || ((SFTX_OUTPUT_HOST == b->output_mode || v.host_output) && v.color_buffer);
         for (u32 t = 0; t < vp.n_tiles; t += 1) {
             tile_view.push_back(vi);
             (v.samples > 1 ? shade_tiles_ms : shade_tiles).push_back(vp.tile_base + t);

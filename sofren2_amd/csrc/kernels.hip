@@ -943,10 +943,38 @@ DEV int calc_lod(const FramePtrs& P, u32 h, f32 dux, f32 duy, f32 dvx, f32 dvy) 
     return min(max(lod, 0), m.mip_levels - 1);
 }
 
-struct TileLights {
+/* a tile's light list and, for views with shadows, each light's capsule
+   records, staged in LDS by the shading workgroup */
+struct TileShading {
     int count;
-    u16 idx[kMaxTileLights];
+    u32 np[kMaxTileLights];                             /* capsules per light (0 without shadows) */
+    PointLightGpu lights[kMaxTileLights];
+    ProxyRec prox[kMaxTileLights][kMaxLightProxies];
 };
+
+/* fills `ts` for tile `tile` of view VW with the workgroup's threads */
+DEV void stage_tile_shading(const FramePtrs& P, const ViewGpu& VW, u32 tile, TileShading& ts, u32* s_li) {
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const FrameConst& fc = VW.fc;
+    const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
+    const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
+    const int count = (int)tl[0];
+    if (tid < count) {
+        const u32 li = tl[1 + tid];
+        s_li[tid] = li;
+        ts.lights[tid] = P.point_lights[VW.pl_base + li];
+        ts.np[tid] = shadows ? P.light_proxy_count[VW.pl_base + li] : 0u;
+    }
+    if (0 == tid) ts.count = count;
+    __syncthreads();
+    if (shadows) {
+        for (int e = tid; e < count * kMaxLightProxies; e += nthreads) {
+            const int i = e / kMaxLightProxies, k = e % kMaxLightProxies;
+            if ((u32)k < ts.np[i]) ts.prox[i][k] = P.light_proxies[(u64)(VW.pl_base + s_li[i]) * kMaxLightProxies + k];
+        }
+    }
+    __syncthreads();
+}
 
 /* shades one pixel of triangle t (sofren2.c:6190-6620, non-MSAA); dux..dvy are
    the block's first-lane UV derivatives.  `group` holds the lanes of this
@@ -955,7 +983,7 @@ struct TileLights {
    wave that reaches the lighting loop runs the same light and proxy
    iterations, so the group's lanes meet at each ballot. */
 DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const Tri& t,
-                   const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileLights& tl, u32& fetch,
+                   const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileShading& ts, u32& fetch,
                    u32 group, int block_lane) {
     const FrameConst& fc = VW.fc;
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
@@ -1043,10 +1071,9 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
         /* on a lightmapped surface point lights act only through capsule
            shadows (sofren2.c:6574-6579): without proxies every light leaves
            lm_shadow at exactly 1, so the loop is skipped */
-        const int n_eval = (lightmapped && !shadows) ? 0 : tl.count;
+        const int n_eval = (lightmapped && !shadows) ? 0 : ts.count;
         for (int i = 0; i < n_eval; i += 1) {
-            const u32 li = tl.idx[i];
-            const PointLightGpu& L = P.point_lights[VW.pl_base + li];
+            const PointLightGpu& L = ts.lights[i];
             const V3 lraw = sub3(V3{L.pos[0], L.pos[1], L.pos[2]}, pos);
             const f32 d2 = dot3(lraw, lraw);
             f32 att = smax(0.f, fmaf(-d2, L.inv_r2, 1.f));
@@ -1063,15 +1090,15 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
             /* capsule soft shadows (6509-6568): closest approach of the pixel-
                to-light segment and each cached capsule, smooth-max blended */
             f32 max_shadow = 0.f;
-            const u32 np = shadows ? P.light_proxy_count[VW.pl_base + li] : 0u;
+            const u32 np = ts.np[i];
             if (np > 0u) {
                 const f32 lrl = sqrtf(d2);
-                const ProxyRec* recs = P.light_proxies + (u64)(VW.pl_base + li) * kMaxLightProxies;
+                const f32 inv_d2 = est.rcp_refined(d2);
                 for (u32 k = 0; k < np; k += 1) {
-                    const ProxyRec& c = recs[k];
+                    const ProxyRec& c = ts.prox[i][k];
                     const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
                     const bool between = wdv > 0.f && wdv < d2;
-                    const f32 tf = wdv * est.rcp_refined(d2);
+                    const f32 tf = wdv * inv_d2;
                     const f32 cd2 = fmaf(-wdv, tf, c.wdw);
                     const bool ray_active = between && cd2 < c.rb2 && (lightmapped || lit);
                     const u64 am = __ballot(ray_active);
@@ -1395,7 +1422,8 @@ __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
    32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
    order (x offsets 0..3, rows 0..1) */
 __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
-    __shared__ TileLights s_tl;
+    __shared__ TileShading s_ts;
+    __shared__ u32 s_li[kMaxTileLights];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     /* 16 blocks of 32x8 pixels per tile */
     const u32 tile = P.shade_tiles[blockIdx.x >> 4];
@@ -1405,13 +1433,8 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const i32 x0 = (i32)(local % (u32)fc.tile_cols) * 64 + (i32)(sub & 1u) * 32;
     const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
     const Est est = make_est(P);
-    if (0 == tid) {
-        const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
-        s_tl.count = (int)tl[0];
-        for (int i = 0; i < s_tl.count; i += 1) s_tl.idx[i] = (u16)tl[1 + i];
-    }
-    __syncthreads();
-    const TileLights tl = s_tl;
+    stage_tile_shading(P, VW, tile, s_ts, s_li);
+    const TileShading& tl = s_ts;
 
     const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
     const i32 bx = x0 + blk * 4, by = y0 + wave * 2;
@@ -1497,7 +1520,8 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    and add the colour weighted by their share of samples.  Each pixel starts
    from the background colour times its empty-sample share. */
 __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
-    __shared__ TileLights s_tl;
+    __shared__ TileShading s_ts;
+    __shared__ u32 s_li[kMaxTileLights];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     /* 16 blocks of 32x8 pixels per tile */
     const u32 tile = P.shade_tiles[P.batch->n_shade_tiles + (blockIdx.x >> 4)];
@@ -1507,13 +1531,8 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     const i32 x0 = (i32)(local % (u32)fc.tile_cols) * 64 + (i32)(sub & 1u) * 32;
     const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
     const Est est = make_est(P);
-    if (0 == tid) {
-        const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
-        s_tl.count = (int)tl[0];
-        for (int i = 0; i < s_tl.count; i += 1) s_tl.idx[i] = (u16)tl[1 + i];
-    }
-    __syncthreads();
-    const TileLights tl = s_tl;
+    stage_tile_shading(P, VW, tile, s_ts, s_li);
+    const TileShading& tl = s_ts;
 
     const int S = fc.samples;
     const f32 inv_s = 1.f / (f32)S;

@@ -146,6 +146,7 @@ ViewRecord make_view(const Template& t, const sftx_request& r) {
     camera_setup(r.view, r.proj, &v.view_proj, v.planes);
     v.color_buffer = r.color;
     v.color_pitch = r.width;
+    v.host_output = nullptr != r.color;
     v.vis64 = r.vis64;
     v.capture_flags = 0;
     return v;

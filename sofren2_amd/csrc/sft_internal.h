@@ -159,6 +159,7 @@ struct ViewRecord {
     u32* color_buffer = nullptr;
     i32 color_pitch = 0;            /* in u32 */
     u64* vis64 = nullptr;           /* [samples][tile_rows*64][tile_cols*64] */
+    bool host_output = false;       /* copy colour to color_buffer whatever the output mode */
     u8 should_clear = 0;
     u32 clear_color = 0;
     i32 debug_mode = 0;
