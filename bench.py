@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=200, help="requests per offered rate (0: skip serving)")
     ap.add_argument("--served-rates", default="100,500,2000", help="offered request rates (req/s)")
     ap.add_argument("--served-scenes", type=int, default=32)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run the streaming copy kernel over this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
@@ -183,6 +185,10 @@ def main():
         raise SystemExit("scene creation failed")
     tris = lib.scene_tris()
     color = np.zeros((h, w), dtype=np.uint32)
+    if args.copy_calibration_mb > 0:
+        for _ in range(3):
+            ms = lib.copy_calibration(args.copy_calibration_mb << 20)
+        print(f"copy calibration: {args.copy_calibration_mb} MiB in {ms:.4f} ms", file=sys.stderr)
 
     # drop-in path with the frame copied to host memory (PCIe included)
     lib.set_output(0)

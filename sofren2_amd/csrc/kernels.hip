@@ -39,6 +39,13 @@ namespace sft {
 
 #define DEV __device__ __forceinline__
 
+/* element `idx` of an array whose byte offset fits 32 bits: lets the
+   compiler use a scalar base with a 32-bit vector offset */
+template <class T>
+DEV T ld32(const T* base, u32 idx) {
+    return *(const T*)((const char*)base + (u32)(idx * (u32)sizeof(T)));
+}
+
 DEV u32 f2u(f32 f) { return __float_as_uint(f); }
 DEV f32 u2f(u32 u) { return __uint_as_float(u); }
 /* _mm256_min_ps(a, b) / _mm256_max_ps(a, b) per lane */
@@ -808,6 +815,9 @@ DEV void load_tri(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
         }
         t.inst = V.inst_base + ((const u32*)r)[39];
         t.slot = P.insts[t.inst].model_base + ((const u32*)r)[40];
+        /* keeps these loads in this branch: merged with the meshlet loads
+           below they would need a full 64-bit address each */
+        asm volatile("" ::: "memory");
         return;
     }
     t.inst = V.inst_base + (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS));
@@ -815,15 +825,18 @@ DEV void load_tri(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
     const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
     const InstGpu* inst = &P.insts[t.inst];
     t.slot = inst->model_base + gm;
-    const u8* ib = P.inds + (u64)t.slot * 512u + (tri >> 3) * 32u + (tri & 7u);
-    const f32* ss = P.ss + (u64)(inst->ss_offset + gm) * 576u;
-    const f32* ap = P.attr + (u64)t.slot * 448u;
+    const u32 ib = t.slot * 512u + (tri >> 3) * 32u + (tri & 7u);
+    const u32 ss = (inst->ss_offset + gm) * 576u;
+    const u32 ap = t.slot * 448u;
     for (int k = 0; k < 3; k += 1) {
-        const int i = ib[k * 8] & 63;
-        t.sx[k] = ss[i]; t.sy[k] = ss[64 + i]; t.iw[k] = ss[128 + i];
-        t.px[k] = ss[192 + i]; t.py[k] = ss[256 + i]; t.pz[k] = ss[320 + i];
-        t.nx[k] = ss[384 + i]; t.ny[k] = ss[448 + i]; t.nz[k] = ss[512 + i];
-        t.u[k] = ap[192 + i]; t.v[k] = ap[256 + i]; t.lu[k] = ap[320 + i]; t.lv[k] = ap[384 + i];
+        const u32 i = ld32(P.inds, ib + (u32)k * 8u) & 63u;
+        /* one address per vertex; the planes are constant offsets from it */
+        const f32* sv = P.ss + (ss + i);
+        const f32* av = P.attr + (ap + i);
+        t.sx[k] = sv[0]; t.sy[k] = sv[64]; t.iw[k] = sv[128];
+        t.px[k] = sv[192]; t.py[k] = sv[256]; t.pz[k] = sv[320];
+        t.nx[k] = sv[384]; t.ny[k] = sv[448]; t.nz[k] = sv[512];
+        t.u[k] = av[192]; t.v[k] = av[256]; t.lu[k] = av[320]; t.lv[k] = av[384];
     }
 }
 
@@ -876,7 +889,7 @@ DEV u32 texel_addr(const TexMeta& m, int lod, i32 x, i32 y) {
 DEV void sample_bilinear(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32& r, f32& g, f32& b) {
     if (SFT_INVALID_HANDLE == h) { r = 1.f; g = 1.f; b = 1.f; return; }
     const TexMeta& m = P.tex_meta[h & SFT_HANDLE_INDEX_MASK];
-    const u32* px = P.texels + m.mip_offset[lod];
+    const u32 px = m.mip_offset[lod];
     const i32 cw = max(m.w >> lod, 1), ch = max(m.h >> lod, 1);
     const f32 upx = cu * (f32)cw - 0.5f;
     const f32 vpx = cv * (f32)ch - 0.5f;
@@ -894,10 +907,10 @@ DEV void sample_bilinear(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32
         u1 = max(min(u0 + 1, cw - 1), 0); v1 = max(min(v0 + 1, ch - 1), 0);
         u0w = max(min(u0, cw - 1), 0); v0w = max(min(v0, ch - 1), 0);
     }
-    const u32 c00 = px[texel_addr(m, lod, u0w, v0w)];
-    const u32 c10 = px[texel_addr(m, lod, u1, v0w)];
-    const u32 c01 = px[texel_addr(m, lod, u0w, v1)];
-    const u32 c11 = px[texel_addr(m, lod, u1, v1)];
+    const u32 c00 = ld32(P.texels, px + texel_addr(m, lod, u0w, v0w));
+    const u32 c10 = ld32(P.texels, px + texel_addr(m, lod, u1, v0w));
+    const u32 c01 = ld32(P.texels, px + texel_addr(m, lod, u0w, v1));
+    const u32 c11 = ld32(P.texels, px + texel_addr(m, lod, u1, v1));
     f32 o[3];
     const int sh[3] = {16, 8, 0};
     for (int k = 0; k < 3; k += 1) {
@@ -914,14 +927,14 @@ DEV void sample_bilinear(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32
 DEV void sample_nearest(const FramePtrs& P, u32 h, int lod, f32 cu, f32 cv, f32& r, f32& g, f32& b) {
     if (SFT_INVALID_HANDLE == h) { r = 1.f; g = 1.f; b = 1.f; return; }
     const TexMeta& m = P.tex_meta[h & SFT_HANDLE_INDEX_MASK];
-    const u32* px = P.texels + m.mip_offset[lod];
+    const u32 px = m.mip_offset[lod];
     const i32 cw = max(m.w >> lod, 1), ch = max(m.h >> lod, 1);
     const i32 u0 = cvtt(floorf(cu * (f32)cw));
     const i32 v0 = cvtt(floorf(cv * (f32)ch));
     i32 uw, vw;
     if (m.is_pot) { uw = u0 & (cw - 1); vw = v0 & (ch - 1); }
     else { uw = max(min(u0, cw - 1), 0); vw = max(min(v0, ch - 1), 0); }
-    const u32 c = px[texel_addr(m, lod, uw, vw)];
+    const u32 c = ld32(P.texels, px + texel_addr(m, lod, uw, vw));
     r = unpack_ch(c, 16); g = unpack_ch(c, 8); b = unpack_ch(c, 0);
 }
 
@@ -1358,17 +1371,22 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
                 sy0 = r[3]; sy1 = r[4]; sy2 = r[5];
                 iw0 = r[6]; iw1 = r[7]; iw2 = r[8];
                 key = P.clip_key[idx];
+                /* keeps these loads in this branch (merged with the meshlet
+                   loads they would need a 64-bit address each) */
+                asm volatile("" ::: "memory");
             }
             else {
                 if (0u == ((mask >> g) & 1u)) continue;
                 const u32 wi = idx >> 4, b = idx & 15u;
                 const WorkItem wk = P.work[wi];
-                const u8* ib = P.inds + (u64)wk.slot * 512u + b * 32u + (u32)g;
-                const f32* ss = P.ss + (u64)wk.ss * 576u;
-                const int i0 = ib[0] & 63, i1 = ib[8] & 63, i2 = ib[16] & 63;
-                sx0 = ss[i0]; sy0 = ss[64 + i0]; iw0 = ss[128 + i0];
-                sx1 = ss[i1]; sy1 = ss[64 + i1]; iw1 = ss[128 + i1];
-                sx2 = ss[i2]; sy2 = ss[64 + i2]; iw2 = ss[128 + i2];
+                const u32 ib = wk.slot * 512u + b * 32u + (u32)g;
+                const u32 ss = wk.ss * 576u;
+                const f32* v0 = P.ss + (ss + (ld32(P.inds, ib) & 63u));
+                const f32* v1 = P.ss + (ss + (ld32(P.inds, ib + 8u) & 63u));
+                const f32* v2 = P.ss + (ss + (ld32(P.inds, ib + 16u) & 63u));
+                sx0 = v0[0]; sy0 = v0[64]; iw0 = v0[128];
+                sx1 = v1[0]; sy1 = v1[64]; iw1 = v1[128];
+                sx2 = v2[0]; sy2 = v2[64]; iw2 = v2[128];
                 key = (wi << 9) | (b << 5) | (16u + (u32)g);
             }
             if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy);
