@@ -108,12 +108,12 @@ struct FrameSet {
     hipEvent_t ev[10] = {};
     bool in_flight = false;
 
-    DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl;
-    DBuf tile_off, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth;
+    DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl, bin_m;
+    DBuf tile_off, tile_count, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth;
     DBuf proxy_vis, light_proxy_count, light_proxies, tile_z, vis64, scan_sums, raster_items, tile_nch, merge;
     Packer pack;                    /* per-batch tables, staged in pinned memory */
     DBuf pack_d;
-    DBuf zero_d;                    /* counters, tile counts and fill cursors, chunk completions */
+    DBuf zero_d;                    /* counters, tile counts, chunk completions */
     u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
     u8* out_h = nullptr;            /* pinned staging of host outputs (colour, 64-bit visibility) */
     size_t out_h_cap = 0;
@@ -133,11 +133,12 @@ struct FrameSet {
     };
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
+    u64 bin_m_words = 0;
     u64 vid_total = 0, color_total = 0;
     size_t zero_words = 0;
     bool any_vis64 = false, cap_vis = false, host_color = false;
     size_t o_batch = 0, o_views = 0, o_tv = 0, o_st = 0, o_insts = 0, o_work = 0, o_bones = 0, o_pl = 0, o_dl = 0;
-    size_t o_mt = 0, o_mc = 0, o_ms = 0, o_tt = 0, o_tc = 0, o_ts = 0, o_mr = 0, o_px = 0, o_tlas = 0;
+    size_t o_mt = 0, o_mc = 0, o_ms = 0, o_tt = 0, o_tc = 0, o_ts = 0, o_mr = 0, o_px = 0, o_tlas = 0, o_seg = 0;
     size_t o_tpos = 0, o_tattr = 0, o_tinds = 0, o_tcull = 0, o_tsm = 0, o_tskin = 0;
     u32 counters[CNT_COUNT] = {};
 };
@@ -206,7 +207,7 @@ void backend_destroy(Backend* b) {
         free_buf(*d);
     }
     for (FrameSet& fs : b->sets) {
-        for (DBuf* d : { &fs.ss, &fs.ss_vis, &fs.clip_count, &fs.clip_base, &fs.clip_mask, &fs.batch_pl, &fs.tile_off,
+        for (DBuf* d : { &fs.ss, &fs.ss_vis, &fs.clip_count, &fs.clip_base, &fs.clip_mask, &fs.batch_pl, &fs.bin_m, &fs.tile_off, &fs.tile_count,
                          &fs.tile_list, &fs.clip_rec, &fs.clip_key, &fs.clip_rect, &fs.color, &fs.vid, &fs.tile_lights,
                          &fs.cap_depth, &fs.proxy_vis, &fs.light_proxy_count, &fs.light_proxies, &fs.tile_z, &fs.vis64,
                          &fs.scan_sums, &fs.raster_items, &fs.tile_nch, &fs.merge, &fs.pack_d, &fs.zero_d }) {
@@ -542,6 +543,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     std::vector<ProxyGpu> proxies;
     std::vector<sft_bvh_node> tlas;
     std::vector<u32> tile_view, shade_tiles, shade_tiles_ms;
+    std::vector<BinSeg> segs;
+    u64 bin_m_words = 0;
     fs.vgpu.assign(nv, ViewGpu{});
     fs.plan.assign(nv, FrameSet::ViewPlan{});
     u32 ss_total = 0, max_samples = 1, occ_total = 0;
@@ -698,12 +701,32 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.tlas_base = vp.tlas_base;
         g.occ_base = vp.occ_base;
         g.occ_count = vp.occ_count;
-        g.pad0 = 0;
+        /* binning segments: the view's batch slots in runs of kSegItems,
+           then its clip slots; their count rows are consecutive */
+        const u32 n_items = vp.n_work * 16u;
+        const u32 n_bseg = (n_items + kSegItems - 1u) / kSegItems;
+        g.seg_base = (u32)segs.size();
+        g.n_segs = n_bseg + 1u;
+        for (u32 k = 0; k <= n_bseg; k += 1) {
+            BinSeg sg;
+            sg.view = vi;
+            sg.first = vp.work_base * 16u + k * kSegItems;
+            sg.count = (k < n_bseg) ? std::min(kSegItems, n_items - k * kSegItems) : kInvalid;
+            sg.m_base = (u32)(bin_m_words + (u64)k * vp.n_tiles);
+            segs.push_back(sg);
+        }
+        bin_m_words += (u64)(n_bseg + 1u) * vp.n_tiles;
+        g.pad0 = g.pad1 = g.pad2 = 0;
         g.vid_base = vp.vid_base;
         g.color_base = vp.color_base;
     }
     if (work.size() >= (1u << 23)) {
         fprintf(stderr, "fatal sofren2 error: %zu meshlets in one batch exceed the 2^23 emission keys\n", work.size());
+        exit(1);
+    }
+    if (bin_m_words >= (1ull << 32)) {
+        fprintf(stderr, "fatal sofren2 error: binning matrix of %llu words exceeds 32-bit offsets\n",
+                (unsigned long long)bin_m_words);
         exit(1);
     }
     const u32 nt = (u32)tile_view.size();
@@ -729,6 +752,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.vis64 = any_vis64 ? 1u : 0u;
     bg.raster_chunk = b->raster_chunk;
     bg.raster_groups = b->raster_groups;
+    bg.n_segs = (u32)segs.size();
+    fs.bin_m_words = bin_m_words;
 
     /* ---- staging ---- */
     Packer& pk = fs.pack;
@@ -751,6 +776,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.o_mr = pk.add(b->mesh_recs);
     fs.o_px = pk.add(proxies);
     fs.o_tlas = pk.add(tlas);
+    fs.o_seg = pk.add(segs);
     /* transient rows, filled in place once every section is reserved */
     fs.o_tpos = pk.reserve((size_t)fs.t_slots * 768);
     fs.o_tattr = pk.reserve((size_t)fs.t_slots * 1792);
@@ -777,7 +803,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.any_vis64 = any_vis64;
     fs.host_color = host_color;
     fs.cap_vis = 1 == nv && 0 != (views[0].capture_flags & SFTX_CAPTURE_VISIBILITY);
-    fs.zero_words = COUNTER_WORDS + 2 * (size_t)nt + (size_t)nt * max_samples;
+    fs.zero_words = COUNTER_WORDS + (size_t)nt * max_samples;
 }
 
 /* Device side of a prepared batch: buffers, one H2D copy of the tables,
@@ -792,7 +818,9 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     dgrow(fs.clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
     dgrow(fs.clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
     dgrow(fs.batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
+    dgrow(fs.bin_m, std::max<size_t>(fs.bin_m_words * 4, 4), false, s);
     dgrow(fs.tile_off, (size_t)nt * 4, false, s);
+    dgrow(fs.tile_count, (size_t)nt * 4, false, s);
     dgrow(fs.scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4, false, s);
     dgrow(fs.zero_d, fs.zero_words * 4, false, s);
     dgrow(fs.color, std::max<size_t>(fs.color_total * 4, 4), false, s);
@@ -886,9 +914,10 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     p.clip_base = (u32*)fs.clip_base.p;
     p.clip_mask = (u64*)fs.clip_mask.p;
     p.batch_pl = (uint2*)fs.batch_pl.p;
-    p.tile_count = zd + COUNTER_WORDS;
+    p.segs = (const BinSeg*)(pd + fs.o_seg);
+    p.bin_m = (u32*)fs.bin_m.p;
+    p.tile_count = (u32*)fs.tile_count.p;
     p.tile_off = (u32*)fs.tile_off.p;
-    p.tile_fill = zd + COUNTER_WORDS + nt;
     p.tile_list = (u64*)fs.tile_list.p;
     p.clip_rec = (f32*)fs.clip_rec.p;
     p.clip_key = (u32*)fs.clip_key.p;
@@ -903,7 +932,7 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     p.scan_sums = (u32*)fs.scan_sums.p;
     p.raster_items = (uint2*)fs.raster_items.p;
     p.tile_nch = (u32*)fs.tile_nch.p;
-    p.tile_done = zd + COUNTER_WORDS + 2 * (size_t)nt;
+    p.tile_done = zd + COUNTER_WORDS;
     p.merge = (u64*)fs.merge.p;
     p.blas = (const sft_bvh_node*)b->blas.p;
     p.tlas = (const sft_bvh_node*)(pd + fs.o_tlas);

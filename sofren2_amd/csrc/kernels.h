@@ -76,9 +76,10 @@ struct FramePtrs {
     u32* clip_base;         /* [n_work] */
     u64* clip_mask;         /* [n_work][2] */
     uint2* batch_pl;        /* [n_work][16]: x = lane mask, y = tile rect */
+    const BinSeg* segs;     /* [n_segs] binning segments */
+    u32* bin_m;             /* per segment: [n_tiles of its view] counts, then cursors */
     u32* tile_count;        /* [n_tiles] */
     u32* tile_off;          /* [n_tiles] */
-    u32* tile_fill;         /* [n_tiles] */
     u64* tile_list;         /* [tile_list_cap] */
     f32* clip_rec;          /* [clip_cap][41] */
     u32* clip_key;          /* [clip_cap] */

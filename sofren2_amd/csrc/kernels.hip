@@ -342,10 +342,12 @@ __global__ __launch_bounds__(256) void k_geom(FramePtrs P) {
 }
 
 /* ================================================================== scans
-   Exclusive scan of n u32 in two passes: k_scan_sums reduces blocks of 4096,
-   k_scan_apply adds the sums of the blocks before it and scans its own block.
-   The last block also writes the total and flags totals over `cap`. */
-constexpr u32 kScanItems = 4096;
+   Exclusive scan of n u32 in two passes: k_scan_sums reduces blocks of 16384,
+   k_scan_apply adds the sums of the blocks before it and scans its own block
+   (a single block needs no first pass).  The last block also writes the total
+   and flags totals over `cap`. */
+constexpr u32 kScanPer = 16;
+constexpr u32 kScanItems = 1024u * kScanPer;
 
 DEV u32 block_sum_1024(u32 v, u32* s_wave) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -360,9 +362,9 @@ DEV u32 block_sum_1024(u32 v, u32* s_wave) {
 
 __global__ __launch_bounds__(1024) void k_scan_sums(const u32* in, u32 n, u32* sums) {
     __shared__ u32 s_wave[16];
-    const u32 base = blockIdx.x * kScanItems + threadIdx.x * 4u;
+    const u32 base = blockIdx.x * kScanItems + threadIdx.x * kScanPer;
     u32 v = 0;
-    for (u32 k = 0; k < 4u; k += 1) v += (base + k < n) ? in[base + k] : 0u;
+    for (u32 k = 0; k < kScanPer; k += 1) v += (base + k < n) ? in[base + k] : 0u;
     const u32 t = block_sum_1024(v, s_wave);
     if (0 == threadIdx.x) sums[blockIdx.x] = t;
 }
@@ -375,9 +377,9 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const u32* in, u32* out, u3
     u32 c = 0;
     for (u32 b = (u32)tid; b < blockIdx.x; b += 1024u) c += sums[b];
     const u32 carry = block_sum_1024(c, s_wave);
-    const u32 base = blockIdx.x * kScanItems + (u32)tid * 4u;
-    u32 v[4], sum = 0;
-    for (int k = 0; k < 4; k += 1) {
+    const u32 base = blockIdx.x * kScanItems + (u32)tid * kScanPer;
+    u32 v[kScanPer], sum = 0;
+    for (u32 k = 0; k < kScanPer; k += 1) {
         v[k] = (base + (u32)k < n) ? in[base + k] : 0u;
         sum += v[k];
     }
@@ -391,8 +393,8 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const u32* in, u32* out, u3
     u32 before = 0;
     for (int k = 0; k < wave; k += 1) before += s_wave[k];
     u32 run = carry + before + (incl - sum);
-    for (int k = 0; k < 4; k += 1) {
-        if (base + (u32)k < n) out[base + k] = run;
+    for (u32 k = 0; k < kScanPer; k += 1) {
+        if (base + k < n) out[base + k] = run;
         run += v[k];
     }
     if (blockIdx.x == gridDim.x - 1u && 1023 == tid) {
@@ -564,110 +566,156 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
 /* ================================================================== binning
    Payload descriptors: bits 0-31 index (work*16+batch, or clip slot), bits
    32-39 lane mask (0 marks a clipped triangle, as in the reference payload).
-   Order inside a tile list is free (depth ties are settled by emission keys),
-   so each block aggregates its payloads per tile in LDS and touches the global
-   tile counters once per tile it covers. */
-constexpr u32 kBinChunk = 1024;          /* payload items per block */
-constexpr u32 kBinLdsTiles = 8192;       /* tiles aggregated in LDS */
+   Binning runs over segments of consecutive payload items of one view
+   (BinSeg) without global atomics on the tile counters:
+     k_bin_count  each segment counts its payloads per tile of its view in LDS
+                  and writes the counts to its row of the binning matrix (the
+                  rows of a view's segments are consecutive, n_tiles apart);
+     k_bin_scan   one block per view sums the rows per tile, scans the totals
+                  into tile offsets, reserves the view's range of the tile list
+                  and turns every row into that segment's start per tile;
+     k_bin        each segment writes its descriptors from those starts.
+   Order inside a tile list is free (depth ties are settled by emission keys). */
+constexpr u32 kBinThreads = 1024;
+constexpr u32 kBinPer = kSegItems / kBinThreads;   /* batch slots per thread */
+constexpr u32 kBinLdsTiles = 8192;                  /* larger views count in HBM */
 
-/* payload item i: a batch slot (work item * 16 + batch) or a clip slot after
-   them; returns its tile rectangle, descriptor and view */
-DEV bool bin_item(const FramePtrs& P, u32 nb, u32 i, u32& rect, u64& desc, u32& view) {
-    if (i < nb) {
-        const uint2 pl = P.batch_pl[i];
-        if (0u == pl.x) return false;
-        rect = pl.y;
-        desc = ((u64)pl.x << 32) | (u64)i;
-        view = P.work[i >> 4].view;
-        return true;
-    }
-    const u32 slot = i - nb;
-    rect = P.clip_rect[slot];
-    if (kInvalid == rect) return false;
-    desc = (u64)slot;
-    view = P.work[P.clip_key[slot] >> 9].view;
-    return true;
-}
-
-/* views of a block's items are contiguous: batch slots and clip slots both
-   follow work order */
-DEV u32 item_view(const FramePtrs& P, u32 nb, u32 i) {
-    return (i < nb) ? P.work[i >> 4].view : P.work[P.clip_key[i - nb] >> 9].view;
-}
-
-/* scatter == false: count payloads per tile; scatter == true: write them */
-template <bool scatter>
-__global__ __launch_bounds__(256) void k_bin(FramePtrs P) {
+/* the clip slots of a view: the clip scan runs in work order */
+DEV void view_clip_range(const FramePtrs& P, const ViewGpu& V, u32& first, u32& end) {
     const BatchGpu& B = *P.batch;
+    const u32 total = min(P.counters[CNT_CLIP_TRIS], B.clip_cap);
+    const u32 we = V.work_base + V.n_work;
+    first = (V.n_work > 0u) ? min(P.clip_base[V.work_base], total) : 0u;
+    end = (we < B.n_work) ? min(P.clip_base[we], total) : total;
+    if (end < first) end = first;
+}
+
+/* calls f(tile index inside the view, descriptor) for every tile of every
+   payload of segment `sg`, items spread over the block's threads */
+template <class F>
+DEV void for_seg_payloads(const FramePtrs& P, const BinSeg& sg, const ViewGpu& V, F&& f) {
+    const i32 cols = V.fc.tile_cols;
+    auto tiles = [&](u32 rect, u64 desc) {
+        const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
+        for (i32 ty = y0; ty <= y1; ty += 1) {
+            for (i32 tx = x0; tx <= x1; tx += 1) f((u32)(ty * cols + tx), desc);
+        }
+    };
+    if (kInvalid != sg.count) {
+        uint2 pl[kBinPer];
+        for (u32 k = 0; k < kBinPer; k += 1) {
+            const u32 j = threadIdx.x + k * kBinThreads;
+            pl[k] = (j < sg.count) ? P.batch_pl[sg.first + j] : make_uint2(0u, 0u);
+        }
+        for (u32 k = 0; k < kBinPer; k += 1) {
+            if (0u == pl[k].x) continue;
+            tiles(pl[k].y, ((u64)pl[k].x << 32) | (u64)(sg.first + threadIdx.x + k * kBinThreads));
+        }
+        return;
+    }
+    u32 c0, c1;
+    view_clip_range(P, V, c0, c1);
+    for (u32 slot = c0 + threadIdx.x; slot < c1; slot += kBinThreads) {
+        const u32 rect = P.clip_rect[slot];
+        if (kInvalid != rect) tiles(rect, (u64)slot);
+    }
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(FramePtrs P) {
     __shared__ u32 s_t[kBinLdsTiles];
-    __shared__ u32 s_range[2];
-    const u32 nb = B.n_work * 16u;
-    const u32 nclip = min(P.counters[CNT_CLIP_TRIS], B.clip_cap);
-    const u32 total = nb + nclip;
-    const u32 base = blockIdx.x * kBinChunk;
-    if (base >= total) return;
-    const u32 end = min(base + kBinChunk, total);
-    /* batch-wide tiles this block can touch: those of the views of its first
-       and last items and every view between */
-    if (0 == threadIdx.x) {
-        const ViewGpu& v0 = P.views[item_view(P, nb, base)];
-        const ViewGpu& v1 = P.views[item_view(P, nb, end - 1u)];
-        s_range[0] = v0.tile_base;
-        s_range[1] = v1.tile_base + (u32)v1.fc.n_tiles;
-    }
-    __syncthreads();
-    const u32 t_lo = s_range[0], t_n = s_range[1] - s_range[0];
-    const bool lds = t_n <= kBinLdsTiles;
-    u32* gcount = scatter ? P.tile_fill : P.tile_count;
-    const u32 cap = B.tile_list_cap;
-    if (lds) {
-        for (u32 t = threadIdx.x; t < t_n; t += 256u) s_t[t] = 0u;
+    const BinSeg sg = P.segs[blockIdx.x];
+    const ViewGpu& V = P.views[sg.view];
+    const u32 nt = (u32)V.fc.n_tiles;
+    u32* row = P.bin_m + sg.m_base;
+    if (nt > kBinLdsTiles) {
+        /* the row itself holds the counts; only this block touches it */
+        for (u32 t = threadIdx.x; t < nt; t += kBinThreads) row[t] = 0u;
         __syncthreads();
+        for_seg_payloads(P, sg, V, [&](u32 t, u64) { atomicAdd(&row[t], 1u); });
+        return;
     }
-    for (u32 i = base + threadIdx.x; i < end; i += 256u) {
-        u32 rect, view;
-        u64 desc;
-        if (!bin_item(P, nb, i, rect, desc, view)) continue;
-        const ViewGpu& V = P.views[view];
-        const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
-        for (i32 ty = y0; ty <= y1; ty += 1) {
-            for (i32 tx = x0; tx <= x1; tx += 1) {
-                const u32 t = V.tile_base + (u32)(ty * V.fc.tile_cols + tx);
-                if (lds) atomicAdd(&s_t[t - t_lo], 1u);
-                else if (!scatter) atomicAdd(&gcount[t], 1u);
-                else {
-                    const u32 pos = P.tile_off[t] + atomicAdd(&gcount[t], 1u);
-                    if (pos < cap) P.tile_list[pos] = desc;
-                }
+    for (u32 t = threadIdx.x; t < nt; t += kBinThreads) s_t[t] = 0u;
+    __syncthreads();
+    for_seg_payloads(P, sg, V, [&](u32 t, u64) { atomicAdd(&s_t[t], 1u); });
+    __syncthreads();
+    for (u32 t = threadIdx.x; t < nt; t += kBinThreads) row[t] = s_t[t];
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_scan(FramePtrs P) {
+    __shared__ u32 s_wave[16];
+    __shared__ u32 s_base;
+    const ViewGpu& V = P.views[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const u32 nt = (u32)V.fc.n_tiles;
+    const u32 per = (nt + kBinThreads - 1u) / kBinThreads;
+    const u32 t0 = min((u32)tid * per, nt), t1 = min(t0 + per, nt);
+    const u32 ns = V.n_segs;
+    u32* rows = P.bin_m + P.segs[V.seg_base].m_base;
+    /* totals of this thread's tiles, eight rows in flight */
+    u32 sum = 0;
+    for (u32 t = t0; t < t1; t += 1) {
+        u32 c = 0;
+        for (u32 k0 = 0; k0 < ns; k0 += 8u) {
+            u32 v[8];
+            for (u32 j = 0; j < 8u; j += 1) v[j] = (k0 + j < ns) ? rows[(k0 + j) * nt + t] : 0u;
+            for (u32 j = 0; j < 8u; j += 1) c += v[j];
+        }
+        P.tile_count[V.tile_base + t] = c;
+        sum += c;
+    }
+    u32 incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (63 == lane) s_wave[wave] = incl;
+    __syncthreads();
+    u32 before = 0, total = 0;
+    for (int k = 0; k < 16; k += 1) {
+        before += (k < wave) ? s_wave[k] : 0u;
+        total += s_wave[k];
+    }
+    if (0 == tid) {
+        /* the view's range of the batch's tile list */
+        const u32 base = atomicAdd(&P.counters[CNT_TILE_PAIRS], total);
+        if (base + total > P.batch->tile_list_cap || base + total < base) atomicOr(&P.counters[CNT_OVERFLOW], 2u);
+        s_base = base;
+    }
+    __syncthreads();
+    u32 run = s_base + before + (incl - sum);
+    for (u32 t = t0; t < t1; t += 1) {
+        P.tile_off[V.tile_base + t] = run;
+        for (u32 k0 = 0; k0 < ns; k0 += 8u) {
+            u32 v[8];
+            for (u32 j = 0; j < 8u; j += 1) v[j] = (k0 + j < ns) ? rows[(k0 + j) * nt + t] : 0u;
+            for (u32 j = 0; j < 8u; j += 1) {
+                if (k0 + j < ns) rows[(k0 + j) * nt + t] = run;
+                run += v[j];
             }
         }
     }
-    if (!lds) return;
-    __syncthreads();
-    /* one global reservation per touched tile */
-    for (u32 t = threadIdx.x; t < t_n; t += 256u) {
-        const u32 c = s_t[t];
-        if (c) {
-            const u32 at = atomicAdd(&gcount[t_lo + t], c);
-            if (scatter) s_t[t] = P.tile_off[t_lo + t] + at;
-        }
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_bin(FramePtrs P) {
+    __shared__ u32 s_t[kBinLdsTiles];
+    const BinSeg sg = P.segs[blockIdx.x];
+    const ViewGpu& V = P.views[sg.view];
+    const u32 nt = (u32)V.fc.n_tiles;
+    u32* row = P.bin_m + sg.m_base;
+    const u32 cap = P.batch->tile_list_cap;
+    if (nt > kBinLdsTiles) {
+        for_seg_payloads(P, sg, V, [&](u32 t, u64 desc) {
+            const u32 pos = atomicAdd(&row[t], 1u);
+            if (pos < cap) P.tile_list[pos] = desc;
+        });
+        return;
     }
-    if (!scatter) return;
+    for (u32 t = threadIdx.x; t < nt; t += kBinThreads) s_t[t] = row[t];
     __syncthreads();
-    for (u32 i = base + threadIdx.x; i < end; i += 256u) {
-        u32 rect, view;
-        u64 desc;
-        if (!bin_item(P, nb, i, rect, desc, view)) continue;
-        const ViewGpu& V = P.views[view];
-        const i32 x0 = rect & 0xFF, x1 = (rect >> 8) & 0xFF, y0 = (rect >> 16) & 0xFF, y1 = rect >> 24;
-        for (i32 ty = y0; ty <= y1; ty += 1) {
-            for (i32 tx = x0; tx <= x1; tx += 1) {
-                const u32 t = V.tile_base + (u32)(ty * V.fc.tile_cols + tx);
-                const u32 pos = atomicAdd(&s_t[t - t_lo], 1u);
-                if (pos < cap) P.tile_list[pos] = desc;
-            }
-        }
-    }
+    for_seg_payloads(P, sg, V, [&](u32 t, u64 desc) {
+        const u32 pos = atomicAdd(&s_t[t], 1u);
+        if (pos < cap) P.tile_list[pos] = desc;
+    });
 }
 
 /* ================================================================== k_tile */
@@ -2077,7 +2125,7 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
     auto scan = [&](const u32* in, u32* out, u32 n, u32* total, u32 cap, u32 bit) {
         const u32 blocks = (n + kScanItems - 1u) / kScanItems;
         if (0u == blocks) return;
-        hipLaunchKernelGGL(k_scan_sums, dim3(blocks), dim3(1024), 0, s, in, n, p.scan_sums);
+        if (blocks > 1u) hipLaunchKernelGGL(k_scan_sums, dim3(blocks), dim3(1024), 0, s, in, n, p.scan_sums);
         hipLaunchKernelGGL(k_scan_apply, dim3(blocks), dim3(1024), 0, s, in, out, n, (const u32*)p.scan_sums, total, cap,
                            p.counters + CNT_OVERFLOW, bit);
     };
@@ -2093,14 +2141,10 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
         scan(p.clip_count, p.clip_base, nw, p.counters + CNT_CLIP_TRIS, B.clip_cap, 1u);
     }
     mark(2);
-    /* upper bound of payload items: batch slots plus every clip slot */
-    const u32 bin_blocks = (nw * 16u + B.clip_cap + kBinChunk - 1u) / kBinChunk;
-    if (nw > 0) {
-        hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_bin<false>, dim3(bin_blocks), dim3(256), 0, s, p);
-    }
-    scan(p.tile_count, p.tile_off, B.n_tiles, p.counters + CNT_TILE_PAIRS, B.tile_list_cap, 2u);
-    if (nw > 0) hipLaunchKernelGGL(k_bin<true>, dim3(bin_blocks), dim3(256), 0, s, p);
+    if (nw > 0) hipLaunchKernelGGL(k_clip, dim3(geom_blocks), dim3(256), 0, s, p);
+    if (B.n_segs) hipLaunchKernelGGL(k_bin_count, dim3(B.n_segs), dim3(kBinThreads), 0, s, p);
+    if (B.n_views) hipLaunchKernelGGL(k_bin_scan, dim3(B.n_views), dim3(kBinThreads), 0, s, p);
+    if (B.n_segs) hipLaunchKernelGGL(k_bin, dim3(B.n_segs), dim3(kBinThreads), 0, s, p);
     mark(3);
     hipLaunchKernelGGL(k_raster_plan, dim3(B.n_tiles), dim3(256), 0, s, p);
     hipLaunchKernelGGL(k_raster, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);

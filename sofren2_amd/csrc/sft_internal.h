@@ -57,7 +57,9 @@ struct ViewGpu {
     FrameConst fc;
     u32 inst_base, work_base, n_work, pl_base;
     u32 dl_base, tmat_base, tile_base, proxy_base;
-    u32 tlas_base, occ_base, occ_count, pad0;
+    u32 tlas_base, occ_base, occ_count, seg_base;
+    u32 n_segs;         /* binning segments of the view, its clip segment last */
+    u32 pad0, pad1, pad2;
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -67,7 +69,15 @@ struct BatchGpu {
     u32 n_views, n_work, n_tiles, max_samples;
     u32 clip_cap, tile_list_cap, rcp_bits, rsq_bits;
     u32 n_shade_tiles, n_shade_tiles_ms, occ_total, n_lights;
-    u32 vis64, raster_chunk, raster_groups, pad2;
+    u32 vis64, raster_chunk, raster_groups, n_segs;
+};
+
+/* A binning segment: up to kSegItems consecutive batch slots of one view, or
+   (count == kInvalid) the view's clip slots.  m_base locates its per-tile
+   counts (then cursors) in the binning matrix, n_tiles of the view wide. */
+constexpr u32 kSegItems = 4096;
+struct BinSeg {
+    u32 view, first, count, m_base;
 };
 
 struct InstGpu {
