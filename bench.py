@@ -64,7 +64,8 @@ def cpu_baseline(scene, name, width, height, frames):
 
 def mixed_batch_leg(lib, steps):
     """64 views over 4 scene instances at mixed resolution and MSAA: one
-    batched submission per step against the same views presented one by one."""
+    batched submission per step, the next batch recorded while the previous
+    one runs (two in flight), against the same views presented one by one."""
     from sofren2_amd.workloads import MixedBatch
     mb = MixedBatch(lib)
     try:
@@ -72,12 +73,9 @@ def mixed_batch_leg(lib, steps):
         mb.render_batched()
         mb.render_per_frame()
         lib.synchronize()
-        lat = []
+        mb.render_pipelined(2)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            a = time.perf_counter()
-            mb.render_batched()
-            lat.append((time.perf_counter() - a) * 1e3)
+        lat = [x * 1e3 for x in mb.render_pipelined(steps)]
         batched_s = (time.perf_counter() - t0) / steps
         t0 = time.perf_counter()
         per_steps = max(1, steps // 2)

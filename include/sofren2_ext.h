@@ -124,11 +124,23 @@ void sftx_set_msaa(i32 samples);
 
    vis64, when given, receives per sample (1/w bits << 32) | visibility ID
    over the view's 64-px tile grid: [samples][tile_rows*64][tile_cols*64].
-   The display size set by sftx_view_begin stays in effect after the batch. */
+   The display size set by sftx_view_begin stays in effect after the batch.
+
+   Pipelined form: sftx_batch_submit_async() in place of sftx_batch_submit
+   launches the batch and returns a ticket (> 0; 0 if nothing was submitted)
+   without waiting, so the next batch can be recorded (batch_end, batch_begin,
+   views ...) while it runs.  Two batches can be in flight; submitting a third
+   first completes the oldest.  sftx_batch_wait(ticket) completes batches in
+   submission order up to that one and returns 0 (-1 for an unknown ticket);
+   until then the batch's colour and vis64 buffers belong to the renderer.
+   sft_present, sftx_batch_submit, sftx_synchronize, sftx_server_start and
+   sft_release complete any batch still in flight first. */
 void sftx_batch_begin(void);
 void sftx_view_begin(i32 width, i32 height, i32 samples);
 i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64);
 void sftx_batch_submit(void);
+i32 sftx_batch_submit_async(void);
+i32 sftx_batch_wait(i32 ticket);
 void sftx_batch_end(void);
 
 /* ---- request server ---------------------------------------------------------

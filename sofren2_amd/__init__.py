@@ -121,6 +121,8 @@ class Library:
         self.view_end = _bind(g, "sftx_view_end", i32, [vp, i32, vp])
         self.batch_submit = _bind(g, "sftx_batch_submit", None, [])
         self.batch_end = _bind(g, "sftx_batch_end", None, [])
+        self.batch_submit_async = _bind(g, "sftx_batch_submit_async", ctypes.c_int32, [])
+        self.batch_wait = _bind(g, "sftx_batch_wait", ctypes.c_int32, [ctypes.c_int32])
         self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
 
 
@@ -163,6 +165,24 @@ def render_batch(lib, views):
     finally:
         lib.batch_end()
     return views
+
+
+def submit_batch_async(lib, views):
+    """Records every view and launches them as one batch without waiting;
+    returns the ticket for lib.batch_wait.  The views' buffers must stay
+    alive until then."""
+    lib.batch_begin()
+    try:
+        for v in views:
+            lib.view_begin(v.width, v.height, v.samples)
+            lib.scene_draw(v.scene, v.frame, v.width, v.height)
+            lib.view_end(v.color.ctypes.data, v.width * 4, v.vis.ctypes.data if v.vis is not None else None)
+        ticket = lib.batch_submit_async()
+    finally:
+        lib.batch_end()
+    if ticket <= 0:
+        raise RuntimeError("batch submission failed")
+    return ticket
 
 
 class SceneRenderer:

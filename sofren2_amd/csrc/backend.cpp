@@ -133,6 +133,7 @@ struct FrameSet {
     };
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
+    u32 t_base = 0, t_skin_base = 0;    /* this set's transient region */
     u64 bin_m_words = 0;
     u64 vid_total = 0, color_total = 0;
     size_t zero_words = 0;
@@ -147,6 +148,11 @@ struct Backend {
     /* resident pools */
     DBuf pos, attr, inds, cull, slot_mesh, skin;
     u32 resident_slots = 0, resident_skins = 0, resident_mesh_recs = 0;
+    /* pool layout: resident rows in [0, res_cap), then one transient region
+       of t_cap rows per frame set (dmesh geometry, rewritten every batch), so
+       a batch can stage its transient rows while the other set's batch runs;
+       skins alike */
+    u32 res_cap = 0, res_skin_cap = 0, t_cap = 0, t_skin_cap = 0;
     std::vector<MeshRec> mesh_recs;
     std::unordered_map<const sft_model*, ModelRes> models;
     DBuf texels, tex_meta_d;
@@ -309,6 +315,8 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
 /* grows the shared pools; no batch may be in flight (quiesce first) */
 static void grow_pools(Backend* b, size_t slots, size_t skins) {
     hipStream_t s = b->sets[0].stream;
+    slots = std::max(slots, (size_t)b->res_cap + 2 * (size_t)b->t_cap);
+    skins = std::max(skins, (size_t)b->res_skin_cap + 2 * (size_t)b->t_skin_cap);
     dgrow(b->pos, slots * 768, true, s);
     dgrow(b->attr, slots * 1792, true, s);
     dgrow(b->inds, slots * 512, true, s);
@@ -323,6 +331,10 @@ static void upload_model(Backend* b, const sft_model* m, ModelRes& r) {
     u32 n_skin = 0;
     plan_model(m, r, b->resident_slots, b->resident_mesh_recs, b->mesh_recs, b->resident_skins, &n_skin);
     const u32 T = m->total_meshlets;
+    /* the transient regions move up when the resident range grows (callers
+       hold no batch in flight here) */
+    if (b->resident_slots + T > b->res_cap) b->res_cap = std::max(b->res_cap * 2u, b->resident_slots + T);
+    if (b->resident_skins + n_skin > b->res_skin_cap) b->res_skin_cap = std::max(b->res_skin_cap * 2u, b->resident_skins + n_skin);
     grow_pools(b, (size_t)r.base + T, (size_t)b->resident_skins + n_skin);
     hipStream_t s = b->sets[0].stream;
     if (T) {
@@ -464,8 +476,18 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     const u32 nv = (u32)views.size();
     /* shared pools change only with no batch in flight */
     bool shared_change = !b->est_uploaded || texture_changes(b, c);
+    /* transient rows need no quiesce unless this set's region must grow */
+    u32 need_t = 0, need_tskin = 0;
     for (const ViewRecord& v : views) {
-        if (!v.transient_models.empty()) shared_change = true;
+        for (const sft_model* m : v.transient_models) {
+            need_t += m->total_meshlets;
+            for (u32 i = 0; i < m->mesh_count; i += 1) {
+                if (m->meshes[i].meshlet_skins) need_tskin += m->meshes[i].meshlet_count;
+            }
+        }
+    }
+    if (need_t > b->t_cap || need_tskin > b->t_skin_cap) shared_change = true;
+    for (const ViewRecord& v : views) {
         for (const sft_instance& in : v.instances) {
             auto it = b->models.find(in.model);
             if (!is_transient(v, in.model) && (it == b->models.end() || it->second.total != in.model->total_meshlets)) {
@@ -515,7 +537,15 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     /* transient models (dmesh geometry) take the slots after the resident
        ones; their rows are staged with the batch tables and copied in bulk */
     std::vector<std::unordered_map<const sft_model*, ModelRes>> tmodels(nv);
-    u32 tslot = b->resident_slots, tskin = b->resident_skins;
+    if (need_t > b->t_cap || need_tskin > b->t_skin_cap) {
+        b->t_cap = std::max(b->t_cap, std::max(need_t, b->t_cap * 2u));
+        b->t_skin_cap = std::max(b->t_skin_cap, std::max(need_tskin, b->t_skin_cap * 2u));
+        grow_pools(b, 0, 0);
+    }
+    const u32 set_index = (u32)(&fs - b->sets);
+    fs.t_base = b->res_cap + set_index * b->t_cap;
+    fs.t_skin_base = b->res_skin_cap + set_index * b->t_skin_cap;
+    u32 tslot = fs.t_base, tskin = fs.t_skin_base;
     for (u32 vi = 0; vi < nv; vi += 1) {
         for (const sft_model* m : views[vi].transient_models) {
             ModelRes r;
@@ -526,9 +556,12 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
             tmodels[vi][m] = std::move(r);
         }
     }
-    fs.t_slots = tslot - b->resident_slots;
-    fs.t_skins = tskin - b->resident_skins;
-    if (fs.t_slots) grow_pools(b, tslot, tskin);
+    fs.t_slots = tslot - fs.t_base;
+    fs.t_skins = tskin - fs.t_skin_base;
+    if (fs.t_slots > b->t_cap || fs.t_skins > b->t_skin_cap) {
+        fprintf(stderr, "fatal sofren2 error: transient rows exceed their region\n");
+        exit(1);
+    }
     if (texture_changes(b, c)) sync_textures(b, c);
     dgrow(b->texels, 4, false, s0);
 
@@ -786,7 +819,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.o_tskin = pk.reserve((size_t)fs.t_skins * 768);
     for (u32 vi = 0; vi < nv; vi += 1) {
         for (const auto& kv : tmodels[vi]) {
-            fill_model_rows(kv.first, kv.second, b->resident_slots, b->resident_skins, (f32*)(pk.host + fs.o_tpos),
+            fill_model_rows(kv.first, kv.second, fs.t_base, fs.t_skin_base, (f32*)(pk.host + fs.o_tpos),
                             (f32*)(pk.host + fs.o_tattr), pk.host + fs.o_tinds, (f32*)(pk.host + fs.o_tcull),
                             (u32*)(pk.host + fs.o_tsm), pk.host + fs.o_tskin);
         }
@@ -855,14 +888,14 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     HIP_CHECK(hipMemcpyAsync(fs.pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
     u8* pd = (u8*)fs.pack_d.p;
     if (fs.t_slots) {
-        const size_t r0 = b->resident_slots;
+        const size_t r0 = fs.t_base;
         HIP_CHECK(hipMemcpyAsync((u8*)b->pos.p + r0 * 768, pd + fs.o_tpos, (size_t)fs.t_slots * 768, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync((u8*)b->attr.p + r0 * 1792, pd + fs.o_tattr, (size_t)fs.t_slots * 1792, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync((u8*)b->inds.p + r0 * 512, pd + fs.o_tinds, (size_t)fs.t_slots * 512, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync((u8*)b->cull.p + r0 * 32, pd + fs.o_tcull, (size_t)fs.t_slots * 32, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync((u8*)b->slot_mesh.p + r0 * 4, pd + fs.o_tsm, (size_t)fs.t_slots * 4, hipMemcpyDeviceToDevice, s));
         if (fs.t_skins) {
-            HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)b->resident_skins * 768, pd + fs.o_tskin, (size_t)fs.t_skins * 768,
+            HIP_CHECK(hipMemcpyAsync((u8*)b->skin.p + (size_t)fs.t_skin_base * 768, pd + fs.o_tskin, (size_t)fs.t_skins * 768,
                                      hipMemcpyDeviceToDevice, s));
         }
     }
@@ -1168,7 +1201,11 @@ extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captur
 
 extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->sets[0].color.p; }
 
-extern "C" void sftx_synchronize(void) { quiesce(ensure_backend()); }
+extern "C" void sftx_synchronize(void) {
+    Backend* b = ensure_backend();
+    drain_async_batches(*g_ctx);
+    quiesce(b);
+}
 
 extern "C" f32 sftx_copy_calibration(u64 bytes) {
     Backend* b = ensure_backend();

@@ -358,6 +358,7 @@ extern "C" void sft_init(i32 width, i32 height, i32 requested_threads) {
 extern "C" void sft_release(void) {
     if (!g_ctx) return;
     sftx_server_release();
+    drain_async_batches(*g_ctx);
     backend_destroy(g_ctx->backend);
     munmap(g_ctx->scratch_arena.base, g_ctx->scratch_arena.max_size);
     delete g_ctx;
@@ -435,6 +436,7 @@ extern "C" void sft_present(void) {
         return;
     }
     if (!c.backend) c.backend = backend_create();   /* exits if there is no GPU */
+    drain_async_batches(c);
     std::vector<ViewRecord> one(1);
     snapshot_view(c, one[0]);
     backend_render(c.backend, &c, one);
@@ -512,7 +514,62 @@ extern "C" void sftx_batch_submit(void) {
         return;
     }
     if (!c.backend) c.backend = backend_create();
+    drain_async_batches(c);
     backend_render(c.backend, &c, c.batch_views);
+}
+
+/* The recorded views as a batch on a free frame set, without waiting: the
+   host can record the next batch while this one runs.  Two batches can be in
+   flight; a third submission completes the oldest first.  Output buffers stay
+   the caller's until sftx_batch_wait returns for the ticket. */
+extern "C" i32 sftx_batch_submit_async(void) {
+    Context& c = C();
+    if (!c.batch_open || c.batch_views.empty()) return 0;
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sftx_batch_submit_async while the request server runs; submit requests instead\n");
+        return 0;
+    }
+    if (!c.backend) c.backend = backend_create();
+    if (c.async_batches.size() >= 2) {
+        Context::AsyncBatch& old = c.async_batches.front();
+        backend_complete(c.backend, &c, old.views, old.set);
+        c.async_batches.pop_front();
+    }
+    Context::AsyncBatch ab;
+    ab.ticket = c.next_ticket;
+    c.next_ticket = (c.next_ticket == 0x7FFFFFFF) ? 1 : c.next_ticket + 1;
+    ab.set = c.async_batches.empty() ? 0 : 1 - c.async_batches.back().set;
+    ab.views.swap(c.batch_views);
+    c.async_batches.push_back(std::move(ab));
+    Context::AsyncBatch& cur = c.async_batches.back();
+    backend_submit(c.backend, &c, cur.views, cur.set);
+    return cur.ticket;
+}
+
+/* completes batches in submission order up to the ticket's; 0 when done,
+   -1 for a ticket that is not in flight */
+extern "C" i32 sftx_batch_wait(i32 ticket) {
+    if (!g_ctx) return -1;
+    Context& c = *g_ctx;
+    bool found = false;
+    for (const Context::AsyncBatch& ab : c.async_batches) found = found || ab.ticket == ticket;
+    if (!found) return -1;
+    while (!c.async_batches.empty()) {
+        Context::AsyncBatch& ab = c.async_batches.front();
+        const i32 t = ab.ticket;
+        backend_complete(c.backend, &c, ab.views, ab.set);
+        c.async_batches.pop_front();
+        if (t == ticket) break;
+    }
+    return 0;
+}
+
+void sft::drain_async_batches(Context& c) {
+    while (!c.async_batches.empty()) {
+        Context::AsyncBatch& ab = c.async_batches.front();
+        backend_complete(c.backend, &c, ab.views, ab.set);
+        c.async_batches.pop_front();
+    }
 }
 
 extern "C" void sftx_batch_end(void) {

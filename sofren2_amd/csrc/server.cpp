@@ -268,6 +268,7 @@ extern "C" i32 sftx_server_start(u64 hbm_budget_bytes, i32 max_views_per_batch) 
     Server* sv = g_server;
     if (sv->running) return -1;
     if (!c.backend) c.backend = backend_create();
+    drain_async_batches(c);
     sv->budget = hbm_budget_bytes;
     sv->max_views = std::max(1, max_views_per_batch);
     memset(&sv->stats, 0, sizeof(sv->stats));

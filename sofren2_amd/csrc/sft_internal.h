@@ -21,6 +21,7 @@
  */
 #pragma once
 #include <stdint.h>
+#include <deque>
 #include <vector>
 #include <unordered_map>
 #include "../../include/sofren2.h"
@@ -212,6 +213,8 @@ void backend_destroy(Backend* b);
 /* renders a batch of views (one pass of kernels over all of them) and writes
    each view's outputs; blocks until they are in host memory */
 void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
+/* completes every batch of sftx_batch_submit_async still in flight */
+void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be
    in flight; complete blocks until its batch is done */
 void backend_submit(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
@@ -289,6 +292,15 @@ struct Context {
     bool batch_open = false;
     bool server_running = false;    /* the request server owns the backend */
     std::vector<ViewRecord> batch_views;
+    /* batches submitted with sftx_batch_submit_async, oldest first (at most
+       one per frame set) */
+    struct AsyncBatch {
+        i32 ticket;
+        int set;
+        std::vector<ViewRecord> views;
+    };
+    std::deque<AsyncBatch> async_batches;
+    i32 next_ticket = 1;
 
     EstimateTables est;
     std::vector<sftx_texture_desc> export_tex;

@@ -52,6 +52,30 @@ class MixedBatch:
     def render_batched(self):
         self.S.render_batch(self.lib, self.views)
 
+    def render_pipelined(self, steps):
+        """`steps` batches with two in flight: batch k + 1 is recorded and
+        launched while batch k runs; outputs alternate between two buffer
+        sets.  Returns each batch's latency (record start to completion, s)."""
+        import time
+        S = self.S
+        if not hasattr(self, "views_b"):
+            self.views_b = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=False) for v in self.views]
+        sets = [self.views, self.views_b]
+        pending = []
+        lat = []
+        for k in range(steps):
+            t0 = time.perf_counter()
+            ticket = S.submit_batch_async(self.lib, sets[k & 1])
+            pending.append((ticket, t0))
+            if len(pending) == 2:
+                t, ts = pending.pop(0)
+                self.lib.batch_wait(t)
+                lat.append(time.perf_counter() - ts)
+        for t, ts in pending:
+            self.lib.batch_wait(t)
+            lat.append(time.perf_counter() - ts)
+        return lat
+
     def render_per_frame(self):
         lib = self.lib
         for v in self.views:

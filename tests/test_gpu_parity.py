@@ -140,6 +140,44 @@ def test_batched_views_match_reference():
         lib.sft_release()
 
 
+def test_pipelined_batches_match_blocking_batches():
+    """Batches submitted without waiting (two in flight, a third forcing the
+    oldest to complete, transient dmesh rows in both frame sets) give the same
+    colour and visibility as blocking submissions of the same views."""
+    lib = S.library()
+    lib.sft_init(640, 360, 0)
+    try:
+        handles = {}
+        for scene, seed, w, h, frame, samples in BATCH_VIEWS:
+            if (scene, seed) not in handles:
+                handles[(scene, seed)] = lib.scene_new(scene, seed, w, h)
+                assert handles[(scene, seed)] >= 0
+
+        def views_for(shift):
+            return [S.View(handles[(sc, sd)], (fr + shift) % 120, w + 16 * shift, h, ns)
+                    for sc, sd, w, h, fr, ns in BATCH_VIEWS]
+
+        blocking = {}
+        for shift in (0, 1):
+            vs = views_for(shift)
+            S.render_batch(lib, vs)
+            blocking[shift] = [(v.color.copy(), v.vis.copy()) for v in vs]
+        order = [0, 1, 0]
+        sets = [views_for(k) for k in order]
+        tickets = [S.submit_batch_async(lib, vs) for vs in sets]
+        print("tickets", tickets, flush=True)
+        assert lib.batch_wait(tickets[1]) == 0
+        assert lib.batch_wait(tickets[2]) == 0
+        assert lib.batch_wait(tickets[0]) == -1  # already completed
+        for k, vs in zip(order, sets):
+            for (c, vis), v in zip(blocking[k], vs):
+                np.testing.assert_array_equal(v.color, c)
+                np.testing.assert_array_equal(v.vis, vis)
+    finally:
+        lib.scene_destroy()
+        lib.sft_release()
+
+
 @pytest.mark.parametrize("scene", [S.SCENE_BOX_ROOM, S.SCENE_CROWD_SHADOWS])
 def test_graph_replays_match_eager_launches(scene):
     """Frames replayed from the captured HIP graph equal frames launched
