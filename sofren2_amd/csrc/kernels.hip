@@ -1,23 +1,40 @@
 /*
  * kernels.hip — the per-frame render path on gfx950.
  *
- * Pipeline of one sft_present (one HIP stream):
- *   k_geom     one wave64 per drawn meshlet, lane = vertex: frustum + normal-cone
- *              cull, 4-joint skinning, world transform, projection into the
- *              screen-space cache, then lane = triangle (two passes of 64):
- *              near-plane classification, area cull, 64-px tile bounds and the
- *              8-triangle batch payloads (sofren2.c:4901-5145, 4775-4884).
- *   k_scan     exclusive scan of clipped-triangle counts in emission order, so
- *              clip slots are numbered exactly as the reference's single worker
- *              numbers them (sofren2.c:4351-4361).
- *   k_clip     one wave per meshlet with straddling triangles: near-plane
- *              clipping into slot-numbered records (sofren2.c:4245-4381,
- *              4691-4772).
- *   k_scan     exclusive scan of per-tile payload counts.
- *   k_scatter  writes payload descriptors into per-tile lists.
- *   k_tile     one workgroup per 64x64 tile: rasterises the tile's payloads into
- *              an LDS visibility buffer, then resolves and shades it
- *              (sofren2.c:6756-6813, 5148-5499, 5809-6753).
+ * Pipeline of one batch of views (one sft_present is a batch of one), all
+ * kernels on the batch's HIP stream, every view's rows back to back:
+ *   k_zero        clears counters and chunk completion counts (a kernel, so
+ *                 the batch can be replayed from a HIP graph).
+ *   k_occlude     proxy-occlusion packets: one wave per (light, 8-proxy packet,
+ *                 ray), frustum-culled TLAS/BLAS traversal, any-hit OR
+ *                 (sofren2.c:8163-8280, 3840-4243).
+ *   k_proxy_cache per light: the proxies with visibility left and their cached
+ *                 capsule constants (sofren2.c:5936-5990).
+ *   k_geom        one wave64 per drawn meshlet, lane = vertex: frustum + normal-
+ *                 cone cull, 4-joint skinning, world transform, projection into
+ *                 the screen-space cache; then lane = triangle: near-plane
+ *                 classification, area cull, tile bounds and the 8-triangle
+ *                 batch payloads (sofren2.c:4901-5145, 4775-4884).
+ *   k_scan_*      exclusive scan of clipped-triangle counts in emission order, so
+ *                 clip slots are numbered as the reference's single worker
+ *                 numbers them (sofren2.c:4351-4361).
+ *   k_clip        one wave per meshlet with straddling triangles: near-plane
+ *                 clipping into slot-numbered records (sofren2.c:4245-4381,
+ *                 4691-4772).
+ *   k_bin_count, k_bin_scan, k_bin
+ *                 per-tile payload lists without global tile-counter atomics.
+ *   k_raster_plan splits long tile lists into chunks of visibility work.
+ *   k_raster      persistent workgroups over (tile, chunk, sample) items:
+ *                 rasterise into an LDS 64-bit visibility plane, merge split
+ *                 tiles in HBM, the last chunk publishes IDs, the tile's 1/w
+ *                 range and its light list (sofren2.c:5148-5499, 6756-6813).
+ *   k_tile_lights light lists of MSAA tiles over every sample plane.
+ *   k_shade       32x8 pixels per workgroup, lane = pixel in the reference's
+ *                 4x2 block order: interpolation, LOD, texturing, lighting,
+ *                 capsule shadows, ARGB pack (sofren2.c:5809-6753).
+ *   k_shade_ms    the same over sample planes: per block, triangles in first
+ *                 (sample, lane) order, weighted accumulate.
+ *   k_stats       folds the spread statistics counters.
  *
  * Bit-exactness: every float operation of the reference is replayed in the
  * same order with explicit fmaf where the reference uses an FMA intrinsic, the
