@@ -29,6 +29,48 @@ METRIC = "1080p frames/s & Mpix/s; p50/p99 request latency, served vs per-frame 
 HBM_PEAK_GBS = 8000.0
 SCENES = {"plane_spheres": 0, "box_room": 1, "crowd_shadows": 2}
 KERNEL_OF_PASS = {"geometry": "k_geom", "binning": "k_clip+k_bin", "visibility": "k_raster", "shading": "k_shade"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_{workload}.json")
+
+
+def summarize_pmc(fetch_csv, write_csv, copy_bytes, workload):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the factor
+    that makes the streaming copy kernel over `copy_bytes` read and write
+    exactly copy_bytes."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        for row in csv.DictReader(open(path)):
+            if row["Counter_Name"] != counter:
+                continue
+            name = row["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            acc.setdefault(name, []).append(float(row["Counter_Value"]) * 1024.0)
+        return {k: (float(np.mean(v)), len(v)) for k, v in acc.items()}
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    f_factor = copy_bytes / fetch["sft::k_copy"][0]
+    w_factor = copy_bytes / write["sft::k_copy"][0]
+    kernels = {}
+    for k in sorted(set(fetch) & set(write)):
+        if not k.startswith("sft::"):
+            continue
+        kernels[k.replace("sft::", "")] = {
+            "dispatches": fetch[k][1], "fetch_bytes": round(fetch[k][0] * f_factor),
+            "write_bytes": round(write[k][0] * w_factor),
+            "hbm_bytes": round(fetch[k][0] * f_factor + write[k][0] * w_factor)}
+    return {"workload": workload, "copy_calibration_bytes": copy_bytes, "fetch_factor": round(f_factor, 4),
+            "write_factor": round(w_factor, 4), "kernels": kernels}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of this
+    workload, or None."""
+    path = TRAFFIC_FILE.format(workload=workload)
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path)).get("kernels", {}).get(kernel)
+    return int(k["hbm_bytes"]) if k else None
 
 
 def algorithmic_bytes(st):
@@ -168,7 +210,17 @@ def main():
     ap.add_argument("--served-scenes", type=int, default=32)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run the streaming copy kernel over this many MiB first (HBM counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) of a "
+                         "run with --copy-calibration-mb into per-kernel HBM bytes (JSON on stdout) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        if args.copy_calibration_mb <= 0:
+            raise SystemExit("--pmc-summary needs the --copy-calibration-mb of the profiled run")
+        workload = "{}_{}x{}".format(args.scene, *{0: (640, 360), 1: (1920, 1080), 2: (1920, 1080)}[SCENES[args.scene]])
+        print(json.dumps(summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_calibration_mb << 20,
+                                       workload), indent=1))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
               "(views and requests would shard across GPUs with no exchange).", file=sys.stderr)
@@ -279,7 +331,8 @@ def main():
         "occlusion_mrays_s": (round(counters_sum["occlusion_rays"] / args.steps / (float(np.mean(shadow_ms)) * 1e3), 2)
                               if counters_sum["occlusion_rays"] else None),
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PASS[dominant], "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(f"{args.scene}_{w}x{h}", KERNEL_OF_PASS[dominant]),
                      "algorithmic_bytes_per_launch": int(dom_bytes)},
         "pass_roofline_gbs": {k: round(v[0] / (v[1] * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
         "counters_per_frame": counters_avg,
