@@ -783,7 +783,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.occ_total = occ_total;
     bg.n_lights = n_lights_cache;
     bg.vis64 = any_vis64 ? 1u : 0u;
-    bg.raster_chunk = b->raster_chunk;
+    bg.raster_chunk = std::max(1u, std::min(b->raster_chunk, kRasterMaxChunk));
     bg.raster_groups = b->raster_groups;
     bg.n_segs = (u32)segs.size();
     fs.bin_m_words = bin_m_words;

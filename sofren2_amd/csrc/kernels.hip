@@ -745,15 +745,19 @@ struct Tri {
 };
 
 /* rasterises one triangle of the tile into one sample plane, lane j = x offset
-   inside an 8-pixel chunk, the triangle's 8 lanes at group `g` of the wave
+   inside an 8-pixel chunk, 8 lanes at group `g` of the wave
    (sofren2.c:5213-5497).  With MSAA (`aa`) the sample sits at (ox, oy) from
    the pixel centre: edge and depth values gain A*ox + B*oy, trivial accept
    needs a margin of 0.375 (|A| + |B|), and a chunk whose 8 pixel centres are
-   all beyond that margin of an edge is skipped. */
+   all beyond that margin of an edge is skipped.
+   Several groups can share a triangle: the group sweeps the chunks of rows
+   min_y + r, min_y + r + m, ... (m a power of two) but steps the row values
+   through every row, so each row's edge and depth values are the same
+   sequence of additions as in the reference. */
 template <bool aa>
 DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u32 key,
                     f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2,
-                    f32 ox, f32 oy) {
+                    f32 ox, f32 oy, int m, int r) {
     const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
     const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
     const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
@@ -804,13 +808,16 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
     const f32 dz_chunk = dzdx * 8.f;
     f32 z_row = sz + dzdx * fj;
     const int row0 = (min_y - tmy) * 64 + (amin - tmx) + j;
+    const i32 rmask = m - 1;
     if (trivial) {
         for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
-            f32 z = z_row;
-            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                const f32 zs = aa ? z + dzs : z;
-                if (zs > 0.f) atomicMax((unsigned long long*)&vis[p], ((u64)f2u(zs) << 32) | nkey);
-                z = z + dz_chunk;
+            if (((y - min_y) & rmask) == r) {
+                f32 z = z_row;
+                for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
+                    const f32 zs = aa ? z + dzs : z;
+                    atomicMax((unsigned long long*)&vis[p], (zs > 0.f) ? (((u64)f2u(zs) << 32) | nkey) : 0ull);
+                    z = z + dz_chunk;
+                }
             }
             z_row = z_row + dzdy;
         }
@@ -820,11 +827,12 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
         f32 e0r = se0 + A0 * fj, e1r = se1 + A1 * fj, e2r = se2 + A2 * fj;
         for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
             f32 e0 = e0r, e1 = e1r, e2 = e2r, z = z_row;
-            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
+            const i32 xend = (((y - min_y) & rmask) == r) ? amax : amin;
+            for (i32 x = amin, p = row; x < xend; x += 8, p += 8) {
                 if constexpr (!aa) {
-                    if (e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f) {
-                        atomicMax((unsigned long long*)&vis[p], ((u64)f2u(z) << 32) | nkey);
-                    }
+                    /* a miss adds 0 to the max: no branch in the chunk loop */
+                    const bool hit = e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f;
+                    atomicMax((unsigned long long*)&vis[p], hit ? (((u64)f2u(z) << 32) | nkey) : 0ull);
                 }
                 else {
                     /* the triangle's 8 lanes run the same rows and chunks */
@@ -1356,18 +1364,39 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
     const i32 gw = fc.tile_cols * 64;
     const u64 plane = V.vid_base + (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
     f32 zmin = 1e9f, zmax = -1e9f;
-    for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
-        const u64 pk = get(i);
-        const f32 z = u2f((u32)(pk >> 32));
+    /* all of a thread's pixels first, then their work-item loads together,
+       so the thread waits on one round of loads rather than one per pixel */
+    constexpr int kPer = SFT_TILE_PIXELS / kRasterThreads;
+    u64 pk[kPer];
+    uint2 ig[kPer];
+    for (int k = 0; k < kPer; k += 1) pk[k] = get(tid + k * kRasterThreads);
+    for (int k = 0; k < kPer; k += 1) {
+        const u32 key = ~(u32)pk[k];
+        ig[k] = make_uint2(0u, 0u);
+        if (0ull != pk[k] && (key & 31u) >= 16u) {
+            const WorkItem& w = P.work[key >> 9];
+            ig[k] = make_uint2(w.inst, w.gm);
+        }
+    }
+    for (int k = 0; k < kPer; k += 1) {
+        const int i = tid + k * kRasterThreads;
+        const f32 z = u2f((u32)(pk[k] >> 32));
         if (z > 0.0001f) {
             zmin = fminf(zmin, z);
             zmax = fmaxf(zmax, z);
         }
         const u64 gi = plane + (u64)(tmy + (i >> 6)) * gw + (tmx + (i & 63));
-        const u32 vid = (0ull == pk) ? kInvalid : key_to_vid(P, ~(u32)pk, n_clip, clip_base);
+        const u32 key = ~(u32)pk[k];
+        u32 vid = kInvalid;
+        if (0ull != pk[k]) {
+            vid = ((key & 31u) >= 16u)
+                ? (ig[k].x << (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) | (ig[k].y << SFT_TRIANGLE_BITS) |
+                  (((key >> 5) & 15u) * 8u + ((key & 31u) - 16u))
+                : key_to_vid(P, key, n_clip, clip_base);
+        }
         P.vid[gi] = vid;
         if (P.cap_depth) P.cap_depth[gi] = z;
-        if (P.vis64) P.vis64[gi] = ((u64)(u32)(pk >> 32) << 32) | vid;
+        if (P.vis64) P.vis64[gi] = ((u64)(u32)(pk[k] >> 32) << 32) | vid;
     }
     for (int o = 32; o > 0; o >>= 1) {
         zmin = fminf(zmin, __shfl_xor(zmin, o));
@@ -1394,14 +1423,59 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
     }
 }
 
+/* screen positions, 1/w and emission key of one triangle of a payload;
+   key kInvalid marks a lane-mask hole (keys never reach it: sub <= 23) */
+struct RasterTri {
+    f32 v[9];   /* sx0 sy0 iw0 sx1 sy1 iw1 sx2 sy2 iw2 */
+    u32 key;
+};
+
+/* triangle g of the payload at tile-list position `at` */
+DEV RasterTri fetch_raster_tri(const FramePtrs& P, u32 at, u32 g) {
+    RasterTri tr;
+    tr.key = kInvalid;
+    const u64 desc = P.tile_list[at];
+    const u32 mask = (u32)(desc >> 32) & 0xFFu;
+    const u32 idx = (u32)desc;
+    if (0u == mask) {
+        if (0u != g) return tr;
+        const f32* r = P.clip_rec + (u64)idx * 41u;
+        tr.v[0] = r[0]; tr.v[3] = r[1]; tr.v[6] = r[2];
+        tr.v[1] = r[3]; tr.v[4] = r[4]; tr.v[7] = r[5];
+        tr.v[2] = r[6]; tr.v[5] = r[7]; tr.v[8] = r[8];
+        tr.key = P.clip_key[idx];
+        /* keeps these loads in this branch (merged with the meshlet loads
+           they would need a 64-bit address each) */
+        asm volatile("" ::: "memory");
+        return tr;
+    }
+    if (0u == ((mask >> g) & 1u)) return tr;
+    const u32 wi = idx >> 4, b = idx & 15u;
+    const WorkItem wk = P.work[wi];
+    const u32 ib = wk.slot * 512u + b * 32u + g;
+    const u32 ss = wk.ss * 576u;
+    for (u32 k = 0; k < 3u; k += 1) {
+        const f32* vp = P.ss + (ss + (ld32(P.inds, ib + k * 8u) & 63u));
+        tr.v[k * 3u] = vp[0];
+        tr.v[k * 3u + 1u] = vp[64];
+        tr.v[k * 3u + 2u] = vp[128];
+    }
+    tr.key = (wi << 9) | (b << 5) | (16u + g);
+    return tr;
+}
+
 __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
+    /* the work item's triangles, fetched by the whole group at once so the
+       waves rasterise from LDS instead of each waiting on its own chain of
+       descriptor, work-item, index and vertex loads */
+    __shared__ RasterTri s_tri[kRasterMaxChunk * 8];
     __shared__ f32 s_red[2][kRasterWaves];
     __shared__ u32 s_item, s_last;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Est est = make_est(P);
     const u32 n_items = P.counters[CNT_RASTER_ITEMS];
-    const u32 K = P.batch->raster_chunk, maxS = P.batch->max_samples;
+    const u32 K = min(P.batch->raster_chunk, kRasterMaxChunk), maxS = P.batch->max_samples;
     for (;;) {
         if (0 == tid) s_item = atomicAdd(&P.counters[CNT_RASTER_NEXT], 1u);
         for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
@@ -1418,44 +1492,49 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         const u32 local = tile - V.tile_base;
         const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
 
-        /* a wave per payload, 8 lanes per triangle, lane = x inside the 8-px chunk */
         u32 off;
         const u32 cnt = tile_payloads(P, tile, off);
         const u32 e0 = chunk * K, e1 = min(cnt, e0 + K);
+        const u32 n_tri = (e1 > e0) ? (e1 - e0) * 8u : 0u;
+        for (u32 q = (u32)tid; q < n_tri; q += kRasterThreads) s_tri[q] = fetch_raster_tri(P, off + e0 + (q >> 3), q & 7u);
+        __syncthreads();
+        /* a wave per payload, 8 lanes per group, lane = x inside the 8-px
+           chunk.  The payload's triangles that reach the tile share the
+           wave's 8 groups: with n of them each gets 8/n groups (rounded
+           down to a power of two) that split its rows, so one large
+           triangle is swept by the whole wave instead of by 8 lanes. */
         const int g = lane >> 3, j = lane & 7;
         for (u32 e = e0 + (u32)wave; e < e1; e += (u32)kRasterWaves) {
-            const u64 desc = P.tile_list[off + e];
-            const u32 mask = (u32)(desc >> 32) & 0xFFu;
-            const u32 idx = (u32)desc;
-            f32 sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2;
-            u32 key;
-            if (0u == mask) {
-                if (0 != g) continue;
-                const f32* r = P.clip_rec + (u64)idx * 41u;
-                sx0 = r[0]; sx1 = r[1]; sx2 = r[2];
-                sy0 = r[3]; sy1 = r[4]; sy2 = r[5];
-                iw0 = r[6]; iw1 = r[7]; iw2 = r[8];
-                key = P.clip_key[idx];
-                /* keeps these loads in this branch (merged with the meshlet
-                   loads they would need a 64-bit address each) */
-                asm volatile("" ::: "memory");
+            const RasterTri* pt = &s_tri[(e - e0) * 8u];
+            bool act = false;
+            {
+                const RasterTri& tr = pt[g];
+                if (kInvalid != tr.key) {
+                    const i32 mnx = cvtt(smax(smin(tr.v[0], smin(tr.v[3], tr.v[6])), (f32)tmx));
+                    const i32 mxx = cvtt(smin(smax(tr.v[0], smax(tr.v[3], tr.v[6])), (f32)(tmx + 63)));
+                    const i32 mny = cvtt(smax(smin(tr.v[1], smin(tr.v[4], tr.v[7])), (f32)tmy));
+                    const i32 mxy = cvtt(smin(smax(tr.v[1], smax(tr.v[4], tr.v[7])), (f32)(tmy + 63)));
+                    act = mnx <= mxx && mny <= mxy;
+                }
             }
-            else {
-                if (0u == ((mask >> g) & 1u)) continue;
-                const u32 wi = idx >> 4, b = idx & 15u;
-                const WorkItem wk = P.work[wi];
-                const u32 ib = wk.slot * 512u + b * 32u + (u32)g;
-                const u32 ss = wk.ss * 576u;
-                const f32* v0 = P.ss + (ss + (ld32(P.inds, ib) & 63u));
-                const f32* v1 = P.ss + (ss + (ld32(P.inds, ib + 8u) & 63u));
-                const f32* v2 = P.ss + (ss + (ld32(P.inds, ib + 16u) & 63u));
-                sx0 = v0[0]; sy0 = v0[64]; iw0 = v0[128];
-                sx1 = v1[0]; sy1 = v1[64]; iw1 = v1[128];
-                sx2 = v2[0]; sy2 = v2[64]; iw2 = v2[128];
-                key = (wi << 9) | (b << 5) | (16u + (u32)g);
-            }
-            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy);
-            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy);
+            const u64 ball = __ballot(act && 0 == j);
+            u32 amask = 0;
+            for (int k = 0; k < 8; k += 1) amask |= (u32)((ball >> (k * 8)) & 1ull) << k;
+            const int n = __popc(amask);
+            if (0 == n) continue;
+            const int m = (n <= 1) ? 8 : (n <= 2) ? 4 : (n <= 4) ? 2 : 1;
+            const int t = g / m, r = g & (m - 1);
+            if (t >= n) continue;
+            /* the t-th active triangle */
+            u32 bits = amask;
+            for (int k = 0; k < t; k += 1) bits &= bits - 1u;
+            const RasterTri& tr = pt[__ffs(bits) - 1];
+            const u32 key = tr.key;
+            const f32 sx0 = tr.v[0], sy0 = tr.v[1], iw0 = tr.v[2];
+            const f32 sx1 = tr.v[3], sy1 = tr.v[4], iw1 = tr.v[5];
+            const f32 sx2 = tr.v[6], sy2 = tr.v[7], iw2 = tr.v[8];
+            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, m, r);
+            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, m, r);
         }
         __syncthreads();
 

@@ -54,10 +54,14 @@ typedef struct {
     sft_mat_handle sphere_mat[SFTX_MAX_SPHERES];
     f32 sphere_r[SFTX_MAX_SPHERES], sphere_cx[SFTX_MAX_SPHERES], sphere_cz[SFTX_MAX_SPHERES];
 
+    /* box lattice, lightmap page and crowd sizes (the served kind varies them) */
+    i32 lattice, lm_size, n_chars, n_lights;
+
     u32 tri_count, meshlet_count;
 } sftx_scene_common;
 
 #define SFTX_CHARS 32
+#define SFTX_MAX_CHARS 63
 #define SFTX_BONES 10
 #define SFTX_BANDS 96
 #define SFTX_SEGS 32
@@ -74,11 +78,11 @@ typedef struct {
 
 typedef struct {
     sft_model* human;
-    sft_model* person[SFTX_CHARS];     /* shallow copies: own mesh record (material), shared meshlets */
-    sft_mat_handle mat[SFTX_CHARS];
-    sftmat xform[SFTX_CHARS];
-    f32 phase[SFTX_CHARS][SFTX_BONES][2];
-    sftmat posed[SFTX_CHARS][SFTX_BONES];
+    sft_model* person[SFTX_MAX_CHARS];     /* shallow copies: own mesh record (material), shared meshlets */
+    sft_mat_handle mat[SFTX_MAX_CHARS];
+    sftmat xform[SFTX_MAX_CHARS];
+    f32 phase[SFTX_MAX_CHARS][SFTX_BONES][2];
+    sftmat posed[SFTX_MAX_CHARS][SFTX_BONES];
     sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
 } sftx_crowd_state;
 
@@ -269,8 +273,8 @@ static void sftx_push_face(sftx_mbuild* m, i32 n, sftvec o, sftvec du, sftvec dv
             m->nx[k] = nrm.x; m->ny[k] = nrm.y; m->nz[k] = nrm.z;
             m->u[k] = (px * ua.x + py * ua.y + pz * ua.z) * 0.5f;
             m->v[k] = (px * va.x + py * va.y + pz * va.z) * 0.5f;
-            m->lu[k] = ((f32)(lx + i) + 0.5f) / (f32)SFTX_LM_SIZE;
-            m->lv[k] = ((f32)(ly + j) + 0.5f) / (f32)SFTX_LM_SIZE;
+            m->lu[k] = ((f32)(lx + i) + 0.5f) / (f32)g_scene.lm_size;
+            m->lv[k] = ((f32)(ly + j) + 0.5f) / (f32)g_scene.lm_size;
         }
     }
     for (i32 j = 0; j < n; j += 1) {
@@ -335,7 +339,7 @@ static void sftx_write_qsf(const char* path, sftx_mlist* meshes, i32 mesh_count,
     sftx_fwrite(f, "QSF2", 4);
     sftx_fwrite(f, &version, 4);
     sftx_fwrite(f, &game_data, 8);
-    const u32 tc = (u32)tex_count, ts = SFTX_TEX_SIZE, ls = SFTX_LM_SIZE, one = 1;
+    const u32 tc = (u32)tex_count, ts = SFTX_TEX_SIZE, ls = (u32)g_scene.lm_size, one = 1;
     sftx_fwrite(f, &tc, 4);
     for (i32 t = 0; t < tex_count; t += 1) {
         sftx_fwrite(f, &ts, 4);
@@ -537,8 +541,10 @@ static void sftx_box_room_create(int with_blas) {
         tex[t] = (u32*)malloc(SFTX_TEX_SIZE * SFTX_TEX_SIZE * 4);
         sftx_value_noise_texture(&rng, tex[t]);
     }
-    u32* lm = (u32*)malloc((u64)SFTX_LM_SIZE * SFTX_LM_SIZE * 4);
-    for (u32 i = 0; i < (u32)SFTX_LM_SIZE * SFTX_LM_SIZE; i += 1) {
+    const i32 L = g_scene.lattice;
+    const f32 spacing = 80.f / (f32)L, origin = spacing * (f32)(L - 1) * 0.5f;
+    u32* lm = (u32*)malloc((u64)g_scene.lm_size * g_scene.lm_size * 4);
+    for (u32 i = 0; i < (u32)g_scene.lm_size * g_scene.lm_size; i += 1) {
         const f32 l = sftx_uniform(&rng, 0.2f, 1.f);
         lm[i] = sftx_pack_rgb(l, l, l);
     }
@@ -547,12 +553,12 @@ static void sftx_box_room_create(int with_blas) {
     i32 cell = 0;   /* 4x4-texel lightmap cells, 256 per row */
     const f32 ln_lo = logf(0.1f), ln_hi = logf(4.f);
     u32 tris = 0;
-    for (i32 bz = 0; bz < SFTX_LATTICE; bz += 1) {
-        for (i32 by = 0; by < SFTX_LATTICE; by += 1) {
-            for (i32 bx = 0; bx < SFTX_LATTICE; bx += 1) {
-                const sftvec c = SFT_V((f32)bx * 5.f - 37.5f + sftx_uniform(&rng, -1.f, 1.f),
-                                       (f32)by * 5.f - 37.5f + sftx_uniform(&rng, -1.f, 1.f),
-                                       (f32)bz * 5.f - 37.5f + sftx_uniform(&rng, -1.f, 1.f));
+    for (i32 bz = 0; bz < L; bz += 1) {
+        for (i32 by = 0; by < L; by += 1) {
+            for (i32 bx = 0; bx < L; bx += 1) {
+                const sftvec c = SFT_V((f32)bx * spacing - origin + sftx_uniform(&rng, -1.f, 1.f),
+                                       (f32)by * spacing - origin + sftx_uniform(&rng, -1.f, 1.f),
+                                       (f32)bz * spacing - origin + sftx_uniform(&rng, -1.f, 1.f));
                 const sftvec h = SFT_V(0.5f * expf(sftx_uniform(&rng, ln_lo, ln_hi)),
                                        0.5f * expf(sftx_uniform(&rng, ln_lo, ln_hi)),
                                        0.5f * expf(sftx_uniform(&rng, ln_lo, ln_hi)));
@@ -804,8 +810,8 @@ static void sftx_crowd_create(void) {
     sftx_box_room_create(1);
     sftx_rng rng;
     sftx_rng_seed(&rng, g_scene.seed, 0x9E3779B9ULL);
-    g_crowd.human = sftx_build_humanoid(&g_room.arena, &rng);
-    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+    if (g_scene.n_chars > 0) g_crowd.human = sftx_build_humanoid(&g_room.arena, &rng);
+    for (i32 c = 0; c < g_scene.n_chars; c += 1) {
         /* 8 x 4 grid around the origin, jittered, rotated about y */
         const f32 x = -6.f + 12.f * (f32)(c % 8) / 7.f + sftx_uniform(&rng, -0.3f, 0.3f);
         const f32 z = -3.f + 6.f * (f32)(c / 8) / 3.f + sftx_uniform(&rng, -0.3f, 0.3f);
@@ -824,20 +830,20 @@ static void sftx_crowd_create(void) {
             g_crowd.phase[c][j][1] = sftx_uniform(&rng, 0.f, 6.2831853f);
         }
     }
-    for (i32 l = 0; l < SFTX_SHADOW_LIGHTS; l += 1) {
+    for (i32 l = 0; l < g_scene.n_lights; l += 1) {
         g_crowd.light_pos[l] = SFT_V(sftx_uniform(&rng, -15.f, 15.f), sftx_uniform(&rng, 15.f, 25.f), sftx_uniform(&rng, -15.f, 15.f));
         g_crowd.light_col[l] = SFT_V(sftx_uniform(&rng, 0.4f, 1.f), sftx_uniform(&rng, 0.4f, 1.f), sftx_uniform(&rng, 0.4f, 1.f));
     }
     sft_set_shadows_enabled(1);
-    g_scene.tri_count += SFTX_CHARS * SFTX_BANDS * SFTX_SEGS * 2;
-    g_scene.meshlet_count += SFTX_CHARS * SFTX_BANDS;
+    g_scene.tri_count += (u32)g_scene.n_chars * SFTX_BANDS * SFTX_SEGS * 2;
+    g_scene.meshlet_count += (u32)g_scene.n_chars * SFTX_BANDS;
 }
 
 /* posed = inverse_bind * global, global_j = local_j * global_(j-1)
    (row vectors, sofren2.c:9102-9104) */
 static void sftx_crowd_pose(i32 frame) {
     const f32 t = 2.f * (f32)SFT_PI * (f32)(frame % 120) / 120.f;
-    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+    for (i32 c = 0; c < g_scene.n_chars; c += 1) {
         sftmat global = sft_m_identity();
         for (i32 j = 0; j < SFTX_BONES; j += 1) {
             const f32 a = 0.35f * sinf(t * 2.f + g_crowd.phase[c][j][0]);
@@ -854,13 +860,54 @@ static void sftx_crowd_render(i32 frame) {
     sftx_split_frame(frame, &pose, &cam);
     sftx_set_scene_camera(frame);
     sft_set_ambient_light(0.3f, 0.3f, 0.3f);
-    for (i32 l = 0; l < SFTX_SHADOW_LIGHTS; l += 1) {
+    for (i32 l = 0; l < g_scene.n_lights; l += 1) {
         sft_draw_point_light(g_crowd.light_pos[l], g_crowd.light_col[l].x, g_crowd.light_col[l].y, g_crowd.light_col[l].z,
                              1.f, 40.f, (u32)(l + 1));
     }
     sft_draw_instance(&g_room.inst);
     sftx_crowd_pose(pose);
-    for (i32 c = 0; c < SFTX_CHARS; c += 1) {
+    for (i32 c = 0; c < g_scene.n_chars; c += 1) {
+        sft_instance inst;
+        memset(&inst, 0, sizeof(inst));
+        inst.model = g_crowd.person[c];
+        inst.transform = g_crowd.xform[c];
+        inst.posed_bone_matrices = g_crowd.posed[c];
+        sft_draw_instance(&inst);
+    }
+}
+
+/* ---------------- workload 3: served scene variants -----------------------
+   The crowd scene with sizes drawn from the seed: a box lattice of L^3 boxes
+   with L = round(cbrt(T / 63)) for T log-uniform in [50k, 1M] triangles
+   (L in [9, 25]; a 2048^2 lightmap page above L = 16), 0-63 skinned humanoids
+   (with the room at most 64 instances, the default visibility-ID split) and
+   0-16 shadow-casting point lights.  Without humanoids or lights the view is
+   lit by the box room's 8 point lights without shadows. */
+static void sftx_served_create(void) {
+    sftx_rng rng;
+    sftx_rng_seed(&rng, g_scene.seed, 0xC2B2AE35ULL);
+    const f32 tris = expf(sftx_uniform(&rng, logf(50000.f), logf(1000000.f)));
+    i32 L = (i32)(cbrtf(tris / 63.f) + 0.5f);
+    L = L < 9 ? 9 : (L > 25 ? 25 : L);
+    g_scene.lattice = L;
+    g_scene.lm_size = L > 16 ? 2048 : 1024;
+    g_scene.n_chars = (i32)(sftx_rng_next(&rng) % (SFTX_MAX_CHARS + 1));
+    g_scene.n_lights = (i32)(sftx_rng_next(&rng) % (SFTX_SHADOW_LIGHTS + 1));
+    sftx_crowd_create();
+}
+
+static void sftx_served_render(i32 frame) {
+    if (g_scene.n_chars > 0 && g_scene.n_lights > 0) {
+        sft_set_shadows_enabled(1);
+        sftx_crowd_render(frame);
+        return;
+    }
+    sft_set_shadows_enabled(0);
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    sftx_box_room_render(frame);
+    sftx_crowd_pose(pose);
+    for (i32 c = 0; c < g_scene.n_chars; c += 1) {
         sft_instance inst;
         memset(&inst, 0, sizeof(inst));
         inst.model = g_crowd.person[c];
@@ -873,10 +920,15 @@ static void sftx_crowd_render(i32 frame) {
 /* ---------------- dispatch ---------------- */
 static i32 sftx_build(i32 kind) {
     sft_set_clear(1);
+    g_scene.lattice = SFTX_LATTICE;
+    g_scene.lm_size = SFTX_LM_SIZE;
+    g_scene.n_chars = SFTX_CHARS;
+    g_scene.n_lights = SFTX_SHADOW_LIGHTS;
     switch (kind) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_create(); return 0;
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_create(0); return 0;
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_create(); return 0;
+        case SFTX_SCENE_SERVED: sftx_served_create(); return 0;
         default: return -1;
     }
 }
@@ -912,6 +964,7 @@ static void sftx_draw_current(i32 frame) {
         case SFTX_SCENE_PLANE_SPHERES: sftx_plane_spheres_render(frame); break;
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_render(frame); break;
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_render(frame); break;
+        case SFTX_SCENE_SERVED: sftx_served_render(frame); break;
         default: break;
     }
 }

@@ -21,6 +21,9 @@
  *       skinned humanoids (6144 triangles, 10-joint chain, 4 weights per
  *       vertex), 10 capsule shadow proxies each, 16 shadow-casting point
  *       lights, shadows enabled.
+ *   SFTX_SCENE_SERVED (3): variants of workload 2 drawn from the seed for the
+ *       request server: 50k-1M triangles (a 9^3 to 25^3 box lattice), 0-63
+ *       skinned humanoids, 0-16 shadow-casting lights.
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H
@@ -35,6 +38,7 @@ enum sftx_scene_kind {
     SFTX_SCENE_PLANE_SPHERES = 0,
     SFTX_SCENE_BOX_ROOM = 1,
     SFTX_SCENE_CROWD_SHADOWS = 2,
+    SFTX_SCENE_SERVED = 3,
     SFTX_SCENE_COUNT
 };
 

@@ -96,10 +96,10 @@ class MixedBatch:
 
 # ---------------------------------------------------------------- serving
 def served_scene_specs(n_scenes=32):
-    """Seeded variants of the three scene kinds for the served workload:
-    (kind, seed, pose_frame).  Kinds cycle interior, crowd, plane."""
-    kinds = [1, 2, 0]
-    return [(kinds[i % 3], 10 + i, (7 * i) % 120) for i in range(n_scenes)]
+    """Resident scenes of the served workload: (kind, seed, pose_frame) of
+    served-scene variants (50k-1M triangles, 0-63 skinned humanoids, 0-16
+    shadow-casting lights, drawn from the seed)."""
+    return [(3, 10 + i, (7 * i) % 120) for i in range(n_scenes)]
 
 
 def served_requests(n, rate_hz, n_scenes, seed=1, min_px=128 * 128, max_px=3840 * 2160):
@@ -164,6 +164,36 @@ class Server:
         if self.lib.server_wait(rid, self.ct.byref(lat)) != 0:
             raise RuntimeError("unknown request")
         return lat.value
+
+    def present_alone(self, template_index, camera_frame, width, height, samples, keep=False):
+        """The request rendered through sft_present (the per-frame path, the
+        server stopped): the template's scene at its pose with the request's
+        camera, colour delivered to host memory.  With keep, returns colour
+        and the 64-bit visibility planes ((1/w bits << 32) | ID)."""
+        ct, lib, S = self.ct, self.lib, self.S
+        kind, seed, pose = self.specs[template_index]
+        lib.set_output(0)
+        lib.sft_resize(width, height)
+        lib.set_msaa(samples)
+        lib.set_capture(S.CAPTURE_VISIBILITY if keep else 0)
+        color = np.zeros((height, width), dtype=np.uint32)
+        lib.sft_frame_begin()
+        lib.set_color_buffer(color.ctypes.data, width * 4)
+        lib.scene_draw(self.handles[template_index], reference_frame_code(pose, camera_frame), width, height)
+        lib.sft_present()
+        lib.sft_frame_end()
+        out = None
+        if keep:
+            cols, rows, ns = ct.c_int32(), ct.c_int32(), ct.c_int32()
+            lib.cap_grid(ct.byref(cols), ct.byref(rows), ct.byref(ns))
+            n = cols.value * 64 * rows.value * 64 * ns.value
+            vid = np.zeros(n, dtype=np.uint32)
+            depth = np.zeros(n, dtype=np.float32)
+            lib.cap_visibility(vid.ctypes.data, depth.ctypes.data)
+            out = (color, (depth.view(np.uint32).astype(np.uint64) << np.uint64(32)) | vid.astype(np.uint64))
+            lib.set_capture(0)
+        lib.set_msaa(1)
+        return out
 
     def stats(self):
         st = self.S.ServerStats()

@@ -10,10 +10,11 @@ from tests.refrun import ref_dump
 
 pytestmark = pytest.mark.gpu
 
-SPECS = [(1, 21, 5), (2, 22, 40), (0, 23, 0)]          # (kind, seed, pose frame)
+SPECS = [(1, 21, 5), (2, 22, 40), (0, 23, 0), (3, 12, 30)]          # (kind, seed, pose frame)
 REQUESTS = [  # (template, camera frame, width, height, samples)
     (0, 17, 640, 360, 1), (1, 40, 480, 300, 2), (2, 0, 256, 256, 4), (0, 99, 1024, 576, 1),
     (1, 3, 320, 180, 1), (2, 0, 700, 400, 2), (1, 40, 512, 512, 4), (0, 60, 300, 200, 2),
+    (3, 8, 800, 448, 1), (3, 51, 400, 240, 4),
 ]
 
 
@@ -45,7 +46,7 @@ def test_served_requests_same_alone_and_in_the_mix():
             np.testing.assert_array_equal(c, c_mix)
             np.testing.assert_array_equal(v, v_mix)
         # against the reference: the template's pose with the request's camera
-        for k in (1, 2, 3):
+        for k in (1, 2, 3, 8, 9):
             t, cf, w, h, ns = REQUESTS[k]
             kind, seed, pose = SPECS[t]
             ref = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns)

@@ -79,3 +79,17 @@ def test_port_matches_reference_msaa(scene, frame, samples):
     print(rep)
     parity.assert_parity(rep)
     assert rep["color_exact_frac"] == 1.0, rep
+
+
+@pytest.mark.parametrize("seed", [10, 12, 14])
+def test_port_matches_reference_served_variants(seed):
+    """Served-scene variants (box lattice size, humanoid and shadow-light
+    counts drawn from the seed) at a small resolution."""
+    w, h = 480, 272
+    port = port_render_scene(S.SCENE_SERVED, seed, w, h, 1000 * 8 + 21)
+    ref = ref_dump(S.SCENE_SERVED, seed, w, h, 1000 * 8 + 21)
+    rep = parity.compare_frame(port, ref)
+    print(rep)
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
+    assert np.array_equal(port["proxy_vis"], ref["proxy_vis"])
