@@ -26,10 +26,11 @@ SCENE_PLANE_SPHERES = 0
 SCENE_BOX_ROOM = 1
 SCENE_CROWD_SHADOWS = 2
 SCENE_SERVED = 3
+SCENE_MAP = 4
 
 # default resolution of each workload
 SCENE_SIZE = {SCENE_PLANE_SPHERES: (640, 360), SCENE_BOX_ROOM: (1920, 1080), SCENE_CROWD_SHADOWS: (1920, 1080),
-              SCENE_SERVED: (1920, 1080)}
+              SCENE_SERVED: (1920, 1080), SCENE_MAP: (1280, 720)}
 
 
 class FrameStats(ctypes.Structure):

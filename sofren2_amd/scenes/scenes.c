@@ -74,6 +74,10 @@ typedef struct {
     sft_instance inst;
     sftvec light_pos[8];
     sftvec light_col[8];
+    /* the map workload: lights read from the file's game data */
+    i32 map_lights;
+    sftvec map_pos[64], map_col[64];
+    f32 map_radius[64];
 } sftx_room_state;
 
 typedef struct {
@@ -917,6 +921,71 @@ static void sftx_served_render(i32 frame) {
     }
 }
 
+/* ---------------- workload 4: a QSF2 map file ------------------------------
+   The model of the QSF2 file named by the SFTX_QSF environment variable, lit
+   as the reference's demo lights it (main.c:213-236, 396-406): ambient 0.35,
+   the file's game-data point lights (type, position, colour, intensity,
+   attenuation; radius = attenuation), or one directional light when it has
+   none.  The camera stands at (0, 2, 0) and turns once around the vertical
+   axis over 120 frames. */
+static void sftx_map_create(void) {
+    const char* path = getenv("SFTX_QSF");
+    if (!path) {
+        fprintf(stderr, "SFTX_QSF names no map file\n");
+        exit(2);
+    }
+    sft_arena_init(&g_room.arena, 1ULL << 32);
+    g_room.model = sft_model_load_qsf(path, &g_room.arena);
+    if (!g_room.model) {
+        fprintf(stderr, "map %s failed to load\n", path);
+        exit(2);
+    }
+    memset(&g_room.inst, 0, sizeof(g_room.inst));
+    g_room.inst.model = g_room.model;
+    g_room.inst.transform = sft_m_identity();
+    g_room.map_lights = 0;
+    FILE* f = fopen(path, "rb");
+    if (f) {
+        u64 off = 0;
+        i32 n = 0;
+        if (0 == fseek(f, 8, SEEK_SET) && 1 == fread(&off, 8, 1, f) && 0 == fseek(f, (long)off, SEEK_SET) &&
+            1 == fread(&n, 4, 1, f)) {
+            for (i32 i = 0; i < n && i < 64; i += 1) {
+                struct { u32 type; f32 pos[3], color[3], intensity, attenuation; } l;
+                if (1 != fread(&l, sizeof(l), 1, f)) break;
+                g_room.map_pos[i] = SFT_V(l.pos[0], l.pos[1], l.pos[2]);
+                g_room.map_col[i] = SFT_V(l.color[0], l.color[1], l.color[2]);
+                g_room.map_radius[i] = (l.attenuation * 2.f) / 2.f;
+                g_room.map_lights = i + 1;
+            }
+        }
+        fclose(f);
+    }
+    g_scene.tri_count = 0;
+    g_scene.meshlet_count = g_room.model->total_meshlets;
+}
+
+static void sftx_map_render(i32 frame) {
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    const f32 yaw = 2.f * (f32)SFT_PI * (f32)(cam % 120) / 120.f, pitch = -0.15f;
+    const sftvec pos = SFT_V(0.f, 2.f, 0.f);
+    const sftvec fwd = SFT_V(cosf(pitch) * sinf(yaw), sinf(pitch), cosf(pitch) * cosf(yaw));
+    sft_set_camera(pos, sft_m_view_look_at(pos, sft_v_add(pos, fwd), SFT_V(0.f, 1.f, 0.f)),
+                   sft_m_proj(70.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    sft_set_ambient_light(0.35f, 0.35f, 0.35f);
+    if (g_room.map_lights > 0) {
+        for (i32 i = 0; i < g_room.map_lights; i += 1) {
+            sft_draw_point_light(g_room.map_pos[i], g_room.map_col[i].x, g_room.map_col[i].y, g_room.map_col[i].z, 1.f,
+                                 g_room.map_radius[i], (u32)(i + 1));
+        }
+    }
+    else {
+        sft_draw_directional_light(sft_v_norm(SFT_V(1.f, 0.5f, 0.f)), 1.f, 1.f, 1.f, 0.8f);
+    }
+    sft_draw_instance(&g_room.inst);
+}
+
 /* ---------------- dispatch ---------------- */
 static i32 sftx_build(i32 kind) {
     sft_set_clear(1);
@@ -929,6 +998,7 @@ static i32 sftx_build(i32 kind) {
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_create(0); return 0;
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_create(); return 0;
         case SFTX_SCENE_SERVED: sftx_served_create(); return 0;
+        case SFTX_SCENE_MAP: sftx_map_create(); return 0;
         default: return -1;
     }
 }
@@ -965,6 +1035,7 @@ static void sftx_draw_current(i32 frame) {
         case SFTX_SCENE_BOX_ROOM: sftx_box_room_render(frame); break;
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_render(frame); break;
         case SFTX_SCENE_SERVED: sftx_served_render(frame); break;
+        case SFTX_SCENE_MAP: sftx_map_render(frame); break;
         default: break;
     }
 }

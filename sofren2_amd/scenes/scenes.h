@@ -24,6 +24,9 @@
  *   SFTX_SCENE_SERVED (3): variants of workload 2 drawn from the seed for the
  *       request server: 50k-1M triangles (a 9^3 to 25^3 box lattice), 0-63
  *       skinned humanoids, 0-16 shadow-casting lights.
+ *   SFTX_SCENE_MAP (4): the QSF2 map file named by $SFTX_QSF (a real,
+ *       lightmapped scene), lit by its own game-data lights, camera turning
+ *       at (0, 2, 0).
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H
@@ -39,6 +42,7 @@ enum sftx_scene_kind {
     SFTX_SCENE_BOX_ROOM = 1,
     SFTX_SCENE_CROWD_SHADOWS = 2,
     SFTX_SCENE_SERVED = 3,
+    SFTX_SCENE_MAP = 4,
     SFTX_SCENE_COUNT
 };
 

@@ -51,6 +51,25 @@ def test_box_room_parity(frame):
     parity.assert_parity(rep)
 
 
+@pytest.mark.parametrize("frame,samples", [(0, 1), (40, 1), (40, 2)])
+def test_map_file_parity(frame, samples):
+    """The reference's own lightmapped map (res/simple.qsf): meshlets, BLAS and
+    lightmaps built by the reference's offline tools, lit by the file's
+    game-data lights."""
+    from tests.mapfile import map_path
+    map_path()
+    w, h = S.SCENE_SIZE[S.SCENE_MAP]
+    print("map frame", frame, "samples", samples, flush=True)
+    if samples > 1:
+        gpu, st = render_gpu_ms(S.SCENE_MAP, 0, w, h, frame, samples)
+    else:
+        gpu, st = render_gpu(S.SCENE_MAP, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_MAP, 0, w, h, frame, samples=samples)
+    rep = parity.compare_frame(gpu, ref)
+    print("map frame", frame, samples, rep, "clips", st.clip_triangles, "ms", st.ms_total, flush=True)
+    parity.assert_parity(rep)
+
+
 @pytest.mark.parametrize("frame", [0, 50])
 def test_crowd_shadows_parity(frame):
     """Skinned crowd in the box room with 16 shadow lights: skinned screen-space

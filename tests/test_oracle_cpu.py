@@ -93,3 +93,20 @@ def test_port_matches_reference_served_variants(seed):
     parity.assert_parity(rep)
     assert rep["color_exact_frac"] == 1.0, rep
     assert np.array_equal(port["proxy_vis"], ref["proxy_vis"])
+
+
+@pytest.mark.parametrize("frame", [0, 40])
+def test_port_matches_reference_map_file(frame):
+    """The reference's own lightmapped map (res/simple.qsf), lit by its
+    game-data lights."""
+    from tests.mapfile import map_path
+    map_path()
+    w, h = S.SCENE_SIZE[S.SCENE_MAP]
+    port = port_render_scene(S.SCENE_MAP, 0, w, h, frame)
+    ref = ref_dump(S.SCENE_MAP, 0, w, h, frame)
+    rep = parity.compare_frame(port, ref)
+    print(rep)
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
+    covered = (ref["vid"] != 0xFFFFFFFF).mean()
+    assert covered > 0.3, covered
