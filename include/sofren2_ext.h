@@ -199,8 +199,17 @@ f32 sftx_copy_calibration(u64 bytes);
    since sft_frame_begin: the reference's frame state (sofren2.c:1335-1495)
    minus its scratch.  Pointers stay valid until the next sft_frame_end or
    resource call.  The CPU restatement of the render path consumes this. */
+/* The texture's mip chain as the GPU built it in HBM: all levels in the 4x4-block
+   layout (sofren2.c:9229-9289), `capacity` texels at `out`.  Returns the chain's
+   texel count (with out == NULL just the count), -1 for an invalid handle or a
+   short buffer. */
+i64 sftx_cap_texture(sft_tex_handle tex, u32* out, u64 capacity);
+
 typedef struct sftx_texture_desc {
-    const u32* texels;          /* all mip levels, 4x4-block swizzled, contiguous */
+    const u32* texels;          /* all mip levels, 4x4-block swizzled, contiguous; NULL when the
+                                   chain is built on the GPU from level0 */
+    const u32* level0;          /* the w x h linear pixels the texture was created from */
+    u32 total;                  /* texels of the whole chain */
     i32 w, h, mip_levels, is_pot;
     u32 mip_offset[14];         /* in texels from texels[0] */
     i32 blocks_w[14];

@@ -494,6 +494,65 @@ static void raster_payload(port_state* ps, u64 payload, f32* depth, u32* vid, i3
     }
 }
 
+/* ------------------------------------------------------------ mip chains
+   sofren2.c:9229-9289: level sizes, 4x4-block swizzle, [1 2 2 1]^2 tent with
+   edge clamp and truncating division per channel */
+u32 sftx_port_build_texture(const u32* level0, i32 w, i32 h, u32* out) {
+    i32 lw[14], lh[14], levels = 0;
+    u32 offs[15];
+    u32 total = 0;
+    i32 cw = w, chh = h;
+    while (levels < 14) {
+        lw[levels] = cw;
+        lh[levels] = chh;
+        offs[levels] = total;
+        total += (u32)(((cw + 3) & ~3) * ((chh + 3) & ~3));
+        levels += 1;
+        if (1 == cw && 1 == chh) break;
+        cw = cw > 1 ? cw / 2 : 1;
+        chh = chh > 1 ? chh / 2 : 1;
+    }
+    if (!out) return total;
+    memset(out, 0, (size_t)total * 4);
+    u32* cur = (u32*)malloc((size_t)w * h * 4);
+    memcpy(cur, level0, (size_t)w * h * 4);
+    for (i32 l = 0; l < levels; l += 1) {
+        const i32 bw = ((lw[l] + 3) & ~3) / 4;
+        for (i32 y = 0; y < lh[l]; y += 1) {
+            for (i32 x = 0; x < lw[l]; x += 1) {
+                out[offs[l] + (u32)(((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3))] = cur[y * lw[l] + x];
+            }
+        }
+        if (l + 1 == levels) break;
+        const i32 nw = lw[l + 1], nh = lh[l + 1];
+        u32* next = (u32*)malloc((size_t)nw * nh * 4);
+        for (i32 y = 0; y < nh; y += 1) {
+            for (i32 x = 0; x < nw; x += 1) {
+                u32 r = 0, g = 0, b = 0, a = 0, ws = 0;
+                for (i32 ky = -1; ky <= 2; ky += 1) {
+                    const i32 sy = SFT_CLAMP(y * 2 + ky, 0, lh[l] - 1);
+                    const u32 wy = (-1 == ky || 2 == ky) ? 1u : 2u;
+                    for (i32 kx = -1; kx <= 2; kx += 1) {
+                        const i32 sx = SFT_CLAMP(x * 2 + kx, 0, lw[l] - 1);
+                        const u32 wt = ((-1 == kx || 2 == kx) ? 1u : 2u) * wy;
+                        const u32 px = cur[sy * lw[l] + sx];
+                        r += ((px >> 16) & 255u) * wt;
+                        g += ((px >> 8) & 255u) * wt;
+                        b += (px & 255u) * wt;
+                        a += ((px >> 24) & 255u) * wt;
+                        ws += wt;
+                    }
+                }
+                next[y * nw + x] = ((a / ws) << 24) | ((r / ws) << 16) | ((g / ws) << 8) | (b / ws);
+            }
+        }
+        free(cur);
+        cur = next;
+    }
+    free(cur);
+    return total;
+}
+
 /* ------------------------------------------------------------ texturing
    sofren2.c:5501-5729 */
 static const sftx_texture_desc* tex_of(const sftx_frame_desc* f, u32 h) {
@@ -1445,7 +1504,22 @@ static void resolve_tile(port_state* ps, const f32* depth, const u32* vid, i32 t
 
 /* ------------------------------------------------------------ frame
    sft_present + worker phases, sofren2.c:8139-8289, 6822-6867 */
-int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
+int sftx_port_render(const sftx_frame_desc* f_in, sftx_port_out* out) {
+    /* textures exported as level 0 only: build their chains here */
+    sftx_frame_desc fl = *f_in;
+    const sftx_frame_desc* f = &fl;
+    sftx_texture_desc* texs = NULL;
+    if (fl.n_textures > 0) {
+        texs = (sftx_texture_desc*)malloc((size_t)fl.n_textures * sizeof(sftx_texture_desc));
+        memcpy(texs, f_in->textures, (size_t)fl.n_textures * sizeof(sftx_texture_desc));
+        for (i32 i = 0; i < fl.n_textures; i += 1) {
+            if (texs[i].texels || !texs[i].level0) continue;
+            u32* chain = (u32*)malloc((size_t)texs[i].total * 4);
+            sftx_port_build_texture(texs[i].level0, texs[i].w, texs[i].h, chain);
+            texs[i].texels = chain;
+        }
+        fl.textures = texs;
+    }
     port_state ps;
     memset(&ps, 0, sizeof(ps));
     ps.f = f;
@@ -1489,6 +1563,12 @@ int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out) {
     free(ps.bins);
     free(ps.proxies);
     free(ps.tlas);
+    if (texs) {
+        for (i32 i = 0; i < fl.n_textures; i += 1) {
+            if (!f_in->textures[i].texels && texs[i].texels) free((void*)texs[i].texels);
+        }
+        free(texs);
+    }
     return ps.clip_overflow ? -1 : 0;
 }
 

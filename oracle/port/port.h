@@ -33,6 +33,13 @@ typedef struct sftx_port_out {
 /* Renders the frame.  Returns 0, or -1 if the clip buffer was too small. */
 int sftx_port_render(const sftx_frame_desc* f, sftx_port_out* out);
 
+/* The swizzled mip chain of a w x h texture from its linear level-0 pixels,
+   laid out as sft_texture_create lays it out (sofren2.c:9181-9310): levels
+   halve (floor, min 1) down to 1x1, at most 14, each padded to a multiple of 4
+   in both axes and stored in 4x4 blocks.  Writes the chain (padding zero) to
+   out when non-null; returns its texel count. */
+u32 sftx_port_build_texture(const u32* level0, i32 w, i32 h, u32* out);
+
 /* sizes the caller needs for the output buffers */
 u32 sftx_port_ss_count(const sftx_frame_desc* f);
 i32 sftx_port_proxy_count(const sftx_frame_desc* f);

@@ -155,7 +155,7 @@ struct Backend {
     u32 res_cap = 0, res_skin_cap = 0, t_cap = 0, t_skin_cap = 0;
     std::vector<MeshRec> mesh_recs;
     std::unordered_map<const sft_model*, ModelRes> models;
-    DBuf texels, tex_meta_d;
+    DBuf texels, tex_meta_d, tex_stage;
     u64 texels_used = 0;
     std::vector<TexMeta> tex_meta;
     bool tex_meta_dirty = true;
@@ -208,7 +208,7 @@ static void free_buf(DBuf& d) {
 void backend_destroy(Backend* b) {
     if (!b) return;
     for (FrameSet& fs : b->sets) HIP_CHECK(hipStreamSynchronize(fs.stream));
-    for (DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d,
+    for (DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d, &b->tex_stage,
                      &b->est_rcp, &b->est_rsq, &b->blas }) {
         free_buf(*d);
     }
@@ -390,12 +390,20 @@ static void sync_textures(Backend* b, Context& c) {
         b->tex_meta.resize(c.tex.size());
         b->tex_meta_dirty = true;
     }
+    /* new textures: level 0 to HBM, the chain built there (padding texels
+       zero); the texel heap only grows, so a heap offset stays 32-bit */
     for (size_t i = 0; i < c.tex.size(); i += 1) {
         HostTexture& t = c.tex[i];
         if (!t.live || t.uploaded) continue;
-        const u64 n = t.texels.size();
+        const u64 n = t.total;
+        if (b->texels_used + n >= (1ull << 32)) {
+            fprintf(stderr, "fatal sofren2 error: texel heap exceeds 2^32 texels\n");
+            exit(1);
+        }
         dgrow(b->texels, (size_t)(b->texels_used + n) * 4, true, stream);
-        HIP_CHECK(hipMemcpyAsync((u32*)b->texels.p + b->texels_used, t.texels.data(), n * 4, hipMemcpyHostToDevice, stream));
+        dgrow(b->tex_stage, t.level0.size() * 4, false, stream);
+        HIP_CHECK(hipMemcpyAsync(b->tex_stage.p, t.level0.data(), t.level0.size() * 4, hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemsetAsync((u32*)b->texels.p + b->texels_used, 0, n * 4, stream));
         t.dev_offset = b->texels_used;
         t.uploaded = true;
         b->texels_used += n;
@@ -410,6 +418,9 @@ static void sync_textures(Backend* b, Context& c) {
             m.blocks_w[l] = t.blocks_w[l];
             m.stride_shift[l] = t.stride_shift[l];
         }
+        launch_texture_build((const u32*)b->tex_stage.p, (u32*)b->texels.p, m, stream);
+        /* the staging buffer is reused by the next texture */
+        HIP_CHECK(hipStreamSynchronize(stream));
         b->tex_meta_dirty = true;
     }
     if (b->tex_meta_dirty) {
@@ -1139,6 +1150,27 @@ u64 backend_resident_bytes(const Backend* b) {
 }  // namespace sft
 
 using namespace sft;
+
+static Backend* ensure_backend();
+
+/* the texture's mip chain as built in HBM (texels of the whole chain, in the
+   4x4-block layout); makes pending textures resident first */
+extern "C" i64 sftx_cap_texture(sft_tex_handle tex, u32* out, u64 capacity) {
+    if (!g_ctx || SFT_INVALID_HANDLE == tex) return -1;
+    Context& c = *g_ctx;
+    const u32 i = SFT_HANDLE_INDEX(tex);
+    if (i >= c.tex.size() || !c.tex[i].live || c.tex[i].generation != (tex >> SFT_HANDLE_GEN_SHIFT)) return -1;
+    Backend* b = ensure_backend();
+    if (texture_changes(b, c)) {
+        quiesce(b);
+        sync_textures(b, c);
+    }
+    const HostTexture& t = c.tex[i];
+    if (!out) return (i64)t.total;
+    if (capacity < t.total) return -1;
+    HIP_CHECK(hipMemcpy(out, (const u32*)b->texels.p + t.dev_offset, t.total * 4, hipMemcpyDeviceToHost));
+    return (i64)t.total;
+}
 
 extern "C" void sftx_cap_grid(i32* cols, i32* rows, i32* samples) {
     Backend* b = g_ctx ? g_ctx->backend : nullptr;

@@ -77,7 +77,9 @@ extern "C" void sft_arena_frame_begin(sft_arena* arena) {
 
 /* ================================================================ textures
    4x4-block swizzled mip chains with an integer tent filter
-   (sofren2.c:9181-9310) */
+   (sofren2.c:9181-9310).  The host keeps the level-0 pixels and the chain's
+   layout; the chain itself is built in HBM by the backend's texture kernels
+   when the texture is first made resident. */
 static void tex_pool_grow(Context& c) {
     const i32 old_cap = (i32)c.tex.size();
     const i32 new_cap = (0 == old_cap) ? 1024 : old_cap * 2;
@@ -91,17 +93,6 @@ static void tex_pool_grow(Context& c) {
 }
 
 static i32 ilog2(i32 v) { return 31 - __builtin_clz((u32)v); }
-
-/* copies a linear level into the 4x4-block layout: texel (x,y) goes to
-   ((y/4)*bw + x/4)*16 + (y%4)*4 + x%4 */
-static void swizzle_level(u32* dst, const u32* src, i32 w, i32 h) {
-    const i32 bw = ((w + 3) & ~3) / 4;
-    for (i32 y = 0; y < h; y += 1) {
-        for (i32 x = 0; x < w; x += 1) {
-            dst[((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3)] = src[y * w + x];
-        }
-    }
-}
 
 extern "C" sft_tex_handle sft_texture_create(i32 width, i32 height, const u32* pixels) {
     Context& c = C();
@@ -138,45 +129,16 @@ extern "C" sft_tex_handle sft_texture_create(i32 width, i32 height, const u32* p
     ht.h = height;
     ht.mip_levels = levels;
     ht.is_pot = is_pot;
-    ht.texels.assign(total, 0u);
+    ht.level0.assign(pixels, pixels + (u64)width * height);
+    ht.total = total;
     ht.uploaded = false;
-
     u64 off = 0;
-    std::vector<u32> cur(pixels, pixels + (u64)width * height), next;
     for (i32 l = 0; l < levels; l += 1) {
         const i32 pw = (lw[l] + 3) & ~3, ph = (lh[l] + 3) & ~3;
         ht.mip_offset[l] = (u32)off;
         ht.blocks_w[l] = pw / 4;
         ht.stride_shift[l] = is_pot ? ilog2(pw / 4) : 0;
-        swizzle_level(ht.texels.data() + off, cur.data(), lw[l], lh[l]);
         off += (u64)pw * ph;
-        if (l + 1 == levels) break;
-
-        /* [1 2 2 1] x [1 2 2 1] tent with edge clamp, truncating division */
-        const i32 cw = lw[l], ch = lh[l], nw = lw[l + 1], nh = lh[l + 1];
-        next.assign((u64)nw * nh, 0u);
-        for (i32 y = 0; y < nh; y += 1) {
-            for (i32 x = 0; x < nw; x += 1) {
-                u32 r = 0, g = 0, b = 0, a = 0, wsum = 0;
-                for (i32 ky = -1; ky <= 2; ky += 1) {
-                    const i32 sy = SFT_CLAMP(y * 2 + ky, 0, ch - 1);
-                    const u32 wy = (-1 == ky || 2 == ky) ? 1u : 2u;
-                    for (i32 kx = -1; kx <= 2; kx += 1) {
-                        const i32 sx = SFT_CLAMP(x * 2 + kx, 0, cw - 1);
-                        const u32 wx = (-1 == kx || 2 == kx) ? 1u : 2u;
-                        const u32 w = wx * wy;
-                        const u32 px = cur[(u64)sy * cw + sx];
-                        r += ((px >> 16) & 255u) * w;
-                        g += ((px >> 8) & 255u) * w;
-                        b += (px & 255u) * w;
-                        a += ((px >> 24) & 255u) * w;
-                        wsum += w;
-                    }
-                }
-                next[(u64)y * nw + x] = ((a / wsum) << 24) | ((r / wsum) << 16) | ((g / wsum) << 8) | (b / wsum);
-            }
-        }
-        cur.swap(next);
     }
     return (ht.generation << SFT_HANDLE_GEN_SHIFT) | (u32)t;
 }
@@ -207,12 +169,7 @@ extern "C" u32* sft_texture_get_linear_pixels(sft_tex_handle tex) {
     HostTexture* t = tex_lookup(tex);
     if (!t) return NULL;
     u32* out = (u32*)malloc((u64)t->w * t->h * sizeof(u32));
-    const i32 bw = t->blocks_w[0];
-    for (i32 y = 0; y < t->h; y += 1) {
-        for (i32 x = 0; x < t->w; x += 1) {
-            out[(u64)y * t->w + x] = t->texels[t->mip_offset[0] + ((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3)];
-        }
-    }
+    memcpy(out, t->level0.data(), (u64)t->w * t->h * sizeof(u32));
     return out;
 }
 
@@ -224,8 +181,8 @@ extern "C" void sft_texture_release(sft_tex_handle tex) {
     if (i >= c.tex.size() || c.tex[i].generation != gen) return;
     HostTexture& t = c.tex[i];
     backend_release_texture(c.backend, i);
-    t.texels.clear();
-    t.texels.shrink_to_fit();
+    t.level0.clear();
+    t.level0.shrink_to_fit();
     t.live = false;
     t.uploaded = false;
     t.generation += 1;
@@ -1202,7 +1159,9 @@ extern "C" void sftx_export_frame(sftx_frame_desc* f) {
         const HostTexture& t = c.tex[i];
         sftx_texture_desc& d = c.export_tex[i];
         memset(&d, 0, sizeof(d));
-        d.texels = t.texels.empty() ? nullptr : t.texels.data();
+        d.texels = nullptr;
+        d.level0 = t.level0.empty() ? nullptr : t.level0.data();
+        d.total = (u32)t.total;
         d.w = t.w;
         d.h = t.h;
         d.mip_levels = t.mip_levels;

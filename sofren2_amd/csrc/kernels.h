@@ -112,6 +112,9 @@ struct FramePtrs {
    passes when non-null */
 void launch_frame(const FramePtrs& p, const BatchGpu& batch, u64 zero_words, hipStream_t s, hipEvent_t* ev);
 void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s);
+/* builds texture `m`'s swizzled mip chain in `heap` (offsets in m are heap
+   texel offsets, padding already zeroed) from its linear level-0 pixels */
+void launch_texture_build(const u32* level0, u32* heap, const TexMeta& m, hipStream_t s);
 u32 raster_groups_per_device();
 
 }  // namespace sft

@@ -2201,9 +2201,65 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
     for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (u64)gridDim.x * 256ull) dst[i] = src[i];
 }
 
+/* ================================================================== textures
+   The mip chain of sft_texture_create built in HBM (sofren2.c:9229-9289):
+   level 0 is the caller's pixels in the 4x4-block layout, every next level a
+   [1 2 2 1] x [1 2 2 1] tent over the previous one with edge clamp and
+   truncating integer division per channel.  One thread per destination
+   texel; the previous level is read back from its swizzled place. */
+DEV u32 swz(i32 x, i32 y, i32 bw) {
+    return (u32)(((y >> 2) * bw + (x >> 2)) * 16 + ((y & 3) << 2) + (x & 3));
+}
+
+__global__ __launch_bounds__(256) void k_tex_level0(const u32* __restrict__ src, u32* __restrict__ heap, u32 off,
+                                                    i32 w, i32 h, i32 bw) {
+    const u32 i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= (u32)w * (u32)h) return;
+    const i32 x = (i32)(i % (u32)w), y = (i32)(i / (u32)w);
+    heap[off + swz(x, y, bw)] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_tex_mip(u32* heap, u32 src_off, i32 cw, i32 ch, i32 sbw, u32 dst_off,
+                                                 i32 nw, i32 nh, i32 dbw) {
+    const u32 i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= (u32)nw * (u32)nh) return;
+    const i32 x = (i32)(i % (u32)nw), y = (i32)(i / (u32)nw);
+    u32 r = 0, g = 0, b = 0, a = 0, wsum = 0;
+    for (i32 ky = -1; ky <= 2; ky += 1) {
+        const i32 sy = min(max(y * 2 + ky, 0), ch - 1);
+        const u32 wy = (-1 == ky || 2 == ky) ? 1u : 2u;
+        for (i32 kx = -1; kx <= 2; kx += 1) {
+            const i32 sx = min(max(x * 2 + kx, 0), cw - 1);
+            const u32 wt = ((-1 == kx || 2 == kx) ? 1u : 2u) * wy;
+            const u32 px = heap[src_off + swz(sx, sy, sbw)];
+            r += ((px >> 16) & 255u) * wt;
+            g += ((px >> 8) & 255u) * wt;
+            b += (px & 255u) * wt;
+            a += ((px >> 24) & 255u) * wt;
+            wsum += wt;
+        }
+    }
+    heap[dst_off + swz(x, y, dbw)] = ((a / wsum) << 24) | ((r / wsum) << 16) | ((g / wsum) << 8) | (b / wsum);
+}
+
 /* ================================================================== host */
 void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n16);
+}
+
+void launch_texture_build(const u32* level0, u32* heap, const TexMeta& m, hipStream_t s) {
+    i32 cw = m.w, ch = m.h;
+    const u32 n0 = (u32)cw * (u32)ch;
+    hipLaunchKernelGGL(k_tex_level0, dim3((n0 + 255u) / 256u), dim3(256), 0, s, level0, heap, m.mip_offset[0], cw, ch,
+                       m.blocks_w[0]);
+    for (i32 l = 1; l < m.mip_levels; l += 1) {
+        const i32 nw = max(cw / 2, 1), nh = max(ch / 2, 1);
+        const u32 n = (u32)nw * (u32)nh;
+        hipLaunchKernelGGL(k_tex_mip, dim3((n + 255u) / 256u), dim3(256), 0, s, heap, m.mip_offset[l - 1], cw, ch,
+                           m.blocks_w[l - 1], m.mip_offset[l], nw, nh, m.blocks_w[l]);
+        cw = nw;
+        ch = nh;
+    }
 }
 
 /* resident workgroups of the persistent visibility kernel on the device */

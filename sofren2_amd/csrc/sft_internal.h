@@ -153,8 +153,10 @@ struct HostTexture {
     bool live = false;
     i32 w = 0, h = 0, mip_levels = 0;
     u8 is_pot = 0;
-    std::vector<u32> texels;    /* all swizzled mip levels, contiguous */
-    u32 mip_offset[14] = {0};   /* in texels from texels[0] */
+    std::vector<u32> level0;    /* the linear pixels it was created from; the swizzled
+                                   mip chain is built on the GPU */
+    u64 total = 0;              /* texels of the chain, every level padded to 4x4 */
+    u32 mip_offset[14] = {0};   /* in texels from the chain's first texel */
     i32 blocks_w[14] = {0};
     i32 stride_shift[14] = {0};
     u64 dev_offset = 0;         /* texel offset in the device heap */

@@ -38,6 +38,8 @@ def port_lib():
         lib.sftx_port_samples.argtypes = [ctypes.c_void_p]
         lib.sftx_port_grid.restype = None
         lib.sftx_port_grid.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        lib.sftx_port_build_texture.restype = ctypes.c_uint32
+        lib.sftx_port_build_texture.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
         _PORT = lib
     return _PORT
 
@@ -87,4 +89,15 @@ def port_render_scene(scene, seed, width, height, frame=0, samples=1):
         lib.scene_destroy()
     finally:
         lib.sft_release()
+    return out
+
+
+def port_mip_chain(level0):
+    """The oracle's swizzled mip chain of a (h, w) uint32 image."""
+    lib = port_lib()
+    h, w = level0.shape
+    src = np.ascontiguousarray(level0, dtype=np.uint32)
+    n = lib.sftx_port_build_texture(src.ctypes.data, w, h, None)
+    out = np.zeros(n, dtype=np.uint32)
+    lib.sftx_port_build_texture(src.ctypes.data, w, h, out.ctypes.data)
     return out
