@@ -107,33 +107,38 @@ def cpu_baseline(scene, name, width, height, frames):
 def mixed_batch_leg(lib, steps):
     """64 views over 4 scene instances at mixed resolution and MSAA: one
     batched submission per step, the next batch recorded while the previous
-    one runs (two in flight), against the same views presented one by one."""
+    one runs (two in flight), against the same views presented one by one
+    through sft_present.  Both with the frames left in HBM (device output:
+    presents also keep two frames in flight) and with every frame copied to
+    its host buffer (the drop-in output)."""
     from sofren2_amd.workloads import MixedBatch
     mb = MixedBatch(lib)
+    out = {"views": len(mb.views), "megapixels_per_batch": round(mb.pixels() / 1e6, 3)}
     try:
-        lib.set_output(1)
-        mb.render_batched()
-        mb.render_per_frame()
-        lib.synchronize()
-        mb.render_pipelined(2)
-        t0 = time.perf_counter()
-        lat = [x * 1e3 for x in mb.render_pipelined(steps)]
-        batched_s = (time.perf_counter() - t0) / steps
-        t0 = time.perf_counter()
-        per_steps = max(1, steps // 2)
-        for _ in range(per_steps):
+        for mode, tag in ((1, "device"), (0, "host")):
+            lib.set_output(mode)
+            mb.render_batched()
             mb.render_per_frame()
-        lib.synchronize()
-        per_frame_s = (time.perf_counter() - t0) / per_steps
-        n = len(mb.views)
-        px = mb.pixels()
-        lat = np.array(lat)
-        return {"views": n, "megapixels_per_batch": round(px / 1e6, 3),
-                "batched_views_s": round(n / batched_s, 2), "per_frame_views_s": round(n / per_frame_s, 2),
-                "batched_mpix_s": round(px / batched_s / 1e6, 1), "per_frame_mpix_s": round(px / per_frame_s / 1e6, 1),
-                "speedup": round(per_frame_s / batched_s, 3),
-                "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                                     "p99": round(float(np.percentile(lat, 99)), 3)}}
+            lib.synchronize()
+            mb.render_pipelined(2)
+            t0 = time.perf_counter()
+            lat = np.array([x * 1e3 for x in mb.render_pipelined(steps)])
+            batched_s = (time.perf_counter() - t0) / steps
+            t0 = time.perf_counter()
+            per_steps = max(1, steps // 2)
+            for _ in range(per_steps):
+                mb.render_per_frame()
+            lib.synchronize()
+            per_frame_s = (time.perf_counter() - t0) / per_steps
+            n, px = len(mb.views), mb.pixels()
+            out[tag] = {"batched_views_s": round(n / batched_s, 2), "per_frame_views_s": round(n / per_frame_s, 2),
+                        "batched_mpix_s": round(px / batched_s / 1e6, 1),
+                        "per_frame_mpix_s": round(px / per_frame_s / 1e6, 1),
+                        "speedup": round(per_frame_s / batched_s, 3),
+                        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                             "p99": round(float(np.percentile(lat, 99)), 3)}}
+        lib.set_output(1)
+        return out
     finally:
         mb.close()
 
@@ -263,22 +268,24 @@ def main():
     for i in range(args.warmup):
         lib.scene_render(i % 120, color.ctypes.data, w * 4)
     lib.synchronize()
-    lat, total_gpu_ms = [], []
     st = S.FrameStats()
     counter_names = [n for n, _ in S.FrameStats._fields_ if not n.startswith("ms_")]
     counters_sum = {n: 0 for n in counter_names}
+    # with device output sft_present returns once the frame is launched (two
+    # frames in flight); the clock stops after the last frame completes
     t0 = time.perf_counter()
     for i in range(args.steps):
-        a = time.perf_counter()
         lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
-        lat.append((time.perf_counter() - a) * 1e3)
-        lib.get_stats(st)
-        total_gpu_ms.append(st.ms_total)
-        for n in counter_names:
-            counters_sum[n] += getattr(st, n)
     lib.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
+    # per-frame latency: present to completion, one frame at a time
+    lat = []
+    for i in range(min(args.steps, 60)):
+        a = time.perf_counter()
+        lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
+        lib.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
 
     # per-pass kernel durations: the same frames launched eagerly so HIP
     # events can bracket each pass on the render stream
@@ -286,9 +293,13 @@ def main():
     n_inst = min(args.steps, 60)
     tile_ms, geom_ms, bin_ms, rast_ms, shade_ms, shadow_ms = [], [], [], [], [], []
     bytes_acc = {"geometry": 0, "binning": 0, "visibility": 0, "shading": 0}
+    total_gpu_ms = []
     for i in range(n_inst):
         lib.scene_render((args.warmup + i) % 120, color.ctypes.data, w * 4)
         lib.get_stats(st)
+        total_gpu_ms.append(st.ms_total)
+        for n in counter_names:
+            counters_sum[n] += getattr(st, n)
         tile_ms.append(st.ms_raster_shade)
         geom_ms.append(st.ms_geometry)
         bin_ms.append(st.ms_binning)
@@ -306,7 +317,7 @@ def main():
         "visibility": (bytes_acc["visibility"] / n_div, float(np.mean(rast_ms))),
         "shading": (bytes_acc["shading"] / n_div, float(np.mean(shade_ms))),
     }
-    counters_avg = {n: round(v / args.steps, 1) for n, v in counters_sum.items()}
+    counters_avg = {n: round(v / n_inst, 1) for n, v in counters_sum.items()}
     dominant = max(per_pass, key=lambda k: per_pass[k][1])
     dom_bytes, dom_ms = per_pass[dominant]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -333,7 +344,7 @@ def main():
                     "raster_shade": round(tile_avg_ms, 4), "raster": round(float(np.mean(rast_ms)), 4),
                     "shade": round(float(np.mean(shade_ms)), 4), "shadow_occlusion": round(float(np.mean(shadow_ms)), 4),
                     "gpu_total": round(float(np.mean(total_gpu_ms)), 4)},
-        "occlusion_mrays_s": (round(counters_sum["occlusion_rays"] / args.steps / (float(np.mean(shadow_ms)) * 1e3), 2)
+        "occlusion_mrays_s": (round(counters_sum["occlusion_rays"] / n_inst / (float(np.mean(shadow_ms)) * 1e3), 2)
                               if counters_sum["occlusion_rays"] else None),
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PASS[dominant], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -95,13 +95,17 @@ typedef struct sftx_frame_stats {
 void sftx_get_stats(sftx_frame_stats* out);
 
 /* Output target of sft_present.  SFTX_OUTPUT_HOST (default) copies the frame
-   into the buffer given to sft_set_color_buffer, as the reference does.
-   SFTX_OUTPUT_DEVICE leaves the ARGB frame in HBM (sftx_device_color) for
-   consumers on the GPU; nothing crosses PCIe. */
+   into the buffer given to sft_set_color_buffer, as the reference does, and
+   sft_present returns when it is there.  SFTX_OUTPUT_DEVICE leaves the ARGB
+   frame in HBM for consumers on the GPU; nothing crosses PCIe, and (without
+   captures) sft_present returns once the frame is launched, so the next frame
+   is recorded while it renders (two frames in flight).  sftx_device_color,
+   sftx_get_stats and sftx_synchronize wait for the frames in flight; the
+   frame's inputs are copied before sft_present returns, as always. */
 #define SFTX_OUTPUT_HOST   0
 #define SFTX_OUTPUT_DEVICE 1
 void sftx_set_output(i32 mode);
-const u32* sftx_device_color(void);     /* device pointer, width*height ARGB */
+const u32* sftx_device_color(void);     /* device pointer, width*height ARGB of the last frame */
 
 /* Multisampling of the following frames: 1 (off), 2 or 4 samples per pixel
    at the reference's rotated-grid offsets (sofren2.c:186-196); the reference

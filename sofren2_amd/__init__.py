@@ -125,6 +125,7 @@ class Library:
         self.batch_submit = _bind(g, "sftx_batch_submit", None, [])
         self.batch_end = _bind(g, "sftx_batch_end", None, [])
         self.batch_submit_async = _bind(g, "sftx_batch_submit_async", ctypes.c_int32, [])
+        self.device_color = _bind(g, "sftx_device_color", vp, [])
         self.texture_create = _bind(g, "sft_texture_create", u32, [i32, i32, vp])
         self.texture_release = _bind(g, "sft_texture_release", None, [u32])
         self.cap_texture = _bind(g, "sftx_cap_texture", ctypes.c_int64, [u32, vp, ctypes.c_uint64])

@@ -167,6 +167,7 @@ struct Backend {
     /* settings */
     i32 output_mode = SFTX_OUTPUT_HOST;
     u32 raster_chunk = 128;         /* payloads per visibility work item */
+    int last_set = 0;               /* frame set of the last completed batch */
     u32 raster_groups = 512;        /* persistent visibility workgroups */
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
     u64 graph_captures = 0;
@@ -1120,6 +1121,7 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     prepare_batch(b, *cp, views, fs);
     launch_batch(b, views, fs);
     complete_batch(b, *cp, views, fs);
+    b->last_set = 0;
 }
 
 void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {
@@ -1130,7 +1132,10 @@ void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int
 
 void backend_complete(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {
     complete_batch(b, *cp, views, b->sets[set]);
+    b->last_set = set;
 }
+
+bool backend_async_present(const Backend* b) { return SFTX_OUTPUT_DEVICE == b->output_mode; }
 
 bool backend_batch_done(Backend* b, int set) {
     return hipSuccess == hipEventQuery(b->sets[set].ev[5]);
@@ -1231,7 +1236,11 @@ extern "C" void sftx_set_graphs(i32 enabled) { ensure_backend()->use_graphs = 0 
 
 extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captures; }
 
-extern "C" const u32* sftx_device_color(void) { return (const u32*)ensure_backend()->sets[0].color.p; }
+extern "C" const u32* sftx_device_color(void) {
+    Backend* b = ensure_backend();
+    drain_async_batches(*g_ctx);
+    return (const u32*)b->sets[b->last_set].color.p;
+}
 
 extern "C" void sftx_synchronize(void) {
     Backend* b = ensure_backend();

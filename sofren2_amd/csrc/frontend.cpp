@@ -393,6 +393,24 @@ extern "C" void sft_present(void) {
         return;
     }
     if (!c.backend) c.backend = backend_create();   /* exits if there is no GPU */
+    if (backend_async_present(c.backend) && 0u == c.capture_flags) {
+        /* device output: a batch of one in flight, completed later */
+        if (c.async_batches.size() >= 2) {
+            Context::AsyncBatch& old = c.async_batches.front();
+            backend_complete(c.backend, &c, old.views, old.set);
+            c.async_batches.pop_front();
+        }
+        Context::AsyncBatch ab;
+        ab.ticket = 0;
+        ab.set = c.async_batches.empty() ? 0 : 1 - c.async_batches.back().set;
+        ab.views.resize(1);
+        /* its inputs are packed by backend_submit before sft_present returns */
+        snapshot_view(c, ab.views[0]);
+        c.async_batches.push_back(std::move(ab));
+        Context::AsyncBatch& cur = c.async_batches.back();
+        backend_submit(c.backend, &c, cur.views, cur.set);
+        return;
+    }
     drain_async_batches(c);
     std::vector<ViewRecord> one(1);
     snapshot_view(c, one[0]);
@@ -1116,7 +1134,11 @@ extern "C" sft_model* sft_model_load_qsf(const char* path, sft_arena* arena) {
 
 /* ================================================================ capture + stats */
 extern "C" void sftx_set_capture(u32 flags) { C().capture_flags = flags; }
-extern "C" void sftx_get_stats(sftx_frame_stats* out) { *out = C().stats; }
+extern "C" void sftx_get_stats(sftx_frame_stats* out) {
+    Context& c = C();
+    drain_async_batches(c);
+    *out = c.stats;
+}
 
 extern "C" void sftx_export_frame(sftx_frame_desc* f) {
     Context& c = C();

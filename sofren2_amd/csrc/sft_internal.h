@@ -215,6 +215,8 @@ void backend_destroy(Backend* b);
 /* renders a batch of views (one pass of kernels over all of them) and writes
    each view's outputs; blocks until they are in host memory */
 void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
+/* sft_present may return before its frame completes (device output) */
+bool backend_async_present(const Backend* b);
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be

@@ -221,3 +221,29 @@ def test_graph_replays_match_eager_launches(scene):
         assert 1 <= captures <= 2  # the second frame of a shape captures it
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("scene", [S.SCENE_PLANE_SPHERES, S.SCENE_CROWD_SHADOWS])
+def test_device_output_frames_in_flight(scene):
+    """With device output sft_present returns before the frame completes (two
+    in flight, dmesh rows in both frame sets); the last frame in HBM equals
+    the same frame rendered with host output."""
+    import ctypes
+    w, h = S.SCENE_SIZE[scene]
+    r = S.SceneRenderer(scene, 0, w, h)
+    try:
+        frames = [3, 17, 29, 41, 53, 60]
+        want = r.render(frames[-1]).copy()
+        r.lib.set_output(1)
+        for f in frames:
+            r.render(f)
+        ptr = r.lib.device_color()
+        got = np.zeros((h, w), dtype=np.uint32)
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(got.ctypes.data, ptr, got.nbytes, 2) == 0  # device to host
+        print("device output frame", frames[-1], flush=True)
+        np.testing.assert_array_equal(got, want)
+        r.lib.set_output(0)
+    finally:
+        r.close()
