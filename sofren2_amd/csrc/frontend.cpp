@@ -900,9 +900,32 @@ extern "C" void sft_draw_cube_inv(sft_dmesh* dm) {
     sft_dmesh_push_quad(dm, SFT_V(-h,-h,-h), SFT_V(-h,-h, h), SFT_V( h,-h, h), SFT_V( h,-h,-h), 0.f,0.f, 1.f,1.f, SFT_V(0.f, 1.f, 0.f));
 }
 
-/* UV sphere: rings+1 latitude bands of `slices` quads, starting at the pole
-   and rotating the generating edge about y then z */
-extern "C" void sft_draw_sphere(sft_dmesh* dm, i32 rings, i32 slices) {
+/* The dmesh state a sphere leaves depends only on (rings, slices) and the
+   state it starts from.  Started on an empty quad dmesh, the resulting
+   meshlet blocks, bounds and counters are kept per thread and copied into the
+   next empty dmesh asked for the same sphere: the same bytes the pushes would
+   write. */
+namespace {
+struct SphereSnapshot {
+    i32 rings = -1, slices = -1;
+    u32 meshlets = 0, curr_meshlet = 0, curr_vert = 0;
+    std::vector<sft_meshlet> m;
+    std::vector<sft_meshlet_attribs> a;
+    std::vector<sft_meshlet_inds> ind;
+    std::vector<f32> bounds;    /* 14 arrays of `meshlets` floats */
+};
+
+f32** dmesh_bound_arrays(sft_dmesh* dm, f32** out) {
+    f32* const arrays[14] = { dm->bounds_center_x, dm->bounds_center_y, dm->bounds_center_z, dm->bounds_radius,
+                              dm->bounds_min_x, dm->bounds_min_y, dm->bounds_min_z, dm->bounds_max_x,
+                              dm->bounds_max_y, dm->bounds_max_z, dm->cone_axis_x, dm->cone_axis_y,
+                              dm->cone_axis_z, dm->cone_cutoff };
+    for (int k = 0; k < 14; k += 1) out[k] = arrays[k];
+    return out;
+}
+}  // namespace
+
+static void push_sphere_quads(sft_dmesh* dm, i32 rings, i32 slices) {
     const f32 ring_angle = (f32)(SFT_PI / (rings + 1));
     const f32 slice_angle = (f32)(2.f * SFT_PI / slices);
     const f32 cr = cosf(ring_angle), sr = sinf(ring_angle);
@@ -922,6 +945,42 @@ extern "C" void sft_draw_sphere(sft_dmesh* dm, i32 rings, i32 slices) {
         const sftvec b = v[3];
         v[3] = SFT_V(cr * b.x + sr * b.y, -sr * b.x + cr * b.y, b.z);
     }
+}
+
+/* UV sphere: rings+1 latitude bands of `slices` quads, starting at the pole
+   and rotating the generating edge about y then z */
+extern "C" void sft_draw_sphere(sft_dmesh* dm, i32 rings, i32 slices) {
+    thread_local SphereSnapshot snap;
+    const bool fresh = dm && SFT_DMESH_QUADS == dm->type && 0u == dm->curr_meshlet && 0u == dm->curr_vert &&
+                       64u == dm->max_verts_per_chunk;
+    f32* arr[14];
+    if (fresh && snap.rings == rings && snap.slices == slices && snap.meshlets <= dm->max_meshlets) {
+        const u32 n = snap.meshlets;
+        memcpy(dm->meshlets, snap.m.data(), n * sizeof(sft_meshlet));
+        memcpy(dm->attribs, snap.a.data(), n * sizeof(sft_meshlet_attribs));
+        memcpy(dm->inds, snap.ind.data(), n * sizeof(sft_meshlet_inds));
+        dmesh_bound_arrays(dm, arr);
+        for (int k = 0; k < 14; k += 1) memcpy(arr[k], snap.bounds.data() + (size_t)k * n, n * sizeof(f32));
+        dm->curr_meshlet = snap.curr_meshlet;
+        dm->curr_vert = snap.curr_vert;
+        return;
+    }
+    push_sphere_quads(dm, rings, slices);
+    /* kept only when no push was dropped for want of meshlets */
+    if (!fresh || dm->curr_meshlet >= dm->max_meshlets) return;
+    /* meshlets written so far: the finalized ones plus the open chunk */
+    const u32 n = std::min(dm->curr_meshlet + (dm->curr_vert > 0 ? 1u : 0u), dm->max_meshlets);
+    snap.rings = rings;
+    snap.slices = slices;
+    snap.meshlets = n;
+    snap.curr_meshlet = dm->curr_meshlet;
+    snap.curr_vert = dm->curr_vert;
+    snap.m.assign(dm->meshlets, dm->meshlets + n);
+    snap.a.assign(dm->attribs, dm->attribs + n);
+    snap.ind.assign(dm->inds, dm->inds + n);
+    snap.bounds.resize((size_t)14 * n);
+    dmesh_bound_arrays(dm, arr);
+    for (int k = 0; k < 14; k += 1) memcpy(snap.bounds.data() + (size_t)k * n, arr[k], n * sizeof(f32));
 }
 
 static sftvec capsule_ring_point(sftvec c, sftvec b0, sftvec b1, sftvec b2, f32 sl, f32 rg, f32 radius) {
