@@ -147,10 +147,10 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
     """Poisson arrivals over resident scene templates at each rate; each
     request a random template, camera frame, resolution (128x128 to 3840x2160
     pixels, log-uniform) and MSAA.  Returns throughput and latency per rate;
-    the requests of the highest rate rendered one by one through sft_present
-    (the per-frame path) for comparison; and a 1% sample of the first rate's
-    served frames checked against the same request presented alone (the
-    per-frame path itself equals the reference in the parity tests)."""
+    the requests of the last rate rendered one by one through sft_present
+    (the per-frame path) for comparison; and a 1% sample of the served frames
+    checked against the same request presented alone (the per-frame path
+    itself equals the reference in the parity tests)."""
     from sofren2_amd import workloads
     specs = workloads.served_scene_specs(n_scenes)
     srv = workloads.Server(lib, specs, budget_bytes=200 << 30, max_views=64)
@@ -162,22 +162,24 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
             srv.wait(r)
         sample, last_reqs = [], []
         for k, rate in enumerate(rates):
-            reqs = workloads.served_requests(n_requests, rate, n_scenes, seed=100 + k)
-            keep = range(0, n_requests, max(1, int(round(1 / sample_frac)))) if k == 0 else ()
+            # at least a second of arrivals, so the last batches' latency
+            # does not dominate the wall time
+            n = max(n_requests, int(rate))
+            reqs = workloads.served_requests(n, rate, n_scenes, seed=100 + k)
+            keep = range(0, n, max(1, int(round(1 / sample_frac))))
             st0 = srv.stats()
             lat, wall, kept = workloads.run_served(srv, reqs, keep=keep)
             st1 = srv.stats()
             batches = st1["batches"] - st0["batches"]
             out["rates"].append({
-                "offered_req_s": rate, "requests": n_requests,
-                "throughput_req_s": round(n_requests / wall, 2),
+                "offered_req_s": rate, "requests": n,
+                "throughput_req_s": round(n / wall, 2),
                 "mpix_s": round(sum(w * h for _, _, _, w, h, _ in reqs) / wall / 1e6, 1),
                 "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                "p99": round(float(np.percentile(lat, 99)), 3)},
                 "mean_batch": round((st1["views_in_batches"] - st0["views_in_batches"]) / max(batches, 1), 2),
                 "batches": batches})
-            if k == 0:
-                sample = [(reqs[i], kept[i]) for i in kept]
+            sample += [(reqs[i], kept[i]) for i in kept]
             last_reqs = reqs
         out["resident_gb"] = round(srv.stats()["resident_bytes"] / 2**30, 3)
         lib.server_stop()
@@ -215,7 +217,8 @@ def main():
     ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
                     help="workload; box_room is the headline 1080p textured interior")
     ap.add_argument("--batch-steps", type=int, default=10, help="steps of the 64-view mixed batch (0: skip)")
-    ap.add_argument("--served-requests", type=int, default=200, help="requests per offered rate (0: skip serving)")
+    ap.add_argument("--served-requests", type=int, default=200,
+                    help="requests per offered rate, at least one second of arrivals (0: skip serving)")
     ap.add_argument("--served-rates", default="100,500,2000", help="offered request rates (req/s)")
     ap.add_argument("--served-scenes", type=int, default=32)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,

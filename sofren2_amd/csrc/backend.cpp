@@ -15,6 +15,7 @@
 #include <string.h>
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include "kernels.h"
 
 #define HIP_CHECK(x) do { \
@@ -1025,15 +1026,44 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
     const u32* counters = fs.counters;
     const u32 nv = (u32)views.size();
     const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
+    /* host outputs out of the pinned staging, rows in parallel when large */
+    struct Span { void* dst; const void* src; size_t n; };
+    std::vector<Span> spans;
+    size_t out_bytes = 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
         if (color_bytes && v.color_buffer) {
             const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
             for (i32 y = 0; y < v.height; y += 1) {
-                memcpy(v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4);
+                spans.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
             }
+            out_bytes += (size_t)v.width * v.height * 4;
         }
-        if (v.vis64) memcpy(v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8);
+        if (v.vis64) {
+            spans.push_back({v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8});
+            out_bytes += fs.plan[vi].grid * 8;
+        }
+    }
+    const size_t n_threads = std::min<size_t>(8, out_bytes / (4u << 20));
+    if (n_threads < 2) {
+        for (const Span& sp : spans) memcpy(sp.dst, sp.src, sp.n);
+    }
+    else {
+        /* contiguous runs of spans of about equal bytes */
+        std::vector<size_t> cut(n_threads + 1, spans.size());
+        cut[0] = 0;
+        size_t acc = 0, k = 1;
+        for (size_t i = 0; i < spans.size() && k < n_threads; i += 1) {
+            acc += spans[i].n;
+            if (acc >= out_bytes * k / n_threads) cut[k++] = i + 1;
+        }
+        std::vector<std::thread> workers;
+        for (size_t t = 0; t < n_threads; t += 1) {
+            workers.emplace_back([&spans, &cut, t]() {
+                for (size_t i = cut[t]; i < cut[t + 1]; i += 1) memcpy(spans[i].dst, spans[i].src, spans[i].n);
+            });
+        }
+        for (std::thread& w : workers) w.join();
     }
 
     u64 px = 0, samples = 0;
