@@ -109,7 +109,7 @@ def mixed_batch_leg(lib, steps):
     batched submission per step, the next batch recorded while the previous
     one runs (two in flight), against the same views presented one by one
     through sft_present.  Both with the frames left in HBM (device output:
-    presents also keep two frames in flight) and with every frame copied to
+    presents also keep three frames in flight) and with every frame copied to
     its host buffer (the drop-in output)."""
     from sofren2_amd.workloads import MixedBatch
     mb = MixedBatch(lib)

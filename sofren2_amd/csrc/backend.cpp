@@ -151,7 +151,7 @@ struct Backend {
     u32 resident_slots = 0, resident_skins = 0, resident_mesh_recs = 0;
     /* pool layout: resident rows in [0, res_cap), then one transient region
        of t_cap rows per frame set (dmesh geometry, rewritten every batch), so
-       a batch can stage its transient rows while the other set's batch runs;
+       a batch can stage its transient rows while the other sets' batches run;
        skins alike */
     u32 res_cap = 0, res_skin_cap = 0, t_cap = 0, t_skin_cap = 0;
     std::vector<MeshRec> mesh_recs;
@@ -174,7 +174,7 @@ struct Backend {
     u64 graph_captures = 0;
 
     /* two batches can be in flight (the serving loop); blocking calls use set 0 */
-    FrameSet sets[2];
+    FrameSet sets[kFrameSets];
 
     /* host copies of the last frame's intermediates (capture) */
     i32 cap_cols = 0, cap_rows = 0, cap_samples = 1;
@@ -317,8 +317,8 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
 /* grows the shared pools; no batch may be in flight (quiesce first) */
 static void grow_pools(Backend* b, size_t slots, size_t skins) {
     hipStream_t s = b->sets[0].stream;
-    slots = std::max(slots, (size_t)b->res_cap + 2 * (size_t)b->t_cap);
-    skins = std::max(skins, (size_t)b->res_skin_cap + 2 * (size_t)b->t_skin_cap);
+    slots = std::max(slots, (size_t)b->res_cap + kFrameSets * (size_t)b->t_cap);
+    skins = std::max(skins, (size_t)b->res_skin_cap + kFrameSets * (size_t)b->t_skin_cap);
     dgrow(b->pos, slots * 768, true, s);
     dgrow(b->attr, slots * 1792, true, s);
     dgrow(b->inds, slots * 512, true, s);

@@ -347,6 +347,16 @@ extern "C" void sft_frame_end(void) {
     c.proxies.clear();
 }
 
+/* the lowest frame set among the first `n` that no batch in flight uses */
+static int free_frame_set(const Context& c, int n) {
+    for (int k = 0; k < n; k += 1) {
+        bool used = false;
+        for (const Context::AsyncBatch& ab : c.async_batches) used = used || ab.set == k;
+        if (!used) return k;
+    }
+    return 0;
+}
+
 /* the frame recorded since sft_frame_begin (or sftx_view_begin) as one view */
 void sft::snapshot_view(const Context& c, ViewRecord& v) {
     v.width = c.width;
@@ -395,14 +405,14 @@ extern "C" void sft_present(void) {
     if (!c.backend) c.backend = backend_create();   /* exits if there is no GPU */
     if (backend_async_present(c.backend) && 0u == c.capture_flags) {
         /* device output: a batch of one in flight, completed later */
-        if (c.async_batches.size() >= 2) {
+        while (c.async_batches.size() >= (size_t)kFrameSets) {
             Context::AsyncBatch& old = c.async_batches.front();
             backend_complete(c.backend, &c, old.views, old.set);
             c.async_batches.pop_front();
         }
         Context::AsyncBatch ab;
         ab.ticket = 0;
-        ab.set = c.async_batches.empty() ? 0 : 1 - c.async_batches.back().set;
+        ab.set = free_frame_set(c, kFrameSets);
         ab.views.resize(1);
         /* its inputs are packed by backend_submit before sft_present returns */
         snapshot_view(c, ab.views[0]);
@@ -505,7 +515,7 @@ extern "C" i32 sftx_batch_submit_async(void) {
         return 0;
     }
     if (!c.backend) c.backend = backend_create();
-    if (c.async_batches.size() >= 2) {
+    while (c.async_batches.size() >= (size_t)kBatchesInFlight) {
         Context::AsyncBatch& old = c.async_batches.front();
         backend_complete(c.backend, &c, old.views, old.set);
         c.async_batches.pop_front();
@@ -513,7 +523,7 @@ extern "C" i32 sftx_batch_submit_async(void) {
     Context::AsyncBatch ab;
     ab.ticket = c.next_ticket;
     c.next_ticket = (c.next_ticket == 0x7FFFFFFF) ? 1 : c.next_ticket + 1;
-    ab.set = c.async_batches.empty() ? 0 : 1 - c.async_batches.back().set;
+    ab.set = free_frame_set(c, kBatchesInFlight);
     ab.views.swap(c.batch_views);
     c.async_batches.push_back(std::move(ab));
     Context::AsyncBatch& cur = c.async_batches.back();

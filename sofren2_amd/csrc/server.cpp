@@ -10,7 +10,7 @@
  * would through sft_present with that camera and display size.
  *
  * One server thread owns the backend while the server runs.  It keeps up to
- * two batches in flight (the backend's two frame sets on two streams): while
+ * two batches in flight (on the backend's frame sets, one stream each): while
  * one batch executes, requests that arrived are admitted into the next one as
  * long as the HBM budget allows (resident scene state plus the framebuffer
  * state of the batches in flight plus the candidate), launched on the other
@@ -158,7 +158,7 @@ void server_loop(Server* sv) {
     int next_set = 0;
     for (;;) {
         if (!sv->running && sv->pending.empty() && sv->inflight.empty()) break;
-        if (sv->inflight.size() < 2 && !sv->pending.empty()) {
+        if (sv->inflight.size() < (size_t)kBatchesInFlight && !sv->pending.empty()) {
             /* admit: arrival order, while the budget and the batch size allow */
             const u64 resident = backend_resident_bytes(c.backend);
             InFlight f;
@@ -176,7 +176,7 @@ void server_loop(Server* sv) {
             if (!f.reqs.empty()) {
                 f.footprint = bytes;
                 f.set = next_set;
-                next_set ^= 1;
+                next_set = (next_set + 1) % kBatchesInFlight;
                 sv->inflight_bytes += bytes;
                 for (Request* q : f.reqs) f.views.push_back(make_view(*sv->templates[(size_t)q->r.scene], q->r));
                 sv->stats.batches += 1;
@@ -196,7 +196,8 @@ void server_loop(Server* sv) {
             /* complete the oldest batch when it is done, when both slots are
                busy, or when waiting requests need its memory; otherwise keep
                admitting arrivals while it runs */
-            if (backend_batch_done(c.backend, f.set) || 2 == sv->inflight.size() || !sv->pending.empty()) {
+            if (backend_batch_done(c.backend, f.set) || (size_t)kBatchesInFlight == sv->inflight.size() ||
+                !sv->pending.empty()) {
                 lk.unlock();
                 backend_complete(c.backend, &c, f.views, f.set);
                 const Clock::time_point now = Clock::now();

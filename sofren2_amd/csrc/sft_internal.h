@@ -217,6 +217,11 @@ void backend_destroy(Backend* b);
 void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
 /* sft_present may return before its frame completes (device output) */
 bool backend_async_present(const Backend* b);
+/* frame sets of the backend.  Device-output frames keep up to kFrameSets in
+   flight (small frames overlap on the GPU); batches, which fill it alone, up
+   to kBatchesInFlight. */
+constexpr int kFrameSets = 3;
+constexpr int kBatchesInFlight = 2;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be
