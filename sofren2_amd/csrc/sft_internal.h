@@ -220,7 +220,7 @@ bool backend_async_present(const Backend* b);
 /* frame sets of the backend.  Device-output frames keep up to kFrameSets in
    flight (small frames overlap on the GPU); batches, which fill it alone, up
    to kBatchesInFlight. */
-constexpr int kFrameSets = 3;
+constexpr int kFrameSets = 4;
 constexpr int kBatchesInFlight = 2;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
