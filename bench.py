@@ -109,7 +109,7 @@ def mixed_batch_leg(lib, steps):
     batched submission per step, the next batch recorded while the previous
     one runs (two in flight), against the same views presented one by one
     through sft_present.  Both with the frames left in HBM (device output:
-    presents also keep three frames in flight) and with every frame copied to
+    presents also keep four frames in flight) and with every frame copied to
     its host buffer (the drop-in output)."""
     from sofren2_amd.workloads import MixedBatch
     mb = MixedBatch(lib)
@@ -274,7 +274,7 @@ def main():
     st = S.FrameStats()
     counter_names = [n for n, _ in S.FrameStats._fields_ if not n.startswith("ms_")]
     counters_sum = {n: 0 for n in counter_names}
-    # with device output sft_present returns once the frame is launched (two
+    # with device output sft_present returns once the frame is launched (four
     # frames in flight); the clock stops after the last frame completes
     t0 = time.perf_counter()
     for i in range(args.steps):

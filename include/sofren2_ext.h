@@ -99,7 +99,7 @@ void sftx_get_stats(sftx_frame_stats* out);
    sft_present returns when it is there.  SFTX_OUTPUT_DEVICE leaves the ARGB
    frame in HBM for consumers on the GPU; nothing crosses PCIe, and (without
    captures) sft_present returns once the frame is launched, so the next frame
-   is recorded while it renders (three frames in flight).  sftx_device_color,
+   is recorded while it renders (four frames in flight).  sftx_device_color,
    sftx_get_stats and sftx_synchronize wait for the frames in flight; the
    frame's inputs are copied before sft_present returns, as always. */
 #define SFTX_OUTPUT_HOST   0
