@@ -588,7 +588,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     std::vector<f32> tmat_shin;
     std::vector<ProxyGpu> proxies;
     std::vector<sft_bvh_node> tlas;
-    std::vector<u32> tile_view, shade_tiles, shade_tiles_ms;
+    std::vector<u32> tile_view, shade_tiles;
     std::vector<BinSeg> segs;
     u64 bin_m_words = 0;
     fs.vgpu.assign(nv, ViewGpu{});
@@ -616,10 +616,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         max_samples = std::max<u32>(max_samples, (u32)v.samples);
         any_vis64 = any_vis64 || v.vis64;
         host_color = host_color || ((SFTX_OUTPUT_HOST == b->output_mode || v.host_output) && v.color_buffer);
-        for (u32 t = 0; t < vp.n_tiles; t += 1) {
-            tile_view.push_back(vi);
-            (v.samples > 1 ? shade_tiles_ms : shade_tiles).push_back(vp.tile_base + t);
-        }
+        for (u32 t = 0; t < vp.n_tiles; t += 1) tile_view.push_back(vi);
         const u32 ss_base = ss_total;
         ss_total += v.total_ss;
 
@@ -776,8 +773,18 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         exit(1);
     }
     const u32 nt = (u32)tile_view.size();
-    const u32 n_shade1 = (u32)shade_tiles.size();
-    shade_tiles.insert(shade_tiles.end(), shade_tiles_ms.begin(), shade_tiles_ms.end());
+    /* shaded tiles in four runs: without MSAA, then with; within each, views
+       without capsule shadows first (the shading kernels are specialised) */
+    u32 n_shade_runs[4] = {0, 0, 0, 0};
+    for (int run = 0; run < 4; run += 1) {
+        for (u32 vi = 0; vi < nv; vi += 1) {
+            const FrameConst& f = fs.vgpu[vi].fc;
+            const bool ms = f.samples > 1, sh = f.shadows_enabled && f.n_proxies > 0;
+            if ((run >> 1) != (ms ? 1 : 0) || (run & 1) != (sh ? 1 : 0)) continue;
+            for (u32 t = 0; t < fs.plan[vi].n_tiles; t += 1) shade_tiles.push_back(fs.plan[vi].tile_base + t);
+            n_shade_runs[run] += fs.plan[vi].n_tiles;
+        }
+    }
     /* proxy lists exist for the lights of views with shadows; the kernel skips the others */
     u32 n_lights_cache = 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
@@ -791,8 +798,10 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.max_samples = max_samples;
     bg.rcp_bits = (u32)c.est.rcp_bits;
     bg.rsq_bits = (u32)c.est.rsq_bits;
-    bg.n_shade_tiles = n_shade1;
-    bg.n_shade_tiles_ms = (u32)shade_tiles_ms.size();
+    bg.n_shade_tiles = n_shade_runs[0] + n_shade_runs[1];
+    bg.n_shade_tiles_ms = n_shade_runs[2] + n_shade_runs[3];
+    bg.n_shade_sh = n_shade_runs[1];
+    bg.n_shade_ms_sh = n_shade_runs[3];
     bg.occ_total = occ_total;
     bg.n_lights = n_lights_cache;
     bg.vis64 = any_vis64 ? 1u : 0u;

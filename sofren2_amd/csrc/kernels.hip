@@ -1062,15 +1062,89 @@ DEV void stage_tile_shading(const FramePtrs& P, const ViewGpu& VW, u32 tile, Til
     __syncthreads();
 }
 
+/* capsule culling for a wave.  The shading positions of the wave's lanes span
+   a box of centre M and half-diagonal R, and every segment from such a
+   position to a light lies within R of the segment M-light.  A lane's ray is
+   active for a capsule only if the capsule centre projects inside the lane's
+   pixel-to-light segment at a distance below the bounding radius
+   (sofren2.c:6521-6530); a capsule whose centre keeps that distance from
+   segment M-light plus R plus a margin (4e-3 of the lengths involved and
+   1e-5 of the coordinates' size, far above the float rounding of the lane's
+   test) has no active ray on any lane, so the reference's per-group skip
+   would pass over it on every group of the wave. */
+struct WaveSpan {
+    f32 m[3];       /* box centre */
+    f32 r;          /* half-diagonal, widened for the rounding of the centre */
+    f32 mag;        /* |m| in the 1-norm */
+};
+
+/* per tile light i (16 bits each, 4 lights per word): capsules near the span */
+struct CapsuleMasks {
+    u64 q[4];
+};
+
+DEV u32 near_capsules(const CapsuleMasks& cm, int i) {
+    const u64 q = (i < 4) ? cm.q[0] : (i < 8) ? cm.q[1] : (i < 12) ? cm.q[2] : cm.q[3];
+    return (u32)(q >> ((i & 3) * 16)) & 0xFFFFu;
+}
+
+/* box of the positions of lanes with `inside` set.  Every lane of the wave
+   must execute this call: the butterfly reads all 64 lanes. */
+DEV WaveSpan wave_span(const V3& p, bool inside) {
+    const f32 inf = __builtin_inff();
+    f32 lx = inside ? p.x : inf, ly = inside ? p.y : inf, lz = inside ? p.z : inf;
+    f32 hx = inside ? p.x : -inf, hy = inside ? p.y : -inf, hz = inside ? p.z : -inf;
+    for (int o = 1; o < 64; o <<= 1) {
+        lx = fminf(lx, __shfl_xor(lx, o)); ly = fminf(ly, __shfl_xor(ly, o)); lz = fminf(lz, __shfl_xor(lz, o));
+        hx = fmaxf(hx, __shfl_xor(hx, o)); hy = fmaxf(hy, __shfl_xor(hy, o)); hz = fmaxf(hz, __shfl_xor(hz, o));
+    }
+    WaveSpan s;
+    s.m[0] = 0.5f * (lx + hx); s.m[1] = 0.5f * (ly + hy); s.m[2] = 0.5f * (lz + hz);
+    const f32 ex = hx - lx, ey = hy - ly, ez = hz - lz;
+    s.mag = fabsf(s.m[0]) + fabsf(s.m[1]) + fabsf(s.m[2]);
+    s.r = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f + 1e-6f * s.mag;
+    return s;
+}
+
+DEV bool capsule_near(const PointLightGpu& L, const ProxyRec& c, const WaveSpan& s) {
+    const f32 dx = s.m[0] - L.pos[0], dy = s.m[1] - L.pos[1], dz = s.m[2] - L.pos[2];
+    const f32 dd = dx * dx + dy * dy + dz * dz;
+    const f32 ud = c.w[0] * dx + c.w[1] * dy + c.w[2] * dz;
+    const f32 t = (dd > 0.f) ? fminf(fmaxf(ud / dd, 0.f), 1.f) : 0.f;
+    const f32 ex = c.w[0] - t * dx, ey = c.w[1] - t * dy, ez = c.w[2] - t * dz;
+    const f32 dist = sqrtf(ex * ex + ey * ey + ez * ez);
+    const f32 absl = fabsf(L.pos[0]) + fabsf(L.pos[1]) + fabsf(L.pos[2]);
+    const f32 thr = sqrtf(c.rb2) + s.r + 4e-3f * (sqrtf(c.wdw) + sqrtf(dd) + s.r) + 1e-5f * (s.mag + absl) + 1e-4f;
+    /* a NaN anywhere keeps the capsule */
+    return !(dist >= thr);
+}
+
+/* lane (light 4r + lane/16, capsule lane%16) tests one pair per round; every
+   lane of the wave must execute this call */
+DEV CapsuleMasks capsule_masks(const TileShading& ts, const WaveSpan& s, int lane) {
+    CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
+    const int k = lane & 15;
+    for (int r = 0; r < 4 && r * 4 < ts.count; r += 1) {
+        const int i = r * 4 + (lane >> 4);
+        const bool near = i < ts.count && (u32)k < ts.np[i] && capsule_near(ts.lights[i], ts.prox[i][k], s);
+        cm.q[r] = __ballot(near);
+    }
+    return cm;
+}
+
 /* shades one pixel of triangle t (sofren2.c:6190-6620, non-MSAA); dux..dvy are
    the block's first-lane UV derivatives.  `group` holds the lanes of this
    lane's 4x2 block (bits 0-7, block starting at lane `block_lane`) covered by
    the same triangle: the capsule-shadow loop couples them.  Every lane of a
    wave that reaches the lighting loop runs the same light and proxy
-   iterations, so the group's lanes meet at each ballot. */
+   iterations, so the group's lanes meet at each ballot.  With shadows, `cm`
+   holds the capsules near the span of every calling lane's position.
+   kShadows is the view's shadows_enabled && n_proxies > 0 (tiles are launched
+   in runs of one kind). */
+template <bool kShadows>
 DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const Tri& t,
                    const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileShading& ts, u32& fetch,
-                   u32 group, int block_lane) {
+                   u32 group, int block_lane, const CapsuleMasks& cm) {
     const FrameConst& fc = VW.fc;
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
     const f32 lu = interp(b, t.lu), lv = interp(b, t.lv);
@@ -1153,7 +1227,7 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
                 }
             }
         }
-        const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
+        const bool shadows = kShadows;
         /* on a lightmapped surface point lights act only through capsule
            shadows (sofren2.c:6574-6579): without proxies every light leaves
            lm_shadow at exactly 1, so the loop is skipped */
@@ -1169,18 +1243,26 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
                with zero attenuation gains nothing from it either, so the skip
                is taken only when the whole wave is out of reach */
             if (0ull == __ballot(active)) continue;
+            const u32 np = ts.np[i];
+            const u32 near_caps = (kShadows && np > 0u) ? near_capsules(cm, i) : 0u;
+            /* no capsule near: a lightmapped lane keeps lm_shadow exactly (its
+               shadow intensity is 1); only a lane with a NaN attenuation would
+               turn it into NaN */
+            if (kShadows && 0u == near_caps && 0ull == __ballot(!lightmapped || !(att == att))) continue;
             const V3 Ld = est.norm_fast(lraw);
             const f32 ndl = smax(dot3(N, Ld), 0.f);
             const bool lit = (ndl > 0.f) && active;
 
             /* capsule soft shadows (6509-6568): closest approach of the pixel-
-               to-light segment and each cached capsule, smooth-max blended */
+               to-light segment and each cached capsule, smooth-max blended.
+               Capsules outside near_caps have no active ray on any lane of the
+               wave, so the reference's per-group skip would pass over them. */
             f32 max_shadow = 0.f;
-            const u32 np = ts.np[i];
-            if (np > 0u) {
+            if (kShadows && near_caps != 0u) {
                 const f32 lrl = sqrtf(d2);
                 const f32 inv_d2 = est.rcp_refined(d2);
-                for (u32 k = 0; k < np; k += 1) {
+                for (u32 rest = near_caps; rest != 0u; rest &= rest - 1u) {
+                    const u32 k = (u32)__ffs(rest) - 1u;
                     const ProxyRec& c = ts.prox[i][k];
                     const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
                     const bool between = wdv > 0.f && wdv < d2;
@@ -1583,12 +1665,14 @@ __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
 /* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
    32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
    order (x offsets 0..3, rows 0..1) */
+template <bool kShadows>
 __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     __shared__ TileShading s_ts;
     __shared__ u32 s_li[kMaxTileLights];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    /* 16 blocks of 32x8 pixels per tile */
-    const u32 tile = P.shade_tiles[blockIdx.x >> 4];
+    /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
+    const u32 run = kShadows ? P.batch->n_shade_tiles - P.batch->n_shade_sh : 0u;
+    const u32 tile = P.shade_tiles[run + (blockIdx.x >> 4)];
     const ViewGpu& VW = P.views[P.tile_view[tile]];
     const FrameConst& fc = VW.fc;
     const u32 local = tile - VW.tile_base, sub = blockIdx.x & 15u;
@@ -1633,6 +1717,12 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     f32 cu1 = __shfl(cu, src1), cv1 = __shfl(cv, src1);
     f32 cu4 = __shfl(cu, src4), cv4 = __shfl(cv, src4);
     const u32 vid1 = __shfl(mine, src1), vid4 = __shfl(mine, src4);
+    /* capsules near this wave's shading positions (all lanes still run here) */
+    CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
+    if (kShadows) {
+        const V3 p = valid ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
+        cm = capsule_masks(tl, wave_span(p, valid), lane);
+    }
     if (!in_bounds) return;
     u32* out = &P.color[VW.color_base + (u64)gy * fc.width + gx];
     if (!valid) {
@@ -1652,7 +1742,8 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         cv4 = interp(q, t.v);
     }
     u32 fetch = 0;
-    const V3 col = shade_pixel(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, lane & ~7);
+    const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group,
+                                         lane & ~7, cm);
     *out = pack_argb(col.x * 255.f, col.y * 255.f, col.z * 255.f);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
@@ -1681,12 +1772,14 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    so the LOD lane's neighbours are at hand, the lanes covering it shade once
    and add the colour weighted by their share of samples.  Each pixel starts
    from the background colour times its empty-sample share. */
+template <bool kShadows>
 __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     __shared__ TileShading s_ts;
     __shared__ u32 s_li[kMaxTileLights];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    /* 16 blocks of 32x8 pixels per tile */
-    const u32 tile = P.shade_tiles[P.batch->n_shade_tiles + (blockIdx.x >> 4)];
+    /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
+    const u32 run = P.batch->n_shade_tiles + (kShadows ? P.batch->n_shade_tiles_ms - P.batch->n_shade_ms_sh : 0u);
+    const u32 tile = P.shade_tiles[run + (blockIdx.x >> 4)];
     const ViewGpu& VW = P.views[P.tile_view[tile]];
     const FrameConst& fc = VW.fc;
     const u32 local = tile - VW.tile_base, sub = blockIdx.x & 15u;
@@ -1723,10 +1816,10 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     remaining |= __shfl_xor(remaining, 2);
     remaining |= __shfl_xor(remaining, 4);
     u32* out = &P.color[VW.color_base + (u64)gy * fc.width + gx];
-    if (0u == remaining) {
-        if (in_bounds && fc.should_clear) *out = fc.clear_color;
-        return;
-    }
+    /* a block without triangles stays in the wave (idle) so that every lane
+       reaches the wave-wide capsule culling below */
+    const bool idle = 0u == remaining;
+    if (idle && in_bounds && fc.should_clear) *out = fc.clear_color;
     const f32 ew = (f32)empty * inv_s;
     f32 acc_r = 0.f, acc_g = 0.f, acc_b = 0.f;
     if (fc.should_clear) {
@@ -1737,45 +1830,61 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     }
     else {
         const u32 any_ew = (u32)(__ballot(ew > 0.f) >> base) & 0xFFu;
-        if (any_ew) {
+        if (any_ew && !idle) {
             const u32 c = in_bounds ? *out : 0u;
             acc_r = unpack_ch(c, 16) * ew;
             acc_g = unpack_ch(c, 8) * ew;
             acc_b = unpack_ch(c, 0) * ew;
         }
     }
-    const bool write = fc.should_clear ? in_bounds : (0u != mine);
+    const bool write = !idle && (fc.should_clear ? in_bounds : (0u != mine));
     const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
     i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
-    while (0u != remaining) {
-        /* next triangle: the first remaining (sample, lane) of the block */
-        const int pos = __ffs(remaining) - 1;
-        const int s0 = pos >> 3, l0 = pos & 7;
-        const u32 own = (0 == s0) ? sv[0] : (1 == s0) ? sv[1] : (2 == s0) ? sv[2] : sv[3];
-        const u32 target = __shfl(own, base | l0);
-        u32 hits = 0, match = 0;
-        for (int s = 0; s < 4; s += 1) {
-            if (s < S && sv[s] == target) {
-                hits += 1;
-                match |= 1u << (s * 8 + l);
-            }
-        }
-        match |= __shfl_xor(match, 1);
-        match |= __shfl_xor(match, 2);
-        match |= __shfl_xor(match, 4);
-        remaining &= ~match;
-        const u32 group = (u32)(__ballot(hits > 0u) >> base) & 0xFFu;
+    /* one triangle per block and round; the wave runs rounds while any block
+       has triangles left */
+    while (0ull != __ballot(0u != remaining)) {
+        u32 hits = 0, group = 0;
+        int first = 0;
         Tri t;
-        load_tri(P, VW, target, t);
-        const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
-        const f32 cu = interp(b, t.u), cv = interp(b, t.v);
-        const int first = __ffs(group) - 1;
-        const f32 cu0 = __shfl(cu, base | first), cv0 = __shfl(cv, base | first);
-        const f32 cu1 = __shfl(cu, base | (first ^ 1)), cv1 = __shfl(cv, base | (first ^ 1));
-        const f32 cu4 = __shfl(cu, base | (first ^ 4)), cv4 = __shfl(cv, base | (first ^ 4));
+        Bary b{0.f, 0.f, 0.f};
+        f32 cu0 = 0.f, cv0 = 0.f, cu1 = 0.f, cv1 = 0.f, cu4 = 0.f, cv4 = 0.f;
+        if (0u != remaining) {
+            /* next triangle: the first remaining (sample, lane) of the block */
+            const int pos = __ffs(remaining) - 1;
+            const int s0 = pos >> 3, l0 = pos & 7;
+            const u32 own = (0 == s0) ? sv[0] : (1 == s0) ? sv[1] : (2 == s0) ? sv[2] : sv[3];
+            const u32 target = __shfl(own, base | l0);
+            u32 match = 0;
+            for (int s = 0; s < 4; s += 1) {
+                if (s < S && sv[s] == target) {
+                    hits += 1;
+                    match |= 1u << (s * 8 + l);
+                }
+            }
+            match |= __shfl_xor(match, 1);
+            match |= __shfl_xor(match, 2);
+            match |= __shfl_xor(match, 4);
+            remaining &= ~match;
+            group = (u32)(__ballot(hits > 0u) >> base) & 0xFFu;
+            load_tri(P, VW, target, t);
+            b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
+            const f32 cu = interp(b, t.u), cv = interp(b, t.v);
+            first = __ffs(group) - 1;
+            cu0 = __shfl(cu, base | first); cv0 = __shfl(cv, base | first);
+            cu1 = __shfl(cu, base | (first ^ 1)); cv1 = __shfl(cv, base | (first ^ 1));
+            cu4 = __shfl(cu, base | (first ^ 4)); cv4 = __shfl(cv, base | (first ^ 4));
+        }
+        /* capsules near this round's shading positions (every lane runs here) */
+        CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
+        if (kShadows) {
+            const bool in = hits > 0u;
+            const V3 p = in ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
+            cm = capsule_masks(tl, wave_span(p, in), lane);
+        }
         if (hits > 0u) {
             u32 fetch = 0;
-            const V3 col = shade_pixel(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group, base);
+            const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch,
+                                                 group, base, cm);
             const f32 w = (f32)hits * inv_s;
             acc_r = fmaf(col.x, w, acc_r);
             acc_g = fmaf(col.y, w, acc_g);
@@ -2302,8 +2411,11 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
     hipLaunchKernelGGL(k_raster, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
     if (B.max_samples > 1u) hipLaunchKernelGGL(k_tile_lights, dim3(B.n_tiles), dim3(64), 0, s, p);
     mark(6);
-    if (B.n_shade_tiles) hipLaunchKernelGGL(k_shade, dim3(B.n_shade_tiles * 16u), dim3(256), 0, s, p);
-    if (B.n_shade_tiles_ms) hipLaunchKernelGGL(k_shade_ms, dim3(B.n_shade_tiles_ms * 16u), dim3(256), 0, s, p);
+    const u32 plain = B.n_shade_tiles - B.n_shade_sh, plain_ms = B.n_shade_tiles_ms - B.n_shade_ms_sh;
+    if (plain) hipLaunchKernelGGL(k_shade<false>, dim3(plain * 16u), dim3(256), 0, s, p);
+    if (B.n_shade_sh) hipLaunchKernelGGL(k_shade<true>, dim3(B.n_shade_sh * 16u), dim3(256), 0, s, p);
+    if (plain_ms) hipLaunchKernelGGL(k_shade_ms<false>, dim3(plain_ms * 16u), dim3(256), 0, s, p);
+    if (B.n_shade_ms_sh) hipLaunchKernelGGL(k_shade_ms<true>, dim3(B.n_shade_ms_sh * 16u), dim3(256), 0, s, p);
     mark(4);
     hipLaunchKernelGGL(k_stats, dim3(CNT_COUNT), dim3(64), 0, s, p.counters);
 }

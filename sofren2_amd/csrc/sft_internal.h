@@ -71,6 +71,8 @@ struct BatchGpu {
     u32 clip_cap, tile_list_cap, rcp_bits, rsq_bits;
     u32 n_shade_tiles, n_shade_tiles_ms, occ_total, n_lights;
     u32 vis64, raster_chunk, raster_groups, n_segs;
+    u32 n_shade_sh, n_shade_ms_sh;  /* of the shaded tiles above, those of views with capsule shadows (listed last) */
+    u32 pad0, pad1;
 };
 
 /* A binning segment: up to kSegItems consecutive batch slots of one view, or
