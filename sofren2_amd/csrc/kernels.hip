@@ -1119,6 +1119,16 @@ DEV bool capsule_near(const PointLightGpu& L, const ProxyRec& c, const WaveSpan&
     return !(dist >= thr);
 }
 
+/* the cheap half of the reference's capsule test (sofren2.c:6521-6530): the
+   capsule's bounding centre projects strictly inside the pixel-to-light
+   segment, closer to it than the bounding radius */
+DEV bool ray_near(const ProxyRec& c, const V3& lraw, f32 d2, f32 inv_d2) {
+    const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
+    const f32 tf = wdv * inv_d2;
+    const f32 cd2 = fmaf(-wdv, tf, c.wdw);
+    return (wdv > 0.f) & (wdv < d2) & (cd2 < c.rb2);
+}
+
 /* lane (light 4r + lane/16, capsule lane%16) tests one pair per round; every
    lane of the wave must execute this call */
 DEV CapsuleMasks capsule_masks(const TileShading& ts, const WaveSpan& s, int lane) {
@@ -1248,27 +1258,45 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
             /* no capsule near: a lightmapped lane keeps lm_shadow exactly (its
                shadow intensity is 1); only a lane with a NaN attenuation would
                turn it into NaN */
-            if (kShadows && 0u == near_caps && 0ull == __ballot(!lightmapped || !(att == att))) continue;
-            const V3 Ld = est.norm_fast(lraw);
-            const f32 ndl = smax(dot3(N, Ld), 0.f);
-            const bool lit = (ndl > 0.f) && active;
+            /* the light's direction: every lane needs it for diffuse and
+               specular or for a capsule; lightmapped lanes only for the latter */
+            V3 Ld{0.f, 0.f, 0.f};
+            f32 ndl = 0.f;
+            bool lit = false;
+            if (!kShadows || !lightmapped) {
+                Ld = est.norm_fast(lraw);
+                ndl = smax(dot3(N, Ld), 0.f);
+                lit = (ndl > 0.f) && active;
+            }
+            /* capsules with an active ray on some lane of the wave (first pass,
+               the same test as below) */
+            u32 hit_caps = 0u;
+            f32 inv_d2 = 0.f;
+            if (kShadows && near_caps != 0u) {
+                inv_d2 = est.rcp_refined(d2);
+                const bool gate = lightmapped || lit;
+                for (u32 rest = near_caps; rest != 0u; rest &= rest - 1u) {
+                    const u32 k = (u32)__ffs(rest) - 1u;
+                    if (0ull != __ballot(ray_near(ts.prox[i][k], lraw, d2, inv_d2) & gate)) hit_caps |= 1u << k;
+                }
+            }
+            /* no capsule reached: a lightmapped lane keeps lm_shadow exactly (its
+               shadow intensity is 1); only a lane with a NaN attenuation would
+               turn it into NaN */
+            if (kShadows && 0u == hit_caps && 0ull == __ballot(!lightmapped || !(att == att))) continue;
 
             /* capsule soft shadows (6509-6568): closest approach of the pixel-
                to-light segment and each cached capsule, smooth-max blended.
-               Capsules outside near_caps have no active ray on any lane of the
+               Capsules outside hit_caps have no active ray on any lane of the
                wave, so the reference's per-group skip would pass over them. */
             f32 max_shadow = 0.f;
-            if (kShadows && near_caps != 0u) {
+            if (kShadows && hit_caps != 0u) {
+                if (lightmapped) Ld = est.norm_fast(lraw);
                 const f32 lrl = sqrtf(d2);
-                const f32 inv_d2 = est.rcp_refined(d2);
-                for (u32 rest = near_caps; rest != 0u; rest &= rest - 1u) {
+                for (u32 rest = hit_caps; rest != 0u; rest &= rest - 1u) {
                     const u32 k = (u32)__ffs(rest) - 1u;
                     const ProxyRec& c = ts.prox[i][k];
-                    const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
-                    const bool between = wdv > 0.f && wdv < d2;
-                    const f32 tf = wdv * inv_d2;
-                    const f32 cd2 = fmaf(-wdv, tf, c.wdw);
-                    const bool ray_active = between && cd2 < c.rb2 && (lightmapped || lit);
+                    const bool ray_active = ray_near(c, lraw, d2, inv_d2) && (lightmapped || lit);
                     const u64 am = __ballot(ray_active);
                     /* a capsule no lane of the group reaches is skipped for the group */
                     if (0u == ((u32)(am >> block_lane) & group)) continue;
