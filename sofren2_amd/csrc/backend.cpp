@@ -118,6 +118,7 @@ struct FrameSet {
     u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
     u8* out_h = nullptr;            /* pinned staging of host outputs (colour, 64-bit visibility) */
     size_t out_h_cap = 0;
+    std::thread copier;             /* an asynchronous batch's host outputs, copied once it is done */
     u32 clip_cap = 1u << 16;
     u32 tile_list_cap = 1u << 20;
     hipGraphExec_t graph_exec = nullptr;
@@ -169,6 +170,7 @@ struct Backend {
     i32 output_mode = SFTX_OUTPUT_HOST;
     u32 raster_chunk = 128;         /* payloads per visibility work item */
     int last_set = 0;               /* frame set of the last completed batch */
+    int ordered_set = -1;           /* frame set of the last ordered submission */
     u32 raster_groups = 512;        /* persistent visibility workgroups */
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
     u64 graph_captures = 0;
@@ -209,7 +211,10 @@ static void free_buf(DBuf& d) {
 
 void backend_destroy(Backend* b) {
     if (!b) return;
-    for (FrameSet& fs : b->sets) HIP_CHECK(hipStreamSynchronize(fs.stream));
+    for (FrameSet& fs : b->sets) {
+        if (fs.copier.joinable()) fs.copier.join();
+        HIP_CHECK(hipStreamSynchronize(fs.stream));
+    }
     for (DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d, &b->tex_stage,
                      &b->est_rcp, &b->est_rsq, &b->blas }) {
         free_buf(*d);
@@ -312,6 +317,34 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
             if (mesh->meshlet_skins) memcpy(&skin[(size_t)(r.skin_of[gm] - skin_row0) * 768], &mesh->meshlet_skins[k], 768);
         }
     }
+}
+
+/* fn(i) for i < n on up to max_threads threads (the calling one among them),
+   in contiguous runs of about equal weight, each at least min_weight */
+template <class W, class F>
+static void parallel_runs(size_t n, W weight, F fn, size_t max_threads, size_t min_weight) {
+    size_t total = 0;
+    for (size_t i = 0; i < n; i += 1) total += weight(i);
+    const size_t hw = std::max<size_t>(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min({max_threads, hw, n, std::max<size_t>(1u, total / std::max<size_t>(min_weight, 1u))});
+    if (nt < 2) {
+        for (size_t i = 0; i < n; i += 1) fn(i);
+        return;
+    }
+    std::vector<size_t> cut(nt + 1, n);
+    cut[0] = 0;
+    size_t acc = 0, k = 1;
+    for (size_t i = 0; i < n && k < nt; i += 1) {
+        acc += weight(i);
+        if (acc >= total * k / nt) cut[k++] = i + 1;
+    }
+    auto run = [&](size_t t) {
+        for (size_t i = cut[t]; i < cut[t + 1]; i += 1) fn(i);
+    };
+    std::vector<std::thread> workers;
+    for (size_t t = 1; t < nt; t += 1) workers.emplace_back(run, t);
+    run(0);
+    for (std::thread& w : workers) w.join();
 }
 
 /* grows the shared pools; no batch may be in flight (quiesce first) */
@@ -839,13 +872,17 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.o_tcull = pk.reserve((size_t)fs.t_slots * 32);
     fs.o_tsm = pk.reserve((size_t)fs.t_slots * 4);
     fs.o_tskin = pk.reserve((size_t)fs.t_skins * 768);
+    std::vector<std::pair<const sft_model*, const ModelRes*>> fills;
     for (u32 vi = 0; vi < nv; vi += 1) {
-        for (const auto& kv : tmodels[vi]) {
-            fill_model_rows(kv.first, kv.second, fs.t_base, fs.t_skin_base, (f32*)(pk.host + fs.o_tpos),
-                            (f32*)(pk.host + fs.o_tattr), pk.host + fs.o_tinds, (f32*)(pk.host + fs.o_tcull),
-                            (u32*)(pk.host + fs.o_tsm), pk.host + fs.o_tskin);
-        }
+        for (const auto& kv : tmodels[vi]) fills.emplace_back(kv.first, &kv.second);
     }
+    /* disjoint rows: filled by up to 8 threads, 512 meshlets (1.5 MB) or more each */
+    parallel_runs(fills.size(), [&](size_t i) { return (size_t)fills[i].first->total_meshlets; },
+                  [&](size_t i) {
+                      fill_model_rows(fills[i].first, *fills[i].second, fs.t_base, fs.t_skin_base,
+                                      (f32*)(pk.host + fs.o_tpos), (f32*)(pk.host + fs.o_tattr), pk.host + fs.o_tinds,
+                                      (f32*)(pk.host + fs.o_tcull), (u32*)(pk.host + fs.o_tsm), pk.host + fs.o_tskin);
+                  }, 8, 512);
 
     fs.nt = nt;
     fs.nw = bg.n_work;
@@ -863,7 +900,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
 
 /* Device side of a prepared batch: buffers, one H2D copy of the tables,
    the kernels, and asynchronous copies of the counters and host outputs. */
-static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs) {
+static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs, const FrameSet* after = nullptr) {
     hipStream_t s = fs.stream;
     const u32 nv = (u32)views.size(), nt = fs.nt, nw = fs.nw;
     HIP_CHECK(hipEventRecord(fs.ev[0], s));
@@ -996,6 +1033,8 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     p.light_proxy_count = (u32*)fs.light_proxy_count.p;
     p.light_proxies = (ProxyRec*)fs.light_proxies.p;
 
+    /* kernels after the previous ordered batch's (ev[4]: its last pass) */
+    if (after && after->in_flight) HIP_CHECK(hipStreamWaitEvent(s, after->ev[4], 0));
     run_frame(b, fs, p);
     HIP_CHECK(hipMemcpyAsync(fs.counters_h, zd, CNT_COUNT * 4, hipMemcpyDeviceToHost, s));
 
@@ -1015,10 +1054,41 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     fs.in_flight = true;
 }
 
+/* the copies of a launched batch's host outputs out of its pinned staging:
+   colour rows into the callers' buffers, 64-bit visibility planes */
+struct Span { void* dst; const void* src; size_t n; };
+static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views, const FrameSet& fs) {
+    std::vector<Span> spans;
+    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
+    for (u32 vi = 0; vi < (u32)views.size(); vi += 1) {
+        const ViewRecord& v = views[vi];
+        if (color_bytes && v.color_buffer) {
+            const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
+            for (i32 y = 0; y < v.height; y += 1) {
+                spans.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
+            }
+        }
+        if (v.vis64) spans.push_back({v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8});
+    }
+    return spans;
+}
+
+/* up to 16 threads, 4 MB or more each */
+static void copy_spans(const std::vector<Span>& spans) {
+    parallel_runs(spans.size(), [&](size_t i) { return spans[i].n; },
+                  [&](size_t i) { memcpy(spans[i].dst, spans[i].src, spans[i].n); }, 16, (size_t)4 << 20);
+}
+
 /* Waits for a launched batch; re-runs it with larger intermediates if any
-   overflowed; writes the host outputs, the statistics and the captures. */
+   overflowed; writes the host outputs (unless its copier already did), the
+   statistics and the captures. */
 static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
     hipStream_t s = fs.stream;
+    bool copied = false;
+    if (fs.copier.joinable()) {
+        fs.copier.join();
+        copied = true;
+    }
     for (int attempt = 0;; attempt += 1) {
         HIP_CHECK(hipStreamSynchronize(s));
         memcpy(fs.counters, fs.counters_h, sizeof(fs.counters));
@@ -1030,50 +1100,12 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
         if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
         launch_batch(b, views, fs);
+        copied = false;
     }
     fs.in_flight = false;
     const u32* counters = fs.counters;
     const u32 nv = (u32)views.size();
-    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
-    /* host outputs out of the pinned staging, rows in parallel when large */
-    struct Span { void* dst; const void* src; size_t n; };
-    std::vector<Span> spans;
-    size_t out_bytes = 0;
-    for (u32 vi = 0; vi < nv; vi += 1) {
-        const ViewRecord& v = views[vi];
-        if (color_bytes && v.color_buffer) {
-            const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
-            for (i32 y = 0; y < v.height; y += 1) {
-                spans.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
-            }
-            out_bytes += (size_t)v.width * v.height * 4;
-        }
-        if (v.vis64) {
-            spans.push_back({v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8});
-            out_bytes += fs.plan[vi].grid * 8;
-        }
-    }
-    const size_t n_threads = std::min<size_t>(8, out_bytes / (4u << 20));
-    if (n_threads < 2) {
-        for (const Span& sp : spans) memcpy(sp.dst, sp.src, sp.n);
-    }
-    else {
-        /* contiguous runs of spans of about equal bytes */
-        std::vector<size_t> cut(n_threads + 1, spans.size());
-        cut[0] = 0;
-        size_t acc = 0, k = 1;
-        for (size_t i = 0; i < spans.size() && k < n_threads; i += 1) {
-            acc += spans[i].n;
-            if (acc >= out_bytes * k / n_threads) cut[k++] = i + 1;
-        }
-        std::vector<std::thread> workers;
-        for (size_t t = 0; t < n_threads; t += 1) {
-            workers.emplace_back([&spans, &cut, t]() {
-                for (size_t i = cut[t]; i < cut[t + 1]; i += 1) memcpy(spans[i].dst, spans[i].src, spans[i].n);
-            });
-        }
-        for (std::thread& w : workers) w.join();
-    }
+    if (!copied) copy_spans(host_output_spans(views, fs));
 
     u64 px = 0, samples = 0;
     for (const ViewRecord& v : views) {
@@ -1163,10 +1195,21 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     b->last_set = 0;
 }
 
-void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {
+void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set, bool ordered) {
     FrameSet& fs = b->sets[set];
     prepare_batch(b, *cp, views, fs);
-    launch_batch(b, views, fs);
+    const FrameSet* after = (ordered && b->ordered_set >= 0 && b->ordered_set != set) ? &b->sets[b->ordered_set] : nullptr;
+    launch_batch(b, views, fs, after);
+    if (ordered) b->ordered_set = set;
+    /* host outputs are copied as soon as the batch is done, while the caller
+       records and submits the next one; completing the batch joins the copy */
+    std::vector<Span> spans = host_output_spans(views, fs);
+    if (!spans.empty()) {
+        fs.copier = std::thread([ev = fs.ev[5], sp = std::move(spans)]() {
+            HIP_CHECK(hipEventSynchronize(ev));
+            copy_spans(sp);
+        });
+    }
 }
 
 void backend_complete(Backend* b, Context* cp, std::vector<ViewRecord>& views, int set) {

@@ -527,7 +527,7 @@ extern "C" i32 sftx_batch_submit_async(void) {
     ab.views.swap(c.batch_views);
     c.async_batches.push_back(std::move(ab));
     Context::AsyncBatch& cur = c.async_batches.back();
-    backend_submit(c.backend, &c, cur.views, cur.set);
+    backend_submit(c.backend, &c, cur.views, cur.set, true);
     return cur.ticket;
 }
 

@@ -913,6 +913,36 @@ DEV void load_tri(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
     }
 }
 
+/* the fields of Tri that barycentrics and UVs need: screen positions, 1/w, uv */
+DEV void load_tri_uv(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
+    if (vid & 0x80000000u) {
+        const f32* r = P.clip_rec + (u64)((vid & 0x7FFFFFFFu) + view_clip_base(P, V)) * 41u;
+        for (int k = 0; k < 3; k += 1) {
+            t.sx[k] = r[0 + k]; t.sy[k] = r[3 + k]; t.iw[k] = r[6 + k];
+            t.u[k] = r[9 + k]; t.v[k] = r[12 + k];
+        }
+        asm volatile("" ::: "memory");
+        return;
+    }
+    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1u);
+    const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
+    const InstGpu* inst = &P.insts[V.inst_base + (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS))];
+    const u32 slot = inst->model_base + gm;
+    const u32 ib = slot * 512u + (tri >> 3) * 32u + (tri & 7u);
+    const u32 ss = (inst->ss_offset + gm) * 576u;
+    const u32 ap = slot * 448u;
+    for (int k = 0; k < 3; k += 1) {
+        const u32 i = ld32(P.inds, ib + (u32)k * 8u) & 63u;
+        const f32* sv = P.ss + (ss + i);
+        const f32* av = P.attr + (ap + i);
+        t.sx[k] = sv[0]; t.sy[k] = sv[64]; t.iw[k] = sv[128];
+        t.u[k] = av[192]; t.v[k] = av[256];
+    }
+}
+
+/* (pixel, triangle) items of one wave in the MSAA resolve: 64 pixels x 4 samples */
+constexpr u32 kMsItems = 256;
+
 /* barycentric setup at a pixel centre (sofren2.c:6273-6312, non-MSAA) */
 struct Bary { f32 pw0, pw1, pw2; };
 /* with MSAA (`aa`) the screen barycentrics are clamped to the triangle and
@@ -1029,20 +1059,22 @@ DEV int calc_lod(const FramePtrs& P, u32 h, f32 dux, f32 duy, f32 dvx, f32 dvy) 
     return min(max(lod, 0), m.mip_levels - 1);
 }
 
-/* a tile's light list and, for views with shadows, each light's capsule
-   records, staged in LDS by the shading workgroup */
+/* a tile's light list and, for views with capsule shadows (kShadows: the
+   view's shadows_enabled && n_proxies > 0), each light's capsule records,
+   staged in LDS by the shading workgroup */
+template <bool kShadows>
 struct TileShading {
     int count;
     u32 np[kMaxTileLights];                             /* capsules per light (0 without shadows) */
     PointLightGpu lights[kMaxTileLights];
-    ProxyRec prox[kMaxTileLights][kMaxLightProxies];
+    ProxyRec prox[kShadows ? kMaxTileLights : 1][kMaxLightProxies];
 };
 
 /* fills `ts` for tile `tile` of view VW with the workgroup's threads */
-DEV void stage_tile_shading(const FramePtrs& P, const ViewGpu& VW, u32 tile, TileShading& ts, u32* s_li) {
+template <bool kShadows>
+DEV void stage_tile_shading(const FramePtrs& P, const ViewGpu& VW, u32 tile, TileShading<kShadows>& ts, u32* s_li) {
     const int tid = threadIdx.x, nthreads = blockDim.x;
-    const FrameConst& fc = VW.fc;
-    const bool shadows = fc.shadows_enabled && fc.n_proxies > 0u;
+    const bool shadows = kShadows;
     const u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
     const int count = (int)tl[0];
     if (tid < count) {
@@ -1131,7 +1163,7 @@ DEV bool ray_near(const ProxyRec& c, const V3& lraw, f32 d2, f32 inv_d2) {
 
 /* lane (light 4r + lane/16, capsule lane%16) tests one pair per round; every
    lane of the wave must execute this call */
-DEV CapsuleMasks capsule_masks(const TileShading& ts, const WaveSpan& s, int lane) {
+DEV CapsuleMasks capsule_masks(const TileShading<true>& ts, const WaveSpan& s, int lane) {
     CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
     const int k = lane & 15;
     for (int r = 0; r < 4 && r * 4 < ts.count; r += 1) {
@@ -1153,7 +1185,7 @@ DEV CapsuleMasks capsule_masks(const TileShading& ts, const WaveSpan& s, int lan
    in runs of one kind). */
 template <bool kShadows>
 DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const Tri& t,
-                   const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileShading& ts, u32& fetch,
+                   const Bary& b, f32 dux, f32 duy, f32 dvx, f32 dvy, const TileShading<kShadows>& ts, u32& fetch,
                    u32 group, int block_lane, const CapsuleMasks& cm) {
     const FrameConst& fc = VW.fc;
     const f32 cu = interp(b, t.u), cv = interp(b, t.v);
@@ -1695,7 +1727,7 @@ __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
    order (x offsets 0..3, rows 0..1) */
 template <bool kShadows>
 __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
-    __shared__ TileShading s_ts;
+    __shared__ TileShading<kShadows> s_ts;
     __shared__ u32 s_li[kMaxTileLights];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
@@ -1708,7 +1740,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
     const Est est = make_est(P);
     stage_tile_shading(P, VW, tile, s_ts, s_li);
-    const TileShading& tl = s_ts;
+    const TileShading<kShadows>& tl = s_ts;
 
     const int blk = lane >> 3, l = lane & 7, ix = l & 3, iy = l >> 2;
     const i32 bx = x0 + blk * 4, by = y0 + wave * 2;
@@ -1747,7 +1779,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const u32 vid1 = __shfl(mine, src1), vid4 = __shfl(mine, src4);
     /* capsules near this wave's shading positions (all lanes still run here) */
     CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
-    if (kShadows) {
+    if constexpr (kShadows) {
         const V3 p = valid ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
         cm = capsule_masks(tl, wave_span(p, valid), lane);
     }
@@ -1802,8 +1834,13 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    from the background colour times its empty-sample share. */
 template <bool kShadows>
 __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
-    __shared__ TileShading s_ts;
+    __shared__ TileShading<kShadows> s_ts;
     __shared__ u32 s_li[kMaxTileLights];
+    /* per wave: up to 64 pixels x 4 samples (pixel, triangle) items */
+    __shared__ u32 s_item_vid[4][kMsItems];
+    __shared__ u32 s_item_meta[4][kMsItems];     /* pixel lane | group start << 8 | group size << 16 | samples << 24 */
+    __shared__ f32 s_item_der[4][4][kMsItems];   /* UV derivatives, at the group's first item */
+    __shared__ f32 s_item_col[4][3][64];         /* shaded colour of the chunk's items */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
     const u32 run = P.batch->n_shade_tiles + (kShadows ? P.batch->n_shade_tiles_ms - P.batch->n_shade_ms_sh : 0u);
@@ -1815,7 +1852,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     const i32 y0 = (i32)(local / (u32)fc.tile_cols) * 64 + (i32)(sub >> 1) * 8;
     const Est est = make_est(P);
     stage_tile_shading(P, VW, tile, s_ts, s_li);
-    const TileShading& tl = s_ts;
+    const TileShading<kShadows>& tl = s_ts;
 
     const int S = fc.samples;
     const f32 inv_s = 1.f / (f32)S;
@@ -1868,20 +1905,27 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
     const bool write = !idle && (fc.should_clear ? in_bounds : (0u != mine));
     const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
     i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
-    /* one triangle per block and round; the wave runs rounds while any block
-       has triangles left */
+    u32* it_vid = s_item_vid[wave];
+    u32* it_meta = s_item_meta[wave];
+
+    /* 1. the (block, triangle) groups: one triangle per block and round, the
+       wave running rounds while any block has triangles left.  Each lane
+       covered by the round's triangle gets an item (its pixel, the triangle,
+       its sample count); a group's items are consecutive, groups follow in
+       round then block order, so a pixel's items are in its triangle order.
+       The group's first item keeps the LOD lane's UV derivatives. */
+    u32 n_items = 0, my_items = 0, my_idx = 0;
     while (0ull != __ballot(0u != remaining)) {
-        u32 hits = 0, group = 0;
+        u32 hits = 0, target = 0;
+        f32 dux = 0.f, duy = 0.f, dvx = 0.f, dvy = 0.f;
         int first = 0;
-        Tri t;
-        Bary b{0.f, 0.f, 0.f};
-        f32 cu0 = 0.f, cv0 = 0.f, cu1 = 0.f, cv1 = 0.f, cu4 = 0.f, cv4 = 0.f;
+        u32 group = 0;
         if (0u != remaining) {
             /* next triangle: the first remaining (sample, lane) of the block */
             const int pos = __ffs(remaining) - 1;
             const int s0 = pos >> 3, l0 = pos & 7;
             const u32 own = (0 == s0) ? sv[0] : (1 == s0) ? sv[1] : (2 == s0) ? sv[2] : sv[3];
-            const u32 target = __shfl(own, base | l0);
+            target = __shfl(own, base | l0);
             u32 match = 0;
             for (int s = 0; s < 4; s += 1) {
                 if (s < S && sv[s] == target) {
@@ -1894,35 +1938,89 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
             match |= __shfl_xor(match, 4);
             remaining &= ~match;
             group = (u32)(__ballot(hits > 0u) >> base) & 0xFFu;
-            load_tri(P, VW, target, t);
-            b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
+            Tri t;
+            load_tri_uv(P, VW, target, t);
+            const Bary b = bary_at(est, t, fbx + (f32)ix, fby + (f32)iy, true);
             const f32 cu = interp(b, t.u), cv = interp(b, t.v);
             first = __ffs(group) - 1;
-            cu0 = __shfl(cu, base | first); cv0 = __shfl(cv, base | first);
-            cu1 = __shfl(cu, base | (first ^ 1)); cv1 = __shfl(cv, base | (first ^ 1));
-            cu4 = __shfl(cu, base | (first ^ 4)); cv4 = __shfl(cv, base | (first ^ 4));
+            const f32 cu0 = __shfl(cu, base | first), cv0 = __shfl(cv, base | first);
+            const f32 cu1 = __shfl(cu, base | (first ^ 1)), cv1 = __shfl(cv, base | (first ^ 1));
+            const f32 cu4 = __shfl(cu, base | (first ^ 4)), cv4 = __shfl(cv, base | (first ^ 4));
+            dux = cu0 - cu1; duy = cu0 - cu4; dvx = cv0 - cv1; dvy = cv0 - cv4;
         }
-        /* capsules near this round's shading positions (every lane runs here) */
-        CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
-        if (kShadows) {
-            const bool in = hits > 0u;
-            const V3 p = in ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
-            cm = capsule_masks(tl, wave_span(p, in), lane);
-        }
+        const u64 hm = __ballot(hits > 0u);
         if (hits > 0u) {
+            const u32 idx = n_items + lanes_below(hm, lane);
+            const u32 gs = n_items + lanes_below(hm, base);
+            it_vid[idx] = target;
+            it_meta[idx] = (u32)lane | (gs << 8) | ((u32)__popc(group) << 16) | (hits << 24);
+            if (l == first) {
+                s_item_der[wave][0][gs] = dux; s_item_der[wave][1][gs] = duy;
+                s_item_der[wave][2][gs] = dvx; s_item_der[wave][3][gs] = dvy;
+            }
+            my_idx |= idx << (8u * my_items);
+            my_items += 1;
+        }
+        n_items += (u32)__popcll(hm);
+    }
+    wave_sync();
+
+    /* 2. the items shaded 64 at a time; a chunk ends at a group boundary so
+       that a group's lanes meet at the capsule ballots (lanes gs - c0 ...) */
+    u32 next_j = 0;
+    for (u32 c0 = 0; c0 < n_items;) {
+        u32 end = min(n_items, c0 + 64u);
+        if (end < n_items) {
+            const u32 m = it_meta[end - 1u];
+            const u32 gs = (m >> 8) & 0xFFu, gn = (m >> 16) & 0xFFu;
+            if (gs + gn > end) end = gs;
+        }
+        const u32 item = c0 + (u32)lane;
+        const bool has = item < end;
+        u32 meta = 0;
+        Tri t;
+        Bary b{0.f, 0.f, 0.f};
+        if (has) {
+            meta = it_meta[item];
+            const int pl = (int)(meta & 63u), pix = pl & 7;
+            load_tri(P, VW, it_vid[item], t);
+            b = bary_at(est, t, (f32)(x0 + (pl >> 3) * 4) + 0.5f + (f32)(pix & 3),
+                        (f32)(y0 + wave * 2) + 0.5f + (f32)(pix >> 2), true);
+        }
+        /* capsules near this chunk's shading positions (every lane runs here) */
+        CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
+        if constexpr (kShadows) {
+            const V3 p = has ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
+            cm = capsule_masks(tl, wave_span(p, has), lane);
+        }
+        if (has) {
+            const u32 gs = (meta >> 8) & 0xFFu, gn = (meta >> 16) & 0xFFu;
             u32 fetch = 0;
-            const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch,
-                                                 group, base, cm);
-            const f32 w = (f32)hits * inv_s;
-            acc_r = fmaf(col.x, w, acc_r);
-            acc_g = fmaf(col.y, w, acc_g);
-            acc_b = fmaf(col.z, w, acc_b);
+            const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, s_item_der[wave][0][gs], s_item_der[wave][1][gs],
+                                                 s_item_der[wave][2][gs], s_item_der[wave][3][gs], tl, fetch,
+                                                 (1u << gn) - 1u, (int)(gs - c0), cm);
+            s_item_col[wave][0][lane] = col.x;
+            s_item_col[wave][1][lane] = col.y;
+            s_item_col[wave][2][lane] = col.z;
             shaded += 1;
-            groups += (first == l) ? 1 : 0;
+            groups += (item == gs) ? 1 : 0;
             n_bil += (fetch & 1u) ? 1 : 0;
             n_near += (fetch & 2u) ? 1 : 0;
             n_lm += (fetch & 4u) ? 1 : 0;
         }
+        wave_sync();
+        /* each pixel adds its items of the chunk, in its triangle order,
+           weighted by their share of samples */
+        for (; next_j < my_items; next_j += 1) {
+            const u32 idx = (my_idx >> (8u * next_j)) & 0xFFu;
+            if (idx >= end) break;
+            const f32 w = (f32)(it_meta[idx] >> 24) * inv_s;
+            acc_r = fmaf(s_item_col[wave][0][idx - c0], w, acc_r);
+            acc_g = fmaf(s_item_col[wave][1][idx - c0], w, acc_g);
+            acc_b = fmaf(s_item_col[wave][2][idx - c0], w, acc_b);
+        }
+        wave_sync();
+        c0 = end;
     }
     if (write) *out = pack_argb(smin(acc_r, 1.f) * 255.f, smin(acc_g, 1.f) * 255.f, smin(acc_b, 1.f) * 255.f);
 

@@ -185,7 +185,7 @@ void server_loop(Server* sv) {
                 sv->inflight.push_back(std::move(f));
                 InFlight& g = sv->inflight.back();
                 lk.unlock();
-                backend_submit(c.backend, &c, g.views, g.set);
+                backend_submit(c.backend, &c, g.views, g.set, true);
                 lk.lock();
                 continue;
             }

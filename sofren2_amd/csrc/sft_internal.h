@@ -227,8 +227,11 @@ constexpr int kBatchesInFlight = 2;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be
-   in flight; complete blocks until its batch is done */
-void backend_submit(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
+   in flight; complete blocks until its batch is done.  An ordered submission
+   starts its kernels after the previous ordered one's (its tables upload
+   meanwhile): large batches then run back to back rather than sharing the
+   GPU, and the one waited for ends first. */
+void backend_submit(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set, bool ordered = false);
 void backend_complete(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int set);
 bool backend_batch_done(Backend* b, int set);
 u64 backend_resident_bytes(const Backend* b);
