@@ -157,8 +157,9 @@ void sftx_batch_end(void);
    and textures, materials and models must not be created or released.
    Requests are admitted in arrival order into the next batch while the
    resident scene state, the framebuffer state of the batches in flight and
-   the request's own fit in the HBM budget; up to two batches are in flight,
-   so a batch is prepared and launched while the previous one executes. */
+   the request's own fit in the HBM budget; up to three batches are in
+   flight, so a batch is prepared and launched, and a finished one's frames
+   copied to host memory, while another executes. */
 typedef struct sftx_request {
     i32 scene;                  /* template handle */
     i32 width, height, samples; /* samples: 1, 2 or 4 */

@@ -131,13 +131,14 @@ struct FrameSet {
         u32 inst_base = 0, work_base = 0, n_work = 0, pl_base = 0, dl_base = 0, tmat_base = 0;
         u32 tile_base = 0, proxy_base = 0, tlas_base = 0, occ_base = 0, occ_count = 0;
         u64 vid_base = 0, color_base = 0, grid = 0;
+        u64 vis_stage = 0;          /* its 64-bit visibility in the host staging (views that ask for it) */
         u32 n_tiles = 0;
     };
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
     u32 t_base = 0, t_skin_base = 0;    /* this set's transient region */
     u64 bin_m_words = 0;
-    u64 vid_total = 0, color_total = 0;
+    u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     size_t zero_words = 0;
     bool any_vis64 = false, cap_vis = false, host_color = false;
     size_t o_batch = 0, o_views = 0, o_tv = 0, o_st = 0, o_insts = 0, o_work = 0, o_bones = 0, o_pl = 0, o_dl = 0;
@@ -627,7 +628,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.vgpu.assign(nv, ViewGpu{});
     fs.plan.assign(nv, FrameSet::ViewPlan{});
     u32 ss_total = 0, max_samples = 1, occ_total = 0;
-    u64 vid_total = 0, color_total = 0;
+    u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     bool any_vis64 = false, host_color = false;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
@@ -792,7 +793,12 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
             segs.push_back(sg);
         }
         bin_m_words += (u64)(n_bseg + 1u) * vp.n_tiles;
-        g.pad0 = g.pad1 = g.pad2 = 0;
+        g.vis64 = v.vis64 ? 1u : 0u;
+        g.pad1 = g.pad2 = 0;
+        if (v.vis64) {
+            vp.vis_stage = vis_stage_total;
+            vis_stage_total += vp.grid;
+        }
         g.vid_base = vp.vid_base;
         g.color_base = vp.color_base;
     }
@@ -891,6 +897,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.n_proxies = (u32)proxies.size();
     fs.n_lights = (u32)pl.size();
     fs.vid_total = vid_total;
+    fs.vis_stage_total = vis_stage_total;
     fs.color_total = color_total;
     fs.any_vis64 = any_vis64;
     fs.host_color = host_color;
@@ -1040,7 +1047,7 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
 
     /* host outputs: the colour planes (and 64-bit visibility) into pinned staging */
     const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
-    const size_t stage = color_bytes + (fs.any_vis64 ? (size_t)fs.vid_total * 8 : 0);
+    const size_t stage = color_bytes + (size_t)fs.vis_stage_total * 8;
     if (stage) {
         if (fs.out_h_cap < stage) {
             if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
@@ -1048,7 +1055,13 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
             fs.out_h_cap = stage;
         }
         if (color_bytes) HIP_CHECK(hipMemcpyAsync(fs.out_h, fs.color.p, color_bytes, hipMemcpyDeviceToHost, s));
-        if (fs.any_vis64) HIP_CHECK(hipMemcpyAsync(fs.out_h + color_bytes, fs.vis64.p, (size_t)fs.vid_total * 8, hipMemcpyDeviceToHost, s));
+        /* 64-bit visibility of the views that ask for it, packed */
+        for (u32 vi = 0; vi < nv; vi += 1) {
+            if (!views[vi].vis64) continue;
+            const FrameSet::ViewPlan& vp = fs.plan[vi];
+            HIP_CHECK(hipMemcpyAsync(fs.out_h + color_bytes + vp.vis_stage * 8, (const u64*)fs.vis64.p + vp.vid_base,
+                                     (size_t)vp.grid * 8, hipMemcpyDeviceToHost, s));
+        }
     }
     HIP_CHECK(hipEventRecord(fs.ev[5], s));
     fs.in_flight = true;
@@ -1068,7 +1081,7 @@ static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views,
                 spans.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
             }
         }
-        if (v.vis64) spans.push_back({v.vis64, fs.out_h + color_bytes + fs.plan[vi].vid_base * 8, fs.plan[vi].grid * 8});
+        if (v.vis64) spans.push_back({v.vis64, fs.out_h + color_bytes + fs.plan[vi].vis_stage * 8, fs.plan[vi].grid * 8});
     }
     return spans;
 }

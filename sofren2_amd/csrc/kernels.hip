@@ -1538,7 +1538,7 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
         }
         P.vid[gi] = vid;
         if (P.cap_depth) P.cap_depth[gi] = z;
-        if (P.vis64) P.vis64[gi] = ((u64)(u32)(pk[k] >> 32) << 32) | vid;
+        if (P.vis64 && V.vis64) P.vis64[gi] = ((u64)(u32)(pk[k] >> 32) << 32) | vid;
     }
     for (int o = 32; o > 0; o >>= 1) {
         zmin = fminf(zmin, __shfl_xor(zmin, o));

@@ -10,12 +10,13 @@
  * would through sft_present with that camera and display size.
  *
  * One server thread owns the backend while the server runs.  It keeps up to
- * two batches in flight (on the backend's frame sets, one stream each): while
- * one batch executes, requests that arrived are admitted into the next one as
- * long as the HBM budget allows (resident scene state plus the framebuffer
- * state of the batches in flight plus the candidate), launched on the other
- * stream, and the oldest batch is completed when both slots are busy.
- * Requests are taken in arrival order.
+ * three batches in flight (on the backend's frame sets, one stream each; each
+ * batch's kernels start after the previous batch's): while batches execute,
+ * requests that arrived are admitted into the next one as long as the HBM
+ * budget allows (resident scene state plus the framebuffer state of the
+ * batches in flight plus the candidate), launched on a free set's stream, and
+ * the oldest batch is completed when every slot is busy.  Requests are taken
+ * in arrival order.
  */
 #include <chrono>
 #include <condition_variable>
@@ -158,7 +159,7 @@ void server_loop(Server* sv) {
     int next_set = 0;
     for (;;) {
         if (!sv->running && sv->pending.empty() && sv->inflight.empty()) break;
-        if (sv->inflight.size() < (size_t)kBatchesInFlight && !sv->pending.empty()) {
+        if (sv->inflight.size() < (size_t)kServerBatchesInFlight && !sv->pending.empty()) {
             /* admit: arrival order, while the budget and the batch size allow */
             const u64 resident = backend_resident_bytes(c.backend);
             InFlight f;
@@ -176,7 +177,7 @@ void server_loop(Server* sv) {
             if (!f.reqs.empty()) {
                 f.footprint = bytes;
                 f.set = next_set;
-                next_set = (next_set + 1) % kBatchesInFlight;
+                next_set = (next_set + 1) % kServerBatchesInFlight;
                 sv->inflight_bytes += bytes;
                 for (Request* q : f.reqs) f.views.push_back(make_view(*sv->templates[(size_t)q->r.scene], q->r));
                 sv->stats.batches += 1;
@@ -193,10 +194,10 @@ void server_loop(Server* sv) {
         }
         if (!sv->inflight.empty()) {
             InFlight& f = sv->inflight.front();
-            /* complete the oldest batch when it is done, when both slots are
+            /* complete the oldest batch when it is done, when every slot is
                busy, or when waiting requests need its memory; otherwise keep
                admitting arrivals while it runs */
-            if (backend_batch_done(c.backend, f.set) || (size_t)kBatchesInFlight == sv->inflight.size() ||
+            if (backend_batch_done(c.backend, f.set) || (size_t)kServerBatchesInFlight == sv->inflight.size() ||
                 !sv->pending.empty()) {
                 lk.unlock();
                 backend_complete(c.backend, &c, f.views, f.set);

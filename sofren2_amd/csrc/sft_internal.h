@@ -60,7 +60,8 @@ struct ViewGpu {
     u32 dl_base, tmat_base, tile_base, proxy_base;
     u32 tlas_base, occ_base, occ_count, seg_base;
     u32 n_segs;         /* binning segments of the view, its clip segment last */
-    u32 pad0, pad1, pad2;
+    u32 vis64;          /* 1: the view's 64-bit visibility planes are written */
+    u32 pad1, pad2;
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -221,9 +222,13 @@ void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
 bool backend_async_present(const Backend* b);
 /* frame sets of the backend.  Device-output frames keep up to kFrameSets in
    flight (small frames overlap on the GPU); batches, which fill it alone, up
-   to kBatchesInFlight. */
+   to kBatchesInFlight (sets 0 and 1, so that a caller alternating two view
+   sets replays each set's graph); the request server up to
+   kServerBatchesInFlight (sets 0-2: its host outputs, copy-out and the next
+   batch's tables overlap the GPU work of the other two). */
 constexpr int kFrameSets = 4;
 constexpr int kBatchesInFlight = 2;
+constexpr int kServerBatchesInFlight = 3;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be

@@ -213,9 +213,10 @@ def reference_frame_code(pose_frame, camera_frame):
 
 
 def run_served(server, requests, keep=None):
-    """Replays Poisson arrivals against the server from a submitting thread;
-    returns per-request latencies (ms), wall time and, for the indices in
-    `keep`, the colour and visibility outputs."""
+    """Replays Poisson arrivals against the server from a submitting thread,
+    each request delivering its frame into a host buffer of its own; returns
+    per-request latencies (ms), wall time and, for the indices in `keep`, the
+    colour and visibility outputs."""
     import threading
     import time
     kept = {}
@@ -229,13 +230,13 @@ def run_served(server, requests, keep=None):
             now = time.perf_counter() - t0
             if ta > now:
                 time.sleep(ta - now)
-            color = vis = None
+            color = np.empty((h, w), dtype=np.uint32)
+            vis = None
             if i in keep:
-                color = np.zeros((h, w), dtype=np.uint32)
                 vis = np.zeros(((w + 63) // 64) * 64 * ((h + 63) // 64) * 64 * ns, dtype=np.uint64)
                 kept[i] = (color, vis)
             q = server.request(si, cf, w, h, ns, color, vis)
-            bufs[i] = q
+            bufs[i] = (q, color, vis)
             ids[i] = server.submit(q)
 
     t_start = time.perf_counter()
@@ -246,6 +247,7 @@ def run_served(server, requests, keep=None):
         while ids[i] is None:
             time.sleep(0.0002)
         lat.append(server.wait(ids[i]))
+        bufs[i] = None
     th.join()
     wall = time.perf_counter() - t_start
     return np.array(lat), wall, kept
