@@ -136,7 +136,10 @@ void sftx_set_msaa(i32 samples);
    views ...) while it runs.  Two batches can be in flight; submitting a third
    first completes the oldest.  sftx_batch_wait(ticket) completes batches in
    submission order up to that one and returns 0 (-1 for an unknown ticket);
-   until then the batch's colour and vis64 buffers belong to the renderer.
+   until then the batch's colour and vis64 buffers belong to the renderer
+   (they are written as soon as the batch is done, by a thread of the
+   renderer, and must not be read or freed before the wait).  A batch's
+   kernels start after those of the batch submitted before it.
    sft_present, sftx_batch_submit, sftx_synchronize, sftx_server_start and
    sft_release complete any batch still in flight first. */
 void sftx_batch_begin(void);
