@@ -615,6 +615,14 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     /* ---- batch tables: views' rows back to back ---- */
     std::vector<InstGpu> insts;
     std::vector<WorkItem> work;
+    struct WorkFill {
+        const ModelRes* r;
+        const sft_model* m;
+        u32 ss0, inst, view, ginst, first, count;
+        bool posed;
+    };
+    std::vector<WorkFill> work_fills;
+    u32 n_work_total = 0;
     std::vector<f32> bones;
     std::vector<PointLightGpu> pl;
     std::vector<DirLightGpu> dl;
@@ -634,7 +642,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         const ViewRecord& v = views[vi];
         FrameSet::ViewPlan& vp = fs.plan[vi];
         vp.inst_base = (u32)insts.size();
-        vp.work_base = (u32)work.size();
+        vp.work_base = n_work_total;
         vp.pl_base = (u32)pl.size();
         vp.dl_base = (u32)dl.size();
         vp.tmat_base = (u32)tmat_tex.size();
@@ -684,23 +692,15 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
             }
             const u32 ginst = (u32)insts.size();
             insts.push_back(g);
-            for (u32 mi = 0; mi < m->mesh_count; mi += 1) {
-                const sft_mesh* mesh = &m->meshes[mi];
-                for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
-                    WorkItem w;
-                    w.gm = mesh->model_meshlet_offset + k;
-                    w.slot = r.base + w.gm;
-                    w.ss = ss_base + in.ss_meshlet_offset + w.gm;
-                    w.inst = ii;
-                    w.cull_bone = (mesh->culling_bone_inds && in.posed_bone_matrices) ? mesh->culling_bone_inds[k & ~7u] : -1;
-                    w.skin = (mesh->meshlet_skins && in.posed_bone_matrices) ? r.skin_of[w.gm] : kInvalid;
-                    w.view = vi;
-                    w.ginst = ginst;
-                    work.push_back(w);
-                }
-            }
+            /* the instance's work items (one per meshlet, mesh by mesh) are
+               filled below, in parallel, from this record */
+            u32 n = 0;
+            for (u32 mi = 0; mi < m->mesh_count; mi += 1) n += m->meshes[mi].meshlet_count;
+            work_fills.push_back(WorkFill{&r, m, ss_base + in.ss_meshlet_offset, ii, vi, ginst, n_work_total, n,
+                                          nullptr != in.posed_bone_matrices});
+            n_work_total += n;
         }
-        vp.n_work = (u32)work.size() - vp.work_base;
+        vp.n_work = n_work_total - vp.work_base;
         for (const sft_point_light& L : v.point_lights) {
             pl.push_back(PointLightGpu{{L.pos.x, L.pos.y, L.pos.z}, {L.color.x, L.color.y, L.color.z}, L.radius, L.inv_radius_sqr});
         }
@@ -802,6 +802,29 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.vid_base = vp.vid_base;
         g.color_base = vp.color_base;
     }
+    /* work items in the reference's job order: views, their instances, each
+       instance's meshes and meshlets; instances filled by up to 8 threads */
+    work.resize(n_work_total);
+    parallel_runs(work_fills.size(), [&](size_t i) { return (size_t)work_fills[i].count; },
+                  [&](size_t i) {
+                      const WorkFill& f = work_fills[i];
+                      WorkItem* out = work.data() + f.first;
+                      for (u32 mi = 0; mi < f.m->mesh_count; mi += 1) {
+                          const sft_mesh* mesh = &f.m->meshes[mi];
+                          for (u32 k = 0; k < mesh->meshlet_count; k += 1) {
+                              WorkItem w;
+                              w.gm = mesh->model_meshlet_offset + k;
+                              w.slot = f.r->base + w.gm;
+                              w.ss = f.ss0 + w.gm;
+                              w.inst = f.inst;
+                              w.cull_bone = (mesh->culling_bone_inds && f.posed) ? mesh->culling_bone_inds[k & ~7u] : -1;
+                              w.skin = (mesh->meshlet_skins && f.posed) ? f.r->skin_of[w.gm] : kInvalid;
+                              w.view = f.view;
+                              w.ginst = f.ginst;
+                              *out++ = w;
+                          }
+                      }
+                  }, 8, 16384);
     if (work.size() >= (1u << 23)) {
         fprintf(stderr, "fatal sofren2 error: %zu meshlets in one batch exceed the 2^23 emission keys\n", work.size());
         exit(1);

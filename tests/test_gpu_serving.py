@@ -83,3 +83,37 @@ def test_admission_under_a_small_budget():
             srv.close()
         lib.scene_destroy()
         lib.sft_release()
+
+
+def test_visibility_only_for_the_requests_that_ask():
+    """In a mix where every other request asks for 64-bit visibility, those
+    requests get the planes they get alone, the others get their colour only
+    (their visibility is neither written nor downloaded), and every colour
+    equals the request rendered alone."""
+    lib = S.library()
+    lib.sft_init(640, 360, 0)
+    srv = None
+    try:
+        srv = workloads.Server(lib, SPECS, budget_bytes=8 << 30, max_views=8)
+        mix = []
+        for k, (_, _, w, h, ns) in enumerate(REQUESTS):
+            c, v = _buffers(w, h, ns)
+            mix.append((c, v if k % 2 == 0 else None))
+        ids = [srv.submit(srv.request(t, cf, w, h, ns, c, v)) for (t, cf, w, h, ns), (c, v) in zip(REQUESTS, mix)]
+        for i in ids:
+            srv.wait(i)
+        st = srv.stats()
+        print("mix stats", st, flush=True)
+        assert st["max_batch"] > 1
+        for k, ((t, cf, w, h, ns), (c_mix, v_mix)) in enumerate(zip(REQUESTS, mix)):
+            c, v = _buffers(w, h, ns)
+            srv.wait(srv.submit(srv.request(t, cf, w, h, ns, c, v)))
+            print("alone vs mix", k, "visibility" if v_mix is not None else "colour only", flush=True)
+            np.testing.assert_array_equal(c, c_mix)
+            if v_mix is not None:
+                np.testing.assert_array_equal(v, v_mix)
+    finally:
+        if srv is not None:
+            srv.close()
+        lib.scene_destroy()
+        lib.sft_release()
