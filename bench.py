@@ -116,7 +116,7 @@ def cpu_baseline(scene, name, width, height, frames):
 def mixed_batch_leg(lib, steps):
     """64 views over 4 scene instances at mixed resolution and MSAA: one
     batched submission per step, the next batch recorded while the previous
-    one runs (two in flight), against the same views presented one by one
+    one runs (three in flight), against the same views presented one by one
     through sft_present.  Both with the frames left in HBM (device output:
     presents also keep four frames in flight) and with every frame copied to
     its host buffer (the drop-in output)."""
@@ -131,7 +131,7 @@ def mixed_batch_leg(lib, steps):
             lib.synchronize()
             # two batches per frame set: the second submission of a set with
             # the same launch shape captures its HIP graph
-            mb.render_pipelined(4)
+            mb.render_pipelined(6)
             t0 = time.perf_counter()
             lat = np.array([x * 1e3 for x in mb.render_pipelined(steps)])
             batched_s = (time.perf_counter() - t0) / steps

@@ -133,8 +133,8 @@ void sftx_set_msaa(i32 samples);
    Pipelined form: sftx_batch_submit_async() in place of sftx_batch_submit
    launches the batch and returns a ticket (> 0; 0 if nothing was submitted)
    without waiting, so the next batch can be recorded (batch_end, batch_begin,
-   views ...) while it runs.  Two batches can be in flight; submitting a third
-   first completes the oldest.  sftx_batch_wait(ticket) completes batches in
+   views ...) while it runs.  Three batches can be in flight; submitting a
+   fourth first completes the oldest.  sftx_batch_wait(ticket) completes batches in
    submission order up to that one and returns 0 (-1 for an unknown ticket);
    until then the batch's colour and vis64 buffers belong to the renderer
    (they are written as soon as the batch is done, by a thread of the

@@ -176,7 +176,7 @@ struct Backend {
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
     u64 graph_captures = 0;
 
-    /* two batches can be in flight (the serving loop); blocking calls use set 0 */
+    /* frames and batches in flight, one set each; blocking calls use set 0 */
     FrameSet sets[kFrameSets];
 
     /* host copies of the last frame's intermediates (capture) */

@@ -222,13 +222,13 @@ void backend_render(Backend* b, Context* ctx, std::vector<ViewRecord>& views);
 bool backend_async_present(const Backend* b);
 /* frame sets of the backend.  Device-output frames keep up to kFrameSets in
    flight (small frames overlap on the GPU); batches, which fill it alone, up
-   to kBatchesInFlight (sets 0 and 1, so that a caller alternating two view
-   sets replays each set's graph); the request server up to
-   kServerBatchesInFlight (sets 0-2: its host outputs, copy-out and the next
-   batch's tables overlap the GPU work of the other two). */
+   to kBatchesInFlight (sets 0-2: a batch's frame download and copy-out and the
+   next batch's recording and tables overlap the GPU work of the others; a
+   caller cycling three view sets replays each set's graph).  The request
+   server keeps as many. */
 constexpr int kFrameSets = 4;
-constexpr int kBatchesInFlight = 2;
-constexpr int kServerBatchesInFlight = 3;
+constexpr int kBatchesInFlight = 3;
+constexpr int kServerBatchesInFlight = kBatchesInFlight;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
 /* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be

@@ -53,21 +53,23 @@ class MixedBatch:
         self.S.render_batch(self.lib, self.views)
 
     def render_pipelined(self, steps):
-        """`steps` batches with two in flight: batch k + 1 is recorded and
-        launched while batch k runs; outputs alternate between two buffer
-        sets.  Returns each batch's latency (record start to completion, s)."""
+        """`steps` batches with three in flight: batches k + 1 and k + 2 are
+        recorded and launched while batch k runs; outputs cycle through three
+        buffer sets.  Returns each batch's latency (record start to
+        completion, s)."""
         import time
         S = self.S
         if not hasattr(self, "views_b"):
             self.views_b = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=False) for v in self.views]
-        sets = [self.views, self.views_b]
+            self.views_c = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=False) for v in self.views]
+        sets = [self.views, self.views_b, self.views_c]
         pending = []
         lat = []
         for k in range(steps):
             t0 = time.perf_counter()
-            ticket = S.submit_batch_async(self.lib, sets[k & 1])
+            ticket = S.submit_batch_async(self.lib, sets[k % 3])
             pending.append((ticket, t0))
-            if len(pending) == 2:
+            if len(pending) == 3:
                 t, ts = pending.pop(0)
                 self.lib.batch_wait(t)
                 lat.append(time.perf_counter() - ts)

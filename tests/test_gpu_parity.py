@@ -160,8 +160,8 @@ def test_batched_views_match_reference():
 
 
 def test_pipelined_batches_match_blocking_batches():
-    """Batches submitted without waiting (two in flight, a third forcing the
-    oldest to complete, transient dmesh rows in both frame sets) give the same
+    """Batches submitted without waiting (three in flight, a fourth forcing the
+    oldest to complete, transient dmesh rows in every frame set) give the same
     colour and visibility as blocking submissions of the same views."""
     lib = S.library()
     lib.sft_init(640, 360, 0)
@@ -181,13 +181,14 @@ def test_pipelined_batches_match_blocking_batches():
             vs = views_for(shift)
             S.render_batch(lib, vs)
             blocking[shift] = [(v.color.copy(), v.vis.copy()) for v in vs]
-        order = [0, 1, 0]
+        order = [0, 1, 0, 1]
         sets = [views_for(k) for k in order]
         tickets = [S.submit_batch_async(lib, vs) for vs in sets]
         print("tickets", tickets, flush=True)
-        assert lib.batch_wait(tickets[1]) == 0
+        assert lib.batch_wait(tickets[0]) == -1  # completed by the fourth submission
         assert lib.batch_wait(tickets[2]) == 0
-        assert lib.batch_wait(tickets[0]) == -1  # already completed
+        assert lib.batch_wait(tickets[3]) == 0
+        assert lib.batch_wait(tickets[1]) == -1  # completed by the wait for a later ticket
         for k, vs in zip(order, sets):
             for (c, vis), v in zip(blocking[k], vs):
                 np.testing.assert_array_equal(v.color, c)
