@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-view mixed batch (0: skip)")
     ap.add_argument("--served-requests", type=int, default=200,
                     help="requests per offered rate, at least one second of arrivals (0: skip serving)")
-    ap.add_argument("--served-rates", default="100,500,2000", help="offered request rates (req/s)")
+    ap.add_argument("--served-rates", default="100,500,2000,5000", help="offered request rates (req/s)")
     ap.add_argument("--served-scenes", type=int, default=32)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run the streaming copy kernel over this many MiB first (HBM counter calibration)")
