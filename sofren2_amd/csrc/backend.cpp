@@ -1109,10 +1109,10 @@ static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views,
     return spans;
 }
 
-/* up to 16 threads, 4 MB or more each */
+/* up to 16 threads, 1 MB or more each */
 static void copy_spans(const std::vector<Span>& spans) {
     parallel_runs(spans.size(), [&](size_t i) { return spans[i].n; },
-                  [&](size_t i) { memcpy(spans[i].dst, spans[i].src, spans[i].n); }, 16, (size_t)4 << 20);
+                  [&](size_t i) { memcpy(spans[i].dst, spans[i].src, spans[i].n); }, 16, (size_t)1 << 20);
 }
 
 /* Waits for a launched batch; re-runs it with larger intermediates if any
