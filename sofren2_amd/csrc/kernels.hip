@@ -1833,14 +1833,14 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    and add the colour weighted by their share of samples.  Each pixel starts
    from the background colour times its empty-sample share. */
 template <bool kShadows>
-__global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade_ms(FramePtrs P) {
     __shared__ TileShading<kShadows> s_ts;
     __shared__ u32 s_li[kMaxTileLights];
     /* per wave: up to 64 pixels x 4 samples (pixel, triangle) items */
     __shared__ u32 s_item_vid[4][kMsItems];
     __shared__ u32 s_item_meta[4][kMsItems];     /* pixel lane | group start << 8 | group size << 16 | samples << 24 */
-    __shared__ f32 s_item_der[4][4][kMsItems];   /* UV derivatives, at the group's first item */
-    __shared__ f32 s_item_col[4][3][64];         /* shaded colour of the chunk's items */
+    __shared__ f32 s_item_der[4][4][kMsItems];   /* UV derivatives at the group's first item, then each item's colour */
+    __shared__ u32 s_item_count[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
     const u32 run = P.batch->n_shade_tiles + (kShadows ? P.batch->n_shade_tiles_ms - P.batch->n_shade_ms_sh : 0u);
@@ -1963,64 +1963,78 @@ __global__ __launch_bounds__(256) void k_shade_ms(FramePtrs P) {
         }
         n_items += (u32)__popcll(hm);
     }
-    wave_sync();
+    if (0 == lane) s_item_count[wave] = n_items;
+    __syncthreads();
 
-    /* 2. the items shaded 64 at a time; a chunk ends at a group boundary so
-       that a group's lanes meet at the capsule ballots (lanes gs - c0 ...) */
-    u32 next_j = 0;
-    for (u32 c0 = 0; c0 < n_items;) {
-        u32 end = min(n_items, c0 + 64u);
-        if (end < n_items) {
-            const u32 m = it_meta[end - 1u];
-            const u32 gs = (m >> 8) & 0xFFu, gn = (m >> 16) & 0xFFu;
+    /* 2. the workgroup's items (wave 0's, then wave 1's, ...) shaded 64 at a
+       time, chunk k by wave k mod 4; a chunk ends at a group boundary so that
+       a group's lanes meet at the capsule ballots (lanes gs - c0 ...).  An
+       item's colour replaces its UV derivatives in LDS: its group's
+       derivatives are read by the chunk's lanes before any of them writes. */
+    const u32 off1 = s_item_count[0], off2 = off1 + s_item_count[1], off3 = off2 + s_item_count[2];
+    const u32 n_all = off3 + s_item_count[3];
+    /* wave of workgroup item g, and that wave's first item */
+    auto item_wave = [&](u32 g) { return (int)(g >= off1) + (int)(g >= off2) + (int)(g >= off3); };
+    auto wave_first = [&](int w) { return (w == 0) ? 0u : (w == 1) ? off1 : (w == 2) ? off2 : off3; };
+    u32 chunk = 0;
+    for (u32 c0 = 0; c0 < n_all; chunk += 1) {
+        u32 end = min(n_all, c0 + 64u);
+        if (end < n_all) {
+            const u32 g = end - 1u;
+            const int w = item_wave(g);
+            const u32 m = s_item_meta[w][g - wave_first(w)];
+            const u32 gs = wave_first(w) + ((m >> 8) & 0xFFu), gn = (m >> 16) & 0xFFu;
             if (gs + gn > end) end = gs;
         }
-        const u32 item = c0 + (u32)lane;
-        const bool has = item < end;
-        u32 meta = 0;
-        Tri t;
-        Bary b{0.f, 0.f, 0.f};
-        if (has) {
-            meta = it_meta[item];
-            const int pl = (int)(meta & 63u), pix = pl & 7;
-            load_tri(P, VW, it_vid[item], t);
-            b = bary_at(est, t, (f32)(x0 + (pl >> 3) * 4) + 0.5f + (f32)(pix & 3),
-                        (f32)(y0 + wave * 2) + 0.5f + (f32)(pix >> 2), true);
+        if ((int)(chunk & 3u) == wave) {
+            const u32 item = c0 + (u32)lane;
+            const bool has = item < end;
+            const int iw = item_wave(item);
+            const u32 li = item - wave_first(iw);
+            u32 meta = 0;
+            Tri t;
+            Bary b{0.f, 0.f, 0.f};
+            if (has) {
+                meta = s_item_meta[iw][li];
+                const int pl = (int)(meta & 63u), pix = pl & 7;
+                load_tri(P, VW, s_item_vid[iw][li], t);
+                b = bary_at(est, t, (f32)(x0 + (pl >> 3) * 4) + 0.5f + (f32)(pix & 3),
+                            (f32)(y0 + iw * 2) + 0.5f + (f32)(pix >> 2), true);
+            }
+            /* capsules near this chunk's shading positions (every lane runs here) */
+            CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
+            if constexpr (kShadows) {
+                const V3 p = has ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
+                cm = capsule_masks(tl, wave_span(p, has), lane);
+            }
+            if (has) {
+                const u32 gs = (meta >> 8) & 0xFFu, gn = (meta >> 16) & 0xFFu;
+                u32 fetch = 0;
+                const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, s_item_der[iw][0][gs], s_item_der[iw][1][gs],
+                                                     s_item_der[iw][2][gs], s_item_der[iw][3][gs], tl, fetch,
+                                                     (1u << gn) - 1u, (int)(wave_first(iw) + gs - c0), cm);
+                s_item_der[iw][0][li] = col.x;
+                s_item_der[iw][1][li] = col.y;
+                s_item_der[iw][2][li] = col.z;
+                shaded += 1;
+                groups += (li == gs) ? 1 : 0;
+                n_bil += (fetch & 1u) ? 1 : 0;
+                n_near += (fetch & 2u) ? 1 : 0;
+                n_lm += (fetch & 4u) ? 1 : 0;
+            }
         }
-        /* capsules near this chunk's shading positions (every lane runs here) */
-        CapsuleMasks cm{{0ull, 0ull, 0ull, 0ull}};
-        if constexpr (kShadows) {
-            const V3 p = has ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
-            cm = capsule_masks(tl, wave_span(p, has), lane);
-        }
-        if (has) {
-            const u32 gs = (meta >> 8) & 0xFFu, gn = (meta >> 16) & 0xFFu;
-            u32 fetch = 0;
-            const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, s_item_der[wave][0][gs], s_item_der[wave][1][gs],
-                                                 s_item_der[wave][2][gs], s_item_der[wave][3][gs], tl, fetch,
-                                                 (1u << gn) - 1u, (int)(gs - c0), cm);
-            s_item_col[wave][0][lane] = col.x;
-            s_item_col[wave][1][lane] = col.y;
-            s_item_col[wave][2][lane] = col.z;
-            shaded += 1;
-            groups += (item == gs) ? 1 : 0;
-            n_bil += (fetch & 1u) ? 1 : 0;
-            n_near += (fetch & 2u) ? 1 : 0;
-            n_lm += (fetch & 4u) ? 1 : 0;
-        }
-        wave_sync();
-        /* each pixel adds its items of the chunk, in its triangle order,
-           weighted by their share of samples */
-        for (; next_j < my_items; next_j += 1) {
-            const u32 idx = (my_idx >> (8u * next_j)) & 0xFFu;
-            if (idx >= end) break;
-            const f32 w = (f32)(it_meta[idx] >> 24) * inv_s;
-            acc_r = fmaf(s_item_col[wave][0][idx - c0], w, acc_r);
-            acc_g = fmaf(s_item_col[wave][1][idx - c0], w, acc_g);
-            acc_b = fmaf(s_item_col[wave][2][idx - c0], w, acc_b);
-        }
-        wave_sync();
         c0 = end;
+    }
+    __syncthreads();
+
+    /* 3. each pixel adds its triangles' colours in order, weighted by their
+       share of samples */
+    for (u32 j = 0; j < my_items; j += 1) {
+        const u32 idx = (my_idx >> (8u * j)) & 0xFFu;
+        const f32 w = (f32)(it_meta[idx] >> 24) * inv_s;
+        acc_r = fmaf(s_item_der[wave][0][idx], w, acc_r);
+        acc_g = fmaf(s_item_der[wave][1][idx], w, acc_g);
+        acc_b = fmaf(s_item_der[wave][2][idx], w, acc_b);
     }
     if (write) *out = pack_argb(smin(acc_r, 1.f) * 255.f, smin(acc_g, 1.f) * 255.f, smin(acc_b, 1.f) * 255.f);
 
@@ -2405,7 +2419,6 @@ __global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
                     rec.rb2 = br * br;
                     rec.pr2 = px.radius * px.radius;
                     rec.vis_mult = (f32)vis * 0.3333333f;
-                    rec.pad[0] = rec.pad[1] = 0.f;
                 }
             }
         }

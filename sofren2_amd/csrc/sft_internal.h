@@ -104,13 +104,15 @@ struct ProxyGpu {
     f32 a[3], b[3], radius, pad;
 };
 
-/* per (light, proxy) shading constants (sofren2.c:5960-5985, 991-998) */
+/* per (light, proxy) shading constants (sofren2.c:5960-5985, 991-998); 56 B,
+   unpadded, so that a tile's 16 x 16 records stay small in LDS */
 struct ProxyRec {
     f32 pa[3], abdab;
     f32 ab[3], wdw;
     f32 w[3], rb2;
-    f32 pr2, vis_mult, pad[2];
+    f32 pr2, vis_mult;
 };
+static_assert(sizeof(ProxyRec) == 56, "ProxyRec layout");
 constexpr int kMaxLightProxies = 16;
 
 struct WorkItem {
