@@ -1832,6 +1832,8 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    so the LOD lane's neighbours are at hand, the lanes covering it shade once
    and add the colour weighted by their share of samples.  Each pixel starts
    from the background colour times its empty-sample share. */
+/* four waves per SIMD: the LDS below (about 39 KB with capsule records)
+   admits four workgroups per CU, and the register budget is held to match */
 template <bool kShadows>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade_ms(FramePtrs P) {
     __shared__ TileShading<kShadows> s_ts;
