@@ -504,9 +504,10 @@ extern "C" void sftx_batch_submit(void) {
 }
 
 /* The recorded views as a batch on a free frame set, without waiting: the
-   host can record the next batch while this one runs.  Two batches can be in
-   flight; a third submission completes the oldest first.  Output buffers stay
-   the caller's until sftx_batch_wait returns for the ticket. */
+   host can record the next batches while this one runs.  kBatchesInFlight
+   batches can be in flight; one more submission completes the oldest first.
+   Output buffers belong to the renderer until sftx_batch_wait returns for the
+   ticket. */
 extern "C" i32 sftx_batch_submit_async(void) {
     Context& c = C();
     if (!c.batch_open || c.batch_views.empty()) return 0;

@@ -233,8 +233,9 @@ constexpr int kBatchesInFlight = 3;
 constexpr int kServerBatchesInFlight = kBatchesInFlight;
 /* completes every batch of sftx_batch_submit_async still in flight */
 void drain_async_batches(Context& c);
-/* asynchronous halves for a serving loop: two batches (sets 0 and 1) can be
-   in flight; complete blocks until its batch is done.  An ordered submission
+/* asynchronous halves for batches and the serving loop: up to
+   kBatchesInFlight batches (sets 0-2) can be in flight; complete blocks until
+   its batch is done and its host outputs are written.  An ordered submission
    starts its kernels after the previous ordered one's (its tables upload
    meanwhile): large batches then run back to back rather than sharing the
    GPU, and the one waited for ends first. */
