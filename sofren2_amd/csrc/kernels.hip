@@ -1307,9 +1307,16 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
             if (kShadows && near_caps != 0u) {
                 inv_d2 = est.rcp_refined(d2);
                 const bool gate = lightmapped || lit;
-                for (u32 rest = near_caps; rest != 0u; rest &= rest - 1u) {
-                    const u32 k = (u32)__ffs(rest) - 1u;
-                    if (0ull != __ballot(ray_near(ts.prox[i][k], lraw, d2, inv_d2) & gate)) hit_caps |= 1u << k;
+                /* two capsules per step, their loads issued together */
+                for (u32 rest = near_caps; rest != 0u;) {
+                    const u32 k0 = (u32)__ffs(rest) - 1u;
+                    rest &= rest - 1u;
+                    const u32 k1 = (0u != rest) ? (u32)__ffs(rest) - 1u : k0;
+                    rest &= rest - 1u;
+                    const bool n0 = ray_near(ts.prox[i][k0], lraw, d2, inv_d2) & gate;
+                    const bool n1 = ray_near(ts.prox[i][k1], lraw, d2, inv_d2) & gate;
+                    if (0ull != __ballot(n0)) hit_caps |= 1u << k0;
+                    if (0ull != __ballot(n1)) hit_caps |= 1u << k1;
                 }
             }
             /* no capsule reached: a lightmapped lane keeps lm_shadow exactly (its
