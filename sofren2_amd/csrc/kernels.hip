@@ -31,9 +31,12 @@
  *   k_tile_lights light lists of MSAA tiles over every sample plane.
  *   k_shade       32x8 pixels per workgroup, lane = pixel in the reference's
  *                 4x2 block order: interpolation, LOD, texturing, lighting,
- *                 capsule shadows, ARGB pack (sofren2.c:5809-6753).
+ *                 capsule shadows (capsules culled per wave), ARGB pack
+ *                 (sofren2.c:5809-6753); specialised for views with and
+ *                 without capsule shadows.
  *   k_shade_ms    the same over sample planes: per block, triangles in first
- *                 (sample, lane) order, weighted accumulate.
+ *                 (sample, lane) order; (pixel, triangle) items shaded 64 at
+ *                 a time; weighted accumulate in each pixel's triangle order.
  *   k_stats       folds the spread statistics counters.
  *
  * Bit-exactness: every float operation of the reference is replayed in the
@@ -1838,9 +1841,12 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
    the block evaluates the triangle at its pixel centre (clamped barycentrics)
    so the LOD lane's neighbours are at hand, the lanes covering it shade once
    and add the colour weighted by their share of samples.  Each pixel starts
-   from the background colour times its empty-sample share. */
-/* four waves per SIMD: the LDS below (about 39 KB with capsule records)
-   admits four workgroups per CU, and the register budget is held to match */
+   from the background colour times its empty-sample share.  The shading
+   itself runs over a compacted list of (pixel, triangle) items, pooled over
+   the workgroup's four waves, so that lanes whose pixel is done do not idle
+   through another pixel's extra triangles.  Four waves per SIMD: the LDS
+   below (about 39 KB with capsule records) admits four workgroups per CU,
+   and the register budget is held to match. */
 template <bool kShadows>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade_ms(FramePtrs P) {
     __shared__ TileShading<kShadows> s_ts;
