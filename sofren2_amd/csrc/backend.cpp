@@ -15,6 +15,9 @@
 #include <string.h>
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include "kernels.h"
 
@@ -320,6 +323,78 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
     }
 }
 
+/* Worker threads kept for the host copies and row fills of every frame (a
+   thread started per call costs more than a small frame's copy).  One caller
+   at a time owns the pool; a second, concurrent caller starts threads of its
+   own.  run(k, fn) calls fn(0) on the caller and fn(w + 1) on worker w for
+   w + 1 < k, and returns when all are done; a worker takes no part in a job
+   after its own index, so no job outlives its call. */
+class WorkerPool {
+public:
+    static WorkerPool& get() {
+        static WorkerPool pool;
+        return pool;
+    }
+    size_t size() const { return workers_.size(); }
+    bool try_acquire() { return owner_.try_lock(); }
+    void release() { owner_.unlock(); }
+    void run(size_t k, const std::function<void(size_t)>& fn) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &fn;
+            n_ = k;
+            pending_ = k - 1;
+            gen_ += 1;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return 0 == pending_; });
+        job_ = nullptr;
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread& t : workers_) t.join();
+    }
+
+private:
+    WorkerPool() {
+        const size_t hw = std::max<size_t>(1u, std::thread::hardware_concurrency());
+        for (size_t t = 0; t + 1 < std::min<size_t>(hw, 16u); t += 1) workers_.emplace_back([this, t] { loop(t); });
+    }
+    void loop(size_t id) {
+        u64 seen = 0;
+        for (;;) {
+            const std::function<void(size_t)>* job;
+            size_t n;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                job = job_;
+                n = n_;
+            }
+            if (id + 1 < n) {
+                (*job)(id + 1);
+                std::lock_guard<std::mutex> lk(mu_);
+                if (0 == --pending_) done_.notify_all();
+            }
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex owner_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)>* job_ = nullptr;
+    size_t n_ = 0, pending_ = 0;
+    u64 gen_ = 0;
+    bool stop_ = false;
+};
+
 /* fn(i) for i < n on up to max_threads threads (the calling one among them),
    in contiguous runs of about equal weight, each at least min_weight */
 template <class W, class F>
@@ -342,6 +417,12 @@ static void parallel_runs(size_t n, W weight, F fn, size_t max_threads, size_t m
     auto run = [&](size_t t) {
         for (size_t i = cut[t]; i < cut[t + 1]; i += 1) fn(i);
     };
+    WorkerPool& pool = WorkerPool::get();
+    if (nt <= pool.size() + 1 && pool.try_acquire()) {
+        pool.run(nt, std::function<void(size_t)>(run));
+        pool.release();
+        return;
+    }
     std::vector<std::thread> workers;
     for (size_t t = 1; t < nt; t += 1) workers.emplace_back(run, t);
     run(0);
