@@ -331,9 +331,11 @@ static void fill_model_rows(const sft_model* m, const ModelRes& r, u32 row0, u32
    after its own index, so no job outlives its call. */
 class WorkerPool {
 public:
+    /* never destroyed: a batch's copier may still use it while the process
+       exits, and its blocked workers end with the process */
     static WorkerPool& get() {
-        static WorkerPool pool;
-        return pool;
+        static WorkerPool* pool = new WorkerPool();
+        return *pool;
     }
     size_t size() const { return workers_.size(); }
     bool try_acquire() { return owner_.try_lock(); }
@@ -352,15 +354,6 @@ public:
         done_.wait(lk, [&] { return 0 == pending_; });
         job_ = nullptr;
     }
-    ~WorkerPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (std::thread& t : workers_) t.join();
-    }
-
 private:
     WorkerPool() {
         const size_t hw = std::max<size_t>(1u, std::thread::hardware_concurrency());
@@ -373,8 +366,7 @@ private:
             size_t n;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
+                cv_.wait(lk, [&] { return gen_ != seen; });
                 seen = gen_;
                 job = job_;
                 n = n_;
@@ -392,7 +384,6 @@ private:
     const std::function<void(size_t)>* job_ = nullptr;
     size_t n_ = 0, pending_ = 0;
     u64 gen_ = 0;
-    bool stop_ = false;
 };
 
 /* fn(i) for i < n on up to max_threads threads (the calling one among them),
