@@ -167,10 +167,13 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
     srv = workloads.Server(lib, specs, budget_bytes=200 << 30, max_views=64)
     out = {"scenes": n_scenes, "budget_gb": 200, "rates": []}
     try:
-        # warm-up: make every template resident
+        # warm-up: make every template resident, then one burst of full
+        # batches so that the frame sets' buffers reach their working size
         warm = [srv.submit(srv.request(i, 0, 256, 256, 1)) for i in range(n_scenes)]
         for r in warm:
             srv.wait(r)
+        burst = [(0.0,) + q[1:] for q in workloads.served_requests(3 * 64, 1000.0, n_scenes, seed=99)]
+        workloads.run_served(srv, burst)
         sample, last_reqs = [], []
         for k, rate in enumerate(rates):
             # at least a second of arrivals, so the last batches' latency
