@@ -110,7 +110,16 @@ struct FramePtrs {
 /* enqueues the kernels of one batch of views, starting with the clear of the
    `zero_words` counter words at p.counters; `ev` (10 events) brackets the
    passes when non-null */
-void launch_frame(const FramePtrs& p, const BatchGpu& batch, u64 zero_words, hipStream_t s, hipEvent_t* ev);
+/* `side` (optional, eager launches only): a second stream and its events
+   (fork, join, start, end) on which the shadow-proxy occlusion and capsule
+   lists run beside geometry, binning and rasterisation, joined before the
+   shading kernels */
+struct SideStream {
+    hipStream_t stream;
+    hipEvent_t fork, join, start, end;
+};
+void launch_frame(const FramePtrs& p, const BatchGpu& batch, u64 zero_words, hipStream_t s, hipEvent_t* ev,
+                  const SideStream* side = nullptr);
 void launch_copy(const uint4* src, uint4* dst, u64 n16, hipStream_t s);
 /* builds texture `m`'s swizzled mip chain in `heap` (offsets in m are heap
    texel offsets, padding already zeroed) from its linear level-0 pixels */

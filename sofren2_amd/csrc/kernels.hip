@@ -2533,7 +2533,8 @@ u32 raster_groups_per_device() {
     return (u32)std::max(cus, 1) * (u32)std::max(per_cu, 1);
 }
 
-void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStream_t s, hipEvent_t* ev) {
+void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStream_t s, hipEvent_t* ev,
+                  const SideStream* side) {
     auto mark = [&](int k) { if (ev) hipEventRecord(ev[k], s); };
     hipLaunchKernelGGL(k_zero, dim3((u32)std::min<u64>((zero_words + 255u) / 256u, 1024u)), dim3(256), 0, s, p.counters,
                        zero_words);
@@ -2547,9 +2548,23 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
     const u32 nw = B.n_work;
     const u32 geom_blocks = (nw + 3u) / 4u;
     mark(1);
-    /* shadow-proxy visibility and the per-light capsule lists */
-    if (B.occ_total > 0u) hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, s, p);
-    if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, s, p);
+    /* shadow-proxy visibility and the per-light capsule lists: on the side
+       stream when there are rays to trace (their long, narrow traversal then
+       overlaps the passes below), else in order */
+    const bool fork = side && B.occ_total > 0u;
+    if (fork) {
+        hipEventRecord(side->fork, s);
+        hipStreamWaitEvent(side->stream, side->fork, 0);
+        hipEventRecord(side->start, side->stream);
+        hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, side->stream, p);
+        if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, side->stream, p);
+        hipEventRecord(side->end, side->stream);
+        hipEventRecord(side->join, side->stream);
+    }
+    else {
+        if (B.occ_total > 0u) hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, s, p);
+        if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, s, p);
+    }
     mark(7);
     if (nw > 0) {
         hipLaunchKernelGGL(k_geom, dim3(geom_blocks), dim3(256), 0, s, p);
@@ -2565,6 +2580,7 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
     hipLaunchKernelGGL(k_raster, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
     if (B.max_samples > 1u) hipLaunchKernelGGL(k_tile_lights, dim3(B.n_tiles), dim3(64), 0, s, p);
     mark(6);
+    if (fork) hipStreamWaitEvent(s, side->join, 0);
     const u32 plain = B.n_shade_tiles - B.n_shade_sh, plain_ms = B.n_shade_tiles_ms - B.n_shade_ms_sh;
     if (plain) hipLaunchKernelGGL(k_shade<false>, dim3(plain * 16u), dim3(256), 0, s, p);
     if (B.n_shade_sh) hipLaunchKernelGGL(k_shade<true>, dim3(B.n_shade_sh * 16u), dim3(256), 0, s, p);
