@@ -124,6 +124,15 @@ def mixed_batch_leg(lib, steps):
     mb = MixedBatch(lib)
     out = {"views": len(mb.views), "megapixels_per_batch": round(mb.pixels() / 1e6, 3)}
     try:
+        # the autotuner picks the visibility work split per resolution class
+        # on this view set before anything is timed; both legs use its choice
+        lib.set_output(0)
+        before = mb.S.Tuning()
+        lib.get_tuning(before)
+        out["tuning_default"] = before.as_dict()
+        t0 = time.perf_counter()
+        out["tuning"] = mb.tune().as_dict()
+        out["tuning"]["tune_s"] = round(time.perf_counter() - t0, 2)
         for mode, tag in ((1, "device"), (0, "host")):
             lib.set_output(mode)
             mb.render_batched()

@@ -193,6 +193,35 @@ void sftx_server_release(void);                        /* stops and drops the te
    of a frame is captured once and replayed while launch sizes and buffers are
    unchanged.  sftx_graph_captures counts the captures made so far. */
 void sftx_set_graphs(i32 enabled);
+
+/* Autotuner.  No counterpart in the reference (its tiles and job sizes are
+   compile-time constants, sofren2.c:124-130); it exists for batches and the
+   request server, whose views span many resolutions.  sftx_batch_tune renders
+   the recorded batch like sftx_batch_submit (blocking; outputs written), after
+   timing candidate settings on it: per resolution class (views below 512x512
+   pixels, below 1024x1024, larger) the payloads per visibility work item
+   (16-128; the item's triangles, 8 per payload, are staged in LDS), then the
+   persistent visibility grid (half, one or two occupancy-sized grids).  The
+   chosen settings apply to every later sft_present, batch and server batch;
+   no setting changes an output bit.  The bin tile stays 64 px: the
+   reference's edge accumulation starts at the 64-px tile's clamped bounding
+   box (sofren2.c:5148-5499), so another tile size would change depth and
+   visibility bits.  The occlusion traversal stack has a fixed depth
+   (occlusion_stack entries per wave) that no BVH the builders make can fill;
+   a packet that would overflow it is counted in sftx_frame_stats. */
+#define SFTX_RES_CLASSES 3
+typedef struct sftx_tuning {
+    u32 raster_chunk[SFTX_RES_CLASSES];   /* payloads per visibility work item, by class */
+    u32 raster_groups;                    /* persistent visibility workgroups */
+    u32 raster_groups_resident;           /* workgroups resident at once on this device */
+    u32 tile_px;                          /* bin / raster tile side (fixed) */
+    u32 occlusion_stack;                  /* traversal stack entries per wave (fixed) */
+    f32 ms_class[SFTX_RES_CLASSES];       /* tuned visibility-pass time of each class's views (0: none) */
+    f32 ms_batch;                         /* tuned visibility-pass time of the whole batch */
+} sftx_tuning;
+i32 sftx_batch_tune(i32 repeats, sftx_tuning* report);   /* 0, or -1 without a recorded batch */
+void sftx_get_tuning(sftx_tuning* out);
+void sftx_set_tuning(const sftx_tuning* in);
 u64 sftx_graph_captures(void);
 
 /* Waits for all GPU work of the renderer. */

@@ -43,6 +43,20 @@ class FrameStats(ctypes.Structure):
         "ms_total", "ms_shadows")]
 
 
+class Tuning(ctypes.Structure):
+    """sftx_tuning: the autotuner's settings and timings."""
+    _fields_ = [("raster_chunk", ctypes.c_uint32 * 3), ("raster_groups", ctypes.c_uint32),
+                ("raster_groups_resident", ctypes.c_uint32), ("tile_px", ctypes.c_uint32),
+                ("occlusion_stack", ctypes.c_uint32), ("ms_class", ctypes.c_float * 3), ("ms_batch", ctypes.c_float)]
+
+    def as_dict(self):
+        return {"raster_chunk_by_class": list(self.raster_chunk), "raster_groups": self.raster_groups,
+                "raster_groups_resident": self.raster_groups_resident, "tile_px": self.tile_px,
+                "occlusion_stack": self.occlusion_stack,
+                "visibility_ms_by_class": [round(x, 4) for x in self.ms_class],
+                "visibility_ms_batch": round(self.ms_batch, 4)}
+
+
 class Request(ctypes.Structure):
     """sftx_request: a render request against a scene template."""
     _fields_ = [("scene", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
@@ -130,6 +144,9 @@ class Library:
         self.texture_release = _bind(g, "sft_texture_release", None, [u32])
         self.cap_texture = _bind(g, "sftx_cap_texture", ctypes.c_int64, [u32, vp, ctypes.c_uint64])
         self.batch_wait = _bind(g, "sftx_batch_wait", ctypes.c_int32, [ctypes.c_int32])
+        self.batch_tune = _bind(g, "sftx_batch_tune", i32, [i32, ctypes.POINTER(Tuning)])
+        self.get_tuning = _bind(g, "sftx_get_tuning", None, [ctypes.POINTER(Tuning)])
+        self.set_tuning = _bind(g, "sftx_set_tuning", None, [ctypes.POINTER(Tuning)])
         self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
 
 
@@ -172,6 +189,24 @@ def render_batch(lib, views):
     finally:
         lib.batch_end()
     return views
+
+
+def tune_batch(lib, views, repeats=3):
+    """Records every view and lets the autotuner choose its settings on them
+    (sftx_batch_tune); the views are rendered as by render_batch.  Returns
+    the chosen settings."""
+    t = Tuning()
+    lib.batch_begin()
+    try:
+        for v in views:
+            lib.view_begin(v.width, v.height, v.samples)
+            lib.scene_draw(v.scene, v.frame, v.width, v.height)
+            lib.view_end(v.color.ctypes.data, v.width * 4, v.vis.ctypes.data if v.vis is not None else None)
+        if lib.batch_tune(repeats, ctypes.byref(t)) != 0:
+            raise RuntimeError("autotuning failed")
+    finally:
+        lib.batch_end()
+    return t
 
 
 def submit_batch_async(lib, views):

@@ -174,7 +174,9 @@ struct Backend {
 
     /* settings */
     i32 output_mode = SFTX_OUTPUT_HOST;
-    u32 raster_chunk = 128;         /* payloads per visibility work item */
+    /* payloads per visibility work item, by the view's resolution class
+       (sftx_batch_tune chooses them) */
+    u32 raster_chunk[SFTX_RES_CLASSES] = { 128, 128, 128 };
     int last_set = 0;               /* frame set of the last completed batch */
     int ordered_set = -1;           /* frame set of the last ordered submission */
     u32 raster_groups = 512;        /* persistent visibility workgroups */
@@ -596,6 +598,12 @@ static void run_frame(Backend* b, FrameSet& fs, const FramePtrs& p) {
     for (int k : { 7, 2, 3, 6, 4 }) HIP_CHECK(hipEventRecord(fs.ev[k], s));
 }
 
+/* resolution class of a view: below 512x512 pixels, below 1024x1024, larger */
+int res_class(i32 width, i32 height) {
+    const i64 px = (i64)width * height;
+    return (px < (1 << 18)) ? 0 : (px < (1 << 20)) ? 1 : 2;
+}
+
 static bool is_transient(const ViewRecord& v, const sft_model* m) {
     return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
 }
@@ -718,7 +726,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     u64 bin_m_words = 0;
     fs.vgpu.assign(nv, ViewGpu{});
     fs.plan.assign(nv, FrameSet::ViewPlan{});
-    u32 ss_total = 0, max_samples = 1, occ_total = 0;
+    u32 ss_total = 0, max_samples = 1, occ_total = 0, min_chunk = kRasterMaxChunk;
     u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     bool any_vis64 = false, host_color = false;
     for (u32 vi = 0; vi < nv; vi += 1) {
@@ -877,7 +885,9 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         }
         bin_m_words += (u64)(n_bseg + 1u) * vp.n_tiles;
         g.vis64 = v.vis64 ? 1u : 0u;
-        g.pad1 = g.pad2 = 0;
+        g.raster_chunk = std::max(1u, std::min(b->raster_chunk[res_class(v.width, v.height)], kRasterMaxChunk));
+        min_chunk = std::min(min_chunk, g.raster_chunk);
+        g.pad2 = 0;
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
             vis_stage_total += vp.grid;
@@ -950,7 +960,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.occ_total = occ_total;
     bg.n_lights = n_lights_cache;
     bg.vis64 = any_vis64 ? 1u : 0u;
-    bg.raster_chunk = std::max(1u, std::min(b->raster_chunk, kRasterMaxChunk));
+    bg.raster_chunk = min_chunk;   /* sizes the work-item list */
     bg.raster_groups = b->raster_groups;
     bg.n_segs = (u32)segs.size();
     fs.bin_m_words = bin_m_words;
@@ -1343,6 +1353,94 @@ bool backend_batch_done(Backend* b, int set) {
 }
 
 void backend_forget_model(Backend* b, const sft_model* m) { b->models.erase(m); }
+
+void backend_get_tuning(const Backend* b, sftx_tuning* t) {
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) t->raster_chunk[k] = b->raster_chunk[k];
+    t->raster_groups = b->raster_groups;
+    t->raster_groups_resident = raster_groups_per_device();
+    t->tile_px = SFT_TILE_SIZE;
+    t->occlusion_stack = kOccStackEntries;
+}
+
+void backend_set_tuning(Backend* b, const sftx_tuning* t) {
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) {
+        b->raster_chunk[k] = std::max(1u, std::min(t->raster_chunk[k], kRasterMaxChunk));
+    }
+    b->raster_groups = std::max(1u, std::min(t->raster_groups, 4u * raster_groups_per_device()));
+}
+
+/* Autotuner (blocking, on frame set 0).  The knobs leave every output bit
+   unchanged and change only how the visibility pass is cut into work: the
+   payloads per work item of each resolution class (the item's triangles are
+   staged in LDS, up to 8 per payload; fewer payloads per item spread a dense
+   tile over more workgroups and merge more planes in HBM) and the size of
+   the persistent visibility grid.  Each candidate renders the views of the
+   class being tuned `repeats` times with eager launches; the objective is the
+   smallest visibility-pass time (k_raster_plan + k_raster, HIP events).
+   Classes are tuned one after the other, then the grid size on the whole
+   batch, which is rendered once more with the chosen settings so its
+   outputs are written as by a blocking submit. */
+void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int repeats, sftx_tuning* report) {
+    if (views.empty()) return;
+    repeats = std::max(1, std::min(repeats, 64));
+    const bool graphs = b->use_graphs;
+    b->use_graphs = false;
+    auto measure = [&](std::vector<ViewRecord>& vs) {
+        f32 best = 3.4e38f;
+        for (int r = 0; r < repeats; r += 1) {
+            backend_render(b, cp, vs);
+            best = std::min(best, cp->stats.ms_raster);
+        }
+        return best;
+    };
+    static const u32 chunks[] = { 16, 32, 64, 96, 128 };
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) {
+        report->ms_class[k] = 0.f;
+        std::vector<ViewRecord> sub;
+        for (const ViewRecord& v : views) {
+            if (res_class(v.width, v.height) == k) sub.push_back(v);
+        }
+        if (sub.empty()) continue;
+        u32 best_chunk = b->raster_chunk[k];
+        f32 best_ms = measure(sub);
+        for (u32 ch : chunks) {
+            if (ch == best_chunk) continue;
+            const u32 keep = b->raster_chunk[k];
+            b->raster_chunk[k] = ch;
+            const f32 ms = measure(sub);
+            b->raster_chunk[k] = keep;
+            /* a candidate replaces the current choice only when clearly faster */
+            if (ms < best_ms * 0.97f) {
+                best_ms = ms;
+                best_chunk = ch;
+            }
+        }
+        b->raster_chunk[k] = best_chunk;
+        report->ms_class[k] = best_ms;
+    }
+    const u32 resident = raster_groups_per_device();
+    u32 best_groups = b->raster_groups;
+    f32 best_ms = measure(views);
+    for (u32 g : { resident / 2u, resident, 2u * resident }) {
+        if (g == best_groups || g == 0u) continue;
+        const u32 keep = b->raster_groups;
+        b->raster_groups = g;
+        const f32 ms = measure(views);
+        b->raster_groups = keep;
+        if (ms < best_ms * 0.97f) {
+            best_ms = ms;
+            best_groups = g;
+        }
+    }
+    b->raster_groups = best_groups;
+    b->use_graphs = graphs;
+    backend_render(b, cp, views);
+    report->ms_batch = best_ms;
+    const f32 mc[SFTX_RES_CLASSES] = { report->ms_class[0], report->ms_class[1], report->ms_class[2] };
+    backend_get_tuning(b, report);
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) report->ms_class[k] = mc[k];
+    report->ms_batch = best_ms;
+}
 
 u64 backend_resident_bytes(const Backend* b) {
     u64 n = 0;

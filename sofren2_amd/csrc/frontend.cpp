@@ -503,6 +503,40 @@ extern "C" void sftx_batch_submit(void) {
     backend_render(c.backend, &c, c.batch_views);
 }
 
+/* sftx_batch_submit after the autotuner has timed candidate settings on the
+   recorded views (backend_tune) */
+extern "C" i32 sftx_batch_tune(i32 repeats, sftx_tuning* report) {
+    Context& c = C();
+    if (!c.batch_open || c.batch_views.empty()) return -1;
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sftx_batch_tune while the request server runs\n");
+        return -1;
+    }
+    if (!c.backend) c.backend = backend_create();
+    drain_async_batches(c);
+    sftx_tuning local{};
+    backend_tune(c.backend, &c, c.batch_views, repeats, report ? report : &local);
+    return 0;
+}
+
+extern "C" void sftx_get_tuning(sftx_tuning* out) {
+    Context& c = C();
+    if (!c.backend) c.backend = backend_create();
+    *out = sftx_tuning{};
+    backend_get_tuning(c.backend, out);
+}
+
+extern "C" void sftx_set_tuning(const sftx_tuning* in) {
+    Context& c = C();
+    if (!c.backend) c.backend = backend_create();
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sftx_set_tuning while the request server runs\n");
+        return;
+    }
+    drain_async_batches(c);
+    backend_set_tuning(c.backend, in);
+}
+
 /* The recorded views as a batch on a free frame set, without waiting: the
    host can record the next batches while this one runs.  kBatchesInFlight
    batches can be in flight; one more submission completes the oldest first.

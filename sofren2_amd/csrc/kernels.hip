@@ -1486,7 +1486,7 @@ __global__ __launch_bounds__(256) void k_raster_plan(FramePtrs P) {
     __shared__ u32 s_at;
     const u32 tile = blockIdx.x;
     const ViewGpu& V = P.views[P.tile_view[tile]];
-    const u32 S = (u32)V.fc.samples, K = P.batch->raster_chunk, maxS = P.batch->max_samples;
+    const u32 S = (u32)V.fc.samples, K = min(max(V.raster_chunk, 1u), kRasterMaxChunk), maxS = P.batch->max_samples;
     u32 off;
     const u32 cnt = tile_payloads(P, tile, off);
     const u32 nch = max(1u, (cnt + K - 1u) / K);
@@ -1627,7 +1627,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Est est = make_est(P);
     const u32 n_items = P.counters[CNT_RASTER_ITEMS];
-    const u32 K = min(P.batch->raster_chunk, kRasterMaxChunk), maxS = P.batch->max_samples;
+    const u32 maxS = P.batch->max_samples;
     for (;;) {
         if (0 == tid) s_item = atomicAdd(&P.counters[CNT_RASTER_NEXT], 1u);
         for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
@@ -1643,6 +1643,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
         const u32 local = tile - V.tile_base;
         const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
+        const u32 K = min(max(V.raster_chunk, 1u), kRasterMaxChunk);
 
         u32 off;
         const u32 cnt = tile_payloads(P, tile, off);
@@ -2081,7 +2082,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
    time against the 8 rays (lane = triangle x ray).  A ray's result is the OR
    over every triangle the frustum reaches, so the visiting order (the
    reference's depth-first stack) does not change it. */
-constexpr int kOccStack = 1024;
+constexpr int kOccStack = (int)kOccStackEntries;
 constexpr int kOccWaves = 3;
 
 struct Frustum {

@@ -61,7 +61,8 @@ struct ViewGpu {
     u32 tlas_base, occ_base, occ_count, seg_base;
     u32 n_segs;         /* binning segments of the view, its clip segment last */
     u32 vis64;          /* 1: the view's 64-bit visibility planes are written */
-    u32 pad1, pad2;
+    u32 raster_chunk;   /* payloads per visibility work item (the view's resolution class) */
+    u32 pad2;
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -246,6 +247,11 @@ u64 backend_resident_bytes(const Backend* b);
 /* forgets a model's residency (its slots stay allocated) */
 void backend_forget_model(Backend* b, const sft_model* m);
 void backend_release_texture(Backend* b, u32 index);
+/* resolution class of a view (sftx_tuning) */
+int res_class(i32 width, i32 height);
+void backend_get_tuning(const Backend* b, sftx_tuning* t);
+void backend_set_tuning(Backend* b, const sftx_tuning* t);
+void backend_tune(Backend* b, Context* ctx, std::vector<ViewRecord>& views, int repeats, sftx_tuning* report);
 
 /* host estimate tables for _mm_rcp_ps / _mm_rsqrt_ps (rcp_probe.cpp) */
 struct EstimateTables {

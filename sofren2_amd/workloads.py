@@ -52,6 +52,10 @@ class MixedBatch:
     def render_batched(self):
         self.S.render_batch(self.lib, self.views)
 
+    def tune(self, repeats=3):
+        """The autotuner's settings for this view set (kept by the renderer)."""
+        return self.S.tune_batch(self.lib, self.views, repeats)
+
     def render_pipelined(self, steps):
         """`steps` batches with three in flight: batches k + 1 and k + 2 are
         recorded and launched while batch k runs; outputs cycle through three

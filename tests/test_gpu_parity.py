@@ -159,6 +159,60 @@ def test_batched_views_match_reference():
         lib.sft_release()
 
 
+def test_tuned_settings_change_no_output_bit():
+    """The autotuner's candidates only re-cut the visibility pass into work
+    items: the tuned batch, and batches under extreme settings (one payload
+    per work item, half an occupancy-sized grid), give the default settings'
+    colour and visibility bit for bit, in every resolution class."""
+    lib = S.library()
+    lib.sft_init(640, 360, 0)
+    try:
+        handles = {}
+        spec = BATCH_VIEWS + [(S.SCENE_BOX_ROOM, 1, 1280, 832, 40, 1), (S.SCENE_CROWD_SHADOWS, 3, 1408, 768, 3, 2)]
+        for scene, seed, w, h, frame, samples in spec:
+            if (scene, seed) not in handles:
+                handles[(scene, seed)] = lib.scene_new(scene, seed, w, h)
+                assert handles[(scene, seed)] >= 0
+
+        def views():
+            return [S.View(handles[(sc, sd)], fr, w, h, ns) for sc, sd, w, h, fr, ns in spec]
+
+        base = views()
+        S.render_batch(lib, base)
+        default = S.Tuning()
+        lib.get_tuning(default)
+        assert list(default.raster_chunk) == [128, 128, 128] and default.tile_px == 64
+        tuned = views()
+        t = S.tune_batch(lib, tuned, repeats=2)
+        print("tuned", t.as_dict(), flush=True)
+        assert all(c in (16, 32, 64, 96, 128) for c in t.raster_chunk)
+        assert all(ms > 0 for ms in t.ms_class), "every class has views here"
+        g = t.raster_groups_resident
+        assert t.raster_groups in (g // 2, g, 2 * g)
+        got = S.Tuning()
+        lib.get_tuning(got)
+        assert list(got.raster_chunk) == list(t.raster_chunk) and got.raster_groups == t.raster_groups
+        for v, b in zip(tuned, base):
+            np.testing.assert_array_equal(v.color, b.color)
+            np.testing.assert_array_equal(v.vis, b.vis)
+        extreme = S.Tuning()
+        extreme.raster_chunk[:] = [1, 16, 1]
+        extreme.raster_groups = max(1, g // 2)
+        lib.set_tuning(extreme)
+        again = views()
+        S.render_batch(lib, again)
+        st = S.FrameStats()
+        lib.get_stats(st)
+        print("extreme settings: raster ms", st.ms_raster, flush=True)
+        for v, b in zip(again, base):
+            np.testing.assert_array_equal(v.color, b.color)
+            np.testing.assert_array_equal(v.vis, b.vis)
+        lib.set_tuning(default)
+    finally:
+        lib.scene_destroy()
+        lib.sft_release()
+
+
 def test_pipelined_batches_match_blocking_batches():
     """Batches submitted without waiting (three in flight, a fourth forcing the
     oldest to complete, transient dmesh rows in every frame set) give the same
