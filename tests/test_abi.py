@@ -37,3 +37,33 @@ def test_public_api_surface():
               "sft_set_camera", "sft_draw_instance", "sft_draw_point_light", "sft_texture_create",
               "sft_material_create", "sft_model_load_qsf", "sft_dmesh_begin", "sft_arena_alloc"):
         assert n in names, n
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """The ctypes mirrors used by tests and the bench have the C layout of
+    the structs in include/sofren2_ext.h (size and field offsets)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    checks = {
+        "sftx_tuning": (S.Tuning, ["raster_chunk", "raster_groups", "raster_groups_resident", "tile_px",
+                                   "occlusion_stack", "ms_class", "ms_batch"]),
+        "sftx_frame_stats": (S.FrameStats, ["meshlets_tested", "shadow_proxies", "ms_upload", "ms_shadows"]),
+        "sftx_request": (S.Request, ["scene", "camera_pos", "view", "proj", "color", "vis64"]),
+    }
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "sofren2_ext.h"', "int main(void) {"]
+    for name, (_, fields) in checks.items():
+        src.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        src += [f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));' for f in fields]
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=gnu11", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(v) for a, b, v in (line.split() for line in out if line)}
+    for name, (cls, fields) in checks.items():
+        assert got[(name, "size")] == ctypes.sizeof(cls), name
+        for f in fields:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
