@@ -381,8 +381,14 @@ void sft_dmesh_push_quad_4n(sft_dmesh* dm, sftvec p0, sftvec p1, sftvec p2, sftv
 void sft_dmesh_push_tri(sft_dmesh* dm, sftvec p0, sftvec p1, sftvec p2, f32 u0, f32 v0, f32 u1, f32 v1, f32 u2, f32 v2, sftvec normal);
 u8   sft_dmesh_alloc_quads(sft_dmesh* dm, u32 count, sft_dmesh_qb* out_batch);
 void sft_dmesh_end(sft_dmesh* dm, const sftmat* opt_transform);
+/* skeletal poses (sofren2.c:8940-9107): NULL for a model without a skeleton;
+   sft_pose_update samples animation anim_id at `time` into
+   posed_bone_matrices, the array an sft_instance passes for skinning */
 sft_pose* sft_pose_create(const sft_model* model, sft_arena* arena);
 void sft_pose_update(sft_pose* pose, i32 anim_id, f32 time);
+/* closest hit of a ray with an instance's model through its BLAS
+   (sofren2.c:3733-3838; public in the reference without a prototype) */
+sft_ray_hit sft_raycast_instance(const sft_instance* inst, sftvec origin, sftvec dir, f32 max_dist);
 
 /* ---- lights (sofren2.c:841-846) ---- */
 void sft_set_ambient_light(f32 r, f32 g, f32 b);

@@ -17,11 +17,17 @@
  *     way so unused slots are recognisable (sofren2.c:4351-4381).
  *
  * Usage:
- *   sofren2_ref dump  <scene> <seed> <w> <h> <threads> <frame> <outdir>
+ *   sofren2_ref dump  <scene> <seed> <w> <h> <threads> <frame> <outdir> [pad prefill]
  *   sofren2_ref bench <scene> <seed> <w> <h> <threads> <warmup> <frames>
- * dump writes color.bin (u32 ARGB, w*h), vid.bin / depth.bin (per sample over
- * the tile grid), ss.bin, ssvis.bin, clip.bin and meta.txt into outdir.
- * bench prints one JSON line with median / p99 frame time.
+ *   sofren2_ref pose  <seed> <times.bin> <out.bin>
+ *   sofren2_ref ray   <scene> <seed> <rays> <out.bin>
+ * dump writes color.bin (u32 ARGB, h rows of w + pad/4 words), vid.bin /
+ * depth.bin (per sample over the tile grid), ss.bin, ssvis.bin, clip.bin and
+ * meta.txt into outdir; with prefill != 0 the colour buffer (padding
+ * included) starts as sfth_prefill's pattern instead of zeros.  bench prints
+ * one JSON line with median / p99 frame time.  pose and ray run the shared
+ * host-query checks of the scene generator (sftx_pose_check,
+ * sftx_raycast_check) and write their floats.
  */
 #define _GNU_SOURCE
 #include <time.h>
@@ -144,9 +150,10 @@ static void sfth_present(void) {
 #include "scenes.c"
 #undef sft_present
 
+/* writes dir/name, or the path `dir` itself when name is empty */
 static void sfth_write(const char* dir, const char* name, const void* data, u64 bytes) {
     char path[1024];
-    snprintf(path, sizeof(path), "%s/%s", dir, name);
+    snprintf(path, sizeof(path), name[0] ? "%s/%s" : "%s%s", dir, name);
     FILE* f = fopen(path, "wb");
     if (!f) {
         fprintf(stderr, "cannot write %s\n", path);
@@ -170,7 +177,51 @@ static int sfth_cmp_f64(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
+/* the pre-filled colour buffer of a dump: a hash of (x, y, seed) per word */
+static u32 sfth_prefill(u32 x, u32 y, u32 seed) {
+    return (x * 73856093u ^ y * 19349663u ^ seed * 83492791u) * 2654435761u;
+}
+
+static void* sfth_read_all(const char* path, u64* bytes) {
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        fprintf(stderr, "cannot read %s\n", path);
+        exit(2);
+    }
+    fseek(f, 0, SEEK_END);
+    *bytes = (u64)ftell(f);
+    fseek(f, 0, SEEK_SET);
+    void* p = malloc(*bytes + 1);
+    if (*bytes && 1 != fread(p, *bytes, 1, f)) {
+        fprintf(stderr, "short read %s\n", path);
+        exit(2);
+    }
+    fclose(f);
+    return p;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 5 && 0 == strcmp(argv[1], "pose")) {
+        u64 bytes = 0;
+        const f32* times = (const f32*)sfth_read_all(argv[3], &bytes);
+        const i32 n = (i32)(bytes / 4);
+        f32* out = (f32*)malloc((u64)(n + 1) * 2 * 12 * 16 * sizeof(f32));
+        const i32 got = sftx_pose_check((u32)strtoul(argv[2], NULL, 10), n, times, out);
+        sfth_write(argv[4], "", out, (u64)got * 4);
+        return 0;
+    }
+    if (argc >= 6 && 0 == strcmp(argv[1], "ray")) {
+        const i32 n = atoi(argv[4]);
+        sft_init(64, 64, 1);
+        if (0 != sftx_scene_create(atoi(argv[2]), 0, 64, 64)) {
+            fprintf(stderr, "unknown scene %s\n", argv[2]);
+            return 1;
+        }
+        f32* out = (f32*)calloc((u64)n * 8 + 1, sizeof(f32));
+        const i32 got = sftx_raycast_check((u32)strtoul(argv[3], NULL, 10), n, out);
+        sfth_write(argv[5], "", out, (u64)got * 8 * 4);
+        return 0;
+    }
     if (argc < 8) {
         fprintf(stderr, "usage: %s dump|bench scene seed w h threads ...\n", argv[0]);
         return 1;
@@ -186,7 +237,16 @@ int main(int argc, char** argv) {
         fprintf(stderr, "unknown scene %d\n", scene);
         return 1;
     }
-    u32* color = (u32*)calloc((u64)w * h, sizeof(u32));
+    /* dump: optional row padding (bytes) and pre-fill */
+    const i32 pad = (0 == strcmp(mode, "dump") && argc >= 11) ? atoi(argv[9]) : 0;
+    const u32 prefill = (0 == strcmp(mode, "dump") && argc >= 11) ? (u32)strtoul(argv[10], NULL, 10) : 0u;
+    const i32 pitch = w + pad / 4;
+    u32* color = (u32*)calloc((u64)pitch * h, sizeof(u32));
+    if (prefill) {
+        for (i32 y = 0; y < h; y += 1) {
+            for (i32 x = 0; x < pitch; x += 1) color[(u64)y * pitch + x] = sfth_prefill((u32)x, (u32)y, prefill);
+        }
+    }
 
     if (0 == strcmp(mode, "dump")) {
         if (argc < 9) {
@@ -196,9 +256,9 @@ int main(int argc, char** argv) {
         const i32 frame = atoi(argv[7]);
         const char* dir = argv[8];
         mkdir(dir, 0755);
-        sftx_scene_render(frame, color, w * 4);
+        sftx_scene_render(frame, color, pitch * 4);
         const u64 grid = (u64)g_cap.cols * 64 * g_cap.rows * 64 * SFT_AA_SAMPLES;
-        sfth_write(dir, "color.bin", color, (u64)w * h * 4);
+        sfth_write(dir, "color.bin", color, (u64)pitch * h * 4);
         sfth_write(dir, "vid.bin", g_cap.vid, grid * 4);
         sfth_write(dir, "depth.bin", g_cap.depth, grid * 4);
         sfth_write(dir, "ss.bin", g_cap.ss, (u64)g_cap.ss_count * sizeof(struct sft_ss_meshlet));

@@ -27,10 +27,11 @@ SCENE_BOX_ROOM = 1
 SCENE_CROWD_SHADOWS = 2
 SCENE_SERVED = 3
 SCENE_MAP = 4
+SCENE_FEATURES = 5
 
 # default resolution of each workload
 SCENE_SIZE = {SCENE_PLANE_SPHERES: (640, 360), SCENE_BOX_ROOM: (1920, 1080), SCENE_CROWD_SHADOWS: (1920, 1080),
-              SCENE_SERVED: (1920, 1080), SCENE_MAP: (1280, 720)}
+              SCENE_SERVED: (1920, 1080), SCENE_MAP: (1280, 720), SCENE_FEATURES: (640, 360)}
 
 
 class FrameStats(ctypes.Structure):
@@ -148,6 +149,8 @@ class Library:
         self.get_tuning = _bind(g, "sftx_get_tuning", None, [ctypes.POINTER(Tuning)])
         self.set_tuning = _bind(g, "sftx_set_tuning", None, [ctypes.POINTER(Tuning)])
         self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
+        self.pose_check = _bind(s, "sftx_pose_check", i32, [u32, i32, vp, vp])
+        self.raycast_check = _bind(s, "sftx_raycast_check", i32, [u32, i32, vp])
 
 
 _LIB = None

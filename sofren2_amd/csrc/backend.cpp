@@ -142,6 +142,7 @@ struct FrameSet {
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
     u32 t_base = 0, t_skin_base = 0;    /* this set's transient region */
+    u64 layout_epoch = 0;           /* Backend::layout_epoch when the batch was prepared */
     u64 bin_m_words = 0;
     u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     size_t zero_words = 0;
@@ -161,6 +162,9 @@ struct Backend {
        a batch can stage its transient rows while the other sets' batches run;
        skins alike */
     u32 res_cap = 0, res_skin_cap = 0, t_cap = 0, t_skin_cap = 0;
+    /* bumped whenever the pools grow or their layout may move: a prepared
+       batch whose epoch is older must be prepared again before a re-run */
+    u64 layout_epoch = 0;
     std::vector<MeshRec> mesh_recs;
     std::unordered_map<const sft_model*, ModelRes> models;
     DBuf texels, tex_meta_d, tex_stage;
@@ -431,6 +435,7 @@ static void parallel_runs(size_t n, W weight, F fn, size_t max_threads, size_t m
 /* grows the shared pools; no batch may be in flight (quiesce first) */
 static void grow_pools(Backend* b, size_t slots, size_t skins) {
     hipStream_t s = b->sets[0].stream;
+    b->layout_epoch += 1;
     slots = std::max(slots, (size_t)b->res_cap + kFrameSets * (size_t)b->t_cap);
     skins = std::max(skins, (size_t)b->res_skin_cap + kFrameSets * (size_t)b->t_skin_cap);
     dgrow(b->pos, slots * 768, true, s);
@@ -1019,6 +1024,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.host_color = host_color;
     fs.cap_vis = 1 == nv && 0 != (views[0].capture_flags & SFTX_CAPTURE_VISIBILITY);
     fs.zero_words = COUNTER_WORDS + (size_t)nt * max_samples;
+    fs.layout_epoch = b->layout_epoch;
 }
 
 /* Device side of a prepared batch: buffers, one H2D copy of the tables,
@@ -1228,6 +1234,10 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         }
         if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
         if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
+        /* another set's batch may have grown the pools since this one was
+           prepared, moving this set's transient region: its rows and work
+           items are then planned again */
+        if (fs.layout_epoch != b->layout_epoch) prepare_batch(b, c, views, fs);
         launch_batch(b, views, fs);
         copied = false;
     }
@@ -1334,9 +1344,12 @@ void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int
        records and submits the next one; completing the batch joins the copy */
     std::vector<Span> spans = host_output_spans(views, fs);
     if (!spans.empty()) {
-        fs.copier = std::thread([ev = fs.ev[5], sp = std::move(spans)]() {
+        /* an overflowed batch is re-run by backend_complete, which may read
+           the caller's buffers again (views without clearing start from
+           them): its outputs are not copied */
+        fs.copier = std::thread([ev = fs.ev[5], counters = fs.counters_h, sp = std::move(spans)]() {
             HIP_CHECK(hipEventSynchronize(ev));
-            copy_spans(sp);
+            if (0u == counters[CNT_OVERFLOW]) copy_spans(sp);
         });
     }
 }

@@ -1107,29 +1107,6 @@ extern "C" void sft_draw_cylinder(sft_dmesh* dm, sftvec start_pos, sftvec end_po
     }
 }
 
-/* ================================================================ poses
-   Keyframe animation import is outside the accelerated path; a pose holds
-   per-bone matrices the caller may fill, initialised to identity. */
-extern "C" sft_pose* sft_pose_create(const sft_model* model, sft_arena* arena) {
-    if (!model || !arena) return NULL;
-    sft_pose* p = (sft_pose*)sft_arena_alloc(arena, sizeof(sft_pose));
-    p->model = model;
-    const i32 n = model->skeleton.bone_count > 0 ? model->skeleton.bone_count : 1;
-    p->global_matrices = (sftmat*)sft_arena_alloc(arena, n * sizeof(sftmat));
-    p->posed_bone_matrices = (sftmat*)sft_arena_alloc(arena, n * sizeof(sftmat));
-    for (i32 i = 0; i < n; i += 1) {
-        p->global_matrices[i] = sft_m_identity();
-        p->posed_bone_matrices[i] = sft_m_identity();
-    }
-    return p;
-}
-
-extern "C" void sft_pose_update(sft_pose* pose, i32 anim_id, f32 time) {
-    (void)pose;
-    (void)anim_id;
-    (void)time;
-}
-
 /* ================================================================ QSF2 loader
    Binary scene format (sofren2.c:7816-7967): header, textures (each becomes a
    material), lightmap pages, meshes with SoA meshlets, flat triangle indices

@@ -6,6 +6,7 @@
  * -ffp-contract=off so the host-side float math is identical for every build.
  */
 #include <unistd.h>
+#include <sys/mman.h>
 #include "scenes.h"
 
 /* ---------------- PCG32 (O'Neill, pcg-random.org, XSH RR variant) -------- */
@@ -90,10 +91,24 @@ typedef struct {
     sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
 } sftx_crowd_state;
 
+/* the features workload: every shading branch the other scenes miss */
+#define SFTX_FEAT_OBJECTS 66
+#define SFTX_FEAT_LIGHTS 24
+typedef struct {
+    sft_tex_handle tex[6];
+    sft_mat_handle mat[8];
+    i32 obj_kind[SFTX_FEAT_OBJECTS], obj_mat[SFTX_FEAT_OBJECTS];
+    sftmat obj_xform[SFTX_FEAT_OBJECTS];
+    f32 obj_param[SFTX_FEAT_OBJECTS][4];
+    sftvec light_pos[SFTX_FEAT_LIGHTS], light_col[SFTX_FEAT_LIGHTS];
+    f32 light_radius[SFTX_FEAT_LIGHTS];
+} sftx_features_state;
+
 struct sftx_scene_state {
     sftx_scene_common scene;
     sftx_room_state room;
     sftx_crowd_state crowd;
+    sftx_features_state feat;
 };
 #define SFTX_MAX_SCENES 64
 static struct sftx_scene_state* sftx_slots[SFTX_MAX_SCENES];
@@ -105,6 +120,7 @@ static struct sftx_scene_state* sftx_cur;
 #define g_scene (sftx_cur->scene)
 #define g_room (sftx_cur->room)
 #define g_crowd (sftx_cur->crowd)
+#define g_feat (sftx_cur->feat)
 
 /* ---------------- workload 0: ground plane + 16 dmesh spheres -------------- */
 static void sftx_plane_spheres_create(void) {
@@ -986,6 +1002,298 @@ static void sftx_map_render(i32 frame) {
     sft_draw_instance(&g_room.inst);
 }
 
+/* ---------------- workload 5: shading features --------------------------
+   A 640x360 scene built to reach the branches of the shading path that the
+   other workloads leave out: textures of non-power-of-two sizes (edge clamp
+   instead of wrap, sofren2.c:5594-5599; bilinear near the camera, nearest on
+   the far floor), materials with negative shininess (unlit, 6416-6423), two
+   directional lights (6440-6470), 24 point lights of which 20 overlap the
+   middle tiles (the 16-light cut of a tile's list, 5929-5931), and 70 draw
+   calls of which the last six exceed the 64 instances of the default ID
+   split and are dropped (8412-8417).  The view does not clear
+   (sft_set_clear(0)): uncovered pixels keep the caller's buffer and, with
+   MSAA, partly covered ones blend with it (6067-6124, 6732-6743).  Every
+   object is dmesh geometry: ground plane, three poster quads, and spheres,
+   cubes, inverted cubes, capsules, cylinders and triangle fans.  UVs on
+   the non-power-of-two textures stay well above 0 (the reference indexes
+   before the level for negative texel coordinates). */
+static void sftx_noise_image(sftx_rng* rng, i32 w, i32 h, u32* px) {
+    f32 g[3][32 * 32];
+    const i32 gs[3] = { 4, 8, 16 };
+    for (i32 o = 0; o < 3; o += 1) {
+        for (i32 i = 0; i < gs[o] * gs[o]; i += 1) g[o][i] = sftx_u01(rng);
+    }
+    const f32 tint[3] = { sftx_uniform(rng, 0.3f, 1.f), sftx_uniform(rng, 0.3f, 1.f), sftx_uniform(rng, 0.3f, 1.f) };
+    for (i32 y = 0; y < h; y += 1) {
+        for (i32 x = 0; x < w; x += 1) {
+            f32 n = 0.f, amp = 0.5f;
+            for (i32 o = 0; o < 3; o += 1) {
+                n += amp * sftx_noise_lattice(g[o], gs[o], (f32)x * (f32)gs[o] / (f32)w, (f32)y * (f32)gs[o] / (f32)h);
+                amp *= 0.5f;
+            }
+            n = n / 0.875f;
+            px[y * w + x] = sftx_pack_rgb(n * tint[0], n * tint[1], n * tint[2]);
+        }
+    }
+}
+
+static void sftx_features_create(void) {
+    sftx_rng rng;
+    sftx_rng_seed(&rng, g_scene.seed, 0x85EBCA6BULL);
+    sft_set_clear(0);
+    /* 0-3 non-power-of-two, 4-5 power-of-two */
+    static const i32 tw[6] = { 300, 3, 37, 129, 64, 32 }, th[6] = { 111, 200, 100, 65, 64, 32 };
+    for (i32 t = 0; t < 6; t += 1) {
+        u32* px = (u32*)malloc((size_t)tw[t] * th[t] * 4);
+        sftx_noise_image(&rng, tw[t], th[t], px);
+        g_feat.tex[t] = sft_texture_create(tw[t], th[t], px);
+        free(px);
+    }
+    /* floor (NPOT, lit), unlit NPOT poster, shiny NPOT poster, NPOT lit,
+       POT lit, POT unlit, untextured lit, untextured unlit */
+    g_feat.mat[0] = sft_material_create(g_feat.tex[0], 0xFFFFFFFFu, 16.f);
+    g_feat.mat[1] = sft_material_create(g_feat.tex[1], sftx_pack_rgb(0.9f, 0.7f, 0.5f), -1.25f);
+    g_feat.mat[2] = sft_material_create(g_feat.tex[2], sftx_pack_rgb(sftx_u01(&rng), sftx_u01(&rng), sftx_u01(&rng)), 48.f);
+    g_feat.mat[3] = sft_material_create(g_feat.tex[3], 0xFFE0E0E0u, 4.f);
+    g_feat.mat[4] = sft_material_create(g_feat.tex[4], sftx_pack_rgb(sftx_u01(&rng), sftx_u01(&rng), sftx_u01(&rng)), 24.f);
+    g_feat.mat[5] = sft_material_create(g_feat.tex[5], 0xFFFFFFFFu, -0.8f);
+    g_feat.mat[6] = sft_material_create(SFT_INVALID_HANDLE, sftx_pack_rgb(sftx_u01(&rng), sftx_u01(&rng), sftx_u01(&rng)), 8.f);
+    g_feat.mat[7] = sft_material_create(SFT_INVALID_HANDLE, sftx_pack_rgb(0.2f, 0.9f, 0.6f), -1.f);
+    for (i32 i = 0; i < SFTX_FEAT_OBJECTS; i += 1) {
+        g_feat.obj_kind[i] = i % 6;
+        /* primitives have UVs from 0: power-of-two textures or none */
+        static const i32 prim_mats[5] = { 4, 5, 6, 7, 4 };
+        g_feat.obj_mat[i] = (5 == i % 6) ? 3 : prim_mats[(i / 6) % 5];
+        const f32 sc = sftx_uniform(&rng, 0.4f, 1.2f);
+        const f32 x = sftx_uniform(&rng, -9.f, 9.f), z = sftx_uniform(&rng, -6.f, 10.f);
+        g_feat.obj_xform[i] = sft_m_mul(sft_m_mul(sft_m_scale(sc, sc, sc), sft_m_rot_y(sftx_uniform(&rng, 0.f, 6.2831853f))),
+                                        sft_m_translate(x, sc * 0.6f, z));
+        for (i32 k = 0; k < 4; k += 1) g_feat.obj_param[i][k] = sftx_uniform(&rng, 0.2f, 1.f);
+    }
+    /* the last six draws stand between the camera and the scene: were they
+       not dropped, they would cover much of the frame */
+    for (i32 i = SFTX_FEAT_OBJECTS - 6; i < SFTX_FEAT_OBJECTS; i += 1) {
+        g_feat.obj_kind[i] = 0;
+        g_feat.obj_mat[i] = 6;
+        const f32 x = -5.f + 2.f * (f32)(i - (SFTX_FEAT_OBJECTS - 6));
+        g_feat.obj_xform[i] = sft_m_mul(sft_m_scale(1.5f, 1.5f, 1.5f), sft_m_translate(x, 2.5f, -6.f));
+    }
+    for (i32 l = 0; l < SFTX_FEAT_LIGHTS; l += 1) {
+        const int clustered = l < 20;
+        g_feat.light_pos[l] = clustered
+            ? SFT_V(sftx_uniform(&rng, -1.5f, 1.5f), sftx_uniform(&rng, 1.f, 3.f), sftx_uniform(&rng, -1.5f, 1.5f))
+            : SFT_V(sftx_uniform(&rng, -9.f, 9.f), sftx_uniform(&rng, 1.f, 4.f), sftx_uniform(&rng, -6.f, 10.f));
+        g_feat.light_col[l] = SFT_V(sftx_uniform(&rng, 0.05f, 0.35f), sftx_uniform(&rng, 0.05f, 0.35f), sftx_uniform(&rng, 0.05f, 0.35f));
+        g_feat.light_radius[l] = clustered ? sftx_uniform(&rng, 6.f, 12.f) : sftx_uniform(&rng, 3.f, 6.f);
+    }
+    g_scene.tri_count = 0;
+    g_scene.meshlet_count = 0;
+}
+
+static void sftx_features_render(i32 frame) {
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    const f32 th = 0.05f * (f32)(cam % 120);
+    const sftvec pos = SFT_V(9.f * sinf(th), 5.f, -13.f * cosf(th));
+    sft_set_camera(pos, sft_m_view_look_at(pos, SFT_V(0.f, 1.f, 0.f), SFT_V(0.f, 1.f, 0.f)),
+                   sft_m_proj(60.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
+    sft_set_ambient_light(0.15f, 0.15f, 0.18f);
+    sft_draw_directional_light(SFT_V(1.f, -1.f, 0.5f), 0.6f, 0.5f, 0.4f, 1.f);
+    sft_draw_directional_light(SFT_V(-0.3f, -1.f, -0.8f), 0.2f, 0.3f, 0.5f, 0.9f);
+    for (i32 l = 0; l < SFTX_FEAT_LIGHTS; l += 1) {
+        sft_draw_point_light(g_feat.light_pos[l], g_feat.light_col[l].x, g_feat.light_col[l].y, g_feat.light_col[l].z, 1.f,
+                             g_feat.light_radius[l], (u32)(l + 1));
+    }
+    /* floor: 20x16 unit quads, UV continuous over the plane in [0.4, 1.6] x
+       [0.35, 1.55] (clamped past 1 on the 300x111 texture) */
+    sft_dmesh* fl = sft_dmesh_begin(SFT_DMESH_QUADS, g_feat.mat[0], 320);
+    for (i32 z = 0; z < 16; z += 1) {
+        for (i32 x = 0; x < 20; x += 1) {
+            const f32 x0 = (f32)(x - 10), x1 = (f32)(x - 9), z0 = (f32)(z - 6), z1 = (f32)(z - 5);
+            sft_dmesh_push_quad(fl, SFT_V(x0, 0.f, z0), SFT_V(x0, 0.f, z1), SFT_V(x1, 0.f, z1), SFT_V(x1, 0.f, z0),
+                                0.4f + 0.06f * (f32)x, 0.35f + 0.075f * (f32)z, 0.4f + 0.06f * (f32)(x + 1),
+                                0.35f + 0.075f * (f32)(z + 1), SFT_V(0.f, 1.f, 0.f));
+        }
+    }
+    sft_dmesh_end(fl, NULL);
+    /* posters: unlit NPOT, shiny NPOT through a transient clone of its
+       material, and a 4-normal quad */
+    sft_dmesh* p0 = sft_dmesh_begin(SFT_DMESH_QUADS, g_feat.mat[1], 1);
+    sft_dmesh_push_quad(p0, SFT_V(-7.f, 0.5f, 6.f), SFT_V(-7.f, 4.5f, 6.f), SFT_V(-3.f, 4.5f, 7.f), SFT_V(-3.f, 0.5f, 7.f),
+                        0.3f, 0.3f, 1.4f, 1.2f, SFT_V(0.24f, 0.f, -0.97f));
+    sft_dmesh_end(p0, NULL);
+    sft_dmesh* p1 = sft_dmesh_begin(SFT_DMESH_QUADS, sft_material_clone(g_feat.mat[2]), 1);
+    sft_dmesh_push_quad(p1, SFT_V(2.f, 0.2f, 7.f), SFT_V(2.f, 3.8f, 7.f), SFT_V(6.5f, 3.8f, 5.5f), SFT_V(6.5f, 0.2f, 5.5f),
+                        0.35f, 0.3f, 2.2f, 1.3f, SFT_V(-0.32f, 0.f, -0.95f));
+    sft_dmesh_end(p1, NULL);
+    sft_dmesh* p2 = sft_dmesh_begin(SFT_DMESH_QUADS, g_feat.mat[3], 1);
+    sft_dmesh_push_quad_4n(p2, SFT_V(-2.f, 0.05f, 2.f), SFT_V(-2.f, 2.5f, 3.f), SFT_V(1.5f, 2.5f, 3.f), SFT_V(1.5f, 0.05f, 2.f),
+                           0.3f, 0.4f, 1.1f, 1.3f, sft_v_norm(SFT_V(-0.5f, 0.2f, -1.f)), sft_v_norm(SFT_V(-0.4f, 0.6f, -1.f)),
+                           sft_v_norm(SFT_V(0.5f, 0.6f, -1.f)), sft_v_norm(SFT_V(0.4f, 0.2f, -1.f)));
+    sft_dmesh_end(p2, NULL);
+    for (i32 i = 0; i < SFTX_FEAT_OBJECTS; i += 1) {
+        const f32* q = g_feat.obj_param[i];
+        const sft_mat_handle mat = g_feat.mat[g_feat.obj_mat[i]];
+        sft_dmesh* dm;
+        switch (g_feat.obj_kind[i]) {
+            case 0:
+                dm = sft_dmesh_begin(SFT_DMESH_QUADS, mat, 7 * 10);
+                sft_draw_sphere(dm, 6, 10);
+                break;
+            case 1:
+                dm = sft_dmesh_begin(SFT_DMESH_QUADS, mat, 6);
+                sft_draw_cube(dm);
+                break;
+            case 2:
+                dm = sft_dmesh_begin(SFT_DMESH_QUADS, mat, 6);
+                sft_draw_cube_inv(dm);
+                break;
+            case 3:
+                dm = sft_dmesh_begin(SFT_DMESH_QUADS, mat, 256);
+                sft_draw_capsule(dm, SFT_V(0.f, -0.3f * q[0], 0.f), SFT_V(0.2f * q[1], 0.5f * q[2], 0.f), 0.25f + 0.2f * q[3], 8, 4);
+                break;
+            case 4:
+                dm = sft_dmesh_begin(SFT_DMESH_QUADS, mat, 64);
+                sft_draw_cylinder(dm, SFT_V(0.f, -0.5f, 0.f), SFT_V(0.f, 0.4f + 0.4f * q[0], 0.f), 0.3f + 0.3f * q[1],
+                                  0.1f + 0.3f * q[2], 10);
+                break;
+            default: {
+                /* a fan of 12 triangles, UVs in [0.3, 1.5] on an NPOT texture */
+                dm = sft_dmesh_begin(SFT_DMESH_TRIANGLES, mat, 12);
+                for (i32 k = 0; k < 12; k += 1) {
+                    const f32 a0 = 6.2831853f * (f32)k / 12.f, a1 = 6.2831853f * (f32)(k + 1) / 12.f;
+                    const sftvec c = SFT_V(0.f, 0.2f + 0.3f * q[0], 0.f);
+                    const sftvec e0 = SFT_V(0.8f * cosf(a0), 0.f, 0.8f * sinf(a0));
+                    const sftvec e1 = SFT_V(0.8f * cosf(a1), 0.f, 0.8f * sinf(a1));
+                    sft_dmesh_push_tri(dm, c, e1, e0, 0.9f, 0.9f, 0.9f + 0.6f * cosf(a1), 0.9f + 0.6f * sinf(a1),
+                                       0.9f + 0.6f * cosf(a0), 0.9f + 0.6f * sinf(a0), sft_v_face_normal(c, e1, e0));
+                }
+                break;
+            }
+        }
+        sft_dmesh_end(dm, &g_feat.obj_xform[i]);
+    }
+}
+
+/* ---------------- shared checks of host-side queries ---------------------
+   Compiled into every build of the API, so the results of two builds can
+   be compared word for word. */
+
+/* A seeded skeleton of 12 bones (parents anywhere before the child, bone 0
+   and one more a root) with two animations: channels of every path with
+   linear and step samplers of 1-6 keys.  For each of the n times, samples
+   animation (k % 2) with sft_pose_update and writes the bones' global
+   matrices then their posed matrices (2 * 12 * 16 floats per time).
+   Returns the floats written. */
+i32 sftx_pose_check(u32 seed, i32 n_times, const f32* times, f32* out) {
+    enum { NB = 12, NCH = 14 };
+    sftx_rng rng;
+    sftx_rng_seed(&rng, seed, 0x27D4EB2FULL);
+    sft_arena ar;
+    sft_arena_init(&ar, 1ULL << 24);
+    sft_model* m = (sft_model*)sft_arena_alloc(&ar, sizeof(sft_model));
+    memset(m, 0, sizeof(*m));
+    m->skeleton.bone_count = NB;
+    m->skeleton.bones = (struct sft_bone*)sft_arena_alloc(&ar, NB * sizeof(struct sft_bone));
+    memset(m->skeleton.bones, 0, NB * sizeof(struct sft_bone));
+    for (i32 i = 0; i < NB; i += 1) {
+        struct sft_bone* b = &m->skeleton.bones[i];
+        b->parent_ind = (0 == i || 5 == i) ? -1 : (i32)(sftx_rng_next(&rng) % (u32)i);
+        b->base_t = SFT_V(sftx_uniform(&rng, -1.f, 1.f), sftx_uniform(&rng, 0.f, 1.f), sftx_uniform(&rng, -1.f, 1.f));
+        const sftvec ax = sft_v_norm(SFT_V(sftx_uniform(&rng, -1.f, 1.f), sftx_uniform(&rng, -1.f, 1.f), 0.5f));
+        const f32 ang = sftx_uniform(&rng, -1.5f, 1.5f);
+        b->base_r.x = ax.x * sinf(ang * 0.5f);
+        b->base_r.y = ax.y * sinf(ang * 0.5f);
+        b->base_r.z = ax.z * sinf(ang * 0.5f);
+        b->base_r.w = cosf(ang * 0.5f);
+        b->base_s = SFT_V(sftx_uniform(&rng, 0.8f, 1.2f), sftx_uniform(&rng, 0.8f, 1.2f), sftx_uniform(&rng, 0.8f, 1.2f));
+        b->local_bind_matrix = sft_m_mul(sft_m_scale(b->base_s.x, b->base_s.y, b->base_s.z),
+                                         sft_m_mul(sft_m_from_quat(b->base_r), sft_m_translate(b->base_t.x, b->base_t.y, b->base_t.z)));
+        b->inverse_bind_matrix = sft_m_qinv(sft_m_mul(sft_m_rot_y(sftx_uniform(&rng, 0.f, 6.f)),
+                                                      sft_m_translate(sftx_uniform(&rng, -2.f, 2.f), sftx_uniform(&rng, 0.f, 2.f), 0.f)));
+    }
+    m->anim_count = 2;
+    m->animations = (struct sft_animation*)sft_arena_alloc(&ar, 2 * sizeof(struct sft_animation));
+    for (i32 a = 0; a < 2; a += 1) {
+        struct sft_animation* an = &m->animations[a];
+        an->channel_count = NCH;
+        an->sampler_count = NCH;
+        an->channels = (struct sft_anim_channel*)sft_arena_alloc(&ar, NCH * sizeof(struct sft_anim_channel));
+        an->samplers = (struct sft_anim_sampler*)sft_arena_alloc(&ar, NCH * sizeof(struct sft_anim_sampler));
+        an->duration = 0.f;
+        for (i32 c = 0; c < NCH; c += 1) {
+            struct sft_anim_channel* ch = &an->channels[c];
+            struct sft_anim_sampler* sm = &an->samplers[c];
+            ch->sampler_ind = c;
+            /* one channel names a bone the skeleton lacks */
+            ch->node_ind = (NCH - 1 == c) ? NB + 3 : (i32)(sftx_rng_next(&rng) % NB);
+            ch->path = (enum sft_anim_path)(sftx_rng_next(&rng) % 3);
+            sm->count = 1 + (i32)(sftx_rng_next(&rng) % 6);
+            sm->is_step = (u8)(0 == sftx_rng_next(&rng) % 3);
+            sm->is_cubic = 0;
+            sm->inputs = (f32*)sft_arena_alloc(&ar, (u64)sm->count * sizeof(f32));
+            sm->outputs = (f32*)sft_arena_alloc(&ar, (u64)sm->count * 4 * sizeof(f32));
+            f32 t = sftx_uniform(&rng, 0.f, 0.3f);
+            for (i32 k = 0; k < sm->count; k += 1) {
+                sm->inputs[k] = t;
+                t += sftx_uniform(&rng, 0.05f, 0.6f);
+                f32* o = &sm->outputs[k * 4];
+                if (SFT_ANIM_PATH_ROTATION == ch->path) {
+                    const sftvec ax = sft_v_norm(SFT_V(sftx_uniform(&rng, -1.f, 1.f), sftx_uniform(&rng, -1.f, 1.f),
+                                                       sftx_uniform(&rng, -1.f, 1.f)));
+                    const f32 ang = sftx_uniform(&rng, -3.f, 3.f);
+                    o[0] = ax.x * sinf(ang * 0.5f); o[1] = ax.y * sinf(ang * 0.5f); o[2] = ax.z * sinf(ang * 0.5f);
+                    o[3] = cosf(ang * 0.5f);
+                }
+                else if (SFT_ANIM_PATH_SCALE == ch->path) {
+                    o[0] = sftx_uniform(&rng, 0.5f, 1.5f); o[1] = sftx_uniform(&rng, 0.5f, 1.5f);
+                    o[2] = sftx_uniform(&rng, 0.5f, 1.5f); o[3] = 0.f;
+                }
+                else {
+                    o[0] = sftx_uniform(&rng, -1.f, 1.f); o[1] = sftx_uniform(&rng, -1.f, 1.f);
+                    o[2] = sftx_uniform(&rng, -1.f, 1.f); o[3] = 0.f;
+                }
+            }
+            if (sm->inputs[sm->count - 1] > an->duration) an->duration = sm->inputs[sm->count - 1];
+        }
+    }
+    sft_pose* pose = sft_pose_create(m, &ar);
+    i32 n = 0;
+    for (i32 k = 0; k < n_times && pose; k += 1) {
+        sft_pose_update(pose, k % 2, times[k]);
+        memcpy(out + n, pose->global_matrices, NB * sizeof(sftmat));
+        n += NB * 16;
+        memcpy(out + n, pose->posed_bone_matrices, NB * sizeof(sftmat));
+        n += NB * 16;
+    }
+    munmap(ar.base, ar.max_size);
+    return n;
+}
+
+/* n seeded rays against the current scene's room model (it needs a BLAS:
+   the crowd and served workloads and map files have one) placed by a
+   rotated, translated instance transform; per ray 8 floats: hit, distance,
+   position, normal.  Returns the rays cast (0 without a room model). */
+i32 sftx_raycast_check(u32 seed, i32 n_rays, f32* out) {
+    if (!sftx_cur || !g_room.model) return 0;
+    sftx_rng rng;
+    sftx_rng_seed(&rng, seed, 0x165667B1ULL);
+    sft_instance inst = g_room.inst;
+    inst.transform = sft_m_mul(sft_m_rot_y(0.7f), sft_m_translate(3.f, -1.f, 2.f));
+    for (i32 i = 0; i < n_rays; i += 1) {
+        const sftvec o = SFT_V(sftx_uniform(&rng, -30.f, 30.f), sftx_uniform(&rng, -2.f, 10.f), sftx_uniform(&rng, -30.f, 30.f));
+        sftvec d = SFT_V(sftx_uniform(&rng, -1.f, 1.f), sftx_uniform(&rng, -1.f, 1.f), sftx_uniform(&rng, -1.f, 1.f));
+        if (0 == i % 7) d = SFT_V(0.f, -1.f, 0.f);  /* axis-aligned: the epsilon guard of 1/dir */
+        const f32 maxd = (0 == i % 5) ? sftx_uniform(&rng, 1.f, 20.f) : 1000.f;
+        const sft_ray_hit h = sft_raycast_instance(&inst, o, d, maxd);
+        f32* r = out + (size_t)i * 8;
+        r[0] = (f32)h.hit; r[1] = h.distance;
+        r[2] = h.pos.x; r[3] = h.pos.y; r[4] = h.pos.z;
+        r[5] = h.normal.x; r[6] = h.normal.y; r[7] = h.normal.z;
+    }
+    return n_rays;
+}
+
 /* ---------------- dispatch ---------------- */
 static i32 sftx_build(i32 kind) {
     sft_set_clear(1);
@@ -999,6 +1307,7 @@ static i32 sftx_build(i32 kind) {
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_create(); return 0;
         case SFTX_SCENE_SERVED: sftx_served_create(); return 0;
         case SFTX_SCENE_MAP: sftx_map_create(); return 0;
+        case SFTX_SCENE_FEATURES: sftx_features_create(); return 0;
         default: return -1;
     }
 }
@@ -1036,6 +1345,7 @@ static void sftx_draw_current(i32 frame) {
         case SFTX_SCENE_CROWD_SHADOWS: sftx_crowd_render(frame); break;
         case SFTX_SCENE_SERVED: sftx_served_render(frame); break;
         case SFTX_SCENE_MAP: sftx_map_render(frame); break;
+        case SFTX_SCENE_FEATURES: sftx_features_render(frame); break;
         default: break;
     }
 }

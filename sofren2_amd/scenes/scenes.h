@@ -27,6 +27,10 @@
  *   SFTX_SCENE_MAP (4): the QSF2 map file named by $SFTX_QSF (a real,
  *       lightmapped scene), lit by its own game-data lights, camera turning
  *       at (0, 2, 0).
+ *   SFTX_SCENE_FEATURES (5): 640x360 dmesh scene for the shading branches the
+ *       others miss: non-power-of-two textures, unlit materials, directional
+ *       lights, more than 16 point lights on a tile, 70 draws (6 dropped),
+ *       no clearing (the caller's buffer shows through and blends under MSAA).
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H
@@ -43,6 +47,7 @@ enum sftx_scene_kind {
     SFTX_SCENE_CROWD_SHADOWS = 2,
     SFTX_SCENE_SERVED = 3,
     SFTX_SCENE_MAP = 4,
+    SFTX_SCENE_FEATURES = 5,
     SFTX_SCENE_COUNT
 };
 
@@ -77,6 +82,15 @@ u32 sftx_scene_triangle_count(void);
 u32 sftx_scene_meshlet_count(void);
 
 void sftx_scene_destroy(void);
+
+/* Host-side query checks, word-comparable between builds of the API:
+   sftx_pose_check samples a seeded 12-bone skeleton's two animations at the
+   given times (sft_pose_create / sft_pose_update) and writes 2 * 12 * 16
+   floats per time (global then posed matrices); sftx_raycast_check casts
+   seeded rays at the current scene's room model with sft_raycast_instance
+   and writes 8 floats per ray (hit, distance, position, normal). */
+i32 sftx_pose_check(u32 seed, i32 n_times, const f32* times, f32* out);
+i32 sftx_raycast_check(u32 seed, i32 n_rays, f32* out);
 
 #ifdef __cplusplus
 }

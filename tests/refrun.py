@@ -24,20 +24,37 @@ def have_ref(samples=1):
     return os.path.exists(ref_path(samples))
 
 
+def prefill_pattern(rows, cols, seed):
+    """The colour buffer a pre-filled dump starts from (ref_harness.c
+    sfth_prefill): a hash of (x, y, seed) per 32-bit word."""
+    y, x = np.meshgrid(np.arange(rows, dtype=np.uint32), np.arange(cols, dtype=np.uint32), indexing="ij")
+    with np.errstate(over="ignore"):
+        v = (x * np.uint32(73856093)) ^ (y * np.uint32(19349663)) ^ np.uint32((seed * 83492791) & 0xFFFFFFFF)
+        return (v * np.uint32(2654435761)).astype(np.uint32)
+
+
 @functools.lru_cache(maxsize=16)
-def ref_dump(scene, seed, width, height, frame=0, samples=1, threads=1):
+def ref_dump(scene, seed, width, height, frame=0, samples=1, threads=1, pad=0, prefill=0):
+    """The reference's render of a scene frame.  `pad` bytes of row padding
+    and a non-zero `prefill` seed make the colour buffer start as
+    prefill_pattern over (width + pad/4) words per row; out["color_buffer"]
+    is then the whole padded buffer after the render."""
     exe = ref_path(samples)
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} missing: build it with make -C oracle ref")
     with tempfile.TemporaryDirectory() as d:
-        subprocess.run([exe, "dump", str(scene), str(seed), str(width), str(height), str(threads), str(frame), d],
-                       check=True, timeout=600)
+        args = [exe, "dump", str(scene), str(seed), str(width), str(height), str(threads), str(frame), d]
+        if pad or prefill:
+            args += [str(pad), str(prefill)]
+        subprocess.run(args, check=True, timeout=600)
         meta = {}
         for line in open(os.path.join(d, "meta.txt")):
             k, v = line.split()
             meta[k] = int(v)
         out = {"meta": meta}
-        out["color"] = np.fromfile(os.path.join(d, "color.bin"), dtype=np.uint32).reshape(height, width)
+        buf = np.fromfile(os.path.join(d, "color.bin"), dtype=np.uint32).reshape(height, width + pad // 4)
+        out["color_buffer"] = buf
+        out["color"] = np.ascontiguousarray(buf[:, :width])
         out["vid"] = np.fromfile(os.path.join(d, "vid.bin"), dtype=np.uint32)
         out["depth"] = np.fromfile(os.path.join(d, "depth.bin"), dtype=np.float32)
         out["ss"] = np.fromfile(os.path.join(d, "ss.bin"), dtype=np.uint32).reshape(-1, 9, 64)
@@ -56,3 +73,21 @@ def ref_bench(scene, seed, width, height, threads=0, warmup=10, frames=120, samp
                            str(warmup), str(frames)], check=True, stdout=subprocess.PIPE, text=True, timeout=1200)
     import json
     return json.loads(proc.stdout.strip().splitlines()[-1])
+
+
+def ref_pose(seed, times):
+    """sftx_pose_check of the reference build: (n, 2, 12, 16) float32."""
+    times = np.ascontiguousarray(times, dtype=np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        tp, op = os.path.join(d, "t.bin"), os.path.join(d, "o.bin")
+        times.tofile(tp)
+        subprocess.run([ref_path(), "pose", str(seed), tp, op], check=True, timeout=120)
+        return np.fromfile(op, dtype=np.float32).reshape(-1, 2, 12, 16)
+
+
+def ref_rays(scene, seed, n):
+    """sftx_raycast_check of the reference build: (n, 8) float32."""
+    with tempfile.TemporaryDirectory() as d:
+        op = os.path.join(d, "o.bin")
+        subprocess.run([ref_path(), "ray", str(scene), str(seed), str(n), op], check=True, timeout=300)
+        return np.fromfile(op, dtype=np.float32).reshape(-1, 8)
