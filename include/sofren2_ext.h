@@ -90,6 +90,12 @@ typedef struct sftx_frame_stats {
     f32 ms_shade;             /* deferred shading */
     f32 ms_total;             /* whole sft_present on the GPU stream */
     f32 ms_shadows;           /* proxy occlusion + per-light capsule lists */
+    u32 tex_npot;             /* albedo samples of non-power-of-two textures (edge clamp) */
+    u32 unlit_shadings;       /* shadings of materials with negative shininess */
+    u32 dirlit_shadings;      /* shadings that evaluated directional lights */
+    u32 light_cut_tiles;      /* tiles whose light list stopped at 16 with more lights in reach */
+    u32 bg_blend_pixels;      /* MSAA pixels without clearing that blend the caller's colour */
+    u32 instances_dropped;    /* draws beyond the 64 instances of the visibility-ID split */
 } sftx_frame_stats;
 
 void sftx_get_stats(sftx_frame_stats* out);
@@ -128,6 +134,9 @@ void sftx_set_msaa(i32 samples);
 
    vis64, when given, receives per sample (1/w bits << 32) | visibility ID
    over the view's 64-px tile grid: [samples][tile_rows*64][tile_cols*64].
+   Views (and the sft_init / sft_resize display, and requests) are 1 to
+   16384 pixels a side (256 tiles: the binning packs tile coordinates in 8
+   bits); sftx_view_end returns -1 and drops a view outside that.
    The display size set by sftx_view_begin stays in effect after the batch.
 
    Pipelined form: sftx_batch_submit_async() in place of sftx_batch_submit
@@ -226,6 +235,15 @@ u64 sftx_graph_captures(void);
 
 /* Waits for all GPU work of the renderer. */
 void sftx_synchronize(void);
+
+/* Test hook: the capacities every frame set starts from for clipped
+   triangles and tile-list entries (defaults 65536 and 1048576).  The kernels
+   flag a batch whose intermediates exceed them and the batch is run again
+   with larger ones (they then stay larger); small values make tests take
+   that path.  sftx_overflow_reruns counts such re-runs.  Waits for every
+   batch in flight first; not while the request server runs. */
+void sftx_set_initial_caps(u32 clip_cap, u32 tile_list_cap);
+u64 sftx_overflow_reruns(void);
 
 /* Copies `bytes` between two device buffers with a plain streaming kernel
    and returns its duration in ms; a calibration point for HBM counters. */

@@ -41,7 +41,8 @@ class FrameStats(ctypes.Structure):
         "tex_nearest", "lightmapped_pixels", "occlusion_rays", "occlusion_nodes", "occlusion_tri_tests",
         "occlusion_invalid", "occlusion_overflow", "shadow_proxies")] + [(n, ctypes.c_float) for n in (
         "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_raster", "ms_shade",
-        "ms_total", "ms_shadows")]
+        "ms_total", "ms_shadows")] + [(n, ctypes.c_uint32) for n in (
+        "tex_npot", "unlit_shadings", "dirlit_shadings", "light_cut_tiles", "bg_blend_pixels", "instances_dropped")]
 
 
 class Tuning(ctypes.Structure):
@@ -150,6 +151,9 @@ class Library:
         self.set_tuning = _bind(g, "sftx_set_tuning", None, [ctypes.POINTER(Tuning)])
         self.scene_meshlets = _bind(s, "sftx_scene_meshlet_count", u32, [])
         self.pose_check = _bind(s, "sftx_pose_check", i32, [u32, i32, vp, vp])
+        self.set_initial_caps = _bind(g, "sftx_set_initial_caps", None, [u32, u32])
+        self.overflow_reruns = _bind(g, "sftx_overflow_reruns", ctypes.c_uint64, [])
+        self.set_clear = _bind(g, "sft_set_clear", None, [u8])
         self.raycast_check = _bind(s, "sftx_raycast_check", i32, [u32, i32, vp])
 
 

@@ -142,7 +142,8 @@ struct FrameSet {
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
     u32 t_base = 0, t_skin_base = 0;    /* this set's transient region */
-    u64 layout_epoch = 0;           /* Backend::layout_epoch when the batch was prepared */
+    u64 layout_epoch = 0;           /* Backend::layout_epoch the transient rows are placed for */
+    u32 n_insts = 0;                /* instances in the staged tables */
     u64 bin_m_words = 0;
     u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     size_t zero_words = 0;
@@ -186,6 +187,7 @@ struct Backend {
     u32 raster_groups = 512;        /* persistent visibility workgroups */
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
     u64 graph_captures = 0;
+    u64 reruns = 0;                 /* batches run again after an intermediate overflowed */
 
     /* frames and batches in flight, one set each; blocking calls use set 0 */
     FrameSet sets[kFrameSets];
@@ -1013,6 +1015,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
 
     fs.nt = nt;
     fs.nw = bg.n_work;
+    fs.n_insts = (u32)insts.size();
     fs.max_samples = max_samples;
     fs.ss_total = ss_total;
     fs.n_proxies = (u32)proxies.size();
@@ -1214,6 +1217,35 @@ static void copy_spans(const std::vector<Span>& spans) {
                   [&](size_t i) { memcpy(spans[i].dst, spans[i].src, spans[i].n); }, 16, (size_t)1 << 20);
 }
 
+/* Moves a prepared batch's transient rows to where its frame set's
+   transient region is now.  Another set's batch can grow the pools between
+   this batch's prepare and its re-run after an overflow (new resident models
+   raise res_cap, larger dmesh batches raise t_cap), which moves the region.
+   The batch's staged tables already hold its transient rows and its work
+   items and instances refer to them by slot: those slot numbers are shifted
+   (the dmesh models they came from may be gone by now). */
+static void relocate_transient(Backend* b, FrameSet& fs) {
+    const u32 set = (u32)(&fs - b->sets);
+    const u32 nb = b->res_cap + set * b->t_cap, nsb = b->res_skin_cap + set * b->t_skin_cap;
+    if (fs.t_slots > b->t_cap || fs.t_skins > b->t_skin_cap) {
+        fprintf(stderr, "fatal sofren2 error: transient rows exceed their region\n");
+        exit(1);
+    }
+    if (nb != fs.t_base || nsb != fs.t_skin_base) {
+        auto moved = [](u32 v, u32 from, u32 n, u32 to) { return (v >= from && v < from + n) ? v - from + to : v; };
+        WorkItem* w = (WorkItem*)(fs.pack.host + fs.o_work);
+        for (u32 i = 0; i < fs.nw; i += 1) {
+            w[i].slot = moved(w[i].slot, fs.t_base, fs.t_slots, nb);
+            if (kInvalid != w[i].skin) w[i].skin = moved(w[i].skin, fs.t_skin_base, fs.t_skins, nsb);
+        }
+        InstGpu* in = (InstGpu*)(fs.pack.host + fs.o_insts);
+        for (u32 i = 0; i < fs.n_insts; i += 1) in[i].model_base = moved(in[i].model_base, fs.t_base, fs.t_slots, nb);
+        fs.t_base = nb;
+        fs.t_skin_base = nsb;
+    }
+    fs.layout_epoch = b->layout_epoch;
+}
+
 /* Waits for a launched batch; re-runs it with larger intermediates if any
    overflowed; writes the host outputs (unless its copier already did), the
    statistics and the captures. */
@@ -1235,9 +1267,9 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
         if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
         /* another set's batch may have grown the pools since this one was
-           prepared, moving this set's transient region: its rows and work
-           items are then planned again */
-        if (fs.layout_epoch != b->layout_epoch) prepare_batch(b, c, views, fs);
+           prepared, moving this set's transient region */
+        if (fs.layout_epoch != b->layout_epoch) relocate_transient(b, fs);
+        b->reruns += 1;
         launch_batch(b, views, fs);
         copied = false;
     }
@@ -1273,6 +1305,12 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
     st.occlusion_invalid = counters[CNT_OCC_INVALID];
     st.occlusion_overflow = counters[CNT_OCC_OVERFLOW];
     st.shadow_proxies = fs.n_proxies;
+    st.tex_npot = counters[CNT_TEX_NPOT];
+    st.unlit_shadings = counters[CNT_UNLIT];
+    st.dirlit_shadings = counters[CNT_DIR_LIT];
+    st.light_cut_tiles = counters[CNT_LIGHT_CUT];
+    st.bg_blend_pixels = counters[CNT_BG_BLEND];
+    for (const ViewRecord& v : views) st.instances_dropped += v.dropped;
     st.ms_upload = ev_ms(fs.ev[0], fs.ev[1]);
     st.ms_shadows = fs.side_used ? ev_ms(fs.side.start, fs.side.end) : ev_ms(fs.ev[1], fs.ev[7]);
     st.ms_geometry = ev_ms(fs.ev[7], fs.ev[2]);
@@ -1547,6 +1585,22 @@ extern "C" void sftx_set_output(i32 mode) { ensure_backend()->output_mode = mode
 extern "C" void sftx_set_graphs(i32 enabled) { ensure_backend()->use_graphs = 0 != enabled; }
 
 extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captures; }
+
+extern "C" void sftx_set_initial_caps(u32 clip_cap, u32 tile_list_cap) {
+    Backend* b = ensure_backend();
+    if (g_ctx->server_running) {
+        fprintf(stderr, "sofren2: sftx_set_initial_caps while the request server runs\n");
+        return;
+    }
+    drain_async_batches(*g_ctx);
+    quiesce(b);
+    for (FrameSet& fs : b->sets) {
+        fs.clip_cap = std::max(clip_cap, 1u);
+        fs.tile_list_cap = std::max(tile_list_cap, 1u);
+    }
+}
+
+extern "C" u64 sftx_overflow_reruns(void) { return ensure_backend()->reruns; }
 
 extern "C" const u32* sftx_device_color(void) {
     Backend* b = ensure_backend();

@@ -273,6 +273,9 @@ extern "C" void sft_init(i32 width, i32 height, i32 requested_threads) {
     if (g_ctx) {
         SFT_ERR_EXIT("sft_init called twice without sft_release\n");
     }
+    if (!view_size_ok(width, height)) {
+        SFT_ERR_EXIT("display %dx%d is outside 1..%d pixels a side\n", (int)width, (int)height, (int)kMaxViewSide);
+    }
     g_ctx = new Context();
     Context& c = *g_ctx;
     sft_arena_init(&c.scratch_arena, 2ULL * 1024 * 1024 * 1024);
@@ -324,6 +327,10 @@ extern "C" void sft_release(void) {
 
 extern "C" void sft_resize(i32 width, i32 height) {
     if (!g_ctx) return;
+    if (!view_size_ok(width, height)) {
+        SFT_ERR_RET(, "display %dx%d is outside 1..%d pixels a side; size unchanged\n", (int)width, (int)height,
+                    (int)kMaxViewSide);
+    }
     set_display(*g_ctx, width, height);
 }
 
@@ -384,6 +391,7 @@ void sft::snapshot_view(const Context& c, ViewRecord& v) {
     v.shadows_enabled = c.shadows_enabled;
     v.proxies = c.proxies;
     v.instances = c.instances;
+    v.dropped = (u32)std::max<i32>(0, c.instance_count - (i32)c.instances.size());
     v.total_ss = c.total_ss;
     v.transient_models = c.transient_models;
     v.tmat_tex = c.tmat_tex;
@@ -444,7 +452,14 @@ extern "C" void sftx_view_begin(i32 width, i32 height, i32 samples) {
         fprintf(stderr, "sofren2: sftx_view_begin outside sftx_batch_begin/sftx_batch_end\n");
         return;
     }
-    set_display(c, width, height);
+    c.view_rejected = !view_size_ok(width, height);
+    if (c.view_rejected) {
+        fprintf(stderr, "sofren2: view %dx%d is outside 1..%d pixels a side; sftx_view_end will drop it\n", (int)width,
+                (int)height, (int)kMaxViewSide);
+    }
+    else {
+        set_display(c, width, height);
+    }
     sftx_set_msaa(samples);
     c.instance_count = 0;
     c.instances.clear();
@@ -462,6 +477,10 @@ extern "C" i32 sftx_view_end(u32* color, i32 pitch_bytes, u64* vis64) {
     Context& c = C();
     if (!c.batch_open) {
         fprintf(stderr, "sofren2: sftx_view_end outside sftx_batch_begin/sftx_batch_end\n");
+        return -1;
+    }
+    if (c.view_rejected) {
+        c.view_rejected = false;
         return -1;
     }
     c.batch_views.emplace_back();
@@ -1111,77 +1130,124 @@ extern "C" void sft_draw_cylinder(sft_dmesh* dm, sftvec start_pos, sftvec end_po
    Binary scene format (sofren2.c:7816-7967): header, textures (each becomes a
    material), lightmap pages, meshes with SoA meshlets, flat triangle indices
    re-laid into 8-wide batches, bounds, cones, meshlet->mesh map, BLAS nodes. */
-static bool rd(FILE* f, void* dst, u64 bytes) { return 0 == bytes || 1 == fread(dst, bytes, 1, f); }
+/* A QSF2 file that ends early, or whose counts or indices do not fit its
+   own contents, is rejected as a whole: the reference reads such files
+   without checks (sofren2.c:7816-7967); here the load returns NULL with a
+   message, its recoverable-error convention. */
+struct QsfBad { const char* what; };
+
+struct QsfReader {
+    FILE* f;
+    u64 left;       /* bytes not yet read */
+    void read(void* dst, u64 bytes) {
+        if (bytes > left || (bytes && 1 != fread(dst, bytes, 1, f))) throw QsfBad{"file ends early"};
+        left -= bytes;
+    }
+    /* a count of `n` records of `each` bytes must fit in what is left */
+    u64 fits(u64 n, u64 each) {
+        if (each && n > left / each) throw QsfBad{"a count exceeds the file"};
+        return n;
+    }
+};
+
+static sft_model* load_qsf(Context& c, QsfReader& r, sft_arena* arena);
 
 extern "C" sft_model* sft_model_load_qsf(const char* path, sft_arena* arena) {
     Context& c = C();
     FILE* f = fopen(path, "rb");
     if (!f) SFT_ERR_RET(NULL, "failed to open '%s'\n", path);
+    fseek(f, 0, SEEK_END);
+    const long size = ftell(f);
+    fseek(f, 0, SEEK_SET);
     char magic[4];
-    if (4 != fread(magic, 1, 4, f) || 0 != memcmp(magic, "QSF2", 4)) {
+    if (size < 4 || 4 != fread(magic, 1, 4, f) || 0 != memcmp(magic, "QSF2", 4)) {
         fclose(f);
         SFT_ERR_RET(NULL, "invalid QSF2 magic in '%s'\n", path);
     }
-    u32 version = 0;
-    rd(f, &version, 4);
-    if (2 != version) {
-        fclose(f);
-        SFT_ERR_RET(NULL, "unsupported QSF2 version\n");
+    QsfReader r{f, (u64)size - 4u};
+    sft_model* model = nullptr;
+    try {
+        u32 version = 0;
+        r.read(&version, 4);
+        if (2 != version) {
+            fclose(f);
+            SFT_ERR_RET(NULL, "unsupported QSF2 version\n");
+        }
+        model = load_qsf(c, r, arena);
     }
+    catch (const QsfBad& e) {
+        fclose(f);
+        SFT_ERR_RET(NULL, "corrupt QSF2 file '%s': %s\n", path, e.what);
+    }
+    fclose(f);
+    return model;
+}
+
+static sft_model* load_qsf(Context& c, QsfReader& r, sft_arena* arena) {
     u64 game_data_offset = 0;
-    rd(f, &game_data_offset, 8);
+    r.read(&game_data_offset, 8);
 
     u32 tex_count = 0;
-    rd(f, &tex_count, 4);
+    r.read(&tex_count, 4);
+    r.fits(tex_count, 8);
     sft_mat_handle* mats = (sft_mat_handle*)sft_arena_alloc(arena, (u64)tex_count * sizeof(sft_mat_handle) + 1);
     std::vector<u32> px;
     for (u32 i = 0; i < tex_count; i += 1) {
         u32 w = 0, h = 0;
-        rd(f, &w, 4);
-        rd(f, &h, 4);
-        px.resize((u64)w * h);
-        rd(f, px.data(), (u64)w * h * 4);
+        r.read(&w, 4);
+        r.read(&h, 4);
+        px.resize(r.fits((u64)w * h, 4));
+        r.read(px.data(), (u64)w * h * 4);
         mats[i] = sft_material_create(sft_texture_create((i32)w, (i32)h, px.data()), 0xFFFFFFFFu, 2.f);
     }
 
     sft_model* model = (sft_model*)sft_arena_alloc(arena, sizeof(sft_model));
     memset(model, 0, sizeof(sft_model));
-    rd(f, &model->lightmap_count, 4);
+    r.read(&model->lightmap_count, 4);
     if (model->lightmap_count > 0) {
+        r.fits(model->lightmap_count, 8);
         model->lightmaps = (sft_tex_handle*)sft_arena_alloc(arena, model->lightmap_count * sizeof(sft_tex_handle));
         for (u32 i = 0; i < model->lightmap_count; i += 1) {
             u32 w = 0, h = 0;
-            rd(f, &w, 4);
-            rd(f, &h, 4);
-            px.resize((u64)w * h);
-            rd(f, px.data(), (u64)w * h * 4);
+            r.read(&w, 4);
+            r.read(&h, 4);
+            px.resize(r.fits((u64)w * h, 4));
+            r.read(px.data(), (u64)w * h * 4);
             model->lightmaps[i] = sft_texture_create((i32)w, (i32)h, px.data());
         }
     }
-    rd(f, &model->mesh_count, 4);
-    rd(f, &model->total_meshlets, 4);
-    rd(f, &model->blas_node_count, 4);
-    rd(f, &model->blas_root_ind, 4);
+    r.read(&model->mesh_count, 4);
+    r.read(&model->total_meshlets, 4);
+    r.read(&model->blas_node_count, 4);
+    r.read(&model->blas_root_ind, 4);
+    r.fits(model->mesh_count, 16);
+    r.fits(model->total_meshlets, 2);
+    r.fits(model->blas_node_count, sizeof(sft_bvh_node));
+    if (model->blas_node_count > 0 && model->blas_root_ind >= model->blas_node_count) throw QsfBad{"BLAS root outside its nodes"};
 
     model->meshes = (sft_mesh*)sft_arena_alloc(arena, model->mesh_count * sizeof(sft_mesh));
     memset(model->meshes, 0, model->mesh_count * sizeof(sft_mesh));
     for (u32 i = 0; i < model->mesh_count; i += 1) {
         sft_mesh* mesh = &model->meshes[i];
-        rd(f, &mesh->meshlet_count, 4);
-        rd(f, &mesh->model_meshlet_offset, 4);
+        r.read(&mesh->meshlet_count, 4);
+        r.read(&mesh->model_meshlet_offset, 4);
         u32 tex_ind = 0;
-        rd(f, &tex_ind, 4);
+        r.read(&tex_ind, 4);
         mesh->material = (tex_ind < tex_count) ? mats[tex_ind] : c.base_mat;
-        rd(f, &mesh->lightmap_page, 4);
-        const u32 mc = mesh->meshlet_count;
+        r.read(&mesh->lightmap_page, 4);
+        if (0xFFFFFFFFu != mesh->lightmap_page && mesh->lightmap_page >= model->lightmap_count) {
+            throw QsfBad{"a mesh names a lightmap page the file lacks"};
+        }
+        const u32 mc = (u32)r.fits(mesh->meshlet_count, sizeof(sft_meshlet) + sizeof(sft_meshlet_attribs) + 384);
+        if ((u64)mesh->model_meshlet_offset + mc > model->total_meshlets) throw QsfBad{"a mesh's meshlets lie outside the model"};
         mesh->meshlets = (sft_meshlet*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet));
-        rd(f, mesh->meshlets, (u64)mc * sizeof(sft_meshlet));
+        r.read(mesh->meshlets, (u64)mc * sizeof(sft_meshlet));
         mesh->meshlet_attribs = (sft_meshlet_attribs*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet_attribs));
-        rd(f, mesh->meshlet_attribs, (u64)mc * sizeof(sft_meshlet_attribs));
+        r.read(mesh->meshlet_attribs, (u64)mc * sizeof(sft_meshlet_attribs));
         mesh->inds = (sft_meshlet_inds*)sft_arena_alloc(arena, (u64)mc * sizeof(sft_meshlet_inds));
         u8 flat[384];
         for (u32 m = 0; m < mc; m += 1) {
-            rd(f, flat, 384);
+            r.read(flat, 384);
             sft_meshlet_inds* mi = &mesh->inds[m];
             memset(mi, 0, sizeof(*mi));
             for (u32 t = 0; t < 128; t += 1) {
@@ -1198,18 +1264,20 @@ extern "C" sft_model* sft_model_load_qsf(const char* path, sft_arena* arena) {
             &mesh->cone_axis_x, &mesh->cone_axis_y, &mesh->cone_axis_z, &mesh->cone_cutoff };
         for (f32** a : groups) {
             *a = (f32*)sft_arena_alloc(arena, (u64)mc * sizeof(f32));
-            rd(f, *a, (u64)mc * sizeof(f32));
+            r.read(*a, (u64)mc * sizeof(f32));
         }
     }
     if (model->total_meshlets > 0) {
         model->meshlet_mesh_inds = (u16*)sft_arena_alloc(arena, model->total_meshlets * sizeof(u16));
-        rd(f, model->meshlet_mesh_inds, model->total_meshlets * sizeof(u16));
+        r.read(model->meshlet_mesh_inds, model->total_meshlets * sizeof(u16));
+        for (u32 k = 0; k < model->total_meshlets; k += 1) {
+            if (model->meshlet_mesh_inds[k] >= model->mesh_count) throw QsfBad{"a meshlet names a mesh the model lacks"};
+        }
     }
     if (model->blas_node_count > 0) {
         model->blas_nodes = (sft_bvh_node*)sft_arena_alloc(arena, model->blas_node_count * sizeof(sft_bvh_node));
-        rd(f, model->blas_nodes, model->blas_node_count * sizeof(sft_bvh_node));
+        r.read(model->blas_nodes, model->blas_node_count * sizeof(sft_bvh_node));
     }
-    fclose(f);
     return model;
 }
 

@@ -28,6 +28,11 @@ enum Counter : int {
     CNT_OCC_TRIS,           /* ray-triangle tests */
     CNT_OCC_INVALID,        /* packets whose frustum the reference leaves unset */
     CNT_OCC_OVERFLOW,       /* packets whose traversal stack overflowed */
+    CNT_TEX_NPOT,           /* albedo samples of non-power-of-two textures (edge clamp) */
+    CNT_UNLIT,              /* shadings of unlit materials (negative shininess) */
+    CNT_DIR_LIT,            /* shadings that evaluated directional lights */
+    CNT_LIGHT_CUT,          /* tiles whose light list stopped at 16 with more lights in reach */
+    CNT_BG_BLEND,           /* MSAA pixels blending the caller's colour under empty samples */
     CNT_RASTER_ITEMS,       /* visibility work items (tile, sample, chunk) listed */
     CNT_RASTER_NEXT,        /* queue head of the visibility pass */
     CNT_COUNT

@@ -1228,11 +1228,15 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
     f32 tr, tg, tb;
     if (lod > 1) sample_nearest(P, tex, lod, cu, cv, tr, tg, tb);
     else sample_bilinear(P, tex, lod, cu, cv, tr, tg, tb);
-    if (SFT_INVALID_HANDLE != tex) fetch |= (lod > 1) ? 2u : 1u;
+    if (SFT_INVALID_HANDLE != tex) {
+        fetch |= (lod > 1) ? 2u : 1u;
+        if (!P.tex_meta[tex & SFT_HANDLE_INDEX_MASK].is_pot) fetch |= 8u;
+    }
     const f32 alb_r = tr * base_r, alb_g = tg * base_g, alb_b = tb * base_b;
 
     f32 fr, fg, fb;
     if (shin < 0.f) {
+        fetch |= 16u;
         fr = alb_r * -shin;
         fg = alb_g * -shin;
         fb = alb_b * -shin;
@@ -1252,6 +1256,7 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
         }
         else {
             dr = fc.ambient[0]; dg = fc.ambient[1]; db = fc.ambient[2];
+            if (fc.n_dir > 0u) fetch |= 32u;
             for (u32 l = 0; l < fc.n_dir; l += 1) {
                 const DirLightGpu& L = P.dir_lights[VW.dl_base + l];
                 const V3 Ld{L.dir[0], L.dir[1], L.dir[2]};
@@ -1447,6 +1452,7 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
     const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
     u32* tl = P.tile_lights + (u64)tile * (kMaxTileLights + 1);
     u32 n = 0;
+    bool cut = false;
     if (zmin <= zmax) {
         for (u32 base = 0; base < fc.n_point && n < (u32)kMaxTileLights; base += 64u) {
             const u32 l = base + (u32)lane;
@@ -1454,10 +1460,14 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
             const u64 m = __ballot(pass);
             const u32 rank = n + lanes_below(m, lane);
             if (pass && rank < (u32)kMaxTileLights) tl[1 + rank] = l;
+            cut = cut || n + (u32)__popcll(m) > (u32)kMaxTileLights;
             n = min(n + (u32)__popcll(m), (u32)kMaxTileLights);
         }
     }
-    if (0 == lane) tl[0] = n;
+    if (0 == lane) {
+        tl[0] = n;
+        if (cut) stat_add(P.counters, CNT_LIGHT_CUT, 1u, stat_slot());
+    }
 }
 
 /* Visibility pass.  A work item is (tile, sample plane, chunk of at most
@@ -1733,6 +1743,20 @@ __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
     build_tile_lights(P, V, tile, zmin, zmax, lane);
 }
 
+/* the wave's shading counters from each lane's `fetch` bits of shade_pixel
+   (1 bilinear, 2 nearest, 4 lightmap, 8 non-power-of-two albedo, 16 unlit,
+   32 directional lights); `act` holds the lanes that shaded */
+DEV void count_fetches(const FramePtrs& P, u32 fetch, u64 act, int lane) {
+    static constexpr int kCnt[6] = { CNT_TEX_BILINEAR, CNT_TEX_NEAREST, CNT_LIGHTMAPPED, CNT_TEX_NPOT, CNT_UNLIT,
+                                     CNT_DIR_LIT };
+    u32 n[6];
+    for (int k = 0; k < 6; k += 1) n[k] = (u32)__popcll(__ballot((fetch >> k) & 1u));
+    if (lane == __ffsll((long long)act) - 1) {
+        const u32 sl = stat_slot();
+        for (int k = 0; k < 6; k += 1) stat_add(P.counters, kCnt[k], n[k], sl);
+    }
+}
+
 /* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
    32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
    order (x offsets 0..3, rows 0..1) */
@@ -1818,22 +1842,16 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     *out = pack_argb(col.x * 255.f, col.y * 255.f, col.z * 255.f);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
-    i32 n_bil = (fetch & 1u) ? 1 : 0, n_near = (fetch & 2u) ? 1 : 0, n_lm = (fetch & 4u) ? 1 : 0;
     /* counters: one atomic per wave (inactive lanes contribute nothing) */
     const u64 act = __ballot(true);
     shaded = __popcll(act);
     groups = __popcll(__ballot(groups != 0));
-    n_bil = __popcll(__ballot(n_bil != 0));
-    n_near = __popcll(__ballot(n_near != 0));
-    n_lm = __popcll(__ballot(n_lm != 0));
     if (lane == __ffsll((long long)act) - 1) {
         const u32 sl = stat_slot();
         stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
         stat_add(P.counters, CNT_QGROUPS, (u32)groups, sl);
-        stat_add(P.counters, CNT_TEX_BILINEAR, (u32)n_bil, sl);
-        stat_add(P.counters, CNT_TEX_NEAREST, (u32)n_near, sl);
-        stat_add(P.counters, CNT_LIGHTMAPPED, (u32)n_lm, sl);
     }
+    count_fetches(P, fetch, act, lane);
 }
 
 /* MSAA resolve (sofren2.c:6048-6135, 6677-6743).  Lanes are laid out as in
@@ -1920,7 +1938,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     const bool write = !idle && (fc.should_clear ? in_bounds : (0u != mine));
     const f32 fbx = (f32)bx + 0.5f, fby = (f32)by + 0.5f;
-    i32 shaded = 0, groups = 0, n_bil = 0, n_near = 0, n_lm = 0;
+    i32 shaded = 0, groups = 0;
+    /* six 5-bit counts of shade_pixel's fetch bits (a lane shades at most
+       16 items: 4 waves x 256 items over 64 lanes) */
+    u32 fetches = 0;
     u32* it_vid = s_item_vid[wave];
     u32* it_meta = s_item_meta[wave];
 
@@ -2034,9 +2055,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 s_item_der[iw][2][li] = col.z;
                 shaded += 1;
                 groups += (li == gs) ? 1 : 0;
-                n_bil += (fetch & 1u) ? 1 : 0;
-                n_near += (fetch & 2u) ? 1 : 0;
-                n_lm += (fetch & 4u) ? 1 : 0;
+                for (int k = 0; k < 6; k += 1) fetches += ((fetch >> k) & 1u) << (5 * k);
             }
         }
         c0 = end;
@@ -2057,16 +2076,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const u64 act = __ballot(true);
     shaded = wave_sum(shaded);
     groups = wave_sum(groups);
-    n_bil = wave_sum(n_bil);
-    n_near = wave_sum(n_near);
-    n_lm = wave_sum(n_lm);
+    const i32 blended = wave_sum((write && !fc.should_clear && ew > 0.f) ? 1 : 0);
+    u32 fsum[6];
+    for (int k = 0; k < 6; k += 1) fsum[k] = (u32)wave_sum((i32)((fetches >> (5 * k)) & 31u));
     if (lane == __ffsll((long long)act) - 1) {
         const u32 sl = stat_slot();
         stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
         stat_add(P.counters, CNT_QGROUPS, (u32)groups, sl);
-        stat_add(P.counters, CNT_TEX_BILINEAR, (u32)n_bil, sl);
-        stat_add(P.counters, CNT_TEX_NEAREST, (u32)n_near, sl);
-        stat_add(P.counters, CNT_LIGHTMAPPED, (u32)n_lm, sl);
+        stat_add(P.counters, CNT_BG_BLEND, (u32)blended, sl);
+        static constexpr int kCnt[6] = { CNT_TEX_BILINEAR, CNT_TEX_NEAREST, CNT_LIGHTMAPPED, CNT_TEX_NPOT, CNT_UNLIT,
+                                         CNT_DIR_LIT };
+        for (int k = 0; k < 6; k += 1) stat_add(P.counters, kCnt[k], fsum[k], sl);
     }
 }
 

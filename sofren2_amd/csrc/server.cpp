@@ -285,7 +285,7 @@ extern "C" u64 sftx_server_submit(const sftx_request* r) {
     Server* sv = g_server;
     if (!sv || !sv->running || !r) return 0;
     std::lock_guard<std::mutex> lk(sv->mu);
-    if (r->scene < 0 || (size_t)r->scene >= sv->templates.size() || r->width <= 0 || r->height <= 0 ||
+    if (r->scene < 0 || (size_t)r->scene >= sv->templates.size() || !view_size_ok(r->width, r->height) ||
         (1 != r->samples && 2 != r->samples && 4 != r->samples)) {
         fprintf(stderr, "sofren2: invalid render request (scene %d, %dx%d, %d samples)\n", r->scene, r->width, r->height,
                 r->samples);

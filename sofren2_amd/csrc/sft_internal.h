@@ -31,6 +31,10 @@ namespace sft {
 
 constexpr u32 kInvalid = 0xFFFFFFFFu;
 constexpr int kMaxTileLights = 16;
+/* tile rectangles of payloads and clipped triangles pack each 64-px tile
+   coordinate into 8 bits: a view is at most 256 tiles (16384 px) on a side */
+constexpr i32 kMaxViewSide = 256 * SFT_TILE_SIZE;
+inline bool view_size_ok(i32 w, i32 h) { return w > 0 && h > 0 && w <= kMaxViewSide && h <= kMaxViewSide; }
 
 /* GPU-side per-frame constants. */
 struct FrameConst {
@@ -192,6 +196,7 @@ struct ViewRecord {
     u8 shadows_enabled = 0;
     std::vector<WorldProxy> proxies;
     std::vector<sft_instance> instances;
+    u32 dropped = 0;                /* draws beyond the 64-instance cap (sofren2.c:8412-8417) */
     u32 total_ss = 0;
     std::vector<const sft_model*> transient_models;
     std::vector<u32> tmat_tex, tmat_color;
@@ -318,6 +323,7 @@ struct Context {
 
     /* batched views being recorded (sftx_batch_begin .. sftx_batch_submit) */
     bool batch_open = false;
+    bool view_rejected = false;     /* the view begun last has an unsupported size */
     bool server_running = false;    /* the request server owns the backend */
     std::vector<ViewRecord> batch_views;
     /* batches submitted with sftx_batch_submit_async, oldest first (at most

@@ -92,14 +92,15 @@ typedef struct {
 } sftx_crowd_state;
 
 /* the features workload: every shading branch the other scenes miss */
-#define SFTX_FEAT_OBJECTS 66
+#define SFTX_FEAT_OBJECTS 60
+#define SFTX_FEAT_EXTRA 6
 #define SFTX_FEAT_LIGHTS 24
 typedef struct {
     sft_tex_handle tex[6];
     sft_mat_handle mat[8];
-    i32 obj_kind[SFTX_FEAT_OBJECTS], obj_mat[SFTX_FEAT_OBJECTS];
-    sftmat obj_xform[SFTX_FEAT_OBJECTS];
-    f32 obj_param[SFTX_FEAT_OBJECTS][4];
+    i32 obj_kind[SFTX_FEAT_OBJECTS + SFTX_FEAT_EXTRA], obj_mat[SFTX_FEAT_OBJECTS + SFTX_FEAT_EXTRA];
+    sftmat obj_xform[SFTX_FEAT_OBJECTS + SFTX_FEAT_EXTRA];
+    f32 obj_param[SFTX_FEAT_OBJECTS + SFTX_FEAT_EXTRA][4];
     sftvec light_pos[SFTX_FEAT_LIGHTS], light_col[SFTX_FEAT_LIGHTS];
     f32 light_radius[SFTX_FEAT_LIGHTS];
 } sftx_features_state;
@@ -1008,15 +1009,21 @@ static void sftx_map_render(i32 frame) {
    instead of wrap, sofren2.c:5594-5599; bilinear near the camera, nearest on
    the far floor), materials with negative shininess (unlit, 6416-6423), two
    directional lights (6440-6470), 24 point lights of which 20 overlap the
-   middle tiles (the 16-light cut of a tile's list, 5929-5931), and 70 draw
-   calls of which the last six exceed the 64 instances of the default ID
-   split and are dropped (8412-8417).  The view does not clear
+   middle tiles (the 16-light cut of a tile's list, 5929-5931), and 64 draw
+   calls, the most the default ID split holds.  Frames 500-999 (camera frame
+   f - 500) add six draws that stand between the camera and the scene and
+   must be dropped (8412-8417); the reference itself cannot render those
+   frames: its TLAS build walks every counted instance, dropped ones
+   included, and reads their never-written slots (3630-3634).  The view does not clear
    (sft_set_clear(0)): uncovered pixels keep the caller's buffer and, with
    MSAA, partly covered ones blend with it (6067-6124, 6732-6743).  Every
    object is dmesh geometry: ground plane, three poster quads, and spheres,
    cubes, inverted cubes, capsules, cylinders and triangle fans.  UVs on
-   the non-power-of-two textures stay well above 0 (the reference indexes
-   before the level for negative texel coordinates). */
+   the non-power-of-two textures stay well above 0, also where the
+   reference extrapolates them to block lanes off the triangle, whose
+   samples it computes and discards: it indexes before the level for
+   negative texel coordinates (a 3-texel-wide texture is 1 texel wide at
+   level 1, so u stays above 0.5 there). */
 static void sftx_noise_image(sftx_rng* rng, i32 w, i32 h, u32* px) {
     f32 g[3][32 * 32];
     const i32 gs[3] = { 4, 8, 16 };
@@ -1063,19 +1070,19 @@ static void sftx_features_create(void) {
         g_feat.obj_kind[i] = i % 6;
         /* primitives have UVs from 0: power-of-two textures or none */
         static const i32 prim_mats[5] = { 4, 5, 6, 7, 4 };
-        g_feat.obj_mat[i] = (5 == i % 6) ? 3 : prim_mats[(i / 6) % 5];
+        g_feat.obj_mat[i] = (5 == i % 6) ? ((i / 6) % 2 ? 4 : 6) : prim_mats[(i / 6) % 5];
         const f32 sc = sftx_uniform(&rng, 0.4f, 1.2f);
         const f32 x = sftx_uniform(&rng, -9.f, 9.f), z = sftx_uniform(&rng, -6.f, 10.f);
         g_feat.obj_xform[i] = sft_m_mul(sft_m_mul(sft_m_scale(sc, sc, sc), sft_m_rot_y(sftx_uniform(&rng, 0.f, 6.2831853f))),
                                         sft_m_translate(x, sc * 0.6f, z));
         for (i32 k = 0; k < 4; k += 1) g_feat.obj_param[i][k] = sftx_uniform(&rng, 0.2f, 1.f);
     }
-    /* the last six draws stand between the camera and the scene: were they
+    /* the extra draws stand between the camera and the scene: were they
        not dropped, they would cover much of the frame */
-    for (i32 i = SFTX_FEAT_OBJECTS - 6; i < SFTX_FEAT_OBJECTS; i += 1) {
+    for (i32 i = SFTX_FEAT_OBJECTS; i < SFTX_FEAT_OBJECTS + SFTX_FEAT_EXTRA; i += 1) {
         g_feat.obj_kind[i] = 0;
         g_feat.obj_mat[i] = 6;
-        const f32 x = -5.f + 2.f * (f32)(i - (SFTX_FEAT_OBJECTS - 6));
+        const f32 x = -5.f + 2.f * (f32)(i - SFTX_FEAT_OBJECTS);
         g_feat.obj_xform[i] = sft_m_mul(sft_m_scale(1.5f, 1.5f, 1.5f), sft_m_translate(x, 2.5f, -6.f));
     }
     for (i32 l = 0; l < SFTX_FEAT_LIGHTS; l += 1) {
@@ -1091,10 +1098,13 @@ static void sftx_features_create(void) {
 }
 
 static void sftx_features_render(i32 frame) {
+    const int extra = frame >= 500 && frame < 1000;
+    if (extra) frame -= 500;
     i32 pose, cam;
     sftx_split_frame(frame, &pose, &cam);
-    const f32 th = 0.05f * (f32)(cam % 120);
-    const sftvec pos = SFT_V(9.f * sinf(th), 5.f, -13.f * cosf(th));
+    /* a front arc of +-0.6 rad: the posters never turn edge-on */
+    const f32 th = 0.6f * sinf(2.f * (f32)SFT_PI * (f32)(cam % 120) / 120.f);
+    const sftvec pos = SFT_V(13.f * sinf(th), 5.f, -13.f * cosf(th));
     sft_set_camera(pos, sft_m_view_look_at(pos, SFT_V(0.f, 1.f, 0.f), SFT_V(0.f, 1.f, 0.f)),
                    sft_m_proj(60.f, (f32)g_scene.width / (f32)g_scene.height, 0.1f, 1000.f));
     sft_set_ambient_light(0.15f, 0.15f, 0.18f);
@@ -1120,7 +1130,7 @@ static void sftx_features_render(i32 frame) {
        material, and a 4-normal quad */
     sft_dmesh* p0 = sft_dmesh_begin(SFT_DMESH_QUADS, g_feat.mat[1], 1);
     sft_dmesh_push_quad(p0, SFT_V(-7.f, 0.5f, 6.f), SFT_V(-7.f, 4.5f, 6.f), SFT_V(-3.f, 4.5f, 7.f), SFT_V(-3.f, 0.5f, 7.f),
-                        0.3f, 0.3f, 1.4f, 1.2f, SFT_V(0.24f, 0.f, -0.97f));
+                        0.6f, 0.3f, 1.4f, 1.2f, SFT_V(0.24f, 0.f, -0.97f));
     sft_dmesh_end(p0, NULL);
     sft_dmesh* p1 = sft_dmesh_begin(SFT_DMESH_QUADS, sft_material_clone(g_feat.mat[2]), 1);
     sft_dmesh_push_quad(p1, SFT_V(2.f, 0.2f, 7.f), SFT_V(2.f, 3.8f, 7.f), SFT_V(6.5f, 3.8f, 5.5f), SFT_V(6.5f, 0.2f, 5.5f),
@@ -1128,10 +1138,10 @@ static void sftx_features_render(i32 frame) {
     sft_dmesh_end(p1, NULL);
     sft_dmesh* p2 = sft_dmesh_begin(SFT_DMESH_QUADS, g_feat.mat[3], 1);
     sft_dmesh_push_quad_4n(p2, SFT_V(-2.f, 0.05f, 2.f), SFT_V(-2.f, 2.5f, 3.f), SFT_V(1.5f, 2.5f, 3.f), SFT_V(1.5f, 0.05f, 2.f),
-                           0.3f, 0.4f, 1.1f, 1.3f, sft_v_norm(SFT_V(-0.5f, 0.2f, -1.f)), sft_v_norm(SFT_V(-0.4f, 0.6f, -1.f)),
+                           0.45f, 0.5f, 1.1f, 1.3f, sft_v_norm(SFT_V(-0.5f, 0.2f, -1.f)), sft_v_norm(SFT_V(-0.4f, 0.6f, -1.f)),
                            sft_v_norm(SFT_V(0.5f, 0.6f, -1.f)), sft_v_norm(SFT_V(0.4f, 0.2f, -1.f)));
     sft_dmesh_end(p2, NULL);
-    for (i32 i = 0; i < SFTX_FEAT_OBJECTS; i += 1) {
+    for (i32 i = 0; i < SFTX_FEAT_OBJECTS + (extra ? SFTX_FEAT_EXTRA : 0); i += 1) {
         const f32* q = g_feat.obj_param[i];
         const sft_mat_handle mat = g_feat.mat[g_feat.obj_mat[i]];
         sft_dmesh* dm;
@@ -1158,7 +1168,9 @@ static void sftx_features_render(i32 frame) {
                                   0.1f + 0.3f * q[2], 10);
                 break;
             default: {
-                /* a fan of 12 triangles, UVs in [0.3, 1.5] on an NPOT texture */
+                /* a fan of 12 triangles (a power-of-two texture or none: far,
+                   small fans extrapolate UVs below 0 in lanes the reference
+                   computes but does not store) */
                 dm = sft_dmesh_begin(SFT_DMESH_TRIANGLES, mat, 12);
                 for (i32 k = 0; k < 12; k += 1) {
                     const f32 a0 = 6.2831853f * (f32)k / 12.f, a1 = 6.2831853f * (f32)(k + 1) / 12.f;

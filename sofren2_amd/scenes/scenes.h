@@ -29,8 +29,9 @@
  *       at (0, 2, 0).
  *   SFTX_SCENE_FEATURES (5): 640x360 dmesh scene for the shading branches the
  *       others miss: non-power-of-two textures, unlit materials, directional
- *       lights, more than 16 point lights on a tile, 70 draws (6 dropped),
- *       no clearing (the caller's buffer shows through and blends under MSAA).
+ *       lights, more than 16 point lights on a tile, 64 draws (frames
+ *       500-999: six more, which must be dropped), no clearing (the caller's
+ *       buffer shows through and blends under MSAA).
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H

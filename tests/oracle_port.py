@@ -44,8 +44,10 @@ def port_lib():
     return _PORT
 
 
-def port_render_desc(desc, clip_cap=1 << 16):
-    """Renders an exported frame description; returns the dump dict."""
+def port_render_desc(desc, clip_cap=1 << 16, prefill=0):
+    """Renders an exported frame description; returns the dump dict.  A
+    non-zero `prefill` starts the colour buffer as the reference dump's
+    pre-filled pattern (refrun.prefill_pattern)."""
     lib = port_lib()
     cols, rows, w, h = (ctypes.c_int32() for _ in range(4))
     lib.sftx_port_grid(desc, ctypes.byref(cols), ctypes.byref(rows), ctypes.byref(w), ctypes.byref(h))
@@ -54,6 +56,9 @@ def port_render_desc(desc, clip_cap=1 << 16):
     samples = lib.sftx_port_samples(desc)
     grid = cols.value * 64 * rows.value * 64 * samples
     color = np.zeros((h.value, w.value), dtype=np.uint32)
+    if prefill:
+        from tests.refrun import prefill_pattern
+        color[:] = prefill_pattern(h.value, w.value, prefill)
     vid = np.zeros(grid, dtype=np.uint32)
     depth = np.zeros(grid, dtype=np.float32)
     ss = np.zeros(max(n_ss, 1) * S.SS_MESHLET_BYTES, dtype=np.uint8)
@@ -72,7 +77,7 @@ def port_render_desc(desc, clip_cap=1 << 16):
             "proxy_vis": proxy_vis[:n_proxies], "invalid_packets": out.invalid_packets}
 
 
-def port_render_scene(scene, seed, width, height, frame=0, samples=1):
+def port_render_scene(scene, seed, width, height, frame=0, samples=1, prefill=0):
     """Records a frame with the product front-end and renders it with the restatement (no GPU)."""
     lib = S.library()
     lib.sft_init(width, height, 0)
@@ -84,7 +89,7 @@ def port_render_scene(scene, seed, width, height, frame=0, samples=1):
         lib.scene_record(frame, color.ctypes.data, width * 4)
         desc = ctypes.create_string_buffer(DESC_BYTES)
         lib.export_frame(desc)
-        out = port_render_desc(desc)
+        out = port_render_desc(desc, prefill=prefill)
         lib.sft_frame_end()
         lib.scene_destroy()
     finally:

@@ -49,7 +49,8 @@ def test_ctypes_structs_match_the_header(tmp_path):
     checks = {
         "sftx_tuning": (S.Tuning, ["raster_chunk", "raster_groups", "raster_groups_resident", "tile_px",
                                    "occlusion_stack", "ms_class", "ms_batch"]),
-        "sftx_frame_stats": (S.FrameStats, ["meshlets_tested", "shadow_proxies", "ms_upload", "ms_shadows"]),
+        "sftx_frame_stats": (S.FrameStats, ["meshlets_tested", "shadow_proxies", "ms_upload", "ms_shadows", "tex_npot",
+                                            "instances_dropped"]),
         "sftx_request": (S.Request, ["scene", "camera_pos", "view", "proj", "color", "vis64"]),
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "sofren2_ext.h"', "int main(void) {"]

@@ -11,7 +11,7 @@ import pytest
 import sofren2_amd as S
 from tests import parity
 from tests.oracle_port import port_render_scene
-from tests.refrun import ref_dump, have_ref
+from tests.refrun import ref_dump, have_ref, prefill_pattern
 
 pytestmark = pytest.mark.skipif(not have_ref(), reason="reference checker build missing")
 
@@ -110,3 +110,38 @@ def test_port_matches_reference_map_file(frame):
     assert rep["color_exact_frac"] == 1.0, rep
     covered = (ref["vid"] != 0xFFFFFFFF).mean()
     assert covered > 0.3, covered
+
+
+@pytest.mark.parametrize("samples", [1, 2, 4])
+@pytest.mark.parametrize("frame", [0, 37])
+def test_port_matches_reference_features(frame, samples):
+    """The features workload: non-power-of-two textures (edge clamp,
+    bilinear and nearest), unlit materials, directional lights, more than 16
+    lights reaching a tile, no clearing over a pre-filled buffer with 64
+    bytes of row padding (the reference leaves the padding alone)."""
+    if not have_ref(samples):
+        pytest.skip("MSAA reference build missing")
+    w, h = S.SCENE_SIZE[S.SCENE_FEATURES]
+    port = port_render_scene(S.SCENE_FEATURES, 0, w, h, frame, samples=samples, prefill=7)
+    ref = ref_dump(S.SCENE_FEATURES, 0, w, h, frame, samples=samples, pad=64, prefill=7)
+    rep = parity.compare_frame(port, ref)
+    print(rep)
+    parity.assert_parity(rep)
+    assert rep["color_exact_frac"] == 1.0, rep
+    assert np.array_equal(ref["color_buffer"][:, w:], prefill_pattern(h, w + 16, 7)[:, w:])
+    covered = (ref["vid"] != 0xFFFFFFFF).mean()
+    assert 0.2 < covered < 0.6, covered
+    # the caller's buffer shows through where nothing is drawn
+    assert (ref["color"] == prefill_pattern(h, w, 7)).mean() > 0.3
+
+
+def test_port_drops_instances_beyond_64():
+    """Frames 500-999 of the features workload draw six more instances after
+    the 64 the visibility-ID split holds; the front-end drops them, so the
+    frame equals the same camera frame without them (the reference itself
+    cannot render such a frame: its TLAS build reads the dropped slots)."""
+    w, h = S.SCENE_SIZE[S.SCENE_FEATURES]
+    a = port_render_scene(S.SCENE_FEATURES, 0, w, h, 537, prefill=7)
+    b = port_render_scene(S.SCENE_FEATURES, 0, w, h, 37, prefill=7)
+    for k in ("color", "vid", "depth"):
+        assert np.array_equal(a[k], b[k]), k
