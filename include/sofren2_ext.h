@@ -236,6 +236,37 @@ u64 sftx_graph_captures(void);
 /* Waits for all GPU work of the renderer. */
 void sftx_synchronize(void);
 
+/* Rendering without allocation.  No counterpart in the reference; it is for
+   callers that cannot allocate device memory: code under HIP graph capture,
+   tight serving loops, hosts that forbid allocation after start-up.
+   sftx_present_noalloc renders the frame recorded since sft_frame_begin
+   exactly as sft_present with host output does (the colour into the buffer
+   of sft_set_color_buffer, at its pitch), but calls no hipMalloc, hipFree or
+   pinned-memory allocation: every per-frame intermediate is carved from one
+   device array of sftx_noalloc_scratch_bytes() (64 MiB) that the library
+   holds on the GPU for the life of the process, tables are staged in
+   ordinary host memory and kernels are launched eagerly.  It returns 0, or
+   a negative SFTX_NOALLOC_* code with a message and nothing rendered:
+     UNREADY        before sft_init or the first sft_present (which sets up
+                    the renderer), inside a batch, or while the server runs;
+     IN_FLIGHT      device-output frames or batches still in flight
+                    (call sftx_synchronize first);
+     NOT_RESIDENT   the frame draws models or textures not yet resident, or
+                    more dmesh rows than any earlier frame (present the scene
+                    once with sft_present);
+     TOO_LARGE      the frame's intermediates exceed the scratch array (a
+                    1920x1080 box-room frame without MSAA fits).
+   Captures and 64-bit visibility are not written on this path.  Every call
+   shares the array and the renderer's streams: calls must not overlap (from
+   several threads, or with other rendering calls); each call clears what
+   must start at zero itself, whatever the previous call left. */
+#define SFTX_NOALLOC_UNREADY       (-1)
+#define SFTX_NOALLOC_IN_FLIGHT     (-2)
+#define SFTX_NOALLOC_NOT_RESIDENT  (-3)
+#define SFTX_NOALLOC_TOO_LARGE     (-4)
+i32 sftx_present_noalloc(void);
+u64 sftx_noalloc_scratch_bytes(void);
+
 /* Test hook: the capacities every frame set starts from for clipped
    triangles and tile-list entries (defaults 65536 and 1048576).  The kernels
    flag a batch whose intermediates exceed them and the batch is run again

@@ -31,6 +31,14 @@
 
 namespace sft {
 
+/* Device memory of the no-allocation entry point (sftx_present_noalloc):
+   every per-frame intermediate of its frames is carved from this array,
+   which each GPU the library runs on holds until the process ends.  Host
+   code reaches it only through the address hipGetSymbolAddress returns for
+   the device (Backend::scratch), never through the array's name. */
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) u8 scratch[kScratchBytes];
+
 struct DBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -63,6 +71,7 @@ static void upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
 struct Packer {
     u8* host = nullptr;
     size_t cap = 0, used = 0;
+    bool pinned = true;             /* false: ordinary host memory (the no-allocation path) */
     std::vector<std::pair<size_t, size_t>> pending;   /* (offset, bytes) */
     size_t add(const void* src, size_t bytes) {
         const size_t off = (used + 255) & ~(size_t)255;
@@ -84,13 +93,22 @@ struct Packer {
     void grow(size_t need) {
         size_t nc = std::max<size_t>(need * 2, 1 << 20);
         u8* nh = nullptr;
-        HIP_CHECK(hipHostMalloc((void**)&nh, nc, hipHostMallocDefault));
+        if (pinned) HIP_CHECK(hipHostMalloc((void**)&nh, nc, hipHostMallocDefault));
+        else if (!(nh = (u8*)malloc(nc))) {
+            fprintf(stderr, "fatal sofren2 error: out of host memory for %zu bytes of batch tables\n", nc);
+            exit(1);
+        }
         if (host) {
             memcpy(nh, host, cap);
-            HIP_CHECK(hipHostFree(host));
+            release();
         }
         host = nh;
         cap = nc;
+    }
+    void release() {
+        if (host && pinned) HIP_CHECK(hipHostFree(host));
+        else if (host) free(host);
+        host = nullptr;
     }
 };
 
@@ -111,6 +129,11 @@ struct FrameSet {
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
     bool in_flight = false;
+    u32 region = 0;                 /* transient region of the pools this set's batches use */
+    /* non-null: the per-batch device buffers are carved from this device
+       memory (the no-allocation path) instead of being allocated */
+    u8* carve_base = nullptr;
+    size_t carve_bytes = 0;
 
     DBuf ss, ss_vis, clip_count, clip_base, clip_mask, batch_pl, bin_m;
     DBuf tile_off, tile_count, tile_list, clip_rec, clip_key, clip_rect, color, vid, tile_lights, cap_depth;
@@ -191,6 +214,11 @@ struct Backend {
 
     /* frames and batches in flight, one set each; blocking calls use set 0 */
     FrameSet sets[kFrameSets];
+    /* the no-allocation path: its buffers are carved from `scratch` (the
+       device address of the array above on this device); it uses set 0's
+       transient region, and runs only with no frame or batch in flight */
+    FrameSet noalloc;
+    u8* scratch = nullptr;
 
     /* host copies of the last frame's intermediates (capture) */
     i32 cap_cols = 0, cap_rows = 0, cap_samples = 1;
@@ -208,13 +236,29 @@ Backend* backend_create() {
         exit(1);
     }
     HIP_CHECK(hipSetDevice(0));
-    for (FrameSet& fs : b->sets) {
+    auto init_set = [](FrameSet& fs) {
         HIP_CHECK(hipStreamCreateWithFlags(&fs.stream, hipStreamNonBlocking));
         for (hipEvent_t& e : fs.ev) HIP_CHECK(hipEventCreate(&e));
         HIP_CHECK(hipStreamCreateWithFlags(&fs.side.stream, hipStreamNonBlocking));
         for (hipEvent_t* e : { &fs.side.fork, &fs.side.join, &fs.side.start, &fs.side.end }) HIP_CHECK(hipEventCreate(e));
         HIP_CHECK(hipHostMalloc((void**)&fs.counters_h, CNT_COUNT * 4, hipHostMallocDefault));
+    };
+    for (int k = 0; k < kFrameSets; k += 1) {
+        init_set(b->sets[k]);
+        b->sets[k].region = (u32)k;
     }
+    /* the scratch array's address on this device (the array's name is a
+       host-side stand-in) */
+    void* sp = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    b->scratch = (u8*)sp;
+    init_set(b->noalloc);
+    b->noalloc.region = 0;
+    b->noalloc.carve_base = b->scratch;
+    b->noalloc.carve_bytes = kScratchBytes;
+    b->noalloc.pack.pinned = false;
+    b->noalloc.clip_cap = 1u << 12;
+    b->noalloc.tile_list_cap = 1u << 17;
     b->raster_groups = raster_groups_per_device();
     return b;
 }
@@ -231,6 +275,7 @@ void backend_destroy(Backend* b) {
         if (fs.copier.joinable()) fs.copier.join();
         HIP_CHECK(hipStreamSynchronize(fs.stream));
     }
+    HIP_CHECK(hipStreamSynchronize(b->noalloc.stream));
     for (DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d, &b->tex_stage,
                      &b->est_rcp, &b->est_rsq, &b->blas }) {
         free_buf(*d);
@@ -242,7 +287,14 @@ void backend_destroy(Backend* b) {
                          &fs.scan_sums, &fs.raster_items, &fs.tile_nch, &fs.merge, &fs.pack_d, &fs.zero_d }) {
             free_buf(*d);
         }
-        if (fs.pack.host) HIP_CHECK(hipHostFree(fs.pack.host));
+    }
+    /* the no-allocation set's device buffers lie in the scratch array */
+    std::vector<FrameSet*> all;
+    for (FrameSet& fs : b->sets) all.push_back(&fs);
+    all.push_back(&b->noalloc);
+    for (FrameSet* fsp : all) {
+        FrameSet& fs = *fsp;
+        fs.pack.release();
         if (fs.counters_h) HIP_CHECK(hipHostFree(fs.counters_h));
         if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
         if (fs.graph_exec) HIP_CHECK(hipGraphExecDestroy(fs.graph_exec));
@@ -615,14 +667,15 @@ static bool is_transient(const ViewRecord& v, const sft_model* m) {
     return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
 }
 
-/* Host side of a batch: residency, then every view's rows of the batch
-   tables packed into the set's pinned staging. */
-static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
-    const u32 nv = (u32)views.size();
-    /* shared pools change only with no batch in flight */
-    bool shared_change = !b->est_uploaded || texture_changes(b, c);
-    /* transient rows need no quiesce unless this set's region must grow */
-    u32 need_t = 0, need_tskin = 0;
+/* Whether preparing these views changes state the frame sets share, and so
+   waits for every batch in flight and may allocate: estimate tables or
+   textures to upload, models to make resident, or dmesh rows beyond the
+   transient regions (their row and skin counts go to need_t, need_tskin). */
+static bool shared_state_changes(const Backend* b, const Context& c, const std::vector<ViewRecord>& views, u32& need_t,
+                                 u32& need_tskin) {
+    bool change = !b->est_uploaded || texture_changes(b, c) || b->texels.cap < 4;
+    need_t = 0;
+    need_tskin = 0;
     for (const ViewRecord& v : views) {
         for (const sft_model* m : v.transient_models) {
             need_t += m->total_meshlets;
@@ -631,16 +684,26 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
             }
         }
     }
-    if (need_t > b->t_cap || need_tskin > b->t_skin_cap) shared_change = true;
+    if (need_t > b->t_cap || need_tskin > b->t_skin_cap) change = true;
     for (const ViewRecord& v : views) {
         for (const sft_instance& in : v.instances) {
             auto it = b->models.find(in.model);
             if (!is_transient(v, in.model) && (it == b->models.end() || it->second.total != in.model->total_meshlets)) {
-                shared_change = true;
+                change = true;
             }
         }
     }
-    if (shared_change) quiesce(b);
+    return change;
+}
+
+/* Host side of a batch: residency, then every view's rows of the batch
+   tables packed into the set's pinned staging. */
+static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
+    const u32 nv = (u32)views.size();
+    /* shared pools change only with no batch in flight; transient rows need
+       no quiesce unless the regions must grow */
+    u32 need_t = 0, need_tskin = 0;
+    if (shared_state_changes(b, c, views, need_t, need_tskin)) quiesce(b);
     hipStream_t s0 = b->sets[0].stream;
     if (!b->est_uploaded) {
         upload(b->est_rcp, c.est.rcp, s0);
@@ -687,9 +750,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         b->t_skin_cap = std::max(b->t_skin_cap, std::max(need_tskin, b->t_skin_cap * 2u));
         grow_pools(b, 0, 0);
     }
-    const u32 set_index = (u32)(&fs - b->sets);
-    fs.t_base = b->res_cap + set_index * b->t_cap;
-    fs.t_skin_base = b->res_skin_cap + set_index * b->t_skin_cap;
+    fs.t_base = b->res_cap + fs.region * b->t_cap;
+    fs.t_skin_base = b->res_skin_cap + fs.region * b->t_skin_cap;
     u32 tslot = fs.t_base, tskin = fs.t_skin_base;
     for (u32 vi = 0; vi < nv; vi += 1) {
         for (const sft_model* m : views[vi].transient_models) {
@@ -1030,44 +1092,79 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.layout_epoch = b->layout_epoch;
 }
 
-/* Device side of a prepared batch: buffers, one H2D copy of the tables,
-   the kernels, and asynchronous copies of the counters and host outputs. */
-static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs, const FrameSet* after = nullptr) {
+/* The per-batch device buffers of a prepared batch, each at least the
+   bytes it needs: grown by allocation (the frame sets), or carved from the
+   set's carve memory (the no-allocation set), or only summed (`measure`):
+   returns the bytes the carve takes. */
+static size_t acquire_buffers(FrameSet& fs, bool measure) {
     hipStream_t s = fs.stream;
-    const u32 nv = (u32)views.size(), nt = fs.nt, nw = fs.nw;
-    HIP_CHECK(hipEventRecord(fs.ev[0], s));
-    dgrow(fs.ss, std::max<size_t>((size_t)fs.ss_total * 2304, 4), false, s);
-    dgrow(fs.ss_vis, std::max<size_t>(fs.ss_total, 4), false, s);
-    dgrow(fs.clip_count, std::max<size_t>((size_t)nw * 4, 4), false, s);
-    dgrow(fs.clip_base, std::max<size_t>((size_t)nw * 4, 4), false, s);
-    dgrow(fs.clip_mask, std::max<size_t>((size_t)nw * 16, 16), false, s);
-    dgrow(fs.batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8), false, s);
-    dgrow(fs.bin_m, std::max<size_t>(fs.bin_m_words * 4, 4), false, s);
-    dgrow(fs.tile_off, (size_t)nt * 4, false, s);
-    dgrow(fs.tile_count, (size_t)nt * 4, false, s);
-    dgrow(fs.scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4, false, s);
-    dgrow(fs.zero_d, fs.zero_words * 4, false, s);
-    dgrow(fs.color, std::max<size_t>(fs.color_total * 4, 4), false, s);
-    dgrow(fs.vid, std::max<size_t>(fs.vid_total * 4, 4), false, s);
-    dgrow(fs.tile_z, (size_t)nt * fs.max_samples * 8, false, s);
-    dgrow(fs.tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4, false, s);
-    dgrow(fs.tile_nch, (size_t)nt * 4, false, s);
-    dgrow(fs.merge, (size_t)nt * fs.max_samples * 4096 * 8, false, s);
-    if (fs.cap_vis) dgrow(fs.cap_depth, fs.vid_total * 4, false, s);
-    if (fs.any_vis64) dgrow(fs.vis64, fs.vid_total * 8, false, s);
-    dgrow(fs.proxy_vis, std::max<size_t>((size_t)fs.n_proxies * 64, 64), false, s);
-    dgrow(fs.light_proxy_count, std::max<size_t>((size_t)fs.n_lights * 4, 4), false, s);
-    dgrow(fs.light_proxies, std::max<size_t>((size_t)fs.n_lights * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)), false, s);
-    dgrow(fs.tile_list, (size_t)fs.tile_list_cap * 8, false, s);
-    dgrow(fs.clip_rec, (size_t)fs.clip_cap * 164, false, s);
-    dgrow(fs.clip_key, (size_t)fs.clip_cap * 4, false, s);
-    dgrow(fs.clip_rect, (size_t)fs.clip_cap * 4, false, s);
-    fs.bg.clip_cap = fs.clip_cap;
-    fs.bg.tile_list_cap = fs.tile_list_cap;
+    const u32 nt = fs.nt, nw = fs.nw;
+    size_t off = 0;
+    auto get = [&](DBuf& d, size_t bytes) {
+        if (!fs.carve_base) {
+            dgrow(d, bytes, false, s);
+            return;
+        }
+        if (!measure) {
+            d.p = fs.carve_base + off;
+            d.cap = bytes;
+        }
+        off = (off + std::max<size_t>(bytes, 4) + 255) & ~(size_t)255;
+    };
+    get(fs.ss, std::max<size_t>((size_t)fs.ss_total * 2304, 4));
+    get(fs.ss_vis, std::max<size_t>(fs.ss_total, 4));
+    get(fs.clip_count, std::max<size_t>((size_t)nw * 4, 4));
+    get(fs.clip_base, std::max<size_t>((size_t)nw * 4, 4));
+    get(fs.clip_mask, std::max<size_t>((size_t)nw * 16, 16));
+    get(fs.batch_pl, std::max<size_t>((size_t)nw * 16 * 8, 8));
+    get(fs.bin_m, std::max<size_t>(fs.bin_m_words * 4, 4));
+    get(fs.tile_off, (size_t)nt * 4);
+    get(fs.tile_count, (size_t)nt * 4);
+    get(fs.scan_sums, ((size_t)std::max(nw, nt) / 4096 + 2) * 4);
+    get(fs.zero_d, fs.zero_words * 4);
+    get(fs.color, std::max<size_t>(fs.color_total * 4, 4));
+    get(fs.vid, std::max<size_t>(fs.vid_total * 4, 4));
+    get(fs.tile_z, (size_t)nt * fs.max_samples * 8);
+    get(fs.tile_lights, (size_t)nt * (kMaxTileLights + 1) * 4);
+    get(fs.tile_nch, (size_t)nt * 4);
+    get(fs.merge, (size_t)nt * fs.max_samples * 4096 * 8);
+    if (fs.cap_vis) get(fs.cap_depth, fs.vid_total * 4);
+    if (fs.any_vis64) get(fs.vis64, fs.vid_total * 8);
+    get(fs.proxy_vis, std::max<size_t>((size_t)fs.n_proxies * 64, 64));
+    get(fs.light_proxy_count, std::max<size_t>((size_t)fs.n_lights * 4, 4));
+    get(fs.light_proxies, std::max<size_t>((size_t)fs.n_lights * kMaxLightProxies * sizeof(ProxyRec), sizeof(ProxyRec)));
+    get(fs.tile_list, (size_t)fs.tile_list_cap * 8);
+    get(fs.clip_rec, (size_t)fs.clip_cap * 164);
+    get(fs.clip_key, (size_t)fs.clip_cap * 4);
+    get(fs.clip_rect, (size_t)fs.clip_cap * 4);
     /* visibility work items: at most one per tile and sample plus one per
        raster_chunk payloads */
     const size_t max_items = ((size_t)nt + fs.tile_list_cap / fs.bg.raster_chunk + 1) * fs.max_samples;
-    dgrow(fs.raster_items, max_items * 8, false, s);
+    get(fs.raster_items, max_items * 8);
+    get(fs.pack_d, fs.pack.used);
+    return off;
+}
+
+/* Device side of a prepared batch: buffers, one H2D copy of the tables,
+   the kernels, and asynchronous copies of the counters and host outputs.
+   A set with carve memory runs its kernels eagerly and leaves its colour on
+   the device; it returns false, having queued nothing, when its buffers do
+   not fit the carve memory. */
+static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs, const FrameSet* after = nullptr) {
+    hipStream_t s = fs.stream;
+    const u32 nv = (u32)views.size();
+    if (fs.carve_base) {
+        const size_t need = acquire_buffers(fs, true);
+        if (need > fs.carve_bytes) {
+            fprintf(stderr, "sofren2: the frame needs %zu bytes of intermediates; the no-allocation path has %zu\n", need,
+                    fs.carve_bytes);
+            return false;
+        }
+    }
+    HIP_CHECK(hipEventRecord(fs.ev[0], s));
+    acquire_buffers(fs, false);
+    fs.bg.clip_cap = fs.clip_cap;
+    fs.bg.tile_list_cap = fs.tile_list_cap;
     for (u32 vi = 0; vi < nv; vi += 1) {
         fs.vgpu[vi].fc.clip_cap = fs.clip_cap;
         fs.vgpu[vi].fc.tile_list_cap = fs.tile_list_cap;
@@ -1075,7 +1172,6 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     Packer& pk = fs.pack;
     memcpy(pk.host + fs.o_batch, &fs.bg, sizeof(fs.bg));
     memcpy(pk.host + fs.o_views, fs.vgpu.data(), fs.vgpu.size() * sizeof(ViewGpu));
-    dgrow(fs.pack_d, pk.used, false, s);
     HIP_CHECK(hipMemcpyAsync(fs.pack_d.p, pk.host, pk.used, hipMemcpyHostToDevice, s));
     u8* pd = (u8*)fs.pack_d.p;
     if (fs.t_slots) {
@@ -1167,8 +1263,23 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
 
     /* kernels after the previous ordered batch's (ev[4]: its last pass) */
     if (after && after->in_flight) HIP_CHECK(hipStreamWaitEvent(s, after->ev[4], 0));
-    run_frame(b, fs, p);
+    if (fs.carve_base) {
+        /* eager launches: instantiating a graph may allocate */
+        launch_frame(p, fs.bg, fs.zero_words, s, fs.ev, &fs.side);
+        HIP_CHECK(hipGetLastError());
+        fs.side_used = fs.bg.occ_total > 0u;
+    }
+    else {
+        run_frame(b, fs, p);
+    }
     HIP_CHECK(hipMemcpyAsync(fs.counters_h, zd, CNT_COUNT * 4, hipMemcpyDeviceToHost, s));
+    if (fs.carve_base) {
+        /* the colour goes to the callers' buffers once the batch is known
+           not to have overflowed (render_noalloc) */
+        HIP_CHECK(hipEventRecord(fs.ev[5], s));
+        fs.in_flight = true;
+        return true;
+    }
 
     /* host outputs: the colour planes (and 64-bit visibility) into pinned staging */
     const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
@@ -1190,6 +1301,7 @@ static void launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     }
     HIP_CHECK(hipEventRecord(fs.ev[5], s));
     fs.in_flight = true;
+    return true;
 }
 
 /* the copies of a launched batch's host outputs out of its pinned staging:
@@ -1225,8 +1337,7 @@ static void copy_spans(const std::vector<Span>& spans) {
    items and instances refer to them by slot: those slot numbers are shifted
    (the dmesh models they came from may be gone by now). */
 static void relocate_transient(Backend* b, FrameSet& fs) {
-    const u32 set = (u32)(&fs - b->sets);
-    const u32 nb = b->res_cap + set * b->t_cap, nsb = b->res_skin_cap + set * b->t_skin_cap;
+    const u32 nb = b->res_cap + fs.region * b->t_cap, nsb = b->res_skin_cap + fs.region * b->t_skin_cap;
     if (fs.t_slots > b->t_cap || fs.t_skins > b->t_skin_cap) {
         fprintf(stderr, "fatal sofren2 error: transient rows exceed their region\n");
         exit(1);
@@ -1245,6 +1356,9 @@ static void relocate_transient(Backend* b, FrameSet& fs) {
     }
     fs.layout_epoch = b->layout_epoch;
 }
+
+static void write_stats(Context& c, const std::vector<ViewRecord>& views, const FrameSet& fs);
+static void capture_outputs(Backend* b, const std::vector<ViewRecord>& views, const FrameSet& fs);
 
 /* Waits for a launched batch; re-runs it with larger intermediates if any
    overflowed; writes the host outputs (unless its copier already did), the
@@ -1274,10 +1388,14 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         copied = false;
     }
     fs.in_flight = false;
-    const u32* counters = fs.counters;
-    const u32 nv = (u32)views.size();
     if (!copied) copy_spans(host_output_spans(views, fs));
+    write_stats(c, views, fs);
+    capture_outputs(b, views, fs);
+}
 
+/* the frame statistics of a completed batch (sftx_get_stats) */
+static void write_stats(Context& c, const std::vector<ViewRecord>& views, const FrameSet& fs) {
+    const u32* counters = fs.counters;
     u64 px = 0, samples = 0;
     for (const ViewRecord& v : views) {
         px += (u64)v.width * v.height;
@@ -1320,8 +1438,13 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
     st.ms_shade = ev_ms(fs.ev[6], fs.ev[4]);
     st.ms_download = ev_ms(fs.ev[4], fs.ev[5]);
     st.ms_total = ev_ms(fs.ev[0], fs.ev[5]);
+}
 
-    /* ---- capture (single-view presents) ---- */
+/* host copies of a completed single-view batch's intermediates, as its
+   capture flags ask */
+static void capture_outputs(Backend* b, const std::vector<ViewRecord>& views, const FrameSet& fs) {
+    const u32* counters = fs.counters;
+    const u32 nv = (u32)views.size();
     if (1 != nv) return;
     const ViewRecord& v0 = views[0];
     const u32 cap_flags = v0.capture_flags;
@@ -1493,6 +1616,56 @@ void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int r
     report->ms_batch = best_ms;
 }
 
+/* sftx_present_noalloc: the recorded frame rendered on the no-allocation
+   set, every intermediate carved from the scratch array, tables staged in
+   ordinary host memory, kernels launched eagerly, the colour copied into
+   the caller's buffer.  Nothing is allocated on the device or pinned on the
+   host: a frame that would need it is refused with an error code. */
+static i32 render_noalloc(Backend* b, Context& c) {
+    if (!c.async_batches.empty()) {
+        fprintf(stderr, "sofren2: sftx_present_noalloc with frames or batches in flight; call sftx_synchronize first\n");
+        return SFTX_NOALLOC_IN_FLIGHT;
+    }
+    std::vector<ViewRecord> views(1);
+    snapshot_view(c, views[0]);
+    ViewRecord& v = views[0];
+    v.capture_flags = 0;
+    v.vis64 = nullptr;
+    v.host_output = true;
+    u32 need_t = 0, need_tskin = 0;
+    if (shared_state_changes(b, c, views, need_t, need_tskin)) {
+        fprintf(stderr, "sofren2: sftx_present_noalloc needs models, textures or dmesh room made resident first "
+                        "(present the scene once with sft_present)\n");
+        return SFTX_NOALLOC_NOT_RESIDENT;
+    }
+    /* the set's transient rows use region 0: nothing else may run */
+    quiesce(b);
+    FrameSet& fs = b->noalloc;
+    prepare_batch(b, c, views, fs);
+    for (int attempt = 0;; attempt += 1) {
+        if (!launch_batch(b, views, fs)) return SFTX_NOALLOC_TOO_LARGE;
+        HIP_CHECK(hipStreamSynchronize(fs.stream));
+        fs.in_flight = false;
+        memcpy(fs.counters, fs.counters_h, sizeof(fs.counters));
+        if (0u == fs.counters[CNT_OVERFLOW]) break;
+        if (attempt > 8) {
+            fprintf(stderr, "fatal sofren2 error: frame intermediates keep overflowing\n");
+            exit(1);
+        }
+        if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
+        if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
+        b->reruns += 1;
+    }
+    if (v.color_buffer && v.width > 0 && v.height > 0) {
+        HIP_CHECK(hipMemcpy2DAsync(v.color_buffer, (size_t)v.color_pitch * 4, (const u32*)fs.color.p + fs.plan[0].color_base,
+                                   (size_t)v.width * 4, (size_t)v.width * 4, v.height, hipMemcpyDeviceToHost, fs.stream));
+        HIP_CHECK(hipStreamSynchronize(fs.stream));
+    }
+    write_stats(c, views, fs);
+    b->last_set = 0;
+    return 0;
+}
+
 u64 backend_resident_bytes(const Backend* b) {
     u64 n = 0;
     for (const DBuf* d : { &b->pos, &b->attr, &b->inds, &b->cull, &b->slot_mesh, &b->skin, &b->texels, &b->tex_meta_d,
@@ -1601,6 +1774,22 @@ extern "C" void sftx_set_initial_caps(u32 clip_cap, u32 tile_list_cap) {
 }
 
 extern "C" u64 sftx_overflow_reruns(void) { return ensure_backend()->reruns; }
+
+extern "C" i32 sftx_present_noalloc(void) {
+    if (!g_ctx) {
+        fprintf(stderr, "sofren2: sftx_present_noalloc before sft_init\n");
+        return SFTX_NOALLOC_UNREADY;
+    }
+    Context& c = *g_ctx;
+    if (!c.backend || c.server_running || c.batch_open) {
+        fprintf(stderr, "sofren2: sftx_present_noalloc needs a renderer set up by sft_present, outside batches and "
+                        "the request server\n");
+        return SFTX_NOALLOC_UNREADY;
+    }
+    return render_noalloc(c.backend, c);
+}
+
+extern "C" u64 sftx_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 extern "C" const u32* sftx_device_color(void) {
     Backend* b = ensure_backend();
