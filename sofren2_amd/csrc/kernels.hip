@@ -753,14 +753,15 @@ struct Tri {
    the pixel centre: edge and depth values gain A*ox + B*oy, trivial accept
    needs a margin of 0.375 (|A| + |B|), and a chunk whose 8 pixel centres are
    all beyond that margin of an edge is skipped.
-   Several groups can share a triangle: the group sweeps the chunks of rows
-   min_y + r, min_y + r + m, ... (m a power of two) but steps the row values
-   through every row, so each row's edge and depth values are the same
-   sequence of additions as in the reference. */
+   Several groups, in this wave and others, can share a triangle: this group
+   sweeps the chunks of rows min_y + phase, min_y + phase + stride, ...
+   (stride a power of two) and steps the row values through every row in
+   between with additions only, so each row's edge and depth values are the
+   same sequence of additions as in the reference. */
 template <bool aa>
 DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u32 key,
                     f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2,
-                    f32 ox, f32 oy, int m, int r) {
+                    f32 ox, f32 oy, int stride, int phase) {
     const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
     const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
     const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
@@ -810,28 +811,38 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
     const f32 fj = (f32)j;
     const f32 dz_chunk = dzdx * 8.f;
     f32 z_row = sz + dzdx * fj;
-    const int row0 = (min_y - tmy) * 64 + (amin - tmx) + j;
-    const i32 rmask = m - 1;
+    const i32 first = min_y + phase;
+    if (first > max_y) return;
+    const int row_step = 64 * stride;
+    int row = (first - tmy) * 64 + (amin - tmx) + j;
     if (trivial) {
-        for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
-            if (((y - min_y) & rmask) == r) {
-                f32 z = z_row;
-                for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                    const f32 zs = aa ? z + dzs : z;
-                    atomicMax((unsigned long long*)&vis[p], (zs > 0.f) ? (((u64)f2u(zs) << 32) | nkey) : 0ull);
-                    z = z + dz_chunk;
-                }
+        /* only the depth row values are needed */
+        for (i32 y = min_y; y < first; y += 1) z_row = z_row + dzdy;
+        for (i32 y = first;;) {
+            f32 z = z_row;
+            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
+                const f32 zs = aa ? z + dzs : z;
+                atomicMax((unsigned long long*)&vis[p], (zs > 0.f) ? (((u64)f2u(zs) << 32) | nkey) : 0ull);
+                z = z + dz_chunk;
             }
-            z_row = z_row + dzdy;
+            const i32 next = y + stride;
+            if (next > max_y) break;
+            for (; y < next; y += 1) z_row = z_row + dzdy;
+            row += row_step;
         }
     }
     else {
         const f32 A0c = A0 * 8.f, A1c = A1 * 8.f, A2c = A2 * 8.f;
         f32 e0r = se0 + A0 * fj, e1r = se1 + A1 * fj, e2r = se2 + A2 * fj;
-        for (i32 y = min_y, row = row0; y <= max_y; y += 1, row += 64) {
+        for (i32 y = min_y; y < first; y += 1) {
+            e0r = e0r + B0;
+            e1r = e1r + B1;
+            e2r = e2r + B2;
+            z_row = z_row + dzdy;
+        }
+        for (i32 y = first;;) {
             f32 e0 = e0r, e1 = e1r, e2 = e2r, z = z_row;
-            const i32 xend = (((y - min_y) & rmask) == r) ? amax : amin;
-            for (i32 x = amin, p = row; x < xend; x += 8, p += 8) {
+            for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
                 if constexpr (!aa) {
                     /* a miss adds 0 to the max: no branch in the chunk loop */
                     const bool hit = e0 >= 0.f && e1 >= 0.f && e2 >= 0.f && z > 0.f;
@@ -851,10 +862,15 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
                 e2 = e2 + A2c;
                 z = z + dz_chunk;
             }
-            e0r = e0r + B0;
-            e1r = e1r + B1;
-            e2r = e2r + B2;
-            z_row = z_row + dzdy;
+            const i32 next = y + stride;
+            if (next > max_y) break;
+            for (; y < next; y += 1) {
+                e0r = e0r + B0;
+                e1r = e1r + B1;
+                e2r = e2r + B2;
+                z_row = z_row + dzdy;
+            }
+            row += row_step;
         }
     }
 }
@@ -1446,7 +1462,10 @@ DEV bool light_in_tile(const FrameConst& fc, const PointLightGpu& L, f32 tmin, f
 
 /* first 16 point lights touching the tile, in index order, by one wave
    (sofren2.c:5862-5934) */
-DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 zmin, f32 zmax, int lane) {
+/* `first` (optional): light `lane` of the view, loaded by the caller ahead of
+   time, for the first 64 lights */
+DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 zmin, f32 zmax, int lane,
+                           const PointLightGpu* first = nullptr) {
     const FrameConst& fc = V.fc;
     const u32 local = tile - V.tile_base;
     const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
@@ -1456,7 +1475,8 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
     if (zmin <= zmax) {
         for (u32 base = 0; base < fc.n_point && n < (u32)kMaxTileLights; base += 64u) {
             const u32 l = base + (u32)lane;
-            const bool pass = l < fc.n_point && light_in_tile(fc, P.point_lights[V.pl_base + l], zmin, zmax, tmx, tmy);
+            const bool pass = l < fc.n_point &&
+                light_in_tile(fc, (first && 0u == base) ? *first : P.point_lights[V.pl_base + l], zmin, zmax, tmx, tmy);
             const u64 m = __ballot(pass);
             const u32 rank = n + lanes_below(m, lane);
             if (pass && rank < (u32)kMaxTileLights) tl[1 + rank] = l;
@@ -1526,6 +1546,9 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
     const i32 gw = fc.tile_cols * 64;
     const u64 plane = V.vid_base + (u64)sample * (u64)gw * (u64)(fc.tile_rows * 64);
     f32 zmin = 1e9f, zmax = -1e9f;
+    /* the light records the list needs, loaded while the pixels are published */
+    PointLightGpu light{};
+    if (0 == wave && fc.samples < 2 && (u32)lane < fc.n_point) light = P.point_lights[V.pl_base + (u32)lane];
     /* all of a thread's pixels first, then their work-item loads together,
        so the thread waits on one round of loads rather than one per pixel */
     constexpr int kPer = SFT_TILE_PIXELS / kRasterThreads;
@@ -1580,7 +1603,7 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
             }
         }
         else {
-            build_tile_lights(P, V, tile, zmin, zmax, lane);
+            build_tile_lights(P, V, tile, zmin, zmax, lane, &light);
         }
     }
 }
@@ -1626,14 +1649,45 @@ DEV RasterTri fetch_raster_tri(const FramePtrs& P, u32 at, u32 g) {
     return tr;
 }
 
+/* bounding box of a staged triangle clamped to the tile, in rows and 8-px
+   chunks; false when it misses the tile (sofren2.c:5236-5246) */
+DEV bool tri_extent(const RasterTri& tr, i32 tmx, i32 tmy, i32& rows, i32& chunks) {
+    if (kInvalid == tr.key) return false;
+    const i32 mnx = cvtt(smax(smin(tr.v[0], smin(tr.v[3], tr.v[6])), (f32)tmx));
+    const i32 mxx = cvtt(smin(smax(tr.v[0], smax(tr.v[3], tr.v[6])), (f32)(tmx + 63)));
+    const i32 mny = cvtt(smax(smin(tr.v[1], smin(tr.v[4], tr.v[7])), (f32)tmy));
+    const i32 mxy = cvtt(smin(smax(tr.v[1], smax(tr.v[4], tr.v[7])), (f32)(tmy + 63)));
+    if (mnx > mxx || mny > mxy) return false;
+    rows = mxy - mny + 1;
+    chunks = (((mxx & ~7) + 8) - (mnx & ~7)) >> 3;
+    return true;
+}
+
+/* groups of a wave that share each of a payload's n triangles (a power of
+   two, 8 / n rounded down) */
+DEV int groups_per_tri(int n) { return (n <= 1) ? 8 : (n <= 2) ? 4 : (n <= 4) ? 2 : 1; }
+
+/* Work of one item's rasterisation: a job is a payload (its 8 triangles on
+   the wave's 8 lane groups) or one of K parts of a payload whose largest
+   triangle would keep one wave busy far longer than the others: part k of
+   K takes every K-th of the rows its groups would sweep.  Waves take jobs
+   from an LDS counter, the parts of split payloads first. */
+constexpr u32 kRasterJobCost = 2048;     /* estimated instructions of one job's sweep */
+constexpr u32 kRasterMaxParts = 8;
+
 __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     /* the work item's triangles, fetched by the whole group at once so the
        waves rasterise from LDS instead of each waiting on its own chain of
        descriptor, work-item, index and vertex loads */
     __shared__ RasterTri s_tri[kRasterMaxChunk * 8];
+    /* jobs: payload | part << 8 | log2(parts) << 12 */
+    __shared__ u16 s_jobs[kRasterMaxChunk * kRasterMaxParts];
+    /* each staged triangle's rows | chunks << 8 in the tile, 0 if it misses it */
+    __shared__ u16 s_ext[kRasterMaxChunk * 8];
     __shared__ f32 s_red[2][kRasterWaves];
-    __shared__ u32 s_item, s_last;
+    __shared__ u32 s_item, s_last, s_njobs, s_next;
+    __shared__ u32 s_tot[2][2];     /* heavy parts and light payloads of waves 0 and 1 */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Est est = make_est(P);
     const u32 n_items = P.counters[CNT_RASTER_ITEMS];
@@ -1657,18 +1711,87 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
 
         u32 off;
         const u32 cnt = tile_payloads(P, tile, off);
+        const u32 tile_chunks = P.tile_nch[tile];
         const u32 e0 = chunk * K, e1 = min(cnt, e0 + K);
         const u32 n_tri = (e1 > e0) ? (e1 - e0) * 8u : 0u;
-        for (u32 q = (u32)tid; q < n_tri; q += kRasterThreads) s_tri[q] = fetch_raster_tri(P, off + e0 + (q >> 3), q & 7u);
+        for (u32 q = (u32)tid; q < n_tri; q += kRasterThreads) {
+            const RasterTri tr = fetch_raster_tri(P, off + e0 + (q >> 3), q & 7u);
+            s_tri[q] = tr;
+            i32 rows = 0, chunks = 0;
+            s_ext[q] = tri_extent(tr, tmx, tmy, rows, chunks) ? (u16)(rows | (chunks << 8)) : (u16)0;
+        }
+        if (0 == tid) s_next = 0u;
         __syncthreads();
-        /* a wave per payload, 8 lanes per group, lane = x inside the 8-px
+        /* each payload's estimated sweep: its groups walk every row of their
+           triangle and run the chunks of 1/(groups per triangle) of them */
+        const u32 n_pl = e1 - e0;
+        u32 parts_log = 0u;
+        if ((u32)tid < n_pl) {
+            int n = 0;
+            u32 cost = 0u;
+            i32 rows[8], chunks[8];
+            for (int k = 0; k < 8; k += 1) {
+                const u32 e = s_ext[(u32)tid * 8u + (u32)k];
+                rows[k] = (i32)(e & 0xFFu);
+                chunks[k] = (i32)(e >> 8);
+                n += e ? 1 : 0;
+            }
+            const int m = groups_per_tri(n);
+            for (int k = 0; k < 8; k += 1) {
+                const u32 c = (u32)rows[k] * 4u + (u32)((rows[k] + m - 1) / m) * (u32)chunks[k] * 20u;
+                cost = max(cost, c);
+            }
+            while (parts_log < 3u && (cost >> parts_log) > kRasterJobCost) parts_log += 1u;
+        }
+        /* job list: the parts of split payloads first, then the others, each
+           in payload order; positions from prefix sums over waves 0 and 1
+           (at most 128 payloads, one per thread) */
+        const u32 n_parts = ((u32)tid < n_pl) ? (1u << parts_log) : 0u;
+        const u32 heavy = parts_log ? n_parts : 0u, light = (n_parts && !parts_log) ? 1u : 0u;
+        u32 hx = heavy, lx = light;
+        if (wave < 2) {
+            for (int o = 1; o < 64; o <<= 1) {
+                const u32 hy = __shfl_up(hx, o), ly = __shfl_up(lx, o);
+                if (lane >= o) {
+                    hx += hy;
+                    lx += ly;
+                }
+            }
+            if (63 == lane) {
+                s_tot[wave][0] = hx;
+                s_tot[wave][1] = lx;
+            }
+        }
+        __syncthreads();
+        if (wave < 2 && n_parts) {
+            const u32 h_all = s_tot[0][0] + s_tot[1][0];
+            const u32 hb = hx - heavy + (1 == wave ? s_tot[0][0] : 0u);
+            const u32 lb = lx - light + (1 == wave ? s_tot[0][1] : 0u);
+            if (parts_log) {
+                for (u32 k = 0; k < n_parts; k += 1) s_jobs[hb + k] = (u16)((u32)tid | (k << 8) | (parts_log << 12));
+            }
+            else {
+                s_jobs[h_all + lb] = (u16)tid;
+            }
+        }
+        if (0 == tid) s_njobs = s_tot[0][0] + s_tot[1][0] + s_tot[0][1] + s_tot[1][1];
+        __syncthreads();
+        /* a wave per job, 8 lanes per group, lane = x inside the 8-px
            chunk.  The payload's triangles that reach the tile share the
            wave's 8 groups: with n of them each gets 8/n groups (rounded
            down to a power of two) that split its rows, so one large
-           triangle is swept by the whole wave instead of by 8 lanes. */
+           triangle is swept by the whole wave instead of by 8 lanes; the
+           parts of a split payload split those rows again. */
         const int g = lane >> 3, j = lane & 7;
-        for (u32 e = e0 + (u32)wave; e < e1; e += (u32)kRasterWaves) {
-            const RasterTri* pt = &s_tri[(e - e0) * 8u];
+        const u32 n_jobs = s_njobs;
+        for (;;) {
+            u32 jn = 0u;
+            if (0 == lane) jn = atomicAdd(&s_next, 1u);
+            jn = __shfl(jn, 0);
+            if (jn >= n_jobs) break;
+            const u32 job = s_jobs[jn];
+            const u32 part = (job >> 8) & 7u, plog = (job >> 12) & 3u;
+            const RasterTri* pt = &s_tri[(job & 0xFFu) * 8u];
             bool act = false;
             {
                 const RasterTri& tr = pt[g];
@@ -1685,7 +1808,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             for (int k = 0; k < 8; k += 1) amask |= (u32)((ball >> (k * 8)) & 1ull) << k;
             const int n = __popc(amask);
             if (0 == n) continue;
-            const int m = (n <= 1) ? 8 : (n <= 2) ? 4 : (n <= 4) ? 2 : 1;
+            const int m = groups_per_tri(n);
             const int t = g / m, r = g & (m - 1);
             if (t >= n) continue;
             /* the t-th active triangle */
@@ -1696,12 +1819,15 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             const f32 sx0 = tr.v[0], sy0 = tr.v[1], iw0 = tr.v[2];
             const f32 sx1 = tr.v[3], sy1 = tr.v[4], iw1 = tr.v[5];
             const f32 sx2 = tr.v[6], sy2 = tr.v[7], iw2 = tr.v[8];
-            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, m, r);
-            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, m, r);
+            /* rows of the triangle: group r of part `part` takes phase
+               r + m * part of stride m * parts */
+            const int stride = m << plog, phase = r + m * (int)part;
+            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
+            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
         }
         __syncthreads();
 
-        if (1u == P.tile_nch[tile]) {
+        if (1u == tile_chunks) {
             publish_plane(P, V, tile, sample, tmx, tmy, [&](int i) { return s_vis[i]; }, s_red);
         }
         else {
@@ -1714,7 +1840,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             __syncthreads();
             if (0 == tid) {
                 const u32 done = atomicAdd(&P.tile_done[(u64)tile * maxS + (u32)sample], 1u);
-                s_last = (done + 1u == P.tile_nch[tile]) ? 1u : 0u;
+                s_last = (done + 1u == tile_chunks) ? 1u : 0u;
             }
             __syncthreads();
             if (s_last) {
