@@ -95,6 +95,17 @@ def algorithmic_bytes(st):
     return {"geometry": geometry, "binning": binning, "visibility": visibility, "shading": shading}
 
 
+def physical_cores():
+    """Physical cores of this host (lscpu: cores per socket x sockets), or
+    None."""
+    try:
+        txt = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=30).stdout
+        info = dict(line.split(":", 1) for line in txt.splitlines() if ":" in line)
+        return int(info["Core(s) per socket"].strip()) * int(info["Socket(s)"].strip())
+    except (OSError, subprocess.SubprocessError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(scene, name, width, height, frames):
     exe = os.path.join(ROOT, "oracle", "_ref", "sofren2_ref")
     if not os.path.exists(exe):
@@ -108,6 +119,7 @@ def cpu_baseline(scene, name, width, height, frames):
         print(f"cpu baseline failed: {e}", file=sys.stderr)
         return None
     return {"value": round(r["fps"], 3), "unit": "frames/s", "cores": r["threads"], "kind": "reference",
+            "logical_cpus": cores, "physical_cores": physical_cores(),
             "sample": f"{name} {width}x{height}, {frames} frames of the camera path after 5 warm-up frames, "
                       f"reference AVX2 build with {r['threads']} worker threads (logical CPUs); "
                       f"median {r['ms_median']:.2f} ms, p99 {r['ms_p99']:.2f} ms"}
@@ -117,23 +129,32 @@ def mixed_batch_leg(lib, steps):
     """64 views over 4 scene instances at mixed resolution and MSAA: one
     batched submission per step, the next batch recorded while the previous
     one runs (three in flight), against the same views presented one by one
-    through sft_present.  Both with the frames left in HBM (device output:
-    presents also keep four frames in flight) and with every frame copied to
-    its host buffer (the drop-in output)."""
+    through sft_present.  With the frames left in HBM (device output:
+    presents also keep four frames in flight), with every frame copied to its
+    host buffer (the drop-in output), and with every view's ARGB frame and
+    64-bit visibility delivered to host memory (the full output of the
+    workload; per frame, the visibility capture of each present)."""
     from sofren2_amd.workloads import MixedBatch
-    mb = MixedBatch(lib)
+    out = _mixed_modes(lib, steps, MixedBatch(lib), ((1, "device"), (0, "host")), tune=True)
+    out["host_vis64"] = _mixed_modes(lib, steps, MixedBatch(lib, vis64=True), ((0, "host_vis64"),))["host_vis64"]
+    return out
+
+
+def _mixed_modes(lib, steps, mb, modes, tune=False):
     out = {"views": len(mb.views), "megapixels_per_batch": round(mb.pixels() / 1e6, 3)}
     try:
-        # the autotuner picks the visibility work split per resolution class
-        # on this view set before anything is timed; both legs use its choice
         lib.set_output(0)
-        before = mb.S.Tuning()
-        lib.get_tuning(before)
-        out["tuning_default"] = before.as_dict()
-        t0 = time.perf_counter()
-        out["tuning"] = mb.tune().as_dict()
-        out["tuning"]["tune_s"] = round(time.perf_counter() - t0, 2)
-        for mode, tag in ((1, "device"), (0, "host")):
+        if tune:
+            # the autotuner picks the visibility work split per resolution
+            # class on this view set before anything is timed; every leg uses
+            # its choice
+            before = mb.S.Tuning()
+            lib.get_tuning(before)
+            out["tuning_default"] = before.as_dict()
+            t0 = time.perf_counter()
+            out["tuning"] = mb.tune().as_dict()
+            out["tuning"]["tune_s"] = round(time.perf_counter() - t0, 2)
+        for mode, tag in modes:
             lib.set_output(mode)
             mb.render_batched()
             mb.render_per_frame()
@@ -161,6 +182,91 @@ def mixed_batch_leg(lib, steps):
         return out
     finally:
         mb.close()
+
+
+def crowd_leg(lib, steps, warmup):
+    """The skinned crowd with capsule shadows at 1920x1080 (the box room, 32
+    humanoids of 6144 triangles, 16 shadow-casting lights): frames/s with
+    the frame kept in HBM as in the headline leg, then per-frame counters
+    from eager launches: BVH proxy-occlusion rays (Mrays/s over the
+    occlusion pass and at the frame rate) and the capsule ray tests of the
+    reference's shading (per shaded pixel and light reaching it, every
+    capsule cached for the light; the kernel itself first drops the capsules
+    no ray of the wave passes near, so it evaluates fewer)."""
+    import sofren2_amd as S
+    scene, w, h = SCENES["crowd_shadows"], 1920, 1080
+    lib.sft_resize(w, h)
+    if lib.scene_create(scene, 0, w, h) != 0:
+        raise SystemExit("crowd scene creation failed")
+    tris = lib.scene_tris()
+    color = np.zeros((h, w), dtype=np.uint32)
+    lib.set_output(1)
+    lib.set_graphs(1)
+    for i in range(warmup):
+        lib.scene_render(i % 120, color.ctypes.data, w * 4)
+    lib.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        lib.scene_render((warmup + i) % 120, color.ctypes.data, w * 4)
+    lib.synchronize()
+    fps = steps / (time.perf_counter() - t0)
+    lib.set_graphs(0)
+    st = S.FrameStats()
+    rays, caps, occ_ms, shade_ms, n = 0, 0, [], [], min(steps, 40)
+    for i in range(n):
+        lib.scene_render((warmup + i) % 120, color.ctypes.data, w * 4)
+        lib.get_stats(st)
+        rays += st.occlusion_rays
+        caps += st.capsule_tests
+        occ_ms.append(st.ms_shadows)
+        shade_ms.append(st.ms_shade)
+    lib.set_graphs(1)
+    lib.scene_destroy()
+    rays_f, caps_f = rays / n, caps / n
+    return {"workload": f"crowd_shadows_{w}x{h}", "triangles": int(tris), "frames_s": round(fps, 2),
+            "mpix_s": round(fps * w * h / 1e6, 1), "mtri_s": round(fps * tris / 1e6, 1),
+            "occlusion_rays_per_frame": round(rays_f, 1),
+            "occlusion_mrays_s": round(rays_f * fps / 1e6, 2),
+            "occlusion_pass_ms": round(float(np.mean(occ_ms)), 4),
+            "occlusion_mrays_s_in_pass": round(rays_f / (float(np.mean(occ_ms)) * 1e3), 2),
+            "capsule_tests_per_frame": round(caps_f, 1),
+            "capsule_mtests_s": round(caps_f * fps / 1e6, 1),
+            "shade_pass_ms": round(float(np.mean(shade_ms)), 4),
+            "capsule_mtests_s_in_shade_pass": round(caps_f / (float(np.mean(shade_ms)) * 1e3), 1)}
+
+
+def served_accuracy(sample, specs, max_checks=48, workers=8):
+    """Served frames of the 1% sample against the reference renderer
+    (oracle/_ref, one worker thread, run on this host) rendering the same
+    template pose and request camera: visibility IDs and depth bit-exact,
+    colour within 1 LSB on at least 99.9% of pixels.  Part of the
+    CPU-baseline leg: the reference is the checker here."""
+    from concurrent.futures import ThreadPoolExecutor
+    from sofren2_amd import workloads
+    sys.path.insert(0, ROOT)
+    from tests import parity
+    from tests.refrun import have_ref, ref_dump
+    picks = sample[:max_checks]
+    if not picks or not all(have_ref(ns) for ns in (1, 2, 4)):
+        return None
+
+    def check(item):
+        (_, si, cf, w, h, ns), (color, vis) = item
+        kind, seed, pose = specs[si]
+        ref = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns)
+        got = {"color": color, "vid": (vis & 0xFFFFFFFF).astype(np.uint32),
+               "depth": (vis >> np.uint64(32)).astype(np.uint32).view(np.float32)}
+        rep = parity.compare_frame(got, ref)
+        ids_equal = rep["vid_mismatch"] == 0 and rep["depth_mismatch"] == 0
+        return ids_equal, rep["color_within1_frac"], rep["color_exact_frac"], w * h
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        res = list(ex.map(check, picks))
+    return {"checked": len(res), "visibility_bit_exact": sum(r[0] for r in res),
+            "color_within_1lsb_999": sum(r[1] >= 0.999 for r in res),
+            "min_within_1lsb_frac": round(min(r[1] for r in res), 6),
+            "mean_exact_frac": round(float(np.mean([r[2] for r in res])), 6),
+            "largest_request_px": max(r[3] for r in res)}
 
 
 def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
@@ -225,6 +331,8 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
             checked += 1
             same += int(np.array_equal(c, color) and np.array_equal(v, vis))
         out["accuracy_sample"] = {"checked": checked, "equal_to_per_frame": same}
+        out["_sample"] = sample
+        out["_specs"] = specs
     finally:
         srv.close()
     return out
@@ -240,6 +348,8 @@ def main():
     ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
                     help="workload; box_room is the headline 1080p textured interior")
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-view mixed batch (0: skip)")
+    ap.add_argument("--crowd-steps", type=int, default=120,
+                    help="frames of the crowd-with-shadows 1080p leg (0: skip)")
     ap.add_argument("--served-requests", type=int, default=200,
                     help="requests per offered rate, at least one second of arrivals (0: skip serving)")
     ap.add_argument("--served-rates", default="100,500,2000,5000", help="offered request rates (req/s)")
@@ -364,6 +474,7 @@ def main():
         "config": {"workload": f"{args.scene}_{w}x{h}", "width": w, "height": h, "triangles": int(tris),
                    "frames_in_path": 120, "msaa": 1, "output": "device"},
         "mpix_s": round(fps * w * h / 1e6, 2),
+        "mtri_s": round(fps * tris / 1e6, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "pcie_inclusive_frames_s": round(host_fps, 3),
         "pass_ms": {"geometry": round(float(np.mean(geom_ms)), 4), "binning": round(float(np.mean(bin_ms)), 4),
@@ -381,14 +492,20 @@ def main():
         "cpu_baseline": None,
     }
     lib.scene_destroy()
+    if args.crowd_steps > 0 and args.scene != "crowd_shadows":
+        out["crowd"] = crowd_leg(lib, args.crowd_steps, args.warmup)
     if args.batch_steps > 0:
         out["mixed_batch"] = mixed_batch_leg(lib, args.batch_steps)
     if args.served_requests > 0:
         rates = [float(x) for x in args.served_rates.split(",") if x]
         out["served"] = served_leg(lib, rates, args.served_requests, args.served_scenes)
     lib.sft_release()
+    served = out.get("served") or {}
+    sample, specs = served.pop("_sample", []), served.pop("_specs", [])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.scene, w, h, args.cpu_frames)
+        if sample:
+            served["accuracy_vs_reference"] = served_accuracy(sample, specs)
     print(json.dumps(out), flush=True)
 
 

@@ -96,6 +96,7 @@ typedef struct sftx_frame_stats {
     u32 light_cut_tiles;      /* tiles whose light list stopped at 16 with more lights in reach */
     u32 bg_blend_pixels;      /* MSAA pixels without clearing that blend the caller's colour */
     u32 instances_dropped;    /* draws beyond the 64 instances of the visibility-ID split */
+    u32 capsule_tests;        /* capsule ray tests: per shaded pixel and light reaching it, the light's cached capsules */
 } sftx_frame_stats;
 
 void sftx_get_stats(sftx_frame_stats* out);

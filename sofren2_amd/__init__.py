@@ -42,7 +42,8 @@ class FrameStats(ctypes.Structure):
         "occlusion_invalid", "occlusion_overflow", "shadow_proxies")] + [(n, ctypes.c_float) for n in (
         "ms_upload", "ms_geometry", "ms_binning", "ms_raster_shade", "ms_download", "ms_raster", "ms_shade",
         "ms_total", "ms_shadows")] + [(n, ctypes.c_uint32) for n in (
-        "tex_npot", "unlit_shadings", "dirlit_shadings", "light_cut_tiles", "bg_blend_pixels", "instances_dropped")]
+        "tex_npot", "unlit_shadings", "dirlit_shadings", "light_cut_tiles", "bg_blend_pixels", "instances_dropped",
+        "capsule_tests")]
 
 
 class Tuning(ctypes.Structure):

@@ -1428,6 +1428,7 @@ static void write_stats(Context& c, const std::vector<ViewRecord>& views, const 
     st.dirlit_shadings = counters[CNT_DIR_LIT];
     st.light_cut_tiles = counters[CNT_LIGHT_CUT];
     st.bg_blend_pixels = counters[CNT_BG_BLEND];
+    st.capsule_tests = counters[CNT_CAP_TESTS];
     for (const ViewRecord& v : views) st.instances_dropped += v.dropped;
     st.ms_upload = ev_ms(fs.ev[0], fs.ev[1]);
     st.ms_shadows = fs.side_used ? ev_ms(fs.side.start, fs.side.end) : ev_ms(fs.ev[1], fs.ev[7]);

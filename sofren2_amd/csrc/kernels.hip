@@ -1310,6 +1310,9 @@ DEV V3 shade_pixel(const FramePtrs& P, const ViewGpu& VW, const Est& est, const 
                is taken only when the whole wave is out of reach */
             if (0ull == __ballot(active)) continue;
             const u32 np = ts.np[i];
+            /* capsule ray tests of the reference: the light's cached capsules
+               against this pixel's segment to the light (counted from bit 8) */
+            if (kShadows && active) fetch += np << 8;
             const u32 near_caps = (kShadows && np > 0u) ? near_capsules(cm, i) : 0u;
             /* no capsule near: a lightmapped lane keeps lm_shadow exactly (its
                shadow intensity is 1); only a lane with a NaN attenuation would
@@ -1869,17 +1872,20 @@ __global__ __launch_bounds__(64) void k_tile_lights(FramePtrs P) {
     build_tile_lights(P, V, tile, zmin, zmax, lane);
 }
 
-/* the wave's shading counters from each lane's `fetch` bits of shade_pixel
-   (1 bilinear, 2 nearest, 4 lightmap, 8 non-power-of-two albedo, 16 unlit,
-   32 directional lights); `act` holds the lanes that shaded */
+/* the wave's shading counters from each lane's `fetch` word of shade_pixel
+   (bits: 1 bilinear, 2 nearest, 4 lightmap, 8 non-power-of-two albedo, 16
+   unlit, 32 directional lights; from bit 8 the capsule ray tests); `act`
+   holds the lanes that shaded */
 DEV void count_fetches(const FramePtrs& P, u32 fetch, u64 act, int lane) {
     static constexpr int kCnt[6] = { CNT_TEX_BILINEAR, CNT_TEX_NEAREST, CNT_LIGHTMAPPED, CNT_TEX_NPOT, CNT_UNLIT,
                                      CNT_DIR_LIT };
     u32 n[6];
     for (int k = 0; k < 6; k += 1) n[k] = (u32)__popcll(__ballot((fetch >> k) & 1u));
+    const u32 caps = (u32)wave_sum((i32)(fetch >> 8));
     if (lane == __ffsll((long long)act) - 1) {
         const u32 sl = stat_slot();
         for (int k = 0; k < 6; k += 1) stat_add(P.counters, kCnt[k], n[k], sl);
+        stat_add(P.counters, CNT_CAP_TESTS, caps, sl);
     }
 }
 
@@ -2067,7 +2073,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     i32 shaded = 0, groups = 0;
     /* six 5-bit counts of shade_pixel's fetch bits (a lane shades at most
        16 items: 4 waves x 256 items over 64 lanes) */
-    u32 fetches = 0;
+    u32 fetches = 0, cap_tests = 0;
     u32* it_vid = s_item_vid[wave];
     u32* it_meta = s_item_meta[wave];
 
@@ -2182,6 +2188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 shaded += 1;
                 groups += (li == gs) ? 1 : 0;
                 for (int k = 0; k < 6; k += 1) fetches += ((fetch >> k) & 1u) << (5 * k);
+                if constexpr (kShadows) cap_tests += fetch >> 8;
             }
         }
         c0 = end;
@@ -2205,6 +2212,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const i32 blended = wave_sum((write && !fc.should_clear && ew > 0.f) ? 1 : 0);
     u32 fsum[6];
     for (int k = 0; k < 6; k += 1) fsum[k] = (u32)wave_sum((i32)((fetches >> (5 * k)) & 31u));
+    const u32 caps = kShadows ? (u32)wave_sum((i32)cap_tests) : 0u;
     if (lane == __ffsll((long long)act) - 1) {
         const u32 sl = stat_slot();
         stat_add(P.counters, CNT_SHADED, (u32)shaded, sl);
@@ -2213,6 +2221,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         static constexpr int kCnt[6] = { CNT_TEX_BILINEAR, CNT_TEX_NEAREST, CNT_LIGHTMAPPED, CNT_TEX_NPOT, CNT_UNLIT,
                                          CNT_DIR_LIT };
         for (int k = 0; k < 6; k += 1) stat_add(P.counters, kCnt[k], fsum[k], sl);
+        stat_add(P.counters, CNT_CAP_TESTS, caps, sl);
     }
 }
 

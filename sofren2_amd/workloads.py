@@ -33,10 +33,11 @@ class MixedBatch:
     """The mixed-resolution view set over resident scene instances, rendered
     either as one batch per step or view by view through sft_present."""
 
-    def __init__(self, lib, views=None):
+    def __init__(self, lib, views=None, vis64=False):
         import sofren2_amd as S
         self.S = S
         self.lib = lib
+        self.vis64 = vis64
         self.spec = views if views is not None else mixed_views()
         self.handles = []
         for kind, seed in MIXED_SCENES:
@@ -44,7 +45,7 @@ class MixedBatch:
             if h < 0:
                 raise RuntimeError("scene creation failed")
             self.handles.append(h)
-        self.views = [S.View(self.handles[si], fr, w, h, ns, vis64=False) for si, fr, w, h, ns in self.spec]
+        self.views = [S.View(self.handles[si], fr, w, h, ns, vis64=vis64) for si, fr, w, h, ns in self.spec]
 
     def pixels(self):
         return sum(v.width * v.height for v in self.views)
@@ -64,8 +65,8 @@ class MixedBatch:
         import time
         S = self.S
         if not hasattr(self, "views_b"):
-            self.views_b = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=False) for v in self.views]
-            self.views_c = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=False) for v in self.views]
+            self.views_b = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=self.vis64) for v in self.views]
+            self.views_c = [S.View(v.scene, v.frame, v.width, v.height, v.samples, vis64=self.vis64) for v in self.views]
         sets = [self.views, self.views_b, self.views_c]
         pending = []
         lat = []
@@ -83,7 +84,12 @@ class MixedBatch:
         return lat
 
     def render_per_frame(self):
+        """Every view through sft_present; with vis64 views each frame also
+        delivers its visibility IDs and depth (the capture of the last
+        present: the same 64 bits per sample)."""
         lib = self.lib
+        if self.vis64:
+            lib.set_capture(self.S.CAPTURE_VISIBILITY)
         for v in self.views:
             lib.sft_resize(v.width, v.height)
             lib.set_msaa(v.samples)
@@ -91,7 +97,14 @@ class MixedBatch:
             lib.set_color_buffer(v.color.ctypes.data, v.width * 4)
             lib.scene_draw(v.scene, v.frame, v.width, v.height)
             lib.sft_present()
+            if self.vis64:
+                n = v.vis.size
+                if not hasattr(v, "_vid"):
+                    v._vid = np.zeros(n, np.uint32)
+                    v._depth = np.zeros(n, np.float32)
+                lib.cap_visibility(v._vid.ctypes.data, v._depth.ctypes.data)
             lib.sft_frame_end()
+        lib.set_capture(0)
         lib.set_msaa(1)
 
     def close(self):
