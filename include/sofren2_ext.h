@@ -277,6 +277,29 @@ u64 sftx_noalloc_scratch_bytes(void);
 void sftx_set_initial_caps(u32 clip_cap, u32 tile_list_cap);
 u64 sftx_overflow_reruns(void);
 
+/* Resident models and edits.  A model drawn by sft_draw_instance (not a
+   dmesh) is copied to HBM at its first draw and drawn from there; the
+   reference instead reads the model's arrays at every present
+   (sofren2.c:8139-8289).  Every present compares each resident model it
+   draws with what was uploaded: the model header, every mesh header except
+   material and lightmap page (which are read at every present), and the
+   first eight vertex positions of each mesh's first and last meshlet.  A
+   difference uploads the model again, so replacing a model's arrays or
+   rewriting all of its vertices is seen.  Other in-place edits (some
+   vertices, bounds, attributes, indices, skin weights or BLAS nodes behind
+   unchanged pointers) must be declared: sftx_model_changed drops the
+   resident copy (after the batches in flight complete) and the next draw
+   uploads the model again.  The rows of a dropped copy stay allocated until
+   sft_release.  Returns 0, SFTX_MODEL_UNKNOWN for a model that is not
+   resident (nothing to do), or SFTX_MODEL_BUSY while the request server
+   runs (its templates hold copies of their models, which no edit reaches).
+   sftx_model_edits counts the resident copies dropped for edits, declared
+   or detected. */
+#define SFTX_MODEL_UNKNOWN (-1)
+#define SFTX_MODEL_BUSY    (-2)
+i32 sftx_model_changed(const sft_model* model);
+u64 sftx_model_edits(void);
+
 /* Copies `bytes` between two device buffers with a plain streaming kernel
    and returns its duration in ms; a calibration point for HBM counters. */
 f32 sftx_copy_calibration(u64 bytes);

@@ -156,6 +156,10 @@ class Library:
         self.overflow_reruns = _bind(g, "sftx_overflow_reruns", ctypes.c_uint64, [])
         self.set_clear = _bind(g, "sft_set_clear", None, [u8])
         self.raycast_check = _bind(s, "sftx_raycast_check", i32, [u32, i32, vp])
+        self.scene_edit_room = _bind(s, "sftx_scene_edit_room", i32, [i32, ctypes.c_float])
+        self.scene_room_model = _bind(s, "sftx_scene_room_model", vp, [])
+        self.model_changed = _bind(g, "sftx_model_changed", i32, [vp])
+        self.model_edits = _bind(g, "sftx_model_edits", ctypes.c_uint64, [])
 
 
 _LIB = None

@@ -120,6 +120,9 @@ struct ModelRes {
     u32 mesh_rec_base = 0;
     u32 max_bone = 0;               /* bone matrices a pose must provide */
     std::vector<u32> skin_of;       /* skin record per model meshlet, kInvalid if none */
+    u64 fp = 0;                     /* model_fingerprint when the rows were uploaded */
+    u64 checked = 0;                /* Backend::check_serial of the last comparison */
+    bool edited = false;            /* that comparison found another fingerprint */
 };
 
 /* Everything one batch in flight owns: its stream and events, the per-batch
@@ -191,6 +194,8 @@ struct Backend {
     u64 layout_epoch = 0;
     std::vector<MeshRec> mesh_recs;
     std::unordered_map<const sft_model*, ModelRes> models;
+    u64 check_serial = 0;           /* one per residency check: fingerprints once per model */
+    u64 model_edits = 0;              /* resident models uploaded again after an edit */
     DBuf texels, tex_meta_d, tex_stage;
     u64 texels_used = 0;
     std::vector<TexMeta> tex_meta;
@@ -667,13 +672,72 @@ static bool is_transient(const ViewRecord& v, const sft_model* m) {
     return std::find(v.transient_models.begin(), v.transient_models.end(), m) != v.transient_models.end();
 }
 
+/* What the draw-time check of a resident model compares: the model header,
+   every mesh header except its material and lightmap page (both read at
+   every present anyway), and the first eight vertex positions of each
+   mesh's first and last meshlet.  The reference reads a model's arrays at
+   every present (sofren2.c:8139-8289); rows kept in HBM follow that only as
+   far as this check sees edits, and other in-place edits are declared with
+   sftx_model_changed. */
+static u64 model_fingerprint(const sft_model* m) {
+    u64 h = 0x243F6A8885A308D3ull;
+    auto mix = [&h](u64 w) {
+        h ^= w * 0x9E3779B97F4A7C15ull;
+        h = ((h << 27) | (h >> 37)) * 0xC2B2AE3D27D4EB4Full;
+    };
+    auto ptr = [&mix](const void* p) { mix((u64)(uintptr_t)p); };
+    ptr(m->meshes);
+    mix(((u64)m->mesh_count << 32) | m->total_meshlets);
+    ptr(m->meshlet_mesh_inds);
+    ptr(m->blas_nodes);
+    mix(((u64)m->blas_node_count << 32) | m->blas_root_ind);
+    mix((u64)(u32)m->skeleton.bone_count);
+    for (u32 i = 0; i < m->mesh_count && m->meshes; i += 1) {
+        const sft_mesh& s = m->meshes[i];
+        for (const void* p : { (const void*)s.meshlets, (const void*)s.meshlet_attribs, (const void*)s.meshlet_skins,
+                               (const void*)s.inds, (const void*)s.bounds_center_x, (const void*)s.bounds_center_y,
+                               (const void*)s.bounds_center_z, (const void*)s.bounds_radius, (const void*)s.bounds_min_x,
+                               (const void*)s.bounds_max_x, (const void*)s.bounds_min_y, (const void*)s.bounds_max_y,
+                               (const void*)s.bounds_min_z, (const void*)s.bounds_max_z, (const void*)s.cone_axis_x,
+                               (const void*)s.cone_axis_y, (const void*)s.cone_axis_z, (const void*)s.cone_cutoff,
+                               (const void*)s.culling_bone_inds }) {
+            ptr(p);
+        }
+        mix(((u64)s.meshlet_count << 32) | s.model_meshlet_offset);
+        if (!s.meshlets || 0u == s.meshlet_count) continue;
+        for (const sft_meshlet* ml : { &s.meshlets[0], &s.meshlets[s.meshlet_count - 1u] }) {
+            for (const f32* a : { ml->x, ml->y, ml->z }) {
+                u64 w[4];
+                memcpy(w, a, sizeof(w));
+                for (u64 x : w) mix(x);
+            }
+        }
+    }
+    return h ^ (h >> 29);
+}
+
+/* whether a model drawn as resident must be uploaded (again): not resident,
+   another meshlet count, or another fingerprint (computed once per model
+   and check_serial) */
+static bool needs_upload(Backend* b, const sft_model* m) {
+    auto it = b->models.find(m);
+    if (it == b->models.end() || it->second.total != m->total_meshlets) return true;
+    ModelRes& r = it->second;
+    if (r.checked != b->check_serial) {
+        r.checked = b->check_serial;
+        r.edited = r.fp != model_fingerprint(m);
+    }
+    return r.edited;
+}
+
 /* Whether preparing these views changes state the frame sets share, and so
    waits for every batch in flight and may allocate: estimate tables or
    textures to upload, models to make resident, or dmesh rows beyond the
    transient regions (their row and skin counts go to need_t, need_tskin). */
-static bool shared_state_changes(const Backend* b, const Context& c, const std::vector<ViewRecord>& views, u32& need_t,
+static bool shared_state_changes(Backend* b, const Context& c, const std::vector<ViewRecord>& views, u32& need_t,
                                  u32& need_tskin) {
     bool change = !b->est_uploaded || texture_changes(b, c) || b->texels.cap < 4;
+    b->check_serial += 1;
     need_t = 0;
     need_tskin = 0;
     for (const ViewRecord& v : views) {
@@ -687,10 +751,7 @@ static bool shared_state_changes(const Backend* b, const Context& c, const std::
     if (need_t > b->t_cap || need_tskin > b->t_skin_cap) change = true;
     for (const ViewRecord& v : views) {
         for (const sft_instance& in : v.instances) {
-            auto it = b->models.find(in.model);
-            if (!is_transient(v, in.model) && (it == b->models.end() || it->second.total != in.model->total_meshlets)) {
-                change = true;
-            }
+            if (!is_transient(v, in.model) && needs_upload(b, in.model)) change = true;
         }
     }
     return change;
@@ -716,12 +777,15 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     for (const ViewRecord& v : views) {
         for (const sft_instance& in : v.instances) {
             const sft_model* m = in.model;
-            if (is_transient(v, m)) continue;
-            auto it = b->models.find(m);
-            if (it != b->models.end() && it->second.total == m->total_meshlets) continue;
+            if (is_transient(v, m) || !needs_upload(b, m)) continue;
+            /* an edited model gets new rows; its old rows stay unused until
+               sft_release (batches in flight were waited for above) */
+            if (b->models.count(m)) b->model_edits += 1;
             ModelRes r;
             b->mesh_recs.resize(b->resident_mesh_recs);
             upload_model(b, m, r);
+            r.fp = model_fingerprint(m);
+            r.checked = b->check_serial;
             b->models[m] = std::move(r);
         }
     }
@@ -1775,6 +1839,23 @@ extern "C" void sftx_set_initial_caps(u32 clip_cap, u32 tile_list_cap) {
 }
 
 extern "C" u64 sftx_overflow_reruns(void) { return ensure_backend()->reruns; }
+
+extern "C" i32 sftx_model_changed(const sft_model* model) {
+    if (!g_ctx || !model) return SFTX_MODEL_UNKNOWN;
+    Context& c = *g_ctx;
+    if (c.server_running) {
+        fprintf(stderr, "sofren2: sftx_model_changed while the request server runs\n");
+        return SFTX_MODEL_BUSY;
+    }
+    if (!c.backend || !c.backend->models.count(model)) return SFTX_MODEL_UNKNOWN;
+    drain_async_batches(c);
+    quiesce(c.backend);
+    backend_forget_model(c.backend, model);
+    c.backend->model_edits += 1;
+    return 0;
+}
+
+extern "C" u64 sftx_model_edits(void) { return ensure_backend()->model_edits; }
 
 extern "C" i32 sftx_present_noalloc(void) {
     if (!g_ctx) {

@@ -54,6 +54,7 @@
  * earliest-drawn fragment wins at equal depth as with its strict depth test.
  */
 #include "kernels.h"
+#include "capsule_cull.h"
 
 namespace sft {
 
@@ -468,6 +469,81 @@ DEV ClipVert load_cv(const FrameConst& fc, const f32* ss, const f32* ap, int i) 
     return c;
 }
 
+DEV ClipVert pick_cv(const ClipVert& a, const ClipVert& b, const ClipVert& c, int s) {
+    const bool s0 = 0 == s, s1 = 1 == s;
+    ClipVert r;
+    r.tx = s0 ? a.tx : (s1 ? b.tx : c.tx);
+    r.ty = s0 ? a.ty : (s1 ? b.ty : c.ty);
+    r.tz = s0 ? a.tz : (s1 ? b.tz : c.tz);
+    r.w = s0 ? a.w : (s1 ? b.w : c.w);
+    r.u = s0 ? a.u : (s1 ? b.u : c.u);
+    r.v = s0 ? a.v : (s1 ? b.v : c.v);
+    r.lu = s0 ? a.lu : (s1 ? b.lu : c.lu);
+    r.lv = s0 ? a.lv : (s1 ? b.lv : c.lv);
+    r.px = s0 ? a.px : (s1 ? b.px : c.px);
+    r.py = s0 ? a.py : (s1 ? b.py : c.py);
+    r.pz = s0 ? a.pz : (s1 ? b.pz : c.pz);
+    r.nx = s0 ? a.nx : (s1 ? b.nx : c.nx);
+    r.ny = s0 ? a.ny : (s1 ? b.ny : c.ny);
+    r.nz = s0 ? a.nz : (s1 ? b.nz : c.nz);
+    return r;
+}
+
+/* one clipped triangle: its record (screen x, y, 1/w and the interpolated
+   attributes per vertex, then instance and mesh), its sort key (work item,
+   batch, lane, output c) and its tile rectangle; returns its tile pairs */
+DEV i32 emit_clip(const FramePtrs& P, const FrameConst& fc, const WorkItem& w, u32 wi, int b, int l, int c, u32 slot,
+                  u32 clip_cap, i32 max_tx, i32 max_ty, const ClipVert& q0, const ClipVert& q1, const ClipVert& q2) {
+    if (slot >= clip_cap) {
+        atomicOr(&P.counters[CNT_OVERFLOW], 1u);
+        return 0;
+    }
+    f32* rec = P.clip_rec + (u64)slot * 41u;
+    f32 sxv[3], syv[3];
+    auto put = [&](int k, const ClipVert& q) {
+        const f32 iw = 1.f / q.w;
+        sxv[k] = (q.tx * iw + 1.f) * fc.hw + fc.vp_x;
+        syv[k] = (-q.ty * iw + 1.f) * fc.hh + fc.vp_y;
+        rec[0 + k] = sxv[k];
+        rec[3 + k] = syv[k];
+        rec[6 + k] = iw;
+        rec[9 + k] = q.u;
+        rec[12 + k] = q.v;
+        rec[15 + k] = q.lu;
+        rec[18 + k] = q.lv;
+        rec[21 + k] = q.px;
+        rec[24 + k] = q.py;
+        rec[27 + k] = q.pz;
+        rec[30 + k] = q.nx;
+        rec[33 + k] = q.ny;
+        rec[36 + k] = q.nz;
+    };
+    put(0, q0);
+    put(1, q1);
+    put(2, q2);
+    ((u32*)rec)[39] = w.inst;
+    ((u32*)rec)[40] = w.gm;
+    P.clip_key[slot] = (wi << 9) | ((u32)b << 5) | (u32)(l * 2 + c);
+
+    u32 rect = kInvalid;
+    i32 pairs = 0;
+    const f32 area = (sxv[1] - sxv[0]) * (syv[2] - syv[0]) - (syv[1] - syv[0]) * (sxv[2] - sxv[0]);
+    if (!(area <= 0.1f)) {
+        const i32 x0 = cvtt(floorf(smin(sxv[0], smin(sxv[1], sxv[2])))) >> 6;
+        const i32 x1 = cvtt(ceilf(smax(sxv[0], smax(sxv[1], sxv[2])))) >> 6;
+        const i32 y0 = cvtt(floorf(smin(syv[0], smin(syv[1], syv[2])))) >> 6;
+        const i32 y1 = cvtt(ceilf(smax(syv[0], smax(syv[1], syv[2])))) >> 6;
+        if (!(x1 < 0 || x0 > max_tx || y1 < 0 || y0 > max_ty)) {
+            const i32 cx0 = max(x0, 0), cx1 = min(x1, max_tx);
+            const i32 cy0 = max(y0, 0), cy1 = min(y1, max_ty);
+            rect = (u32)cx0 | ((u32)cx1 << 8) | ((u32)cy0 << 16) | ((u32)cy1 << 24);
+            pairs = (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
+        }
+    }
+    P.clip_rect[slot] = rect;
+    return pairs;
+}
+
 __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const u32 wi = blockIdx.x * 4u + (u32)wave;
@@ -507,76 +583,29 @@ __global__ __launch_bounds__(256) void k_clip(FramePtrs P) {
         base += (u32)__shfl(incl, 63);
         if (0 == nout) continue;
 
-        ClipVert out[2][3];
+        /* the triangle rotated so that its lone vertex (the one in front for
+           one output, the one behind for two) comes first: R1 follows it and
+           R2 precedes it in the triangle's order.  Selects, not indexed
+           arrays, keep every vertex in registers. */
+        const int s = (1 == nout) ? (int)(cm >> 1) : (int)((~cm & 7u) >> 1);
+        const ClipVert R0 = pick_cv(in[0], in[1], in[2], s);
+        const ClipVert R1 = pick_cv(in[0], in[1], in[2], (2 == s) ? 0 : s + 1);
+        const ClipVert R2 = pick_cv(in[0], in[1], in[2], (0 == s) ? 2 : s - 1);
         if (1 == nout) {
-            const int a = (int)(cm >> 1);
-            const int bi = (2 == a) ? 0 : a + 1;
-            const int ci = (0 == a) ? 2 : a - 1;
-            const f32 da = in[a].tz;
-            const f32 t_ab = da / (da - in[bi].tz);
-            const f32 t_ac = da / (da - in[ci].tz);
-            out[0][0] = in[a];
-            out[0][1] = lerp_cv(in[a], in[bi], t_ab);
-            out[0][2] = lerp_cv(in[a], in[ci], t_ac);
+            const f32 da = R0.tz;
+            const f32 t_ab = da / (da - R1.tz);
+            const f32 t_ac = da / (da - R2.tz);
+            pairs += emit_clip(P, fc, w, wi, b, l, 0, first, clip_cap, max_tx, max_ty, R0, lerp_cv(R0, R1, t_ab),
+                               lerp_cv(R0, R2, t_ac));
         }
         else {
-            const int o = (int)((~cm & 7u) >> 1);
-            const int a = (0 == o) ? 2 : o - 1;
-            const int ci = (2 == o) ? 0 : o + 1;
-            const f32 da = in[a].tz, db = in[o].tz, dc = in[ci].tz;
+            const f32 da = R2.tz, db = R0.tz, dc = R1.tz;
             const f32 t_ab = da / (da - db);
             const f32 t_cb = dc / (dc - db);
-            const ClipVert nab = lerp_cv(in[a], in[o], t_ab);
-            const ClipVert ncb = lerp_cv(in[ci], in[o], t_cb);
-            out[0][0] = in[a]; out[0][1] = nab; out[0][2] = in[ci];
-            out[1][0] = nab;   out[1][1] = ncb; out[1][2] = in[ci];
-        }
-        for (int c = 0; c < nout; c += 1) {
-            const u32 slot = first + (u32)c;
-            if (slot >= clip_cap) {
-                atomicOr(&P.counters[CNT_OVERFLOW], 1u);
-                continue;
-            }
-            f32* rec = P.clip_rec + (u64)slot * 41u;
-            f32 sxv[3], syv[3];
-            for (int k = 0; k < 3; k += 1) {
-                const ClipVert& q = out[c][k];
-                const f32 iw = 1.f / q.w;
-                sxv[k] = (q.tx * iw + 1.f) * fc.hw + fc.vp_x;
-                syv[k] = (-q.ty * iw + 1.f) * fc.hh + fc.vp_y;
-                rec[0 + k] = sxv[k];
-                rec[3 + k] = syv[k];
-                rec[6 + k] = iw;
-                rec[9 + k] = q.u;
-                rec[12 + k] = q.v;
-                rec[15 + k] = q.lu;
-                rec[18 + k] = q.lv;
-                rec[21 + k] = q.px;
-                rec[24 + k] = q.py;
-                rec[27 + k] = q.pz;
-                rec[30 + k] = q.nx;
-                rec[33 + k] = q.ny;
-                rec[36 + k] = q.nz;
-            }
-            ((u32*)rec)[39] = w.inst;
-            ((u32*)rec)[40] = w.gm;
-            P.clip_key[slot] = (wi << 9) | ((u32)b << 5) | (u32)(l * 2 + c);
-
-            u32 rect = kInvalid;
-            const f32 area = (sxv[1] - sxv[0]) * (syv[2] - syv[0]) - (syv[1] - syv[0]) * (sxv[2] - sxv[0]);
-            if (!(area <= 0.1f)) {
-                const i32 x0 = cvtt(floorf(smin(sxv[0], smin(sxv[1], sxv[2])))) >> 6;
-                const i32 x1 = cvtt(ceilf(smax(sxv[0], smax(sxv[1], sxv[2])))) >> 6;
-                const i32 y0 = cvtt(floorf(smin(syv[0], smin(syv[1], syv[2])))) >> 6;
-                const i32 y1 = cvtt(ceilf(smax(syv[0], smax(syv[1], syv[2])))) >> 6;
-                if (!(x1 < 0 || x0 > max_tx || y1 < 0 || y0 > max_ty)) {
-                    const i32 cx0 = max(x0, 0), cx1 = min(x1, max_tx);
-                    const i32 cy0 = max(y0, 0), cy1 = min(y1, max_ty);
-                    rect = (u32)cx0 | ((u32)cx1 << 8) | ((u32)cy0 << 16) | ((u32)cy1 << 24);
-                    pairs += (cx1 - cx0 + 1) * (cy1 - cy0 + 1);
-                }
-            }
-            P.clip_rect[slot] = rect;
+            const ClipVert nab = lerp_cv(R2, R0, t_ab);
+            const ClipVert ncb = lerp_cv(R1, R0, t_cb);
+            pairs += emit_clip(P, fc, w, wi, b, l, 0, first, clip_cap, max_tx, max_ty, R2, nab, R1);
+            pairs += emit_clip(P, fc, w, wi, b, l, 1, first + 1u, clip_cap, max_tx, max_ty, nab, ncb, R1);
         }
     }
     pairs = wave_sum(pairs);
@@ -1113,21 +1142,6 @@ DEV void stage_tile_shading(const FramePtrs& P, const ViewGpu& VW, u32 tile, Til
     __syncthreads();
 }
 
-/* capsule culling for a wave.  The shading positions of the wave's lanes span
-   a box of centre M and half-diagonal R, and every segment from such a
-   position to a light lies within R of the segment M-light.  A lane's ray is
-   active for a capsule only if the capsule centre projects inside the lane's
-   pixel-to-light segment at a distance below the bounding radius
-   (sofren2.c:6521-6530); a capsule whose centre keeps that distance from
-   segment M-light plus R plus a margin (4e-3 of the lengths involved and
-   1e-5 of the coordinates' size, far above the float rounding of the lane's
-   test) has no active ray on any lane, so the reference's per-group skip
-   would pass over it on every group of the wave. */
-struct WaveSpan {
-    f32 m[3];       /* box centre */
-    f32 r;          /* half-diagonal, widened for the rounding of the centre */
-    f32 mag;        /* |m| in the 1-norm */
-};
 
 /* per tile light i (16 bits each, 4 lights per word): capsules near the span */
 struct CapsuleMasks {
@@ -1149,35 +1163,11 @@ DEV WaveSpan wave_span(const V3& p, bool inside) {
         lx = fminf(lx, __shfl_xor(lx, o)); ly = fminf(ly, __shfl_xor(ly, o)); lz = fminf(lz, __shfl_xor(lz, o));
         hx = fmaxf(hx, __shfl_xor(hx, o)); hy = fmaxf(hy, __shfl_xor(hy, o)); hz = fmaxf(hz, __shfl_xor(hz, o));
     }
-    WaveSpan s;
-    s.m[0] = 0.5f * (lx + hx); s.m[1] = 0.5f * (ly + hy); s.m[2] = 0.5f * (lz + hz);
-    const f32 ex = hx - lx, ey = hy - ly, ez = hz - lz;
-    s.mag = fabsf(s.m[0]) + fabsf(s.m[1]) + fabsf(s.m[2]);
-    s.r = 0.5f * sqrtf(ex * ex + ey * ey + ez * ez) * 1.0001f + 1e-6f * s.mag;
-    return s;
+    return span_of_box(lx, ly, lz, hx, hy, hz);
 }
 
-DEV bool capsule_near(const PointLightGpu& L, const ProxyRec& c, const WaveSpan& s) {
-    const f32 dx = s.m[0] - L.pos[0], dy = s.m[1] - L.pos[1], dz = s.m[2] - L.pos[2];
-    const f32 dd = dx * dx + dy * dy + dz * dz;
-    const f32 ud = c.w[0] * dx + c.w[1] * dy + c.w[2] * dz;
-    const f32 t = (dd > 0.f) ? fminf(fmaxf(ud / dd, 0.f), 1.f) : 0.f;
-    const f32 ex = c.w[0] - t * dx, ey = c.w[1] - t * dy, ez = c.w[2] - t * dz;
-    const f32 dist = sqrtf(ex * ex + ey * ey + ez * ez);
-    const f32 absl = fabsf(L.pos[0]) + fabsf(L.pos[1]) + fabsf(L.pos[2]);
-    const f32 thr = sqrtf(c.rb2) + s.r + 4e-3f * (sqrtf(c.wdw) + sqrtf(dd) + s.r) + 1e-5f * (s.mag + absl) + 1e-4f;
-    /* a NaN anywhere keeps the capsule */
-    return !(dist >= thr);
-}
-
-/* the cheap half of the reference's capsule test (sofren2.c:6521-6530): the
-   capsule's bounding centre projects strictly inside the pixel-to-light
-   segment, closer to it than the bounding radius */
 DEV bool ray_near(const ProxyRec& c, const V3& lraw, f32 d2, f32 inv_d2) {
-    const f32 wdv = 0.f - dot3(V3{c.w[0], c.w[1], c.w[2]}, lraw);
-    const f32 tf = wdv * inv_d2;
-    const f32 cd2 = fmaf(-wdv, tf, c.wdw);
-    return (wdv > 0.f) & (wdv < d2) & (cd2 < c.rb2);
+    return ray_near_xyz(c, lraw.x, lraw.y, lraw.z, d2, inv_d2);
 }
 
 /* lane (light 4r + lane/16, capsule lane%16) tests one pair per round; every

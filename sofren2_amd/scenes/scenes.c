@@ -1379,6 +1379,26 @@ void sftx_scene_camera(i32 handle, i32 frame, i32 width, i32 height, sftvec* pos
     sftx_camera_for(sftx_slots[handle]->scene.kind, cam, width, height, pos, view, proj);
 }
 
+const sft_model* sftx_scene_room_model(void) { return sftx_cur ? g_room.model : NULL; }
+
+i32 sftx_scene_edit_room(i32 mode, f32 dy) {
+    if (!sftx_cur || !g_room.model || (mode != 1 && mode != 2)) return -1;
+    const sft_model* m = g_room.model;
+    i32 moved = 0;
+    for (u32 i = 0; i < m->mesh_count; i += 1) {
+        sft_mesh* s = &m->meshes[i];
+        for (u32 k = 0; k < s->meshlet_count; k += 1) {
+            if (2 == mode && (0u == k || k + 1u == s->meshlet_count)) continue;
+            for (i32 v = 0; v < SFT_VERTS_PER_MESHLET; v += 1) s->meshlets[k].y[v] += dy;
+            s->bounds_center_y[k] += dy;
+            s->bounds_min_y[k] += dy;
+            s->bounds_max_y[k] += dy;
+            moved += 1;
+        }
+    }
+    return moved;
+}
+
 void sftx_scene_free(i32 handle) {
     if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return;
     if (sftx_cur == sftx_slots[handle]) sftx_cur = NULL;

@@ -93,6 +93,14 @@ void sftx_scene_destroy(void);
 i32 sftx_pose_check(u32 seed, i32 n_times, const f32* times, f32* out);
 i32 sftx_raycast_check(u32 seed, i32 n_rays, f32* out);
 
+/* Edits the current scene's room model in place (box room, crowd and served
+   kinds), as an application editing a loaded model would: moves meshlets up
+   by dy (vertex positions and bounds).  Mode 1 moves every meshlet, mode 2
+   only those strictly between each mesh's first and last meshlet.  Returns
+   the meshlets moved, -1 without a room model. */
+i32 sftx_scene_edit_room(i32 mode, f32 dy);
+const sft_model* sftx_scene_room_model(void);
+
 #ifdef __cplusplus
 }
 #endif
