@@ -204,6 +204,19 @@ void sftx_server_release(void);                        /* stops and drops the te
    unchanged.  sftx_graph_captures counts the captures made so far. */
 void sftx_set_graphs(i32 enabled);
 
+/* Colour delivered to host memory (sft_present with host output, host-output
+   batches and served requests) streamed band by band (default on): the
+   shading kernels write each pixel into pinned staging as well as HBM and
+   raise a flag per tile row once its pixels are there; the host copies the
+   finished rows into the caller's buffer while later rows shade, instead of
+   downloading the frame after it.  An overflow re-run, or a band that is
+   never flagged, falls back to the download after the frame.  The results
+   are the same either way.  Off: the frame is downloaded after it.
+   sftx_stream_stats counts the batches delivered streamed and those that
+   fell back. */
+void sftx_set_stream_download(i32 enabled);
+void sftx_stream_stats(u64* streamed, u64* fallbacks);
+
 /* Autotuner.  No counterpart in the reference (its tiles and job sizes are
    compile-time constants, sofren2.c:124-130); it exists for batches and the
    request server, whose views span many resolutions.  sftx_batch_tune renders

@@ -115,6 +115,8 @@ class Library:
         self.set_output = _bind(g, "sftx_set_output", None, [i32])
         self.set_msaa = _bind(g, "sftx_set_msaa", None, [i32])
         self.set_graphs = _bind(g, "sftx_set_graphs", None, [i32])
+        self.set_stream_download = _bind(g, "sftx_set_stream_download", None, [i32])
+        self._stream_stats = _bind(g, "sftx_stream_stats", None, [vp, vp])
         self.graph_captures = _bind(g, "sftx_graph_captures", ctypes.c_uint64, [])
         self.synchronize = _bind(g, "sftx_synchronize", None, [])
         self.copy_calibration = _bind(g, "sftx_copy_calibration", ctypes.c_float, [ctypes.c_uint64])
@@ -160,6 +162,12 @@ class Library:
         self.scene_room_model = _bind(s, "sftx_scene_room_model", vp, [])
         self.model_changed = _bind(g, "sftx_model_changed", i32, [vp])
         self.model_edits = _bind(g, "sftx_model_edits", ctypes.c_uint64, [])
+
+    def stream_stats(self):
+        """(batches delivered with the streamed colour download, fallbacks)"""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self._stream_stats(ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
 
 _LIB = None

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <functional>
@@ -147,6 +148,11 @@ struct FrameSet {
     u32* counters_h = nullptr;      /* pinned copy of the narrow counters */
     u8* out_h = nullptr;            /* pinned staging of host outputs (colour, 64-bit visibility) */
     size_t out_h_cap = 0;
+    u32* band_flag_h = nullptr;     /* pinned per-band flags of a streamed colour download */
+    u32 band_flag_cap = 0;
+    u32 n_bands = 0;                /* bands (tile rows) of the views with host colour */
+    bool streamed = false;          /* the last launch streamed its colour to out_h */
+    bool host_copied = false;       /* the copier delivered every host output */
     std::thread copier;             /* an asynchronous batch's host outputs, copied once it is done */
     SideStream side{};              /* shadow-proxy occlusion beside the main passes (eager launches) */
     bool side_used = false;         /* the last launch ran occlusion on the side stream */
@@ -164,6 +170,7 @@ struct FrameSet {
         u64 vid_base = 0, color_base = 0, grid = 0;
         u64 vis_stage = 0;          /* its 64-bit visibility in the host staging (views that ask for it) */
         u32 n_tiles = 0;
+        u32 band_base = kInvalid;   /* its first band when its colour goes to host memory */
     };
     std::vector<ViewPlan> plan;
     u32 nt = 0, nw = 0, max_samples = 1, ss_total = 0, n_proxies = 0, n_lights = 0, t_slots = 0, t_skins = 0;
@@ -214,6 +221,8 @@ struct Backend {
     int ordered_set = -1;           /* frame set of the last ordered submission */
     u32 raster_groups = 512;        /* persistent visibility workgroups */
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
+    bool stream_download = true;    /* host colour streamed band by band while the frame shades */
+    u64 streamed_batches = 0, stream_fallbacks = 0;
     u64 graph_captures = 0;
     u64 reruns = 0;                 /* batches run again after an intermediate overflowed */
 
@@ -302,6 +311,7 @@ void backend_destroy(Backend* b) {
         fs.pack.release();
         if (fs.counters_h) HIP_CHECK(hipHostFree(fs.counters_h));
         if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
+        if (fs.band_flag_h) HIP_CHECK(hipHostFree(fs.band_flag_h));
         if (fs.graph_exec) HIP_CHECK(hipGraphExecDestroy(fs.graph_exec));
         for (hipEvent_t& e : fs.ev) HIP_CHECK(hipEventDestroy(e));
         HIP_CHECK(hipStreamDestroy(fs.stream));
@@ -862,6 +872,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     u32 ss_total = 0, max_samples = 1, occ_total = 0, min_chunk = kRasterMaxChunk;
     u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
     bool any_vis64 = false, host_color = false;
+    u32 n_bands = 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
         FrameSet::ViewPlan& vp = fs.plan[vi];
@@ -881,7 +892,12 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         color_total += (u64)v.width * v.height;
         max_samples = std::max<u32>(max_samples, (u32)v.samples);
         any_vis64 = any_vis64 || v.vis64;
-        host_color = host_color || ((SFTX_OUTPUT_HOST == b->output_mode || v.host_output) && v.color_buffer);
+        const bool view_host = (SFTX_OUTPUT_HOST == b->output_mode || v.host_output) && v.color_buffer;
+        host_color = host_color || view_host;
+        if (view_host && v.width > 0 && v.height > 0) {
+            vp.band_base = n_bands;
+            n_bands += (u32)v.tile_rows;
+        }
         for (u32 t = 0; t < vp.n_tiles; t += 1) tile_view.push_back(vi);
         const u32 ss_base = ss_total;
         ss_total += v.total_ss;
@@ -1020,7 +1036,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.vis64 = v.vis64 ? 1u : 0u;
         g.raster_chunk = std::max(1u, std::min(b->raster_chunk[res_class(v.width, v.height)], kRasterMaxChunk));
         min_chunk = std::min(min_chunk, g.raster_chunk);
-        g.pad2 = 0;
+        g.band_base = kInvalid;
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
             vis_stage_total += vp.grid;
@@ -1152,7 +1168,9 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.any_vis64 = any_vis64;
     fs.host_color = host_color;
     fs.cap_vis = 1 == nv && 0 != (views[0].capture_flags & SFTX_CAPTURE_VISIBILITY);
-    fs.zero_words = COUNTER_WORDS + (size_t)nt * max_samples;
+    fs.n_bands = n_bands;
+    /* counters, chunk completions per tile and sample, band completions */
+    fs.zero_words = COUNTER_WORDS + (size_t)nt * max_samples + n_bands;
     fs.layout_epoch = b->layout_epoch;
 }
 
@@ -1214,7 +1232,8 @@ static size_t acquire_buffers(FrameSet& fs, bool measure) {
    A set with carve memory runs its kernels eagerly and leaves its colour on
    the device; it returns false, having queued nothing, when its buffers do
    not fit the carve memory. */
-static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs, const FrameSet* after = nullptr) {
+static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& fs, const FrameSet* after = nullptr,
+                         bool allow_stream = true) {
     hipStream_t s = fs.stream;
     const u32 nv = (u32)views.size();
     if (fs.carve_base) {
@@ -1225,6 +1244,28 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
             return false;
         }
     }
+    /* host outputs go through pinned staging: colour (streamed into it by
+       the shading kernels band by band, or copied after the frame) and the
+       64-bit visibility planes.  An overflow re-run does not stream. */
+    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
+    const size_t stage = color_bytes + (size_t)fs.vis_stage_total * 8;
+    const bool stream = allow_stream && !fs.carve_base && color_bytes && fs.n_bands > 0 && b->stream_download;
+    if (stage && !fs.carve_base && fs.out_h_cap < stage) {
+        if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
+        HIP_CHECK(hipHostMalloc((void**)&fs.out_h, stage, hipHostMallocCoherent));
+        fs.out_h_cap = stage;
+    }
+    if (stream) {
+        if (fs.band_flag_cap < fs.n_bands) {
+            if (fs.band_flag_h) HIP_CHECK(hipHostFree(fs.band_flag_h));
+            fs.band_flag_cap = std::max(fs.n_bands, 256u);
+            HIP_CHECK(hipHostMalloc((void**)&fs.band_flag_h, (size_t)fs.band_flag_cap * 4, hipHostMallocCoherent));
+        }
+        /* the previous batch of this set is complete: nothing writes them now */
+        memset(fs.band_flag_h, 0, (size_t)fs.n_bands * 4);
+    }
+    fs.streamed = stream;
+    for (u32 vi = 0; vi < nv; vi += 1) fs.vgpu[vi].band_base = stream ? fs.plan[vi].band_base : kInvalid;
     HIP_CHECK(hipEventRecord(fs.ev[0], s));
     acquire_buffers(fs, false);
     fs.bg.clip_cap = fs.clip_cap;
@@ -1324,6 +1365,9 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     p.proxy_vis = (u8*)fs.proxy_vis.p;
     p.light_proxy_count = (u32*)fs.light_proxy_count.p;
     p.light_proxies = (ProxyRec*)fs.light_proxies.p;
+    p.color_host = stream ? (u32*)fs.out_h : nullptr;
+    p.band_count = zd + COUNTER_WORDS + (size_t)fs.nt * fs.max_samples;
+    p.band_flag = stream ? fs.band_flag_h : nullptr;
 
     /* kernels after the previous ordered batch's (ev[4]: its last pass) */
     if (after && after->in_flight) HIP_CHECK(hipStreamWaitEvent(s, after->ev[4], 0));
@@ -1345,16 +1389,9 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
         return true;
     }
 
-    /* host outputs: the colour planes (and 64-bit visibility) into pinned staging */
-    const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
-    const size_t stage = color_bytes + (size_t)fs.vis_stage_total * 8;
+    /* host outputs: the colour planes (unless streamed) and 64-bit visibility */
     if (stage) {
-        if (fs.out_h_cap < stage) {
-            if (fs.out_h) HIP_CHECK(hipHostFree(fs.out_h));
-            HIP_CHECK(hipHostMalloc((void**)&fs.out_h, stage, hipHostMallocDefault));
-            fs.out_h_cap = stage;
-        }
-        if (color_bytes) HIP_CHECK(hipMemcpyAsync(fs.out_h, fs.color.p, color_bytes, hipMemcpyDeviceToHost, s));
+        if (color_bytes && !stream) HIP_CHECK(hipMemcpyAsync(fs.out_h, fs.color.p, color_bytes, hipMemcpyDeviceToHost, s));
         /* 64-bit visibility of the views that ask for it, packed */
         for (u32 vi = 0; vi < nv; vi += 1) {
             if (!views[vi].vis64) continue;
@@ -1369,14 +1406,15 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
 }
 
 /* the copies of a launched batch's host outputs out of its pinned staging:
-   colour rows into the callers' buffers, 64-bit visibility planes */
+   colour rows into the callers' buffers (unless `color` is false),
+   64-bit visibility planes */
 struct Span { void* dst; const void* src; size_t n; };
-static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views, const FrameSet& fs) {
+static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views, const FrameSet& fs, bool color = true) {
     std::vector<Span> spans;
     const size_t color_bytes = fs.host_color ? (size_t)fs.color_total * 4 : 0;
     for (u32 vi = 0; vi < (u32)views.size(); vi += 1) {
         const ViewRecord& v = views[vi];
-        if (color_bytes && v.color_buffer) {
+        if (color && color_bytes && v.color_buffer) {
             const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
             for (i32 y = 0; y < v.height; y += 1) {
                 spans.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
@@ -1391,6 +1429,75 @@ static std::vector<Span> host_output_spans(const std::vector<ViewRecord>& views,
 static void copy_spans(const std::vector<Span>& spans) {
     parallel_runs(spans.size(), [&](size_t i) { return spans[i].n; },
                   [&](size_t i) { memcpy(spans[i].dst, spans[i].src, spans[i].n); }, 16, (size_t)1 << 20);
+}
+
+/* A streamed batch's colour rows by band (a view's tile row, whose flag the
+   shading kernels raise once its pixels are in the pinned staging). */
+struct StreamCopy {
+    std::vector<Span> rows;
+    struct Band { u32 flag; size_t first, end; };   /* rows [first, end) */
+    std::vector<Band> bands;
+    const u32* flags = nullptr;
+    hipEvent_t done = nullptr;      /* the batch's end */
+};
+
+static StreamCopy stream_plan(const std::vector<ViewRecord>& views, const FrameSet& fs) {
+    StreamCopy sc;
+    sc.flags = fs.band_flag_h;
+    sc.done = fs.ev[5];
+    for (u32 vi = 0; vi < (u32)views.size(); vi += 1) {
+        const ViewRecord& v = views[vi];
+        const u32 base = fs.plan[vi].band_base;
+        if (kInvalid == base || !v.color_buffer) continue;
+        const u32* src = (const u32*)fs.out_h + fs.plan[vi].color_base;
+        for (i32 ty = 0; ty < v.tile_rows; ty += 1) {
+            const size_t first = sc.rows.size();
+            for (i32 y = ty * 64; y < std::min(v.height, ty * 64 + 64); y += 1) {
+                sc.rows.push_back({v.color_buffer + (size_t)y * v.color_pitch, src + (size_t)y * v.width, (size_t)v.width * 4});
+            }
+            sc.bands.push_back({base + (u32)ty, first, sc.rows.size()});
+        }
+    }
+    return sc;
+}
+
+/* Copies a streamed batch's colour out of the pinned staging band by band,
+   each band once the shading kernels have raised its flag, while the later
+   bands still shade; `threads` threads take the bands in turn.  Returns
+   false, leaving further bands uncopied, when a band reports overflowed
+   intermediates (the batch runs again) or the batch ends with a band not
+   flagged (the caller then downloads the frame after the batch). */
+static bool stream_copy(const StreamCopy& sc, size_t threads) {
+    std::atomic<bool> fail{false};
+    auto run = [&](size_t t) {
+        for (size_t i = t; i < sc.bands.size(); i += threads) {
+            const StreamCopy::Band& bd = sc.bands[i];
+            u32 f = 0;
+            for (u32 spin = 1;; spin += 1) {
+                f = __atomic_load_n(&sc.flags[bd.flag], __ATOMIC_ACQUIRE);
+                if (0u != f || fail.load(std::memory_order_relaxed)) break;
+                if (0u == (spin & 4095u) && hipSuccess == hipEventQuery(sc.done)) {
+                    f = __atomic_load_n(&sc.flags[bd.flag], __ATOMIC_ACQUIRE);
+                    break;
+                }
+            }
+            if (1u != f) {
+                fail.store(true, std::memory_order_relaxed);
+                return;
+            }
+            for (size_t k = bd.first; k < bd.end; k += 1) memcpy(sc.rows[k].dst, sc.rows[k].src, sc.rows[k].n);
+        }
+    };
+    threads = std::max<size_t>(1u, std::min(threads, sc.bands.size()));
+    WorkerPool& pool = WorkerPool::get();
+    if (threads > 1 && threads <= pool.size() + 1 && pool.try_acquire()) {
+        pool.run(threads, std::function<void(size_t)>(run));
+        pool.release();
+    }
+    else {
+        for (size_t t = 0; t < threads; t += 1) run(t);
+    }
+    return !fail.load();
 }
 
 /* Moves a prepared batch's transient rows to where its frame set's
@@ -1429,10 +1536,12 @@ static void capture_outputs(Backend* b, const std::vector<ViewRecord>& views, co
    statistics and the captures. */
 static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& views, FrameSet& fs) {
     hipStream_t s = fs.stream;
-    bool copied = false;
-    if (fs.copier.joinable()) {
-        fs.copier.join();
-        copied = true;
+    if (fs.copier.joinable()) fs.copier.join();
+    bool copied = fs.host_copied;
+    fs.host_copied = false;
+    if (fs.streamed) {
+        if (copied) b->streamed_batches += 1;
+        else b->stream_fallbacks += 1;
     }
     for (int attempt = 0;; attempt += 1) {
         HIP_CHECK(hipStreamSynchronize(s));
@@ -1448,11 +1557,18 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
            prepared, moving this set's transient region */
         if (fs.layout_epoch != b->layout_epoch) relocate_transient(b, fs);
         b->reruns += 1;
-        launch_batch(b, views, fs);
+        launch_batch(b, views, fs, nullptr, false);
         copied = false;
     }
     fs.in_flight = false;
-    if (!copied) copy_spans(host_output_spans(views, fs));
+    if (!copied) {
+        /* a streamed launch whose bands were not all delivered: download
+           the frame now */
+        if (fs.streamed && fs.host_color) {
+            HIP_CHECK(hipMemcpy(fs.out_h, fs.color.p, (size_t)fs.color_total * 4, hipMemcpyDeviceToHost));
+        }
+        copy_spans(host_output_spans(views, fs));
+    }
     write_stats(c, views, fs);
     capture_outputs(b, views, fs);
 }
@@ -1556,6 +1672,16 @@ void backend_render(Backend* b, Context* cp, std::vector<ViewRecord>& views) {
     if (fs.in_flight) HIP_CHECK(hipStreamSynchronize(fs.stream));
     prepare_batch(b, *cp, views, fs);
     launch_batch(b, views, fs);
+    if (fs.streamed) {
+        /* the colour while the frame shades, then the rest after it */
+        if (stream_copy(stream_plan(views, fs), 2)) {
+            HIP_CHECK(hipEventSynchronize(fs.ev[5]));
+            if (0u == fs.counters_h[CNT_OVERFLOW]) {
+                copy_spans(host_output_spans(views, fs, false));
+                fs.host_copied = true;
+            }
+        }
+    }
     complete_batch(b, *cp, views, fs);
     b->last_set = 0;
 }
@@ -1566,16 +1692,24 @@ void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int
     const FrameSet* after = (ordered && b->ordered_set >= 0 && b->ordered_set != set) ? &b->sets[b->ordered_set] : nullptr;
     launch_batch(b, views, fs, after);
     if (ordered) b->ordered_set = set;
-    /* host outputs are copied as soon as the batch is done, while the caller
-       records and submits the next one; completing the batch joins the copy */
-    std::vector<Span> spans = host_output_spans(views, fs);
-    if (!spans.empty()) {
+    /* host outputs are copied as the batch delivers them (streamed colour
+       band by band, the rest once it is done), while the caller records and
+       submits the next one; completing the batch joins the copy */
+    std::vector<Span> spans = host_output_spans(views, fs, !fs.streamed);
+    if (!spans.empty() || fs.streamed) {
         /* an overflowed batch is re-run by backend_complete, which may read
            the caller's buffers again (views without clearing start from
            them): its outputs are not copied */
-        fs.copier = std::thread([ev = fs.ev[5], counters = fs.counters_h, sp = std::move(spans)]() {
+        StreamCopy sc;
+        if (fs.streamed) sc = stream_plan(views, fs);
+        fs.host_copied = false;
+        fs.copier = std::thread([streamed = fs.streamed, ev = fs.ev[5], counters = fs.counters_h, copied = &fs.host_copied,
+                                 sc = std::move(sc), sp = std::move(spans)]() {
+            if (streamed && !stream_copy(sc, 1)) return;
             HIP_CHECK(hipEventSynchronize(ev));
-            if (0u == counters[CNT_OVERFLOW]) copy_spans(sp);
+            if (0u != counters[CNT_OVERFLOW]) return;
+            copy_spans(sp);
+            *copied = true;
         });
     }
 }
@@ -1821,6 +1955,19 @@ static Backend* ensure_backend() {
 extern "C" void sftx_set_output(i32 mode) { ensure_backend()->output_mode = mode; }
 
 extern "C" void sftx_set_graphs(i32 enabled) { ensure_backend()->use_graphs = 0 != enabled; }
+
+extern "C" void sftx_stream_stats(u64* streamed, u64* fallbacks) {
+    Backend* b = ensure_backend();
+    if (streamed) *streamed = b->streamed_batches;
+    if (fallbacks) *fallbacks = b->stream_fallbacks;
+}
+
+extern "C" void sftx_set_stream_download(i32 enabled) {
+    Backend* b = ensure_backend();
+    drain_async_batches(*g_ctx);
+    quiesce(b);
+    b->stream_download = 0 != enabled;
+}
 
 extern "C" u64 sftx_graph_captures(void) { return ensure_backend()->graph_captures; }
 

@@ -112,6 +112,14 @@ struct FramePtrs {
     u32* tile_nch;          /* [n_tiles] payload chunks of each tile */
     u32* tile_done;         /* [n_tiles][max_samples] chunks finished (zeroed per frame) */
     u64* merge;             /* [n_tiles][max_samples][4096] merge planes of split tiles */
+    /* colour streamed to host memory while the frame shades (views with a
+       band_base): pixels at color_base offsets, and per band (a view's tile
+       row) the shading blocks finished (zeroed per frame) and a host-visible
+       flag set when the band's pixels are all in host memory (1, or 2 when
+       the frame's intermediates overflowed); null when nothing streams */
+    u32* color_host;
+    u32* band_count;
+    u32* band_flag;
 };
 
 /* enqueues the kernels of one batch of views, starting with the clear of the

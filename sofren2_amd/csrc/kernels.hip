@@ -1879,6 +1879,39 @@ DEV void count_fetches(const FramePtrs& P, u32 fetch, u64 act, int lane) {
     }
 }
 
+/* Colour streamed to host memory: a system-scope store, written through to
+   host memory (no cache line of it stays dirty in an L2, so no cache
+   write-back has to precede the band's flag). */
+DEV void host_store(u32* p, u32 v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+
+/* Views whose colour streams to host memory: every thread of a shading
+   block (256 per block, 16 blocks per tile) counts itself done after its
+   host store, the lanes of a wave that end at the same point together once
+   their wave's stores have completed (the wait covers all of the wave's
+   stores, those of lanes that ended earlier included), in the block's LDS
+   counter `lanes` (zeroed before the block's first barrier).  The block's
+   last lanes count the block done for its band (the view's tile row), and
+   the band's last block raises the band's host flag: every store of the
+   band completed before its block's count, so the host, reading the flag,
+   finds the band's pixels in place and copies them out while later bands
+   shade. */
+DEV void band_lanes_done(const FramePtrs& P, const ViewGpu& VW, u32 local_tile, u32* lanes) {
+    __builtin_amdgcn_s_waitcnt(0);
+    const u64 act = __ballot(true);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+        const u32 n = (u32)__popcll(act);
+        if (atomicAdd(lanes, n) + n == 256u) {
+            const u32 band = VW.band_base + local_tile / (u32)VW.fc.tile_cols;
+            const u32 need = (u32)VW.fc.tile_cols * 16u;
+            if (atomicAdd(&P.band_count[band], 1u) + 1u == need) {
+                /* the overflow bits are final: the passes that set them ran before */
+                const u32 ovf = __hip_atomic_load(&P.counters[CNT_OVERFLOW], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                host_store(&P.band_flag[band], (0u == ovf) ? 1u : 2u);
+            }
+        }
+    }
+}
+
 /* per-pixel deferred shading; a block covers 32x8 pixels of one tile, a wave a
    32x2 strip of eight 4x2 blocks, lane = pixel in the reference's block lane
    order (x offsets 0..3, rows 0..1) */
@@ -1886,7 +1919,9 @@ template <bool kShadows>
 __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     __shared__ TileShading<kShadows> s_ts;
     __shared__ u32 s_li[kMaxTileLights];
+    __shared__ u32 s_band_lanes;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (0 == tid) s_band_lanes = 0u;
     /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
     const u32 run = kShadows ? P.batch->n_shade_tiles - P.batch->n_shade_sh : 0u;
     const u32 tile = P.shade_tiles[run + (blockIdx.x >> 4)];
@@ -1940,10 +1975,19 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         const V3 p = valid ? V3{interp(b, t.px), interp(b, t.py), interp(b, t.pz)} : V3{0.f, 0.f, 0.f};
         cm = capsule_masks(tl, wave_span(p, valid), lane);
     }
-    if (!in_bounds) return;
-    u32* out = &P.color[VW.color_base + (u64)gy * fc.width + gx];
+    const bool streamed = kInvalid != VW.band_base;
+    if (!in_bounds) {
+        if (streamed) band_lanes_done(P, VW, local, &s_band_lanes);
+        return;
+    }
+    const u64 px = VW.color_base + (u64)gy * fc.width + gx;
+    u32* out = &P.color[px];
     if (!valid) {
         if (fc.should_clear) *out = fc.clear_color;
+        if (streamed) {
+            host_store(&P.color_host[px], *out);
+            band_lanes_done(P, VW, local, &s_band_lanes);
+        }
         return;
     }
     if (vid1 != vid) {
@@ -1962,6 +2006,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
     const V3 col = shade_pixel<kShadows>(P, VW, est, t, b, cu0 - cu1, cu0 - cu4, cv0 - cv1, cv0 - cv4, tl, fetch, group,
                                          lane & ~7, cm);
     *out = pack_argb(col.x * 255.f, col.y * 255.f, col.z * 255.f);
+    if (streamed) host_store(&P.color_host[px], *out);
 
     i32 shaded = 1, groups = (first == l) ? 1 : 0;
     /* counters: one atomic per wave (inactive lanes contribute nothing) */
@@ -1974,6 +2019,7 @@ __global__ __launch_bounds__(256) void k_shade(FramePtrs P) {
         stat_add(P.counters, CNT_QGROUPS, (u32)groups, sl);
     }
     count_fetches(P, fetch, act, lane);
+    if (streamed) band_lanes_done(P, VW, local, &s_band_lanes);
 }
 
 /* MSAA resolve (sofren2.c:6048-6135, 6677-6743).  Lanes are laid out as in
@@ -1997,7 +2043,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ u32 s_item_meta[4][kMsItems];     /* pixel lane | group start << 8 | group size << 16 | samples << 24 */
     __shared__ f32 s_item_der[4][4][kMsItems];   /* UV derivatives at the group's first item, then each item's colour */
     __shared__ u32 s_item_count[4];
+    __shared__ u32 s_band_lanes;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (0 == tid) s_band_lanes = 0u;
     /* 16 blocks of 32x8 pixels per tile; tiles of views with shadows last */
     const u32 run = P.batch->n_shade_tiles + (kShadows ? P.batch->n_shade_tiles_ms - P.batch->n_shade_ms_sh : 0u);
     const u32 tile = P.shade_tiles[run + (blockIdx.x >> 4)];
@@ -2195,6 +2243,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         acc_b = fmaf(s_item_der[wave][2][idx], w, acc_b);
     }
     if (write) *out = pack_argb(smin(acc_r, 1.f) * 255.f, smin(acc_g, 1.f) * 255.f, smin(acc_b, 1.f) * 255.f);
+    const bool streamed = kInvalid != VW.band_base;
+    if (streamed && in_bounds) {
+        /* the host copy's address from `out` plus a uniform offset */
+        const uintptr_t delta = (uintptr_t)P.color_host - (uintptr_t)P.color;
+        host_store((u32*)((uintptr_t)out + delta), *out);
+    }
 
     const u64 act = __ballot(true);
     shaded = wave_sum(shaded);
@@ -2213,6 +2267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int k = 0; k < 6; k += 1) stat_add(P.counters, kCnt[k], fsum[k], sl);
         stat_add(P.counters, CNT_CAP_TESTS, caps, sl);
     }
+    if (streamed) band_lanes_done(P, VW, local, &s_band_lanes);
 }
 
 /* ================================================================ shadows

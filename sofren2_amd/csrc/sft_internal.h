@@ -66,7 +66,8 @@ struct ViewGpu {
     u32 n_segs;         /* binning segments of the view, its clip segment last */
     u32 vis64;          /* 1: the view's 64-bit visibility planes are written */
     u32 raster_chunk;   /* payloads per visibility work item (the view's resolution class) */
-    u32 pad2;
+    u32 band_base;      /* colour streamed to host memory: the view's first band (tile row) of
+                           FramePtrs::band_count / band_flag; kInvalid: not streamed */
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
