@@ -307,7 +307,12 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
                 "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                "p99": round(float(np.percentile(lat, 99)), 3)},
                 "mean_batch": round((st1["views_in_batches"] - st0["views_in_batches"]) / max(batches, 1), 2),
-                "batches": batches})
+                "batches": batches,
+                # the server thread's time per request: preparing and
+                # launching, completing (waits included), and GPU spans
+                "host_submit_ms_per_req": round((st1["host_submit_ms"] - st0["host_submit_ms"]) / n, 4),
+                "host_complete_ms_per_req": round((st1["host_complete_ms"] - st0["host_complete_ms"]) / n, 4),
+                "gpu_ms_per_req": round((st1["gpu_ms"] - st0["gpu_ms"]) / n, 4)})
             sample += [(reqs[i], kept[i]) for i in kept]
             last_reqs = reqs
         out["resident_gb"] = round(srv.stats()["resident_bytes"] / 2**30, 3)

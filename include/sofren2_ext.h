@@ -188,6 +188,11 @@ typedef struct sftx_server_stats {
     u64 admission_waits;        /* times waiting requests did not fit the budget */
     u64 resident_bytes;         /* scene, BVH and texture state in HBM */
     u64 pending, in_flight;
+    /* time of the server thread: preparing and launching batches (host
+       tables, TLAS builds, uploads), completing them (waiting for them and
+       delivering their outputs), and the batches' GPU spans (first pass to
+       last, not counting waits for the batch before) */
+    f64 host_submit_ms, host_complete_ms, gpu_ms;
 } sftx_server_stats;
 
 i32 sftx_server_template(void);

@@ -1495,7 +1495,10 @@ static bool stream_copy(const StreamCopy& sc, size_t threads) {
         pool.release();
     }
     else {
-        for (size_t t = 0; t < threads; t += 1) run(t);
+        std::vector<std::thread> helpers;
+        for (size_t t = 1; t < threads; t += 1) helpers.emplace_back(run, t);
+        run(0);
+        for (std::thread& h : helpers) h.join();
     }
     return !fail.load();
 }
@@ -1705,7 +1708,11 @@ void backend_submit(Backend* b, Context* cp, std::vector<ViewRecord>& views, int
         fs.host_copied = false;
         fs.copier = std::thread([streamed = fs.streamed, ev = fs.ev[5], counters = fs.counters_h, copied = &fs.host_copied,
                                  sc = std::move(sc), sp = std::move(spans)]() {
-            if (streamed && !stream_copy(sc, 1)) return;
+            /* two threads per frame-sized band set, up to eight for batches
+               of several megabytes (first-touch page faults in the callers'
+               buffers slow a single thread well below the PCIe rate) */
+            const size_t threads = std::min<size_t>(8u, std::max<size_t>(2u, sc.rows.size() / 2048u));
+            if (streamed && !stream_copy(sc, threads)) return;
             HIP_CHECK(hipEventSynchronize(ev));
             if (0u != counters[CNT_OVERFLOW]) return;
             copy_spans(sp);

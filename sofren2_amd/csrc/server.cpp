@@ -186,8 +186,11 @@ void server_loop(Server* sv) {
                 sv->inflight.push_back(std::move(f));
                 InFlight& g = sv->inflight.back();
                 lk.unlock();
+                const Clock::time_point t0 = Clock::now();
                 backend_submit(c.backend, &c, g.views, g.set, true);
+                const f64 ms = std::chrono::duration<f64, std::milli>(Clock::now() - t0).count();
                 lk.lock();
+                sv->stats.host_submit_ms += ms;
                 continue;
             }
             sv->stats.admission_waits += 1;
@@ -200,9 +203,12 @@ void server_loop(Server* sv) {
             if (backend_batch_done(c.backend, f.set) || (size_t)kServerBatchesInFlight == sv->inflight.size() ||
                 !sv->pending.empty()) {
                 lk.unlock();
+                const Clock::time_point t0 = Clock::now();
                 backend_complete(c.backend, &c, f.views, f.set);
                 const Clock::time_point now = Clock::now();
                 lk.lock();
+                sv->stats.host_complete_ms += std::chrono::duration<f64, std::milli>(now - t0).count();
+                sv->stats.gpu_ms += (f64)(c.stats.ms_total - c.stats.ms_upload - c.stats.ms_download);
                 for (Request* q : f.reqs) {
                     q->t_done = now;
                     q->done = true;

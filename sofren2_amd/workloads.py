@@ -217,7 +217,8 @@ class Server:
     def stats(self):
         st = self.S.ServerStats()
         self.lib.server_get_stats(self.ct.byref(st))
-        return {n: int(getattr(st, n)) for n, _ in self.S.ServerStats._fields_}
+        return {n: (getattr(st, n) if t is self.ct.c_double else int(getattr(st, n)))
+                for n, t in self.S.ServerStats._fields_}
 
     def close(self):
         self.lib.server_release()
