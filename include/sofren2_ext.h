@@ -240,12 +240,20 @@ void sftx_stream_stats(u64* streamed, u64* fallbacks);
 #define SFTX_RES_CLASSES 3
 typedef struct sftx_tuning {
     u32 raster_chunk[SFTX_RES_CLASSES];   /* payloads per visibility work item, by class */
+    /* by class: estimated sweep work (instructions) above which a payload's
+       triangles are swept in 2, 4 or 8 row-band jobs; each job keeps the
+       64-px tile's accumulation origin (rows reached by add-only steps from
+       it), so the split never changes a bit; 0xFFFFFFFF: never split */
+    u32 raster_split[SFTX_RES_CLASSES];
     u32 raster_groups;                    /* persistent visibility workgroups */
     u32 raster_groups_resident;           /* workgroups resident at once on this device */
-    u32 tile_px;                          /* bin / raster tile side (fixed) */
-    u32 occlusion_stack;                  /* traversal stack entries per wave (fixed) */
+    u32 tile_px;                          /* bin / raster tile side (fixed: the reference's 64) */
+    /* occlusion traversal stack entries per wave: 256, 512 or 1024; a batch
+       whose traversal fills a smaller stack runs again with 1024 */
+    u32 occlusion_stack;
     f32 ms_class[SFTX_RES_CLASSES];       /* tuned visibility-pass time of each class's views (0: none) */
     f32 ms_batch;                         /* tuned visibility-pass time of the whole batch */
+    f32 ms_occlusion;                     /* tuned occlusion-pass time of the batch (0: no occlusion) */
 } sftx_tuning;
 i32 sftx_batch_tune(i32 repeats, sftx_tuning* report);   /* 0, or -1 without a recorded batch */
 void sftx_get_tuning(sftx_tuning* out);

@@ -48,16 +48,19 @@ class FrameStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """sftx_tuning: the autotuner's settings and timings."""
-    _fields_ = [("raster_chunk", ctypes.c_uint32 * 3), ("raster_groups", ctypes.c_uint32),
+    _fields_ = [("raster_chunk", ctypes.c_uint32 * 3), ("raster_split", ctypes.c_uint32 * 3),
+                ("raster_groups", ctypes.c_uint32),
                 ("raster_groups_resident", ctypes.c_uint32), ("tile_px", ctypes.c_uint32),
-                ("occlusion_stack", ctypes.c_uint32), ("ms_class", ctypes.c_float * 3), ("ms_batch", ctypes.c_float)]
+                ("occlusion_stack", ctypes.c_uint32), ("ms_class", ctypes.c_float * 3), ("ms_batch", ctypes.c_float),
+                ("ms_occlusion", ctypes.c_float)]
 
     def as_dict(self):
-        return {"raster_chunk_by_class": list(self.raster_chunk), "raster_groups": self.raster_groups,
+        return {"raster_chunk_by_class": list(self.raster_chunk), "raster_split_by_class": list(self.raster_split),
+                "raster_groups": self.raster_groups,
                 "raster_groups_resident": self.raster_groups_resident, "tile_px": self.tile_px,
                 "occlusion_stack": self.occlusion_stack,
                 "visibility_ms_by_class": [round(x, 4) for x in self.ms_class],
-                "visibility_ms_batch": round(self.ms_batch, 4)}
+                "visibility_ms_batch": round(self.ms_batch, 4), "occlusion_ms": round(self.ms_occlusion, 4)}
 
 
 class Request(ctypes.Structure):

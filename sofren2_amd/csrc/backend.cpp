@@ -217,6 +217,9 @@ struct Backend {
     /* payloads per visibility work item, by the view's resolution class
        (sftx_batch_tune chooses them) */
     u32 raster_chunk[SFTX_RES_CLASSES] = { 128, 128, 128 };
+    /* row-band split threshold of heavy payloads, by class */
+    u32 raster_split[SFTX_RES_CLASSES] = { kRasterJobCost, kRasterJobCost, kRasterJobCost };
+    u32 occ_stack = kOccStackEntries;    /* occlusion traversal stack entries per wave */
     int last_set = 0;               /* frame set of the last completed batch */
     int ordered_set = -1;           /* frame set of the last ordered submission */
     u32 raster_groups = 512;        /* persistent visibility workgroups */
@@ -1037,6 +1040,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.raster_chunk = std::max(1u, std::min(b->raster_chunk[res_class(v.width, v.height)], kRasterMaxChunk));
         min_chunk = std::min(min_chunk, g.raster_chunk);
         g.band_base = kInvalid;
+        g.raster_split = std::max(1u, b->raster_split[res_class(v.width, v.height)]);
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
             vis_stage_total += vp.grid;
@@ -1556,6 +1560,8 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         }
         if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
         if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
+        /* a smaller occlusion stack than the full one filled up: trace again with the full one */
+        if (fs.counters[CNT_OVERFLOW] & 4u) fs.bg.occ_stack = kOccStackEntries;
         /* another set's batch may have grown the pools since this one was
            prepared, moving this set's transient region */
         if (fs.layout_epoch != b->layout_epoch) relocate_transient(b, fs);
@@ -1739,7 +1745,8 @@ void backend_get_tuning(const Backend* b, sftx_tuning* t) {
     t->raster_groups = b->raster_groups;
     t->raster_groups_resident = raster_groups_per_device();
     t->tile_px = SFT_TILE_SIZE;
-    t->occlusion_stack = kOccStackEntries;
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) t->raster_split[k] = b->raster_split[k];
+    t->occlusion_stack = b->occ_stack;
 }
 
 void backend_set_tuning(Backend* b, const sftx_tuning* t) {
@@ -1747,31 +1754,58 @@ void backend_set_tuning(Backend* b, const sftx_tuning* t) {
         b->raster_chunk[k] = std::max(1u, std::min(t->raster_chunk[k], kRasterMaxChunk));
     }
     b->raster_groups = std::max(1u, std::min(t->raster_groups, 4u * raster_groups_per_device()));
+    for (int k = 0; k < SFTX_RES_CLASSES; k += 1) b->raster_split[k] = std::max(1u, t->raster_split[k]);
+    b->occ_stack = (t->occlusion_stack <= 256u) ? 256u : (t->occlusion_stack <= 512u) ? 512u : kOccStackEntries;
 }
 
 /* Autotuner (blocking, on frame set 0).  The knobs leave every output bit
-   unchanged and change only how the visibility pass is cut into work: the
-   payloads per work item of each resolution class (the item's triangles are
-   staged in LDS, up to 8 per payload; fewer payloads per item spread a dense
-   tile over more workgroups and merge more planes in HBM) and the size of
-   the persistent visibility grid.  Each candidate renders the views of the
-   class being tuned `repeats` times with eager launches; the objective is the
-   smallest visibility-pass time (k_raster_plan + k_raster, HIP events).
-   Classes are tuned one after the other, then the grid size on the whole
-   batch, which is rendered once more with the chosen settings so its
-   outputs are written as by a blocking submit. */
+   unchanged and change only how passes are cut into work: for each
+   resolution class the payloads per visibility work item (the item's
+   triangles are staged in LDS, up to 8 per payload; fewer payloads per item
+   spread a dense tile over more workgroups and merge more planes in HBM) and
+   the row-band split threshold (a payload whose estimated sweep exceeds it
+   is swept as 2, 4 or 8 jobs over interleaved rows, each reaching its rows
+   by add-only steps from the tile's origin); for the batch the size of the
+   persistent visibility grid and the occlusion traversal stack per wave
+   (LDS per workgroup; a stack that fills up re-runs the batch with the full
+   one).  Each candidate renders the views being tuned `repeats` times with
+   eager launches; the objective is the smallest pass time (HIP events:
+   k_raster_plan + k_raster for the visibility knobs, the occlusion pass for
+   the stack).  Classes are tuned one after the other, then the batch knobs
+   on the whole batch, which is rendered once more with the chosen settings
+   so its outputs are written as by a blocking submit.  A candidate replaces
+   the current choice only when at least 3% faster. */
 void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int repeats, sftx_tuning* report) {
     if (views.empty()) return;
     repeats = std::max(1, std::min(repeats, 64));
     const bool graphs = b->use_graphs;
     b->use_graphs = false;
-    auto measure = [&](std::vector<ViewRecord>& vs) {
+    auto measure_pass = [&](std::vector<ViewRecord>& vs, bool occlusion) {
         f32 best = 3.4e38f;
         for (int r = 0; r < repeats; r += 1) {
             backend_render(b, cp, vs);
-            best = std::min(best, cp->stats.ms_raster);
+            best = std::min(best, occlusion ? cp->stats.ms_shadows : cp->stats.ms_raster);
         }
         return best;
+    };
+    auto measure = [&](std::vector<ViewRecord>& vs) { return measure_pass(vs, false); };
+    /* one knob over candidate values: keeps the first value 3% faster than the best so far */
+    auto tune_knob = [&](u32& knob, std::initializer_list<u32> values, std::vector<ViewRecord>& vs, bool occlusion) {
+        u32 best_v = knob;
+        f32 best_ms = measure_pass(vs, occlusion);
+        for (u32 val : values) {
+            if (val == best_v) continue;
+            const u32 keep = knob;
+            knob = val;
+            const f32 ms = measure_pass(vs, occlusion);
+            knob = keep;
+            if (ms < best_ms * 0.97f) {
+                best_ms = ms;
+                best_v = val;
+            }
+        }
+        knob = best_v;
+        return best_ms;
     };
     static const u32 chunks[] = { 16, 32, 64, 96, 128 };
     for (int k = 0; k < SFTX_RES_CLASSES; k += 1) {
@@ -1796,6 +1830,7 @@ void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int r
             }
         }
         b->raster_chunk[k] = best_chunk;
+        best_ms = std::min(best_ms, tune_knob(b->raster_split[k], { 1024u, 2048u, 4096u, 8192u, 0xFFFFFFFFu }, sub, false));
         report->ms_class[k] = best_ms;
     }
     const u32 resident = raster_groups_per_device();
@@ -1813,13 +1848,17 @@ void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int r
         }
     }
     b->raster_groups = best_groups;
+    /* the occlusion stack, when the batch traces shadow packets */
+    bool occlusion = false;
+    for (const ViewRecord& v : views) occlusion = occlusion || (v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty());
+    const f32 ms_occ = occlusion ? tune_knob(b->occ_stack, { 256u, 512u, kOccStackEntries }, views, true) : 0.f;
     b->use_graphs = graphs;
     backend_render(b, cp, views);
-    report->ms_batch = best_ms;
     const f32 mc[SFTX_RES_CLASSES] = { report->ms_class[0], report->ms_class[1], report->ms_class[2] };
     backend_get_tuning(b, report);
     for (int k = 0; k < SFTX_RES_CLASSES; k += 1) report->ms_class[k] = mc[k];
     report->ms_batch = best_ms;
+    report->ms_occlusion = ms_occ;
 }
 
 /* sftx_present_noalloc: the recorded frame rendered on the no-allocation
@@ -1860,6 +1899,8 @@ static i32 render_noalloc(Backend* b, Context& c) {
         }
         if (fs.counters[CNT_OVERFLOW] & 1u) fs.clip_cap = std::max(fs.clip_cap * 2u, fs.counters[CNT_CLIP_TRIS] + 1024u);
         if (fs.counters[CNT_OVERFLOW] & 2u) fs.tile_list_cap = std::max(fs.tile_list_cap * 2u, fs.counters[CNT_TILE_PAIRS] + 4096u);
+        /* a smaller occlusion stack than the full one filled up: trace again with the full one */
+        if (fs.counters[CNT_OVERFLOW] & 4u) fs.bg.occ_stack = kOccStackEntries;
         b->reruns += 1;
     }
     if (v.color_buffer && v.width > 0 && v.height > 0) {

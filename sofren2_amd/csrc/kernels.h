@@ -17,7 +17,7 @@ enum Counter : int {
     CNT_CLIP_TRIS,          /* clipped-triangle slots used */
     CNT_CLIP_TILE_PAIRS,    /* (clip payload, tile) pairs */
     CNT_TILE_PAIRS,         /* total entries of the tile lists */
-    CNT_OVERFLOW,           /* bit 0: clip pool, bit 1: tile lists */
+    CNT_OVERFLOW,           /* bit 0: clip pool, bit 1: tile lists, bit 2: a reduced occlusion stack */
     CNT_SHADED,             /* pixels with geometry */
     CNT_QGROUPS,            /* (block, triangle) shading groups */
     CNT_TEX_BILINEAR,
@@ -47,6 +47,7 @@ constexpr int CNT_STAT_STRIDE = 32;     /* u32 per slot: one 128-byte line */
 /* largest payload count of one visibility work item (its triangles are staged in LDS) */
 constexpr u32 kRasterMaxChunk = 128;
 constexpr u32 kOccStackEntries = 1024;   /* occlusion traversal stack per wave */
+constexpr u32 kRasterJobCost = 2048;     /* default row-band split threshold (ViewGpu::raster_split) */
 constexpr size_t COUNTER_WORDS = CNT_COUNT + (size_t)CNT_COUNT * CNT_STAT_SLOTS * CNT_STAT_STRIDE;
 
 struct FramePtrs {

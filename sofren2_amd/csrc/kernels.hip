@@ -1665,7 +1665,6 @@ DEV int groups_per_tri(int n) { return (n <= 1) ? 8 : (n <= 2) ? 4 : (n <= 4) ? 
    triangle would keep one wave busy far longer than the others: part k of
    K takes every K-th of the rows its groups would sweep.  Waves take jobs
    from an LDS counter, the parts of split payloads first. */
-constexpr u32 kRasterJobCost = 2048;     /* estimated instructions of one job's sweep */
 constexpr u32 kRasterMaxParts = 8;
 
 __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
@@ -1734,7 +1733,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
                 const u32 c = (u32)rows[k] * 4u + (u32)((rows[k] + m - 1) / m) * (u32)chunks[k] * 20u;
                 cost = max(cost, c);
             }
-            while (parts_log < 3u && (cost >> parts_log) > kRasterJobCost) parts_log += 1u;
+            while (parts_log < 3u && (cost >> parts_log) > V.raster_split) parts_log += 1u;
         }
         /* job list: the parts of split payloads first, then the others, each
            in payload order; positions from prefix sums over waves 0 and 1
@@ -2282,15 +2281,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
    time against the 8 rays (lane = triangle x ray).  A ray's result is the OR
    over every triangle the frustum reaches, so the visiting order (the
    reference's depth-first stack) does not change it. */
-constexpr int kOccStack = (int)kOccStackEntries;
 constexpr int kOccWaves = 3;
 
 struct Frustum {
     f32 n[6][3], an[6][3], d[6];
 };
 
+template <int kStack>
 struct OccLds {
-    u32 stack[kOccStack];
+    u32 stack[kStack];
     u32 tris[72];
     u32 insts[64];
 };
@@ -2412,10 +2411,11 @@ DEV bool ray_hits_tri(const Est& est, V3 v0, V3 v1, V3 v2, V3 o, V3 d, f32 tmax)
    pushes the internal hits; leaf hits are returned in `leaf`/`child`.  The pop
    count is limited so that pushes never pass the stack's end; a stack with no
    room left sets `overflow`. */
-DEV void bvh_step(OccLds& W, u32& top, const sft_bvh_node* nodes, u32 n_nodes, const Frustum& f, int lane,
+template <int kStack>
+DEV void bvh_step(OccLds<kStack>& W, u32& top, const sft_bvh_node* nodes, u32 n_nodes, const Frustum& f, int lane,
                   bool& leaf, u32& child, u32& tested, bool& overflow) {
     u32 k = min(8u, top);
-    k = min(k, ((u32)kOccStack - top) / 7u);
+    k = min(k, ((u32)kStack - top) / 7u);
     leaf = false;
     child = kInvalid;
     if (0u == k) {
@@ -2444,7 +2444,8 @@ DEV void bvh_step(OccLds& W, u32& top, const sft_bvh_node* nodes, u32 n_nodes, c
 
 /* traces one packet (8 rays, `active` lanes) against the frame's static
    geometry; returns the rays that hit (4086-4243) */
-DEV u32 trace_packet(const FramePtrs& P, const ViewGpu& VW, const Est& est, OccLds& W, const Frustum& fr,
+template <int kStack>
+DEV u32 trace_packet(const FramePtrs& P, const ViewGpu& VW, const Est& est, OccLds<kStack>& W, const Frustum& fr,
                      sftvec o_r, sftvec l_r, u32 active, int lane, u32& nodes_tested, u32& tris_tested, bool& overflow) {
     const FrameConst& fc = VW.fc;
     u32 sat = 0;
@@ -2534,9 +2535,14 @@ DEV u32 trace_packet(const FramePtrs& P, const ViewGpu& VW, const Est& est, OccL
     return sat;
 }
 
-/* one workgroup per (light, packet of 8 proxies); wave w traces ray w */
+/* one workgroup per (light, packet of 8 proxies); wave w traces ray w.
+   kStack: traversal stack entries per wave (the autotuner's choice); a
+   packet that finds a stack smaller than kOccStackEntries full flags the
+   batch to run again with the full stack, so the results never depend on
+   the choice. */
+template <int kStack>
 __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
-    __shared__ OccLds s_w[kOccWaves];
+    __shared__ OccLds<kStack> s_w[kOccWaves];
     __shared__ u32 s_hit[kOccWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const ViewGpu& VW = P.views[find_view(P, blockIdx.x, [](const ViewGpu& v) { return v.occ_base; })];
@@ -2587,6 +2593,7 @@ __global__ __launch_bounds__(64 * kOccWaves) void k_occlude(FramePtrs P) {
         stat_add(P.counters, CNT_OCC_TRIS, tris_tested, sl);
         stat_add(P.counters, CNT_OCC_INVALID, valid ? 0u : 1u, sl);
         stat_add(P.counters, CNT_OCC_OVERFLOW, overflow ? 1u : 0u, sl);
+        if (overflow && kStack < (int)kOccStackEntries) atomicOr(&P.counters[CNT_OVERFLOW], 4u);
     }
     __syncthreads();
     if (0 == wave && r < n && lane < 8) {
@@ -2734,6 +2741,14 @@ u32 raster_groups_per_device() {
     return (u32)std::max(cus, 1) * (u32)std::max(per_cu, 1);
 }
 
+/* the occlusion pass with the batch's traversal stack size */
+static void launch_occlude(const FramePtrs& p, const BatchGpu& B, hipStream_t s) {
+    const dim3 grid(B.occ_total), block(64 * kOccWaves);
+    if (B.occ_stack <= 256u) hipLaunchKernelGGL(k_occlude<256>, grid, block, 0, s, p);
+    else if (B.occ_stack <= 512u) hipLaunchKernelGGL(k_occlude<512>, grid, block, 0, s, p);
+    else hipLaunchKernelGGL(k_occlude<(int)kOccStackEntries>, grid, block, 0, s, p);
+}
+
 void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStream_t s, hipEvent_t* ev,
                   const SideStream* side) {
     auto mark = [&](int k) { if (ev) hipEventRecord(ev[k], s); };
@@ -2757,13 +2772,13 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
         hipEventRecord(side->fork, s);
         hipStreamWaitEvent(side->stream, side->fork, 0);
         hipEventRecord(side->start, side->stream);
-        hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, side->stream, p);
+        launch_occlude(p, B, side->stream);
         if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, side->stream, p);
         hipEventRecord(side->end, side->stream);
         hipEventRecord(side->join, side->stream);
     }
     else {
-        if (B.occ_total > 0u) hipLaunchKernelGGL(k_occlude, dim3(B.occ_total), dim3(64 * kOccWaves), 0, s, p);
+        if (B.occ_total > 0u) launch_occlude(p, B, s);
         if (B.n_lights > 0u) hipLaunchKernelGGL(k_proxy_cache, dim3(B.n_lights), dim3(64), 0, s, p);
     }
     mark(7);

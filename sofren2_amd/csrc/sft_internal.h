@@ -68,6 +68,8 @@ struct ViewGpu {
     u32 raster_chunk;   /* payloads per visibility work item (the view's resolution class) */
     u32 band_base;      /* colour streamed to host memory: the view's first band (tile row) of
                            FramePtrs::band_count / band_flag; kInvalid: not streamed */
+    u32 raster_split;   /* estimated sweep above which a payload's rows are split into 2-8 row-band jobs */
+    u32 pad3;
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -79,7 +81,8 @@ struct BatchGpu {
     u32 n_shade_tiles, n_shade_tiles_ms, occ_total, n_lights;
     u32 vis64, raster_chunk, raster_groups, n_segs;
     u32 n_shade_sh, n_shade_ms_sh;  /* of the shaded tiles above, those of views with capsule shadows (listed last) */
-    u32 pad0, pad1;
+    u32 occ_stack;                  /* occlusion traversal stack entries per wave: 256, 512 or 1024 */
+    u32 pad1;
 };
 
 /* A binning segment: up to kSegItems consecutive batch slots of one view, or

@@ -47,8 +47,8 @@ def test_ctypes_structs_match_the_header(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("no C compiler")
     checks = {
-        "sftx_tuning": (S.Tuning, ["raster_chunk", "raster_groups", "raster_groups_resident", "tile_px",
-                                   "occlusion_stack", "ms_class", "ms_batch"]),
+        "sftx_tuning": (S.Tuning, ["raster_chunk", "raster_split", "raster_groups", "raster_groups_resident",
+                                   "tile_px", "occlusion_stack", "ms_class", "ms_batch", "ms_occlusion"]),
         "sftx_frame_stats": (S.FrameStats, ["meshlets_tested", "shadow_proxies", "ms_upload", "ms_shadows", "tex_npot",
                                             "instances_dropped"]),
         "sftx_request": (S.Request, ["scene", "camera_pos", "view", "proj", "color", "vis64"]),

@@ -160,10 +160,12 @@ def test_batched_views_match_reference():
 
 
 def test_tuned_settings_change_no_output_bit():
-    """The autotuner's candidates only re-cut the visibility pass into work
-    items: the tuned batch, and batches under extreme settings (one payload
-    per work item, half an occupancy-sized grid), give the default settings'
-    colour and visibility bit for bit, in every resolution class."""
+    """The autotuner's candidates only re-cut passes into work: the tuned
+    batch, and batches under extreme settings (one payload per work item,
+    every payload split into row bands or none, half an occupancy-sized grid,
+    the smallest occlusion stack), give the default settings' colour,
+    visibility and shadow-proxy visibility bit for bit, in every resolution
+    class."""
     lib = S.library()
     lib.sft_init(640, 360, 0)
     try:
@@ -182,31 +184,41 @@ def test_tuned_settings_change_no_output_bit():
         default = S.Tuning()
         lib.get_tuning(default)
         assert list(default.raster_chunk) == [128, 128, 128] and default.tile_px == 64
+        assert list(default.raster_split) == [2048, 2048, 2048] and default.occlusion_stack == 1024
         tuned = views()
         t = S.tune_batch(lib, tuned, repeats=2)
         print("tuned", t.as_dict(), flush=True)
         assert all(c in (16, 32, 64, 96, 128) for c in t.raster_chunk)
+        assert all(c in (1024, 2048, 4096, 8192, 0xFFFFFFFF) for c in t.raster_split)
+        assert t.occlusion_stack in (256, 512, 1024) and t.ms_occlusion > 0
         assert all(ms > 0 for ms in t.ms_class), "every class has views here"
         g = t.raster_groups_resident
         assert t.raster_groups in (g // 2, g, 2 * g)
         got = S.Tuning()
         lib.get_tuning(got)
         assert list(got.raster_chunk) == list(t.raster_chunk) and got.raster_groups == t.raster_groups
+        assert list(got.raster_split) == list(t.raster_split) and got.occlusion_stack == t.occlusion_stack
         for v, b in zip(tuned, base):
             np.testing.assert_array_equal(v.color, b.color)
             np.testing.assert_array_equal(v.vis, b.vis)
-        extreme = S.Tuning()
-        extreme.raster_chunk[:] = [1, 16, 1]
-        extreme.raster_groups = max(1, g // 2)
-        lib.set_tuning(extreme)
-        again = views()
-        S.render_batch(lib, again)
-        st = S.FrameStats()
-        lib.get_stats(st)
-        print("extreme settings: raster ms", st.ms_raster, flush=True)
-        for v, b in zip(again, base):
-            np.testing.assert_array_equal(v.color, b.color)
-            np.testing.assert_array_equal(v.vis, b.vis)
+        for chunks, split, stack in (([1, 16, 1], [1, 1, 1], 256), ([128, 1, 32], [0xFFFFFFFF] * 3, 512)):
+            extreme = S.Tuning()
+            extreme.raster_chunk[:] = chunks
+            extreme.raster_split[:] = split
+            extreme.raster_groups = max(1, g // 2)
+            extreme.occlusion_stack = stack
+            lib.set_tuning(extreme)
+            r0 = lib.overflow_reruns()
+            again = views()
+            S.render_batch(lib, again)
+            st = S.FrameStats()
+            lib.get_stats(st)
+            print("extreme settings", chunks, split, stack, ": raster ms", st.ms_raster, "occlusion ms", st.ms_shadows,
+                  "re-runs", lib.overflow_reruns() - r0, flush=True)
+            assert st.occlusion_overflow == 0
+            for v, b in zip(again, base):
+                np.testing.assert_array_equal(v.color, b.color)
+                np.testing.assert_array_equal(v.vis, b.vis)
         lib.set_tuning(default)
     finally:
         lib.scene_destroy()
