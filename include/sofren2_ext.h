@@ -58,6 +58,8 @@ void sftx_cap_clip_records(void* out);
    struct sft_world_shadow_proxy::light_vis, sofren2.c:1432-1437). */
 u32 sftx_cap_proxy_count(void);
 void sftx_cap_proxy_visibility(u8* out);    /* proxy_count * 64 bytes */
+u32 sftx_cap_tlas_count(void);               /* nodes of the frame's TLAS (SFTX_CAPTURE_SHADOWS) */
+void sftx_cap_tlas(void* out);               /* tlas_count sft_bvh_node records as the occlusion pass read them */
 
 /* Per-frame counters and pass timings of the last present. */
 typedef struct sftx_frame_stats {
@@ -221,6 +223,13 @@ void sftx_set_graphs(i32 enabled);
    fell back. */
 void sftx_set_stream_download(i32 enabled);
 void sftx_stream_stats(u64* streamed, u64* fallbacks);
+
+/* The frame's top-level BVH over the drawn instances (shadow packets only):
+   built on the device, one wave per view, by default; off, it is built on
+   the host and uploaded with the frame's tables.  The nodes are the same
+   bytes either way (the reference's builder, sofren2.c:2895-3062,
+   3619-3703). */
+void sftx_set_gpu_tlas(i32 enabled);
 
 /* Autotuner.  No counterpart in the reference (its tiles and job sizes are
    compile-time constants, sofren2.c:124-130); it exists for batches and the

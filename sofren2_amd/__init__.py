@@ -28,10 +28,12 @@ SCENE_CROWD_SHADOWS = 2
 SCENE_SERVED = 3
 SCENE_MAP = 4
 SCENE_FEATURES = 5
+SCENE_INSTANCED = 6
 
 # default resolution of each workload
 SCENE_SIZE = {SCENE_PLANE_SPHERES: (640, 360), SCENE_BOX_ROOM: (1920, 1080), SCENE_CROWD_SHADOWS: (1920, 1080),
-              SCENE_SERVED: (1920, 1080), SCENE_MAP: (1280, 720), SCENE_FEATURES: (640, 360)}
+              SCENE_SERVED: (1920, 1080), SCENE_MAP: (1280, 720), SCENE_FEATURES: (640, 360),
+              SCENE_INSTANCED: (960, 540)}
 
 
 class FrameStats(ctypes.Structure):
@@ -108,6 +110,9 @@ class Library:
         self.cap_clip_records = _bind(g, "sftx_cap_clip_records", None, [vp])
         self.cap_proxy_count = _bind(g, "sftx_cap_proxy_count", u32, [])
         self.cap_proxy_visibility = _bind(g, "sftx_cap_proxy_visibility", None, [vp])
+        self.cap_tlas_count = _bind(g, "sftx_cap_tlas_count", u32, [])
+        self.cap_tlas = _bind(g, "sftx_cap_tlas", None, [vp])
+        self.set_gpu_tlas = _bind(g, "sftx_set_gpu_tlas", None, [i32])
         self.get_stats = _bind(g, "sftx_get_stats", None, [ctypes.POINTER(FrameStats)])
         self.export_frame = _bind(g, "sftx_export_frame", None, [vp])
         self.sft_present = _bind(g, "sft_present", None, [])

@@ -225,6 +225,7 @@ struct Backend {
     u32 raster_groups = 512;        /* persistent visibility workgroups */
     bool use_graphs = true;         /* replay the frame's launches from a HIP graph */
     bool stream_download = true;    /* host colour streamed band by band while the frame shades */
+    bool gpu_tlas = true;           /* frame TLAS built on the device (k_tlas) */
     u64 streamed_batches = 0, stream_fallbacks = 0;
     u64 graph_captures = 0;
     u64 reruns = 0;                 /* batches run again after an intermediate overflowed */
@@ -241,8 +242,8 @@ struct Backend {
     i32 cap_cols = 0, cap_rows = 0, cap_samples = 1;
     std::vector<u32> h_vid;
     std::vector<f32> h_depth;
-    std::vector<u8> h_ss, h_ss_vis, h_clips, h_proxy_vis;
-    u32 h_ss_count = 0, h_clip_count = 0, h_proxy_count = 0;
+    std::vector<u8> h_ss, h_ss_vis, h_clips, h_proxy_vis, h_tlas;
+    u32 h_ss_count = 0, h_clip_count = 0, h_proxy_count = 0, h_tlas_count = 0;
 };
 
 Backend* backend_create() {
@@ -874,7 +875,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.plan.assign(nv, FrameSet::ViewPlan{});
     u32 ss_total = 0, max_samples = 1, occ_total = 0, min_chunk = kRasterMaxChunk;
     u64 vid_total = 0, color_total = 0, vis_stage_total = 0;
-    bool any_vis64 = false, host_color = false;
+    bool any_vis64 = false, host_color = false, any_tlas_dev = false;
     u32 n_bands = 0;
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
@@ -921,7 +922,24 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
             g.blas_root = r.blas_root;
             g.blas_count = r.blas_count;
             g.n_meshlets = r.total;
-            g.pad = 0;
+            /* the model-space box of the BLAS root's children: the TLAS item
+               of a model with a BLAS (sofren2.c:3630-3660) */
+            g.tlas_item = (m->blas_node_count > 0 && m->blas_nodes) ? 1u : 0u;
+            for (int k = 0; k < 3; k += 1) {
+                g.root_lo[k] = g.tlas_item ? 1e30f : 0.f;
+                g.root_hi[k] = g.tlas_item ? -1e30f : 0.f;
+            }
+            if (g.tlas_item) {
+                const sft_bvh_node& rn = m->blas_nodes[m->blas_root_ind];
+                for (u32 c = 0; c < rn.child_count; c += 1) {
+                    const f32 clo[3] = { rn.min_x[c], rn.min_y[c], rn.min_z[c] };
+                    const f32 chi[3] = { rn.max_x[c], rn.max_y[c], rn.max_z[c] };
+                    for (int k = 0; k < 3; k += 1) {
+                        if (clo[k] < g.root_lo[k]) g.root_lo[k] = clo[k];
+                        if (chi[k] > g.root_hi[k]) g.root_hi[k] = chi[k];
+                    }
+                }
+            }
             if (in.posed_bone_matrices) {
                 g.bone_base = (u32)(bones.size() / 16);
                 const u32 n = std::max(r.max_bone, 1u);
@@ -960,7 +978,18 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
         std::vector<sft_bvh_node> vt;
         u32 tlas_root = 0;
-        if (occlusion) tlas_root = build_tlas(v.instances, vt);
+        bool tlas_dev = false;
+        if (occlusion && b->gpu_tlas && v.instances.size() <= 64) {
+            /* built on the device by k_tlas: reserve its nodes */
+            u32 items = 0;
+            for (const sft_instance& in : v.instances) items += (in.model->blas_node_count > 0 && in.model->blas_nodes) ? 1u : 0u;
+            vt.assign(tlas_node_count(items), sft_bvh_node{});
+            tlas_dev = true;
+            any_tlas_dev = true;
+        }
+        else if (occlusion) {
+            tlas_root = build_tlas(v.instances, vt);
+        }
         tlas.insert(tlas.end(), vt.begin(), vt.end());
 
         FrameConst& fc = fs.vgpu[vi].fc;
@@ -1041,6 +1070,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         min_chunk = std::min(min_chunk, g.raster_chunk);
         g.band_base = kInvalid;
         g.raster_split = std::max(1u, b->raster_split[res_class(v.width, v.height)]);
+        g.tlas_build = tlas_dev ? 1u : 0u;
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
             vis_stage_total += vp.grid;
@@ -1115,6 +1145,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.vis64 = any_vis64 ? 1u : 0u;
     bg.raster_chunk = min_chunk;   /* sizes the work-item list */
     bg.raster_groups = b->raster_groups;
+    bg.occ_stack = b->occ_stack;
+    bg.tlas_gpu = any_tlas_dev ? 1u : 0u;
     bg.n_segs = (u32)segs.size();
     fs.bin_m_words = bin_m_words;
 
@@ -1661,6 +1693,13 @@ static void capture_outputs(Backend* b, const std::vector<ViewRecord>& views, co
         const bool occlusion = v0.shadows_enabled && !v0.point_lights.empty() && !v0.proxies.empty();
         b->h_proxy_count = fs.n_proxies;
         b->h_proxy_vis.assign((size_t)fs.n_proxies * 64, 0);
+        /* the view's TLAS nodes as the occlusion pass read them */
+        b->h_tlas_count = occlusion ? fs.vgpu[0].fc.tlas_nodes : 0u;
+        b->h_tlas.resize((size_t)b->h_tlas_count * sizeof(sft_bvh_node));
+        if (b->h_tlas_count) {
+            HIP_CHECK(hipMemcpy(b->h_tlas.data(), (const u8*)fs.pack_d.p + fs.o_tlas + (size_t)fs.plan[0].tlas_base * sizeof(sft_bvh_node),
+                                b->h_tlas.size(), hipMemcpyDeviceToHost));
+        }
         if (occlusion) {
             HIP_CHECK(hipMemcpy(b->h_proxy_vis.data(), fs.proxy_vis.p, b->h_proxy_vis.size(), hipMemcpyDeviceToHost));
             /* counts of lights beyond the traced ones are not kept */
@@ -1991,6 +2030,14 @@ extern "C" void sftx_cap_proxy_visibility(u8* out) {
     if (!b->h_proxy_vis.empty()) memcpy(out, b->h_proxy_vis.data(), b->h_proxy_vis.size());
 }
 
+extern "C" u32 sftx_cap_tlas_count(void) { return (g_ctx && g_ctx->backend) ? g_ctx->backend->h_tlas_count : 0u; }
+
+extern "C" void sftx_cap_tlas(void* out) {
+    Backend* b = g_ctx ? g_ctx->backend : nullptr;
+    if (!b || !out) return;
+    if (!b->h_tlas.empty()) memcpy(out, b->h_tlas.data(), b->h_tlas.size());
+}
+
 static Backend* ensure_backend() {
     if (!g_ctx) {
         fprintf(stderr, "fatal sofren2 error: used before sft_init\n");
@@ -2008,6 +2055,13 @@ extern "C" void sftx_stream_stats(u64* streamed, u64* fallbacks) {
     Backend* b = ensure_backend();
     if (streamed) *streamed = b->streamed_batches;
     if (fallbacks) *fallbacks = b->stream_fallbacks;
+}
+
+extern "C" void sftx_set_gpu_tlas(i32 enabled) {
+    Backend* b = ensure_backend();
+    drain_async_batches(*g_ctx);
+    quiesce(b);
+    b->gpu_tlas = 0 != enabled;
 }
 
 extern "C" void sftx_set_stream_download(i32 enabled) {

@@ -2741,8 +2741,226 @@ u32 raster_groups_per_device() {
     return (u32)std::max(cus, 1) * (u32)std::max(per_cu, 1);
 }
 
-/* the occlusion pass with the batch's traversal stack size */
+/* ================================================================ TLAS
+   The frame's top-level BVH of a view that traces shadow packets, built on
+   the device (sofren2.c:3619-3703) into the view's nodes (reserved by the
+   host, which knows their count: it depends on the item count only).  Items
+   are the draw slots whose model has a BLAS, numbered over the first
+   `built` slots as the reference numbers them (a slot without a BLAS drawn
+   before one with a BLAS leaves a zeroed item).  The lanes compute the
+   items' world boxes (the root box's 8 corners through the transform, the
+   reference's operation order); lane 0 then groups them as the reference's
+   builder does (sofren2.c:2895-3062): three levels of halving, each range
+   sorted by the Hoare quicksort around the middle element along its
+   centroids' widest axis, ranges of one item or level 3 becoming groups.
+   The partitions of a range touch only that range, so sorting them in any
+   order gives the reference's permutation.  With at most 64 items (the
+   instance cap) every group holds at most 8: the root, then one leaf per
+   group in group order.  Each node is written by the 64 lanes, a 32-bit
+   word each (256 bytes, padding zero). */
+constexpr int kTlasMaxItems = 64;
+
+struct TlasItem {
+    u32 id;
+    f32 lo[3], hi[3], mid[3];
+};
+
+struct TlasLds {
+    TlasItem it[kTlasMaxItems];
+    u32 gstart[8], gcount[8], ng;
+    u32 rs[2][8], rc[2][8], fin[2][8];   /* ranges of the current and next halving level */
+    u16 stk[2 * kTlasMaxItems];
+};
+
+/* sorts it[0, n) by mid[axis] (sofren2.c:2895-2924), explicit stack */
+DEV void tlas_sort(TlasItem* it, u16* stk, int n0, int axis) {
+    int sp = 0;
+    stk[sp++] = 0;
+    stk[sp++] = (u16)n0;
+    while (sp > 0) {
+        int n = stk[--sp];
+        int off = stk[--sp];
+        while (n > 1) {
+            TlasItem* a = it + off;
+            const f32 pivot = a[n / 2].mid[axis];
+            int i = 0, j = n - 1;
+            while (i <= j) {
+                while (a[i].mid[axis] < pivot) i += 1;
+                while (a[j].mid[axis] > pivot) j -= 1;
+                if (i <= j) {
+                    u32* pi = (u32*)&a[i];
+                    u32* pj = (u32*)&a[j];
+                    for (int w = 0; w < (int)(sizeof(TlasItem) / 4); w += 1) {
+                        const u32 x = pi[w];
+                        pi[w] = pj[w];
+                        pj[w] = x;
+                    }
+                    i += 1;
+                    j -= 1;
+                }
+            }
+            if (j > 0) {
+                stk[sp++] = (u16)off;
+                stk[sp++] = (u16)(j + 1);
+            }
+            if (i >= n) break;
+            off += i;
+            n -= i;
+        }
+    }
+}
+
+DEV void tlas_grow(f32* b, const f32* lo, const f32* hi) {
+    for (int k = 0; k < 3; k += 1) {
+        if (lo[k] < b[k]) b[k] = lo[k];
+        if (hi[k] > b[3 + k]) b[3 + k] = hi[k];
+    }
+}
+
+/* lane = 32-bit word of an sft_bvh_node whose children c < count have boxes
+   lo(c)/hi(c) and indices idx(c) */
+template <class Lo, class Hi, class Idx>
+DEV void tlas_write_node(sft_bvh_node* nd, int lane, u32 count, u32 leaf, Lo lo, Hi hi, Idx idx) {
+    u32 w = 0u;
+    if (lane < 48) {
+        const int arr = lane >> 3, c = lane & 7;
+        f32 v = (arr < 3) ? 1e30f : -1e30f;
+        if ((u32)c < count) v = (arr < 3) ? lo(c)[arr] : hi(c)[arr - 3];
+        w = f2u(v);
+    }
+    else if (lane < 56) {
+        const int c = lane - 48;
+        w = ((u32)c < count) ? idx(c) : kInvalid;
+    }
+    else if (lane == 56) {
+        w = count | (leaf << 8);
+    }
+    ((u32*)nd)[lane] = w;
+}
+
+__global__ __launch_bounds__(64) void k_tlas(FramePtrs P) {
+    __shared__ TlasLds S;
+    const int lane = threadIdx.x;
+    const ViewGpu& VW = P.views[blockIdx.x];
+    const FrameConst& fc = VW.fc;
+    if (0u == VW.tlas_build || 0u == fc.tlas_nodes) return;
+    const u32 n_inst = min(fc.n_inst, (u32)kTlasMaxItems);
+    /* items by draw slot */
+    bool has = false;
+    if ((u32)lane < n_inst) {
+        const InstGpu& in = P.insts[VW.inst_base + (u32)lane];
+        has = 0u != in.tlas_item;
+        TlasItem t;
+        t.id = 0u;
+        for (int k = 0; k < 3; k += 1) t.lo[k] = t.hi[k] = t.mid[k] = 0.f;
+        if (has) {
+            for (int k = 0; k < 3; k += 1) {
+                t.lo[k] = 1e30f;
+                t.hi[k] = -1e30f;
+            }
+            const f32* T = in.t;
+            for (int corner = 0; corner < 8; corner += 1) {
+                const f32 x = (corner & 1) ? in.root_hi[0] : in.root_lo[0];
+                const f32 y = (corner & 2) ? in.root_hi[1] : in.root_lo[1];
+                const f32 z = (corner & 4) ? in.root_hi[2] : in.root_lo[2];
+                for (int k = 0; k < 3; k += 1) {
+                    const f32 wk = x * T[k] + y * T[4 + k] + z * T[8 + k] + T[12 + k];
+                    if (wk < t.lo[k]) t.lo[k] = wk;
+                    if (wk > t.hi[k]) t.hi[k] = wk;
+                }
+            }
+            t.id = (u32)lane;
+            for (int k = 0; k < 3; k += 1) t.mid[k] = (t.lo[k] + t.hi[k]) * 0.5f;
+        }
+        u32* dst = (u32*)&S.it[lane];
+        const u32* src = (const u32*)&t;
+        for (int w = 0; w < (int)(sizeof(TlasItem) / 4); w += 1) dst[w] = src[w];
+    }
+    const u32 built = (u32)__popcll(__ballot(has));
+    __syncthreads();
+    /* grouping (lane 0) */
+    if (0 == lane && built > 8u) {
+        u32 nr = 1;
+        S.rs[0][0] = 0u;
+        S.rc[0][0] = built;
+        S.fin[0][0] = 0u;
+        for (u32 level = 0; level < 3u; level += 1) {
+            const u32 cur = level & 1u, nxt = cur ^ 1u;
+            u32* rs = S.rs[cur];
+            u32* rc = S.rc[cur];
+            u32* fin = S.fin[cur];
+            u32* ns = S.rs[nxt];
+            u32* nc = S.rc[nxt];
+            u32* nf = S.fin[nxt];
+            u32 nn = 0;
+            for (u32 r = 0; r < nr; r += 1) {
+                const u32 st = rs[r], ct = rc[r];
+                if (fin[r] || 1u == ct) {
+                    ns[nn] = st; nc[nn] = ct; nf[nn] = 1u; nn += 1;
+                    continue;
+                }
+                f32 lo[3] = { 1e30f, 1e30f, 1e30f }, hi[3] = { -1e30f, -1e30f, -1e30f };
+                for (u32 i = st; i < st + ct; i += 1) {
+                    for (int k = 0; k < 3; k += 1) {
+                        if (S.it[i].mid[k] < lo[k]) lo[k] = S.it[i].mid[k];
+                        if (S.it[i].mid[k] > hi[k]) hi[k] = S.it[i].mid[k];
+                    }
+                }
+                const f32 ext0 = hi[0] - lo[0], ext1 = hi[1] - lo[1], ext2 = hi[2] - lo[2];
+                int axis = 0;
+                if (ext1 > ext0 && ext1 > ext2) axis = 1;
+                else if (ext2 > ext0 && ext2 > ext1) axis = 2;
+                tlas_sort(S.it + st, S.stk, (int)ct, axis);
+                const u32 half = ct / 2u;
+                ns[nn] = st; nc[nn] = half; nf[nn] = 0u; nn += 1;
+                ns[nn] = st + half; nc[nn] = ct - half; nf[nn] = 0u; nn += 1;
+            }
+            nr = nn;
+        }
+        /* three levels: the ranges are in buffer 1 */
+        for (u32 r = 0; r < nr; r += 1) {
+            S.gstart[r] = S.rs[1][r];
+            S.gcount[r] = S.rc[1][r];
+        }
+        S.ng = nr;
+    }
+    __syncthreads();
+    sft_bvh_node* nodes = const_cast<sft_bvh_node*>(P.tlas) + VW.tlas_base;
+    if (built <= 8u) {
+        tlas_write_node(nodes, lane, built, 1u, [&](int c) { return S.it[c].lo; }, [&](int c) { return S.it[c].hi; },
+                        [&](int c) { return 0x80000000u | S.it[c].id; });
+        return;
+    }
+    const u32 ng = S.ng;
+    /* leaf g: its group's items; the root: the leaves' boxes */
+    for (u32 g = 0; g < ng; g += 1) {
+        const u32 s0 = S.gstart[g];
+        tlas_write_node(nodes + 1u + g, lane, S.gcount[g], 1u, [&](int c) { return S.it[s0 + c].lo; },
+                        [&](int c) { return S.it[s0 + c].hi; }, [&](int c) { return 0x80000000u | S.it[s0 + c].id; });
+    }
+    f32 gb[6] = { 1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f };
+    if ((u32)lane < ng) {
+        for (u32 i = 0; i < S.gcount[lane]; i += 1) tlas_grow(gb, S.it[S.gstart[lane] + i].lo, S.it[S.gstart[lane] + i].hi);
+    }
+    /* the root's words come from the lanes holding each leaf's box */
+    u32 w = 0u;
+    const int arr = lane >> 3, c = lane & 7;
+    /* array `arr` of leaf c's box, from lane c */
+    f32 v = 0.f;
+    for (int k = 0; k < 6; k += 1) {
+        const f32 g = __shfl(gb[k], c);
+        v = (arr == k) ? g : v;
+    }
+    if (lane < 48) w = f2u(((u32)c < ng) ? v : ((arr < 3) ? 1e30f : -1e30f));
+    else if (lane < 56) w = ((u32)(lane - 48) < ng) ? 1u + (u32)(lane - 48) : kInvalid;
+    else if (lane == 56) w = ng;
+    ((u32*)nodes)[lane] = w;
+}
+
+/* the occlusion pass with the batch's traversal stack size, after the
+   device-built TLAS */
 static void launch_occlude(const FramePtrs& p, const BatchGpu& B, hipStream_t s) {
+    if (B.tlas_gpu) hipLaunchKernelGGL(k_tlas, dim3(B.n_views), dim3(64), 0, s, p);
     const dim3 grid(B.occ_total), block(64 * kOccWaves);
     if (B.occ_stack <= 256u) hipLaunchKernelGGL(k_occlude<256>, grid, block, 0, s, p);
     else if (B.occ_stack <= 512u) hipLaunchKernelGGL(k_occlude<512>, grid, block, 0, s, p);

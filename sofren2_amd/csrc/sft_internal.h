@@ -69,7 +69,7 @@ struct ViewGpu {
     u32 band_base;      /* colour streamed to host memory: the view's first band (tile row) of
                            FramePtrs::band_count / band_flag; kInvalid: not streamed */
     u32 raster_split;   /* estimated sweep above which a payload's rows are split into 2-8 row-band jobs */
-    u32 pad3;
+    u32 tlas_build;     /* 1: k_tlas builds the view's TLAS into its (reserved) nodes */
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -82,7 +82,7 @@ struct BatchGpu {
     u32 vis64, raster_chunk, raster_groups, n_segs;
     u32 n_shade_sh, n_shade_ms_sh;  /* of the shaded tiles above, those of views with capsule shadows (listed last) */
     u32 occ_stack;                  /* occlusion traversal stack entries per wave: 256, 512 or 1024 */
-    u32 pad1;
+    u32 tlas_gpu;                   /* 1: some view's TLAS is built on the device (k_tlas) */
 };
 
 /* A binning segment: up to kSegItems consecutive batch slots of one view, or
@@ -105,7 +105,8 @@ struct InstGpu {
     u32 blas_root;      /* root node relative to blas_base */
     u32 blas_count;     /* nodes of the model's BLAS */
     u32 n_meshlets;     /* total_meshlets of the model */
-    u32 pad;
+    u32 tlas_item;      /* 1: the model has a BLAS, so the instance is a TLAS item */
+    f32 root_lo[3], root_hi[3];   /* model-space box of the BLAS root's children */
 };
 
 /* world-space capsule proxy as the GPU reads it */
@@ -223,6 +224,7 @@ void snapshot_view(const Context& c, ViewRecord& v);
 
 /* frame TLAS over the drawn instances (tlas.cpp); returns the root node */
 u32 build_tlas(const std::vector<sft_instance>& instances, std::vector<sft_bvh_node>& nodes);
+u32 tlas_node_count(u32 items);     /* nodes k_tlas writes for `items` items (at most 64) */
 
 struct Backend;
 Backend* backend_create();

@@ -121,6 +121,17 @@ u32 build_node(Item* it, u32 n, std::vector<sft_bvh_node>& nodes, f32* bounds) {
 
 }  // namespace
 
+/* nodes of the TLAS over `items` items: one leaf root up to 8 items, else the
+   root and one leaf per group of the three halving levels (every group holds
+   at most 8 items up to 64) */
+static u32 halving_groups(u32 count, u32 level) {
+    if (0 == count) return 0;
+    if (3 == level || 1 == count) return 1;
+    return halving_groups(count / 2, level + 1) + halving_groups(count - count / 2, level + 1);
+}
+
+u32 tlas_node_count(u32 items) { return (items <= 8u) ? 1u : 1u + halving_groups(items, 0); }
+
 /* Items are indexed by draw slot while only instances with a BLAS are counted:
    an instance without one, drawn before one with a BLAS, leaves a zeroed item
    inside the built range (the reference leaves that item unwritten). */

@@ -89,6 +89,7 @@ typedef struct {
     f32 phase[SFTX_MAX_CHARS][SFTX_BONES][2];
     sftmat posed[SFTX_MAX_CHARS][SFTX_BONES];
     sftvec light_pos[SFTX_SHADOW_LIGHTS], light_col[SFTX_SHADOW_LIGHTS];
+    sftmat copy_xform[40];                 /* the instanced workload's room copies */
 } sftx_crowd_state;
 
 /* the features workload: every shading branch the other scenes miss */
@@ -897,6 +898,62 @@ static void sftx_crowd_render(i32 frame) {
     }
 }
 
+/* ---------------- workload 6: instanced BLAS models -----------------------
+   The crowd scene on a 6^3 lattice with 8 humanoids and 12 shadow-casting
+   lights, its room model (with its BLAS) drawn 40 more times under rigid
+   transforms (a yaw and an offset of up to 30 units), so that the frame's
+   TLAS groups many instances with a BLAS.  4 humanoids (no BLAS) are drawn
+   before the room: they leave zeroed items in the TLAS's item range, as
+   the reference numbers items by draw slot (sofren2.c:3630-3660). */
+#define SFTX_INST_COPIES 40
+static void sftx_instanced_create(void) {
+    g_scene.lattice = 6;
+    g_scene.lm_size = 1024;
+    g_scene.n_chars = 8;
+    g_scene.n_lights = 12;
+    sftx_crowd_create();
+    sftx_rng rng;
+    sftx_rng_seed(&rng, g_scene.seed, 0x27D4EB2FULL);
+    for (i32 k = 0; k < SFTX_INST_COPIES; k += 1) {
+        const f32 yaw = sftx_uniform(&rng, 0.f, 6.2831853f);
+        const f32 x = sftx_uniform(&rng, -30.f, 30.f), y = sftx_uniform(&rng, -10.f, 10.f);
+        const f32 z = sftx_uniform(&rng, -30.f, 30.f);
+        g_crowd.copy_xform[k] = sft_m_mul(sft_m_rot_y(yaw), sft_m_translate(x, y, z));
+    }
+    const u32 room_tris = g_scene.tri_count - (u32)g_scene.n_chars * SFTX_BANDS * SFTX_SEGS * 2;
+    const u32 room_meshlets = g_scene.meshlet_count - (u32)g_scene.n_chars * SFTX_BANDS;
+    g_scene.tri_count += room_tris * SFTX_INST_COPIES;
+    g_scene.meshlet_count += room_meshlets * SFTX_INST_COPIES;
+}
+
+static void sftx_instanced_render(i32 frame) {
+    i32 pose, cam;
+    sftx_split_frame(frame, &pose, &cam);
+    sftx_set_scene_camera(frame);
+    sft_set_ambient_light(0.3f, 0.3f, 0.3f);
+    for (i32 l = 0; l < g_scene.n_lights; l += 1) {
+        sft_draw_point_light(g_crowd.light_pos[l], g_crowd.light_col[l].x, g_crowd.light_col[l].y, g_crowd.light_col[l].z,
+                             1.f, 40.f, (u32)(l + 1));
+    }
+    sftx_crowd_pose(pose);
+    for (i32 c = 0; c < g_scene.n_chars; c += 1) {
+        if (c == g_scene.n_chars / 2) {
+            sft_draw_instance(&g_room.inst);
+            for (i32 k = 0; k < SFTX_INST_COPIES; k += 1) {
+                sft_instance copy = g_room.inst;
+                copy.transform = g_crowd.copy_xform[k];
+                sft_draw_instance(&copy);
+            }
+        }
+        sft_instance inst;
+        memset(&inst, 0, sizeof(inst));
+        inst.model = g_crowd.person[c];
+        inst.transform = g_crowd.xform[c];
+        inst.posed_bone_matrices = g_crowd.posed[c];
+        sft_draw_instance(&inst);
+    }
+}
+
 /* ---------------- workload 3: served scene variants -----------------------
    The crowd scene with sizes drawn from the seed: a box lattice of L^3 boxes
    with L = round(cbrt(T / 63)) for T log-uniform in [50k, 1M] triangles
@@ -1320,6 +1377,7 @@ static i32 sftx_build(i32 kind) {
         case SFTX_SCENE_SERVED: sftx_served_create(); return 0;
         case SFTX_SCENE_MAP: sftx_map_create(); return 0;
         case SFTX_SCENE_FEATURES: sftx_features_create(); return 0;
+        case SFTX_SCENE_INSTANCED: sftx_instanced_create(); return 0;
         default: return -1;
     }
 }
@@ -1358,6 +1416,7 @@ static void sftx_draw_current(i32 frame) {
         case SFTX_SCENE_SERVED: sftx_served_render(frame); break;
         case SFTX_SCENE_MAP: sftx_map_render(frame); break;
         case SFTX_SCENE_FEATURES: sftx_features_render(frame); break;
+        case SFTX_SCENE_INSTANCED: sftx_instanced_render(frame); break;
         default: break;
     }
 }

@@ -32,6 +32,10 @@
  *       lights, more than 16 point lights on a tile, 64 draws (frames
  *       500-999: six more, which must be dropped), no clearing (the caller's
  *       buffer shows through and blends under MSAA).
+ *   SFTX_SCENE_INSTANCED (6): the crowd scene on a 6^3 lattice, 8 humanoids,
+ *       12 shadow-casting lights, the room model drawn 40 more times under
+ *       rigid transforms (a TLAS over many instances with a BLAS, humanoids
+ *       without one drawn first).
  */
 #ifndef SFTX_SCENES_H
 #define SFTX_SCENES_H
@@ -49,6 +53,7 @@ enum sftx_scene_kind {
     SFTX_SCENE_SERVED = 3,
     SFTX_SCENE_MAP = 4,
     SFTX_SCENE_FEATURES = 5,
+    SFTX_SCENE_INSTANCED = 6,
     SFTX_SCENE_COUNT
 };
 
