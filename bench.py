@@ -125,6 +125,70 @@ def cpu_baseline(scene, name, width, height, frames):
                       f"median {r['ms_median']:.2f} ms, p99 {r['ms_p99']:.2f} ms"}
 
 
+def tuning_table(lib, repeats=3):
+    """Every autotuner candidate timed on the 64-view mixed batch (eager
+    launches, best of `repeats`): the visibility pass per payloads-per-item and
+    per split threshold for each resolution class's views, the visibility
+    grid and the occlusion stack on the whole batch.  One JSON object."""
+    import ctypes
+    import sofren2_amd as S
+    from sofren2_amd import workloads
+    lib.sft_init(640, 360, 0)
+    lib.set_graphs(0)
+    mb = workloads.MixedBatch(lib)
+
+    def res_class(w, h):
+        return 0 if w * h < (1 << 18) else (1 if w * h < (1 << 20) else 2)
+
+    classes = {k: [mb.views[i] for i, (_, _, w, h, _) in enumerate(mb.spec) if res_class(w, h) == k] for k in range(3)}
+    base = S.Tuning()
+    lib.get_tuning(base)
+
+    def measure(views, field):
+        best = 1e9
+        for _ in range(repeats):
+            S.render_batch(lib, views)
+            st = S.FrameStats()
+            lib.get_stats(ctypes.byref(st))
+            best = min(best, getattr(st, field))
+        return round(best, 4)
+
+    def tuned(**kw):
+        t = S.Tuning()
+        lib.get_tuning(t)
+        for k, v in kw.items():
+            if k in ("raster_chunk", "raster_split"):
+                getattr(t, k)[:] = v
+            else:
+                setattr(t, k, v)
+        lib.set_tuning(t)
+
+    out = {"views_per_class": {k: len(v) for k, v in classes.items()}, "classes": {}}
+    for k, views in classes.items():
+        row = {"payloads_per_item": {}, "split_threshold": {}}
+        for ch in (16, 32, 64, 96, 128):
+            tuned(raster_chunk=[ch] * 3, raster_split=list(base.raster_split))
+            row["payloads_per_item"][ch] = measure(views, "ms_raster")
+        for sp in (1024, 2048, 4096, 8192, 0xFFFFFFFF):
+            tuned(raster_chunk=list(base.raster_chunk), raster_split=[sp] * 3)
+            row["split_threshold"]["never" if sp == 0xFFFFFFFF else sp] = measure(views, "ms_raster")
+        out["classes"][k] = row
+        lib.set_tuning(base)
+    out["visibility_grid"] = {}
+    for g in (base.raster_groups_resident // 2, base.raster_groups_resident, 2 * base.raster_groups_resident):
+        tuned(raster_groups=g)
+        out["visibility_grid"][g] = measure(mb.views, "ms_raster")
+    lib.set_tuning(base)
+    out["occlusion_stack"] = {}
+    for stack in (256, 512, 1024):
+        tuned(occlusion_stack=stack)
+        out["occlusion_stack"][stack] = measure(mb.views, "ms_shadows")
+    lib.set_tuning(base)
+    mb.close()
+    lib.sft_release()
+    return out
+
+
 def mixed_batch_leg(lib, steps):
     """64 views over 4 scene instances at mixed resolution and MSAA: one
     batched submission per step, the next batch recorded while the previous
@@ -353,6 +417,8 @@ def main():
     ap.add_argument("--scene", choices=sorted(SCENES), default="box_room",
                     help="workload; box_room is the headline 1080p textured interior")
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-view mixed batch (0: skip)")
+    ap.add_argument("--tuning-table", action="store_true",
+                    help="time every autotuner candidate on the 64-view mixed batch and exit")
     ap.add_argument("--crowd-steps", type=int, default=120,
                     help="frames of the crowd-with-shadows 1080p leg (0: skip)")
     ap.add_argument("--served-requests", type=int, default=200,
@@ -371,6 +437,10 @@ def main():
         workload = "{}_{}x{}".format(args.scene, *{0: (640, 360), 1: (1920, 1080), 2: (1920, 1080)}[SCENES[args.scene]])
         print(json.dumps(summarize_pmc(args.pmc_summary[0], args.pmc_summary[1], args.copy_calibration_mb << 20,
                                        workload), indent=1))
+        return
+    if args.tuning_table:
+        import sofren2_amd as S
+        print(json.dumps({"tuning_table": tuning_table(S.library())}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X and has no multi-GPU path "
