@@ -335,6 +335,47 @@ u64 sftx_overflow_reruns(void);
 i32 sftx_model_changed(const sft_model* model);
 u64 sftx_model_edits(void);
 
+/* ---- offline asset builders -------------------------------------------
+   The reference builds meshlets, BLAS and shadow proxies inside its glTF
+   importer with static functions; these entry points expose the same
+   builders for applications that construct models in memory (the QSF2
+   loader reads all of it from the file).  Host code, run once per asset.
+   Results equal the reference's builders bit for bit; the bounding spheres
+   draw from the C library's rand() as the reference's do.
+
+   sftx_mesh_build replaces sft__mesh_build (sofren2.c:2526-2813): meshlets of
+   a triangle soup (positions and indices only; meshlet_attribs, skins,
+   material and lightmap page are left to the caller).  Arena memory for one
+   meshlet per triangle, as the reference allocates.
+
+   sftx_mesh_build_attribs replaces the glTF importer's per-mesh meshlet
+   build (sofren2.c:7420-7660 with sft__builder_finalize_attribs,
+   6916-7045): normals and UVs are required, skin data (joints and weights,
+   all eight arrays) optional, bone_count bounds the culling-bone search.
+   Returns the meshlet count, 0 on invalid input (message on stderr).
+
+   sftx_model_link fills the model-relative meshlet offsets and the
+   meshlet -> mesh table (sofren2.c:7668-7681) after the meshes are built.
+
+   sftx_blas_build replaces sft__blas_build_model (sofren2.c:3526-3617);
+   sftx_shadow_proxies_build replaces sft__generate_shadow_proxies
+   (sofren2.c:3511-3524: bone clusters for a skinned model, BVH subtrees
+   otherwise; nothing without a BLAS or a positive budget). */
+typedef struct sftx_raw_geometry {
+    const f32 *vx, *vy, *vz;
+    const f32 *nx, *ny, *nz;
+    const f32 *u, *v;
+    const u16 *j0, *j1, *j2, *j3;   /* optional skinning data */
+    const u8 *w0, *w1, *w2, *w3;
+} sftx_raw_geometry;
+void sftx_mesh_build(sft_mesh* out_mesh, sft_arena* arena, const f32* vx, const f32* vy, const f32* vz,
+                     const i32* inds, i32 tri_count);
+u32 sftx_mesh_build_attribs(sft_mesh* out_mesh, sft_arena* arena, const sftx_raw_geometry* raw, u32 vertex_count,
+                            const i32* inds, i32 tri_count, u32 bone_count);
+void sftx_model_link(sft_model* model, sft_arena* arena);
+void sftx_blas_build(sft_model* model, sft_arena* arena);
+void sftx_shadow_proxies_build(sft_model* model, sft_arena* arena, i32 budget);
+
 /* Copies `bytes` between two device buffers with a plain streaming kernel
    and returns its duration in ms; a calibration point for HBM counters. */
 f32 sftx_copy_calibration(u64 bytes);

@@ -21,13 +21,16 @@
  *   sofren2_ref bench <scene> <seed> <w> <h> <threads> <warmup> <frames>
  *   sofren2_ref pose  <seed> <times.bin> <out.bin>
  *   sofren2_ref ray   <scene> <seed> <rays> <out.bin>
+ *   sofren2_ref build <seed> <kind> <out.bin>
  * dump writes color.bin (u32 ARGB, h rows of w + pad/4 words), vid.bin /
  * depth.bin (per sample over the tile grid), ss.bin, ssvis.bin, clip.bin and
  * meta.txt into outdir; with prefill != 0 the colour buffer (padding
  * included) starts as sfth_prefill's pattern instead of zeros.  bench prints
  * one JSON line with median / p99 frame time.  pose and ray run the shared
  * host-query checks of the scene generator (sftx_pose_check,
- * sftx_raycast_check) and write their floats.
+ * sftx_raycast_check) and write their floats; build runs the generator's
+ * builder check (sftx_builder_check) on the reference's own static builders,
+ * which this file exposes under the names the check calls.
  */
 #define _GNU_SOURCE
 #include <time.h>
@@ -150,6 +153,17 @@ static void sfth_present(void) {
 #include "scenes.c"
 #undef sft_present
 
+/* The builder check calls the additive builder names of the GPU library;
+   here they are the reference's static builders, called unchanged. */
+void sftx_mesh_build(sft_mesh* out_mesh, sft_arena* arena, const f32* vx, const f32* vy, const f32* vz,
+                     const i32* inds, i32 tri_count) {
+    sft__mesh_build(out_mesh, arena, vx, vy, vz, inds, tri_count);
+}
+void sftx_blas_build(sft_model* model, sft_arena* arena) { sft__blas_build_model(model, arena); }
+void sftx_shadow_proxies_build(sft_model* model, sft_arena* arena, i32 budget) {
+    sft__generate_shadow_proxies(model, arena, budget);
+}
+
 /* writes dir/name, or the path `dir` itself when name is empty */
 static void sfth_write(const char* dir, const char* name, const void* data, u64 bytes) {
     char path[1024];
@@ -208,6 +222,15 @@ int main(int argc, char** argv) {
         f32* out = (f32*)malloc((u64)(n + 1) * 2 * 12 * 16 * sizeof(f32));
         const i32 got = sftx_pose_check((u32)strtoul(argv[2], NULL, 10), n, times, out);
         sfth_write(argv[4], "", out, (u64)got * 4);
+        return 0;
+    }
+    if (argc >= 5 && 0 == strcmp(argv[1], "build")) {
+        const u32 seed = (u32)strtoul(argv[2], NULL, 10);
+        const i32 kind = atoi(argv[3]);
+        const i64 need = sftx_builder_check(seed, kind, NULL, 0);
+        u8* out = (u8*)malloc((u64)need + 1);
+        const i64 got = sftx_builder_check(seed, kind, out, (u64)need);
+        sfth_write(argv[4], "", out, (u64)got);
         return 0;
     }
     if (argc >= 6 && 0 == strcmp(argv[1], "ray")) {

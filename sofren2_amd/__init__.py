@@ -48,6 +48,45 @@ class FrameStats(ctypes.Structure):
         "capsule_tests")]
 
 
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class Arena(ctypes.Structure):
+    """sft_arena (40 bytes): reserve, commit and offset of an arena."""
+    _fields_ = [("base", ctypes.c_void_p), ("max_size", ctypes.c_uint64), ("committed_size", ctypes.c_uint64),
+                ("offset", ctypes.c_uint64), ("commit_lock", ctypes.c_int32)]
+
+
+class Mesh(ctypes.Structure):
+    """sft_mesh with the reference layout (meshlet arrays are raw pointers)."""
+    _fields_ = [("meshlets", ctypes.c_void_p), ("meshlet_attribs", ctypes.c_void_p), ("meshlet_skins", ctypes.c_void_p),
+                ("meshlet_count", ctypes.c_uint32), ("model_meshlet_offset", ctypes.c_uint32),
+                ("inds", ctypes.c_void_p)] + [(n, _fp) for n in (
+                    "bounds_center_x", "bounds_center_y", "bounds_center_z", "bounds_radius", "bounds_min_x",
+                    "bounds_max_x", "bounds_min_y", "bounds_max_y", "bounds_min_z", "bounds_max_z", "cone_axis_x",
+                    "cone_axis_y", "cone_axis_z", "cone_cutoff")] + [
+                ("culling_bone_inds", ctypes.POINTER(ctypes.c_int32)), ("material", ctypes.c_uint32),
+                ("lightmap_page", ctypes.c_uint32)]
+
+
+class Model(ctypes.Structure):
+    """sft_model with the reference layout."""
+    _fields_ = [("meshes", ctypes.POINTER(Mesh)), ("mesh_count", ctypes.c_uint32), ("total_meshlets", ctypes.c_uint32),
+                ("meshlet_mesh_inds", ctypes.POINTER(ctypes.c_uint16)), ("blas_nodes", ctypes.c_void_p),
+                ("blas_node_count", ctypes.c_uint32), ("blas_root_ind", ctypes.c_uint32),
+                ("lightmaps", ctypes.c_void_p), ("lightmap_count", ctypes.c_uint32),
+                ("shadow_proxies", ctypes.c_void_p), ("shadow_proxy_count", ctypes.c_uint32),
+                ("bones", ctypes.c_void_p), ("bone_count", ctypes.c_int32),
+                ("animations", ctypes.c_void_p), ("anim_count", ctypes.c_int32)]
+
+
+class RawGeometry(ctypes.Structure):
+    """sftx_raw_geometry: per-vertex arrays of a mesh for sftx_mesh_build_attribs."""
+    _fields_ = [(n, _fp) for n in ("vx", "vy", "vz", "nx", "ny", "nz", "u", "v")] + [
+        (n, ctypes.POINTER(ctypes.c_uint16)) for n in ("j0", "j1", "j2", "j3")] + [
+        (n, ctypes.POINTER(ctypes.c_uint8)) for n in ("w0", "w1", "w2", "w3")]
+
+
 class Tuning(ctypes.Structure):
     """sftx_tuning: the autotuner's settings and timings."""
     _fields_ = [("raster_chunk", ctypes.c_uint32 * 3), ("raster_split", ctypes.c_uint32 * 3),
@@ -166,6 +205,13 @@ class Library:
         self.overflow_reruns = _bind(g, "sftx_overflow_reruns", ctypes.c_uint64, [])
         self.set_clear = _bind(g, "sft_set_clear", None, [u8])
         self.raycast_check = _bind(s, "sftx_raycast_check", i32, [u32, i32, vp])
+        self.builder_check = _bind(s, "sftx_builder_check", ctypes.c_int64, [u32, i32, vp, ctypes.c_uint64])
+        self.mesh_build = _bind(g, "sftx_mesh_build", None, [vp, vp, vp, vp, vp, vp, i32])
+        self.mesh_build_attribs = _bind(g, "sftx_mesh_build_attribs", u32, [vp, vp, vp, u32, vp, i32, u32])
+        self.model_link = _bind(g, "sftx_model_link", None, [vp, vp])
+        self.blas_build = _bind(g, "sftx_blas_build", None, [vp, vp])
+        self.shadow_proxies_build = _bind(g, "sftx_shadow_proxies_build", None, [vp, vp, i32])
+        self.arena_init = _bind(g, "sft_arena_init", None, [vp, ctypes.c_uint64])
         self.scene_edit_room = _bind(s, "sftx_scene_edit_room", i32, [i32, ctypes.c_float])
         self.scene_room_model = _bind(s, "sftx_scene_room_model", vp, [])
         self.model_changed = _bind(g, "sftx_model_changed", i32, [vp])

@@ -39,7 +39,7 @@ def _run(cmd):
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    for src in ("frontend.cpp", "host_queries.cpp", "rcp_probe.cpp", "tlas.cpp", "server.cpp"):
+    for src in ("frontend.cpp", "host_queries.cpp", "builders.cpp", "rcp_probe.cpp", "tlas.cpp", "server.cpp"):
         o = os.path.join(OBJ, src + ".o")
         _run(["g++", "-std=c++17", "-c"] + HOST_FLAGS + [os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -223,6 +223,13 @@ void camera_setup(const sftmat& view, const sftmat& proj, sftmat* view_proj, f32
 void snapshot_view(const Context& c, ViewRecord& v);
 
 /* frame TLAS over the drawn instances (tlas.cpp); returns the root node */
+/* a BVH build item: payload id, box, box centre (sofren2.c:961-965) */
+struct BvhItem {
+    u32 id;
+    f32 lo[3], hi[3];
+    f32 mid[3];
+};
+u32 build_bvh(BvhItem* items, u32 n, std::vector<sft_bvh_node>& nodes);
 u32 build_tlas(const std::vector<sft_instance>& instances, std::vector<sft_bvh_node>& nodes);
 u32 tlas_node_count(u32 items);     /* nodes k_tlas writes for `items` items (at most 64) */
 

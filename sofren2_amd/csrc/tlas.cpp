@@ -1,7 +1,9 @@
 /*
- * tlas.cpp — the frame's top-level BVH over the drawn instances, built on the
+ * tlas.cpp — the 8-wide BVH builder of the reference (sofren2.c:2895-3062)
+ * and the frame's top-level BVH over the drawn instances, built on the
  * host at present time (sofren2.c:3619-3703) and uploaded with the frame's
- * tables.  The shadow-ray traversal on the GPU culls TLAS nodes against the
+ * tables when the device build is switched off.  The same builder makes a
+ * model's triangle BLAS (builders.cpp).  The shadow-ray traversal on the GPU culls TLAS nodes against the
  * packet frustum, so the node grouping is kept identical to the reference
  * builder (median splits into up to 8 groups, sofren2.c:2895-3062): a different
  * grouping would change the bounds a packet is tested against.
@@ -13,11 +15,7 @@ namespace sft {
 
 namespace {
 
-struct Item {
-    u32 id;
-    f32 lo[3], hi[3];
-    f32 mid[3];
-};
+using Item = BvhItem;
 
 /* Hoare partition around the middle element's key, recursing into both sides
    (sofren2.c:2895-2924) */
@@ -120,6 +118,13 @@ u32 build_node(Item* it, u32 n, std::vector<sft_bvh_node>& nodes, f32* bounds) {
 }
 
 }  // namespace
+
+/* The 8-wide BVH over items[0, n) (reordered in place), appended to `nodes`;
+   returns the root's index.  Shared by the frame TLAS and the model BLAS
+   builder (sofren2.c:2967-3062). */
+u32 build_bvh(BvhItem* items, u32 n, std::vector<sft_bvh_node>& nodes) {
+    return build_node(items, n, nodes, nullptr);
+}
 
 /* nodes of the TLAS over `items` items: one leaf root up to 8 items, else the
    root and one leaf per group of the three halving levels (every group holds

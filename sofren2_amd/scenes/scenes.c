@@ -5,6 +5,7 @@
  * compiles against any implementation of the API.  It must be compiled with
  * -ffp-contract=off so the host-side float math is identical for every build.
  */
+#include <stddef.h>
 #include <unistd.h>
 #include <sys/mman.h>
 #include "scenes.h"
@@ -1361,6 +1362,214 @@ i32 sftx_raycast_check(u32 seed, i32 n_rays, f32* out) {
         r[5] = h.normal.x; r[6] = h.normal.y; r[7] = h.normal.z;
     }
     return n_rays;
+}
+
+/* ---------------- shared check of the offline asset builders -------------
+   The builders are additive entry points of the GPU library; the reference
+   build provides the same names as thin calls of its own static builders
+   (sofren2.c:2526, 3526, 3511), so these prototypes serve both. */
+void sftx_mesh_build(sft_mesh* out_mesh, sft_arena* arena, const f32* vx, const f32* vy, const f32* vz,
+                     const i32* inds, i32 tri_count);
+void sftx_blas_build(sft_model* model, sft_arena* arena);
+void sftx_shadow_proxies_build(sft_model* model, sft_arena* arena, i32 budget);
+
+typedef struct {
+    f32* x; f32* y; f32* z;
+    i32* inds;
+    i32 nv, nt, cap_v, cap_t;
+} sftx_soup;
+
+static i32 sftx_soup_vert(sftx_soup* s, f32 x, f32 y, f32 z) {
+    if (s->nv == s->cap_v) return 0;
+    s->x[s->nv] = x; s->y[s->nv] = y; s->z[s->nv] = z;
+    return s->nv++;
+}
+
+static void sftx_soup_tri(sftx_soup* s, i32 a, i32 b, i32 c) {
+    if (s->nt == s->cap_t) return;
+    s->inds[s->nt * 3] = a; s->inds[s->nt * 3 + 1] = b; s->inds[s->nt * 3 + 2] = c;
+    s->nt += 1;
+}
+
+/* Triangle soups that reach the builders' branches:
+   0: a jittered closed sphere (32 x 48), shared vertices, curved normals;
+   1: 300 boxes of log-uniform size under random rotations about y, eight
+      shared corners each (disconnected parts: the candidate list runs dry);
+   2: a 60 x 60 height field with spikes (the 100-degree cone limit rejects);
+   3: random triangles over a pool of 900 vertices, with repeated corners and
+      collinear points (degenerate normals and sphere fall-backs). */
+static void sftx_make_soup(sftx_soup* s, i32 kind, sftx_rng* r) {
+    if (0 == kind) {
+        const i32 rings = 32, segs = 48;
+        for (i32 i = 0; i <= rings; i += 1) {
+            for (i32 j = 0; j < segs; j += 1) {
+                const f32 th = (f32)SFT_PI * (f32)i / (f32)rings, ph = 2.f * (f32)SFT_PI * (f32)j / (f32)segs;
+                const f32 rr = 2.f + sftx_uniform(r, -0.05f, 0.05f);
+                sftx_soup_vert(s, rr * sinf(th) * cosf(ph), rr * cosf(th), rr * sinf(th) * sinf(ph));
+            }
+        }
+        for (i32 i = 0; i < rings; i += 1) {
+            for (i32 j = 0; j < segs; j += 1) {
+                const i32 a = i * segs + j, b = i * segs + (j + 1) % segs, c = a + segs, d = b + segs;
+                sftx_soup_tri(s, a, c, d);
+                sftx_soup_tri(s, a, d, b);
+            }
+        }
+    }
+    else if (1 == kind) {
+        static const u8 faces[12][3] = { {0,2,3},{0,3,1},{4,5,7},{4,7,6},{0,1,5},{0,5,4},
+                                         {2,6,7},{2,7,3},{0,4,6},{0,6,2},{1,3,7},{1,7,5} };
+        for (i32 b = 0; b < 300; b += 1) {
+            const f32 cx = sftx_uniform(r, -20.f, 20.f), cy = sftx_uniform(r, 0.f, 8.f), cz = sftx_uniform(r, -20.f, 20.f);
+            const f32 hx = 0.05f * expf(sftx_uniform(r, 0.f, 3.7f)), hy = 0.05f * expf(sftx_uniform(r, 0.f, 3.7f));
+            const f32 hz = 0.05f * expf(sftx_uniform(r, 0.f, 3.7f)), ang = sftx_uniform(r, 0.f, 6.28f);
+            const f32 ca = cosf(ang), sa = sinf(ang);
+            i32 base = 0;
+            for (i32 k = 0; k < 8; k += 1) {
+                const f32 lx = (k & 1) ? hx : -hx, ly = (k & 2) ? hy : -hy, lz = (k & 4) ? hz : -hz;
+                const i32 v = sftx_soup_vert(s, cx + lx * ca - lz * sa, cy + ly, cz + lx * sa + lz * ca);
+                if (0 == k) base = v;
+            }
+            for (i32 f = 0; f < 12; f += 1) sftx_soup_tri(s, base + faces[f][0], base + faces[f][1], base + faces[f][2]);
+        }
+    }
+    else if (2 == kind) {
+        const i32 n = 60;
+        for (i32 i = 0; i <= n; i += 1) {
+            for (i32 j = 0; j <= n; j += 1) {
+                f32 h = 0.3f * sinf(0.4f * (f32)i) * cosf(0.3f * (f32)j) + sftx_uniform(r, -0.02f, 0.02f);
+                if (0 == sftx_rng_next(r) % 23) h += sftx_uniform(r, -3.f, 3.f);
+                sftx_soup_vert(s, (f32)j * 0.5f, h, (f32)i * 0.5f);
+            }
+        }
+        for (i32 i = 0; i < n; i += 1) {
+            for (i32 j = 0; j < n; j += 1) {
+                const i32 a = i * (n + 1) + j, b = a + 1, c = a + n + 1, d = c + 1;
+                sftx_soup_tri(s, a, c, d);
+                sftx_soup_tri(s, a, d, b);
+            }
+        }
+    }
+    else {
+        for (i32 v = 0; v < 900; v += 1) {
+            if (v >= 3 && 0 == v % 17) {
+                /* on the line through the previous two */
+                const f32 t = sftx_uniform(r, -1.f, 2.f);
+                sftx_soup_vert(s, s->x[v - 2] + t * (s->x[v - 1] - s->x[v - 2]), s->y[v - 2] + t * (s->y[v - 1] - s->y[v - 2]),
+                               s->z[v - 2] + t * (s->z[v - 1] - s->z[v - 2]));
+            }
+            else {
+                sftx_soup_vert(s, sftx_uniform(r, -5.f, 5.f), sftx_uniform(r, -5.f, 5.f), sftx_uniform(r, -5.f, 5.f));
+            }
+        }
+        for (i32 t = 0; t < 2000; t += 1) {
+            const i32 a = (i32)(sftx_rng_next(r) % 900u);
+            i32 b = (i32)(sftx_rng_next(r) % 900u), c = (i32)(sftx_rng_next(r) % 900u);
+            if (0 == t % 29) b = a;                       /* repeated corner */
+            if (0 == t % 31) { b = a; c = a; }            /* a point */
+            if (0 == t % 13 && a >= 2) { b = a - 1; c = a - 2; }
+            sftx_soup_tri(s, a, b, c);
+        }
+    }
+}
+
+static u64 sftx_put(u8* out, u64 at, u64 cap, const void* p, u64 n) {
+    if (at + n <= cap) memcpy(out + at, p, n);
+    return at + n;
+}
+
+/* every BLAS node (fields only: the bytes after is_leaf_group are zero) and
+   every proxy of the model */
+static u64 sftx_put_blas_proxies(const sft_model* m, u8* out, u64 at, u64 cap) {
+    const u32 head[3] = { m->blas_node_count, m->blas_root_ind, m->shadow_proxy_count };
+    at = sftx_put(out, at, cap, head, sizeof(head));
+    for (u32 i = 0; i < m->blas_node_count; i += 1) {
+        u8 node[sizeof(struct sft_bvh_node)];
+        memset(node, 0, sizeof(node));
+        memcpy(node, &m->blas_nodes[i], offsetof(struct sft_bvh_node, is_leaf_group) + 1);
+        at = sftx_put(out, at, cap, node, sizeof(node));
+    }
+    return sftx_put(out, at, cap, m->shadow_proxies, (u64)m->shadow_proxy_count * sizeof(struct sft_shadow_proxy));
+}
+
+/* Runs the builders on a seeded input and writes what they produced.
+   Kinds 0-3: the soups above through sftx_mesh_build (rand() seeded with
+   `seed` first), the one-mesh model's BLAS and BVH-subtree proxies with
+   budgets 1, 8 and 40.  Kind 4: the crowd humanoid's BLAS and bone-cluster
+   proxies with budgets 10, 4 and 1.  Kind 5: the same mesh under a seeded
+   skeleton with two roots and scattered bind positions (merges into a
+   parent's cluster and into the largest one).  Returns the bytes the output
+   needs (nothing past `cap` is written). */
+i64 sftx_builder_check(u32 seed, i32 kind, u8* out, u64 cap) {
+    sftx_rng rng;
+    sftx_rng_seed(&rng, seed, 0x61C88647ULL + (u64)kind);
+    sft_arena ar;
+    sft_arena_init(&ar, 1ULL << 30);
+    sft_model* m = NULL;
+    u64 at = 0;
+    if (kind >= 0 && kind <= 3) {
+        sftx_soup s;
+        s.cap_v = 20000; s.cap_t = 20000; s.nv = s.nt = 0;
+        s.x = (f32*)malloc((u64)s.cap_v * 3 * sizeof(f32));
+        s.y = s.x + s.cap_v; s.z = s.y + s.cap_v;
+        s.inds = (i32*)malloc((u64)s.cap_t * 3 * sizeof(i32));
+        sftx_make_soup(&s, kind, &rng);
+        m = (sft_model*)sft_arena_alloc(&ar, sizeof(sft_model));
+        memset(m, 0, sizeof(*m));
+        sft_mesh* mesh = (sft_mesh*)sft_arena_alloc(&ar, sizeof(sft_mesh));
+        memset(mesh, 0, sizeof(*mesh));
+        srand(seed);
+        sftx_mesh_build(mesh, &ar, s.x, s.y, s.z, s.inds, s.nt);
+        free(s.x);
+        free(s.inds);
+        const u32 mc = mesh->meshlet_count;
+        const u32 head[2] = { (u32)s.nt, mc };
+        at = sftx_put(out, at, cap, head, sizeof(head));
+        for (u32 i = 0; i < mc; i += 1) {
+            at = sftx_put(out, at, cap, &mesh->meshlets[i], sizeof(struct sft_meshlet));
+            at = sftx_put(out, at, cap, &mesh->inds[i], sizeof(struct sft_meshlet_inds));
+            const f32 b[14] = { mesh->bounds_center_x[i], mesh->bounds_center_y[i], mesh->bounds_center_z[i],
+                                mesh->bounds_radius[i], mesh->bounds_min_x[i], mesh->bounds_min_y[i],
+                                mesh->bounds_min_z[i], mesh->bounds_max_x[i], mesh->bounds_max_y[i],
+                                mesh->bounds_max_z[i], mesh->cone_axis_x[i], mesh->cone_axis_y[i],
+                                mesh->cone_axis_z[i], mesh->cone_cutoff[i] };
+            at = sftx_put(out, at, cap, b, sizeof(b));
+        }
+        m->meshes = mesh;
+        m->mesh_count = 1;
+        m->total_meshlets = mc;
+        mesh->model_meshlet_offset = 0;
+        m->meshlet_mesh_inds = (u16*)sft_arena_alloc(&ar, ((u64)mc + 1) * sizeof(u16));
+        memset(m->meshlet_mesh_inds, 0, ((u64)mc + 1) * sizeof(u16));
+        sftx_blas_build(m, &ar);
+        static const i32 budgets[3] = { 1, 8, 40 };
+        for (i32 b = 0; b < 3; b += 1) {
+            sftx_shadow_proxies_build(m, &ar, budgets[b]);
+            at = sftx_put_blas_proxies(m, out, at, cap);
+        }
+    }
+    else if (4 == kind || 5 == kind) {
+        m = sftx_build_humanoid(&ar, &rng);
+        if (5 == kind) {
+            for (i32 j = 0; j < m->skeleton.bone_count; j += 1) {
+                struct sft_bone* bn = &m->skeleton.bones[j];
+                bn->parent_ind = (0 == j || 4 == j) ? -1 : (i32)(sftx_rng_next(&rng) % (u32)j);
+                bn->bind_pos = SFT_V(sftx_uniform(&rng, -0.3f, 0.3f), SFTX_BONE_LEN * (f32)j + sftx_uniform(&rng, -0.1f, 0.1f),
+                                     sftx_uniform(&rng, -0.3f, 0.3f));
+            }
+        }
+        sftx_blas_build(m, &ar);
+        static const i32 budgets[3] = { 10, 4, 1 };
+        for (i32 b = 0; b < 3; b += 1) {
+            sftx_shadow_proxies_build(m, &ar, budgets[b]);
+            at = sftx_put_blas_proxies(m, out, at, cap);
+        }
+    }
+    else {
+        at = 0;
+    }
+    munmap(ar.base, ar.max_size);
+    return (i64)at;
 }
 
 /* ---------------- dispatch ---------------- */

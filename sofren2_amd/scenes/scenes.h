@@ -98,6 +98,12 @@ void sftx_scene_destroy(void);
 i32 sftx_pose_check(u32 seed, i32 n_times, const f32* times, f32* out);
 i32 sftx_raycast_check(u32 seed, i32 n_rays, f32* out);
 
+/* Offline builder check, byte-comparable between builds of the API: builds
+   meshlets (kinds 0-3: seeded triangle soups), the BLAS and shadow proxies
+   (kinds 4-5: the crowd humanoid under two skeletons) and writes every
+   output byte into out[0, cap).  Returns the bytes the output needs. */
+i64 sftx_builder_check(u32 seed, i32 kind, u8* out, u64 cap);
+
 /* Edits the current scene's room model in place (box room, crowd and served
    kinds), as an application editing a loaded model would: moves meshlets up
    by dy (vertex positions and bounds).  Mode 1 moves every meshlet, mode 2

@@ -91,3 +91,11 @@ def ref_rays(scene, seed, n):
         op = os.path.join(d, "o.bin")
         subprocess.run([ref_path(), "ray", str(scene), str(seed), str(n), op], check=True, timeout=300)
         return np.fromfile(op, dtype=np.float32).reshape(-1, 8)
+
+
+def ref_build(seed, kind):
+    """sftx_builder_check of the reference build: the raw output bytes."""
+    with tempfile.TemporaryDirectory() as d:
+        op = os.path.join(d, "o.bin")
+        subprocess.run([ref_path(), "build", str(seed), str(kind), op], check=True, timeout=300)
+        return np.fromfile(op, dtype=np.uint8)
