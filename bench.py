@@ -299,7 +299,7 @@ def crowd_leg(lib, steps, warmup):
             "capsule_mtests_s_in_shade_pass": round(caps_f / (float(np.mean(shade_ms)) * 1e3), 1)}
 
 
-def served_accuracy(sample, specs, max_checks=48, workers=8):
+def served_accuracy(sample, specs, id_splits, max_checks=48, workers=8):
     """Served frames of the 1% sample against the reference renderer
     (oracle/_ref, one worker thread, run on this host) rendering the same
     template pose and request camera: visibility IDs and depth bit-exact,
@@ -311,13 +311,14 @@ def served_accuracy(sample, specs, max_checks=48, workers=8):
     from tests import parity
     from tests.refrun import have_ref, ref_dump
     picks = sample[:max_checks]
-    if not picks or not all(have_ref(ns) for ns in (1, 2, 4)):
+    if not picks or not all(have_ref(ns, bits) for ns in (1, 2, 4) for bits in set(id_splits)):
         return None
 
     def check(item):
         (_, si, cf, w, h, ns), (color, vis) = item
         kind, seed, pose = specs[si]
-        ref = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns)
+        ref = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns,
+                       id_bits=id_splits[si])
         got = {"color": color, "vid": (vis & 0xFFFFFFFF).astype(np.uint32),
                "depth": (vis >> np.uint64(32)).astype(np.uint32).view(np.float32)}
         rep = parity.compare_frame(got, ref)
@@ -402,6 +403,8 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
         out["accuracy_sample"] = {"checked": checked, "equal_to_per_frame": same}
         out["_sample"] = sample
         out["_specs"] = specs
+        out["_id_splits"] = list(srv.id_splits)
+        out["id_split_7_templates"] = sum(1 for b in srv.id_splits if b == 7)
     finally:
         srv.close()
     return out
@@ -577,10 +580,11 @@ def main():
     lib.sft_release()
     served = out.get("served") or {}
     sample, specs = served.pop("_sample", []), served.pop("_specs", [])
+    id_splits = served.pop("_id_splits", [])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.scene, w, h, args.cpu_frames)
         if sample:
-            served["accuracy_vs_reference"] = served_accuracy(sample, specs)
+            served["accuracy_vs_reference"] = served_accuracy(sample, specs, id_splits)
     print(json.dumps(out), flush=True)
 
 

@@ -37,7 +37,8 @@ void sftx_set_capture(u32 flags);
    including pixels past the screen edge inside border tiles. */
 void sftx_cap_grid(i32* cols, i32* rows, i32* samples);
 
-/* Per-sample reference 32-bit visibility IDs (inst<<25 | meshlet<<7 | tri, or
+/* Per-sample reference 32-bit visibility IDs (inst<<25 | meshlet<<7 | tri under
+   the default split, inst<<24 under 7/17 (sftx_set_id_split), or
    clip_slot | 1<<31, 0xFFFFFFFF = empty) and depth (interpolated 1/w, 0 =
    empty), laid out [sample][cols*64 * rows*64] row-major.  Either pointer may
    be NULL. */
@@ -97,7 +98,7 @@ typedef struct sftx_frame_stats {
     u32 dirlit_shadings;      /* shadings that evaluated directional lights */
     u32 light_cut_tiles;      /* tiles whose light list stopped at 16 with more lights in reach */
     u32 bg_blend_pixels;      /* MSAA pixels without clearing that blend the caller's colour */
-    u32 instances_dropped;    /* draws beyond the 64 instances of the visibility-ID split */
+    u32 instances_dropped;    /* draws beyond the instances of the visibility-ID split (64 by default) */
     u32 capsule_tests;        /* capsule ray tests: per shaded pixel and light reaching it, the light's cached capsules */
 } sftx_frame_stats;
 
@@ -121,6 +122,18 @@ const u32* sftx_device_color(void);     /* device pointer, width*height ARGB of 
    fixes this at compile time with SFT_USE_AA / SFT_AA_SAMPLES.  Other values
    are rejected with a message and leave the setting unchanged. */
 void sftx_set_msaa(i32 samples);
+
+/* Visibility-ID split of the following frames: instance_bits bits of
+   instance index and 24 - instance_bits of model meshlet index above the 7
+   triangle bits, so 1 << instance_bits instances per frame; draws beyond
+   that are dropped.  Replaces the reference's compile-time SFT_INSTANCE_BITS /
+   SFT_MESHLET_BITS (sofren2.c:154-165; default 6 / 18): an application built
+   with -DSFT_INSTANCE_BITS=7 -DSFT_MESHLET_BITS=17 calls
+   sftx_set_id_split(7) after sft_init.  Set it between frames; views of a
+   batch and server templates keep the split in effect when they were
+   recorded.  Values outside 1..16 are rejected with a message. */
+void sftx_set_id_split(i32 instance_bits);
+i32 sftx_get_id_split(void);
 
 /* ---- batched views ---------------------------------------------------------
    Records several views with the ordinary calls and renders them with one

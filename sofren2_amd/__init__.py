@@ -161,6 +161,8 @@ class Library:
         self.scene_record = _bind(s, "sftx_scene_record", None, [i32, vp, i32])
         self.set_output = _bind(g, "sftx_set_output", None, [i32])
         self.set_msaa = _bind(g, "sftx_set_msaa", None, [i32])
+        self.set_id_split = _bind(g, "sftx_set_id_split", None, [i32])
+        self.get_id_split = _bind(g, "sftx_get_id_split", i32, [])
         self.set_graphs = _bind(g, "sftx_set_graphs", None, [i32])
         self.set_stream_download = _bind(g, "sftx_set_stream_download", None, [i32])
         self._stream_stats = _bind(g, "sftx_stream_stats", None, [vp, vp])
@@ -175,6 +177,8 @@ class Library:
         self.scene_use = _bind(s, "sftx_scene_use", i32, [i32])
         self.scene_draw = _bind(s, "sftx_scene_draw", None, [i32, i32, i32, i32])
         self.scene_free = _bind(s, "sftx_scene_free", None, [i32])
+        self.scene_characters = _bind(s, "sftx_scene_characters", i32, [i32])
+        self.scene_shadow_lights = _bind(s, "sftx_scene_shadow_lights", i32, [i32])
         fp = ctypes.POINTER(ctypes.c_float)
         self.scene_camera = _bind(s, "sftx_scene_camera", None, [i32, i32, i32, i32, fp, fp, fp])
         self.server_template = _bind(g, "sftx_server_template", i32, [])
@@ -304,15 +308,16 @@ def submit_batch_async(lib, views):
 class SceneRenderer:
     """Renders a seeded workload through the public API on the GPU."""
 
-    def __init__(self, scene, seed=0, width=None, height=None, capture=0, samples=1):
+    def __init__(self, scene, seed=0, width=None, height=None, capture=0, samples=1, id_bits=6):
         self.lib = library()
-        w, h = SCENE_SIZE[scene]
+        w, h = SCENE_SIZE.get(scene, (640, 360))
         self.width = width or w
         self.height = height or h
         self.samples = samples
         self.lib.sft_init(self.width, self.height, 0)
         self.lib.set_capture(capture)
         self.lib.set_msaa(samples)
+        self.lib.set_id_split(id_bits)
         if self.lib.scene_create(scene, seed, self.width, self.height) != 0:
             raise ValueError(f"unknown scene {scene}")
         self.color = np.zeros((self.height, self.width), dtype=np.uint32)

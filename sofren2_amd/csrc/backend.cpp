@@ -1071,6 +1071,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.band_base = kInvalid;
         g.raster_split = std::max(1u, b->raster_split[res_class(v.width, v.height)]);
         g.tlas_build = tlas_dev ? 1u : 0u;
+        g.id_shift = (u32)SFT_TRIANGLE_BITS + 24u - v.instance_bits;
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
             vis_stage_total += vp.grid;

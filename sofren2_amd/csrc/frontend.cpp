@@ -392,6 +392,7 @@ void sft::snapshot_view(const Context& c, ViewRecord& v) {
     v.proxies = c.proxies;
     v.instances = c.instances;
     v.dropped = (u32)std::max<i32>(0, c.instance_count - (i32)c.instances.size());
+    v.instance_bits = c.instance_bits;
     v.total_ss = c.total_ss;
     v.transient_models = c.transient_models;
     v.tmat_tex = c.tmat_tex;
@@ -678,6 +679,17 @@ extern "C" void sftx_set_msaa(i32 samples) {
     }
     C().samples = samples;
 }
+/* the reference's SFT_INSTANCE_BITS / SFT_MESHLET_BITS (sofren2.c:154-165),
+   chosen at run time: instance_bits + meshlet bits = 24 */
+extern "C" void sftx_set_id_split(i32 instance_bits) {
+    if (instance_bits < 1 || instance_bits > 16) {
+        fprintf(stderr, "sofren2: %d instance bits not supported (1..16, meshlet bits = 24 - instance bits)\n",
+                instance_bits);
+        return;
+    }
+    C().instance_bits = (u32)instance_bits;
+}
+extern "C" i32 sftx_get_id_split(void) { return (i32)C().instance_bits; }
 extern "C" void sft_set_clear_color(u32 color) { C().clear_color = color; }
 extern "C" void sft_set_color_buffer(u32* buf, i32 pitch_in_bytes) {
     Context& c = C();
@@ -719,13 +731,14 @@ extern "C" void sft_set_shadows_enabled(u8 enabled) { C().shadows_enabled = enab
 extern "C" void sft_set_lightmap_brightness(f32 b) { C().lightmap_brightness = b; }
 
 /* ================================================================ instances
-   (sofren2.c:8410-8478): slot claim with the 64-instance cap, screen-space
-   cache offset, world-space shadow proxies.  Geometry jobs are implicit: the
-   backend walks instances, meshes and meshlets in this same order. */
+   (sofren2.c:8410-8478): slot claim with the instance cap of the ID split
+   (64 by default), screen-space cache offset, world-space shadow proxies.
+   Geometry jobs are implicit: the backend walks instances, meshes and
+   meshlets in this same order. */
 extern "C" void sft_draw_instance(const sft_instance* inst) {
     Context& c = C();
     const i32 slot = c.instance_count++;
-    if (slot >= SFT_MAX_INSTANCES_PER_FRAME) {
+    if (slot >= (i32)(1u << c.instance_bits)) {
         return;
     }
     sft_instance copy = *inst;

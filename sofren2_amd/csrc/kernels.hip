@@ -907,12 +907,12 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
 /* resolves a winning emission key to the reference's 32-bit visibility ID */
 /* emission key -> the view's reference visibility ID; `clip_base` is the
    view's first batch-wide clip slot */
-DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip, u32 clip_base) {
+DEV u32 key_to_vid(const FramePtrs& P, u32 key, u32 n_clip, u32 clip_base, u32 id_shift) {
     const u32 sub = key & 31u;
     if (sub >= 16u) {
         const WorkItem w = P.work[key >> 9];
         const u32 tri = ((key >> 5) & 15u) * 8u + (sub - 16u);
-        return (w.inst << (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) | (w.gm << SFT_TRIANGLE_BITS) | tri;
+        return (w.inst << id_shift) | (w.gm << SFT_TRIANGLE_BITS) | tri;
     }
     /* clip keys are increasing in slot order */
     u32 lo = 0u, hi = n_clip;
@@ -941,8 +941,8 @@ DEV void load_tri(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
         asm volatile("" ::: "memory");
         return;
     }
-    t.inst = V.inst_base + (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS));
-    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1u);
+    t.inst = V.inst_base + (vid >> V.id_shift);
+    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << (V.id_shift - SFT_TRIANGLE_BITS)) - 1u);
     const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
     const InstGpu* inst = &P.insts[t.inst];
     t.slot = inst->model_base + gm;
@@ -972,9 +972,9 @@ DEV void load_tri_uv(const FramePtrs& P, const ViewGpu& V, u32 vid, Tri& t) {
         asm volatile("" ::: "memory");
         return;
     }
-    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << SFT_MESHLET_BITS) - 1u);
+    const u32 gm = (vid >> SFT_TRIANGLE_BITS) & ((1u << (V.id_shift - SFT_TRIANGLE_BITS)) - 1u);
     const u32 tri = vid & ((1u << SFT_TRIANGLE_BITS) - 1u);
-    const InstGpu* inst = &P.insts[V.inst_base + (vid >> (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS))];
+    const InstGpu* inst = &P.insts[V.inst_base + (vid >> V.id_shift)];
     const u32 slot = inst->model_base + gm;
     const u32 ib = slot * 512u + (tri >> 3) * 32u + (tri & 7u);
     const u32 ss = (inst->ss_offset + gm) * 576u;
@@ -1568,9 +1568,9 @@ DEV void publish_plane(const FramePtrs& P, const ViewGpu& V, u32 tile, int sampl
         u32 vid = kInvalid;
         if (0ull != pk[k]) {
             vid = ((key & 31u) >= 16u)
-                ? (ig[k].x << (SFT_TRIANGLE_BITS + SFT_MESHLET_BITS)) | (ig[k].y << SFT_TRIANGLE_BITS) |
+                ? (ig[k].x << V.id_shift) | (ig[k].y << SFT_TRIANGLE_BITS) |
                   (((key >> 5) & 15u) * 8u + ((key & 31u) - 16u))
-                : key_to_vid(P, key, n_clip, clip_base);
+                : key_to_vid(P, key, n_clip, clip_base, V.id_shift);
         }
         P.vid[gi] = vid;
         if (P.cap_depth) P.cap_depth[gi] = z;

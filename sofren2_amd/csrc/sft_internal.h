@@ -70,6 +70,8 @@ struct ViewGpu {
                            FramePtrs::band_count / band_flag; kInvalid: not streamed */
     u32 raster_split;   /* estimated sweep above which a payload's rows are split into 2-8 row-band jobs */
     u32 tlas_build;     /* 1: k_tlas builds the view's TLAS into its (reserved) nodes */
+    u32 id_shift;       /* visibility IDs: instance << id_shift | meshlet << SFT_TRIANGLE_BITS | triangle,
+                           id_shift = SFT_TRIANGLE_BITS + meshlet bits of the view's ID split */
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -201,7 +203,8 @@ struct ViewRecord {
     u8 shadows_enabled = 0;
     std::vector<WorldProxy> proxies;
     std::vector<sft_instance> instances;
-    u32 dropped = 0;                /* draws beyond the 64-instance cap (sofren2.c:8412-8417) */
+    u32 dropped = 0;                /* draws beyond the instance cap (sofren2.c:8412-8417) */
+    u32 instance_bits = SFT_INSTANCE_BITS;  /* ID split: 1 << instance_bits instances per frame */
     u32 total_ss = 0;
     std::vector<const sft_model*> transient_models;
     std::vector<u32> tmat_tex, tmat_color;
@@ -308,6 +311,7 @@ struct Context {
     std::vector<WorldProxy> proxies;
 
     i32 instance_count = 0;         /* counts dropped instances too, like the reference */
+    u32 instance_bits = SFT_INSTANCE_BITS;  /* visibility-ID split (sftx_set_id_split) */
     std::vector<sft_instance> instances;
     u32 total_ss = 0;
 

@@ -63,7 +63,7 @@ typedef struct {
 } sftx_scene_common;
 
 #define SFTX_CHARS 32
-#define SFTX_MAX_CHARS 63
+#define SFTX_MAX_CHARS 64
 #define SFTX_BONES 10
 #define SFTX_BANDS 96
 #define SFTX_SEGS 32
@@ -958,9 +958,10 @@ static void sftx_instanced_render(i32 frame) {
 /* ---------------- workload 3: served scene variants -----------------------
    The crowd scene with sizes drawn from the seed: a box lattice of L^3 boxes
    with L = round(cbrt(T / 63)) for T log-uniform in [50k, 1M] triangles
-   (L in [9, 25]; a 2048^2 lightmap page above L = 16), 0-63 skinned humanoids
-   (with the room at most 64 instances, the default visibility-ID split) and
-   0-16 shadow-casting point lights.  Without humanoids or lights the view is
+   (L in [9, 25]; a 2048^2 lightmap page above L = 16), 0-64 skinned humanoids
+   (64 of them and the room are 65 instances: such a scene needs the 7/17
+   visibility-ID split, sftx_scene_characters tells the caller) and 0-16
+   shadow-casting point lights.  Without humanoids or lights the view is
    lit by the box room's 8 point lights without shadows. */
 static void sftx_served_create(void) {
     sftx_rng rng;
@@ -1689,6 +1690,16 @@ void sftx_scene_render(i32 frame, u32* color, i32 pitch_bytes) {
     sftx_scene_record(frame, color, pitch_bytes);
     sft_present();
     sft_frame_end();
+}
+
+i32 sftx_scene_characters(i32 handle) {
+    if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return -1;
+    return sftx_slots[handle]->scene.n_chars;
+}
+
+i32 sftx_scene_shadow_lights(i32 handle) {
+    if (handle < 0 || handle >= SFTX_MAX_SCENES || !sftx_slots[handle]) return -1;
+    return sftx_slots[handle]->scene.n_lights;
 }
 
 u32 sftx_scene_triangle_count(void) { return sftx_cur ? g_scene.tri_count : 0u; }

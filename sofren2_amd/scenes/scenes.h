@@ -22,8 +22,9 @@
  *       vertex), 10 capsule shadow proxies each, 16 shadow-casting point
  *       lights, shadows enabled.
  *   SFTX_SCENE_SERVED (3): variants of workload 2 drawn from the seed for the
- *       request server: 50k-1M triangles (a 9^3 to 25^3 box lattice), 0-63
- *       skinned humanoids, 0-16 shadow-casting lights.
+ *       request server: 50k-1M triangles (a 9^3 to 25^3 box lattice), 0-64
+ *       skinned humanoids (64 and the room need the 7/17 ID split), 0-16
+ *       shadow-casting lights.
  *   SFTX_SCENE_MAP (4): the QSF2 map file named by $SFTX_QSF (a real,
  *       lightmapped scene), lit by its own game-data lights, camera turning
  *       at (0, 2, 0).
@@ -82,6 +83,11 @@ void sftx_scene_free(i32 handle);
 /* the camera sftx_scene_draw would set for that frame and view size; frame
    numbers 1000 * (p + 1) + c pose the scene at frame p with camera frame c */
 void sftx_scene_camera(i32 handle, i32 frame, i32 width, i32 height, sftvec* pos, sftmat* view, sftmat* proj);
+
+/* skinned humanoids the scene draws (each one instance, beside the room) */
+i32 sftx_scene_characters(i32 handle);
+/* shadow-casting point lights of the crowd kinds (0 for the others) */
+i32 sftx_scene_shadow_lights(i32 handle);
 
 /* Frame statistics of the scene description (not of the renderer). */
 u32 sftx_scene_triangle_count(void);

@@ -116,9 +116,12 @@ class MixedBatch:
 # ---------------------------------------------------------------- serving
 def served_scene_specs(n_scenes=32):
     """Resident scenes of the served workload: (kind, seed, pose_frame) of
-    served-scene variants (50k-1M triangles, 0-63 skinned humanoids, 0-16
-    shadow-casting lights, drawn from the seed)."""
-    return [(3, 10 + i, (7 * i) % 120) for i in range(n_scenes)]
+    served-scene variants (50k-1M triangles, 0-64 skinned humanoids, 0-16
+    shadow-casting lights, drawn from the seed).  The last one is seed 102,
+    a 990k-triangle variant with 64 humanoids (65 instances: the 7/17
+    visibility-ID split), so the workload reaches the top of both ranges."""
+    seeds = [10 + i for i in range(n_scenes - 1)] + [102] if n_scenes > 1 else [10]
+    return [(3, seed, (7 * i) % 120) for i, seed in enumerate(seeds)]
 
 
 def served_requests(n, rate_hz, n_scenes, seed=1, min_px=128 * 128, max_px=3840 * 2160):
@@ -146,16 +149,21 @@ class Server:
         import sofren2_amd as S
         self.S, self.lib, self.ct = S, lib, ctypes
         self.specs = specs
-        self.handles, self.templates = [], []
+        self.handles, self.templates, self.id_splits = [], [], []
         for kind, seed, pose in specs:
             h = lib.scene_new(kind, seed, 640, 360)
             if h < 0:
                 raise RuntimeError("scene creation failed")
             self.handles.append(h)
+            # the room and 64 humanoids are 65 instances: the 7/17 ID split
+            split = id_split_for(lib.scene_characters(h) + 1)
+            self.id_splits.append(split)
+            lib.set_id_split(split)
             lib.sft_frame_begin()
             lib.scene_draw(h, pose, 640, 360)
             self.templates.append(lib.server_template())
             lib.sft_frame_end()
+            lib.set_id_split(6)
         if lib.server_start(budget_bytes, max_views) != 0:
             raise RuntimeError("server start failed")
 
@@ -194,6 +202,7 @@ class Server:
         lib.set_output(0)
         lib.sft_resize(width, height)
         lib.set_msaa(samples)
+        lib.set_id_split(self.id_splits[template_index])
         lib.set_capture(S.CAPTURE_VISIBILITY if keep else 0)
         color = np.zeros((height, width), dtype=np.uint32)
         lib.sft_frame_begin()
@@ -212,6 +221,7 @@ class Server:
             out = (color, (depth.view(np.uint32).astype(np.uint64) << np.uint64(32)) | vid.astype(np.uint64))
             lib.set_capture(0)
         lib.set_msaa(1)
+        lib.set_id_split(6)
         return out
 
     def stats(self):
@@ -224,6 +234,13 @@ class Server:
         self.lib.server_release()
         for h in self.handles:
             self.lib.scene_free(h)
+
+
+def id_split_for(instances):
+    """Instance bits of the visibility-ID split a frame of `instances` draws
+    needs: the default 6 up to 64 instances, else the reference's 7/17 build
+    (up to 128)."""
+    return 6 if instances <= 64 else 7
 
 
 def reference_frame_code(pose_frame, camera_frame):

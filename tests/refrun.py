@@ -16,12 +16,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_BIN = {1: "sofren2_ref", 2: "sofren2_ref_aa2", 4: "sofren2_ref_aa4"}
 
 
-def ref_path(samples=1):
-    return os.path.join(ROOT, "oracle", "_ref", REF_BIN[samples])
+def ref_path(samples=1, id_bits=6):
+    """The checker build for `samples` per pixel and the visibility-ID split
+    with `id_bits` instance bits (6: the default; 7: SFT_INSTANCE_BITS=7)."""
+    suffix = {6: "", 7: "_i7"}[id_bits]
+    return os.path.join(ROOT, "oracle", "_ref", REF_BIN[samples] + suffix)
 
 
-def have_ref(samples=1):
-    return os.path.exists(ref_path(samples))
+def have_ref(samples=1, id_bits=6):
+    return os.path.exists(ref_path(samples, id_bits))
 
 
 def prefill_pattern(rows, cols, seed):
@@ -34,12 +37,13 @@ def prefill_pattern(rows, cols, seed):
 
 
 @functools.lru_cache(maxsize=16)
-def ref_dump(scene, seed, width, height, frame=0, samples=1, threads=1, pad=0, prefill=0):
+def ref_dump(scene, seed, width, height, frame=0, samples=1, threads=1, pad=0, prefill=0, id_bits=6):
     """The reference's render of a scene frame.  `pad` bytes of row padding
     and a non-zero `prefill` seed make the colour buffer start as
     prefill_pattern over (width + pad/4) words per row; out["color_buffer"]
-    is then the whole padded buffer after the render."""
-    exe = ref_path(samples)
+    is then the whole padded buffer after the render.  `id_bits` selects the
+    build with that many instance bits in its visibility IDs."""
+    exe = ref_path(samples, id_bits)
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} missing: build it with make -C oracle ref")
     with tempfile.TemporaryDirectory() as d:

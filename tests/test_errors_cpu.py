@@ -170,3 +170,18 @@ def test_oversized_view_and_resize_are_refused(ctx):
         lib.batch_end()
     lib.sft_resize(0, 10)          # refused: the display stays 16384 x 64 from the last view
     lib.sft_resize(17000, 10)
+
+
+def test_id_split_range_and_default(ctx):
+    """sftx_set_id_split takes 1..16 instance bits (the reference's
+    SFT_INSTANCE_BITS + SFT_MESHLET_BITS = 24, sofren2.c:154-165) and leaves
+    the split unchanged for anything else."""
+    lib = ctx
+    assert lib.get_id_split() == 6
+    lib.set_id_split(7)
+    assert lib.get_id_split() == 7
+    for bad in (0, 17, -1):
+        lib.set_id_split(bad)
+        assert lib.get_id_split() == 7
+    lib.set_id_split(6)
+    assert lib.get_id_split() == 6

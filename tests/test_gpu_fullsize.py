@@ -5,9 +5,10 @@ The whole 64-view mixed-resolution batch, rendered as one submission with
 ten of its views compared against the reference: the largest view, the
 largest 4x MSAA view, every crowd-with-shadows view among the first ones and
 views of each other scene kind.  And one 3840x2160 4x MSAA request served on
-a template of over a million triangles (box lattice L = 25 plus 43 skinned
-humanoids under 10 shadow-casting lights), compared against the reference
-rendering the template's pose with the request's camera.
+a template of 1.39 million triangles (box lattice L = 25 plus 64 skinned
+humanoids under 6 shadow-casting lights: 65 instances, so the 7/17
+visibility-ID split), compared against the reference's 7/17 build rendering
+the template's pose with the request's camera.
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -73,7 +74,7 @@ def test_mixed_batch_of_64_views_matches_reference():
         lib.sft_release()
 
 
-SERVED_SPEC = (S.SCENE_SERVED, 82, 30)      # L = 25 lattice, 43 humanoids, 10 lights
+SERVED_SPEC = (S.SCENE_SERVED, 568, 30)     # L = 25 lattice, 64 humanoids, 6 lights
 
 
 def test_served_4k_msaa4_request_on_a_million_triangle_template():
@@ -83,15 +84,15 @@ def test_served_4k_msaa4_request_on_a_million_triangle_template():
     try:
         srv = workloads.Server(lib, [SERVED_SPEC], budget_bytes=16 << 30, max_views=4)
         tris = lib.scene_tris()
-        print("template triangles", tris, flush=True)
-        assert tris >= 1_000_000
+        print("template triangles", tris, "ID split", srv.id_splits, flush=True)
+        assert tris >= 1_000_000 and srv.id_splits == [7]
         w, h, ns, cf = 3840, 2160, 4, 17
         color = np.zeros((h, w), dtype=np.uint32)
         vis = np.zeros(((w + 63) // 64) * 64 * ((h + 63) // 64) * 64 * ns, dtype=np.uint64)
         srv.wait(srv.submit(srv.request(0, cf, w, h, ns, color, vis)))
         print("served", srv.stats(), flush=True)
         kind, seed, pose = SERVED_SPEC
-        r = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns, threads=8)
+        r = ref_dump(kind, seed, w, h, workloads.reference_frame_code(pose, cf), samples=ns, id_bits=7)
         got = {"color": color, "vid": (vis & 0xFFFFFFFF).astype(np.uint32),
                "depth": (vis >> np.uint64(32)).astype(np.uint32).view(np.float32)}
         rep = parity.compare_frame(got, r)
