@@ -208,6 +208,9 @@ typedef struct sftx_server_stats {
        delivering their outputs), and the batches' GPU spans (first pass to
        last, not counting waits for the batch before) */
     f64 host_submit_ms, host_complete_ms, gpu_ms;
+    /* views of templates with capsule shadows rendered from the template's
+       traced shadow records (no occlusion rays, no frame TLAS) */
+    u64 shadow_cache_views;
 } sftx_server_stats;
 
 i32 sftx_server_template(void);

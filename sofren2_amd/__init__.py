@@ -114,7 +114,8 @@ class Request(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "completed", "batches", "views_in_batches", "max_batch", "admission_waits", "resident_bytes",
-        "pending", "in_flight")] + [(n, ctypes.c_double) for n in ("host_submit_ms", "host_complete_ms", "gpu_ms")]
+        "pending", "in_flight")] + [(n, ctypes.c_double) for n in ("host_submit_ms", "host_complete_ms", "gpu_ms")] + [
+        ("shadow_cache_views", ctypes.c_uint64)]
 
 
 def _bind(lib, name, res, args):

@@ -183,6 +183,7 @@ struct FrameSet {
     bool any_vis64 = false, cap_vis = false, host_color = false;
     size_t o_batch = 0, o_views = 0, o_tv = 0, o_st = 0, o_insts = 0, o_work = 0, o_bones = 0, o_pl = 0, o_dl = 0;
     size_t o_mt = 0, o_mc = 0, o_ms = 0, o_tt = 0, o_tc = 0, o_ts = 0, o_mr = 0, o_px = 0, o_tlas = 0, o_seg = 0;
+    size_t o_clp = 0, o_clpc = 0;
     size_t o_tpos = 0, o_tattr = 0, o_tinds = 0, o_tcull = 0, o_tsm = 0, o_tskin = 0;
     u32 counters[CNT_COUNT] = {};
 };
@@ -868,6 +869,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     std::vector<f32> tmat_shin;
     std::vector<ProxyGpu> proxies;
     std::vector<sft_bvh_node> tlas;
+    std::vector<ProxyRec> cached_lp;    /* capsule records of views served from a template's shadow cache */
+    std::vector<u32> cached_lpc;
     std::vector<u32> tile_view, shade_tiles;
     std::vector<BinSeg> segs;
     u64 bin_m_words = 0;
@@ -974,8 +977,19 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         for (const WorldProxy& w : v.proxies) {
             proxies.push_back(ProxyGpu{{w.posa.x, w.posa.y, w.posa.z}, {w.posb.x, w.posb.y, w.posb.z}, w.radius, 0.f});
         }
-        /* frame TLAS when the occlusion pass runs (sofren2.c:8161-8164) */
-        const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
+        /* frame TLAS when the occlusion pass runs (sofren2.c:8161-8164); a
+           served template's view whose capsule records were read back from
+           an earlier batch skips both and uploads them */
+        const bool shadowed = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
+        const bool cached = shadowed && v.shadow_cache && v.shadow_cache->filled &&
+                            v.shadow_cache->counts.size() == v.point_lights.size();
+        const bool occlusion = shadowed && !cached;
+        u32 lp_cache = kInvalid;
+        if (cached) {
+            lp_cache = (u32)cached_lpc.size();
+            cached_lp.insert(cached_lp.end(), v.shadow_cache->recs.begin(), v.shadow_cache->recs.end());
+            cached_lpc.insert(cached_lpc.end(), v.shadow_cache->counts.begin(), v.shadow_cache->counts.end());
+        }
         std::vector<sft_bvh_node> vt;
         u32 tlas_root = 0;
         bool tlas_dev = false;
@@ -1071,6 +1085,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         g.band_base = kInvalid;
         g.raster_split = std::max(1u, b->raster_split[res_class(v.width, v.height)]);
         g.tlas_build = tlas_dev ? 1u : 0u;
+        g.lp_cache = lp_cache;
         g.id_shift = (u32)SFT_TRIANGLE_BITS + 24u - v.instance_bits;
         if (v.vis64) {
             vp.vis_stage = vis_stage_total;
@@ -1173,6 +1188,8 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     fs.o_px = pk.add(proxies);
     fs.o_tlas = pk.add(tlas);
     fs.o_seg = pk.add(segs);
+    fs.o_clp = pk.add(cached_lp);
+    fs.o_clpc = pk.add(cached_lpc);
     /* transient rows, filled in place once every section is reserved */
     fs.o_tpos = pk.reserve((size_t)fs.t_slots * 768);
     fs.o_tattr = pk.reserve((size_t)fs.t_slots * 1792);
@@ -1330,7 +1347,8 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     }
     for (u32 vi = 0; vi < nv; vi += 1) {
         const ViewRecord& v = views[vi];
-        const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty();
+        const bool occlusion = v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty() &&
+                               kInvalid == fs.vgpu[vi].lp_cache;
         if (occlusion && 0 == fs.vgpu[vi].fc.tlas_nodes) {
             /* no instance to occlude: every ray reaches its light */
             HIP_CHECK(hipMemsetAsync((u8*)fs.proxy_vis.p + (size_t)fs.plan[vi].proxy_base * 64, 3, v.proxies.size() * 64, s));
@@ -1402,6 +1420,8 @@ static bool launch_batch(Backend* b, std::vector<ViewRecord>& views, FrameSet& f
     p.proxy_vis = (u8*)fs.proxy_vis.p;
     p.light_proxy_count = (u32*)fs.light_proxy_count.p;
     p.light_proxies = (ProxyRec*)fs.light_proxies.p;
+    p.cached_lp = (const ProxyRec*)(pd + fs.o_clp);
+    p.cached_lpc = (const u32*)(pd + fs.o_clpc);
     p.color_host = stream ? (u32*)fs.out_h : nullptr;
     p.band_count = zd + COUNTER_WORDS + (size_t)fs.nt * fs.max_samples;
     p.band_flag = stream ? fs.band_flag_h : nullptr;
@@ -1570,6 +1590,7 @@ static void relocate_transient(Backend* b, FrameSet& fs) {
 
 static void write_stats(Context& c, const std::vector<ViewRecord>& views, const FrameSet& fs);
 static void capture_outputs(Backend* b, const std::vector<ViewRecord>& views, const FrameSet& fs);
+static void fill_shadow_caches(Backend* b, const std::vector<ViewRecord>& views, const FrameSet& fs);
 
 /* Waits for a launched batch; re-runs it with larger intermediates if any
    overflowed; writes the host outputs (unless its copier already did), the
@@ -1611,8 +1632,28 @@ static void complete_batch(Backend* b, Context& c, std::vector<ViewRecord>& view
         }
         copy_spans(host_output_spans(views, fs));
     }
+    fill_shadow_caches(b, views, fs);
     write_stats(c, views, fs);
     capture_outputs(b, views, fs);
+}
+
+/* served templates' shadow caches: the capsule records and counts of each
+   view that traced its template's proxies, read back once the batch is done */
+static void fill_shadow_caches(Backend* b, const std::vector<ViewRecord>& views, const FrameSet& fs) {
+    (void)b;
+    for (u32 vi = 0; vi < (u32)views.size(); vi += 1) {
+        const ViewRecord& v = views[vi];
+        ShadowCache* sc = v.shadow_cache.get();
+        if (!sc || sc->filled || kInvalid != fs.vgpu[vi].lp_cache) continue;
+        if (!(v.shadows_enabled && !v.point_lights.empty() && !v.proxies.empty())) continue;
+        const size_t nl = v.point_lights.size(), l0 = fs.vgpu[vi].pl_base;
+        sc->recs.resize(nl * kMaxLightProxies);
+        sc->counts.resize(nl);
+        HIP_CHECK(hipMemcpy(sc->recs.data(), (const ProxyRec*)fs.light_proxies.p + l0 * kMaxLightProxies,
+                            nl * kMaxLightProxies * sizeof(ProxyRec), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(sc->counts.data(), (const u32*)fs.light_proxy_count.p + l0, nl * 4, hipMemcpyDeviceToHost));
+        sc->filled = true;
+    }
 }
 
 /* the frame statistics of a completed batch (sftx_get_stats) */

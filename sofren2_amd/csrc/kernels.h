@@ -105,6 +105,8 @@ struct FramePtrs {
     u8* proxy_vis;              /* [n_proxies][64] rays of 3 reaching each light */
     u32* light_proxy_count;     /* [n_point] */
     ProxyRec* light_proxies;    /* [n_point][16] */
+    const ProxyRec* cached_lp;  /* capsule records of views with lp_cache set, [lights][16] */
+    const u32* cached_lpc;      /* their counts, [lights] */
     f32* cap_depth;         /* [samples][grid] 1/w when capturing, else null */
     u64* vis64;             /* [samples][grid] (1/w bits << 32) | ID when requested, else null */
     u32* counters;          /* [CNT_COUNT] */

@@ -2612,6 +2612,16 @@ __global__ __launch_bounds__(64) void k_proxy_cache(FramePtrs P) {
     const FrameConst& fc = VW.fc;
     if (!(fc.shadows_enabled && fc.n_proxies > 0u)) return;
     const u32 l = blockIdx.x - VW.pl_base;
+    if (kInvalid != VW.lp_cache) {
+        /* a served template's records, traced by an earlier batch */
+        const u32 n = P.cached_lpc[VW.lp_cache + l];
+        if ((u32)lane < n) {
+            P.light_proxies[(u64)(VW.pl_base + l) * kMaxLightProxies + (u32)lane] =
+                P.cached_lp[(u64)(VW.lp_cache + l) * kMaxLightProxies + (u32)lane];
+        }
+        if (0 == lane) P.light_proxy_count[VW.pl_base + l] = n;
+        return;
+    }
     const PointLightGpu& L = P.point_lights[VW.pl_base + l];
     const sftvec lpos = SFT_V(L.pos[0], L.pos[1], L.pos[2]);
     u32 count = 0;

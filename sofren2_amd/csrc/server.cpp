@@ -17,6 +17,11 @@
  * batches in flight plus the candidate), launched on a free set's stream, and
  * the oldest batch is completed when every slot is busy.  Requests are taken
  * in arrival order.
+ *
+ * A template with capsule shadows keeps the per-light capsule records of its
+ * proxies once a batch has traced them (they depend on the template only):
+ * later requests upload them instead of tracing the proxy-occlusion rays and
+ * building the frame TLAS again.
  */
 #include <chrono>
 #include <condition_variable>
@@ -179,7 +184,10 @@ void server_loop(Server* sv) {
                 f.set = next_set;
                 next_set = (next_set + 1) % kServerBatchesInFlight;
                 sv->inflight_bytes += bytes;
-                for (Request* q : f.reqs) f.views.push_back(make_view(*sv->templates[(size_t)q->r.scene], q->r));
+                for (Request* q : f.reqs) {
+                    f.views.push_back(make_view(*sv->templates[(size_t)q->r.scene], q->r));
+                    if (f.views.back().shadow_cache && f.views.back().shadow_cache->filled) sv->stats.shadow_cache_views += 1;
+                }
                 sv->stats.batches += 1;
                 sv->stats.views_in_batches += f.reqs.size();
                 sv->stats.max_batch = std::max<u64>(sv->stats.max_batch, f.reqs.size());
@@ -247,6 +255,10 @@ extern "C" i32 sftx_server_template(void) {
         t->owned.push_back(std::move(o));
     }
     t->frame.transient_models.clear();
+    /* capsule shadows: the records every request shares, filled by the first batch that traces them */
+    if (t->frame.shadows_enabled && !t->frame.point_lights.empty() && !t->frame.proxies.empty()) {
+        t->frame.shadow_cache = std::make_shared<ShadowCache>();
+    }
     /* poses as they are now */
     size_t nb = 0;
     for (const sft_instance& in : t->frame.instances) {

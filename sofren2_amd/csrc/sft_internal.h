@@ -22,6 +22,7 @@
 #pragma once
 #include <stdint.h>
 #include <deque>
+#include <memory>
 #include <vector>
 #include <unordered_map>
 #include "../../include/sofren2.h"
@@ -72,6 +73,8 @@ struct ViewGpu {
     u32 tlas_build;     /* 1: k_tlas builds the view's TLAS into its (reserved) nodes */
     u32 id_shift;       /* visibility IDs: instance << id_shift | meshlet << SFT_TRIANGLE_BITS | triangle,
                            id_shift = SFT_TRIANGLE_BITS + meshlet bits of the view's ID split */
+    u32 lp_cache;       /* first light of the view's capsule records in FramePtrs::cached_lp
+                           (uploaded from a served template's ShadowCache); kInvalid: traced */
     u64 vid_base;       /* first element of the view's [samples][grid] planes */
     u64 color_base;     /* first pixel of the view's width x height colour */
 };
@@ -126,6 +129,18 @@ struct ProxyRec {
 };
 static_assert(sizeof(ProxyRec) == 56, "ProxyRec layout");
 constexpr int kMaxLightProxies = 16;
+
+/* A served template's shadow data: per point light, the capsule records the
+   shading reads (k_proxy_cache's selection over the proxies' occlusion
+   counts).  Proxies, lights, instances and BLAS are the template's, the
+   request's camera takes no part, so every request of the template gets the
+   same records: they are read back from the first batch that traced them
+   and uploaded with later batches instead of tracing again. */
+struct ShadowCache {
+    bool filled = false;
+    std::vector<ProxyRec> recs;     /* [n_point][kMaxLightProxies] */
+    std::vector<u32> counts;        /* [n_point] */
+};
 
 struct WorkItem {
     u32 slot;           /* resident meshlet slot */
@@ -205,6 +220,7 @@ struct ViewRecord {
     std::vector<sft_instance> instances;
     u32 dropped = 0;                /* draws beyond the instance cap (sofren2.c:8412-8417) */
     u32 instance_bits = SFT_INSTANCE_BITS;  /* ID split: 1 << instance_bits instances per frame */
+    std::shared_ptr<ShadowCache> shadow_cache;  /* served templates with capsule shadows */
     u32 total_ss = 0;
     std::vector<const sft_model*> transient_models;
     std::vector<u32> tmat_tex, tmat_color;

@@ -52,6 +52,8 @@ def test_ctypes_structs_match_the_header(tmp_path):
         "sftx_frame_stats": (S.FrameStats, ["meshlets_tested", "shadow_proxies", "ms_upload", "ms_shadows", "tex_npot",
                                             "instances_dropped"]),
         "sftx_request": (S.Request, ["scene", "camera_pos", "view", "proj", "color", "vis64"]),
+        "sftx_server_stats": (S.ServerStats, ["submitted", "resident_bytes", "in_flight", "host_submit_ms", "gpu_ms",
+                                              "shadow_cache_views"]),
         "sftx_raw_geometry": (S.RawGeometry, ["vx", "v", "j0", "j3", "w0", "w3"]),
         "sft_mesh": (S.Mesh, ["meshlet_count", "inds", "bounds_max_x", "cone_cutoff", "culling_bone_inds",
                               "lightmap_page"]),

@@ -1,5 +1,7 @@
 """Request server on the GPU: continuous batching gives each request the same
-frame it gets alone, and that frame is the reference's."""
+frame it gets alone, and that frame is the reference's.  Requests of a
+template with capsule shadows after its first batch render from the
+template's traced shadow records: the same frames."""
 import numpy as np
 import pytest
 
@@ -45,6 +47,10 @@ def test_served_requests_same_alone_and_in_the_mix():
             print("alone vs mix", k, flush=True)
             np.testing.assert_array_equal(c, c_mix)
             np.testing.assert_array_equal(v, v_mix)
+        # the shadowed templates' requests alone reused the records the mix traced
+        st = srv.stats()
+        print("shadow cache views", st["shadow_cache_views"], flush=True)
+        assert st["shadow_cache_views"] >= sum(1 for t, *_ in REQUESTS if t in (1, 3))
         # against the reference: the template's pose with the request's camera
         for k in (1, 2, 3, 8, 9):
             t, cf, w, h, ns = REQUESTS[k]
