@@ -82,6 +82,28 @@ def measured_traffic(workload, kernel):
     return int(k["hbm_bytes"]) if k else None
 
 
+PASS_KERNELS = {"geometry": ["k_geom", "k_scan_sums", "k_scan_apply"],
+                "binning": ["k_clip", "k_bin_count", "k_bin_scan", "k_bin"],
+                "visibility": ["k_raster_plan", "k_raster", "k_tile_lights"],
+                "shading": ["k_shade<false>", "k_shade<true>", "k_shade_ms<false>", "k_shade_ms<true>"]}
+
+
+def measured_pass_gbs(workload, pass_ms):
+    """Each pass's rocprof-measured HBM bytes per frame (the committed PMC
+    summary of this workload: FETCH_SIZE + WRITE_SIZE of the pass's kernels
+    per dispatch, calibrated) over the pass's event-timed duration, or None
+    without a summary."""
+    path = TRAFFIC_FILE.format(workload=workload)
+    if not os.path.exists(path):
+        return None
+    ks = json.load(open(path)).get("kernels", {})
+    out = {}
+    for name, kernels in PASS_KERNELS.items():
+        b = sum(ks[k]["hbm_bytes"] for k in kernels if k in ks)
+        out[name] = round(b / (pass_ms[name] * 1e-3) / 1e9, 1) if pass_ms[name] > 0 else None
+    return out
+
+
 def algorithmic_bytes(st):
     """Per-frame algorithmic HBM bytes of each pass (counts from the frame's
     own counters): compulsory traffic plus the reference's materialised
@@ -566,6 +588,7 @@ def main():
                      "traffic": measured_traffic(f"{args.scene}_{w}x{h}", pass_kernel(dominant, args.scene)),
                      "algorithmic_bytes_per_launch": int(dom_bytes)},
         "pass_roofline_gbs": {k: round(v[0] / (v[1] * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
+        "pass_measured_gbs": measured_pass_gbs(f"{args.scene}_{w}x{h}", {k: v[1] for k, v in per_pass.items()}),
         "counters_per_frame": counters_avg,
         "cpu_baseline": None,
     }
