@@ -188,7 +188,7 @@ def tuning_table(lib, repeats=3):
     out = {"views_per_class": {k: len(v) for k, v in classes.items()}, "classes": {}}
     for k, views in classes.items():
         row = {"payloads_per_item": {}, "split_threshold": {}}
-        for ch in (16, 32, 64, 96, 128):
+        for ch in (64, 96, 128, 256, 384, 512):
             tuned(raster_chunk=[ch] * 3, raster_split=list(base.raster_split))
             row["payloads_per_item"][ch] = measure(views, "ms_raster")
         for sp in (1024, 2048, 4096, 8192, 0xFFFFFFFF):

@@ -1080,7 +1080,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
         }
         bin_m_words += (u64)(n_bseg + 1u) * vp.n_tiles;
         g.vis64 = v.vis64 ? 1u : 0u;
-        g.raster_chunk = std::max(1u, std::min(b->raster_chunk[res_class(v.width, v.height)], kRasterMaxChunk));
+        g.raster_chunk = std::max(1u, std::min(b->raster_chunk[res_class(v.width, v.height)], kRasterMaxItem));
         min_chunk = std::min(min_chunk, g.raster_chunk);
         g.band_base = kInvalid;
         g.raster_split = std::max(1u, b->raster_split[res_class(v.width, v.height)]);
@@ -1832,7 +1832,7 @@ void backend_get_tuning(const Backend* b, sftx_tuning* t) {
 
 void backend_set_tuning(Backend* b, const sftx_tuning* t) {
     for (int k = 0; k < SFTX_RES_CLASSES; k += 1) {
-        b->raster_chunk[k] = std::max(1u, std::min(t->raster_chunk[k], kRasterMaxChunk));
+        b->raster_chunk[k] = std::max(1u, std::min(t->raster_chunk[k], kRasterMaxItem));
     }
     b->raster_groups = std::max(1u, std::min(t->raster_groups, 4u * raster_groups_per_device()));
     for (int k = 0; k < SFTX_RES_CLASSES; k += 1) b->raster_split[k] = std::max(1u, t->raster_split[k]);
@@ -1888,7 +1888,7 @@ void backend_tune(Backend* b, Context* cp, std::vector<ViewRecord>& views, int r
         knob = best_v;
         return best_ms;
     };
-    static const u32 chunks[] = { 16, 32, 64, 96, 128 };
+    static const u32 chunks[] = { 64, 96, 128, 256, 384, 512 };
     for (int k = 0; k < SFTX_RES_CLASSES; k += 1) {
         report->ms_class[k] = 0.f;
         std::vector<ViewRecord> sub;

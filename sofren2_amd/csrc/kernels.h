@@ -46,6 +46,7 @@ constexpr int CNT_STAT_SLOTS = 64;
 constexpr int CNT_STAT_STRIDE = 32;     /* u32 per slot: one 128-byte line */
 /* largest payload count of one visibility work item (its triangles are staged in LDS) */
 constexpr u32 kRasterMaxChunk = 128;
+constexpr u32 kRasterMaxItem = 1024;     /* payloads per visibility work item: up to 8 LDS stagings */
 constexpr u32 kOccStackEntries = 1024;   /* occlusion traversal stack per wave */
 constexpr u32 kRasterJobCost = 2048;     /* default row-band split threshold (ViewGpu::raster_split) */
 constexpr size_t COUNTER_WORDS = CNT_COUNT + (size_t)CNT_COUNT * CNT_STAT_SLOTS * CNT_STAT_STRIDE;

@@ -1484,7 +1484,8 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
 }
 
 /* Visibility pass.  A work item is (tile, sample plane, chunk of at most
-   kRasterChunk payloads of the tile's list); k_raster_plan lists the items,
+   kRasterMaxItem payloads of the tile's list, staged in LDS kRasterMaxChunk
+   at a time into the same plane); k_raster_plan lists the items,
    and a persistent k_raster takes them from a queue, so a tile covered by
    thousands of payloads is spread over many workgroups.  Each item rasterises
    into an LDS plane; a tile with one chunk publishes from LDS, the chunks of a
@@ -1509,7 +1510,7 @@ __global__ __launch_bounds__(256) void k_raster_plan(FramePtrs P) {
     __shared__ u32 s_at;
     const u32 tile = blockIdx.x;
     const ViewGpu& V = P.views[P.tile_view[tile]];
-    const u32 S = (u32)V.fc.samples, K = min(max(V.raster_chunk, 1u), kRasterMaxChunk), maxS = P.batch->max_samples;
+    const u32 S = (u32)V.fc.samples, K = min(max(V.raster_chunk, 1u), kRasterMaxItem), maxS = P.batch->max_samples;
     u32 off;
     const u32 cnt = tile_payloads(P, tile, off);
     const u32 nch = max(1u, (cnt + K - 1u) / K);
@@ -1699,15 +1700,18 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
         const u32 local = tile - V.tile_base;
         const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
-        const u32 K = min(max(V.raster_chunk, 1u), kRasterMaxChunk);
+        const u32 K = min(max(V.raster_chunk, 1u), kRasterMaxItem);
 
         u32 off;
         const u32 cnt = tile_payloads(P, tile, off);
         const u32 tile_chunks = P.tile_nch[tile];
         const u32 e0 = chunk * K, e1 = min(cnt, e0 + K);
-        const u32 n_tri = (e1 > e0) ? (e1 - e0) * 8u : 0u;
+        /* the item's payloads staged kRasterMaxChunk at a time into the same plane */
+        for (u32 c0 = e0;; c0 += kRasterMaxChunk) {
+        const u32 c1 = min(e1, c0 + kRasterMaxChunk);
+        const u32 n_tri = (c1 > c0) ? (c1 - c0) * 8u : 0u;
         for (u32 q = (u32)tid; q < n_tri; q += kRasterThreads) {
-            const RasterTri tr = fetch_raster_tri(P, off + e0 + (q >> 3), q & 7u);
+            const RasterTri tr = fetch_raster_tri(P, off + c0 + (q >> 3), q & 7u);
             s_tri[q] = tr;
             i32 rows = 0, chunks = 0;
             s_ext[q] = tri_extent(tr, tmx, tmy, rows, chunks) ? (u16)(rows | (chunks << 8)) : (u16)0;
@@ -1716,7 +1720,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         __syncthreads();
         /* each payload's estimated sweep: its groups walk every row of their
            triangle and run the chunks of 1/(groups per triangle) of them */
-        const u32 n_pl = e1 - e0;
+        const u32 n_pl = (c1 > c0) ? c1 - c0 : 0u;
         u32 parts_log = 0u;
         if ((u32)tid < n_pl) {
             int n = 0;
@@ -1816,6 +1820,9 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
             const int stride = m << plog, phase = r + m * (int)part;
             if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
             else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
+        }
+        __syncthreads();
+        if (c1 >= e1) break;
         }
         __syncthreads();
 

@@ -188,7 +188,7 @@ def test_tuned_settings_change_no_output_bit():
         tuned = views()
         t = S.tune_batch(lib, tuned, repeats=2)
         print("tuned", t.as_dict(), flush=True)
-        assert all(c in (16, 32, 64, 96, 128) for c in t.raster_chunk)
+        assert all(c in (64, 96, 128, 256, 384, 512) for c in t.raster_chunk)
         assert all(c in (1024, 2048, 4096, 8192, 0xFFFFFFFF) for c in t.raster_split)
         assert t.occlusion_stack in (256, 512, 1024) and t.ms_occlusion > 0
         assert all(ms > 0 for ms in t.ms_class), "every class has views here"
@@ -201,7 +201,8 @@ def test_tuned_settings_change_no_output_bit():
         for v, b in zip(tuned, base):
             np.testing.assert_array_equal(v.color, b.color)
             np.testing.assert_array_equal(v.vis, b.vis)
-        for chunks, split, stack in (([1, 16, 1], [1, 1, 1], 256), ([128, 1, 32], [0xFFFFFFFF] * 3, 512)):
+        for chunks, split, stack in (([1, 16, 1], [1, 1, 1], 256), ([128, 1, 32], [0xFFFFFFFF] * 3, 512),
+                                     ([1024, 300, 129], [2048] * 3, 1024)):
             extreme = S.Tuning()
             extreme.raster_chunk[:] = chunks
             extreme.raster_split[:] = split
