@@ -1657,30 +1657,29 @@ DEV bool tri_extent(const RasterTri& tr, i32 tmx, i32 tmy, i32& rows, i32& chunk
     return true;
 }
 
-/* groups of a wave that share each of a payload's n triangles (a power of
-   two, 8 / n rounded down) */
-DEV int groups_per_tri(int n) { return (n <= 1) ? 8 : (n <= 2) ? 4 : (n <= 4) ? 2 : 1; }
-
-/* Work of one item's rasterisation: a job is a payload (its 8 triangles on
-   the wave's 8 lane groups) or one of K parts of a payload whose largest
-   triangle would keep one wave busy far longer than the others: part k of
-   K takes every K-th of the rows its groups would sweep.  Waves take jobs
-   from an LDS counter, the parts of split payloads first. */
-constexpr u32 kRasterMaxParts = 8;
-
+/* Work of one item's rasterisation: a unit is a staged triangle that
+   reaches the tile, or one of K row parts of a triangle whose estimated
+   sweep exceeds the split threshold (part k of K takes every K-th of its
+   rows).  Units are listed by estimated cost per unit, the most expensive
+   class first, and a wave takes 8 consecutive units at a time from an LDS
+   counter, one per group of 8 lanes: a wave's groups then sweep for about
+   as long as each other.  The visibility plane's 64-bit atomic max makes
+   the result independent of which group sweeps which unit. */
 __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     /* the work item's triangles, fetched by the whole group at once so the
        waves rasterise from LDS instead of each waiting on its own chain of
        descriptor, work-item, index and vertex loads */
     __shared__ RasterTri s_tri[kRasterMaxChunk * 8];
-    /* jobs: payload | part << 8 | log2(parts) << 12 */
-    __shared__ u16 s_jobs[kRasterMaxChunk * kRasterMaxParts];
+    /* units: staged triangle | part << 10 | log2(parts) << 13; a sub-chunk
+       whose split units exceed the list sweeps each triangle as one unit */
+    constexpr u32 kUnitCap = kRasterMaxChunk * 16u;
+    __shared__ u16 s_units[kUnitCap];
+    __shared__ u32 s_hist[32], s_cur[16], s_split;
     /* each staged triangle's rows | chunks << 8 in the tile, 0 if it misses it */
     __shared__ u16 s_ext[kRasterMaxChunk * 8];
     __shared__ f32 s_red[2][kRasterWaves];
     __shared__ u32 s_item, s_last, s_njobs, s_next;
-    __shared__ u32 s_tot[2][2];     /* heavy parts and light payloads of waves 0 and 1 */
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Est est = make_est(P);
     const u32 n_items = P.counters[CNT_RASTER_ITEMS];
@@ -1718,106 +1717,74 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
         }
         if (0 == tid) s_next = 0u;
         __syncthreads();
-        /* each payload's estimated sweep: its groups walk every row of their
-           triangle and run the chunks of 1/(groups per triangle) of them */
-        const u32 n_pl = (c1 > c0) ? c1 - c0 : 0u;
-        u32 parts_log = 0u;
-        if ((u32)tid < n_pl) {
-            int n = 0;
-            u32 cost = 0u;
-            i32 rows[8], chunks[8];
-            for (int k = 0; k < 8; k += 1) {
-                const u32 e = s_ext[(u32)tid * 8u + (u32)k];
-                rows[k] = (i32)(e & 0xFFu);
-                chunks[k] = (i32)(e >> 8);
-                n += e ? 1 : 0;
+        /* units: each staged triangle that reaches the tile, as 1, 2, 4 or 8
+           row parts (part k of K takes every K-th row) when its estimated
+           sweep exceeds the split threshold, listed by cost class (the
+           largest first) so that the 8 units of a wave's job cost about the
+           same; more units than the list holds: one part per triangle */
+        if (tid < 32) s_hist[tid] = 0u;
+        __syncthreads();
+        u32 ucls[2] = {0u, 0u}, ucls1[2] = {0u, 0u}, upl[2] = {0u, 0u};
+        for (int h = 0; h < 2; h += 1) {
+            const u32 q = (u32)tid + (u32)h * kRasterThreads;
+            if (q < n_tri && 0u != s_ext[q]) {
+                const u32 e = s_ext[q], rows = e & 0xFFu, chunks = e >> 8;
+                const u32 cost = rows * 4u + rows * chunks * 20u;
+                u32 pl = 0u;
+                while (pl < 3u && (cost >> pl) > V.raster_split) pl += 1u;
+                upl[h] = pl;
+                ucls[h] = min(15u, 31u - (u32)__clz((int)max(cost >> pl, 1u)));
+                ucls1[h] = min(15u, 31u - (u32)__clz((int)max(cost, 1u)));
+                atomicAdd(&s_hist[ucls[h]], 1u << pl);
+                atomicAdd(&s_hist[16u + ucls1[h]], 1u);
             }
-            const int m = groups_per_tri(n);
-            for (int k = 0; k < 8; k += 1) {
-                const u32 c = (u32)rows[k] * 4u + (u32)((rows[k] + m - 1) / m) * (u32)chunks[k] * 20u;
-                cost = max(cost, c);
-            }
-            while (parts_log < 3u && (cost >> parts_log) > V.raster_split) parts_log += 1u;
         }
-        /* job list: the parts of split payloads first, then the others, each
-           in payload order; positions from prefix sums over waves 0 and 1
-           (at most 128 payloads, one per thread) */
-        const u32 n_parts = ((u32)tid < n_pl) ? (1u << parts_log) : 0u;
-        const u32 heavy = parts_log ? n_parts : 0u, light = (n_parts && !parts_log) ? 1u : 0u;
-        u32 hx = heavy, lx = light;
-        if (wave < 2) {
-            for (int o = 1; o < 64; o <<= 1) {
-                const u32 hy = __shfl_up(hx, o), ly = __shfl_up(lx, o);
-                if (lane >= o) {
-                    hx += hy;
-                    lx += ly;
+        __syncthreads();
+        if (0 == tid) {
+            u32 t0 = 0u, t1 = 0u;
+            for (int c = 0; c < 16; c += 1) { t0 += s_hist[c]; t1 += s_hist[16 + c]; }
+            const bool split = t0 <= kUnitCap;
+            u32 at = 0u;
+            for (int c = 15; c >= 0; c -= 1) {
+                s_cur[c] = at;
+                at += split ? s_hist[c] : s_hist[16 + c];
+            }
+            s_njobs = at;
+            s_split = split ? 1u : 0u;
+            s_next = 0u;
+        }
+        __syncthreads();
+        {
+            const bool split = 0u != s_split;
+            for (int h = 0; h < 2; h += 1) {
+                const u32 q = (u32)tid + (u32)h * kRasterThreads;
+                if (q < n_tri && 0u != s_ext[q]) {
+                    const u32 pl = split ? upl[h] : 0u, n = 1u << pl;
+                    const u32 at = atomicAdd(&s_cur[split ? ucls[h] : ucls1[h]], n);
+                    for (u32 k = 0; k < n; k += 1) s_units[at + k] = (u16)(q | (k << 10) | (pl << 13));
                 }
             }
-            if (63 == lane) {
-                s_tot[wave][0] = hx;
-                s_tot[wave][1] = lx;
-            }
         }
         __syncthreads();
-        if (wave < 2 && n_parts) {
-            const u32 h_all = s_tot[0][0] + s_tot[1][0];
-            const u32 hb = hx - heavy + (1 == wave ? s_tot[0][0] : 0u);
-            const u32 lb = lx - light + (1 == wave ? s_tot[0][1] : 0u);
-            if (parts_log) {
-                for (u32 k = 0; k < n_parts; k += 1) s_jobs[hb + k] = (u16)((u32)tid | (k << 8) | (parts_log << 12));
-            }
-            else {
-                s_jobs[h_all + lb] = (u16)tid;
-            }
-        }
-        if (0 == tid) s_njobs = s_tot[0][0] + s_tot[1][0] + s_tot[0][1] + s_tot[1][1];
-        __syncthreads();
-        /* a wave per job, 8 lanes per group, lane = x inside the 8-px
-           chunk.  The payload's triangles that reach the tile share the
-           wave's 8 groups: with n of them each gets 8/n groups (rounded
-           down to a power of two) that split its rows, so one large
-           triangle is swept by the whole wave instead of by 8 lanes; the
-           parts of a split payload split those rows again. */
+        /* a wave per job of 8 consecutive units, 8 lanes per group, lane =
+           x inside the 8-px chunk; group g sweeps unit 8 * job + g */
         const int g = lane >> 3, j = lane & 7;
-        const u32 n_jobs = s_njobs;
+        const u32 n_units = s_njobs, n_jobs = (n_units + 7u) >> 3;
         for (;;) {
             u32 jn = 0u;
             if (0 == lane) jn = atomicAdd(&s_next, 1u);
             jn = __shfl(jn, 0);
             if (jn >= n_jobs) break;
-            const u32 job = s_jobs[jn];
-            const u32 part = (job >> 8) & 7u, plog = (job >> 12) & 3u;
-            const RasterTri* pt = &s_tri[(job & 0xFFu) * 8u];
-            bool act = false;
-            {
-                const RasterTri& tr = pt[g];
-                if (kInvalid != tr.key) {
-                    const i32 mnx = cvtt(smax(smin(tr.v[0], smin(tr.v[3], tr.v[6])), (f32)tmx));
-                    const i32 mxx = cvtt(smin(smax(tr.v[0], smax(tr.v[3], tr.v[6])), (f32)(tmx + 63)));
-                    const i32 mny = cvtt(smax(smin(tr.v[1], smin(tr.v[4], tr.v[7])), (f32)tmy));
-                    const i32 mxy = cvtt(smin(smax(tr.v[1], smax(tr.v[4], tr.v[7])), (f32)(tmy + 63)));
-                    act = mnx <= mxx && mny <= mxy;
-                }
-            }
-            const u64 ball = __ballot(act && 0 == j);
-            u32 amask = 0;
-            for (int k = 0; k < 8; k += 1) amask |= (u32)((ball >> (k * 8)) & 1ull) << k;
-            const int n = __popc(amask);
-            if (0 == n) continue;
-            const int m = groups_per_tri(n);
-            const int t = g / m, r = g & (m - 1);
-            if (t >= n) continue;
-            /* the t-th active triangle */
-            u32 bits = amask;
-            for (int k = 0; k < t; k += 1) bits &= bits - 1u;
-            const RasterTri& tr = pt[__ffs(bits) - 1];
+            const u32 un = jn * 8u + (u32)g;
+            if (un >= n_units) continue;
+            const u32 unit = s_units[un];
+            const u32 tq = unit & 1023u, part = (unit >> 10) & 7u, plog = (unit >> 13) & 3u;
+            const RasterTri& tr = s_tri[tq];
             const u32 key = tr.key;
             const f32 sx0 = tr.v[0], sy0 = tr.v[1], iw0 = tr.v[2];
             const f32 sx1 = tr.v[3], sy1 = tr.v[4], iw1 = tr.v[5];
             const f32 sx2 = tr.v[6], sy2 = tr.v[7], iw2 = tr.v[8];
-            /* rows of the triangle: group r of part `part` takes phase
-               r + m * part of stride m * parts */
-            const int stride = m << plog, phase = r + m * (int)part;
+            const int stride = 1 << plog, phase = (int)part;
             if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
             else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
         }
