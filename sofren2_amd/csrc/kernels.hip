@@ -1665,7 +1665,9 @@ DEV bool tri_extent(const RasterTri& tr, i32 tmx, i32 tmy, i32& rows, i32& chunk
    counter, one per group of 8 lanes: a wave's groups then sweep for about
    as long as each other.  The visibility plane's 64-bit atomic max makes
    the result independent of which group sweeps which unit. */
-__global__ __launch_bounds__(kRasterThreads) void k_raster(FramePtrs P) {
+/* four waves per SIMD (at most 128 VGPRs): with two 512-thread groups per CU
+   (about 78 KB of LDS each) any more registers would leave one group per CU */
+__global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(FramePtrs P) {
     __shared__ u64 s_vis[SFT_TILE_PIXELS];
     /* the work item's triangles, fetched by the whole group at once so the
        waves rasterise from LDS instead of each waiting on its own chain of
