@@ -253,22 +253,25 @@ void sftx_set_gpu_tlas(i32 enabled);
    the recorded batch like sftx_batch_submit (blocking; outputs written), after
    timing candidate settings on it: per resolution class (views below 512x512
    pixels, below 1024x1024, larger) the payloads per visibility work item
-   (16-128; the item's triangles, 8 per payload, are staged in LDS), then the
-   persistent visibility grid (half, one or two occupancy-sized grids).  The
+   (candidates 64-512, any of 1..1024 settable; the item's triangles, 8 per
+   payload, are staged in LDS 128 payloads at a time into one visibility
+   plane) and the row-part split threshold, then the persistent visibility
+   grid (half, one or two occupancy-sized grids) and the occlusion
+   traversal stack.  The
    chosen settings apply to every later sft_present, batch and server batch;
    no setting changes an output bit.  The bin tile stays 64 px: the
    reference's edge accumulation starts at the 64-px tile's clamped bounding
    box (sofren2.c:5148-5499), so another tile size would change depth and
-   visibility bits.  The occlusion traversal stack has a fixed depth
-   (occlusion_stack entries per wave) that no BVH the builders make can fill;
-   a packet that would overflow it is counted in sftx_frame_stats. */
+   visibility bits.  A packet that fills the full occlusion stack (1024
+   entries per wave; no BVH the builders make can) is counted in
+   sftx_frame_stats. */
 #define SFTX_RES_CLASSES 3
 typedef struct sftx_tuning {
     u32 raster_chunk[SFTX_RES_CLASSES];   /* payloads per visibility work item, by class */
-    /* by class: estimated sweep work (instructions) above which a payload's
-       triangles are swept in 2, 4 or 8 row-band jobs; each job keeps the
-       64-px tile's accumulation origin (rows reached by add-only steps from
-       it), so the split never changes a bit; 0xFFFFFFFF: never split */
+    /* by class: estimated sweep work above which a triangle is swept as 2,
+       4 or 8 row parts by different lane groups; each part keeps the 64-px
+       tile's accumulation origin (rows reached by add-only steps from it),
+       so the split never changes a bit; 0xFFFFFFFF: never split */
     u32 raster_split[SFTX_RES_CLASSES];
     u32 raster_groups;                    /* persistent visibility workgroups */
     u32 raster_groups_resident;           /* workgroups resident at once on this device */
