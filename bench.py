@@ -378,9 +378,10 @@ def served_leg(lib, rates, n_requests, n_scenes, sample_frac=0.01):
         workloads.run_served(srv, burst)
         sample, last_reqs = [], []
         for k, rate in enumerate(rates):
-            # at least a second of arrivals, so the last batches' latency
-            # does not dominate the wall time
-            n = max(n_requests, int(rate))
+            # at least two seconds of arrivals, so the first batches (small,
+            # while the queue builds up) and the last ones (the queue
+            # draining) do not dominate the wall time
+            n = max(n_requests, int(2 * rate))
             reqs = workloads.served_requests(n, rate, n_scenes, seed=100 + k)
             keep = range(0, n, max(1, int(round(1 / sample_frac))))
             st0 = srv.stats()
@@ -447,7 +448,7 @@ def main():
     ap.add_argument("--crowd-steps", type=int, default=120,
                     help="frames of the crowd-with-shadows 1080p leg (0: skip)")
     ap.add_argument("--served-requests", type=int, default=200,
-                    help="requests per offered rate, at least one second of arrivals (0: skip serving)")
+                    help="requests per offered rate, at least two seconds of arrivals (0: skip serving)")
     ap.add_argument("--served-rates", default="100,500,2000,5000", help="offered request rates (req/s)")
     ap.add_argument("--served-scenes", type=int, default=32)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
