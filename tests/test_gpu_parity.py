@@ -201,8 +201,11 @@ def test_tuned_settings_change_no_output_bit():
         for v, b in zip(tuned, base):
             np.testing.assert_array_equal(v.color, b.color)
             np.testing.assert_array_equal(v.vis, b.vis)
+        # the last two: items of up to 1024 payloads, and every triangle split
+        # into 8 row parts, more than a staged chunk's unit list holds (each
+        # triangle then swept as one unit)
         for chunks, split, stack in (([1, 16, 1], [1, 1, 1], 256), ([128, 1, 32], [0xFFFFFFFF] * 3, 512),
-                                     ([1024, 300, 129], [2048] * 3, 1024)):
+                                     ([1024, 300, 129], [2048] * 3, 1024), ([128, 512, 1024], [1, 1, 1], 1024)):
             extreme = S.Tuning()
             extreme.raster_chunk[:] = chunks
             extreme.raster_split[:] = split
