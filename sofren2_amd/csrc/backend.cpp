@@ -1150,6 +1150,7 @@ static void prepare_batch(Backend* b, Context& c, std::vector<ViewRecord>& views
     bg.n_work = (u32)work.size();
     bg.n_tiles = nt;
     bg.max_samples = max_samples;
+    for (u32 vi = 0; vi < nv; vi += 1) bg.n_views_ms += (fs.vgpu[vi].fc.samples > 1) ? 1u : 0u;
     bg.rcp_bits = (u32)c.est.rcp_bits;
     bg.rsq_bits = (u32)c.est.rsq_bits;
     bg.n_shade_tiles = n_shade_runs[0] + n_shade_runs[1];
@@ -1276,7 +1277,9 @@ static size_t acquire_buffers(FrameSet& fs, bool measure) {
     /* visibility work items: at most one per tile and sample plus one per
        raster_chunk payloads */
     const size_t max_items = ((size_t)nt + fs.tile_list_cap / fs.bg.raster_chunk + 1) * fs.max_samples;
-    get(fs.raster_items, max_items * 8);
+    /* two lists: the items of views without MSAA, then those of MSAA views */
+    fs.bg.items2_base = (u32)max_items;
+    get(fs.raster_items, max_items * 2 * 8);
     get(fs.pack_d, fs.pack.used);
     return off;
 }

@@ -34,8 +34,10 @@ enum Counter : int {
     CNT_LIGHT_CUT,          /* tiles whose light list stopped at 16 with more lights in reach */
     CNT_BG_BLEND,           /* MSAA pixels blending the caller's colour under empty samples */
     CNT_CAP_TESTS,          /* capsule ray tests: per shaded pixel and reaching light, its cached capsules */
-    CNT_RASTER_ITEMS,       /* visibility work items (tile, sample, chunk) listed */
-    CNT_RASTER_NEXT,        /* queue head of the visibility pass */
+    CNT_RASTER_ITEMS,       /* visibility work items (tile, chunk) of views without MSAA listed */
+    CNT_RASTER_NEXT,        /* their queue head */
+    CNT_RASTER_ITEMS2,      /* visibility work items (tile, sample pair, chunk) of MSAA views listed */
+    CNT_RASTER_NEXT2,       /* their queue head */
     CNT_COUNT
 };
 

@@ -787,10 +787,10 @@ struct Tri {
    (stride a power of two) and steps the row values through every row in
    between with additions only, so each row's edge and depth values are the
    same sequence of additions as in the reference. */
-template <bool aa>
+template <bool aa, int kPl>
 DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u32 key,
                     f32 sx0, f32 sy0, f32 iw0, f32 sx1, f32 sy1, f32 iw1, f32 sx2, f32 sy2, f32 iw2,
-                    f32 ox, f32 oy, int stride, int phase) {
+                    const f32* ox, const f32* oy, int stride, int phase) {
     const f32 A0 = sy0 - sy1, B0 = sx1 - sx0;
     const f32 A1 = sy1 - sy2, B1 = sx2 - sx1;
     const f32 A2 = sy2 - sy0, B2 = sx0 - sx2;
@@ -833,8 +833,14 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
     const f32 pad1 = aa ? 0.375f * (fabsf(A1) + fabsf(B1)) : 0.f;
     const f32 pad2 = aa ? 0.375f * (fabsf(A2) + fabsf(B2)) : 0.f;
     const bool trivial = me0 >= pad0 && me1 >= pad1 && me2 >= pad2;
-    const f32 de0 = A0 * ox + B0 * oy, de1 = A1 * ox + B1 * oy, de2 = A2 * ox + B2 * oy;
-    const f32 dzs = dzdx * ox + dzdy * oy;
+    /* the sample offsets of each plane (kPl planes, SFT_TILE_PIXELS apart) */
+    f32 de0[kPl], de1[kPl], de2[kPl], dzs[kPl];
+    for (int k = 0; k < kPl; k += 1) {
+        de0[k] = A0 * ox[k] + B0 * oy[k];
+        de1[k] = A1 * ox[k] + B1 * oy[k];
+        de2[k] = A2 * ox[k] + B2 * oy[k];
+        dzs[k] = dzdx * ox[k] + dzdy * oy[k];
+    }
 
     const u32 nkey = ~key;
     const f32 fj = (f32)j;
@@ -850,8 +856,11 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
         for (i32 y = first;;) {
             f32 z = z_row;
             for (i32 x = amin, p = row; x < amax; x += 8, p += 8) {
-                const f32 zs = aa ? z + dzs : z;
-                atomicMax((unsigned long long*)&vis[p], (zs > 0.f) ? (((u64)f2u(zs) << 32) | nkey) : 0ull);
+                for (int k = 0; k < kPl; k += 1) {
+                    const f32 zs = aa ? z + dzs[k] : z;
+                    atomicMax((unsigned long long*)&vis[p + k * SFT_TILE_PIXELS],
+                              (zs > 0.f) ? (((u64)f2u(zs) << 32) | nkey) : 0ull);
+                }
                 z = z + dz_chunk;
             }
             const i32 next = y + stride;
@@ -881,9 +890,13 @@ DEV void raster_tri(u64* vis, const Est& est, i32 tmx, i32 tmy, int j, int g, u3
                     /* the triangle's 8 lanes run the same rows and chunks */
                     const bool near = e0 >= -pad0 && e1 >= -pad1 && e2 >= -pad2;
                     const u32 any_near = (u32)(__ballot(near) >> (g * 8)) & 0xFFu;
-                    const f32 zs = z + dzs;
-                    if (any_near && e0 + de0 >= 0.f && e1 + de1 >= 0.f && e2 + de2 >= 0.f && zs > 0.f) {
-                        atomicMax((unsigned long long*)&vis[p], ((u64)f2u(zs) << 32) | nkey);
+                    if (any_near) {
+                        for (int k = 0; k < kPl; k += 1) {
+                            const f32 zs = z + dzs[k];
+                            if (e0 + de0[k] >= 0.f && e1 + de1[k] >= 0.f && e2 + de2[k] >= 0.f && zs > 0.f) {
+                                atomicMax((unsigned long long*)&vis[p + k * SFT_TILE_PIXELS], ((u64)f2u(zs) << 32) | nkey);
+                            }
+                        }
                     }
                 }
                 e0 = e0 + A0c;
@@ -1505,22 +1518,28 @@ DEV u32 tile_payloads(const FramePtrs& P, u32 tile, u32& off) {
     return cnt;
 }
 
-/* one workgroup per tile: chunk count, work items, zeroed merge planes */
+/* one workgroup per tile: chunk count, work items, zeroed merge planes.
+   Without MSAA an item is (tile, chunk); with MSAA (tile, chunk, first
+   sample of a pair), in the second list, whose items rasterise two sample
+   planes at once. */
 __global__ __launch_bounds__(256) void k_raster_plan(FramePtrs P) {
     __shared__ u32 s_at;
     const u32 tile = blockIdx.x;
     const ViewGpu& V = P.views[P.tile_view[tile]];
     const u32 S = (u32)V.fc.samples, K = min(max(V.raster_chunk, 1u), kRasterMaxItem), maxS = P.batch->max_samples;
+    const bool pairs = S > 1u;
+    const u32 per = pairs ? S / 2u : 1u;
     u32 off;
     const u32 cnt = tile_payloads(P, tile, off);
     const u32 nch = max(1u, (cnt + K - 1u) / K);
     if (0 == threadIdx.x) {
-        s_at = atomicAdd(&P.counters[CNT_RASTER_ITEMS], nch * S);
+        s_at = atomicAdd(&P.counters[pairs ? CNT_RASTER_ITEMS2 : CNT_RASTER_ITEMS], nch * per);
         P.tile_nch[tile] = nch;
     }
     __syncthreads();
-    for (u32 i = threadIdx.x; i < nch * S; i += 256u) {
-        P.raster_items[s_at + i] = make_uint2(tile, ((i / S) << 2) | (i % S));
+    uint2* items = P.raster_items + (pairs ? P.batch->items2_base : 0u);
+    for (u32 i = threadIdx.x; i < nch * per; i += 256u) {
+        items[s_at + i] = make_uint2(tile, ((i / per) << 2) | (pairs ? 2u * (i % per) : 0u));
     }
     if (nch > 1u) {
         u64* m = P.merge + (u64)tile * maxS * SFT_TILE_PIXELS;
@@ -1665,40 +1684,50 @@ DEV bool tri_extent(const RasterTri& tr, i32 tmx, i32 tmy, i32& rows, i32& chunk
    counter, one per group of 8 lanes: a wave's groups then sweep for about
    as long as each other.  The visibility plane's 64-bit atomic max makes
    the result independent of which group sweeps which unit. */
-/* four waves per SIMD (at most 128 VGPRs): with two 512-thread groups per CU
+/* kPl sample planes per item: 1 for the views without MSAA, 2 (a sample
+   pair, whose planes share the staging, the unit list and every edge and
+   depth step; each sample adds its own offsets to them) for MSAA views,
+   which stage 32 payloads at a time to leave LDS for the second plane.
+   Four waves per SIMD (at most 128 VGPRs): with two 512-thread groups per CU
    (about 78 KB of LDS each) any more registers would leave one group per CU */
+template <int kPl>
 __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_raster(FramePtrs P) {
-    __shared__ u64 s_vis[SFT_TILE_PIXELS];
+    constexpr u32 kStage = (1 == kPl) ? kRasterMaxChunk : kRasterMaxChunk / 4u;   /* payloads staged at a time */
+    __shared__ u64 s_vis[kPl * SFT_TILE_PIXELS];
     /* the work item's triangles, fetched by the whole group at once so the
        waves rasterise from LDS instead of each waiting on its own chain of
        descriptor, work-item, index and vertex loads */
-    __shared__ RasterTri s_tri[kRasterMaxChunk * 8];
+    __shared__ RasterTri s_tri[kStage * 8];
     /* units: staged triangle | part << 10 | log2(parts) << 13; a sub-chunk
        whose split units exceed the list sweeps each triangle as one unit */
-    constexpr u32 kUnitCap = kRasterMaxChunk * 16u;
+    constexpr u32 kUnitCap = kStage * 16u;
     __shared__ u16 s_units[kUnitCap];
     __shared__ u32 s_hist[32], s_cur[16], s_split;
     /* each staged triangle's rows | chunks << 8 in the tile, 0 if it misses it */
-    __shared__ u16 s_ext[kRasterMaxChunk * 8];
+    __shared__ u16 s_ext[kStage * 8];
     __shared__ f32 s_red[2][kRasterWaves];
     __shared__ u32 s_item, s_last, s_njobs, s_next;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Est est = make_est(P);
-    const u32 n_items = P.counters[CNT_RASTER_ITEMS];
+    const u32 n_items = P.counters[(1 == kPl) ? CNT_RASTER_ITEMS : CNT_RASTER_ITEMS2];
+    const uint2* items = P.raster_items + ((1 == kPl) ? 0u : P.batch->items2_base);
     const u32 maxS = P.batch->max_samples;
     for (;;) {
-        if (0 == tid) s_item = atomicAdd(&P.counters[CNT_RASTER_NEXT], 1u);
-        for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
+        if (0 == tid) s_item = atomicAdd(&P.counters[(1 == kPl) ? CNT_RASTER_NEXT : CNT_RASTER_NEXT2], 1u);
+        for (int i = tid; i < kPl * SFT_TILE_PIXELS; i += kRasterThreads) s_vis[i] = 0ull;
         __syncthreads();
         const u32 it = s_item;
         if (it >= n_items) break;
-        const uint2 w = P.raster_items[it];
+        const uint2 w = items[it];
         const u32 tile = w.x, chunk = w.y >> 2;
         const int sample = (int)(w.y & 3u);
         const ViewGpu& V = P.views[P.tile_view[tile]];
         const FrameConst& fc = V.fc;
-        const bool aa = fc.samples > 1;
-        const f32 ox = c_msaa_ox[msaa_row(fc.samples)][sample], oy = c_msaa_oy[msaa_row(fc.samples)][sample];
+        f32 ox[kPl], oy[kPl];
+        for (int k = 0; k < kPl; k += 1) {
+            ox[k] = c_msaa_ox[msaa_row(fc.samples)][sample + k];
+            oy[k] = c_msaa_oy[msaa_row(fc.samples)][sample + k];
+        }
         const u32 local = tile - V.tile_base;
         const i32 tmx = (i32)(local % (u32)fc.tile_cols) * 64, tmy = (i32)(local / (u32)fc.tile_cols) * 64;
         const u32 K = min(max(V.raster_chunk, 1u), kRasterMaxItem);
@@ -1707,9 +1736,9 @@ __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(
         const u32 cnt = tile_payloads(P, tile, off);
         const u32 tile_chunks = P.tile_nch[tile];
         const u32 e0 = chunk * K, e1 = min(cnt, e0 + K);
-        /* the item's payloads staged kRasterMaxChunk at a time into the same plane */
-        for (u32 c0 = e0;; c0 += kRasterMaxChunk) {
-        const u32 c1 = min(e1, c0 + kRasterMaxChunk);
+        /* the item's payloads staged kStage at a time into the same planes */
+        for (u32 c0 = e0;; c0 += kStage) {
+        const u32 c1 = min(e1, c0 + kStage);
         const u32 n_tri = (c1 > c0) ? (c1 - c0) * 8u : 0u;
         for (u32 q = (u32)tid; q < n_tri; q += kRasterThreads) {
             const RasterTri tr = fetch_raster_tri(P, off + c0 + (q >> 3), q & 7u);
@@ -1787,8 +1816,9 @@ __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(
             const f32 sx1 = tr.v[3], sy1 = tr.v[4], iw1 = tr.v[5];
             const f32 sx2 = tr.v[6], sy2 = tr.v[7], iw2 = tr.v[8];
             const int stride = 1 << plog, phase = (int)part;
-            if (aa) raster_tri<true>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
-            else raster_tri<false>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy, stride, phase);
+            /* the items of MSAA views are the sample pairs (kPl 2) */
+            raster_tri<(kPl > 1), kPl>(s_vis, est, tmx, tmy, j, g, key, sx0, sy0, iw0, sx1, sy1, iw1, sx2, sy2, iw2, ox, oy,
+                                       stride, phase);
         }
         __syncthreads();
         if (c1 >= e1) break;
@@ -1796,11 +1826,17 @@ __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
 
         if (1u == tile_chunks) {
-            publish_plane(P, V, tile, sample, tmx, tmy, [&](int i) { return s_vis[i]; }, s_red);
+            for (int k = 0; k < kPl; k += 1) {
+                if (k > 0) __syncthreads();   /* the previous plane's 1/w range is read from s_red */
+                const u64* vk = s_vis + k * SFT_TILE_PIXELS;
+                publish_plane(P, V, tile, sample + k, tmx, tmy, [&](int i) { return vk[i]; }, s_red);
+            }
         }
         else {
+            /* the sample planes of a split tile merge in HBM; the item pair's
+               completions count on its first sample */
             u64* m = P.merge + ((u64)tile * maxS + (u32)sample) * SFT_TILE_PIXELS;
-            for (int i = tid; i < SFT_TILE_PIXELS; i += kRasterThreads) {
+            for (int i = tid; i < kPl * SFT_TILE_PIXELS; i += kRasterThreads) {
                 const u64 v = s_vis[i];
                 if (v) atomicMax((unsigned long long*)&m[i], (unsigned long long)v);
             }
@@ -1813,9 +1849,13 @@ __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(
             __syncthreads();
             if (s_last) {
                 __threadfence();
-                publish_plane(P, V, tile, sample, tmx, tmy,
-                              [&](int i) { return __hip_atomic_load(&m[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); },
-                              s_red);
+                for (int k = 0; k < kPl; k += 1) {
+                    if (k > 0) __syncthreads();
+                    const u64* mk = m + k * SFT_TILE_PIXELS;
+                    publish_plane(P, V, tile, sample + k, tmx, tmy,
+                                  [&](int i) { return __hip_atomic_load(&mk[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); },
+                                  s_red);
+                }
             }
         }
         __syncthreads();
@@ -2721,9 +2761,11 @@ void launch_texture_build(const u32* level0, u32* heap, const TexMeta& m, hipStr
 
 /* resident workgroups of the persistent visibility kernel on the device */
 u32 raster_groups_per_device() {
-    int cus = 0, per_cu = 0;
+    int cus = 0, per_cu = 0, per_cu2 = 0;
     if (hipSuccess != hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0)) cus = 1;
-    if (hipSuccess != hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_raster, kRasterThreads, 0)) per_cu = 1;
+    if (hipSuccess != hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_raster<1>, kRasterThreads, 0)) per_cu = 1;
+    if (hipSuccess != hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, k_raster<2>, kRasterThreads, 0)) per_cu2 = 1;
+    per_cu = std::min(per_cu, per_cu2);
     return (u32)std::max(cus, 1) * (u32)std::max(per_cu, 1);
 }
 
@@ -2997,7 +3039,8 @@ void launch_frame(const FramePtrs& p, const BatchGpu& B, u64 zero_words, hipStre
     if (B.n_segs) hipLaunchKernelGGL(k_bin, dim3(B.n_segs), dim3(kBinThreads), 0, s, p);
     mark(3);
     hipLaunchKernelGGL(k_raster_plan, dim3(B.n_tiles), dim3(256), 0, s, p);
-    hipLaunchKernelGGL(k_raster, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
+    if (B.n_views_ms < B.n_views) hipLaunchKernelGGL(k_raster<1>, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
+    if (B.n_views_ms > 0u) hipLaunchKernelGGL(k_raster<2>, dim3(B.raster_groups), dim3(kRasterThreads), 0, s, p);
     if (B.max_samples > 1u) hipLaunchKernelGGL(k_tile_lights, dim3(B.n_tiles), dim3(64), 0, s, p);
     mark(6);
     if (fork) hipStreamWaitEvent(s, side->join, 0);

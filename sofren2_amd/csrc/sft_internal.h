@@ -88,6 +88,8 @@ struct BatchGpu {
     u32 n_shade_sh, n_shade_ms_sh;  /* of the shaded tiles above, those of views with capsule shadows (listed last) */
     u32 occ_stack;                  /* occlusion traversal stack entries per wave: 256, 512 or 1024 */
     u32 tlas_gpu;                   /* 1: some view's TLAS is built on the device (k_tlas) */
+    u32 items2_base;                /* first entry of the MSAA views' visibility work items */
+    u32 n_views_ms;                 /* views with MSAA (their work items are sample pairs) */
 };
 
 /* A binning segment: up to kSegItems consecutive batch slots of one view, or
