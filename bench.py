@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = "1080p frames/s & Mpix/s; p50/p99 request latency, served vs per-frame GPU"
 HBM_PEAK_GBS = 8000.0
 SCENES = {"plane_spheres": 0, "box_room": 1, "crowd_shadows": 2}
-KERNEL_OF_PASS = {"geometry": "k_geom", "binning": "k_clip+k_bin", "visibility": "k_raster", "shading": "k_shade"}
+KERNEL_OF_PASS = {"geometry": "k_geom", "binning": "k_clip+k_bin", "visibility": "k_raster<1>", "shading": "k_shade"}
 # the shading kernel is specialised for views with and without capsule shadows
 SHADOW_SCENES = {"crowd_shadows"}
 
@@ -84,7 +84,7 @@ def measured_traffic(workload, kernel):
 
 PASS_KERNELS = {"geometry": ["k_geom", "k_scan_sums", "k_scan_apply"],
                 "binning": ["k_clip", "k_bin_count", "k_bin_scan", "k_bin"],
-                "visibility": ["k_raster_plan", "k_raster", "k_tile_lights"],
+                "visibility": ["k_raster_plan", "k_raster<1>", "k_raster<2>", "k_tile_lights"],
                 "shading": ["k_shade<false>", "k_shade<true>", "k_shade_ms<false>", "k_shade_ms<true>"]}
 
 

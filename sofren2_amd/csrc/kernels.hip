@@ -24,8 +24,9 @@
  *   k_bin_count, k_bin_scan, k_bin
  *                 per-tile payload lists without global tile-counter atomics.
  *   k_raster_plan splits long tile lists into chunks of visibility work.
- *   k_raster      persistent workgroups over (tile, chunk, sample) items:
- *                 rasterise into an LDS 64-bit visibility plane, merge split
+ *   k_raster      persistent workgroups over (tile, chunk) items, with MSAA
+ *                 (tile, chunk, sample pair) items: rasterise into LDS 64-bit
+ *                 visibility planes, merge split
  *                 tiles in HBM, the last chunk publishes IDs, the tile's 1/w
  *                 range and its light list (sofren2.c:5148-5499, 6756-6813).
  *   k_tile_lights light lists of MSAA tiles over every sample plane.
@@ -776,10 +777,11 @@ struct Tri {
     u32 inst, slot;
 };
 
-/* rasterises one triangle of the tile into one sample plane, lane j = x offset
+/* rasterises one triangle of the tile into kPl sample planes, lane j = x offset
    inside an 8-pixel chunk, 8 lanes at group `g` of the wave
-   (sofren2.c:5213-5497).  With MSAA (`aa`) the sample sits at (ox, oy) from
-   the pixel centre: edge and depth values gain A*ox + B*oy, trivial accept
+   (sofren2.c:5213-5497).  With MSAA (`aa`) plane k's sample sits at
+   (ox[k], oy[k]) from the pixel centre: its edge and depth values gain
+   A*ox + B*oy, added to the plane-free values at each test, trivial accept
    needs a margin of 0.375 (|A| + |B|), and a chunk whose 8 pixel centres are
    all beyond that margin of an edge is skipped.
    Several groups, in this wave and others, can share a triangle: this group
@@ -1496,9 +1498,10 @@ DEV void build_tile_lights(const FramePtrs& P, const ViewGpu& V, u32 tile, f32 z
     }
 }
 
-/* Visibility pass.  A work item is (tile, sample plane, chunk of at most
-   kRasterMaxItem payloads of the tile's list, staged in LDS kRasterMaxChunk
-   at a time into the same plane); k_raster_plan lists the items,
+/* Visibility pass.  A work item is (tile, chunk of at most kRasterMaxItem
+   payloads of the tile's list, staged in LDS kRasterMaxChunk at a time into
+   the same plane), with MSAA (tile, sample pair, chunk), staged 32 payloads
+   at a time into the pair's two planes; k_raster_plan lists the items,
    and a persistent k_raster takes them from a queue, so a tile covered by
    thousands of payloads is spread over many workgroups.  Each item rasterises
    into an LDS plane; a tile with one chunk publishes from LDS, the chunks of a
@@ -1760,7 +1763,8 @@ __global__ __launch_bounds__(kRasterThreads) __attribute__((amdgpu_waves_per_eu(
             const u32 q = (u32)tid + (u32)h * kRasterThreads;
             if (q < n_tri && 0u != s_ext[q]) {
                 const u32 e = s_ext[q], rows = e & 0xFFu, chunks = e >> 8;
-                const u32 cost = rows * 4u + rows * chunks * 20u;
+                /* a sample pair's unit sweeps two planes: twice the cost */
+                const u32 cost = (rows * 4u + rows * chunks * 20u) * (u32)kPl;
                 u32 pl = 0u;
                 while (pl < 3u && (cost >> pl) > V.raster_split) pl += 1u;
                 upl[h] = pl;

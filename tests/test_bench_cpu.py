@@ -63,7 +63,7 @@ def test_pmc_summary_scales_by_the_copy_calibration(tmp_path):
 
 def test_dominant_kernel_names_match_the_profiles():
     b = _bench()
-    assert b.pass_kernel("visibility", "box_room") == "k_raster"
+    assert b.pass_kernel("visibility", "box_room") == "k_raster<1>"
     assert b.pass_kernel("shading", "box_room") == "k_shade<false>"
     assert b.pass_kernel("shading", "crowd_shadows") == "k_shade<true>"
 
@@ -75,6 +75,6 @@ def test_measured_pass_bandwidth_sums_the_pass_kernels():
     b = _bench()
     ks = json.load(open(os.path.join(ROOT, "profiles", "traffic_box_room_1920x1080.json")))["kernels"]
     got = b.measured_pass_gbs("box_room_1920x1080", {"geometry": 0.02, "binning": 0.04, "visibility": 0.1, "shading": 0.1})
-    want = sum(ks[k]["hbm_bytes"] for k in ("k_raster_plan", "k_raster") if k in ks) / 0.1e-3 / 1e9
+    want = sum(ks[k]["hbm_bytes"] for k in ("k_raster_plan", "k_raster<1>", "k_raster<2>", "k_tile_lights") if k in ks) / 0.1e-3 / 1e9
     assert abs(got["visibility"] - want) < 0.1
     assert b.measured_pass_gbs("no_such_workload", {"geometry": 1, "binning": 1, "visibility": 1, "shading": 1}) is None
