@@ -255,7 +255,8 @@ void sftx_set_gpu_tlas(i32 enabled);
    pixels, below 1024x1024, larger) the payloads per visibility work item
    (candidates 64-512, any of 1..1024 settable; the item's triangles, 8 per
    payload, are staged in LDS 128 payloads at a time into one visibility
-   plane) and the row-part split threshold, then the persistent visibility
+   plane; with MSAA an item covers a pair of sample planes and stages 32
+   payloads at a time) and the row-part split threshold, then the persistent visibility
    grid (half, one or two occupancy-sized grids) and the occlusion
    traversal stack.  The
    chosen settings apply to every later sft_present, batch and server batch;
